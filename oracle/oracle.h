@@ -1,0 +1,103 @@
+/*
+ * CPU ORACLE -- test infrastructure only.
+ *
+ * A plain-C restatement of the reference's structured-grid Poisson path
+ * (c/ch6/fish.c with c/ch6/poissonfunctions.c) together with the PETSc
+ * solver semantics that path reaches through its options (KSPCG, PCMG,
+ * KSPCHEBYSHEV, KSPRICHARDSON, PCJACOBI, PCSOR, PCILU, PCLU).  It exists to
+ * check the HIP product library and to time a CPU baseline; nothing in the
+ * product links, loads or calls it.
+ *
+ * Pinning (how each part is checked, see tests/test_oracle.py):
+ *   - discretization (residual, Jacobian):  golden fish.test1/2/6/7 error
+ *     lines, |F(u0)|_2 = 0.925546 (fish.test1:1), closed-form eigen-mode
+ *     errors for the sine problems, brute-force dense solves on tiny grids;
+ *   - CG + PCMG + Chebyshev/SOR:  golden iteration counts fish.test1 (8) and
+ *     fish.test3 (4), and the printed errors of fish.test3;
+ *   - CG + ILU(0):  fish.test5/6/8 (iterations and trajectory-dependent
+ *     errors at the default rtol 1e-5);
+ *   - Q1 transfer:  <P x, y> = <x, P^T y> identity and exact interpolation
+ *     of multilinear functions;
+ *   - V-cycle symmetry <Bx,y> = <x,By>, coarse Cholesky vs dense solve;
+ *   - Lanczos eigen-estimate vs closed-form lambda_max(D^-1 A) (upper bound)
+ *     and vs the GMRES/Hessenberg estimate (equal for constant diagonal).
+ *   - variable-coefficient operator (config "varcoef"): parity unpinned
+ *     against the reference, which has constant cx, cy, cz only; pinned by
+ *     symmetry, constant-k reduction to the constant-coefficient stencil
+ *     and O(h^2) convergence of the manufactured solution.
+ */
+#ifndef P4G_ORACLE_H
+#define P4G_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Result of one fish-style run; the same layout as fish_result in the
+ * product's fish driver (p4pdes_amd/fish/fish.h) so tests compare fields. */
+typedef struct {
+    int    dim, mx, my, mz;
+    int    its;              /* outer KSP iterations (ksp->its)            */
+    int    reason;           /* KSPConvergedReason numbering               */
+    int    inner_its;        /* mixed precision: total inner iterations   */
+    int    nlevels;
+    double fnorm0, fnorm1;   /* |F(u0)|_2, |F(u)|_2                        */
+    double rnorm0, rnorm;    /* norm used by the convergence test          */
+    double true_relres;      /* |b - A x|_2 / |b|_2 of the linear solve    */
+    double errinf, err2h;    /* fish.c:256-276                             */
+    double setup_ms, solve_ms;
+    double emax_fine;        /* Chebyshev upper bound on the finest level  */
+} fish_result;
+
+/* Run the oracle exactly as `fish` would be run with argv (PETSc-style
+ * options).  Text that fish would print goes to outbuf (may be NULL).
+ * u_out (may be NULL) receives the solution, mx*my*mz doubles, x fastest.
+ * Returns 0 on success, a nonzero PETSc-like error code otherwise. */
+int oracle_fish_run(int argc, const char **argv, fish_result *res,
+                    double *u_out, char *outbuf, int outlen);
+
+/* Time `nsolves` solves (after `warmup`) of the linear system fish builds
+ * for argv, with `nthreads` OpenMP threads.  Setup is excluded.  Fills
+ * times_ms[nsolves] and the result of the last solve. */
+int oracle_fish_time(int argc, const char **argv, int nthreads, int warmup,
+                     int nsolves, double *times_ms, fish_result *res);
+
+/* Building blocks exported for the unit tests. ----------------------- */
+
+/* Apply the Poisson operator of a dim-D grid (constant cx,cy,cz, rows as
+ * Poisson{1,2,3}DJacobianLocal assembles them) to x: y = A x.            */
+int oracle_poisson_apply(int dim, int mx, int my, int mz, double Lx, double Ly,
+                         double Lz, double cx, double cy, double cz,
+                         const double *x, double *y);
+
+/* Q1 transfers between vertex-nested grids (fine m = 2(mc-1)+1 per dim). */
+int oracle_interp_add(int dim, int mxc, int myc, int mzc, const double *xc,
+                      double *xf);
+int oracle_restrict(int dim, int mxc, int myc, int mzc, const double *rf,
+                    double *bc);
+
+/* Deterministic start vector used by the Chebyshev eigen-estimate. */
+double oracle_noisy(int64_t index);
+
+/* Philox-4x32-10 block function (Salmon et al. 2011), full counter. */
+void oracle_philox4x32_raw(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+
+/* Philox-4x32-10 normal variate for node `index` and `seed` (RHS noise). */
+double oracle_normal(uint64_t seed, uint64_t index);
+
+/* Variable-coefficient (cell-centred k) operator on a 3D vertex grid:
+ * y = A_k x with the face coefficient the mean of the 4 cells sharing the
+ * face's edge; kcell has (mx-1)(my-1)(mz-1) entries, x fastest.          */
+int oracle_varcoef_apply(int mx, int my, int mz, double Lx, double Ly,
+                         double Lz, const double *kcell, const double *x,
+                         double *y);
+
+/* Cell-coefficient random field k = exp(g) used by the "varcoef" problem. */
+double oracle_varcoef_k(double x, double y, double z, uint64_t seed);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

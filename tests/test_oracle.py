@@ -1,0 +1,222 @@
+"""CPU tests of the oracle (the C restatement used as the parity checker).
+
+Each check is chosen so that a plausible mistake (a dropped boundary
+elimination, a wrong scaling, a transposed transfer, a different Krylov
+convention) fails it:
+  * the reference's own golden outputs c/ch6/output/fish.test1..8, diffed
+    byte for byte as c/testit.sh:32-44 does (2-process tests run with the
+    oracle's simulated 2-rank local-SOR partition);
+  * closed forms: |F(u0)|_2 = h |f(x_i)|_2, the discrete eigen-mode errors
+    of the sine problems, lambda_max(D^-1 A);
+  * brute force: dense numpy solves of the same discrete system;
+  * identities: <P x, y> = <x, P^T y>, exact interpolation of multilinear
+    functions, symmetry, constant-k reduction of the variable operator;
+  * Philox-4x32-10 known-answer vectors (Salmon et al. 2011).
+"""
+import ctypes
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle_py as O
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLDEN = json.load(open(os.path.join(HERE, "golden", "fish_golden.json")))
+
+
+@pytest.mark.parametrize("case", GOLDEN["tests"], ids=lambda c: "fish.test%d" % c["test"])
+def test_golden_text(case):
+    args = case["options"].split()
+    if case["nprocs"] > 1:
+        args += ["-oracle_nranks", str(case["nprocs"])]
+    rc, text, res, _ = O.fish_run(args)
+    assert rc == 0
+    assert text == case["expected"]
+
+
+def test_fnorm0_closed_form():
+    # fish.test1: 1D manupoly, 17 points; u0 = 0 inside, g = 0 at both ends,
+    # so F(u0)_i = -h f(x_i) at interior nodes (poissonfunctions.c:20-21).
+    m = 17
+    h = 1.0 / (m - 1)
+    x = np.arange(1, m - 1) * h
+    f = 12.0 * x * x - 2.0
+    _, _, res, _ = O.fish_run("-fsh_dim 1 -fsh_problem manupoly -da_refine 3 -pc_type mg")
+    assert abs(res.fnorm0 - h * np.linalg.norm(f)) < 1e-14
+    assert "%g" % res.fnorm0 == "0.925546"
+
+
+def _eig_err_2d(m):
+    h = 1.0 / (m - 1)
+    lam = 5 * math.pi ** 2
+    lh = (4 / h ** 2) * (math.sin(math.pi * h / 2) ** 2 + math.sin(math.pi * h) ** 2)
+    return abs(lam / lh - 1)
+
+
+def _eig_err_3d(m):
+    h = 1.0 / (m - 1)
+    lam = 3 * math.pi ** 2
+    lh = (12 / h ** 2) * math.sin(math.pi * h / 2) ** 2
+    return abs(lam / lh - 1)
+
+
+def test_closed_form_c1_2d_129():
+    # C1: 129^2, 6-level V(1,1) Chebyshev(1)/Jacobi, rtol 1e-10
+    rc, text, res, _ = O.fish_run(
+        "-fsh_dim 2 -fsh_problem manusin -da_refine 6 -pc_type mg -pc_mg_levels 6 "
+        "-mg_levels_ksp_type chebyshev -mg_levels_pc_type jacobi -mg_levels_ksp_max_it 1 "
+        "-ksp_rtol 1e-10")
+    assert rc == 0
+    assert abs(res.errinf - _eig_err_2d(129)) < 1e-4 * _eig_err_2d(129)
+    assert abs(_eig_err_2d(129) - 1.706940e-04) < 1e-9
+
+
+@pytest.mark.parametrize("m,refine", [(17, 3), (33, 4), (65, 5)])
+def test_closed_form_3d(m, refine):
+    rc, text, res, _ = O.fish_run(
+        "-fsh_dim 3 -fsh_problem manusin -da_refine %d -pc_type mg -pc_mg_levels %d "
+        "-mg_levels_ksp_type chebyshev -mg_levels_pc_type jacobi -mg_levels_ksp_max_it 1 "
+        "-ksp_rtol 1e-10" % (refine, refine))
+    assert rc == 0
+    e = _eig_err_3d(m)
+    assert abs(res.errinf - e) < 1e-4 * e
+
+
+def _dense_A(dim, mx, my, mz, c=(1.0, 1.0, 1.0)):
+    n = mx * my * mz
+    A = np.zeros((n, n))
+    for q in range(n):
+        e = np.zeros(n)
+        e[q] = 1.0
+        A[:, q] = O.poisson_apply(dim, mx, my, mz, e, c=c)
+    return A
+
+
+@pytest.mark.parametrize("dim,m", [(1, 17), (2, 9), (3, 5)])
+def test_operator_structure(dim, m):
+    mx, my, mz = m, (m if dim > 1 else 1), (m if dim > 2 else 1)
+    A = _dense_A(dim, mx, my, mz, c=(0.5, 2.0, 3.0))
+    assert np.allclose(A, A.T, rtol=0, atol=1e-14)
+    d = np.diag(A)
+    assert np.allclose(d, d[0])                      # constant diagonal
+    idx = np.arange(mx * my * mz)
+    i, j, k = idx % mx, (idx // mx) % my, idx // (mx * my)
+    bd = (i == 0) | (i == mx - 1)
+    if dim > 1:
+        bd |= (j == 0) | (j == my - 1)
+    if dim > 2:
+        bd |= (k == 0) | (k == mz - 1)
+    off = A - np.diag(d)
+    assert np.all(off[bd, :] == 0) and np.all(off[:, bd] == 0)   # boundary decoupled
+    assert np.all(np.linalg.eigvalsh(A) > 0)
+
+
+def test_brute_force_dense_solve():
+    # 3D manupoly 9^3 anisotropic (fish.test6 system): dense solve vs oracle
+    args = "-fsh_dim 3 -da_refine 2 -fsh_problem manupoly -fsh_cx 0.01 -fsh_cy 2 -fsh_cz 100 " \
+           "-pc_type mg -ksp_rtol 1e-13"
+    rc, text, res, u = O.fish_run(args, want_u=True)
+    assert rc == 0
+    # error vs exact solve printed in fish.test6 (reference ILU-CG converged to it)
+    assert "2.301e-04" in text and "7.132e-05" in text
+    A = _dense_A(3, 9, 9, 9, c=(0.01, 2.0, 100.0))
+    # u solves A u = b; rebuild b = A u and re-solve densely
+    b = A @ u
+    assert np.allclose(np.linalg.solve(A, b), u, rtol=0, atol=1e-12)
+
+
+@pytest.mark.parametrize("dim", [1, 2, 3])
+def test_transfer_adjoint_and_exactness(dim):
+    rng = np.random.default_rng(1)
+    mc = [5, 5 if dim > 1 else 1, 5 if dim > 2 else 1]
+    mf = [9, 9 if dim > 1 else 1, 9 if dim > 2 else 1]
+    nc, nf = int(np.prod(mc)), int(np.prod(mf))
+    xc = rng.standard_normal(nc)
+    yf = rng.standard_normal(nf)
+    Pxc = O.interp_add(dim, *mc, xc, np.zeros(nf))
+    PTy = O.restrict(dim, *mc, yf)
+    assert abs(Pxc @ yf - xc @ PTy) < 1e-12 * (abs(Pxc @ yf) + 1)
+    # multilinear functions are interpolated exactly
+    def grid(m):
+        ax = [np.linspace(0, 1, q) for q in m]
+        Z, Y, X = np.meshgrid(ax[2], ax[1], ax[0], indexing="ij")
+        return X.ravel(), Y.ravel(), Z.ravel()
+    Xc, Yc, Zc = grid(mc)
+    Xf, Yf, Zf = grid(mf)
+    fc = 1 + 2 * Xc + 3 * Yc * Xc - Zc * Yc * Xc
+    ff = 1 + 2 * Xf + 3 * Yf * Xf - Zf * Yf * Xf
+    assert np.allclose(O.interp_add(dim, *mc, fc, np.zeros(nf)), ff, atol=1e-14)
+    # restriction weights: P^T 1 = 2^d at interior coarse points
+    ones = O.restrict(dim, *mc, np.ones(nf))
+    assert abs(ones.max() - 2 ** dim) < 1e-14
+
+
+def test_chebyshev_estimate_bounds():
+    # 3D isotropic, Jacobi: lambda_max(D^-1 A) = 1 + cos(pi/(m-1)); the
+    # 10-step Lanczos estimate is below it and close; emax = 1.1 * estimate.
+    for refine in (3, 4):
+        m = 2 ** (refine + 1) + 1      # 3-point base DMDA refined `refine` times
+        _, _, res, _ = O.fish_run(
+            "-fsh_dim 3 -fsh_problem manusin -da_refine %d -pc_type mg "
+            "-mg_levels_ksp_type chebyshev -mg_levels_pc_type jacobi -mg_levels_ksp_max_it 1 "
+            "-ksp_rtol 1e-6" % refine)
+        lmax = 1 + math.cos(math.pi / (m - 1))
+        est = res.emax_fine / 1.1
+        assert est <= lmax * (1 + 1e-12)
+        assert est >= 0.95 * lmax
+
+
+def test_philox_known_answers():
+    L = O.lib()
+    def ph(c, k):
+        C = (ctypes.c_uint32 * 4)(*c)
+        K = (ctypes.c_uint32 * 2)(*k)
+        R = (ctypes.c_uint32 * 4)()
+        L.oracle_philox4x32_raw(C, K, R)
+        return list(R)
+    assert ph([0] * 4, [0] * 2) == [0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8]
+    assert ph([0xffffffff] * 4, [0xffffffff] * 2) == [0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd]
+    assert ph([0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344], [0xa4093822, 0x299f31d0]) == \
+        [0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1]
+    z = np.array([O.lib().oracle_normal(0, i) for i in range(20000)])
+    assert abs(z.mean()) < 0.03 and abs(z.std() - 1) < 0.03
+
+
+def test_varcoef_constant_k_reduces():
+    m = 9
+    kcell = np.ones((m - 1) ** 3)
+    x = np.random.default_rng(2).standard_normal(m ** 3)
+    y1 = O.varcoef_apply(m, m, m, kcell, x)
+    y2 = O.poisson_apply(3, m, m, m, x)
+    assert np.allclose(y1, y2, rtol=1e-14, atol=1e-14)
+    # random k: symmetric positive definite
+    kcell = np.exp(np.random.default_rng(3).standard_normal((m - 1) ** 3))
+    n = m ** 3
+    A = np.zeros((n, n))
+    for q in range(n):
+        e = np.zeros(n); e[q] = 1
+        A[:, q] = O.varcoef_apply(m, m, m, kcell, e)
+    assert np.allclose(A, A.T, atol=1e-13)
+    assert np.linalg.eigvalsh(A).min() > 0
+
+
+def test_varcoef_second_order():
+    errs = []
+    for refine in (3, 4, 5):
+        rc, text, res, _ = O.fish_run(
+            "-fsh_dim 3 -fsh_problem varcoef -da_refine %d -pc_type mg -mg_levels_pc_type jacobi "
+            "-ksp_rtol 1e-11" % refine)
+        assert rc == 0
+        errs.append(res.errinf)
+    r1, r2 = errs[0] / errs[1], errs[1] / errs[2]
+    assert 3.0 < r2 < 5.0, errs
+
+
+@pytest.mark.parametrize("args,code", [
+    ("-fsh_cx -1", 2), ("-fsh_problem manuexp -fsh_cx 2", 3), ("-fsh_dim 4", 1)])
+def test_argument_errors(args, code):
+    rc, text, _, _ = O.fish_run(args)
+    assert rc == code
