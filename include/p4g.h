@@ -1,0 +1,160 @@
+/*
+ * p4g.h -- C ABI of the MI355X (gfx950) CG + geometric-multigrid Poisson
+ * solver.  It stands in for the PETSc objects the reference path creates in
+ * c/ch6/fish.c:197-239 when run as `-ksp_type cg -pc_type mg`:
+ *
+ *   DMDACreate{1,2,3}d + DMSetFromOptions + DMDASetUniformCoordinates
+ *        -> p4g_da_create, p4g_da_set_from_options
+ *   DMDASNESSetFunctionLocal / DMDASNESSetJacobianLocal
+ *        -> p4g_da_set_function_local / p4g_da_set_jacobian_local
+ *   SNESCreate/SNESSetDM/SNESSetType(KSPONLY)/KSPSetType(CG)/SNESSetFromOptions
+ *        -> p4g_solver_create (PETSc-style option string)
+ *   SNESSolve(snes, NULL, u)   -> p4g_snes_solve
+ *   KSPSolve / PCApply / MatMult on the assembled operator
+ *        -> p4g_ksp_solve[_device], p4g_pc_apply[_device], p4g_mat_mult[_device]
+ *
+ * Conventions: every entry point returns 0 or a PETSc error number
+ * (PETSC_ERR_* in p4g_petsc.h) and leaves a message for p4g_last_error()
+ * (thread-local).  No C++ exception crosses the ABI.  The caller owns host
+ * arrays; the library owns device memory.  Host arrays are mx*my*mz doubles,
+ * x fastest (the PETSc natural ordering of a one-process DMDA).  One solver
+ * handle per host thread; each solver launches on its own HIP stream.
+ * Device pointers passed to *_device entry points must be allocated on the
+ * current HIP device (p4g_device_malloc or any hipMalloc).
+ */
+#ifndef P4G_H
+#define P4G_H
+
+#include <stddef.h>
+#include <stdint.h>
+#include "p4g_petsc.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct p4g_solver_s *p4g_Solver;
+
+/* KSPConvergedReason numbering (PETSc) */
+#define P4G_CONVERGED_RTOL 2
+#define P4G_CONVERGED_ATOL 3
+#define P4G_CONVERGED_ITS 4
+#define P4G_DIVERGED_ITS (-3)
+#define P4G_DIVERGED_DTOL (-4)
+#define P4G_DIVERGED_BREAKDOWN (-5)
+#define P4G_DIVERGED_INDEFINITE_MAT (-8)
+#define P4G_DIVERGED_NANORINF (-9)
+
+typedef struct {
+    int    its;            /* outer iterations (PETSc ksp->its)                   */
+    int    reason;         /* KSPConvergedReason                                  */
+    int    inner_its;      /* mixed precision: inner FP32 PCG iterations, total   */
+    int    nlevels;        /* multigrid levels                                    */
+    double rnorm0, rnorm;  /* norm used by the convergence test, first and last  */
+    double true_relres;    /* |b - A x|_2 / |b|_2 computed in FP64 after the solve */
+    double fnorm0, fnorm1; /* p4g_snes_solve: |F(u0)|_2 and |F(u)|_2             */
+    double setup_ms;       /* last set-up (callbacks, hierarchy, eigen-estimates) */
+    double solve_ms;       /* last KSP solve, device time (HIP events)           */
+    double emax_fine;      /* Chebyshev upper bound on the finest level          */
+    double history[64];    /* convergence-test norms of iterations 0..63         */
+} p4g_Report;
+
+/* ---------------------------------------------------------------- grids */
+
+/* DMDACreate{1,2,3}d(DM_BOUNDARY_NONE, STAR, dof 1, width 1) with a global
+ * size of mx (x my (x mz)) vertices on [0,Lx]x[0,Ly]x[0,Lz]
+ * (fish.c:197-220; the reference starts from 3 points per dimension). */
+int p4g_da_create(int dim, int mx, int my, int mz, double Lx, double Ly, double Lz, DM *da);
+/* DMSetFromOptions: -da_grid_x/y/z N, -da_refine R (m <- 2(m-1)+1, R
+ * times); the refinement count is the default number of multigrid levels
+ * minus one, as with PETSc's PCMG on a refined DMDA. */
+int p4g_da_set_from_options(DM da, const char *options);
+int p4g_da_get_sizes(DM da, int *dim, int *mx, int *my, int *mz, int *refine);
+/* DMDASNESSetFunctionLocal(da, INSERT_VALUES, fn, ctx)  (fish.c:225-226) */
+int p4g_da_set_function_local(DM da, DMDASNESFunctionFn *fn, void *ctx);
+/* DMDASNESSetJacobianLocal(da, fn, ctx)                 (fish.c:227-228) */
+int p4g_da_set_jacobian_local(DM da, DMDASNESJacobianFn *fn, void *ctx);
+/* Variable-coefficient operator -div(k grad u) with k given per cell,
+ * (mx-1)(my-1)(mz-1) values (3D only), face coefficient = mean of the 4
+ * cells sharing the face's edge, coarse k = mean of the 8 fine cells.
+ * Replaces the Jacobian callback's operator (the reference's callbacks
+ * have constant cx,cy,cz only, poissonfunctions.h:44-47). */
+int p4g_da_set_cell_coefficient(DM da, const double *kcell);
+int p4g_da_destroy(DM *da);
+
+/* --------------------------------------------------------------- solver */
+
+/* SNES KSPONLY + KSPCG + options (fish.c:223-234).  Supported options:
+ *   -ksp_type cg, -ksp_rtol, -ksp_atol, -ksp_divtol, -ksp_max_it,
+ *   -ksp_norm_type preconditioned|unpreconditioned|natural,
+ *   -pc_type mg|jacobi|none, -pc_mg_levels L, -pc_mg_cycle_type v|w,
+ *   -mg_levels_ksp_type chebyshev|richardson, -mg_levels_pc_type jacobi|none,
+ *   -mg_levels_ksp_max_it n, -mg_levels_ksp_richardson_scale s,
+ *   -mg_levels_ksp_chebyshev_esteig_steps n,
+ *   -mg_coarse_ksp_type preonly, -mg_coarse_pc_type lu|cholesky|redundant,
+ *   -p4g_precision double|mixed  (mixed: FP64 iterative refinement around an
+ *       FP32 PCG whose V-cycle runs in FP32; FP64 residual, outer norms and
+ *       coarsest Cholesky; converged when |b-Ax|_2/|b|_2 <= rtol),
+ *   -p4g_inner_rtol r, -p4g_inner_max_it n.
+ * Options of PETSc objects the device path does not implement (-pc_type
+ * ilu|sor|lu, -mg_levels_pc_type sor, -pc_mg_galerkin, ...) are rejected
+ * with PETSC_ERR_SUP. */
+int p4g_solver_create(DM da, const char *options, p4g_Solver *s);
+/* PCSetUp_MG equivalent: Jacobian callbacks per level, stencil extraction,
+ * device hierarchy, Chebyshev eigen-estimates, coarse Cholesky. */
+int p4g_solver_setup(p4g_Solver s);
+/* SNESSolve(snes, NULL, u), KSPONLY (fish.c:239): F(u0) by the residual
+ * callback, J y = F(u0) by PCG from y = 0, u <- u0 - y.  u is host. */
+int p4g_snes_solve(p4g_Solver s, double *u, p4g_Report *rep);
+/* KSPSolve(ksp, b, x) from x = 0, host arrays (copies over PCIe). */
+int p4g_ksp_solve(p4g_Solver s, const double *b, double *x, p4g_Report *rep);
+/* KSPSolve with device-resident b and x (FP64, n doubles each). */
+int p4g_ksp_solve_device(p4g_Solver s, const double *d_b, double *d_x, p4g_Report *rep);
+/* PCApply: z = B r, one multigrid cycle (PCShellSetApply-compatible). */
+int p4g_pc_apply(p4g_Solver s, const double *r, double *z);
+int p4g_pc_apply_device(p4g_Solver s, const double *d_r, double *d_z);
+/* MatMult with the finest-level operator: y = A x. */
+int p4g_mat_mult(p4g_Solver s, const double *x, double *y);
+int p4g_mat_mult_device(p4g_Solver s, const double *d_x, double *d_y);
+int p4g_solver_get_report(p4g_Solver s, p4g_Report *rep);
+int p4g_solver_destroy(p4g_Solver *s);
+
+/* --------------------------------------------------------- measurement */
+
+/* Kernel classes timed with HIP events on the solver's stream while
+ * timing is on (off by default): one event pair around every launch of
+ * the selected class. */
+enum {
+    P4G_K_STENCIL_DOT = 0, /* w = A p fused with (p,w), the CG operator apply  */
+    P4G_K_CG_UPDATE,       /* x += a p, r -= a w                               */
+    P4G_K_RESID_ZG,        /* r = b - A (c b): zero-guess smoothing + residual */
+    P4G_K_RESTRICT,        /* b_c = P^T r                                      */
+    P4G_K_PROLONG,         /* x = c b + P x_c                                  */
+    P4G_K_SMOOTH,          /* x <- x + s D^-1 (b - A x) (Chebyshev step)       */
+    P4G_K_COARSE,          /* coarsest-grid Cholesky solve                     */
+    P4G_K_REDUCE,          /* finalisation of fused dot products               */
+    P4G_K_OTHER,
+    P4G_K_COUNT
+};
+int p4g_solver_set_timing(p4g_Solver s, int kernel_class, int on);
+/* launches, summed device ms, and algorithmic bytes moved, for the class
+ * (optionally only launches on the finest level: fine_only = 1) */
+int p4g_solver_kernel_stats(p4g_Solver s, int kernel_class, int fine_only, long *launches,
+                            double *ms, double *bytes);
+int p4g_solver_reset_stats(p4g_Solver s);
+
+/* ---------------------------------------------------- device utilities */
+
+int   p4g_device_count(int *n);
+int   p4g_device_synchronize(void);
+void *p4g_device_malloc(size_t bytes);
+int   p4g_device_free(void *p);
+int   p4g_memcpy_htod(void *dst, const void *src, size_t bytes);
+int   p4g_memcpy_dtoh(void *dst, const void *src, size_t bytes);
+const char *p4g_last_error(void);
+const char *p4g_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -64,6 +64,13 @@ int oracle_fish_run(int argc, const char **argv, fish_result *res,
 int oracle_fish_time(int argc, const char **argv, int nthreads, int warmup,
                      int nsolves, double *times_ms, fish_result *res);
 
+/* One preconditioner application z = B r (PCApply), one operator
+ * application y = A x (MatMult), and F(u0), for the system fish builds
+ * from argv.  Arrays have mx*my*mz doubles. */
+int oracle_pc_apply(int argc, const char **argv, const double *r, double *z);
+int oracle_mat_mult(int argc, const char **argv, const double *x, double *y);
+int oracle_rhs(int argc, const char **argv, double *F);
+
 /* Building blocks exported for the unit tests. ----------------------- */
 
 /* Apply the Poisson operator of a dim-D grid (constant cx,cy,cz, rows as

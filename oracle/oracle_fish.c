@@ -236,6 +236,36 @@ int oracle_fish_run(int argc, const char **argv, fish_result *res, double *u_out
     return 0;
 }
 
+/* one application of the preconditioner fish's KSP uses (PCApply) */
+int oracle_pc_apply(int argc, const char **argv, const double *r, double *z) {
+    osession s;
+    int pos = 0;
+    int e = session_setup(&s, argc, argv, NULL, 0, &pos);
+    if (!e) opc_apply(&s.ksp.pc, r, z);
+    session_free(&s);
+    return e;
+}
+
+/* y = A x with the operator fish assembles (MatMult) */
+int oracle_mat_mult(int argc, const char **argv, const double *x, double *y) {
+    osession s;
+    int pos = 0;
+    int e = session_setup(&s, argc, argv, NULL, 0, &pos);
+    if (!e) ocsr_mult(&s.A, x, y);
+    session_free(&s);
+    return e;
+}
+
+/* F(u0) of fish's initial iterate: the right-hand side of the KSP solve */
+int oracle_rhs(int argc, const char **argv, double *F) {
+    osession s;
+    int pos = 0;
+    int e = session_setup(&s, argc, argv, NULL, 0, &pos);
+    if (!e) ovcopy(s.g.n, s.F0, F);
+    session_free(&s);
+    return e;
+}
+
 int oracle_fish_time(int argc, const char **argv, int nthreads, int warmup, int nsolves,
                      double *times_ms, fish_result *res) {
 #ifdef _OPENMP

@@ -131,3 +131,38 @@ def varcoef_apply(mx, my, mz, kcell, x, L=(1.0, 1.0, 1.0)):
     f.argtypes = [ctypes.c_int] * 3 + [ctypes.c_double] * 3 + [ctypes.c_void_p] * 3
     f(mx, my, mz, *L, kcell.ctypes.data, x.ctypes.data, y.ctypes.data)
     return y
+
+
+def _vec_call(fname, args, x):
+    argc, argv = _argv(args)
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.zeros_like(x)
+    f = getattr(lib(), fname)
+    f.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+    rc = f(argc, argv, x.ctypes.data, y.ctypes.data)
+    if rc:
+        raise RuntimeError("%s failed: %d" % (fname, rc))
+    return y
+
+
+def pc_apply(args, r):
+    """z = B r with the preconditioner fish's KSP uses for `args`."""
+    return _vec_call("oracle_pc_apply", args, r)
+
+
+def mat_mult(args, x):
+    """y = A x with the operator fish assembles for `args`."""
+    return _vec_call("oracle_mat_mult", args, x)
+
+
+def rhs(args):
+    """F(u0): the right-hand side of fish's linear solve."""
+    argc, argv = _argv(args)
+    _, mx, my, mz = grid_shape(args)
+    F = np.zeros(mx * my * mz)
+    f = lib().oracle_rhs
+    f.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
+    rc = f(argc, argv, F.ctypes.data)
+    if rc:
+        raise RuntimeError("oracle_rhs failed: %d" % rc)
+    return F

@@ -1,0 +1,656 @@
+// p4g_kernels.hpp -- gfx950 device kernels of the CG + geometric-multigrid
+// Poisson path (vertex grids, x fastest, one array per grid function).
+//
+// Operator rows are those of Poisson{1,2,3}DJacobianLocal
+// (c/ch6/poissonfunctions.c:117-258): every row has the diagonal d (boundary
+// rows: db), interior rows couple with -sx/-sy/-sz to interior neighbours
+// only, boundary rows are diagonal-only.  Q1 transfers act on every vertex,
+// boundary included; restriction is the transpose of interpolation.
+//
+// The stencil kernels march along the slowest dimension (z in 3D, y in 2D):
+// a thread owns one (i,j) column, keeps the operand at k-1, k, k+1 in
+// registers and reads in-plane neighbours through the cache, so each input
+// grid function streams from HBM about once per pass.  One wavefront spans
+// 64 consecutive x points starting at i = 1; the lane at i = 1 also owns the
+// boundary point i = 0, so (mx-1) = 2^k grids fill every lane.
+//
+// Reductions are deterministic: each block writes its partial sums, a
+// single-block pass adds them in a fixed order.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace p4g {
+
+constexpr int TX = 64;   // lanes along x: one wavefront
+constexpr int TY = 4;    // rows (3D) or march chunks (2D) per block
+constexpr int NT = TX * TY;
+
+// ------------------------------------------------------------- operators
+
+// Constant-coefficient stencil.
+template <typename T>
+struct OpConst {
+    T d, db, sx, sy, sz;   // interior diagonal, boundary diagonal, couplings
+    T id, idb;             // Jacobi: 1/d, 1/db (or 1 when no Jacobi scaling)
+    __device__ __forceinline__ void bnd(int, int, int, T u, T& Au, T& idg) const {
+        Au = db * u; idg = idb;
+    }
+    __device__ __forceinline__ void interior(int, int, int, T uc, T w, T e, T s, T n, T dn,
+                                             T up, T& Au, T& idg) const {
+        Au = d * uc - sx * (w + e) - sy * (s + n) - sz * (dn + up);
+        idg = id;
+    }
+};
+
+// Variable coefficient -div(k grad u) with cell-centred k (3D).  The face
+// coefficient is the mean of the 4 cells sharing the face's edge; a
+// boundary row's diagonal is 2(ax+ay+az) times the mean of its cells.
+template <typename T>
+struct OpVar {
+    const T* __restrict__ kc;   // (mx-1)(my-1)(mz-1) cells, x fastest
+    int cx, cy, cz;             // cell counts
+    T ax, ay, az;               // dvol/h^2 per direction
+    int jac;                    // 1: Jacobi scaling, 0: identity
+    __device__ __forceinline__ T cell(int i, int j, int k) const {
+        return kc[((long)k * cy + j) * cx + i];
+    }
+    __device__ __forceinline__ void bnd(int i, int j, int k, T u, T& Au, T& idg) const {
+        T s = T(0);
+        int cnt = 0;
+        for (int dk = -1; dk <= 0; ++dk)
+            for (int dj = -1; dj <= 0; ++dj)
+                for (int di = -1; di <= 0; ++di) {
+                    int a = i + di, b = j + dj, c = k + dk;
+                    if (a < 0 || b < 0 || c < 0 || a >= cx || b >= cy || c >= cz) continue;
+                    s += cell(a, b, c);
+                    ++cnt;
+                }
+        T dg = T(2) * (ax + ay + az) * (s / T(cnt));
+        Au = dg * u;
+        idg = jac ? T(1) / dg : T(1);
+    }
+    __device__ __forceinline__ void interior(int i, int j, int k, T uc, T w, T e, T s, T n,
+                                             T dn, T up, T& Au, T& idg) const {
+        const T c000 = cell(i - 1, j - 1, k - 1), c100 = cell(i, j - 1, k - 1);
+        const T c010 = cell(i - 1, j, k - 1), c110 = cell(i, j, k - 1);
+        const T c001 = cell(i - 1, j - 1, k), c101 = cell(i, j - 1, k);
+        const T c011 = cell(i - 1, j, k), c111 = cell(i, j, k);
+        const T q = T(0.25);
+        const T kw = q * (c000 + c010 + c001 + c011) * ax;
+        const T ke = q * (c100 + c110 + c101 + c111) * ax;
+        const T ks = q * (c000 + c100 + c001 + c101) * ay;
+        const T kn = q * (c010 + c110 + c011 + c111) * ay;
+        const T kd = q * (c000 + c100 + c010 + c110) * az;
+        const T ku = q * (c001 + c101 + c011 + c111) * az;
+        const T dg = kw + ke + ks + kn + kd + ku;
+        Au = dg * uc - kw * w - ke * e - ks * s - kn * n - kd * dn - ku * up;
+        idg = jac ? T(1) / dg : T(1);
+    }
+};
+
+// ----------------------------------------------------- operand sources
+
+template <typename T>
+struct SrcArr {                       // u = x
+    const T* __restrict__ x;
+    __device__ __forceinline__ T load(long q) const { return x[q]; }
+    __device__ __forceinline__ void put(long, T) const {}
+};
+
+template <typename T>
+struct SrcZero {                      // u = 0 (first smoothing step from a zero guess)
+    __device__ __forceinline__ T load(long) const { return T(0); }
+    __device__ __forceinline__ void put(long, T) const {}
+};
+
+template <typename T>
+struct SrcScaled {                    // u = c b   (zero-guess smoothing step)
+    const T* __restrict__ b;
+    T c;
+    __device__ __forceinline__ T load(long q) const { return c * b[q]; }
+    __device__ __forceinline__ void put(long, T) const {}
+};
+
+template <typename T>
+struct SrcScaledPut {                 // u = c b, stored to x
+    const T* __restrict__ b;
+    T c;
+    T* __restrict__ x;
+    __device__ __forceinline__ T load(long q) const { return c * b[q]; }
+    __device__ __forceinline__ void put(long q, T v) const { x[q] = v; }
+};
+
+template <typename T>
+struct SrcDir {                       // p_new = z + beta p_old, stored (CG direction)
+    const T* __restrict__ z;
+    const T* __restrict__ pold;
+    T* __restrict__ pnew;
+    const double* __restrict__ beta;  // device scalar
+    __device__ __forceinline__ T load(long q) const { return z[q] + T(*beta) * pold[q]; }
+    __device__ __forceinline__ void put(long q, T v) const { pnew[q] = v; }
+};
+
+template <typename T, typename E>
+struct SrcSum {                       // x_new = x + e (refinement update), stored
+    const T* __restrict__ x;
+    const E* __restrict__ e;
+    T* __restrict__ xnew;
+    __device__ __forceinline__ T load(long q) const { return x[q] + T(e[q]); }
+    __device__ __forceinline__ void put(long q, T v) const { xnew[q] = v; }
+};
+
+// ------------------------------------------------------ output policies
+
+template <typename T>
+struct OutApplyDot {                  // w = A u ; acc0 += u.w
+    T* __restrict__ w;
+    static constexpr int NDOT = 1;
+    __device__ __forceinline__ void point(long q, T u, T Au, T, double& a0, double&) const {
+        w[q] = Au;
+        a0 += double(u) * double(Au);
+    }
+};
+
+template <typename T>
+struct OutApply {                     // w = A u
+    T* __restrict__ w;
+    static constexpr int NDOT = 0;
+    __device__ __forceinline__ void point(long q, T, T Au, T, double&, double&) const { w[q] = Au; }
+};
+
+template <typename T, int NORM>
+struct OutResid {                     // r = b - A u ; acc0 += r.r (NORM)
+    const T* __restrict__ b;
+    T* __restrict__ r;
+    static constexpr int NDOT = NORM;
+    __device__ __forceinline__ void point(long q, T, T Au, T, double& a0, double&) const {
+        const T v = b[q] - Au;
+        r[q] = v;
+        if (NORM) a0 += double(v) * double(v);
+    }
+};
+
+template <typename T, typename R>
+struct OutResidCvt {                  // r = b - A u in T, stored as R ; acc0 += r.r
+    const T* __restrict__ b;
+    R* __restrict__ r;
+    static constexpr int NDOT = 1;
+    __device__ __forceinline__ void point(long q, T, T Au, T, double& a0, double&) const {
+        const T v = b[q] - Au;
+        r[q] = R(v);
+        a0 += double(v) * double(v);
+    }
+};
+
+// y = c0 prev + c1 u + c2 D^-1 (b - A u); DOTS: acc0 += y.b, acc1 += y.y
+template <typename T, bool PREV, int DOTS>
+struct OutCheb {
+    const T* __restrict__ b;
+    const T* __restrict__ prev;
+    T* __restrict__ y;
+    T c0, c1, c2;
+    static constexpr int NDOT = DOTS;
+    __device__ __forceinline__ void point(long q, T u, T Au, T idg, double& a0,
+                                          double& a1) const {
+        const T bq = b[q];
+        T v = c1 * u + c2 * (idg * (bq - Au));
+        if (PREV) v += c0 * prev[q];
+        y[q] = v;
+        if (DOTS) {
+            a0 += double(v) * double(bq);
+            a1 += double(v) * double(v);
+        }
+    }
+};
+
+// ------------------------------------------------------------ reductions
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// fixed-order block sum of up to two values; thread 0 writes part[2*bid+q]
+template <int NDOT>
+__device__ __forceinline__ void block_partials(double a0, double a1, double* __restrict__ part,
+                                               long bid) {
+    if (NDOT == 0) return;
+    __shared__ double sm[2][NT / 64];
+    const int lin = threadIdx.x + blockDim.x * threadIdx.y;
+    const int wv = lin >> 6, ln = lin & 63;
+    const int nw = (blockDim.x * blockDim.y) >> 6;
+    a0 = wave_sum(a0);
+    if (NDOT > 1) a1 = wave_sum(a1);
+    if (ln == 0) { sm[0][wv] = a0; sm[1][wv] = a1; }
+    __syncthreads();
+    if (lin == 0) {
+        double s0 = 0.0, s1 = 0.0;
+        for (int q = 0; q < nw; ++q) { s0 += sm[0][q]; s1 += sm[1][q]; }
+        part[2 * bid] = s0;
+        part[2 * bid + 1] = s1;
+    }
+}
+
+// ------------------------------------------------------------- stencil
+
+// Generic marching stencil pass: for every vertex, u from Src, A u and the
+// Jacobi factor from Op, result written by Out.  Grid:
+//   3D: (ceil((mx-1)/64), ceil(my/4), ceil(mz/kch)), block (64,4)
+//   2D: (ceil((mx-1)/64), ceil(ceil(my/kch)/4), 1),  block (64,4)
+//   1D: (ceil((mx-1)/64), 1, 1),                     block (64,1)
+template <typename T, int DIM, class Op, class Src, class Out>
+__global__ void __launch_bounds__(NT)
+k_stencil(int mx, int my, int mz, int kch, Op op, Src src, Out out, double* __restrict__ part) {
+    const int i = 1 + blockIdx.x * TX + threadIdx.x;
+    double a0 = 0.0, a1 = 0.0;
+    const T z0 = T(0);
+    if (DIM == 3) {
+        const int j = blockIdx.y * TY + threadIdx.y;
+        const long sy = mx, sz = (long)mx * my;
+        const int k0 = blockIdx.z * kch;
+        const int k1 = min(k0 + kch, mz);
+        if (i < mx && j < my && k0 < k1) {
+            const bool bij = (i == mx - 1) || (j == 0) || (j == my - 1);
+            const bool mw = i - 1 > 0, me = i + 1 < mx - 1, ms = j - 1 > 0, mn = j + 1 < my - 1;
+            long q = (long)k0 * sz + (long)j * sy + i;
+            T ud = (k0 > 0) ? src.load(q - sz) : z0;
+            T uc = src.load(q);
+            for (int k = k0; k < k1; ++k, q += sz) {
+                const T uu = (k + 1 < mz) ? src.load(q + sz) : z0;
+                src.put(q, uc);
+                T Au, idg;
+                if (bij || k == 0 || k == mz - 1) {
+                    op.bnd(i, j, k, uc, Au, idg);
+                } else {
+                    const T w = mw ? src.load(q - 1) : z0;
+                    const T e = me ? src.load(q + 1) : z0;
+                    const T s = ms ? src.load(q - sy) : z0;
+                    const T n = mn ? src.load(q + sy) : z0;
+                    const T dn = (k - 1 > 0) ? ud : z0;
+                    const T up = (k + 1 < mz - 1) ? uu : z0;
+                    op.interior(i, j, k, uc, w, e, s, n, dn, up, Au, idg);
+                }
+                out.point(q, uc, Au, idg, a0, a1);
+                if (i == 1) {
+                    const T u0 = src.load(q - 1);
+                    src.put(q - 1, u0);
+                    op.bnd(0, j, k, u0, Au, idg);
+                    out.point(q - 1, u0, Au, idg, a0, a1);
+                }
+                ud = uc;
+                uc = uu;
+            }
+        }
+    } else if (DIM == 2) {
+        const int chunk = blockIdx.y * TY + threadIdx.y;
+        const long sy = mx;
+        const int j0 = chunk * kch;
+        const int j1 = min(j0 + kch, my);
+        if (i < mx && j0 < j1) {
+            const bool bi = (i == mx - 1);
+            const bool mw = i - 1 > 0, me = i + 1 < mx - 1;
+            long q = (long)j0 * sy + i;
+            T ud = (j0 > 0) ? src.load(q - sy) : z0;
+            T uc = src.load(q);
+            for (int j = j0; j < j1; ++j, q += sy) {
+                const T uu = (j + 1 < my) ? src.load(q + sy) : z0;
+                src.put(q, uc);
+                T Au, idg;
+                if (bi || j == 0 || j == my - 1) {
+                    op.bnd(i, j, 0, uc, Au, idg);
+                } else {
+                    const T w = mw ? src.load(q - 1) : z0;
+                    const T e = me ? src.load(q + 1) : z0;
+                    const T s = (j - 1 > 0) ? ud : z0;
+                    const T n = (j + 1 < my - 1) ? uu : z0;
+                    op.interior(i, j, 0, uc, w, e, s, n, z0, z0, Au, idg);
+                }
+                out.point(q, uc, Au, idg, a0, a1);
+                if (i == 1) {
+                    const T u0 = src.load(q - 1);
+                    src.put(q - 1, u0);
+                    op.bnd(0, j, 0, u0, Au, idg);
+                    out.point(q - 1, u0, Au, idg, a0, a1);
+                }
+                ud = uc;
+                uc = uu;
+            }
+        }
+    } else {
+        if (i < mx) {
+            const long q = i;
+            const T uc = src.load(q);
+            src.put(q, uc);
+            T Au, idg;
+            if (i == mx - 1) {
+                op.bnd(i, 0, 0, uc, Au, idg);
+            } else {
+                const T w = (i - 1 > 0) ? src.load(q - 1) : z0;
+                const T e = (i + 1 < mx - 1) ? src.load(q + 1) : z0;
+                op.interior(i, 0, 0, uc, w, e, z0, z0, z0, z0, Au, idg);
+            }
+            out.point(q, uc, Au, idg, a0, a1);
+            if (i == 1) {
+                const T u0 = src.load(0);
+                src.put(0, u0);
+                op.bnd(0, 0, 0, u0, Au, idg);
+                out.point(0, u0, Au, idg, a0, a1);
+            }
+        }
+    }
+    const long bid = blockIdx.x + (long)gridDim.x * (blockIdx.y + (long)gridDim.y * blockIdx.z);
+    block_partials<Out::NDOT>(a0, a1, part, bid);
+}
+
+// ------------------------------------------------------------ transfers
+
+// b_c = P^T r_f, unscaled full weighting (weights 1, 1/2 per dimension).
+// Thread per coarse column (I,J), marching K; the in-plane weighted sum of
+// the fine plane 2K+1 is reused as plane 2(K+1)-1.
+// Grid: (ceil(mxc/64), ceil(myc/4) [3D] | ceil(ceil(myc/kch)/4) [2D], ceil(mzc/kch) [3D])
+template <typename T, int DIM>
+__global__ void __launch_bounds__(NT)
+k_restrict(int mxf, int myf, int mzf, int mxc, int myc, int mzc, int kch,
+           const T* __restrict__ rf, T* __restrict__ bc) {
+    const int I = blockIdx.x * TX + threadIdx.x;
+    if (I >= mxc) return;
+    const int i0 = 2 * I;
+    const bool hw = i0 - 1 >= 0, he = i0 + 1 < mxf;
+    auto row = [&](long base) -> T {   // x-weighted sum on a fine row
+        T s = rf[base + i0];
+        if (hw) s += T(0.5) * rf[base + i0 - 1];
+        if (he) s += T(0.5) * rf[base + i0 + 1];
+        return s;
+    };
+    if (DIM == 1) {
+        bc[I] = row(0);
+        return;
+    }
+    if (DIM == 2) {
+        const int chunk = blockIdx.y * TY + threadIdx.y;
+        const int J0 = chunk * kch, J1 = min(J0 + kch, myc);
+        if (J0 >= J1) return;
+        T slo = (2 * J0 - 1 >= 0) ? row((long)(2 * J0 - 1) * mxf) : T(0);
+        for (int J = J0; J < J1; ++J) {
+            const int j = 2 * J;
+            const T smid = row((long)j * mxf);
+            const T shi = (j + 1 < myf) ? row((long)(j + 1) * mxf) : T(0);
+            bc[(long)J * mxc + I] = smid + T(0.5) * (slo + shi);
+            slo = shi;
+        }
+        return;
+    }
+    const int J = blockIdx.y * TY + threadIdx.y;
+    if (J >= myc) return;
+    const int K0 = blockIdx.z * kch, K1 = min(K0 + kch, mzc);
+    const int j0 = 2 * J;
+    const long sxy = (long)mxf * myf;
+    auto plane = [&](int k) -> T {     // x,y-weighted sum on fine plane k
+        const long base = (long)k * sxy + (long)j0 * mxf;
+        T s = row(base);
+        if (j0 - 1 >= 0) s += T(0.5) * row(base - mxf);
+        if (j0 + 1 < myf) s += T(0.5) * row(base + mxf);
+        return s;
+    };
+    if (K0 >= K1) return;
+    T slo = (2 * K0 - 1 >= 0) ? plane(2 * K0 - 1) : T(0);
+    for (int K = K0; K < K1; ++K) {
+        const int k = 2 * K;
+        const T smid = plane(k);
+        const T shi = (k + 1 < mzf) ? plane(k + 1) : T(0);
+        bc[((long)K * myc + J) * mxc + I] = smid + T(0.5) * (slo + shi);
+        slo = shi;
+    }
+}
+
+// out = base + P x_c, base = c b (ZG: zero-guess smoothing result) or x.
+// Thread per fine column (i,j) marching k; the coarse in-plane interpolant
+// of plane K is computed once and reused for fine planes 2K-1, 2K, 2K+1.
+// Grid: (ceil(mxf/64), ceil(myf/4) [3D] | chunks [2D], ceil(mzf/kch) [3D])
+template <typename T, int DIM, bool ZG>
+__global__ void __launch_bounds__(NT)
+k_prolong(int mxf, int myf, int mzf, int mxc, int myc, int kch, const T* __restrict__ xb,
+          T c, const T* __restrict__ xc, T* __restrict__ out) {
+    const int i = blockIdx.x * TX + threadIdx.x;
+    if (i >= mxf) return;
+    const int ia = i >> 1, ib = (i + 1) >> 1;          // equal when i is even
+    const bool iodd = i & 1;
+    auto crow = [&](long base) -> T {
+        const T a = xc[base + ia];
+        return iodd ? T(0.5) * (a + xc[base + ib]) : a;
+    };
+    auto basev = [&](long q) -> T { return ZG ? c * xb[q] : xb[q]; };
+    if (DIM == 1) {
+        out[i] = basev(i) + crow(0);
+        return;
+    }
+    if (DIM == 2) {
+        const int chunk = blockIdx.y * TY + threadIdx.y;
+        const int j0 = chunk * kch, j1 = min(j0 + kch, myf);
+        for (int j = j0; j < j1; ++j) {
+            const int ja = j >> 1, jb = (j + 1) >> 1;
+            T v = crow((long)ja * mxc);
+            if (j & 1) v = T(0.5) * (v + crow((long)jb * mxc));
+            const long q = (long)j * mxf + i;
+            out[q] = basev(q) + v;
+        }
+        return;
+    }
+    const int j = blockIdx.y * TY + threadIdx.y;
+    if (j >= myf) return;
+    const int ja = j >> 1, jb = (j + 1) >> 1;
+    const bool jodd = j & 1;
+    const long sxyc = (long)mxc * myc;
+    auto cplane = [&](int K) -> T {
+        const long base = (long)K * sxyc;
+        T v = crow(base + (long)ja * mxc);
+        return jodd ? T(0.5) * (v + crow(base + (long)jb * mxc)) : v;
+    };
+    const int k0 = blockIdx.z * kch, k1 = min(k0 + kch, mzf);
+    if (k0 >= k1) return;
+    const long sxy = (long)mxf * myf;
+    int K = k0 >> 1;
+    T plo = cplane(K);
+    T phi = (2 * K + 1 < mzf) ? cplane(K + 1) : T(0);
+    for (int k = k0; k < k1; ++k) {
+        const int Kk = k >> 1;
+        if (Kk != K) {     // advance one coarse plane
+            K = Kk;
+            plo = phi;
+            phi = (2 * K + 1 < mzf) ? cplane(K + 1) : T(0);
+        }
+        const T v = (k & 1) ? T(0.5) * (plo + phi) : plo;
+        const long q = (long)k * sxy + (long)j * mxf + i;
+        out[q] = basev(q) + v;
+    }
+}
+
+// ------------------------------------------------------- coarse solve
+
+// x = L^-T (L^-1 b) with Linv = L^-1 (n x n, row major, lower triangular)
+// of the Cholesky factor of the coarsest operator, FP64.  One block.
+template <typename T>
+__global__ void __launch_bounds__(1024)
+k_coarse_chol(int n, const double* __restrict__ Linv, const T* __restrict__ b, T* __restrict__ x) {
+    extern __shared__ double sh[];
+    double* bb = sh;
+    double* yy = sh + n;
+    for (int q = threadIdx.x; q < n; q += blockDim.x) bb[q] = double(b[q]);
+    __syncthreads();
+    for (int r = threadIdx.x; r < n; r += blockDim.x) {
+        const double* L = Linv + (long)r * n;
+        double s = 0.0;
+        for (int c = 0; c <= r; ++c) s += L[c] * bb[c];
+        yy[r] = s;
+    }
+    __syncthreads();
+    for (int r = threadIdx.x; r < n; r += blockDim.x) {
+        double s = 0.0;
+        for (int c = r; c < n; ++c) s += Linv[(long)c * n + r] * yy[c];
+        x[r] = T(s);
+    }
+}
+
+// ------------------------------------------------------- vector kernels
+
+// x += a p ; r -= a w  (a = S[ALPHA]); NORM: acc0 += r.r
+template <typename T, int NORM>
+__global__ void __launch_bounds__(256)
+k_cg_update(long n, const double* __restrict__ S, int ialpha, T* __restrict__ x,
+            const T* __restrict__ p, T* __restrict__ r, const T* __restrict__ w,
+            double* __restrict__ part) {
+    const T a = T(S[ialpha]);
+    double acc = 0.0;
+    for (long q = blockIdx.x * 256L + threadIdx.x; q < n; q += (long)gridDim.x * 256) {
+        x[q] += a * p[q];
+        const T rv = r[q] - a * w[q];
+        r[q] = rv;
+        if (NORM) acc += double(rv) * double(rv);
+    }
+    if (NORM) {
+        __shared__ double sm[4];
+        acc = wave_sum(acc);
+        if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = acc;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            part[2 * blockIdx.x] = sm[0] + sm[1] + sm[2] + sm[3];
+            part[2 * blockIdx.x + 1] = 0.0;
+        }
+    }
+}
+
+// y = c x (pointwise), optional dots acc0 += y.x, acc1 += y.y
+template <typename T, int DOTS>
+__global__ void __launch_bounds__(256)
+k_scale(long n, T c, const T* __restrict__ x, T* __restrict__ y, double* __restrict__ part) {
+    double a0 = 0.0, a1 = 0.0;
+    for (long q = blockIdx.x * 256L + threadIdx.x; q < n; q += (long)gridDim.x * 256) {
+        const T xv = x[q];
+        const T v = c * xv;
+        y[q] = v;
+        if (DOTS) { a0 += double(v) * double(xv); a1 += double(v) * double(v); }
+    }
+    if (DOTS) {
+        __shared__ double sm[2][4];
+        a0 = wave_sum(a0);
+        a1 = wave_sum(a1);
+        if ((threadIdx.x & 63) == 0) { sm[0][threadIdx.x >> 6] = a0; sm[1][threadIdx.x >> 6] = a1; }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            part[2 * blockIdx.x] = sm[0][0] + sm[0][1] + sm[0][2] + sm[0][3];
+            part[2 * blockIdx.x + 1] = sm[1][0] + sm[1][1] + sm[1][2] + sm[1][3];
+        }
+    }
+}
+
+// acc0 += x.y, acc1 += x.x
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_dot2(long n, const T* __restrict__ x, const T* __restrict__ y, double* __restrict__ part) {
+    double a0 = 0.0, a1 = 0.0;
+    for (long q = blockIdx.x * 256L + threadIdx.x; q < n; q += (long)gridDim.x * 256) {
+        const double xv = double(x[q]);
+        a0 += xv * double(y[q]);
+        a1 += xv * xv;
+    }
+    __shared__ double sm[2][4];
+    a0 = wave_sum(a0);
+    a1 = wave_sum(a1);
+    if ((threadIdx.x & 63) == 0) { sm[0][threadIdx.x >> 6] = a0; sm[1][threadIdx.x >> 6] = a1; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = sm[0][0] + sm[0][1] + sm[0][2] + sm[0][3];
+        part[2 * blockIdx.x + 1] = sm[1][0] + sm[1][1] + sm[1][2] + sm[1][3];
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_fill(long n, T v, T* __restrict__ x) {
+    for (long q = blockIdx.x * 256L + threadIdx.x; q < n; q += (long)gridDim.x * 256) x[q] = v;
+}
+
+template <typename S, typename D>
+__global__ void __launch_bounds__(256) k_convert(long n, const S* __restrict__ x, D* __restrict__ y) {
+    for (long q = blockIdx.x * 256L + threadIdx.x; q < n; q += (long)gridDim.x * 256) y[q] = D(x[q]);
+}
+
+// deterministic start vector of the Chebyshev eigen-estimate (splitmix64)
+__device__ __forceinline__ double noisy_at(long index) {
+    uint64_t z = (uint64_t)index + 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z = z ^ (z >> 31);
+    return (double)(z >> 11) * (1.0 / 9007199254740992.0);
+}
+
+__global__ void __launch_bounds__(256) k_noisy(long n, double* __restrict__ x) {
+    for (long q = blockIdx.x * 256L + threadIdx.x; q < n; q += (long)gridDim.x * 256)
+        x[q] = noisy_at(q);
+}
+
+// ------------------------------------------------------- finalisation
+
+// Sum nparts pairs in a fixed order, then update the scalar block S.
+// Scalar slots:
+enum {
+    S_RZ = 0,      // beta = (z, r)
+    S_RZOLD,       // betaold
+    S_PW,          // dpi = (p, A p)
+    S_ALPHA,       // beta / dpi
+    S_BCOEF,       // beta / betaold (direction update)
+    S_DP,          // norm of the convergence test
+    S_ZZ,          // (z, z)
+    S_RR,          // (r, r)
+    S_TMP0, S_TMP1,
+    S_COUNT = 16
+};
+enum { FIN_ALPHA = 0, FIN_BETA_PRECOND, FIN_BETA_NATURAL, FIN_BETA_UNPRECOND, FIN_RR, FIN_RAW };
+
+__global__ void __launch_bounds__(256)
+k_finalize(long nparts, const double* __restrict__ part, double* __restrict__ S, int mode,
+           int first) {
+    __shared__ double s0[256], s1[256];
+    double a = 0.0, b = 0.0;
+    for (long q = threadIdx.x; q < nparts; q += 256) { a += part[2 * q]; b += part[2 * q + 1]; }
+    s0[threadIdx.x] = a;
+    s1[threadIdx.x] = b;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+            s0[threadIdx.x] += s0[threadIdx.x + w];
+            s1[threadIdx.x] += s1[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x != 0) return;
+    const double A = s0[0], B = s1[0];
+    switch (mode) {
+    case FIN_ALPHA:            // A = (p, A p)
+        S[S_PW] = A;
+        S[S_RZOLD] = S[S_RZ];
+        S[S_ALPHA] = S[S_RZ] / A;
+        break;
+    case FIN_BETA_PRECOND:     // A = (z, r), B = (z, z)
+    case FIN_BETA_NATURAL:
+    case FIN_BETA_UNPRECOND:
+        S[S_RZ] = A;
+        S[S_ZZ] = B;
+        if (mode == FIN_BETA_PRECOND) S[S_DP] = sqrt(B);
+        else if (mode == FIN_BETA_NATURAL) S[S_DP] = sqrt(fabs(A));
+        else S[S_DP] = sqrt(S[S_RR]);
+        S[S_BCOEF] = first ? 0.0 : A / S[S_RZOLD];
+        break;
+    case FIN_RR:               // A = (r, r)
+        S[S_RR] = A;
+        break;
+    case FIN_RAW:              // raw sums into TMP slots
+        S[S_TMP0] = A;
+        S[S_TMP1] = B;
+        break;
+    }
+}
+
+}  // namespace p4g

@@ -1,0 +1,1561 @@
+// p4g_solver.hip -- host orchestration of the gfx950 CG + geometric
+// multigrid Poisson solver behind the C ABI of include/p4g.h.
+//
+// Semantics follow what the reference path gets from PETSc with
+// `-ksp_type cg -pc_type mg` (c/ch6/fish.c:231-239):
+//   * KSPCG: preconditioned PCG; default test |B r_k| <= max(rtol |B r_0|, atol);
+//     beta == 0 -> CONVERGED_ATOL; a sign change of (p, A p) ->
+//     DIVERGED_INDEFINITE_MAT; |r| >= dtol |r_0| -> DIVERGED_DTOL.
+//   * PCMG multiplicative cycle: pre-smooth, r = b - A x, b_c = P^T r,
+//     x_c = 0, recurse (twice for W), x += P x_c, post-smooth; one smoother
+//     per level with a nonzero initial guess; coarsest level solved directly.
+//   * KSPCHEBYSHEV smoother with Jacobi: first step x += s D^-1 (b - A x),
+//     s = 2/(emax+emin), then the three-term recurrence; bounds
+//     [0.1, 1.1] * lambda_max estimated by 10 PCG steps from a fixed
+//     pseudo-random right-hand side.  KSPRICHARDSON: x += w D^-1 (b - A x).
+// Coarse operators are re-discretized by calling the user's Jacobian
+// callback on each coarsened grid (fish.c:7).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <vector>
+
+#include "p4g_internal.hpp"
+#include "p4g_kernels.hpp"
+
+using namespace p4g;
+
+#define HIP_TRY(call)                                                                      \
+    do {                                                                                   \
+        hipError_t e__ = (call);                                                           \
+        if (e__ != hipSuccess)                                                             \
+            return set_error(PETSC_ERR_LIB, "%s: %s (%s:%d)", #call, hipGetErrorString(e__), \
+                             __FILE__, __LINE__);                                          \
+    } while (0)
+
+#define LAUNCH_CHECK()                                                                     \
+    do {                                                                                   \
+        hipError_t e__ = hipGetLastError();                                                \
+        if (e__ != hipSuccess)                                                             \
+            return set_error(PETSC_ERR_LIB, "kernel launch: %s (%s:%d)", hipGetErrorString(e__), \
+                             __FILE__, __LINE__);                                          \
+    } while (0)
+
+namespace p4g {
+
+enum { PCT_MG = 0, PCT_JACOBI, PCT_NONE };
+enum { SM_CHEB = 0, SM_RICH };
+enum { NT_PRECOND = 0, NT_UNPRECOND, NT_NATURAL };
+
+constexpr int KCH = 16;            // planes per marching chunk
+constexpr int VBLOCKS = 2048;      // blocks of the grid-stride vector kernels
+constexpr int MAX_COARSE = 2048;   // largest coarsest grid for the direct solve
+
+static double now_ms() {
+    using namespace std::chrono;
+    return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
+}
+
+// precision-independent description of one multigrid level
+struct LevelInfo {
+    int  dim = 3, mx = 0, my = 1, mz = 1;
+    long n = 0;
+    double h[3] = {0, 0, 0};
+    bool var = false;
+    double d = 0, db = 0, sx = 0, sy = 0, sz = 0;    // constant stencil
+    std::vector<double> kcell;                       // variable coefficient cells
+    int cx = 0, cy = 0, cz = 0;
+    double ax = 0, ay = 0, az = 0;
+    double emin = 0, emax = 0;                       // Chebyshev bounds
+};
+
+// ------------------------------------------------------------- timing
+
+struct KStat {
+    long launches = 0, fine_launches = 0;
+    double ms = 0, fine_ms = 0, bytes = 0, fine_bytes = 0;
+};
+
+struct Timing {
+    bool on[P4G_K_COUNT] = {};
+    KStat st[P4G_K_COUNT];
+    struct Pending { hipEvent_t a, b; int cls; bool fine; };
+    std::vector<Pending> pending;
+    std::vector<hipEvent_t> pool;
+    hipEvent_t get() {
+        if (!pool.empty()) { hipEvent_t e = pool.back(); pool.pop_back(); return e; }
+        hipEvent_t e;
+        hipEventCreate(&e);
+        return e;
+    }
+    void collect() {
+        for (auto& p : pending) {
+            float ms = 0.f;
+            hipEventSynchronize(p.b);
+            hipEventElapsedTime(&ms, p.a, p.b);
+            st[p.cls].ms += ms;
+            if (p.fine) st[p.cls].fine_ms += ms;
+            pool.push_back(p.a);
+            pool.push_back(p.b);
+        }
+        pending.clear();
+    }
+    ~Timing() {
+        for (auto& p : pending) { hipEventDestroy(p.a); hipEventDestroy(p.b); }
+        for (auto e : pool) hipEventDestroy(e);
+    }
+};
+
+// shared device context of one solver: stream, partials, scalars, timing
+struct Ctx {
+    hipStream_t stream = nullptr;
+    double* part = nullptr;       // block partial sums (2 per block)
+    long npart_cap = 0;
+    double* S = nullptr;          // scalar block (32 doubles: engine 0..15, outer 16..31)
+    double* hS = nullptr;         // pinned mirror
+    Timing tm;
+
+    template <class F>
+    int launch(int cls, bool fine, double bytes, F&& f) {
+        KStat& s = tm.st[cls];
+        s.launches++;
+        s.bytes += bytes;
+        if (fine) { s.fine_launches++; s.fine_bytes += bytes; }
+        if (tm.on[cls]) {
+            hipEvent_t a = tm.get(), b = tm.get();
+            hipEventRecord(a, stream);
+            f();
+            hipEventRecord(b, stream);
+            tm.pending.push_back({a, b, cls, fine});
+        } else {
+            f();
+        }
+        LAUNCH_CHECK();
+        return 0;
+    }
+    int finalize(long nparts, double* Sblk, int mode, int first) {
+        return launch(P4G_K_REDUCE, false, 16.0 * nparts, [&] {
+            k_finalize<<<1, 256, 0, stream>>>(nparts, part, Sblk, mode, first);
+        });
+    }
+    int readS(double* host) {
+        HIP_TRY(hipMemcpyAsync(host, S, 32 * sizeof(double), hipMemcpyDeviceToHost, stream));
+        HIP_TRY(hipStreamSynchronize(stream));
+        return 0;
+    }
+};
+
+static inline int vblocks(long n) { return (int)std::max(1L, std::min<long>(VBLOCKS, (n + 255) / 256)); }
+
+// launch geometry of the marching stencil kernels
+static dim3 stencil_grid(int dim, int mx, int my, int mz) {
+    const int gx = (mx - 1 + TX - 1) / TX;
+    if (dim == 3) return dim3(gx, (my + TY - 1) / TY, (mz + KCH - 1) / KCH);
+    if (dim == 2) return dim3(gx, ((my + KCH - 1) / KCH + TY - 1) / TY, 1);
+    return dim3(gx, 1, 1);
+}
+static dim3 stencil_block(int dim) { return dim == 1 ? dim3(TX, 1, 1) : dim3(TX, TY, 1); }
+static long nblocks(dim3 g) { return (long)g.x * g.y * g.z; }
+// grid covering all points with lanes from i = 0 (transfers)
+static dim3 point_grid(int dim, int mx, int my, int mz) {
+    const int gx = (mx + TX - 1) / TX;
+    if (dim == 3) return dim3(gx, (my + TY - 1) / TY, (mz + KCH - 1) / KCH);
+    if (dim == 2) return dim3(gx, ((my + KCH - 1) / KCH + TY - 1) / TY, 1);
+    return dim3(gx, 1, 1);
+}
+
+// ------------------------------------------------------------- engine
+
+// Device hierarchy and Krylov work vectors in precision T.
+template <typename T>
+struct Engine {
+    Ctx* cx = nullptr;
+    std::vector<LevelInfo>* lv = nullptr;
+    int nlev = 0, cycles = 1, smoother = SM_CHEB, nu = 1, jacobi = 1, pctype = PCT_MG;
+    double rich_scale = 1.0;
+    struct Dev {
+        T *x = nullptr, *b = nullptr, *r = nullptr, *t[4] = {nullptr, nullptr, nullptr, nullptr};
+        T* kc = nullptr;
+    };
+    std::vector<Dev> dv;
+    double* Linv = nullptr;
+    int n0 = 0;
+    // Krylov vectors on the finest level
+    T *r = nullptr, *z = nullptr, *pa = nullptr, *pb = nullptr, *w = nullptr, *xbuf = nullptr,
+      *bbuf = nullptr;
+    std::vector<void*> allocs;
+
+    ~Engine() { for (void* p : allocs) hipFree(p); }
+
+    int alloc(T** p, long n) {
+        void* q = nullptr;
+        hipError_t e = hipMalloc(&q, std::max<long>(n, 1) * sizeof(T));
+        if (e != hipSuccess)
+            return set_error(PETSC_ERR_MEM, "hipMalloc of %ld bytes failed: %s", n * (long)sizeof(T),
+                             hipGetErrorString(e));
+        allocs.push_back(q);
+        *p = (T*)q;
+        return 0;
+    }
+
+    bool zg_fusable(const LevelInfo& L) const {
+        return !L.var && L.d == L.db && (smoother == SM_CHEB || smoother == SM_RICH);
+    }
+
+    int setup(Ctx* c, std::vector<LevelInfo>* levels, const std::vector<double>& Linv_host,
+              int n0_) {
+        cx = c;
+        lv = levels;
+        nlev = (int)levels->size();
+        dv.resize(nlev);
+        const LevelInfo& F = (*lv)[nlev - 1];
+        for (int l = 0; l < nlev; ++l) {
+            const LevelInfo& L = (*lv)[l];
+            Dev& d = dv[l];
+            if (pctype == PCT_MG) {
+                if (l < nlev - 1) { P4G_TRY(alloc(&d.x, L.n)); P4G_TRY(alloc(&d.b, L.n)); }
+                if (l > 0) {
+                    P4G_TRY(alloc(&d.r, L.n));
+                    const int nt = (nu == 1 && cycles == 1 && zg_fusable(L)) ? 1 : 4;
+                    for (int q = 0; q < nt; ++q) P4G_TRY(alloc(&d.t[q], L.n));
+                }
+            }
+            if (L.var) {
+                const long ncell = (long)L.cx * L.cy * L.cz;
+                P4G_TRY(alloc(&d.kc, ncell));
+                std::vector<T> tmp(L.kcell.begin(), L.kcell.end());
+                HIP_TRY(hipMemcpy(d.kc, tmp.data(), ncell * sizeof(T), hipMemcpyHostToDevice));
+            }
+        }
+        P4G_TRY(alloc(&r, F.n));
+        P4G_TRY(alloc(&z, F.n));
+        P4G_TRY(alloc(&pa, F.n));
+        P4G_TRY(alloc(&pb, F.n));
+        P4G_TRY(alloc(&w, F.n));
+        if (pctype == PCT_MG) {
+            n0 = n0_;
+            void* q = nullptr;
+            HIP_TRY(hipMalloc(&q, Linv_host.size() * sizeof(double)));
+            allocs.push_back(q);
+            Linv = (double*)q;
+            HIP_TRY(hipMemcpy(Linv, Linv_host.data(), Linv_host.size() * sizeof(double),
+                              hipMemcpyHostToDevice));
+        }
+        return 0;
+    }
+
+    OpConst<T> opc(const LevelInfo& L, bool jac) const {
+        OpConst<T> o;
+        o.d = T(L.d); o.db = T(L.db); o.sx = T(L.sx); o.sy = T(L.sy); o.sz = T(L.sz);
+        o.id = jac ? T(1.0 / L.d) : T(1); o.idb = jac ? T(1.0 / L.db) : T(1);
+        return o;
+    }
+    OpVar<T> opv(int l, bool jac) const {
+        const LevelInfo& L = (*lv)[l];
+        OpVar<T> o;
+        o.kc = dv[l].kc; o.cx = L.cx; o.cy = L.cy; o.cz = L.cz;
+        o.ax = T(L.ax); o.ay = T(L.ay); o.az = T(L.az); o.jac = jac ? 1 : 0;
+        return o;
+    }
+
+    // one marching stencil pass on level l
+    template <class Src, class Out>
+    int stencil(int l, Src src, Out out, int cls, double passes, long* nb, bool jac) {
+        const LevelInfo& L = (*lv)[l];
+        const dim3 g = stencil_grid(L.dim, L.mx, L.my, L.mz), b = stencil_block(L.dim);
+        if (nb) *nb = nblocks(g);
+        if (Out::NDOT && nblocks(g) > cx->npart_cap)
+            return set_error(PETSC_ERR_LIB, "internal: partial-sum buffer too small");
+        hipStream_t s = cx->stream;
+        const bool fine = (l == nlev - 1);
+        const double bytes = passes * L.n * sizeof(T);
+        if (L.var) {
+            if (L.dim != 3) return set_error(PETSC_ERR_SUP, "variable coefficient needs 3D");
+            OpVar<T> o = opv(l, jac);
+            return cx->launch(cls, fine, bytes, [&] {
+                k_stencil<T, 3, OpVar<T>, Src, Out><<<g, b, 0, s>>>(L.mx, L.my, L.mz, KCH, o, src, out,
+                                                                 cx->part);
+            });
+        }
+        OpConst<T> o = opc(L, jac);
+        return cx->launch(cls, fine, bytes, [&] {
+            if (L.dim == 3)
+                k_stencil<T, 3, OpConst<T>, Src, Out><<<g, b, 0, s>>>(L.mx, L.my, L.mz, KCH, o, src,
+                                                                   out, cx->part);
+            else if (L.dim == 2)
+                k_stencil<T, 2, OpConst<T>, Src, Out><<<g, b, 0, s>>>(L.mx, L.my, L.mz, KCH, o, src,
+                                                                   out, cx->part);
+            else
+                k_stencil<T, 1, OpConst<T>, Src, Out><<<g, b, 0, s>>>(L.mx, L.my, L.mz, KCH, o, src,
+                                                                   out, cx->part);
+        });
+    }
+
+    int restrict_(int l, const T* rf, T* bc) {   // level l -> l-1
+        const LevelInfo& F = (*lv)[l];
+        const LevelInfo& C = (*lv)[l - 1];
+        const dim3 g = point_grid(C.dim, C.mx, C.my, C.mz), b = stencil_block(C.dim);
+        hipStream_t s = cx->stream;
+        return cx->launch(P4G_K_RESTRICT, l == nlev - 1, double(F.n + C.n) * sizeof(T), [&] {
+            if (F.dim == 3)
+                k_restrict<T, 3><<<g, b, 0, s>>>(F.mx, F.my, F.mz, C.mx, C.my, C.mz, KCH, rf, bc);
+            else if (F.dim == 2)
+                k_restrict<T, 2><<<g, b, 0, s>>>(F.mx, F.my, F.mz, C.mx, C.my, C.mz, KCH, rf, bc);
+            else
+                k_restrict<T, 1><<<g, b, 0, s>>>(F.mx, F.my, F.mz, C.mx, C.my, C.mz, KCH, rf, bc);
+        });
+    }
+
+    // out = (zg ? c b : base) + P x_c   (level l-1 -> l)
+    int prolong(int l, bool zg, const T* base, T c, const T* xc, T* out) {
+        const LevelInfo& F = (*lv)[l];
+        const LevelInfo& C = (*lv)[l - 1];
+        const dim3 g = point_grid(F.dim, F.mx, F.my, F.mz), b = stencil_block(F.dim);
+        hipStream_t s = cx->stream;
+        return cx->launch(P4G_K_PROLONG, l == nlev - 1, double(2 * F.n + C.n) * sizeof(T), [&] {
+            if (F.dim == 3) {
+                if (zg) k_prolong<T, 3, true><<<g, b, 0, s>>>(F.mx, F.my, F.mz, C.mx, C.my, KCH, base, c, xc, out);
+                else k_prolong<T, 3, false><<<g, b, 0, s>>>(F.mx, F.my, F.mz, C.mx, C.my, KCH, base, c, xc, out);
+            } else if (F.dim == 2) {
+                if (zg) k_prolong<T, 2, true><<<g, b, 0, s>>>(F.mx, F.my, F.mz, C.mx, C.my, KCH, base, c, xc, out);
+                else k_prolong<T, 2, false><<<g, b, 0, s>>>(F.mx, F.my, F.mz, C.mx, C.my, KCH, base, c, xc, out);
+            } else {
+                if (zg) k_prolong<T, 1, true><<<g, b, 0, s>>>(F.mx, F.my, F.mz, C.mx, C.my, KCH, base, c, xc, out);
+                else k_prolong<T, 1, false><<<g, b, 0, s>>>(F.mx, F.my, F.mz, C.mx, C.my, KCH, base, c, xc, out);
+            }
+        });
+    }
+
+    int coarse(const T* b, T* x) {
+        hipStream_t s = cx->stream;
+        const int nt = std::min(1024, ((n0 + 63) / 64) * 64);
+        return cx->launch(P4G_K_COARSE, nlev == 1, double(n0) * n0 * 8.0, [&] {
+            k_coarse_chol<T><<<1, nt, 2 * n0 * sizeof(double), s>>>(n0, Linv, b, x);
+        });
+    }
+
+    int scale(long n, T c, const T* x, T* y, bool dots, bool fine) {
+        hipStream_t s = cx->stream;
+        const int gb = vblocks(n);
+        return cx->launch(dots ? P4G_K_SMOOTH : P4G_K_OTHER, fine, 2.0 * n * sizeof(T), [&] {
+            if (dots) k_scale<T, 2><<<gb, 256, 0, s>>>(n, c, x, y, cx->part);
+            else k_scale<T, 0><<<gb, 256, 0, s>>>(n, c, x, y, cx->part);
+        });
+    }
+
+    int fill(long n, T v, T* x) {
+        hipStream_t s = cx->stream;
+        return cx->launch(P4G_K_OTHER, false, double(n) * sizeof(T), [&] {
+            k_fill<T><<<vblocks(n), 256, 0, s>>>(n, v, x);
+        });
+    }
+
+    // Chebyshev / Richardson smoothing with Jacobi on level l:
+    //   x0 = nullptr means a zero initial guess.  The result lands in `out`.
+    //   DOTS: fuse (out, b) and (out, out) partials into the final step.
+    int smooth(int l, const T* b, const T* x0, T* out, bool dots, long* nb) {
+        const LevelInfo& L = (*lv)[l];
+        const bool jac = jacobi != 0;
+        const double idg = jac ? 1.0 / L.d : 1.0;
+        const bool fine = l == nlev - 1;
+        double s, alpha = 0, mu = 0, omegaprod = 0, c[3];
+        if (smoother == SM_CHEB) {
+            s = 2.0 / (L.emax + L.emin);
+            alpha = 1.0 - s * L.emin;
+            mu = 1.0 / alpha;
+            omegaprod = 2.0 / alpha;
+            c[0] = 1.0; c[1] = mu;
+        } else {
+            s = rich_scale;
+        }
+        // buffers: p[m] holds iterate m; choose temps distinct from inputs
+        const T* pm2 = nullptr;          // iterate m-2
+        const T* pm1 = x0;               // iterate m-1 (nullptr: zero)
+        bool pm1_implicit = (x0 == nullptr);   // iterate 1 of a zero guess kept implicit (c b)
+        int used = -1;
+        auto pick = [&](const T* a, const T* bq) -> T* {
+            for (int q = 0; q < 4; ++q) {
+                T* t = dv[l].t[q];
+                if (t && t != a && t != bq && t != out) return t;
+            }
+            return nullptr;
+        };
+        for (int m = 1; m <= nu; ++m) {
+            const bool last = (m == nu);
+            T* dst = last ? out : pick(pm1, pm2);
+            if (!dst) return set_error(PETSC_ERR_LIB, "internal: smoother buffers exhausted");
+            double c0 = 0.0, c1 = 1.0, c2 = s;
+            if (smoother == SM_CHEB && m >= 2) {
+                const double ckp1 = 2.0 * mu * c[1] - c[0];
+                const double omega = omegaprod * c[1] / ckp1;
+                c[0] = c[1]; c[1] = ckp1;
+                c0 = 1.0 - omega; c1 = omega; c2 = omega * s;
+            }
+            const bool prev = (smoother == SM_CHEB && m >= 2 && pm2 != nullptr);
+            long nbk = 0;
+            if (m == 1 && x0 == nullptr) {
+                // iterate 1 from zero: s D^-1 b, pointwise; keep implicit if
+                // the next step can form it on the fly
+                if (!last && zg_fusable(L) && !(nu >= 3 && smoother == SM_CHEB)) {
+                    pm1 = nullptr;
+                    pm1_implicit = true;
+                    continue;
+                }
+                if (L.var) {
+                    // variable diagonal: a stencil pass with u = 0 gives
+                    // y = c2 D^-1 b
+                    if (last && dots)
+                        P4G_TRY(stencil(l, SrcZero<T>{}, OutCheb<T, false, 2>{b, nullptr, dst, T(0), T(0), T(c2)},
+                                        P4G_K_SMOOTH, 2, &nbk, jac));
+                    else
+                        P4G_TRY(stencil(l, SrcZero<T>{}, OutCheb<T, false, 0>{b, nullptr, dst, T(0), T(0), T(c2)},
+                                        P4G_K_SMOOTH, 2, &nbk, jac));
+                } else {
+                    P4G_TRY(scale(L.n, T(s * idg), b, dst, last && dots, fine));
+                    nbk = vblocks(L.n);
+                }
+                if (nb) *nb = nbk;
+                pm2 = nullptr;
+                pm1 = dst;
+                pm1_implicit = false;
+                continue;
+            }
+            // the m-th step reads iterate m-1 (array or implicit c b) and m-2
+            if (pm1_implicit) {
+                const T cb = T(s * idg);
+                // m == 2 from zero guess: prev iterate is 0, so no prev term
+                if (nu >= 3 && smoother == SM_CHEB) return set_error(PETSC_ERR_LIB, "internal smoother path");
+                if (last && dots)
+                    P4G_TRY(stencil(l, SrcScaled<T>{b, cb}, OutCheb<T, false, 2>{b, nullptr, dst, T(c0), T(c1), T(c2)},
+                                    P4G_K_SMOOTH, 2, &nbk, jac));
+                else
+                    P4G_TRY(stencil(l, SrcScaled<T>{b, cb}, OutCheb<T, false, 0>{b, nullptr, dst, T(c0), T(c1), T(c2)},
+                                    P4G_K_SMOOTH, 2, &nbk, jac));
+                pm1_implicit = false;
+            } else if (prev) {
+                if (last && dots)
+                    P4G_TRY(stencil(l, SrcArr<T>{pm1}, OutCheb<T, true, 2>{b, pm2, dst, T(c0), T(c1), T(c2)},
+                                    P4G_K_SMOOTH, 4, &nbk, jac));
+                else
+                    P4G_TRY(stencil(l, SrcArr<T>{pm1}, OutCheb<T, true, 0>{b, pm2, dst, T(c0), T(c1), T(c2)},
+                                    P4G_K_SMOOTH, 4, &nbk, jac));
+            } else {
+                if (last && dots)
+                    P4G_TRY(stencil(l, SrcArr<T>{pm1}, OutCheb<T, false, 2>{b, nullptr, dst, T(c0), T(c1), T(c2)},
+                                    P4G_K_SMOOTH, 3, &nbk, jac));
+                else
+                    P4G_TRY(stencil(l, SrcArr<T>{pm1}, OutCheb<T, false, 0>{b, nullptr, dst, T(c0), T(c1), T(c2)},
+                                    P4G_K_SMOOTH, 3, &nbk, jac));
+            }
+            if (nb) *nb = nbk;
+            pm2 = pm1;
+            pm1 = dst;
+            (void)used;
+        }
+        return 0;
+    }
+
+    // PCMGMCycle: b, x of level l (x zero on entry when xzero).  On the
+    // finest level with dots, the last kernel fuses (x,b) and (x,x).
+    int mcycle(int l, const T* b, T* x, bool xzero, bool dots, long* nb) {
+        if (l == 0) {
+            P4G_TRY(coarse(b, x));
+            if (dots) {   // one-level multigrid: (x,b), (x,x) in a separate pass
+                const long n = (*lv)[0].n;
+                hipStream_t s = cx->stream;
+                if (nb) *nb = vblocks(n);
+                P4G_TRY(cx->launch(P4G_K_OTHER, true, 2.0 * n * sizeof(T), [&] {
+                    k_dot2<T><<<vblocks(n), 256, 0, s>>>(n, x, b, cx->part);
+                }));
+            }
+            return 0;
+        }
+        const LevelInfo& L = (*lv)[l];
+        Dev& d = dv[l];
+        const bool jac = jacobi != 0;
+        const double idg = jac ? 1.0 / L.d : 1.0;
+        const double s = smoother == SM_CHEB ? 2.0 / (L.emax + L.emin) : rich_scale;
+        // pre-smoothing and residual
+        bool zgf = false;
+        T* xpre = nullptr;
+        if (xzero && nu == 1 && zg_fusable(L)) {
+            zgf = true;
+            P4G_TRY(stencil(l, SrcScaled<T>{b, T(s * idg)}, OutResid<T, 0>{b, d.r}, P4G_K_RESID_ZG, 2, nullptr, jac));
+        } else {
+            xpre = d.t[3] ? d.t[3] : d.t[0];
+            if (xpre == x && !xzero) return set_error(PETSC_ERR_LIB, "internal buffer aliasing");
+            P4G_TRY(smooth(l, b, xzero ? nullptr : x, xpre, false, nullptr));
+            P4G_TRY(stencil(l, SrcArr<T>{xpre}, OutResid<T, 0>{b, d.r}, P4G_K_RESID_ZG, 3, nullptr, jac));
+        }
+        // coarse-grid correction
+        Dev& c = dv[l - 1];
+        P4G_TRY(restrict_(l, d.r, c.b));
+        for (int q = 0; q < cycles; ++q) P4G_TRY(mcycle(l - 1, c.b, c.x, q == 0, false, nullptr));
+        // x = pre + P x_c, then post-smoothing into x
+        T* x0 = zgf ? d.t[0] : xpre;
+        P4G_TRY(prolong(l, zgf, zgf ? b : xpre, T(s * idg), c.x, x0));
+        P4G_TRY(smooth(l, b, x0, x, dots, nb));
+        return 0;
+    }
+
+    // z = B r with fused (z,r), (z,z) partials; returns the block count
+    int pc_apply_dots(const T* rr, T* zz, long* nb) {
+        const LevelInfo& F = (*lv)[nlev - 1];
+        if (pctype == PCT_MG) return mcycle(nlev - 1, rr, zz, true, true, nb);
+        const double c = (pctype == PCT_JACOBI) ? 1.0 / F.d : 1.0;
+        if (pctype == PCT_JACOBI && (F.var || F.d != F.db))
+            return set_error(PETSC_ERR_SUP, "-pc_type jacobi on the device needs a constant diagonal");
+        *nb = vblocks(F.n);
+        return scale(F.n, T(c), rr, zz, true, true);
+    }
+
+    int pc_apply(const T* rr, T* zz) {
+        long nb = 0;
+        P4G_TRY(pc_apply_dots(rr, zz, &nb));
+        return 0;
+    }
+
+    int mat_mult(const T* xx, T* yy) {
+        return stencil(nlev - 1, SrcArr<T>{xx}, OutApply<T>{yy}, P4G_K_OTHER, 2, nullptr, false);
+    }
+
+    // KSPSolve_CG from x = 0.  history: norms of the convergence test.
+    struct CGOut { int its = 0, reason = 0; double rnorm0 = 0, rnorm = 0; std::vector<double> hist; };
+    int cg(const T* b, T* x, int normtype, double rtol, double atol, double dtol, int maxit,
+           CGOut& o, double* Sblk) {
+        const LevelInfo& F = (*lv)[nlev - 1];
+        const long n = F.n;
+        hipStream_t st = cx->stream;
+        o = CGOut();
+        P4G_TRY(fill(n, T(0), x));
+        P4G_TRY(fill(n, T(0), pa));
+        HIP_TRY(hipMemcpyAsync(r, b, n * sizeof(T), hipMemcpyDeviceToDevice, st));
+        const int fmode = normtype == NT_PRECOND ? FIN_BETA_PRECOND
+                        : normtype == NT_NATURAL ? FIN_BETA_NATURAL : FIN_BETA_UNPRECOND;
+        const int soff = (int)(Sblk - cx->S);
+        if (normtype == NT_UNPRECOND) {   // (r, r) of the initial residual
+            P4G_TRY(cx->launch(P4G_K_OTHER, true, 2.0 * n * sizeof(T), [&] {
+                k_scale<T, 2><<<vblocks(n), 256, 0, st>>>(n, T(1), r, w, cx->part);
+            }));
+            P4G_TRY(cx->finalize(vblocks(n), Sblk, FIN_RAW, 0));
+        }
+        long nb = 0;
+        P4G_TRY(pc_apply_dots(r, z, &nb));
+        if (normtype == NT_UNPRECOND) {
+            // S_RR <- TMP1 of the raw pass above (kept by a copy on device)
+            HIP_TRY(hipMemcpyAsync(Sblk + S_RR, Sblk + S_TMP1, sizeof(double), hipMemcpyDeviceToDevice, st));
+        }
+        P4G_TRY(cx->finalize(nb, Sblk, fmode, 1));
+        P4G_TRY(cx->readS(cx->hS));
+        const double* hs = cx->hS + soff;
+        double dp = hs[S_DP];
+        o.rnorm0 = dp;
+        o.rnorm = dp;
+        o.hist.push_back(dp);
+        const double ttol = std::max(rtol * dp, atol);
+        if (std::isnan(dp) || std::isinf(dp)) { o.reason = P4G_DIVERGED_NANORINF; return 0; }
+        if (dp <= ttol) { o.reason = dp < atol ? P4G_CONVERGED_ATOL : P4G_CONVERGED_RTOL; return 0; }
+        T* pold = pa;
+        T* pnew = pb;
+        double dpi_old = 0.0;
+        int i = 0;
+        for (; i < maxit; ++i) {
+            o.its = i + 1;
+            if (hs[S_RZ] == 0.0) { o.reason = P4G_CONVERGED_ATOL; break; }
+            // p = z + (beta/betaold) p ; w = A p ; (p, w)
+            long nbs = 0;
+            P4G_TRY(stencil(nlev - 1, SrcDir<T>{z, pold, pnew, Sblk + S_BCOEF}, OutApplyDot<T>{w},
+                            P4G_K_STENCIL_DOT, 4, &nbs, false));
+            P4G_TRY(cx->finalize(nbs, Sblk, FIN_ALPHA, 0));
+            std::swap(pold, pnew);   // pold now holds the current direction
+            const int gb = vblocks(n);
+            if (normtype == NT_UNPRECOND) {
+                P4G_TRY(cx->launch(P4G_K_CG_UPDATE, true, 6.0 * n * sizeof(T), [&] {
+                    k_cg_update<T, 1><<<gb, 256, 0, st>>>(n, Sblk, S_ALPHA, x, pold, r, w, cx->part);
+                }));
+                P4G_TRY(cx->finalize(gb, Sblk, FIN_RR, 0));
+            } else {
+                P4G_TRY(cx->launch(P4G_K_CG_UPDATE, true, 6.0 * n * sizeof(T), [&] {
+                    k_cg_update<T, 0><<<gb, 256, 0, st>>>(n, Sblk, S_ALPHA, x, pold, r, w, cx->part);
+                }));
+            }
+            P4G_TRY(pc_apply_dots(r, z, &nb));
+            P4G_TRY(cx->finalize(nb, Sblk, fmode, 0));
+            P4G_TRY(cx->readS(cx->hS));
+            const double dpi = hs[S_PW];
+            if (dpi == 0.0 || (i > 0 && ((dpi > 0) != (dpi_old > 0)))) {
+                o.reason = P4G_DIVERGED_INDEFINITE_MAT;
+                break;
+            }
+            dpi_old = dpi;
+            dp = hs[S_DP];
+            o.rnorm = dp;
+            o.hist.push_back(dp);
+            if (std::isnan(dp) || std::isinf(dp)) { o.reason = P4G_DIVERGED_NANORINF; break; }
+            if (dp <= ttol) { o.reason = dp < atol ? P4G_CONVERGED_ATOL : P4G_CONVERGED_RTOL; break; }
+            if (dp >= dtol * o.rnorm0) { o.reason = P4G_DIVERGED_DTOL; break; }
+        }
+        if (i >= maxit && !o.reason) o.reason = P4G_DIVERGED_ITS;
+        return 0;
+    }
+};
+
+// FP64 Lanczos (PCG) estimate of the extreme eigenvalues of D^-1 A on one
+// level: the same procedure as the oracle's KSPChebyshev estimate.
+static void tridiag_extreme(int m, const double* a, const double* b, double* emin, double* emax) {
+    double T[16][16];
+    std::memset(T, 0, sizeof T);
+    for (int i = 0; i < m; i++) {
+        T[i][i] = a[i];
+        if (i + 1 < m) T[i][i + 1] = T[i + 1][i] = b[i];
+    }
+    for (int sweep = 0; sweep < 100; sweep++) {
+        double off = 0.0;
+        for (int p = 0; p < m; p++)
+            for (int q = p + 1; q < m; q++) off += T[p][q] * T[p][q];
+        if (off < 1e-30) break;
+        for (int p = 0; p < m; p++)
+            for (int q = p + 1; q < m; q++) {
+                if (std::fabs(T[p][q]) < 1e-300) continue;
+                const double th = (T[q][q] - T[p][p]) / (2.0 * T[p][q]);
+                const double t = (th >= 0 ? 1.0 : -1.0) / (std::fabs(th) + std::sqrt(th * th + 1.0));
+                const double c = 1.0 / std::sqrt(t * t + 1.0), s = t * c;
+                for (int r = 0; r < m; r++) {
+                    const double trp = T[r][p], trq = T[r][q];
+                    T[r][p] = c * trp - s * trq;
+                    T[r][q] = s * trp + c * trq;
+                }
+                for (int r = 0; r < m; r++) {
+                    const double tpr = T[p][r], tqr = T[q][r];
+                    T[p][r] = c * tpr - s * tqr;
+                    T[q][r] = s * tpr + c * tqr;
+                }
+            }
+    }
+    *emin = *emax = T[0][0];
+    for (int i = 1; i < m; i++) {
+        *emin = std::min(*emin, T[i][i]);
+        *emax = std::max(*emax, T[i][i]);
+    }
+}
+
+// r -= a w ; z = id r (id array or constant) ; acc0 += r.z
+__global__ void __launch_bounds__(256)
+k_lanczos_update(long n, double a, double* __restrict__ r, const double* __restrict__ w,
+                 const double* __restrict__ idv, double idc, double* __restrict__ z,
+                 double* __restrict__ part) {
+    double acc = 0.0;
+    for (long q = blockIdx.x * 256L + threadIdx.x; q < n; q += (long)gridDim.x * 256) {
+        const double rv = r[q] - a * w[q];
+        r[q] = rv;
+        const double zv = rv * (idv ? idv[q] : idc);
+        z[q] = zv;
+        acc += rv * zv;
+    }
+    __shared__ double sm[4];
+    acc = wave_sum(acc);
+    if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = sm[0] + sm[1] + sm[2] + sm[3];
+        part[2 * blockIdx.x + 1] = 0.0;
+    }
+}
+
+struct OutIdiag {   // stores the Jacobi factor of every row
+    double* __restrict__ id;
+    static constexpr int NDOT = 0;
+    __device__ __forceinline__ void point(long q, double, double, double idg, double&, double&) const {
+        id[q] = idg;
+    }
+};
+
+}  // namespace p4g
+
+// ---------------------------------------------------------------- solver
+
+struct p4g_solver_s {
+    p4g_da_s* da = nullptr;
+    Options opt;
+    std::string opts_str;
+    // configuration
+    int pctype = PCT_MG, nlev = 1, cycles = 1, smoother = SM_CHEB, nu = 2, jacobi = 1;
+    double rich_scale = 1.0;
+    int esteig_steps = 10;
+    int normtype = NT_PRECOND, maxit = 10000;
+    double rtol = 1e-5, atol = 1e-50, dtol = 1e5;
+    bool mixed = false;
+    double inner_rtol = 1e-5;
+    int inner_maxit = 200;
+    // state
+    bool ready = false;
+    std::vector<LevelInfo> lv;
+    std::vector<double> Linv;
+    int n0 = 0;
+    Ctx cx;
+    std::unique_ptr<Engine<double>> e64;
+    std::unique_ptr<Engine<float>> e32;
+    // mixed-precision outer vectors; FP64 fine cells of a variable operator
+    double *xa = nullptr, *xb = nullptr, *kc64 = nullptr;
+    float *r32 = nullptr, *e32v = nullptr;
+    std::vector<void*> allocs;
+    p4g_Report rep;
+
+    ~p4g_solver_s() {
+        e64.reset();
+        e32.reset();
+        for (void* p : allocs) hipFree(p);
+        if (cx.part) hipFree(cx.part);
+        if (cx.S) hipFree(cx.S);
+        if (cx.hS) hipHostFree(cx.hS);
+        if (cx.stream) hipStreamDestroy(cx.stream);
+    }
+};
+
+namespace p4g {
+
+static int dalloc(p4g_solver_s* s, void** p, size_t bytes) {
+    hipError_t e = hipMalloc(p, std::max<size_t>(bytes, 8));
+    if (e != hipSuccess) return set_error(PETSC_ERR_MEM, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
+    s->allocs.push_back(*p);
+    return 0;
+}
+
+static bool is_bnd(const LevelInfo& L, int i, int j, int k) {
+    bool b = i == 0 || i == L.mx - 1;
+    if (L.dim > 1) b = b || j == 0 || j == L.my - 1;
+    if (L.dim > 2) b = b || k == 0 || k == L.mz - 1;
+    return b;
+}
+
+// host row of the level operator (FP64), for the coarse factorization
+static void host_row(const LevelInfo& L, int i, int j, int k, double& dg, double s[6]) {
+    for (int q = 0; q < 6; ++q) s[q] = 0.0;
+    if (!L.var) {
+        if (is_bnd(L, i, j, k)) { dg = L.db; return; }
+        dg = L.d;
+        s[0] = (i - 1 > 0) ? L.sx : 0; s[1] = (i + 1 < L.mx - 1) ? L.sx : 0;
+        if (L.dim > 1) { s[2] = (j - 1 > 0) ? L.sy : 0; s[3] = (j + 1 < L.my - 1) ? L.sy : 0; }
+        if (L.dim > 2) { s[4] = (k - 1 > 0) ? L.sz : 0; s[5] = (k + 1 < L.mz - 1) ? L.sz : 0; }
+        return;
+    }
+    auto cell = [&](int a, int b, int c) { return L.kcell[((long)c * L.cy + b) * L.cx + a]; };
+    if (is_bnd(L, i, j, k)) {
+        double sum = 0.0;
+        int cnt = 0;
+        for (int dk = -1; dk <= 0; ++dk)
+            for (int dj = -1; dj <= 0; ++dj)
+                for (int di = -1; di <= 0; ++di) {
+                    int a = i + di, b = j + dj, c = k + dk;
+                    if (a < 0 || b < 0 || c < 0 || a >= L.cx || b >= L.cy || c >= L.cz) continue;
+                    sum += cell(a, b, c);
+                    ++cnt;
+                }
+        dg = 2.0 * (L.ax + L.ay + L.az) * (sum / cnt);
+        return;
+    }
+    const double c000 = cell(i - 1, j - 1, k - 1), c100 = cell(i, j - 1, k - 1);
+    const double c010 = cell(i - 1, j, k - 1), c110 = cell(i, j, k - 1);
+    const double c001 = cell(i - 1, j - 1, k), c101 = cell(i, j - 1, k);
+    const double c011 = cell(i - 1, j, k), c111 = cell(i, j, k);
+    const double kw = 0.25 * (c000 + c010 + c001 + c011) * L.ax;
+    const double ke = 0.25 * (c100 + c110 + c101 + c111) * L.ax;
+    const double ks = 0.25 * (c000 + c100 + c001 + c101) * L.ay;
+    const double kn = 0.25 * (c010 + c110 + c011 + c111) * L.ay;
+    const double kd = 0.25 * (c000 + c100 + c010 + c110) * L.az;
+    const double ku = 0.25 * (c001 + c101 + c011 + c111) * L.az;
+    dg = kw + ke + ks + kn + kd + ku;
+    s[0] = (i - 1 > 0) ? kw : 0; s[1] = (i + 1 < L.mx - 1) ? ke : 0;
+    s[2] = (j - 1 > 0) ? ks : 0; s[3] = (j + 1 < L.my - 1) ? kn : 0;
+    s[4] = (k - 1 > 0) ? kd : 0; s[5] = (k + 1 < L.mz - 1) ? ku : 0;
+}
+
+// dense Cholesky of the coarsest operator and the inverse of its factor
+static int coarse_factor(const LevelInfo& L, std::vector<double>& Linv) {
+    const int n = (int)L.n;
+    if (n > MAX_COARSE)
+        return set_error(PETSC_ERR_SUP, "coarsest grid has %d points (> %d); use more multigrid levels",
+                         n, MAX_COARSE);
+    std::vector<double> A((size_t)n * n, 0.0);
+    const int mx = L.mx, my = L.my;
+    for (int k = 0; k < L.mz; ++k)
+        for (int j = 0; j < L.my; ++j)
+            for (int i = 0; i < L.mx; ++i) {
+                const long q = ((long)k * my + j) * mx + i;
+                double dg, s[6];
+                host_row(L, i, j, k, dg, s);
+                A[q * n + q] = dg;
+                if (s[0] != 0) A[q * n + q - 1] = -s[0];
+                if (s[1] != 0) A[q * n + q + 1] = -s[1];
+                if (s[2] != 0) A[q * n + q - mx] = -s[2];
+                if (s[3] != 0) A[q * n + q + mx] = -s[3];
+                if (s[4] != 0) A[q * n + q - (long)mx * my] = -s[4];
+                if (s[5] != 0) A[q * n + q + (long)mx * my] = -s[5];
+            }
+    // A = L L^T (lower, in place)
+    for (int c = 0; c < n; ++c) {
+        double dsum = A[(size_t)c * n + c];
+        for (int k = 0; k < c; ++k) dsum -= A[(size_t)c * n + k] * A[(size_t)c * n + k];
+        if (!(dsum > 0.0)) return set_error(PETSC_ERR_LIB, "coarse Cholesky: matrix not positive definite");
+        const double lcc = std::sqrt(dsum);
+        A[(size_t)c * n + c] = lcc;
+        for (int r = c + 1; r < n; ++r) {
+            double v = A[(size_t)r * n + c];
+            for (int k = 0; k < c; ++k) v -= A[(size_t)r * n + k] * A[(size_t)c * n + k];
+            A[(size_t)r * n + c] = v / lcc;
+        }
+    }
+    // Linv = L^-1 by forward substitution on the identity
+    Linv.assign((size_t)n * n, 0.0);
+    for (int c = 0; c < n; ++c) {
+        for (int r = c; r < n; ++r) {
+            double v = (r == c) ? 1.0 : 0.0;
+            for (int k = c; k < r; ++k) v -= A[(size_t)r * n + k] * Linv[(size_t)k * n + c];
+            Linv[(size_t)r * n + c] = v / A[(size_t)r * n + r];
+        }
+    }
+    return 0;
+}
+
+static int estimate_level(p4g_solver_s* s, int l) {
+    LevelInfo& L = s->lv[l];
+    const long n = L.n;
+    hipStream_t st = s->cx.stream;
+    std::vector<void*> tmp;
+    auto get = [&](double** p) -> int {
+        void* q = nullptr;
+        hipError_t e = hipMalloc(&q, n * sizeof(double));
+        if (e != hipSuccess) return set_error(PETSC_ERR_MEM, "hipMalloc (estimate): %s", hipGetErrorString(e));
+        tmp.push_back(q);
+        *p = (double*)q;
+        return 0;
+    };
+    struct Free { std::vector<void*>* t; ~Free() { for (void* q : *t) hipFree(q); } } fr{&tmp};
+    double *r, *z, *pa, *pb, *w, *idv = nullptr, *kc = nullptr;
+    P4G_TRY(get(&r)); P4G_TRY(get(&z)); P4G_TRY(get(&pa)); P4G_TRY(get(&pb)); P4G_TRY(get(&w));
+    const bool jac = s->jacobi != 0;
+    OpConst<double> oc;
+    OpVar<double> ov;
+    const dim3 g = stencil_grid(L.dim, L.mx, L.my, L.mz), bk = stencil_block(L.dim);
+    const long nbs = nblocks(g);
+    if (!L.var) {
+        oc.d = L.d; oc.db = L.db; oc.sx = L.sx; oc.sy = L.sy; oc.sz = L.sz;
+        oc.id = jac ? 1.0 / L.d : 1.0; oc.idb = jac ? 1.0 / L.db : 1.0;
+    } else {
+        const long nc = (long)L.cx * L.cy * L.cz;
+        void* q = nullptr;
+        HIP_TRY(hipMalloc(&q, nc * sizeof(double)));
+        tmp.push_back(q);
+        kc = (double*)q;
+        HIP_TRY(hipMemcpyAsync(kc, L.kcell.data(), nc * sizeof(double), hipMemcpyHostToDevice, st));
+        ov.kc = kc; ov.cx = L.cx; ov.cy = L.cy; ov.cz = L.cz;
+        ov.ax = L.ax; ov.ay = L.ay; ov.az = L.az; ov.jac = jac ? 1 : 0;
+        P4G_TRY(get(&idv));
+        k_stencil<double, 3, OpVar<double>, SrcZero<double>, OutIdiag>
+            <<<g, bk, 0, st>>>(L.mx, L.my, L.mz, KCH, ov, SrcZero<double>{}, OutIdiag{idv}, s->cx.part);
+        LAUNCH_CHECK();
+    }
+    const double idc = jac ? 1.0 / L.d : 1.0;
+    const int gb = vblocks(n);
+    double* S = s->cx.S + 16;
+    // r = noisy, z = D^-1 r, rz = (r,z), p = 0 (so p_new = z at beta = 0)
+    k_noisy<<<gb, 256, 0, st>>>(n, r);
+    k_fill<double><<<gb, 256, 0, st>>>(n, 0.0, pa);
+    HIP_TRY(hipMemsetAsync(S, 0, 16 * sizeof(double), st));
+    k_fill<double><<<gb, 256, 0, st>>>(n, 0.0, w);
+    k_lanczos_update<<<gb, 256, 0, st>>>(n, 0.0, r, w, idv, idc, z, s->cx.part);
+    LAUNCH_CHECK();
+    k_finalize<<<1, 256, 0, st>>>(gb, s->cx.part, S, FIN_RAW, 0);
+    P4G_TRY(s->cx.readS(s->cx.hS));
+    double rz = s->cx.hS[16 + S_TMP0], rzold = 0.0;
+    double al[16], be[16];
+    int m = 0;
+    double* pold = pa;
+    double* pnew = pb;
+    for (int it = 0; it < s->esteig_steps; ++it) {
+        if (!(rz > 0.0)) break;
+        double beta = 0.0;
+        if (it > 0) { beta = rz / rzold; be[it - 1] = beta; }
+        s->cx.hS[16 + S_BCOEF] = beta;
+        HIP_TRY(hipMemcpyAsync(S + S_BCOEF, &s->cx.hS[16 + S_BCOEF], sizeof(double), hipMemcpyHostToDevice, st));
+        if (L.var)
+            k_stencil<double, 3, OpVar<double>, SrcDir<double>, OutApplyDot<double>><<<g, bk, 0, st>>>(
+                L.mx, L.my, L.mz, KCH, ov, SrcDir<double>{z, pold, pnew, S + S_BCOEF}, OutApplyDot<double>{w}, s->cx.part);
+        else if (L.dim == 3)
+            k_stencil<double, 3, OpConst<double>, SrcDir<double>, OutApplyDot<double>><<<g, bk, 0, st>>>(
+                L.mx, L.my, L.mz, KCH, oc, SrcDir<double>{z, pold, pnew, S + S_BCOEF}, OutApplyDot<double>{w}, s->cx.part);
+        else if (L.dim == 2)
+            k_stencil<double, 2, OpConst<double>, SrcDir<double>, OutApplyDot<double>><<<g, bk, 0, st>>>(
+                L.mx, L.my, L.mz, KCH, oc, SrcDir<double>{z, pold, pnew, S + S_BCOEF}, OutApplyDot<double>{w}, s->cx.part);
+        else
+            k_stencil<double, 1, OpConst<double>, SrcDir<double>, OutApplyDot<double>><<<g, bk, 0, st>>>(
+                L.mx, L.my, L.mz, KCH, oc, SrcDir<double>{z, pold, pnew, S + S_BCOEF}, OutApplyDot<double>{w}, s->cx.part);
+        LAUNCH_CHECK();
+        std::swap(pold, pnew);
+        k_finalize<<<1, 256, 0, st>>>(nbs, s->cx.part, S, FIN_RAW, 0);
+        P4G_TRY(s->cx.readS(s->cx.hS));
+        const double pw = s->cx.hS[16 + S_TMP0];
+        if (!(pw > 0.0)) break;
+        const double alpha = rz / pw;
+        al[it] = alpha;
+        m = it + 1;
+        k_lanczos_update<<<gb, 256, 0, st>>>(n, alpha, r, w, idv, idc, z, s->cx.part);
+        LAUNCH_CHECK();
+        k_finalize<<<1, 256, 0, st>>>(gb, s->cx.part, S, FIN_RAW, 0);
+        P4G_TRY(s->cx.readS(s->cx.hS));
+        rzold = rz;
+        rz = s->cx.hS[16 + S_TMP0];
+        if (rz <= 1e-28 * std::fabs(rzold)) break;
+    }
+    double a[16], b[16], emin = 1.0, emax = 1.0;
+    for (int j = 0; j < m; ++j) {
+        a[j] = 1.0 / al[j] + (j > 0 ? be[j - 1] / al[j - 1] : 0.0);
+        if (j + 1 < m) b[j] = std::sqrt(be[j]) / al[j];
+    }
+    if (m > 0) tridiag_extreme(m, a, b, &emin, &emax);
+    L.emin = 0.0 * emin + 0.1 * emax;
+    L.emax = 0.0 * emin + 1.1 * emax;
+    HIP_TRY(hipStreamSynchronize(st));
+    return 0;
+}
+
+static const char* kUnsupported[] = {"pc_mg_galerkin", "pc_mg_type", "pc_mg_multiplicative_cycles",
+                                     "mg_levels_ksp_norm_type", nullptr};
+
+static int configure(p4g_solver_s* s) {
+    Options& o = s->opt;
+    const std::string kt = o.gets("ksp_type", "cg");
+    if (kt != "cg") return set_error(PETSC_ERR_SUP, "-ksp_type %s: the device path implements cg", kt.c_str());
+    const std::string pt = o.gets("pc_type", "");
+    if (pt == "mg") s->pctype = PCT_MG;
+    else if (pt == "jacobi") s->pctype = PCT_JACOBI;
+    else if (pt == "none") s->pctype = PCT_NONE;
+    else if (pt.empty())
+        return set_error(PETSC_ERR_SUP, "the device path needs -pc_type mg (or jacobi/none); PETSc's "
+                                        "default ILU is not implemented");
+    else return set_error(PETSC_ERR_SUP, "-pc_type %s is not implemented on the device", pt.c_str());
+    for (const char** u = kUnsupported; *u; ++u)
+        if (o.has(*u)) {
+            if (std::string(*u) == "pc_mg_type" && o.gets(*u, "") == "multiplicative") continue;
+            return set_error(PETSC_ERR_SUP, "-%s is not implemented on the device", *u);
+        }
+    s->rtol = o.getr("ksp_rtol", 1e-5);
+    s->atol = o.getr("ksp_atol", 1e-50);
+    s->dtol = o.getr("ksp_divtol", 1e5);
+    s->maxit = o.geti("ksp_max_it", 10000);
+    const std::string nt = o.gets("ksp_norm_type", "preconditioned");
+    if (nt == "preconditioned") s->normtype = NT_PRECOND;
+    else if (nt == "unpreconditioned") s->normtype = NT_UNPRECOND;
+    else if (nt == "natural") s->normtype = NT_NATURAL;
+    else return set_error(PETSC_ERR_SUP, "-ksp_norm_type %s not supported", nt.c_str());
+    s->nlev = s->pctype == PCT_MG ? o.geti("pc_mg_levels", s->da->refine + 1) : 1;
+    if (s->nlev < 1) return set_error(PETSC_ERR_ARG_OUTOFRANGE, "-pc_mg_levels must be >= 1");
+    const std::string ct = o.gets("pc_mg_cycle_type", "v");
+    s->cycles = ct == "w" ? 2 : 1;
+    if (ct != "v" && ct != "w") return set_error(PETSC_ERR_ARG_WRONG, "-pc_mg_cycle_type %s", ct.c_str());
+    const std::string lk = o.gets("mg_levels_ksp_type", "chebyshev");
+    if (lk == "chebyshev") s->smoother = SM_CHEB;
+    else if (lk == "richardson") s->smoother = SM_RICH;
+    else return set_error(PETSC_ERR_SUP, "-mg_levels_ksp_type %s not implemented on the device", lk.c_str());
+    const std::string lp = o.gets("mg_levels_pc_type", s->pctype == PCT_MG ? "sor" : "jacobi");
+    if (lp == "jacobi") s->jacobi = 1;
+    else if (lp == "none") s->jacobi = 0;
+    else if (s->nlev == 1) s->jacobi = 1;   /* one level: no smoother is used */
+    else
+        return set_error(PETSC_ERR_SUP, "-mg_levels_pc_type %s is not implemented on the device (PETSc's "
+                                        "default is sor); use -mg_levels_pc_type jacobi", lp.c_str());
+    s->nu = o.geti("mg_levels_ksp_max_it", 2);
+    if (s->nu < 1) return set_error(PETSC_ERR_ARG_OUTOFRANGE, "-mg_levels_ksp_max_it must be >= 1");
+    s->rich_scale = o.getr("mg_levels_ksp_richardson_scale", 1.0);
+    s->esteig_steps = std::max(1, std::min(15, o.geti("mg_levels_ksp_chebyshev_esteig_steps", 10)));
+    const std::string cks = o.gets("mg_coarse_ksp_type", "preonly");
+    const std::string cpc = o.gets("mg_coarse_pc_type", "lu");
+    if (cks != "preonly" || (cpc != "lu" && cpc != "cholesky" && cpc != "redundant"))
+        return set_error(PETSC_ERR_SUP, "coarse solver must be preonly + lu/cholesky/redundant");
+    const std::string pr = o.gets("p4g_precision", "double");
+    if (pr == "mixed") s->mixed = true;
+    else if (pr != "double") return set_error(PETSC_ERR_ARG_WRONG, "-p4g_precision %s", pr.c_str());
+    s->inner_rtol = o.getr("p4g_inner_rtol", 1e-5);
+    s->inner_maxit = o.geti("p4g_inner_max_it", 200);
+    if (s->mixed && s->pctype != PCT_MG) return set_error(PETSC_ERR_SUP, "-p4g_precision mixed needs -pc_type mg");
+    return 0;
+}
+
+// classify the rows the Jacobian callback inserts on one level
+static int extract_level(p4g_solver_s* s, LevelInfo& L, p4g_da_s* lda) {
+    p4g_da_s* da = s->da;
+    L.dim = lda->dim; L.mx = lda->mx; L.my = lda->dim > 1 ? lda->my : 1; L.mz = lda->dim > 2 ? lda->mz : 1;
+    L.n = (long)L.mx * L.my * L.mz;
+    for (int d = 0; d < 3; ++d) L.h[d] = d < L.dim ? lda->h(d) : 0.0;
+    if (!da->kcell.empty()) return 0;   // variable coefficient handled by the caller
+    if (!da->jac) return set_error(PETSC_ERR_ARG_WRONG, "no Jacobian callback: call p4g_da_set_jacobian_local");
+    p4g_mat_s M;
+    M.da = lda;
+    M.seen.assign(L.n, 0);
+    std::vector<double> u(L.n, 0.0);
+    std::vector<double*> rows;
+    std::vector<double**> planes;
+    void* au = make_view(lda, u.data(), rows, planes);
+    DMDALocalInfo info;
+    info_of(lda, &info);
+    int e = da->jac(&info, au, &M, &M, da->jacctx);
+    if (e) return e;
+    if (M.rows != L.n)
+        return set_error(PETSC_ERR_ARG_WRONG, "Jacobian callback set %ld of %ld rows on the %dx%dx%d grid",
+                         M.rows, L.n, L.mx, L.my, L.mz);
+    if (M.bad)
+        return set_error(PETSC_ERR_SUP, "Jacobian on the %dx%dx%d grid is not a constant-coefficient star "
+                         "stencil with eliminated Dirichlet rows (%ld rows differ; first: %s)",
+                         L.mx, L.my, L.mz, M.bad, M.why.c_str());
+    L.var = false;
+    L.d = M.have_i ? M.d : M.db;
+    L.db = M.have_b ? M.db : M.d;
+    L.sx = M.s[0]; L.sy = M.s[1]; L.sz = M.s[2];
+    return 0;
+}
+
+static int build_levels(p4g_solver_s* s) {
+    p4g_da_s* da = s->da;
+    const int nlev = s->nlev;
+    s->lv.assign(nlev, LevelInfo());
+    std::vector<p4g_da_s> grids(nlev);
+    grids[nlev - 1] = *da;
+    grids[nlev - 1].internal = true;
+    for (int l = nlev - 1; l > 0; --l) {
+        p4g_da_s c = grids[l];
+        int m[3] = {c.mx, c.my, c.mz};
+        for (int d = 0; d < c.dim; ++d) {
+            if (m[d] < 3 || (m[d] - 1) % 2)
+                return set_error(PETSC_ERR_ARG_OUTOFRANGE, "grid %dx%dx%d cannot be coarsened for %d levels",
+                                 da->mx, da->my, da->mz, nlev);
+            m[d] = (m[d] - 1) / 2 + 1;
+        }
+        c.mx = m[0]; c.my = m[1]; c.mz = m[2];
+        c.kcell.clear();
+        grids[l - 1] = c;
+    }
+    for (int l = 0; l < nlev; ++l) {
+        grids[l].kcell.clear();
+        P4G_TRY(extract_level(s, s->lv[l], &grids[l]));
+    }
+    if (!da->kcell.empty()) {
+        if (da->dim != 3) return set_error(PETSC_ERR_SUP, "cell coefficients need a 3D grid");
+        std::vector<double> kc = da->kcell;
+        for (int l = nlev - 1; l >= 0; --l) {
+            LevelInfo& L = s->lv[l];
+            L.var = true;
+            L.cx = L.mx - 1; L.cy = L.my - 1; L.cz = L.mz - 1;
+            const double dvol = L.h[0] * L.h[1] * L.h[2];
+            L.ax = dvol / (L.h[0] * L.h[0]); L.ay = dvol / (L.h[1] * L.h[1]); L.az = dvol / (L.h[2] * L.h[2]);
+            L.kcell = kc;
+            L.d = L.db = 1.0;   // unused by the variable operator
+            if (l > 0) {   // coarse cell = mean of its 8 fine cells
+                const int cx = L.cx / 2, cy = L.cy / 2, cz = L.cz / 2;
+                std::vector<double> c((size_t)cx * cy * cz);
+                for (int k = 0; k < cz; ++k)
+                    for (int j = 0; j < cy; ++j)
+                        for (int i = 0; i < cx; ++i) {
+                            double sum = 0.0;
+                            for (int dk = 0; dk < 2; ++dk)
+                                for (int dj = 0; dj < 2; ++dj)
+                                    for (int di = 0; di < 2; ++di)
+                                        sum += kc[((size_t)(2 * k + dk) * L.cy + (2 * j + dj)) * L.cx + (2 * i + di)];
+                            c[((size_t)k * cy + j) * cx + i] = 0.125 * sum;
+                        }
+                kc.swap(c);
+            }
+        }
+    }
+    return 0;
+}
+
+int solver_setup(p4g_solver_s* s) {
+    if (s->ready) return 0;
+    const double t0 = now_ms();
+    P4G_TRY(build_levels(s));
+    if (!s->cx.stream) HIP_TRY(hipStreamCreate(&s->cx.stream));
+    // partial-sum capacity: the largest grid of blocks of any fused pass
+    const LevelInfo& F = s->lv.back();
+    long cap = std::max<long>(VBLOCKS, nblocks(stencil_grid(F.dim, F.mx, F.my, F.mz)));
+    if (!s->cx.part) {
+        HIP_TRY(hipMalloc(&s->cx.part, 2 * cap * sizeof(double)));
+        s->cx.npart_cap = cap;
+        HIP_TRY(hipMalloc(&s->cx.S, 32 * sizeof(double)));
+        HIP_TRY(hipMemset(s->cx.S, 0, 32 * sizeof(double)));
+        HIP_TRY(hipHostMalloc((void**)&s->cx.hS, 32 * sizeof(double), hipHostMallocDefault));
+    }
+    if (s->pctype == PCT_MG) {
+        P4G_TRY(coarse_factor(s->lv[0], s->Linv));
+        s->n0 = (int)s->lv[0].n;
+        if (s->smoother == SM_CHEB)
+            for (int l = 1; l < s->nlev; ++l) P4G_TRY(estimate_level(s, l));
+    }
+    auto cfg = [&](auto& e) {
+        e->pctype = s->pctype; e->cycles = s->cycles; e->smoother = s->smoother; e->nu = s->nu;
+        e->jacobi = s->jacobi; e->rich_scale = s->rich_scale;
+    };
+    if (F.var) {
+        const size_t nc = F.kcell.size();
+        P4G_TRY(dalloc(s, (void**)&s->kc64, nc * sizeof(double)));
+        HIP_TRY(hipMemcpy(s->kc64, F.kcell.data(), nc * sizeof(double), hipMemcpyHostToDevice));
+    }
+    if (s->mixed) {
+        s->e32.reset(new Engine<float>);
+        cfg(s->e32);
+        P4G_TRY(s->e32->setup(&s->cx, &s->lv, s->Linv, s->n0));
+        const long n = F.n;
+        P4G_TRY(dalloc(s, (void**)&s->xa, n * sizeof(double)));
+        P4G_TRY(dalloc(s, (void**)&s->xb, n * sizeof(double)));
+        P4G_TRY(dalloc(s, (void**)&s->r32, n * sizeof(float)));
+        P4G_TRY(dalloc(s, (void**)&s->e32v, n * sizeof(float)));
+    } else {
+        s->e64.reset(new Engine<double>);
+        cfg(s->e64);
+        P4G_TRY(s->e64->setup(&s->cx, &s->lv, s->Linv, s->n0));
+    }
+    HIP_TRY(hipStreamSynchronize(s->cx.stream));
+    s->ready = true;
+    s->rep.nlevels = s->nlev;
+    s->rep.emax_fine = s->nlev > 1 ? s->lv.back().emax : 0.0;
+    s->rep.setup_ms = now_ms() - t0;
+    return 0;
+}
+
+// one FP64 stencil pass on the finest level (outer residual, MatMult)
+template <class Src, class Out>
+static int fine64(p4g_solver_s* s, Src src, Out out, int cls, double bytes, long* nb) {
+    const LevelInfo& F = s->lv.back();
+    const dim3 g = stencil_grid(F.dim, F.mx, F.my, F.mz), bk = stencil_block(F.dim);
+    if (nb) *nb = nblocks(g);
+    hipStream_t st = s->cx.stream;
+    double* part = s->cx.part;
+    if (F.var) {
+        OpVar<double> ov;
+        ov.kc = s->kc64; ov.cx = F.cx; ov.cy = F.cy; ov.cz = F.cz;
+        ov.ax = F.ax; ov.ay = F.ay; ov.az = F.az; ov.jac = 0;
+        return s->cx.launch(cls, true, bytes, [&] {
+            k_stencil<double, 3, OpVar<double>, Src, Out><<<g, bk, 0, st>>>(F.mx, F.my, F.mz, KCH, ov, src, out, part);
+        });
+    }
+    OpConst<double> oc;
+    oc.d = F.d; oc.db = F.db; oc.sx = F.sx; oc.sy = F.sy; oc.sz = F.sz; oc.id = oc.idb = 1.0;
+    return s->cx.launch(cls, true, bytes, [&] {
+        if (F.dim == 3)
+            k_stencil<double, 3, OpConst<double>, Src, Out><<<g, bk, 0, st>>>(F.mx, F.my, F.mz, KCH, oc, src, out, part);
+        else if (F.dim == 2)
+            k_stencil<double, 2, OpConst<double>, Src, Out><<<g, bk, 0, st>>>(F.mx, F.my, F.mz, KCH, oc, src, out, part);
+        else
+            k_stencil<double, 1, OpConst<double>, Src, Out><<<g, bk, 0, st>>>(F.mx, F.my, F.mz, KCH, oc, src, out, part);
+    });
+}
+
+// FP64 iterative refinement around the FP32 PCG + FP32 V-cycle:
+//   r = b - A x (FP64, stored FP32), solve A e = r by FP32 PCG to
+//   inner_rtol, x += e fused into the next residual pass.
+static int mixed_solve(p4g_solver_s* s, const double* b, double* x) {
+    Engine<float>& e = *s->e32;
+    const LevelInfo& F = s->lv.back();
+    const long n = F.n;
+    hipStream_t st = s->cx.stream;
+    double* S = s->cx.S + 16;
+    const int gb = vblocks(n);
+    P4G_TRY(s->cx.launch(P4G_K_OTHER, true, 20.0 * n, [&] {
+        k_fill<double><<<gb, 256, 0, st>>>(n, 0.0, s->xa);
+        k_convert<double, float><<<gb, 256, 0, st>>>(n, b, s->r32);
+        k_dot2<double><<<gb, 256, 0, st>>>(n, b, b, s->cx.part);
+    }));
+    P4G_TRY(s->cx.finalize(gb, S, FIN_RAW, 0));
+    P4G_TRY(s->cx.readS(s->cx.hS));
+    const double bnorm = std::sqrt(s->cx.hS[16 + S_TMP0]);
+    p4g_Report& R = s->rep;
+    R.rnorm0 = bnorm;
+    R.rnorm = bnorm;
+    R.inner_its = 0;
+    R.its = 0;
+    R.reason = 0;
+    for (int q = 0; q < 64; ++q) R.history[q] = 0.0;
+    int hn = 0;
+    R.history[hn++] = bnorm;
+    double* xcur = s->xa;
+    double* xnext = s->xb;
+    if (bnorm == 0.0) R.reason = P4G_CONVERGED_ATOL;
+    for (int it = 0; it < s->maxit && !R.reason; ++it) {
+        typename Engine<float>::CGOut co;
+        P4G_TRY(e.cg(s->r32, s->e32v, NT_PRECOND, s->inner_rtol, 0.0, 1e30, s->inner_maxit, co, s->cx.S));
+        R.inner_its += co.its;
+        long nbk = 0;
+        P4G_TRY(fine64(s, SrcSum<double, float>{xcur, s->e32v, xnext}, OutResidCvt<double, float>{b, s->r32},
+                       P4G_K_OTHER, 32.0 * n, &nbk));
+        std::swap(xcur, xnext);
+        P4G_TRY(s->cx.finalize(nbk, S, FIN_RAW, 0));
+        P4G_TRY(s->cx.readS(s->cx.hS));
+        const double rn = std::sqrt(s->cx.hS[16 + S_TMP0]);
+        R.its = it + 1;
+        R.rnorm = rn;
+        if (hn < 64) R.history[hn++] = rn;
+        if (std::isnan(rn) || std::isinf(rn)) { R.reason = P4G_DIVERGED_NANORINF; break; }
+        if (rn <= std::max(s->rtol * bnorm, s->atol)) {
+            R.reason = rn < s->atol ? P4G_CONVERGED_ATOL : P4G_CONVERGED_RTOL;
+            break;
+        }
+        if (rn >= s->dtol * bnorm) { R.reason = P4G_DIVERGED_DTOL; break; }
+    }
+    if (!R.reason) R.reason = P4G_DIVERGED_ITS;
+    HIP_TRY(hipMemcpyAsync(x, xcur, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+    return 0;
+}
+
+// |b - A x|_2 / |b|_2 in FP64 on the finest level, scratch of n doubles
+static int true_residual(p4g_solver_s* s, const double* b, const double* x, double* scratch, double* rel) {
+    const long n = s->lv.back().n;
+    hipStream_t st = s->cx.stream;
+    double* S = s->cx.S + 16;
+    long nbk = 0;
+    P4G_TRY(fine64(s, SrcArr<double>{x}, OutResid<double, 1>{b, scratch}, P4G_K_OTHER, 24.0 * n, &nbk));
+    P4G_TRY(s->cx.finalize(nbk, S, FIN_RAW, 0));
+    const int gb = vblocks(n);
+    k_dot2<double><<<gb, 256, 0, st>>>(n, b, b, s->cx.part);
+    LAUNCH_CHECK();
+    P4G_TRY(s->cx.finalize(gb, S + 2, FIN_RAW, 0));
+    P4G_TRY(s->cx.readS(s->cx.hS));
+    const double rr = s->cx.hS[16 + S_TMP0], bb = s->cx.hS[16 + 2 + S_TMP0];
+    *rel = bb > 0 ? std::sqrt(rr / bb) : 0.0;
+    return 0;
+}
+
+int ksp_solve_device(p4g_solver_s* s, const double* b, double* x) {
+    P4G_TRY(solver_setup(s));
+    hipStream_t st = s->cx.stream;
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    HIP_TRY(hipEventRecord(e0, st));
+    p4g_Report& R = s->rep;
+    if (s->mixed) {
+        P4G_TRY(mixed_solve(s, b, x));
+    } else {
+        typename Engine<double>::CGOut co;
+        P4G_TRY(s->e64->cg(b, x, s->normtype, s->rtol, s->atol, s->dtol, s->maxit, co, s->cx.S));
+        R.its = co.its; R.reason = co.reason; R.rnorm0 = co.rnorm0; R.rnorm = co.rnorm;
+        R.inner_its = 0;
+        for (size_t q = 0; q < 64; ++q) R.history[q] = q < co.hist.size() ? co.hist[q] : 0.0;
+    }
+    HIP_TRY(hipEventRecord(e1, st));
+    HIP_TRY(hipEventSynchronize(e1));
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, e0, e1);
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    R.solve_ms = ms;
+    s->cx.tm.collect();
+    double rel = 0.0;
+    P4G_TRY(true_residual(s, b, x, s->mixed ? s->xb : s->e64->w, &rel));
+    R.true_relres = rel;
+    return 0;
+}
+
+}  // namespace p4g
+
+// ================================================================== C ABI
+
+extern "C" {
+
+int p4g_da_create(int dim, int mx, int my, int mz, double Lx, double Ly, double Lz, DM* da) {
+    clear_error();
+    if (!da) return set_error(PETSC_ERR_ARG_WRONG, "p4g_da_create: null output");
+    if (dim < 1 || dim > 3) return set_error(1, "invalid dim for DMDA creation\n");
+    if (mx < 2 || (dim > 1 && my < 2) || (dim > 2 && mz < 2))
+        return set_error(PETSC_ERR_ARG_OUTOFRANGE, "p4g_da_create: need at least 2 points per dimension");
+    if (!(Lx > 0) || (dim > 1 && !(Ly > 0)) || (dim > 2 && !(Lz > 0)))
+        return set_error(PETSC_ERR_ARG_OUTOFRANGE, "p4g_da_create: domain lengths must be positive");
+    p4g_da_s* d = new p4g_da_s;
+    d->dim = dim; d->mx = mx; d->my = dim > 1 ? my : 1; d->mz = dim > 2 ? mz : 1;
+    d->L[0] = Lx; d->L[1] = dim > 1 ? Ly : 1.0; d->L[2] = dim > 2 ? Lz : 1.0;
+    *da = d;
+    return 0;
+}
+
+int p4g_da_set_from_options(DM da, const char* options) {
+    clear_error();
+    if (!da) return set_error(PETSC_ERR_ARG_WRONG, "p4g_da_set_from_options: null DM");
+    Options o;
+    o.parse(options);
+    int m[3] = {o.geti("da_grid_x", da->mx), o.geti("da_grid_y", da->my), o.geti("da_grid_z", da->mz)};
+    const int r = o.geti("da_refine", 0);
+    if (r < 0 || r > 14) return set_error(PETSC_ERR_ARG_OUTOFRANGE, "-da_refine %d out of range", r);
+    for (int d = 0; d < 3; ++d)
+        for (int q = 0; q < r; ++q) m[d] = 2 * (m[d] - 1) + 1;
+    da->mx = m[0];
+    da->my = da->dim > 1 ? m[1] : 1;
+    da->mz = da->dim > 2 ? m[2] : 1;
+    da->refine = r;
+    return 0;
+}
+
+int p4g_da_get_sizes(DM da, int* dim, int* mx, int* my, int* mz, int* refine) {
+    if (!da) return set_error(PETSC_ERR_ARG_WRONG, "p4g_da_get_sizes: null DM");
+    if (dim) *dim = da->dim;
+    if (mx) *mx = da->mx;
+    if (my) *my = da->my;
+    if (mz) *mz = da->mz;
+    if (refine) *refine = da->refine;
+    return 0;
+}
+
+int p4g_da_set_function_local(DM da, DMDASNESFunctionFn* fn, void* ctx) {
+    if (!da) return set_error(PETSC_ERR_ARG_WRONG, "p4g_da_set_function_local: null DM");
+    da->fn = fn;
+    da->fnctx = ctx;
+    return 0;
+}
+
+int p4g_da_set_jacobian_local(DM da, DMDASNESJacobianFn* fn, void* ctx) {
+    if (!da) return set_error(PETSC_ERR_ARG_WRONG, "p4g_da_set_jacobian_local: null DM");
+    da->jac = fn;
+    da->jacctx = ctx;
+    return 0;
+}
+
+int p4g_da_set_cell_coefficient(DM da, const double* kcell) {
+    if (!da) return set_error(PETSC_ERR_ARG_WRONG, "p4g_da_set_cell_coefficient: null DM");
+    if (da->dim != 3) return set_error(PETSC_ERR_SUP, "cell coefficients need a 3D grid");
+    const size_t nc = (size_t)(da->mx - 1) * (da->my - 1) * (da->mz - 1);
+    if (!kcell) { da->kcell.clear(); return 0; }
+    for (size_t q = 0; q < nc; ++q)
+        if (!(kcell[q] > 0.0)) return set_error(PETSC_ERR_ARG_OUTOFRANGE, "cell coefficient must be positive");
+    da->kcell.assign(kcell, kcell + nc);
+    return 0;
+}
+
+int p4g_da_destroy(DM* da) {
+    if (da && *da) { delete *da; *da = nullptr; }
+    return 0;
+}
+
+int p4g_solver_create(DM da, const char* options, p4g_Solver* out) {
+    clear_error();
+    if (!da || !out) return set_error(PETSC_ERR_ARG_WRONG, "p4g_solver_create: null argument");
+    p4g_solver_s* s = new p4g_solver_s;
+    s->da = da;
+    s->opts_str = options ? options : "";
+    s->opt.parse(options);
+    std::memset(&s->rep, 0, sizeof s->rep);
+    int e = configure(s);
+    if (e) { delete s; return e; }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+        delete s;
+        return set_error(PETSC_ERR_LIB, "p4g_solver_create: no HIP device is available (the solver "
+                                        "runs on the GPU only)");
+    }
+    *out = s;
+    return 0;
+}
+
+int p4g_solver_setup(p4g_Solver s) {
+    clear_error();
+    if (!s) return set_error(PETSC_ERR_ARG_WRONG, "p4g_solver_setup: null solver");
+    return solver_setup(s);
+}
+
+int p4g_ksp_solve_device(p4g_Solver s, const double* d_b, double* d_x, p4g_Report* rep) {
+    clear_error();
+    if (!s || !d_b || !d_x) return set_error(PETSC_ERR_ARG_WRONG, "p4g_ksp_solve_device: null argument");
+    P4G_TRY(ksp_solve_device(s, d_b, d_x));
+    if (rep) *rep = s->rep;
+    return 0;
+}
+
+int p4g_ksp_solve(p4g_Solver s, const double* b, double* x, p4g_Report* rep) {
+    clear_error();
+    if (!s || !b || !x) return set_error(PETSC_ERR_ARG_WRONG, "p4g_ksp_solve: null argument");
+    P4G_TRY(solver_setup(s));
+    const long n = s->lv.back().n;
+    double *db = nullptr, *dx = nullptr;
+    HIP_TRY(hipMalloc(&db, n * sizeof(double)));
+    if (hipMalloc(&dx, n * sizeof(double)) != hipSuccess) {
+        hipFree(db);
+        return set_error(PETSC_ERR_MEM, "p4g_ksp_solve: hipMalloc failed");
+    }
+    int e = 0;
+    if (hipMemcpy(db, b, n * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+        e = set_error(PETSC_ERR_LIB, "p4g_ksp_solve: upload failed");
+    if (!e) e = ksp_solve_device(s, db, dx);
+    if (!e && hipMemcpy(x, dx, n * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+        e = set_error(PETSC_ERR_LIB, "p4g_ksp_solve: download failed");
+    hipFree(db);
+    hipFree(dx);
+    if (!e && rep) *rep = s->rep;
+    return e;
+}
+
+int p4g_snes_solve(p4g_Solver s, double* u, p4g_Report* rep) {
+    clear_error();
+    if (!s || !u) return set_error(PETSC_ERR_ARG_WRONG, "p4g_snes_solve: null argument");
+    p4g_da_s* da = s->da;
+    if (!da->fn) return set_error(PETSC_ERR_ARG_WRONG, "no residual callback: call p4g_da_set_function_local");
+    const long n = da->n();
+    std::vector<double> F(n, 0.0), y(n, 0.0);
+    std::vector<double*> r1, r2;
+    std::vector<double**> p1, p2;
+    void* au = make_view(da, u, r1, p1);
+    void* aF = make_view(da, F.data(), r2, p2);
+    DMDALocalInfo info;
+    info_of(da, &info);
+    P4G_TRY(da->fn(&info, au, aF, da->fnctx));   // F(u0)
+    double f0 = 0.0;
+    for (double v : F) f0 += v * v;
+    P4G_TRY(p4g_ksp_solve(s, F.data(), y.data(), nullptr));
+    for (long q = 0; q < n; ++q) u[q] -= y[q];
+    P4G_TRY(da->fn(&info, au, aF, da->fnctx));   // F(u)
+    double f1 = 0.0;
+    for (double v : F) f1 += v * v;
+    s->rep.fnorm0 = std::sqrt(f0);
+    s->rep.fnorm1 = std::sqrt(f1);
+    if (rep) *rep = s->rep;
+    return 0;
+}
+
+int p4g_pc_apply_device(p4g_Solver s, const double* d_r, double* d_z) {
+    clear_error();
+    if (!s || !d_r || !d_z) return set_error(PETSC_ERR_ARG_WRONG, "p4g_pc_apply_device: null argument");
+    P4G_TRY(solver_setup(s));
+    if (s->mixed) {
+        Engine<float>& e = *s->e32;
+        const long n = s->lv.back().n;
+        hipStream_t st = s->cx.stream;
+        k_convert<double, float><<<vblocks(n), 256, 0, st>>>(n, d_r, s->r32);
+        P4G_TRY(e.pc_apply(s->r32, s->e32v));
+        k_convert<float, double><<<vblocks(n), 256, 0, st>>>(n, s->e32v, d_z);
+    } else {
+        P4G_TRY(s->e64->pc_apply(d_r, d_z));
+    }
+    HIP_TRY(hipStreamSynchronize(s->cx.stream));
+    s->cx.tm.collect();
+    return 0;
+}
+
+int p4g_pc_apply(p4g_Solver s, const double* r, double* z) {
+    clear_error();
+    if (!s || !r || !z) return set_error(PETSC_ERR_ARG_WRONG, "p4g_pc_apply: null argument");
+    P4G_TRY(solver_setup(s));
+    const long n = s->lv.back().n;
+    double *dr = nullptr, *dz = nullptr;
+    HIP_TRY(hipMalloc(&dr, n * sizeof(double)));
+    if (hipMalloc(&dz, n * sizeof(double)) != hipSuccess) { hipFree(dr); return set_error(PETSC_ERR_MEM, "hipMalloc"); }
+    int e = 0;
+    if (hipMemcpy(dr, r, n * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) e = set_error(PETSC_ERR_LIB, "upload");
+    if (!e) e = p4g_pc_apply_device(s, dr, dz);
+    if (!e && hipMemcpy(z, dz, n * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) e = set_error(PETSC_ERR_LIB, "download");
+    hipFree(dr);
+    hipFree(dz);
+    return e;
+}
+
+int p4g_mat_mult_device(p4g_Solver s, const double* d_x, double* d_y) {
+    clear_error();
+    if (!s || !d_x || !d_y) return set_error(PETSC_ERR_ARG_WRONG, "p4g_mat_mult_device: null argument");
+    P4G_TRY(solver_setup(s));
+    P4G_TRY(fine64(s, SrcArr<double>{d_x}, OutApply<double>{d_y}, P4G_K_OTHER, 16.0 * s->lv.back().n, nullptr));
+    HIP_TRY(hipStreamSynchronize(s->cx.stream));
+    return 0;
+}
+
+int p4g_mat_mult(p4g_Solver s, const double* x, double* y) {
+    clear_error();
+    if (!s || !x || !y) return set_error(PETSC_ERR_ARG_WRONG, "p4g_mat_mult: null argument");
+    P4G_TRY(solver_setup(s));
+    const long n = s->lv.back().n;
+    double *dx = nullptr, *dy = nullptr;
+    HIP_TRY(hipMalloc(&dx, n * sizeof(double)));
+    if (hipMalloc(&dy, n * sizeof(double)) != hipSuccess) { hipFree(dx); return set_error(PETSC_ERR_MEM, "hipMalloc"); }
+    int e = 0;
+    if (hipMemcpy(dx, x, n * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) e = set_error(PETSC_ERR_LIB, "upload");
+    if (!e) e = p4g_mat_mult_device(s, dx, dy);
+    if (!e && hipMemcpy(y, dy, n * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) e = set_error(PETSC_ERR_LIB, "download");
+    hipFree(dx);
+    hipFree(dy);
+    return e;
+}
+
+int p4g_solver_get_report(p4g_Solver s, p4g_Report* rep) {
+    if (!s || !rep) return set_error(PETSC_ERR_ARG_WRONG, "p4g_solver_get_report: null argument");
+    *rep = s->rep;
+    return 0;
+}
+
+int p4g_solver_destroy(p4g_Solver* s) {
+    if (s && *s) {
+        if ((*s)->cx.stream) hipStreamSynchronize((*s)->cx.stream);
+        delete *s;
+        *s = nullptr;
+    }
+    return 0;
+}
+
+int p4g_solver_set_timing(p4g_Solver s, int cls, int on) {
+    if (!s || cls < 0 || cls >= P4G_K_COUNT) return set_error(PETSC_ERR_ARG_WRONG, "p4g_solver_set_timing");
+    s->cx.tm.on[cls] = on != 0;
+    return 0;
+}
+
+int p4g_solver_kernel_stats(p4g_Solver s, int cls, int fine_only, long* launches, double* ms, double* bytes) {
+    if (!s || cls < 0 || cls >= P4G_K_COUNT) return set_error(PETSC_ERR_ARG_WRONG, "p4g_solver_kernel_stats");
+    const KStat& k = s->cx.tm.st[cls];
+    if (launches) *launches = fine_only ? k.fine_launches : k.launches;
+    if (ms) *ms = fine_only ? k.fine_ms : k.ms;
+    if (bytes) *bytes = fine_only ? k.fine_bytes : k.bytes;
+    return 0;
+}
+
+int p4g_solver_reset_stats(p4g_Solver s) {
+    if (!s) return set_error(PETSC_ERR_ARG_WRONG, "p4g_solver_reset_stats");
+    s->cx.tm.collect();
+    for (auto& k : s->cx.tm.st) k = KStat();
+    return 0;
+}
+
+int p4g_device_count(int* n) {
+    int c = 0;
+    if (hipGetDeviceCount(&c) != hipSuccess) c = 0;
+    if (n) *n = c;
+    return 0;
+}
+
+int p4g_device_synchronize(void) {
+    HIP_TRY(hipDeviceSynchronize());
+    return 0;
+}
+
+void* p4g_device_malloc(size_t bytes) {
+    void* p = nullptr;
+    if (hipMalloc(&p, bytes) != hipSuccess) {
+        set_error(PETSC_ERR_MEM, "p4g_device_malloc(%zu) failed", bytes);
+        return nullptr;
+    }
+    return p;
+}
+
+int p4g_device_free(void* p) {
+    if (p) HIP_TRY(hipFree(p));
+    return 0;
+}
+
+int p4g_memcpy_htod(void* dst, const void* src, size_t bytes) {
+    HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return 0;
+}
+
+int p4g_memcpy_dtoh(void* dst, const void* src, size_t bytes) {
+    HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+const char* p4g_version(void) { return "p4g 0.1 (gfx950)"; }
+
+}  // extern "C"

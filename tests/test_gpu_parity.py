@@ -1,0 +1,160 @@
+"""GPU parity tests: the HIP path (through the C ABI) against the CPU oracle
+on the same options, grids and right-hand sides.
+
+Bar (from the reference path's contract): the same CG iteration count
+(within 1), the final solution within 1e-9 relative in the inf-norm, and
+the same discretization error.  Full-size configurations are checked
+through size-independent properties: the closed-form discretization error
+of the sine eigen-modes, the true FP64 residual, and agreement of the
+mixed-precision and FP64 device solutions.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import oracle_py as O
+from p4pdes_amd import capi
+
+pytestmark = pytest.mark.gpu
+
+MG = "-pc_type mg -mg_levels_ksp_type chebyshev -mg_levels_pc_type jacobi"
+
+CASES = {
+    "1d_manupoly_cheb2": "-fsh_dim 1 -fsh_problem manupoly -da_refine 5 %s -ksp_rtol 1e-12" % MG,
+    "2d_manuexp_cheb1": "-fsh_dim 2 -da_refine 4 %s -mg_levels_ksp_max_it 1 -ksp_rtol 1e-10" % MG,
+    "2d_c1_129": ("-fsh_dim 2 -fsh_problem manusin -da_refine 6 %s -pc_mg_levels 6 "
+                  "-mg_levels_ksp_max_it 1 -ksp_rtol 1e-10" % MG),
+    "3d_aniso_cheb2": ("-fsh_dim 3 -fsh_problem manupoly -fsh_cx 0.5 -fsh_cy 2 -fsh_cz 3 "
+                       "-da_refine 4 %s -ksp_rtol 1e-10" % MG),
+    "3d_sin_wcycle_richardson": ("-fsh_dim 3 -fsh_problem manusin -da_refine 4 -pc_type mg "
+                                 "-pc_mg_cycle_type w -mg_levels_ksp_type richardson "
+                                 "-mg_levels_pc_type jacobi -mg_levels_ksp_max_it 1 "
+                                 "-mg_levels_ksp_richardson_scale 0.8 -ksp_rtol 1e-10"),
+    "3d_sin_cheb1_65": ("-fsh_dim 3 -fsh_problem manusin -da_refine 5 %s -pc_mg_levels 5 "
+                        "-mg_levels_ksp_max_it 1 -ksp_rtol 1e-10" % MG),
+    "3d_sin_cheb3": ("-fsh_dim 3 -fsh_problem manusin -da_refine 4 %s -mg_levels_ksp_max_it 3 "
+                     "-ksp_rtol 1e-10" % MG),
+    "3d_sinemodes": ("-fsh_dim 3 -fsh_problem sinemodes -fsh_modes_seed 7 -da_refine 4 %s "
+                     "-mg_levels_ksp_max_it 1 -ksp_rtol 1e-8" % MG),
+    "3d_noise_cheb2": ("-fsh_dim 3 -fsh_problem manusin -fsh_noise_sigma 1e-3 -da_refine 4 %s "
+                       "-ksp_rtol 1e-10" % MG),
+    "3d_varcoef": ("-fsh_dim 3 -fsh_problem varcoef -da_refine 4 %s -mg_levels_ksp_max_it 1 "
+                   "-ksp_rtol 1e-10" % MG),
+    "3d_unprecond_norm": ("-fsh_dim 3 -fsh_problem manusin -da_refine 3 %s -ksp_norm_type "
+                          "unpreconditioned -ksp_rtol 1e-10" % MG),
+    "3d_natural_norm": ("-fsh_dim 3 -fsh_problem manusin -da_refine 3 %s -ksp_norm_type natural "
+                        "-ksp_rtol 1e-10" % MG),
+    "2d_no_gbdry_random": ("-fsh_dim 2 -fsh_initial_gonboundary false -fsh_initial_type random "
+                           "-da_refine 3 %s -ksp_rtol 1e-10" % MG),
+    "2d_pc_jacobi": "-fsh_dim 2 -da_refine 3 -pc_type jacobi -ksp_rtol 1e-10",
+    "2d_pc_none": "-fsh_dim 2 -da_refine 2 -pc_type none -ksp_rtol 1e-10",
+    "3d_one_level": "-fsh_dim 3 -fsh_problem manusin -da_refine 2 -pc_type mg -pc_mg_levels 1",
+}
+
+
+def _rel(a, b):
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_fish_parity(name):
+    args = CASES[name]
+    print("case", name, args, flush=True)
+    rc, text, res, u = capi.fish_run(args, want_u=True)
+    assert rc == 0, text
+    orc, otext, ores, ou = O.fish_run(args, want_u=True)
+    assert orc == 0, otext
+    assert res.reason > 0, text
+    assert abs(res.its - ores.its) <= 1, (res.its, ores.its)
+    assert _rel(u, ou) < 1e-9
+    assert abs(res.errinf - ores.errinf) <= 1e-9 * max(ores.errinf, 1e-3) + 1e-13
+    # printed error line: same text as the oracle's (%.3e)
+    assert text.splitlines()[-1] == otext.splitlines()[-1]
+
+
+def test_golden_lines_with_jacobi_smoother():
+    # the golden fish.test1 problem with a Jacobi-Chebyshev smoother: the
+    # discretization error line matches the reference's exactly
+    args = ("-fsh_dim 1 -fsh_problem manupoly -da_refine 3 -pc_type mg -mg_levels_pc_type jacobi "
+            "-ksp_rtol 1.0e-12 -snes_monitor_short -ksp_converged_reason")
+    rc, text, res, _ = capi.fish_run(args)
+    assert rc == 0, text
+    lines = text.splitlines()
+    assert lines[0] == "  0 SNES Function norm 0.925546"
+    assert lines[2] == "  1 SNES Function norm < 1.e-11"
+    assert lines[3:] == ["problem manupoly on 17 point 1D grid:",
+                         "  error |u-uexact|_inf = 9.766e-04, |u-uexact|_h = 7.132e-04"]
+    orc, otext, ores, _ = O.fish_run(args)
+    assert otext == text
+
+
+OPS = [
+    "-fsh_dim 3 -fsh_problem manusin -da_refine 4 %s -ksp_rtol 1e-10" % MG,
+    "-fsh_dim 2 -fsh_problem manupoly -fsh_cx 0.3 -da_refine 5 %s -mg_levels_ksp_max_it 1" % MG,
+    "-fsh_dim 3 -fsh_problem varcoef -da_refine 3 %s -mg_levels_ksp_max_it 1" % MG,
+    "-fsh_dim 3 -fsh_problem manusin -da_refine 3 -pc_type mg -pc_mg_cycle_type w "
+    "-mg_levels_ksp_type richardson -mg_levels_pc_type jacobi -mg_levels_ksp_max_it 2",
+]
+
+
+@pytest.mark.parametrize("args", OPS, ids=["3d", "2d_aniso", "3d_varcoef", "3d_w_rich"])
+def test_operator_and_vcycle_parity(args):
+    print("case", args, flush=True)
+    with capi.Session(args) as s:
+        sv = capi.Solver(s)
+        rng = np.random.default_rng(5)
+        x = rng.standard_normal(s.n)
+        y = sv.mat_mult(x)
+        assert _rel(y, O.mat_mult(args, x)) < 1e-14
+        z = sv.pc_apply(x)
+        assert _rel(z, O.pc_apply(args, x)) < 1e-11
+        # the V-cycle operator is symmetric: <B x, w> = <x, B w>
+        w = rng.standard_normal(s.n)
+        bw = sv.pc_apply(w)
+        assert abs(z @ w - x @ bw) < 1e-11 * abs(z @ w)
+
+
+def test_rhs_parity():
+    args = "-fsh_dim 3 -fsh_problem manusin -fsh_noise_sigma 1e-3 -da_refine 3 %s" % MG
+    with capi.Session(args) as s:
+        F = np.zeros(s.n)
+        capi.check(capi.lib().p4g_memcpy_dtoh(F.ctypes.data, capi.fishlib().fish_session_rhs_device(s.h),
+                                              8 * s.n))
+    assert _rel(F, O.rhs(args)) < 1e-14
+
+
+def _eig_err_3d(m):
+    h = 1.0 / (m - 1)
+    return abs(3 * math.pi ** 2 / ((12 / h ** 2) * math.sin(math.pi * h / 2) ** 2) - 1)
+
+
+def test_c2_257_fp64_closed_form():
+    # C2: 257^3, 7 levels to 5^3 Cholesky, FP64, rtol 1e-10
+    args = ("-fsh_dim 3 -fsh_problem manusin -da_refine 7 %s -pc_mg_levels 7 "
+            "-mg_levels_ksp_max_it 1 -ksp_rtol 1e-10" % MG)
+    with capi.Session(args) as s:
+        rep = s.solve()
+        print("C2 its", rep.its, "solve_ms", rep.solve_ms, "relres", rep.true_relres, flush=True)
+        assert rep.reason == 2
+        einf, _ = s.errors()
+    assert abs(einf - 1.254995e-05) < 5e-4 * 1.254995e-05
+    assert abs(_eig_err_3d(257) - 1.254995e-05) < 1e-11
+    assert rep.true_relres < 1e-9
+
+
+@pytest.mark.parametrize("refine", [4, 5])
+def test_mixed_matches_fp64(refine):
+    base = ("-fsh_dim 3 -fsh_problem manusin -fsh_noise_sigma 1e-3 -da_refine %d %s -pc_mg_levels %d "
+            "-mg_levels_ksp_max_it 2 -ksp_rtol 1e-10" % (refine, MG, refine))
+    rc, text, res, u = capi.fish_run(base + " -p4g_precision mixed", want_u=True)
+    assert rc == 0, text
+    orc, otext, ores, ou = O.fish_run(base + " -ksp_norm_type unpreconditioned", want_u=True)
+    assert orc == 0
+    assert res.true_relres <= 1e-10 * 1.0001
+    # same discretization error (4 significant digits) and a small distance
+    # to the FP64 oracle solution
+    assert abs(res.errinf - ores.errinf) < 1e-4 * ores.errinf
+    assert _rel(u, ou) < 1e-7
+    print("mixed: outer its", res.its, "inner its", res.inner_its, "rel diff to FP64 oracle",
+          _rel(u, ou))
