@@ -1,0 +1,306 @@
+"""Benchmark: 3D Poisson DOF/s to rtol 1e-10 with CG + geometric multigrid on
+one MI355X, mixed precision vs FP64, p50/p99 solve latency.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-513]
+
+A step is one KSP solve (CG + V(1,1) Chebyshev/Jacobi multigrid, from x = 0)
+of the 257^3 FP64 system (16,974,593 DOF = mx*my*mz, the reference's global
+Vec size), with F(u0) already resident in HBM; set-up (callbacks,
+hierarchy, eigen-estimates, coarse factorization) is outside the timed
+region, as KSPSolve is timed apart from SNESSolve in the reference's
+studies.  Prints ONE JSON line.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+MG = ("-pc_type mg -mg_levels_ksp_type chebyshev -mg_levels_pc_type jacobi "
+      "-mg_coarse_pc_type cholesky")
+C2 = ("-fsh_dim 3 -fsh_problem manusin -da_refine 7 -pc_mg_levels 7 %s "
+      "-mg_levels_ksp_max_it 1 -ksp_rtol 1e-10" % MG)
+# C3: 513^3, f = manufactured + N(0, 1e-3) noise, Chebyshev degree 2; both
+# precisions converge on the true FP64 residual |b - A x|_2 / |b|_2 <= 1e-10
+C3 = ("-fsh_dim 3 -fsh_problem manusin -fsh_noise_sigma 1e-3 -fsh_noise_seed 0 -da_refine 8 "
+      "-pc_mg_levels 8 %s -mg_levels_ksp_max_it 2 -ksp_rtol 1e-10" % MG)
+HBM_PEAK_GBS = 8000.0
+METRIC = "3D Poisson DOF/s to rtol 1e-10, mixed vs FP64 MG-CG; p50/p99 solve latency"
+
+
+def pct(v, q):
+    v = sorted(v)
+    if not v:
+        return None
+    k = (len(v) - 1) * q
+    lo, hi = int(math.floor(k)), int(math.ceil(k))
+    return v[lo] + (v[hi] - v[lo]) * (k - lo)
+
+
+def time_solves(sess, steps, warmup):
+    from p4pdes_amd import capi
+    for _ in range(warmup):
+        sess.solve()
+    capi.check(capi.lib().p4g_device_synchronize())
+    lat = []
+    reps = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        t = time.perf_counter()
+        reps.append(sess.solve())
+        lat.append((time.perf_counter() - t) * 1e3)
+    capi.check(capi.lib().p4g_device_synchronize())
+    total = time.perf_counter() - t0
+    return total, lat, reps
+
+
+def kernel_roofline(sess, cls):
+    """Average duration and algorithmic bytes per launch of kernel class
+    `cls` on the finest level, from HIP events recorded around each launch
+    on the solver's stream during the solves since the last reset."""
+    n, ms, by = sess.kernel_stats(cls, fine_only=True)
+    if n == 0 or ms <= 0:
+        return None
+    return {"launches": n, "avg_ms": ms / n, "bytes_per_launch": by / n,
+            "gbs": (by / n) / (ms / n * 1e-3) / 1e9}
+
+
+def stencil_apply_roofline(sess, reps=20):
+    """y = A x (2 FP64 passes) on the session's finest grid, HIP-event timed."""
+    from p4pdes_amd import capi
+    L = capi.lib()
+    nb = 8 * sess.n
+    x = L.p4g_device_malloc(nb)
+    y = L.p4g_device_malloc(nb)
+    try:
+        rhs = capi.fishlib().fish_session_rhs_device(sess.h)
+        capi.check(L.p4g_mat_mult_device(sess.solver, rhs, x))   # warm + non-trivial input
+        sess.reset_stats()
+        sess.set_timing(capi.K_APPLY, True)
+        for _ in range(reps):
+            capi.check(L.p4g_mat_mult_device(sess.solver, x, y))
+        sess.set_timing(capi.K_APPLY, False)
+        r = kernel_roofline(sess, capi.K_APPLY)
+    finally:
+        L.p4g_device_free(x)
+        L.p4g_device_free(y)
+    return r
+
+
+def cpu_baseline(args, budget_s=30.0):
+    """The oracle (C restatement, CSR operator as PETSc's AIJ) on the host
+    cores, one solve of the same system after one warm-up."""
+    from oracle import oracle_py
+    ncores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    threads = max(1, min(16, ncores))
+    times, res = oracle_py.fish_time(args, nthreads=threads, warmup=1, nsolves=1)
+    _, mx, my, mz = oracle_py.grid_shape(args)
+    n = mx * my * mz
+    return {"value": n / (times[0] * 1e-3), "unit": "DOF/s", "cores": threads, "kind": "port",
+            "sample": "one KSP solve of the full 257^3 system (after one warm-up), oracle CSR "
+                      "CG + V(1,1) Chebyshev/Jacobi multigrid, %d OpenMP threads, %d its, %.1f s"
+                      % (threads, res.its, times[0] * 1e-3)}
+
+
+SERVE_SHAPES = [(3, 4), (3, 5), (3, 6), (3, 7), (2, 7), (2, 9), (2, 10)]   # (dim, refine)
+SERVE_OPTS = ("-fsh_problem zero -pc_type mg -mg_levels_ksp_type chebyshev -mg_levels_pc_type jacobi "
+              "-mg_levels_ksp_max_it 1 -p4g_precision mixed -ksp_rtol 1e-8")
+
+
+def sine_mode_request(dim, m, seed):
+    """b = measure * f at interior vertices for f = sum of 8 sine modes with
+    N(0,1) amplitudes and wavenumbers 1..8 (seeded); the exact discrete
+    solution is sum a_k mode_k / lambda_h(k)."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    h = 1.0 / (m - 1)
+    xs = np.arange(m) * h
+    f = np.zeros((m,) * dim)
+    xh = np.zeros((m,) * dim)
+    for _ in range(8):
+        a = rng.standard_normal()
+        ks = rng.integers(1, 9, size=dim)
+        vecs = [np.sin(k * np.pi * xs) for k in ks]
+        lam = sum(4.0 / h ** 2 * np.sin(k * np.pi * h / 2) ** 2 for k in ks)
+        mode = vecs[0]
+        for v in vecs[1:]:
+            mode = np.multiply.outer(v, mode)    # z (slowest) ... x (fastest)
+        f += a * mode
+        xh += (a / lam) * mode
+    b = (h ** dim) * f
+    for ax in range(dim):        # boundary rows: b = 0 (x = 0 there)
+        sl = [slice(None)] * dim
+        sl[ax] = 0
+        b[tuple(sl)] = 0.0
+        sl[ax] = m - 1
+        b[tuple(sl)] = 0.0
+    return b.ravel(), xh.ravel()
+
+
+def serving(rate=40.0, nreq=80, seed=2026, budget=64 << 30):
+    """Poisson-arrival stream of mixed-size solves through the
+    continuous-batching server; p50/p99 latency and throughput."""
+    import numpy as np
+    from p4pdes_amd import capi
+    rng = np.random.default_rng(seed)
+    sess = {}
+    for dim, r in SERVE_SHAPES:
+        sess[(dim, r)] = capi.Session("-fsh_dim %d -da_refine %d -pc_mg_levels %d %s"
+                                      % (dim, r, r, SERVE_OPTS))
+    shapes = [SERVE_SHAPES[i] for i in rng.integers(0, len(SERVE_SHAPES), size=nreq)]
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, size=nreq))
+    reqs = []
+    for q, (dim, r) in enumerate(shapes):
+        m = 2 ** (r + 1) + 1
+        b, xh = sine_mode_request(dim, m, seed * 1000 + q)
+        reqs.append((dim, r, m, b, xh if q < 12 else None))
+    out = []
+    with capi.Server(budget) as srv:
+        t0 = time.perf_counter()
+        ids = []
+        for q, (dim, r, m, b, _) in enumerate(reqs):
+            delay = arrivals[q] - (time.perf_counter() - t0)
+            if delay > 0:
+                time.sleep(delay)
+            ids.append(srv.submit(sess[(dim, r)].solver, b))
+        for q, rid in enumerate(ids):
+            out.append(srv.wait(rid))
+        st = srv.stats()
+    errs = []
+    for q, (dim, r, m, b, xh) in enumerate(reqs):
+        if xh is not None:
+            errs.append(float(np.max(np.abs(out[q][0] - xh)) / np.max(np.abs(xh))))
+    for s in sess.values():
+        s.close()
+    its = [rep.its for _, rep in out]
+    return {"p50_ms": st.p50_ms, "p99_ms": st.p99_ms, "mean_ms": st.mean_ms, "max_ms": st.max_ms,
+            "queue_p50_ms": st.queue_p50_ms, "throughput_dof_per_s": st.dof_per_s,
+            "requests": int(st.completed), "rate_per_s": rate, "wall_s": st.wall_s,
+            "rounds": int(st.rounds), "graph_captures": int(st.graph_captures),
+            "peak_hbm_gb": st.peak_bytes / 2 ** 30, "its_median": float(np.median(its)),
+            "max_rel_err_vs_exact_discrete": max(errs) if errs else None,
+            "all_converged": all(rep.reason > 0 for _, rep in out),
+            "workload": "Poisson arrivals (seeded) at %.0f/s of %d solves, sizes uniform over 33^3, 65^3, "
+                        "129^3, 257^3, 257^2, 1025^2, 2049^2; RHS = 8 random sine modes, N(0,1) "
+                        "amplitudes; zero Dirichlet; mixed precision to rtol 1e-8; V(1,1) "
+                        "Chebyshev/Jacobi to 5^d; HBM budget %d GiB" % (rate, nreq, budget >> 30)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-513", action="store_true", help="skip the 513^3 mixed/FP64 runs")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving stream")
+    a = ap.parse_args()
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this build runs on one GPU only; --gpus %d is not supported "
+              "(the multi-GPU z-slab decomposition is a design, not built)" % a.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+    from p4pdes_amd import capi
+    if capi.device_count() < 1:
+        print("bench.py: no HIP device", file=sys.stderr)
+        sys.exit(1)
+
+    out = {"metric": METRIC, "unit": "DOF/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+           "data": "synthetic (manufactured sine right-hand side, closed-form solution)"}
+
+    # ---- headline: C2 257^3 FP64 ----------------------------------------
+    print("bench: setting up 257^3", file=sys.stderr, flush=True)
+    with capi.Session(C2) as s:
+        n = s.n
+        s.solve()
+        # roofline of the dominant kernel: time every class once (untimed run
+        # is unaffected), then event-time the dominant class in the timed run
+        s.reset_stats()
+        for c in range(capi.K_COUNT):
+            s.set_timing(c, True)
+        s.solve()
+        by_cls = {c: s.kernel_stats(c)[1] for c in range(capi.K_COUNT)}
+        for c in range(capi.K_COUNT):
+            s.set_timing(c, False)
+        dominant = max((c for c in by_cls if c not in (capi.K_OTHER, capi.K_REDUCE)),
+                       key=lambda c: by_cls[c])
+        breakdown = {capi.KERNEL_NAMES[c]: round(by_cls[c], 4) for c in by_cls}
+        s.reset_stats()
+        s.set_timing(dominant, True)
+        print("bench: timing %d solves" % a.steps, file=sys.stderr, flush=True)
+        total, lat, reps = time_solves(s, a.steps, a.warmup)
+        s.set_timing(dominant, False)
+        roof = kernel_roofline(s, dominant)
+        einf, e2h = s.errors()
+        apply257 = stencil_apply_roofline(s)
+        rep = reps[-1]
+    ms = total / a.steps * 1e3
+    out["value"] = n / (ms * 1e-3)
+    out["ms_per_step"] = ms
+    out["latency_ms"] = {"p50": pct(lat, 0.5), "p99": pct(lat, 0.99)}
+    out["config"] = {"workload": "3D Poisson 257^3 vertex grid (16,974,593 DOF), 7-point stencil, "
+                                 "u=sin(pi x)sin(pi y)sin(pi z), FP64 CG + 7-level V(1,1) "
+                                 "Chebyshev(1)/Jacobi, 5^3 Cholesky, rtol 1e-10 (preconditioned "
+                                 "norm, PETSc's CG default)",
+                     "grid": [257, 257, 257], "levels": 7, "its": rep.its,
+                     "true_relres": rep.true_relres, "errinf": einf,
+                     "errinf_closed_form": 1.254995e-05, "parallelism": "single GPU"}
+    out["roofline"] = {"kernel": capi.KERNEL_NAMES[dominant], "bound": "hbm",
+                       "achieved": roof["gbs"] if roof else None, "peak": HBM_PEAK_GBS,
+                       "unit": "GB/s", "frac": (roof["gbs"] / HBM_PEAK_GBS) if roof else None,
+                       "traffic": None,
+                       "avg_ms": roof["avg_ms"] if roof else None,
+                       "algorithmic_bytes_per_launch": roof["bytes_per_launch"] if roof else None,
+                       "launches": roof["launches"] if roof else None}
+    out["kernel_ms_per_solve"] = breakdown
+    out["stencil_apply_257"] = apply257
+
+    # ---- C3: 513^3 mixed vs FP64 -----------------------------------------
+    if not a.no_513:
+        c3 = {}
+        sols = {}
+        for prec in ("double", "mixed"):
+            extra = " -ksp_norm_type unpreconditioned" if prec == "double" else " -p4g_precision mixed"
+            print("bench: 513^3 %s" % prec, file=sys.stderr, flush=True)
+            with capi.Session(C3 + extra) as s:
+                tot, l3, r3 = time_solves(s, 3, 1)
+                m3 = tot / 3 * 1e3
+                e3, _ = s.errors()
+                sols[prec] = s.solution()
+                c3[prec] = {"dof_per_s": s.n / (m3 * 1e-3), "ms_per_solve": m3, "its": r3[-1].its,
+                            "inner_its": r3[-1].inner_its, "true_relres": r3[-1].true_relres,
+                            "errinf_vs_exact": e3}
+                if prec == "double":
+                    c3["stencil_apply_513"] = stencil_apply_roofline(s)
+        import numpy as np
+        d = float(np.max(np.abs(sols["mixed"] - sols["double"])) / np.max(np.abs(sols["double"])))
+        del sols
+        c3["speedup_mixed_over_fp64"] = c3["mixed"]["dof_per_s"] / c3["double"]["dof_per_s"]
+        c3["mixed_vs_fp64_rel_inf_diff"] = d
+        c3["workload"] = ("3D Poisson 513^3 (135,005,697 DOF), f = manufactured + N(0,1e-3) noise, "
+                          "8-level V(2,2) Chebyshev/Jacobi; FP64: CG on the FP64 hierarchy; mixed: "
+                          "FP64 iterative refinement around FP32 PCG + FP32 V-cycle; both to "
+                          "|b-Ax|_2/|b|_2 <= 1e-10")
+        out["mixed_vs_fp64_513"] = c3
+
+    # ---- C5: serving stream ------------------------------------------------
+    if not a.no_serve:
+        print("bench: serving stream", file=sys.stderr, flush=True)
+        out["serving"] = serving()
+
+    # ---- CPU baseline ------------------------------------------------------
+    if not a.no_cpu:
+        print("bench: CPU baseline (oracle)", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(C2)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

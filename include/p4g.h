@@ -48,7 +48,7 @@ typedef struct p4g_solver_s *p4g_Solver;
 typedef struct {
     int    its;            /* outer iterations (PETSc ksp->its)                   */
     int    reason;         /* KSPConvergedReason                                  */
-    int    inner_its;      /* mixed precision: inner FP32 PCG iterations, total   */
+    int    inner_its;      /* mixed precision: FP64 residual replacements made    */
     int    nlevels;        /* multigrid levels                                    */
     double rnorm0, rnorm;  /* norm used by the convergence test, first and last  */
     double true_relres;    /* |b - A x|_2 / |b|_2 computed in FP64 after the solve */
@@ -92,10 +92,13 @@ int p4g_da_destroy(DM *da);
  *   -mg_levels_ksp_max_it n, -mg_levels_ksp_richardson_scale s,
  *   -mg_levels_ksp_chebyshev_esteig_steps n,
  *   -mg_coarse_ksp_type preonly, -mg_coarse_pc_type lu|cholesky|redundant,
- *   -p4g_precision double|mixed  (mixed: FP64 iterative refinement around an
- *       FP32 PCG whose V-cycle runs in FP32; FP64 residual, outer norms and
- *       coarsest Cholesky; converged when |b-Ax|_2/|b|_2 <= rtol),
- *   -p4g_inner_rtol r, -p4g_inner_max_it n.
+ *   -p4g_precision double|mixed  (mixed: PCG with FP32 vectors and an FP32
+ *       V-cycle, FP64 dot accumulation and coarsest Cholesky, and FP64
+ *       residual replacement: the FP32 correction is folded into an FP64
+ *       iterate and b - A x recomputed in FP64 whenever the iterated residual
+ *       has fallen by -p4g_reliable_delta (default 1e-2); converged when the
+ *       FP64 |b-Ax|_2 <= max(rtol |b|_2, atol)),
+ *   -p4g_reliable_delta d.
  * Options of PETSc objects the device path does not implement (-pc_type
  * ilu|sor|lu, -mg_levels_pc_type sor, -pc_mg_galerkin, ...) are rejected
  * with PETSC_ERR_SUP. */
@@ -133,6 +136,7 @@ enum {
     P4G_K_SMOOTH,          /* x <- x + s D^-1 (b - A x) (Chebyshev step)       */
     P4G_K_COARSE,          /* coarsest-grid Cholesky solve                     */
     P4G_K_REDUCE,          /* finalisation of fused dot products               */
+    P4G_K_APPLY,           /* y = A x, plain operator apply (MatMult)          */
     P4G_K_OTHER,
     P4G_K_COUNT
 };
@@ -142,6 +146,45 @@ int p4g_solver_set_timing(p4g_Solver s, int kernel_class, int on);
 int p4g_solver_kernel_stats(p4g_Solver s, int kernel_class, int fine_only, long *launches,
                             double *ms, double *bytes);
 int p4g_solver_reset_stats(p4g_Solver s);
+
+/* -------------------------------------------------------------- serving */
+
+/* A continuous-batching solve server (no counterpart in the reference,
+ * whose fish.c solves one problem per process).  Requests are KSP solves
+ * A x = b with the operator hierarchy of a set-up solver (a V(1,1)
+ * Chebyshev- or Richardson-Jacobi multigrid on a 2D or 3D constant-
+ * coefficient grid); each is run with the mixed-precision PCG of
+ * -p4g_precision mixed to the solver's -ksp_rtol.  A worker thread admits
+ * requests in arrival order when their device footprint fits the HBM
+ * budget, advances all active requests one iteration per round with
+ * batched launches (equal-sized multigrid levels of different requests
+ * share a launch; the V-cycle launch sequence is replayed from a hipGraph
+ * until the active set changes) and releases a request's memory when it
+ * finishes.  b and x are host arrays owned by the caller; both must stay
+ * valid until the request has finished.
+ * Options: -p4g_serve_graph 1|0, -p4g_serve_max_slots n (default 64). */
+typedef struct p4g_server_s *p4g_Server;
+
+typedef struct {
+    long   submitted, completed, rejected, active, queued, rounds, graph_captures;
+    double p50_ms, p99_ms, max_ms, mean_ms; /* latency: submit -> result on host */
+    double queue_p50_ms;                    /* submit -> admission                */
+    double wall_s;                          /* first submit -> last completion    */
+    double dof_per_s;                       /* DOF of completed requests / wall_s */
+    double peak_bytes;                      /* admitted footprint high-water mark */
+} p4g_ServerStats;
+
+int p4g_server_create(size_t hbm_budget_bytes, const char *options, p4g_Server *srv);
+/* device footprint of one request for solver s (bytes) */
+int p4g_server_request_bytes(p4g_Solver s, size_t *bytes);
+int p4g_server_submit(p4g_Server srv, p4g_Solver s, const double *b, double *x, long *id);
+/* state: 0 queued, 1 starting, 2 iterating, 3 done, -1 failed */
+int p4g_server_poll(p4g_Server srv, long id, int *state);
+/* blocks until the request is done; rep->solve_ms is its latency,
+ * rep->setup_ms its queueing time, rep->inner_its its FP64 replacements */
+int p4g_server_wait(p4g_Server srv, long id, p4g_Report *rep);
+int p4g_server_stats(p4g_Server srv, p4g_ServerStats *st);
+int p4g_server_destroy(p4g_Server *srv);
 
 /* ---------------------------------------------------- device utilities */
 

@@ -525,7 +525,17 @@ int oracle_varcoef_apply(int mx, int my, int mz, double Lx, double Ly, double Lz
 /* InitialState (poissonfunctions.c:260-346): zeros or uniform random, then
  * g on the boundary unless gbdry is false */
 void oracle_initial_state(const ogrid *g, const oprob *p, int random, int gbdry, double *u) {
-    for (long q = 0; q < g->n; q++) u[q] = random ? oracle_noisy(q + 7777777) : 0.0;
+    /* random: the splitmix64 stream of the product's VecSetRandom (seed
+     * 0x12345678, counter 1, 2, ...); PETSc's own generator is not
+     * reproducible here, and the converged u does not depend on u0 */
+    for (long q = 0; q < g->n; q++) {
+        if (!random) { u[q] = 0.0; continue; }
+        uint64_t z = 0x12345678ull + 0x9E3779B97F4A7C15ull * (uint64_t)(q + 1);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z ^= z >> 31;
+        u[q] = (double)(z >> 11) * (1.0 / 9007199254740992.0);
+    }
     if (!gbdry) return;
     for (int k = 0; k < g->mz; k++)
         for (int j = 0; j < g->my; j++)

@@ -14,9 +14,9 @@ LIBFISH = os.path.join(LIBDIR, "libp4g_fish.so")
 
 # kernel classes (enum in p4g.h)
 K_STENCIL_DOT, K_CG_UPDATE, K_RESID_ZG, K_RESTRICT, K_PROLONG, K_SMOOTH, K_COARSE, \
-    K_REDUCE, K_OTHER, K_COUNT = range(10)
+    K_REDUCE, K_APPLY, K_OTHER, K_COUNT = range(11)
 KERNEL_NAMES = ["stencil_dot", "cg_update", "resid_zg", "restrict", "prolong", "smooth",
-                "coarse", "reduce", "other"]
+                "coarse", "reduce", "apply", "other"]
 
 REASONS = {2: "CONVERGED_RTOL", 3: "CONVERGED_ATOL", 4: "CONVERGED_ITS", -3: "DIVERGED_ITS",
            -4: "DIVERGED_DTOL", -5: "DIVERGED_BREAKDOWN", -8: "DIVERGED_INDEFINITE_MAT",
@@ -264,3 +264,82 @@ class Solver:
         rep = Report()
         check(lib().p4g_ksp_solve(self.s, b.ctypes.data, x.ctypes.data, ctypes.byref(rep)))
         return x, rep
+
+
+class ServerStats(ctypes.Structure):
+    """p4g_ServerStats."""
+    _fields_ = [(n, ctypes.c_long) for n in ("submitted", "completed", "rejected", "active", "queued",
+                                             "rounds", "graph_captures")] + \
+               [(n, ctypes.c_double) for n in ("p50_ms", "p99_ms", "max_ms", "mean_ms", "queue_p50_ms",
+                                               "wall_s", "dof_per_s", "peak_bytes")]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+def _server_sigs(L):
+    vp = ctypes.c_void_p
+    L.p4g_server_create.argtypes = [ctypes.c_size_t, ctypes.c_char_p, ctypes.POINTER(vp)]
+    L.p4g_server_request_bytes.argtypes = [vp, ctypes.POINTER(ctypes.c_size_t)]
+    L.p4g_server_submit.argtypes = [vp, vp, vp, vp, ctypes.POINTER(ctypes.c_long)]
+    L.p4g_server_poll.argtypes = [vp, ctypes.c_long, ctypes.POINTER(ctypes.c_int)]
+    L.p4g_server_wait.argtypes = [vp, ctypes.c_long, ctypes.POINTER(Report)]
+    L.p4g_server_stats.argtypes = [vp, ctypes.POINTER(ServerStats)]
+    L.p4g_server_destroy.argtypes = [ctypes.POINTER(vp)]
+
+
+class Server:
+    """Continuous-batching solve server (p4g_server_*).  Requests reference a
+    set-up solver (e.g. a Session's) and a host right-hand side."""
+
+    def __init__(self, hbm_budget_bytes, options=""):
+        L = lib()
+        _server_sigs(L)
+        h = ctypes.c_void_p()
+        check(L.p4g_server_create(int(hbm_budget_bytes), options.encode(), ctypes.byref(h)))
+        self.h = h
+        self._keep = {}
+
+    @staticmethod
+    def request_bytes(solver):
+        L = lib()
+        _server_sigs(L)
+        b = ctypes.c_size_t()
+        check(L.p4g_server_request_bytes(solver, ctypes.byref(b)))
+        return b.value
+
+    def submit(self, solver, b, x=None):
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        if x is None:
+            x = np.zeros_like(b)
+        i = ctypes.c_long()
+        check(lib().p4g_server_submit(self.h, solver, b.ctypes.data, x.ctypes.data, ctypes.byref(i)))
+        self._keep[i.value] = (b, x)
+        return i.value
+
+    def poll(self, rid):
+        s = ctypes.c_int()
+        check(lib().p4g_server_poll(self.h, rid, ctypes.byref(s)))
+        return s.value
+
+    def wait(self, rid):
+        rep = Report()
+        check(lib().p4g_server_wait(self.h, rid, ctypes.byref(rep)))
+        b, x = self._keep.pop(rid)
+        return x, rep
+
+    def stats(self):
+        st = ServerStats()
+        check(lib().p4g_server_stats(self.h, ctypes.byref(st)))
+        return st
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().p4g_server_destroy(ctypes.byref(self.h))
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()

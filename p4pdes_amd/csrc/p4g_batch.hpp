@@ -1,0 +1,222 @@
+// p4g_batch.hpp -- ragged batched launches for the serving loop.
+//
+// One launch covers a list of work items (a grid function of one request on
+// one multigrid level); items may have different sizes.  Item t owns the
+// blocks [start_t, start_t + gx*gy*gz) of a 1-D launch grid; a block finds
+// its item by binary search over the starts and decodes its (bx,by,bz)
+// within the item's own grid, then runs exactly the per-block code of the
+// single-grid kernels (p4g_kernels.hpp), so a request's arithmetic does not
+// depend on which other requests share the launch.
+#pragma once
+
+#include "p4g_kernels.hpp"
+
+namespace p4g {
+
+struct GridRef {
+    int  mx, my, mz, kch;   // grid and marching chunk
+    int  gx, gy, gz;        // the item's block grid
+    long start;             // first block of the item in the launch
+};
+
+template <class Item>
+__device__ __forceinline__ int find_item(const Item* __restrict__ it, int n, long b) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (it[mid].g.start <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ void decode(const GridRef& g, long b, int& bx, int& by, int& bz, long& lb) {
+    lb = b - g.start;
+    long r = lb;
+    bx = (int)(r % g.gx);
+    r /= g.gx;
+    by = (int)(r % g.gy);
+    bz = (int)(r / g.gy);
+}
+
+// ---------------------------------------------------------------- stencil
+
+template <class Op, class Src, class Out>
+struct StencilItem {
+    GridRef g;
+    double* part;          // this item's partial sums (2 per block)
+    const double* flag;    // optional: skip the item when *flag == 0
+    Op op;
+    Src src;
+    Out out;
+};
+
+template <typename T, int DIM, class Op, class Src, class Out>
+__global__ void __launch_bounds__(NT)
+k_stencil_batch(const StencilItem<Op, Src, Out>* __restrict__ items, int nitems) {
+    const int t = find_item(items, nitems, (long)blockIdx.x);
+    const StencilItem<Op, Src, Out>& it = items[t];
+    if (it.flag && *it.flag == 0.0) return;   // whole block, uniform
+    int bx, by, bz;
+    long lb;
+    decode(it.g, blockIdx.x, bx, by, bz, lb);
+    double a0 = 0.0, a1 = 0.0;
+    stencil_body<T, DIM, Op, Src, Out>(bx, by, bz, it.g.mx, it.g.my, it.g.mz, it.g.kch, it.op, it.src, it.out,
+                                       a0, a1);
+    block_partials<Out::NDOT>(a0, a1, it.part, lb);
+}
+
+// y = c1 u + c2 D^-1 (b - A u); with dots (runtime flag): acc0 += y.b, acc1 += y.y
+template <typename T>
+struct OutChebR {
+    const T* __restrict__ b;
+    T* __restrict__ y;
+    T c1, c2;
+    int dots;
+    static constexpr int NDOT = 2;
+    __device__ __forceinline__ void point(long q, T u, T Au, T idg, double& a0, double& a1) const {
+        const T bq = b[q];
+        const T v = c1 * u + c2 * (idg * (bq - Au));
+        y[q] = v;
+        if (dots) {
+            a0 += double(v) * double(bq);
+            a1 += double(v) * double(v);
+        }
+    }
+};
+
+// ---------------------------------------------------------------- transfers
+
+template <typename T>
+struct RestrictItem {
+    GridRef g;             // coarse grid
+    int mxf, myf, mzf;
+    const T* rf;
+    T* bc;
+};
+
+template <typename T, int DIM>
+__global__ void __launch_bounds__(NT)
+k_restrict_batch(const RestrictItem<T>* __restrict__ items, int nitems) {
+    const int t = find_item(items, nitems, (long)blockIdx.x);
+    const RestrictItem<T>& it = items[t];
+    int bx, by, bz;
+    long lb;
+    decode(it.g, blockIdx.x, bx, by, bz, lb);
+    restrict_body<T, DIM>(bx, by, bz, it.mxf, it.myf, it.mzf, it.g.mx, it.g.my, it.g.mz, it.g.kch, it.rf, it.bc);
+}
+
+template <typename T>
+struct ProlongItem {
+    GridRef g;             // fine grid
+    int mxc, myc;
+    const T* xb;
+    T c;
+    const T* xc;
+    T* out;
+};
+
+template <typename T, int DIM>
+__global__ void __launch_bounds__(NT)
+k_prolong_zg_batch(const ProlongItem<T>* __restrict__ items, int nitems) {
+    const int t = find_item(items, nitems, (long)blockIdx.x);
+    const ProlongItem<T>& it = items[t];
+    int bx, by, bz;
+    long lb;
+    decode(it.g, blockIdx.x, bx, by, bz, lb);
+    prolong_body<T, DIM, true>(bx, by, bz, it.g.mx, it.g.my, it.g.mz, it.mxc, it.myc, it.g.kch, it.xb, it.c,
+                               it.xc, it.out);
+}
+
+// ------------------------------------------------------------- coarse solve
+
+template <typename T>
+struct CoarseItem {
+    int n;
+    const double* Linv;
+    const T* b;
+    T* x;
+};
+
+// one block per item; dynamic LDS of 2 * max n doubles
+template <typename T>
+__global__ void __launch_bounds__(1024)
+k_coarse_batch(const CoarseItem<T>* __restrict__ items) {
+    extern __shared__ double sh[];
+    const CoarseItem<T>& it = items[blockIdx.x];
+    coarse_body<T>(it.n, it.Linv, it.b, it.x, sh);
+}
+
+// ---------------------------------------------------------- vector kernels
+
+// x = keep x + a p ; r -= a w ; acc0 += r.r  (a = S[S_ALPHA], keep = *keep)
+template <typename T>
+struct UpdateItem {
+    GridRef g;             // only start and gx (= number of blocks) are used
+    long n;
+    const double* S;
+    const double* keep;
+    T *x, *r;
+    const T *p, *w;
+    double* part;
+};
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_update_batch(const UpdateItem<T>* __restrict__ items, int nitems) {
+    const int t = find_item(items, nitems, (long)blockIdx.x);
+    const UpdateItem<T>& it = items[t];
+    const long lb = blockIdx.x - it.g.start;
+    const T a = T(it.S[S_ALPHA]);
+    const bool keep = *it.keep != 0.0;
+    double acc = 0.0;
+    for (long q = lb * 256 + threadIdx.x; q < it.n; q += (long)it.g.gx * 256) {
+        it.x[q] = keep ? it.x[q] + a * it.p[q] : a * it.p[q];
+        const T rv = it.r[q] - a * it.w[q];
+        it.r[q] = rv;
+        acc += double(rv) * double(rv);
+    }
+    __shared__ double sm[4];
+    acc = wave_sum(acc);
+    if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        it.part[2 * lb] = sm[0] + sm[1] + sm[2] + sm[3];
+        it.part[2 * lb + 1] = 0.0;
+    }
+}
+
+// -------------------------------------------------------------- finalisation
+
+struct FinItem {
+    const double* part;
+    long nparts;
+    double* S;
+    int mode, first;
+    const double* flag;    // optional: skip when *flag == 0
+};
+
+__global__ void __launch_bounds__(256)
+k_finalize_batch(const FinItem* __restrict__ items) {
+    const FinItem& it = items[blockIdx.x];
+    if (it.flag && *it.flag == 0.0) return;
+    __shared__ double s0[256], s1[256];
+    double a = 0.0, b = 0.0;
+    for (long q = threadIdx.x; q < it.nparts; q += 256) {
+        a += it.part[2 * q];
+        b += it.part[2 * q + 1];
+    }
+    s0[threadIdx.x] = a;
+    s1[threadIdx.x] = b;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+            s0[threadIdx.x] += s0[threadIdx.x + w];
+            s1[threadIdx.x] += s1[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) finalize_scalars(it.S, it.mode, it.first, s0[0], s1[0]);
+}
+
+}  // namespace p4g

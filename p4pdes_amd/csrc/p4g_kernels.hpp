@@ -242,15 +242,15 @@ __device__ __forceinline__ void block_partials(double a0, double a1, double* __r
 //   2D: (ceil((mx-1)/64), ceil(ceil(my/kch)/4), 1),  block (64,4)
 //   1D: (ceil((mx-1)/64), 1, 1),                     block (64,1)
 template <typename T, int DIM, class Op, class Src, class Out>
-__global__ void __launch_bounds__(NT)
-k_stencil(int mx, int my, int mz, int kch, Op op, Src src, Out out, double* __restrict__ part) {
-    const int i = 1 + blockIdx.x * TX + threadIdx.x;
-    double a0 = 0.0, a1 = 0.0;
+__device__ __forceinline__ void stencil_body(int bx, int by, int bz, int mx, int my, int mz, int kch,
+                                             const Op& op, const Src& src, const Out& out,
+                                             double& a0, double& a1) {
+    const int i = 1 + bx * TX + threadIdx.x;
     const T z0 = T(0);
     if (DIM == 3) {
-        const int j = blockIdx.y * TY + threadIdx.y;
+        const int j = by * TY + threadIdx.y;
         const long sy = mx, sz = (long)mx * my;
-        const int k0 = blockIdx.z * kch;
+        const int k0 = bz * kch;
         const int k1 = min(k0 + kch, mz);
         if (i < mx && j < my && k0 < k1) {
             const bool bij = (i == mx - 1) || (j == 0) || (j == my - 1);
@@ -285,7 +285,7 @@ k_stencil(int mx, int my, int mz, int kch, Op op, Src src, Out out, double* __re
             }
         }
     } else if (DIM == 2) {
-        const int chunk = blockIdx.y * TY + threadIdx.y;
+        const int chunk = by * TY + threadIdx.y;
         const long sy = mx;
         const int j0 = chunk * kch;
         const int j1 = min(j0 + kch, my);
@@ -341,6 +341,14 @@ k_stencil(int mx, int my, int mz, int kch, Op op, Src src, Out out, double* __re
             }
         }
     }
+}
+
+template <typename T, int DIM, class Op, class Src, class Out>
+__global__ void __launch_bounds__(NT)
+k_stencil(int mx, int my, int mz, int kch, Op op, Src src, Out out, double* __restrict__ part) {
+    double a0 = 0.0, a1 = 0.0;
+    stencil_body<T, DIM, Op, Src, Out>(blockIdx.x, blockIdx.y, blockIdx.z, mx, my, mz, kch, op, src, out,
+                                       a0, a1);
     const long bid = blockIdx.x + (long)gridDim.x * (blockIdx.y + (long)gridDim.y * blockIdx.z);
     block_partials<Out::NDOT>(a0, a1, part, bid);
 }
@@ -352,10 +360,10 @@ k_stencil(int mx, int my, int mz, int kch, Op op, Src src, Out out, double* __re
 // the fine plane 2K+1 is reused as plane 2(K+1)-1.
 // Grid: (ceil(mxc/64), ceil(myc/4) [3D] | ceil(ceil(myc/kch)/4) [2D], ceil(mzc/kch) [3D])
 template <typename T, int DIM>
-__global__ void __launch_bounds__(NT)
-k_restrict(int mxf, int myf, int mzf, int mxc, int myc, int mzc, int kch,
-           const T* __restrict__ rf, T* __restrict__ bc) {
-    const int I = blockIdx.x * TX + threadIdx.x;
+__device__ __forceinline__ void restrict_body(int bx, int by, int bz, int mxf, int myf, int mzf, int mxc,
+                                              int myc, int mzc, int kch, const T* __restrict__ rf,
+                                              T* __restrict__ bc) {
+    const int I = bx * TX + threadIdx.x;
     if (I >= mxc) return;
     const int i0 = 2 * I;
     const bool hw = i0 - 1 >= 0, he = i0 + 1 < mxf;
@@ -370,7 +378,7 @@ k_restrict(int mxf, int myf, int mzf, int mxc, int myc, int mzc, int kch,
         return;
     }
     if (DIM == 2) {
-        const int chunk = blockIdx.y * TY + threadIdx.y;
+        const int chunk = by * TY + threadIdx.y;
         const int J0 = chunk * kch, J1 = min(J0 + kch, myc);
         if (J0 >= J1) return;
         T slo = (2 * J0 - 1 >= 0) ? row((long)(2 * J0 - 1) * mxf) : T(0);
@@ -383,9 +391,9 @@ k_restrict(int mxf, int myf, int mzf, int mxc, int myc, int mzc, int kch,
         }
         return;
     }
-    const int J = blockIdx.y * TY + threadIdx.y;
+    const int J = by * TY + threadIdx.y;
     if (J >= myc) return;
-    const int K0 = blockIdx.z * kch, K1 = min(K0 + kch, mzc);
+    const int K0 = bz * kch, K1 = min(K0 + kch, mzc);
     const int j0 = 2 * J;
     const long sxy = (long)mxf * myf;
     auto plane = [&](int k) -> T {     // x,y-weighted sum on fine plane k
@@ -406,15 +414,22 @@ k_restrict(int mxf, int myf, int mzf, int mxc, int myc, int mzc, int kch,
     }
 }
 
+template <typename T, int DIM>
+__global__ void __launch_bounds__(NT)
+k_restrict(int mxf, int myf, int mzf, int mxc, int myc, int mzc, int kch, const T* __restrict__ rf,
+           T* __restrict__ bc) {
+    restrict_body<T, DIM>(blockIdx.x, blockIdx.y, blockIdx.z, mxf, myf, mzf, mxc, myc, mzc, kch, rf, bc);
+}
+
 // out = base + P x_c, base = c b (ZG: zero-guess smoothing result) or x.
 // Thread per fine column (i,j) marching k; the coarse in-plane interpolant
 // of plane K is computed once and reused for fine planes 2K-1, 2K, 2K+1.
 // Grid: (ceil(mxf/64), ceil(myf/4) [3D] | chunks [2D], ceil(mzf/kch) [3D])
 template <typename T, int DIM, bool ZG>
-__global__ void __launch_bounds__(NT)
-k_prolong(int mxf, int myf, int mzf, int mxc, int myc, int kch, const T* __restrict__ xb,
-          T c, const T* __restrict__ xc, T* __restrict__ out) {
-    const int i = blockIdx.x * TX + threadIdx.x;
+__device__ __forceinline__ void prolong_body(int bx, int by, int bz, int mxf, int myf, int mzf, int mxc,
+                                             int myc, int kch, const T* __restrict__ xb, T c,
+                                             const T* __restrict__ xc, T* __restrict__ out) {
+    const int i = bx * TX + threadIdx.x;
     if (i >= mxf) return;
     const int ia = i >> 1, ib = (i + 1) >> 1;          // equal when i is even
     const bool iodd = i & 1;
@@ -428,7 +443,7 @@ k_prolong(int mxf, int myf, int mzf, int mxc, int myc, int kch, const T* __restr
         return;
     }
     if (DIM == 2) {
-        const int chunk = blockIdx.y * TY + threadIdx.y;
+        const int chunk = by * TY + threadIdx.y;
         const int j0 = chunk * kch, j1 = min(j0 + kch, myf);
         for (int j = j0; j < j1; ++j) {
             const int ja = j >> 1, jb = (j + 1) >> 1;
@@ -439,7 +454,7 @@ k_prolong(int mxf, int myf, int mzf, int mxc, int myc, int kch, const T* __restr
         }
         return;
     }
-    const int j = blockIdx.y * TY + threadIdx.y;
+    const int j = by * TY + threadIdx.y;
     if (j >= myf) return;
     const int ja = j >> 1, jb = (j + 1) >> 1;
     const bool jodd = j & 1;
@@ -449,7 +464,7 @@ k_prolong(int mxf, int myf, int mzf, int mxc, int myc, int kch, const T* __restr
         T v = crow(base + (long)ja * mxc);
         return jodd ? T(0.5) * (v + crow(base + (long)jb * mxc)) : v;
     };
-    const int k0 = blockIdx.z * kch, k1 = min(k0 + kch, mzf);
+    const int k0 = bz * kch, k1 = min(k0 + kch, mzf);
     if (k0 >= k1) return;
     const long sxy = (long)mxf * myf;
     int K = k0 >> 1;
@@ -468,14 +483,20 @@ k_prolong(int mxf, int myf, int mzf, int mxc, int myc, int kch, const T* __restr
     }
 }
 
+template <typename T, int DIM, bool ZG>
+__global__ void __launch_bounds__(NT)
+k_prolong(int mxf, int myf, int mzf, int mxc, int myc, int kch, const T* __restrict__ xb, T c,
+          const T* __restrict__ xc, T* __restrict__ out) {
+    prolong_body<T, DIM, ZG>(blockIdx.x, blockIdx.y, blockIdx.z, mxf, myf, mzf, mxc, myc, kch, xb, c, xc, out);
+}
+
 // ------------------------------------------------------- coarse solve
 
 // x = L^-T (L^-1 b) with Linv = L^-1 (n x n, row major, lower triangular)
 // of the Cholesky factor of the coarsest operator, FP64.  One block.
 template <typename T>
-__global__ void __launch_bounds__(1024)
-k_coarse_chol(int n, const double* __restrict__ Linv, const T* __restrict__ b, T* __restrict__ x) {
-    extern __shared__ double sh[];
+__device__ __forceinline__ void coarse_body(int n, const double* __restrict__ Linv, const T* __restrict__ b,
+                                            T* __restrict__ x, double* __restrict__ sh) {
     double* bb = sh;
     double* yy = sh + n;
     for (int q = threadIdx.x; q < n; q += blockDim.x) bb[q] = double(b[q]);
@@ -494,18 +515,27 @@ k_coarse_chol(int n, const double* __restrict__ Linv, const T* __restrict__ b, T
     }
 }
 
+template <typename T>
+__global__ void __launch_bounds__(1024)
+k_coarse_chol(int n, const double* __restrict__ Linv, const T* __restrict__ b, T* __restrict__ x) {
+    extern __shared__ double sh[];
+    coarse_body<T>(n, Linv, b, x, sh);
+}
+
 // ------------------------------------------------------- vector kernels
 
-// x += a p ; r -= a w  (a = S[ALPHA]); NORM: acc0 += r.r
+// x = keep x + a p ; r -= a w  (a = S[ialpha], keep = S[ikeep] in {0,1});
+// NORM: acc0 += r.r
 template <typename T, int NORM>
 __global__ void __launch_bounds__(256)
-k_cg_update(long n, const double* __restrict__ S, int ialpha, T* __restrict__ x,
+k_cg_update(long n, const double* __restrict__ S, int ialpha, int ikeep, T* __restrict__ x,
             const T* __restrict__ p, T* __restrict__ r, const T* __restrict__ w,
             double* __restrict__ part) {
     const T a = T(S[ialpha]);
+    const bool keep = S[ikeep] != 0.0;
     double acc = 0.0;
     for (long q = blockIdx.x * 256L + threadIdx.x; q < n; q += (long)gridDim.x * 256) {
-        x[q] += a * p[q];
+        x[q] = keep ? x[q] + a * p[q] : a * p[q];
         const T rv = r[q] - a * w[q];
         r[q] = rv;
         if (NORM) acc += double(rv) * double(rv);
@@ -567,6 +597,12 @@ k_dot2(long n, const T* __restrict__ x, const T* __restrict__ y, double* __restr
     }
 }
 
+// x += (double) c   (fold an FP32 correction into an FP64 iterate)
+__global__ void __launch_bounds__(256)
+k_axpy_mixed(long n, double* __restrict__ x, const float* __restrict__ c) {
+    for (long q = blockIdx.x * 256L + threadIdx.x; q < n; q += (long)gridDim.x * 256) x[q] += double(c[q]);
+}
+
 template <typename T>
 __global__ void __launch_bounds__(256) k_fill(long n, T v, T* __restrict__ x) {
     for (long q = blockIdx.x * 256L + threadIdx.x; q < n; q += (long)gridDim.x * 256) x[q] = v;
@@ -605,28 +641,15 @@ enum {
     S_ZZ,          // (z, z)
     S_RR,          // (r, r)
     S_TMP0, S_TMP1,
+    S_ONE,         // constant 1 (set at set-up)
+    S_ZERO,        // constant 0
     S_COUNT = 16
 };
 enum { FIN_ALPHA = 0, FIN_BETA_PRECOND, FIN_BETA_NATURAL, FIN_BETA_UNPRECOND, FIN_RR, FIN_RAW };
 
-__global__ void __launch_bounds__(256)
-k_finalize(long nparts, const double* __restrict__ part, double* __restrict__ S, int mode,
-           int first) {
-    __shared__ double s0[256], s1[256];
-    double a = 0.0, b = 0.0;
-    for (long q = threadIdx.x; q < nparts; q += 256) { a += part[2 * q]; b += part[2 * q + 1]; }
-    s0[threadIdx.x] = a;
-    s1[threadIdx.x] = b;
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if (threadIdx.x < w) {
-            s0[threadIdx.x] += s0[threadIdx.x + w];
-            s1[threadIdx.x] += s1[threadIdx.x + w];
-        }
-        __syncthreads();
-    }
-    if (threadIdx.x != 0) return;
-    const double A = s0[0], B = s1[0];
+// scalar-block update from the two reduced sums A, B
+__device__ __forceinline__ void finalize_scalars(double* __restrict__ S, int mode, int first, double A,
+                                                 double B) {
     switch (mode) {
     case FIN_ALPHA:            // A = (p, A p)
         S[S_PW] = A;
@@ -651,6 +674,25 @@ k_finalize(long nparts, const double* __restrict__ part, double* __restrict__ S,
         S[S_TMP1] = B;
         break;
     }
+}
+
+__global__ void __launch_bounds__(256)
+k_finalize(long nparts, const double* __restrict__ part, double* __restrict__ S, int mode,
+           int first) {
+    __shared__ double s0[256], s1[256];
+    double a = 0.0, b = 0.0;
+    for (long q = threadIdx.x; q < nparts; q += 256) { a += part[2 * q]; b += part[2 * q + 1]; }
+    s0[threadIdx.x] = a;
+    s1[threadIdx.x] = b;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+            s0[threadIdx.x] += s0[threadIdx.x + w];
+            s1[threadIdx.x] += s1[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) finalize_scalars(S, mode, first, s0[0], s1[0]);
 }
 
 }  // namespace p4g

@@ -1,0 +1,807 @@
+// p4g_serve_impl.hpp -- continuous-batching solve server (included by
+// p4g_solver.hip, which defines the solver types it builds on).
+//
+// A worker thread owns one HIP stream.  Each round advances every active
+// request by one PCG iteration of the mixed-precision solver (FP32 vectors
+// and V-cycle, FP64 residual replacement; the same algorithm as
+// -p4g_precision mixed): one ragged-batched launch per phase covers all
+// requests, and the V-cycle batches equal-sized levels of all requests
+// (descending by grid size, coarsest solves, ascending).  The V-cycle launch
+// sequence is captured in a hipGraph and replayed every round until the
+// active set changes.  Requests are admitted between rounds, in arrival
+// order, when their device footprint fits the HBM budget; finished requests
+// release theirs.  Each request's arithmetic is independent of the others
+// (per-request blocks and partial sums), so its result is the same alone
+// and in a mix.
+
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <thread>
+
+#include "p4g_batch.hpp"
+
+namespace p4g {
+
+struct SrvLevel {
+    float *b = nullptr, *x = nullptr, *r = nullptr, *t0 = nullptr;
+};
+
+enum { RQ_QUEUED = 0, RQ_INIT = 1, RQ_ITER = 2, RQ_DONE = 3, RQ_FAILED = -1 };
+
+struct SrvReq {
+    long id = 0;
+    p4g_solver_s* sol = nullptr;
+    const double* hb = nullptr;
+    double* hx = nullptr;
+    int dim = 3, nlev = 0;
+    long n = 0;
+    size_t bytes = 0;
+    int slot = -1;
+    double *b64 = nullptr, *xa = nullptr, *xb = nullptr;
+    float *r32 = nullptr, *z32 = nullptr, *pa = nullptr, *pb = nullptr, *w32 = nullptr, *x32 = nullptr;
+    std::vector<SrvLevel> lv;
+    double* part = nullptr;
+    long npart = 0;
+    std::vector<void*> allocs;
+    int state = RQ_QUEUED;
+    double t_submit = 0, t_admit = 0, t_done = 0;
+    int its = 0, nrep = 0, reason = 0;
+    bool replace_now = false, replaced = false, folded = true, keep_zero = true;
+    double bnorm = 0, target = 0, rr_last = 0, dpi_old = 0, rn = 0;
+    double *xcur = nullptr, *xnext = nullptr;
+    float *pold = nullptr, *pnew = nullptr;
+    std::string err;
+    double* S = nullptr;   // device scalar block (32 doubles) of the slot
+};
+
+}  // namespace p4g
+
+struct p4g_server_s {
+    size_t budget = 0, used = 0, peak = 0;
+    bool use_graph = true;
+    int max_slots = 64;
+    std::mutex mu;
+    std::condition_variable cv, cv_done;
+    std::deque<p4g::SrvReq*> queue;
+    std::vector<p4g::SrvReq*> active;
+    std::map<long, p4g::SrvReq*> all;
+    std::vector<int> free_slots;
+    long next_id = 1;
+    bool stop = false;
+    int device = 0;
+    std::thread th;
+    hipStream_t st = nullptr;
+    double *S = nullptr, *hS = nullptr;
+    char *d_round = nullptr, *h_round = nullptr;
+    size_t round_cap = 0;
+    char *d_vc = nullptr, *h_vc = nullptr;
+    size_t vc_cap = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    std::vector<long> graph_key;
+    // statistics
+    std::vector<double> lat_ms, queue_ms;
+    long submitted = 0, completed = 0, rejected = 0, rounds = 0, captures = 0;
+    double t_first = -1, t_last = 0, dof_done = 0;
+    std::string fatal;
+};
+
+namespace p4g {
+
+static double wall_ms() {
+    using namespace std::chrono;
+    return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
+}
+
+// footprint of one request on the device (bytes)
+static size_t req_bytes(p4g_solver_s* s) {
+    const LevelInfo& F = s->lv.back();
+    const long n = F.n;
+    size_t b = 3 * n * sizeof(double) + 8 * n * sizeof(float);
+    for (int l = 0; l + 1 < s->nlev; ++l) b += (l > 0 ? 4 : 2) * s->lv[l].n * sizeof(float);
+    const long nb = std::max<long>(nblocks(stencil_grid(F.dim, F.mx, F.my, F.mz)), vblocks(n));
+    b += 2 * nb * sizeof(double);
+    return b;
+}
+
+static int serve_supported(p4g_solver_s* s) {
+    if (s->pctype != PCT_MG || s->nlev < 2)
+        return set_error(PETSC_ERR_SUP, "serving needs -pc_type mg with at least 2 levels");
+    if (s->nu != 1 || s->cycles != 1)
+        return set_error(PETSC_ERR_SUP, "serving runs V(1,1) cycles: -mg_levels_ksp_max_it 1, -pc_mg_cycle_type v");
+    if (s->lv.back().dim < 2)
+        return set_error(PETSC_ERR_SUP, "serving supports 2D and 3D grids");
+    for (const LevelInfo& L : s->lv)
+        if (L.var || L.d != L.db)
+            return set_error(PETSC_ERR_SUP, "serving supports constant-coefficient stencils");
+    if (s->smoother != SM_CHEB && s->smoother != SM_RICH)
+        return set_error(PETSC_ERR_SUP, "serving needs a Chebyshev or Richardson smoother");
+    return 0;
+}
+
+static double level_scale(p4g_solver_s* s, const LevelInfo& L) {
+    return s->smoother == SM_CHEB ? 2.0 / (L.emax + L.emin) : s->rich_scale;
+}
+
+static OpConst<float> op32(p4g_solver_s* s, const LevelInfo& L) {
+    OpConst<float> o;
+    o.d = float(L.d); o.db = float(L.db); o.sx = float(L.sx); o.sy = float(L.sy); o.sz = float(L.sz);
+    o.id = s->jacobi ? float(1.0 / L.d) : 1.0f;
+    o.idb = s->jacobi ? float(1.0 / L.db) : 1.0f;
+    return o;
+}
+
+static OpConst<double> op64(const LevelInfo& L) {
+    OpConst<double> o;
+    o.d = L.d; o.db = L.db; o.sx = L.sx; o.sy = L.sy; o.sz = L.sz; o.id = o.idb = 1.0;
+    return o;
+}
+
+static const double* solver_linv(p4g_solver_s* s) {
+    return s->e32 ? s->e32->Linv : (s->e64 ? s->e64->Linv : nullptr);
+}
+
+static GridRef gref_stencil(const LevelInfo& L, long start) {
+    const dim3 g = stencil_grid(L.dim, L.mx, L.my, L.mz);
+    GridRef r;
+    r.mx = L.mx; r.my = L.my; r.mz = L.mz; r.kch = kchf(L.dim, L.my, L.mz);
+    r.gx = g.x; r.gy = g.y; r.gz = g.z; r.start = start;
+    return r;
+}
+
+static GridRef gref_point(const LevelInfo& L, long start) {
+    const dim3 g = point_grid(L.dim, L.mx, L.my, L.mz);
+    GridRef r;
+    r.mx = L.mx; r.my = L.my; r.mz = L.mz; r.kch = kchf(L.dim, L.my, L.mz);
+    r.gx = g.x; r.gy = g.y; r.gz = g.z; r.start = start;
+    return r;
+}
+
+static long gblocks(const GridRef& g) { return (long)g.gx * g.gy * g.gz; }
+
+// ---------------------------------------------------------- item packing
+
+struct Packer {
+    std::vector<char> buf;
+    size_t put(const void* p, size_t bytes) {
+        size_t off = (buf.size() + 255) & ~(size_t)255;
+        buf.resize(off + bytes);
+        std::memcpy(buf.data() + off, p, bytes);
+        return off;
+    }
+};
+
+// a launch of a batched kernel over a list of items
+struct BLaunch {
+    int kind = 0;          // which kernel
+    int dim = 3;
+    size_t off = 0;        // byte offset of the items in the device buffer
+    int nitems = 0;
+    long blocks = 0;
+    int lds = 0;
+};
+
+enum { BK_DIR = 0, BK_FIN, BK_UPD, BK_REP, BK_RESID, BK_RESTRICT, BK_PROLONG, BK_SMOOTH, BK_COARSE };
+
+typedef StencilItem<OpConst<float>, SrcDir<float>, OutApplyDot<float>> DirItem;
+typedef StencilItem<OpConst<double>, SrcSum<double, float>, OutResidCvt<double, float>> RepItem;
+typedef StencilItem<OpConst<float>, SrcScaled<float>, OutResid<float, 0>> ResidItem;
+typedef StencilItem<OpConst<float>, SrcArr<float>, OutChebR<float>> SmoothItem;
+
+static int blaunch(p4g_server_s* v, const BLaunch& L, const char* base) {
+    if (L.nitems == 0 || L.blocks == 0) return 0;
+    const void* items = base + L.off;
+    hipStream_t st = v->st;
+    const dim3 g((unsigned)L.blocks);
+    switch (L.kind) {
+    case BK_DIR:
+        if (L.dim == 3) k_stencil_batch<float, 3, OpConst<float>, SrcDir<float>, OutApplyDot<float>><<<g, dim3(TX, TY), 0, st>>>((const DirItem*)items, L.nitems);
+        else k_stencil_batch<float, 2, OpConst<float>, SrcDir<float>, OutApplyDot<float>><<<g, dim3(TX, TY), 0, st>>>((const DirItem*)items, L.nitems);
+        break;
+    case BK_REP:
+        if (L.dim == 3) k_stencil_batch<double, 3, OpConst<double>, SrcSum<double, float>, OutResidCvt<double, float>><<<g, dim3(TX, TY), 0, st>>>((const RepItem*)items, L.nitems);
+        else k_stencil_batch<double, 2, OpConst<double>, SrcSum<double, float>, OutResidCvt<double, float>><<<g, dim3(TX, TY), 0, st>>>((const RepItem*)items, L.nitems);
+        break;
+    case BK_RESID:
+        if (L.dim == 3) k_stencil_batch<float, 3, OpConst<float>, SrcScaled<float>, OutResid<float, 0>><<<g, dim3(TX, TY), 0, st>>>((const ResidItem*)items, L.nitems);
+        else k_stencil_batch<float, 2, OpConst<float>, SrcScaled<float>, OutResid<float, 0>><<<g, dim3(TX, TY), 0, st>>>((const ResidItem*)items, L.nitems);
+        break;
+    case BK_SMOOTH:
+        if (L.dim == 3) k_stencil_batch<float, 3, OpConst<float>, SrcArr<float>, OutChebR<float>><<<g, dim3(TX, TY), 0, st>>>((const SmoothItem*)items, L.nitems);
+        else k_stencil_batch<float, 2, OpConst<float>, SrcArr<float>, OutChebR<float>><<<g, dim3(TX, TY), 0, st>>>((const SmoothItem*)items, L.nitems);
+        break;
+    case BK_RESTRICT:
+        if (L.dim == 3) k_restrict_batch<float, 3><<<g, dim3(TX, TY), 0, st>>>((const RestrictItem<float>*)items, L.nitems);
+        else k_restrict_batch<float, 2><<<g, dim3(TX, TY), 0, st>>>((const RestrictItem<float>*)items, L.nitems);
+        break;
+    case BK_PROLONG:
+        if (L.dim == 3) k_prolong_zg_batch<float, 3><<<g, dim3(TX, TY), 0, st>>>((const ProlongItem<float>*)items, L.nitems);
+        else k_prolong_zg_batch<float, 2><<<g, dim3(TX, TY), 0, st>>>((const ProlongItem<float>*)items, L.nitems);
+        break;
+    case BK_UPD:
+        k_update_batch<float><<<g, 256, 0, st>>>((const UpdateItem<float>*)items, L.nitems);
+        break;
+    case BK_FIN:
+        k_finalize_batch<<<dim3((unsigned)L.nitems), 256, 0, st>>>((const FinItem*)items);
+        break;
+    case BK_COARSE:
+        k_coarse_batch<float><<<dim3((unsigned)L.nitems), 256, L.lds, st>>>((const CoarseItem<float>*)items);
+        break;
+    }
+    LAUNCH_CHECK();
+    return 0;
+}
+
+static int upload(p4g_server_s* v, const Packer& pk, char** dbuf, char** hbuf, size_t* cap) {
+    const size_t bytes = pk.buf.size();
+    if (bytes == 0) return 0;
+    if (bytes > *cap) {
+        if (*dbuf) hipFree(*dbuf);
+        if (*hbuf) hipHostFree(*hbuf);
+        *cap = std::max<size_t>(bytes * 2, 1 << 16);
+        HIP_TRY(hipMalloc((void**)dbuf, *cap));
+        HIP_TRY(hipHostMalloc((void**)hbuf, *cap, hipHostMallocDefault));
+    }
+    // the previous upload from this staging buffer must have completed
+    HIP_TRY(hipStreamSynchronize(v->st));
+    std::memcpy(*hbuf, pk.buf.data(), bytes);
+    HIP_TRY(hipMemcpyAsync(*dbuf, *hbuf, bytes, hipMemcpyHostToDevice, v->st));
+    return 0;
+}
+
+// ------------------------------------------------------------- V-cycle plan
+
+// Batched V(1,1) over all active requests: equal-sized levels share a
+// launch.  Descending sizes: zero-guess smoothing + residual, restriction;
+// all coarsest solves; ascending sizes: prolongation (+ c b), smoothing.
+static int build_vcycle(p4g_server_s* v, Packer& pk, std::vector<BLaunch>& seq) {
+    struct LvRef { SrvReq* r; int l; };
+    std::map<std::pair<int, int>, std::vector<LvRef>, std::greater<std::pair<int, int>>> groups;
+    int maxn0 = 0;
+    for (SrvReq* r : v->active)
+        for (int l = 1; l < r->nlev; ++l) groups[{r->sol->lv[l].mx, r->dim}].push_back({r, l});
+    auto bvec = [](SrvReq* r, int l) -> float* { return l == r->nlev - 1 ? r->r32 : r->lv[l].b; };
+    // descent
+    for (auto& kv : groups) {
+        const int dim = kv.first.second;
+        std::vector<ResidItem> ri;
+        std::vector<RestrictItem<float>> rs;
+        long s1 = 0, s2 = 0;
+        for (const LvRef& q : kv.second) {
+            p4g_solver_s* s = q.r->sol;
+            const LevelInfo& L = s->lv[q.l];
+            const LevelInfo& C = s->lv[q.l - 1];
+            ResidItem it;
+            it.g = gref_stencil(L, s1);
+            s1 += gblocks(it.g);
+            it.part = nullptr;
+            it.flag = nullptr;
+            it.op = op32(s, L);
+            const float c = float(level_scale(s, L) * (s->jacobi ? 1.0 / L.d : 1.0));
+            it.src = SrcScaled<float>{bvec(q.r, q.l), c};
+            it.out = OutResid<float, 0>{bvec(q.r, q.l), q.r->lv[q.l].r};
+            ri.push_back(it);
+            RestrictItem<float> rt;
+            rt.g = gref_point(C, s2);
+            s2 += gblocks(rt.g);
+            rt.mxf = L.mx; rt.myf = L.my; rt.mzf = L.mz;
+            rt.rf = q.r->lv[q.l].r;
+            rt.bc = q.r->lv[q.l - 1].b;
+            rs.push_back(rt);
+        }
+        BLaunch a; a.kind = BK_RESID; a.dim = dim; a.nitems = (int)ri.size(); a.blocks = s1;
+        a.off = pk.put(ri.data(), ri.size() * sizeof(ResidItem));
+        seq.push_back(a);
+        BLaunch b; b.kind = BK_RESTRICT; b.dim = dim; b.nitems = (int)rs.size(); b.blocks = s2;
+        b.off = pk.put(rs.data(), rs.size() * sizeof(RestrictItem<float>));
+        seq.push_back(b);
+    }
+    // coarsest solves
+    {
+        std::vector<CoarseItem<float>> ci;
+        for (SrvReq* r : v->active) {
+            CoarseItem<float> it;
+            it.n = (int)r->sol->lv[0].n;
+            it.Linv = solver_linv(r->sol);
+            it.b = r->lv[0].b;
+            it.x = r->lv[0].x;
+            maxn0 = std::max(maxn0, it.n);
+            ci.push_back(it);
+        }
+        BLaunch c; c.kind = BK_COARSE; c.nitems = (int)ci.size(); c.blocks = c.nitems;
+        c.lds = 2 * maxn0 * (int)sizeof(double);
+        c.off = pk.put(ci.data(), ci.size() * sizeof(CoarseItem<float>));
+        seq.push_back(c);
+    }
+    // ascent
+    for (auto it = groups.rbegin(); it != groups.rend(); ++it) {
+        const int dim = it->first.second;
+        std::vector<ProlongItem<float>> pi;
+        std::vector<SmoothItem> si;
+        long s1 = 0, s2 = 0;
+        for (const LvRef& q : it->second) {
+            p4g_solver_s* s = q.r->sol;
+            const LevelInfo& L = s->lv[q.l];
+            const LevelInfo& C = s->lv[q.l - 1];
+            const double sc = level_scale(s, L);
+            ProlongItem<float> p;
+            p.g = gref_point(L, s1);
+            s1 += gblocks(p.g);
+            p.mxc = C.mx; p.myc = C.my;
+            p.xb = bvec(q.r, q.l);
+            p.c = float(sc * (s->jacobi ? 1.0 / L.d : 1.0));
+            p.xc = q.r->lv[q.l - 1].x;
+            p.out = q.r->lv[q.l].t0;
+            pi.push_back(p);
+            SmoothItem m;
+            m.g = gref_stencil(L, s2);
+            s2 += gblocks(m.g);
+            m.part = q.r->part;
+            m.flag = nullptr;
+            m.op = op32(s, L);
+            m.src = SrcArr<float>{q.r->lv[q.l].t0};
+            const bool fin = q.l == q.r->nlev - 1;
+            m.out = OutChebR<float>{bvec(q.r, q.l), fin ? q.r->z32 : q.r->lv[q.l].x, 1.0f, float(sc), fin ? 1 : 0};
+            si.push_back(m);
+        }
+        BLaunch a; a.kind = BK_PROLONG; a.dim = dim; a.nitems = (int)pi.size(); a.blocks = s1;
+        a.off = pk.put(pi.data(), pi.size() * sizeof(ProlongItem<float>));
+        seq.push_back(a);
+        BLaunch b; b.kind = BK_SMOOTH; b.dim = dim; b.nitems = (int)si.size(); b.blocks = s2;
+        b.off = pk.put(si.data(), si.size() * sizeof(SmoothItem));
+        seq.push_back(b);
+    }
+    return 0;
+}
+
+// ---------------------------------------------------------- request memory
+
+static int req_alloc(SrvReq* r, void** p, size_t bytes) {
+    hipError_t e = hipMalloc(p, std::max<size_t>(bytes, 8));
+    if (e != hipSuccess) return set_error(PETSC_ERR_MEM, "serving: hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
+    r->allocs.push_back(*p);
+    return 0;
+}
+
+static void req_free(SrvReq* r) {
+    for (void* p : r->allocs) hipFree(p);
+    r->allocs.clear();
+}
+
+static int req_admit(p4g_server_s* v, SrvReq* r) {
+    p4g_solver_s* s = r->sol;
+    const LevelInfo& F = s->lv.back();
+    const long n = F.n;
+    r->nlev = s->nlev;
+    r->dim = F.dim;
+    r->n = n;
+    P4G_TRY(req_alloc(r, (void**)&r->b64, n * 8));
+    P4G_TRY(req_alloc(r, (void**)&r->xa, n * 8));
+    P4G_TRY(req_alloc(r, (void**)&r->xb, n * 8));
+    float** f32[6] = {&r->r32, &r->z32, &r->pa, &r->pb, &r->w32, &r->x32};
+    for (float** q : f32) P4G_TRY(req_alloc(r, (void**)q, n * 4));
+    r->lv.assign(r->nlev, SrvLevel());
+    for (int l = 0; l < r->nlev; ++l) {
+        const long nl = s->lv[l].n;
+        if (l + 1 < r->nlev) {
+            P4G_TRY(req_alloc(r, (void**)&r->lv[l].b, nl * 4));
+            P4G_TRY(req_alloc(r, (void**)&r->lv[l].x, nl * 4));
+        }
+        if (l > 0) {
+            P4G_TRY(req_alloc(r, (void**)&r->lv[l].r, nl * 4));
+            P4G_TRY(req_alloc(r, (void**)&r->lv[l].t0, nl * 4));
+        }
+    }
+    r->npart = std::max<long>(nblocks(stencil_grid(F.dim, F.mx, F.my, F.mz)), vblocks(n));
+    P4G_TRY(req_alloc(r, (void**)&r->part, 2 * r->npart * 8));
+    r->slot = v->free_slots.back();
+    v->free_slots.pop_back();
+    r->S = v->S + 32 * r->slot;
+    // initial state: x = 0, p = 0, r32 = b, |b|^2 -> outer TMP0
+    double init[32] = {0};
+    init[S_ONE] = init[16 + S_ONE] = 1.0;
+    hipStream_t st = v->st;
+    HIP_TRY(hipMemcpyAsync(r->S, init, sizeof init, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(r->b64, r->hb, n * 8, hipMemcpyHostToDevice, st));
+    const int gb = vblocks(n);
+    k_fill<double><<<gb, 256, 0, st>>>(n, 0.0, r->xa);
+    k_fill<float><<<gb, 256, 0, st>>>(n, 0.0f, r->pa);
+    k_convert<double, float><<<gb, 256, 0, st>>>(n, r->b64, r->r32);
+    k_dot2<double><<<gb, 256, 0, st>>>(n, r->b64, r->b64, r->part);
+    k_finalize<<<1, 256, 0, st>>>(gb, r->part, r->S + 16, FIN_RAW, 0);
+    LAUNCH_CHECK();
+    r->xcur = r->xa;
+    r->xnext = r->xb;
+    r->pold = r->pa;
+    r->pnew = r->pb;
+    r->state = RQ_INIT;
+    r->t_admit = wall_ms();
+    return 0;
+}
+
+// ------------------------------------------------------------------ round
+
+static int round_once(p4g_server_s* v) {
+    Packer pk;
+    std::vector<BLaunch> seq;
+    // per-round CG phases (their item tables change every round)
+    std::vector<DirItem> dir3, dir2;
+    std::vector<FinItem> fa, frr, fraw, fbeta;
+    std::vector<UpdateItem<float>> up;
+    std::vector<RepItem> rep3, rep2;
+    long sd3 = 0, sd2 = 0, su = 0, sr3 = 0, sr2 = 0;
+    for (SrvReq* r : v->active) {
+        p4g_solver_s* s = r->sol;
+        const LevelInfo& F = s->lv.back();
+        r->replaced = false;
+        if (r->state == RQ_ITER) {
+            DirItem d;
+            long& sd = r->dim == 3 ? sd3 : sd2;
+            d.g = gref_stencil(F, sd);
+            sd += gblocks(d.g);
+            d.part = r->part;
+            d.flag = nullptr;
+            d.op = op32(s, F);
+            d.src = SrcDir<float>{r->z32, r->pold, r->pnew, r->S + S_BCOEF};
+            d.out = OutApplyDot<float>{r->w32};
+            (r->dim == 3 ? dir3 : dir2).push_back(d);
+            fa.push_back(FinItem{r->part, gblocks(d.g), r->S, FIN_ALPHA, 0, nullptr});
+            std::swap(r->pold, r->pnew);   // pold: the current direction
+            UpdateItem<float> u;
+            u.g.start = su;
+            u.g.gx = vblocks(r->n);
+            u.g.gy = u.g.gz = 1;
+            su += u.g.gx;
+            u.n = r->n;
+            u.S = r->S;
+            u.keep = r->S + (r->keep_zero ? S_ZERO : S_ONE);
+            u.x = r->x32; u.r = r->r32; u.p = r->pold; u.w = r->w32;
+            u.part = r->part;
+            up.push_back(u);
+            frr.push_back(FinItem{r->part, (long)u.g.gx, r->S, FIN_RR, 0, nullptr});
+            r->keep_zero = false;
+            r->folded = false;
+            if (r->replace_now) {
+                RepItem q;
+                long& sr = r->dim == 3 ? sr3 : sr2;
+                q.g = gref_stencil(F, sr);
+                sr += gblocks(q.g);
+                q.part = r->part;
+                q.flag = nullptr;
+                q.op = op64(F);
+                q.src = SrcSum<double, float>{r->xcur, r->x32, r->xnext};
+                q.out = OutResidCvt<double, float>{r->b64, r->r32};
+                (r->dim == 3 ? rep3 : rep2).push_back(q);
+                fraw.push_back(FinItem{r->part, gblocks(q.g), r->S + 16, FIN_RAW, 0, nullptr});
+                std::swap(r->xcur, r->xnext);
+                r->keep_zero = true;
+                r->folded = true;
+                r->replace_now = false;
+                r->replaced = true;
+                r->nrep++;
+            }
+        }
+        const LevelInfo& Ff = s->lv.back();
+        fbeta.push_back(FinItem{r->part, nblocks(stencil_grid(Ff.dim, Ff.mx, Ff.my, Ff.mz)), r->S,
+                                FIN_BETA_PRECOND, r->state == RQ_INIT ? 1 : 0, nullptr});
+    }
+    auto add = [&](int kind, int dim, const void* data, size_t each, size_t cnt, long blocks) {
+        if (!cnt) return;
+        BLaunch a;
+        a.kind = kind; a.dim = dim; a.nitems = (int)cnt; a.blocks = blocks;
+        a.off = pk.put(data, each * cnt);
+        seq.push_back(a);
+    };
+    add(BK_DIR, 3, dir3.data(), sizeof(DirItem), dir3.size(), sd3);
+    add(BK_DIR, 2, dir2.data(), sizeof(DirItem), dir2.size(), sd2);
+    add(BK_FIN, 0, fa.data(), sizeof(FinItem), fa.size(), (long)fa.size());
+    add(BK_UPD, 0, up.data(), sizeof(UpdateItem<float>), up.size(), su);
+    add(BK_FIN, 0, frr.data(), sizeof(FinItem), frr.size(), (long)frr.size());
+    add(BK_REP, 3, rep3.data(), sizeof(RepItem), rep3.size(), sr3);
+    add(BK_REP, 2, rep2.data(), sizeof(RepItem), rep2.size(), sr2);
+    add(BK_FIN, 0, fraw.data(), sizeof(FinItem), fraw.size(), (long)fraw.size());
+    const size_t nbefore = seq.size();
+    add(BK_FIN, 0, fbeta.data(), sizeof(FinItem), fbeta.size(), (long)fbeta.size());
+    P4G_TRY(upload(v, pk, &v->d_round, &v->h_round, &v->round_cap));
+    for (size_t q = 0; q < nbefore; ++q) P4G_TRY(blaunch(v, seq[q], v->d_round));
+    // V-cycle: graph replay, re-captured when the active set changes
+    std::vector<long> key;
+    for (SrvReq* r : v->active) key.push_back(r->id);
+    if (!v->use_graph || key != v->graph_key || !v->gexec) {
+        Packer vp;
+        std::vector<BLaunch> vseq;
+        P4G_TRY(build_vcycle(v, vp, vseq));
+        P4G_TRY(upload(v, vp, &v->d_vc, &v->h_vc, &v->vc_cap));
+        if (v->gexec) { hipGraphExecDestroy(v->gexec); v->gexec = nullptr; }
+        if (v->graph) { hipGraphDestroy(v->graph); v->graph = nullptr; }
+        if (v->use_graph) {
+            HIP_TRY(hipStreamSynchronize(v->st));
+            HIP_TRY(hipStreamBeginCapture(v->st, hipStreamCaptureModeThreadLocal));
+            int e = 0;
+            for (const BLaunch& L : vseq)
+                if (!e) e = blaunch(v, L, v->d_vc);
+            hipGraph_t g = nullptr;
+            hipError_t ce = hipStreamEndCapture(v->st, &g);
+            if (e) { if (g) hipGraphDestroy(g); return e; }
+            if (ce != hipSuccess) return set_error(PETSC_ERR_LIB, "serving: graph capture: %s", hipGetErrorString(ce));
+            v->graph = g;
+            HIP_TRY(hipGraphInstantiate(&v->gexec, g, nullptr, nullptr, 0));
+            v->graph_key = key;
+            v->captures++;
+        } else {
+            for (const BLaunch& L : vseq) P4G_TRY(blaunch(v, L, v->d_vc));
+        }
+    }
+    if (v->use_graph) HIP_TRY(hipGraphLaunch(v->gexec, v->st));
+    for (size_t q = nbefore; q < seq.size(); ++q) P4G_TRY(blaunch(v, seq[q], v->d_round));
+    HIP_TRY(hipMemcpyAsync(v->hS, v->S, sizeof(double) * 32 * v->max_slots, hipMemcpyDeviceToHost, v->st));
+    HIP_TRY(hipStreamSynchronize(v->st));
+    v->rounds++;
+    return 0;
+}
+
+// host decisions after a round; returns true when the request finished
+static bool req_step(p4g_server_s* v, SrvReq* r) {
+    p4g_solver_s* s = r->sol;
+    const double* hs = v->hS + 32 * r->slot;
+    const double* ho = hs + 16;
+    if (r->state == RQ_INIT) {
+        r->bnorm = std::sqrt(ho[S_TMP0]);
+        r->target = std::max(s->rtol * r->bnorm, s->atol);
+        r->rr_last = r->bnorm * r->bnorm;
+        r->rn = r->bnorm;
+        r->state = RQ_ITER;
+        r->keep_zero = true;
+        if (r->bnorm <= s->atol) { r->reason = P4G_CONVERGED_ATOL; return true; }
+        return false;
+    }
+    r->its++;
+    const double dpi = hs[S_PW];
+    if (dpi == 0.0 || (r->its > 1 && ((dpi > 0) != (r->dpi_old > 0)))) { r->reason = P4G_DIVERGED_INDEFINITE_MAT; return true; }
+    r->dpi_old = dpi;
+    double rn;
+    if (r->replaced) { rn = std::sqrt(ho[S_TMP0]); r->rr_last = rn * rn; }
+    else rn = std::sqrt(hs[S_RR]);
+    r->rn = rn;
+    if (std::isnan(rn) || std::isinf(rn)) { r->reason = P4G_DIVERGED_NANORINF; return true; }
+    if (r->replaced && rn <= r->target) { r->reason = rn < s->atol ? P4G_CONVERGED_ATOL : P4G_CONVERGED_RTOL; return true; }
+    if (rn >= s->dtol * r->bnorm) { r->reason = P4G_DIVERGED_DTOL; return true; }
+    if (r->its >= s->maxit) { r->reason = P4G_DIVERGED_ITS; return true; }
+    if (hs[S_RZ] == 0.0 || (!r->replaced && (rn <= r->target || rn * rn <= s->reliable_delta * s->reliable_delta * r->rr_last)))
+        r->replace_now = true;
+    return false;
+}
+
+static int req_finish(p4g_server_s* v, SrvReq* r) {
+    hipStream_t st = v->st;
+    if (!r->folded) {
+        k_axpy_mixed<<<vblocks(r->n), 256, 0, st>>>(r->n, r->xcur, r->x32);
+        LAUNCH_CHECK();
+    }
+    HIP_TRY(hipMemcpyAsync(r->hx, r->xcur, r->n * 8, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    r->t_done = wall_ms();
+    return 0;
+}
+
+static void serve_loop(p4g_server_s* v) {
+    hipSetDevice(v->device);
+    for (;;) {
+        std::vector<SrvReq*> newly;
+        {
+            std::unique_lock<std::mutex> lk(v->mu);
+            v->cv.wait(lk, [&] { return v->stop || !v->queue.empty() || !v->active.empty(); });
+            if (v->stop && v->active.empty()) break;
+            // admission in arrival order, skipping requests that do not fit yet
+            for (auto it = v->queue.begin(); it != v->queue.end();) {
+                SrvReq* r = *it;
+                if ((int)v->active.size() >= v->max_slots) break;
+                if (v->used + r->bytes <= v->budget) {
+                    v->used += r->bytes;
+                    v->peak = std::max(v->peak, v->used);
+                    v->active.push_back(r);
+                    newly.push_back(r);
+                    it = v->queue.erase(it);
+                } else {
+                    ++it;
+                }
+            }
+        }
+        int e = 0;
+        for (SrvReq* r : newly)
+            if (!e) e = req_admit(v, r);
+        if (!e) e = round_once(v);
+        std::vector<SrvReq*> done;
+        if (!e)
+            for (SrvReq* r : v->active)
+                if (req_step(v, r)) done.push_back(r);
+        for (SrvReq* r : done)
+            if (!e) e = req_finish(v, r);
+        std::lock_guard<std::mutex> lk(v->mu);
+        if (e) {
+            v->fatal = p4g_last_error();
+            for (SrvReq* r : v->active) { r->state = RQ_FAILED; r->err = v->fatal; req_free(r); }
+            for (SrvReq* r : v->queue) { r->state = RQ_FAILED; r->err = v->fatal; }
+            v->active.clear();
+            v->queue.clear();
+            v->used = 0;
+            v->cv_done.notify_all();
+            continue;
+        }
+        for (SrvReq* r : done) {
+            req_free(r);
+            v->used -= r->bytes;
+            v->free_slots.push_back(r->slot);
+            r->state = RQ_DONE;
+            v->lat_ms.push_back(r->t_done - r->t_submit);
+            v->queue_ms.push_back(r->t_admit - r->t_submit);
+            v->completed++;
+            v->dof_done += (double)r->n;
+            v->t_last = r->t_done;
+            v->active.erase(std::find(v->active.begin(), v->active.end(), r));
+        }
+        if (!done.empty()) v->cv_done.notify_all();
+    }
+}
+
+static double percentile(std::vector<double> x, double q) {
+    if (x.empty()) return 0.0;
+    std::sort(x.begin(), x.end());
+    const double k = (x.size() - 1) * q;
+    const size_t lo = (size_t)std::floor(k), hi = (size_t)std::ceil(k);
+    return x[lo] + (x[hi] - x[lo]) * (k - lo);
+}
+
+}  // namespace p4g
+
+// ================================================================== C ABI
+
+extern "C" {
+
+int p4g_server_create(size_t hbm_budget_bytes, const char* options, p4g_Server* out) {
+    clear_error();
+    if (!out) return set_error(PETSC_ERR_ARG_WRONG, "p4g_server_create: null output");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+        return set_error(PETSC_ERR_LIB, "p4g_server_create: no HIP device is available");
+    Options o;
+    o.parse(options);
+    p4g_server_s* v = new p4g_server_s;
+    v->budget = hbm_budget_bytes;
+    v->use_graph = o.getb("p4g_serve_graph", true);
+    v->max_slots = std::max(1, std::min(1024, o.geti("p4g_serve_max_slots", 64)));
+    hipGetDevice(&v->device);
+    int e = 0;
+    if (hipStreamCreate(&v->st) != hipSuccess) e = set_error(PETSC_ERR_LIB, "p4g_server_create: stream");
+    if (!e && hipMalloc(&v->S, sizeof(double) * 32 * v->max_slots) != hipSuccess)
+        e = set_error(PETSC_ERR_MEM, "p4g_server_create: scalar blocks");
+    if (!e && hipHostMalloc((void**)&v->hS, sizeof(double) * 32 * v->max_slots, hipHostMallocDefault) != hipSuccess)
+        e = set_error(PETSC_ERR_MEM, "p4g_server_create: pinned scalars");
+    if (e) { delete v; return e; }
+    for (int q = v->max_slots - 1; q >= 0; --q) v->free_slots.push_back(q);
+    v->th = std::thread(serve_loop, v);
+    *out = v;
+    return 0;
+}
+
+int p4g_server_request_bytes(p4g_Solver s, size_t* bytes) {
+    clear_error();
+    if (!s || !bytes) return set_error(PETSC_ERR_ARG_WRONG, "p4g_server_request_bytes: null argument");
+    P4G_TRY(solver_setup(s));
+    *bytes = req_bytes(s);
+    return 0;
+}
+
+int p4g_server_submit(p4g_Server v, p4g_Solver s, const double* b, double* x, long* id) {
+    clear_error();
+    if (!v || !s || !b || !x || !id) return set_error(PETSC_ERR_ARG_WRONG, "p4g_server_submit: null argument");
+    P4G_TRY(solver_setup(s));
+    P4G_TRY(serve_supported(s));
+    SrvReq* r = new SrvReq;
+    r->sol = s;
+    r->hb = b;
+    r->hx = x;
+    r->bytes = req_bytes(s);
+    r->t_submit = wall_ms();
+    std::lock_guard<std::mutex> lk(v->mu);
+    if (!v->fatal.empty()) { delete r; return set_error(PETSC_ERR_LIB, "server failed: %s", v->fatal.c_str()); }
+    r->id = v->next_id++;
+    *id = r->id;
+    v->all[r->id] = r;
+    v->submitted++;
+    if (v->t_first < 0) v->t_first = r->t_submit;
+    if (r->bytes > v->budget) {
+        r->state = RQ_FAILED;
+        r->err = "request footprint exceeds the server's HBM budget";
+        v->rejected++;
+        return set_error(PETSC_ERR_MEM, "p4g_server_submit: request needs %zu bytes, budget %zu", r->bytes, v->budget);
+    }
+    v->queue.push_back(r);
+    v->cv.notify_all();
+    return 0;
+}
+
+int p4g_server_poll(p4g_Server v, long id, int* state) {
+    if (!v || !state) return set_error(PETSC_ERR_ARG_WRONG, "p4g_server_poll: null argument");
+    std::lock_guard<std::mutex> lk(v->mu);
+    auto it = v->all.find(id);
+    if (it == v->all.end()) return set_error(PETSC_ERR_ARG_OUTOFRANGE, "p4g_server_poll: unknown request %ld", id);
+    *state = it->second->state;
+    return 0;
+}
+
+int p4g_server_wait(p4g_Server v, long id, p4g_Report* rep) {
+    clear_error();
+    if (!v) return set_error(PETSC_ERR_ARG_WRONG, "p4g_server_wait: null server");
+    std::unique_lock<std::mutex> lk(v->mu);
+    auto it = v->all.find(id);
+    if (it == v->all.end()) return set_error(PETSC_ERR_ARG_OUTOFRANGE, "p4g_server_wait: unknown request %ld", id);
+    SrvReq* r = it->second;
+    v->cv_done.wait(lk, [&] { return r->state == RQ_DONE || r->state == RQ_FAILED; });
+    if (r->state == RQ_FAILED) return set_error(PETSC_ERR_LIB, "request %ld failed: %s", id, r->err.c_str());
+    if (rep) {
+        std::memset(rep, 0, sizeof *rep);
+        rep->its = r->its;
+        rep->reason = r->reason;
+        rep->inner_its = r->nrep;
+        rep->nlevels = r->nlev;
+        rep->rnorm0 = r->bnorm;
+        rep->rnorm = r->rn;
+        rep->true_relres = r->bnorm > 0 ? r->rn / r->bnorm : 0.0;
+        rep->setup_ms = r->t_admit - r->t_submit;   // queueing
+        rep->solve_ms = r->t_done - r->t_submit;    // latency: submit -> result on host
+    }
+    return 0;
+}
+
+int p4g_server_stats(p4g_Server v, p4g_ServerStats* st) {
+    if (!v || !st) return set_error(PETSC_ERR_ARG_WRONG, "p4g_server_stats: null argument");
+    std::lock_guard<std::mutex> lk(v->mu);
+    std::memset(st, 0, sizeof *st);
+    st->submitted = v->submitted;
+    st->completed = v->completed;
+    st->rejected = v->rejected;
+    st->active = (long)v->active.size();
+    st->queued = (long)v->queue.size();
+    st->rounds = v->rounds;
+    st->graph_captures = v->captures;
+    st->p50_ms = percentile(v->lat_ms, 0.5);
+    st->p99_ms = percentile(v->lat_ms, 0.99);
+    st->max_ms = v->lat_ms.empty() ? 0.0 : *std::max_element(v->lat_ms.begin(), v->lat_ms.end());
+    double sum = 0.0;
+    for (double q : v->lat_ms) sum += q;
+    st->mean_ms = v->lat_ms.empty() ? 0.0 : sum / v->lat_ms.size();
+    st->queue_p50_ms = percentile(v->queue_ms, 0.5);
+    st->wall_s = v->t_first >= 0 ? (v->t_last - v->t_first) * 1e-3 : 0.0;
+    st->dof_per_s = st->wall_s > 0 ? v->dof_done / st->wall_s : 0.0;
+    st->peak_bytes = (double)v->peak;
+    return 0;
+}
+
+int p4g_server_destroy(p4g_Server* pv) {
+    if (!pv || !*pv) return 0;
+    p4g_server_s* v = *pv;
+    {
+        std::lock_guard<std::mutex> lk(v->mu);
+        v->stop = true;
+        v->cv.notify_all();
+    }
+    if (v->th.joinable()) v->th.join();
+    for (auto& kv : v->all) delete kv.second;
+    if (v->gexec) hipGraphExecDestroy(v->gexec);
+    if (v->graph) hipGraphDestroy(v->graph);
+    if (v->d_round) hipFree(v->d_round);
+    if (v->h_round) hipHostFree(v->h_round);
+    if (v->d_vc) hipFree(v->d_vc);
+    if (v->h_vc) hipHostFree(v->h_vc);
+    if (v->S) hipFree(v->S);
+    if (v->hS) hipHostFree(v->hS);
+    if (v->st) hipStreamDestroy(v->st);
+    delete v;
+    *pv = nullptr;
+    return 0;
+}
+
+}  // extern "C"

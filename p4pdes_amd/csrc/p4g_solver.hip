@@ -52,7 +52,12 @@ enum { PCT_MG = 0, PCT_JACOBI, PCT_NONE };
 enum { SM_CHEB = 0, SM_RICH };
 enum { NT_PRECOND = 0, NT_UNPRECOND, NT_NATURAL };
 
-constexpr int KCH = 16;            // planes per marching chunk
+// planes per marching chunk: short chunks on small grids (latency-bound),
+// longer ones on large grids (fewer re-read halo planes)
+static inline int kchf(int dim, int my, int mz) {
+    const int slow = dim == 3 ? mz : (dim == 2 ? my : 1);
+    return slow >= 400 ? 8 : 4;
+}
 constexpr int VBLOCKS = 2048;      // blocks of the grid-stride vector kernels
 constexpr int MAX_COARSE = 2048;   // largest coarsest grid for the direct solve
 
@@ -155,8 +160,8 @@ static inline int vblocks(long n) { return (int)std::max(1L, std::min<long>(VBLO
 // launch geometry of the marching stencil kernels
 static dim3 stencil_grid(int dim, int mx, int my, int mz) {
     const int gx = (mx - 1 + TX - 1) / TX;
-    if (dim == 3) return dim3(gx, (my + TY - 1) / TY, (mz + KCH - 1) / KCH);
-    if (dim == 2) return dim3(gx, ((my + KCH - 1) / KCH + TY - 1) / TY, 1);
+    if (dim == 3) return dim3(gx, (my + TY - 1) / TY, (mz + kchf(dim, my, mz) - 1) / kchf(dim, my, mz));
+    if (dim == 2) return dim3(gx, ((my + kchf(dim, my, mz) - 1) / kchf(dim, my, mz) + TY - 1) / TY, 1);
     return dim3(gx, 1, 1);
 }
 static dim3 stencil_block(int dim) { return dim == 1 ? dim3(TX, 1, 1) : dim3(TX, TY, 1); }
@@ -164,8 +169,8 @@ static long nblocks(dim3 g) { return (long)g.x * g.y * g.z; }
 // grid covering all points with lanes from i = 0 (transfers)
 static dim3 point_grid(int dim, int mx, int my, int mz) {
     const int gx = (mx + TX - 1) / TX;
-    if (dim == 3) return dim3(gx, (my + TY - 1) / TY, (mz + KCH - 1) / KCH);
-    if (dim == 2) return dim3(gx, ((my + KCH - 1) / KCH + TY - 1) / TY, 1);
+    if (dim == 3) return dim3(gx, (my + TY - 1) / TY, (mz + kchf(dim, my, mz) - 1) / kchf(dim, my, mz));
+    if (dim == 2) return dim3(gx, ((my + kchf(dim, my, mz) - 1) / kchf(dim, my, mz) + TY - 1) / TY, 1);
     return dim3(gx, 1, 1);
 }
 
@@ -237,6 +242,7 @@ struct Engine {
         P4G_TRY(alloc(&pa, F.n));
         P4G_TRY(alloc(&pb, F.n));
         P4G_TRY(alloc(&w, F.n));
+        P4G_TRY(alloc(&xbuf, F.n));
         if (pctype == PCT_MG) {
             n0 = n0_;
             void* q = nullptr;
@@ -278,20 +284,20 @@ struct Engine {
             if (L.dim != 3) return set_error(PETSC_ERR_SUP, "variable coefficient needs 3D");
             OpVar<T> o = opv(l, jac);
             return cx->launch(cls, fine, bytes, [&] {
-                k_stencil<T, 3, OpVar<T>, Src, Out><<<g, b, 0, s>>>(L.mx, L.my, L.mz, KCH, o, src, out,
+                k_stencil<T, 3, OpVar<T>, Src, Out><<<g, b, 0, s>>>(L.mx, L.my, L.mz, kchf(L.dim, L.my, L.mz), o, src, out,
                                                                  cx->part);
             });
         }
         OpConst<T> o = opc(L, jac);
         return cx->launch(cls, fine, bytes, [&] {
             if (L.dim == 3)
-                k_stencil<T, 3, OpConst<T>, Src, Out><<<g, b, 0, s>>>(L.mx, L.my, L.mz, KCH, o, src,
+                k_stencil<T, 3, OpConst<T>, Src, Out><<<g, b, 0, s>>>(L.mx, L.my, L.mz, kchf(L.dim, L.my, L.mz), o, src,
                                                                    out, cx->part);
             else if (L.dim == 2)
-                k_stencil<T, 2, OpConst<T>, Src, Out><<<g, b, 0, s>>>(L.mx, L.my, L.mz, KCH, o, src,
+                k_stencil<T, 2, OpConst<T>, Src, Out><<<g, b, 0, s>>>(L.mx, L.my, L.mz, kchf(L.dim, L.my, L.mz), o, src,
                                                                    out, cx->part);
             else
-                k_stencil<T, 1, OpConst<T>, Src, Out><<<g, b, 0, s>>>(L.mx, L.my, L.mz, KCH, o, src,
+                k_stencil<T, 1, OpConst<T>, Src, Out><<<g, b, 0, s>>>(L.mx, L.my, L.mz, kchf(L.dim, L.my, L.mz), o, src,
                                                                    out, cx->part);
         });
     }
@@ -303,11 +309,11 @@ struct Engine {
         hipStream_t s = cx->stream;
         return cx->launch(P4G_K_RESTRICT, l == nlev - 1, double(F.n + C.n) * sizeof(T), [&] {
             if (F.dim == 3)
-                k_restrict<T, 3><<<g, b, 0, s>>>(F.mx, F.my, F.mz, C.mx, C.my, C.mz, KCH, rf, bc);
+                k_restrict<T, 3><<<g, b, 0, s>>>(F.mx, F.my, F.mz, C.mx, C.my, C.mz, kchf(C.dim, C.my, C.mz), rf, bc);
             else if (F.dim == 2)
-                k_restrict<T, 2><<<g, b, 0, s>>>(F.mx, F.my, F.mz, C.mx, C.my, C.mz, KCH, rf, bc);
+                k_restrict<T, 2><<<g, b, 0, s>>>(F.mx, F.my, F.mz, C.mx, C.my, C.mz, kchf(C.dim, C.my, C.mz), rf, bc);
             else
-                k_restrict<T, 1><<<g, b, 0, s>>>(F.mx, F.my, F.mz, C.mx, C.my, C.mz, KCH, rf, bc);
+                k_restrict<T, 1><<<g, b, 0, s>>>(F.mx, F.my, F.mz, C.mx, C.my, C.mz, kchf(C.dim, C.my, C.mz), rf, bc);
         });
     }
 
@@ -319,14 +325,14 @@ struct Engine {
         hipStream_t s = cx->stream;
         return cx->launch(P4G_K_PROLONG, l == nlev - 1, double(2 * F.n + C.n) * sizeof(T), [&] {
             if (F.dim == 3) {
-                if (zg) k_prolong<T, 3, true><<<g, b, 0, s>>>(F.mx, F.my, F.mz, C.mx, C.my, KCH, base, c, xc, out);
-                else k_prolong<T, 3, false><<<g, b, 0, s>>>(F.mx, F.my, F.mz, C.mx, C.my, KCH, base, c, xc, out);
+                if (zg) k_prolong<T, 3, true><<<g, b, 0, s>>>(F.mx, F.my, F.mz, C.mx, C.my, kchf(F.dim, F.my, F.mz), base, c, xc, out);
+                else k_prolong<T, 3, false><<<g, b, 0, s>>>(F.mx, F.my, F.mz, C.mx, C.my, kchf(F.dim, F.my, F.mz), base, c, xc, out);
             } else if (F.dim == 2) {
-                if (zg) k_prolong<T, 2, true><<<g, b, 0, s>>>(F.mx, F.my, F.mz, C.mx, C.my, KCH, base, c, xc, out);
-                else k_prolong<T, 2, false><<<g, b, 0, s>>>(F.mx, F.my, F.mz, C.mx, C.my, KCH, base, c, xc, out);
+                if (zg) k_prolong<T, 2, true><<<g, b, 0, s>>>(F.mx, F.my, F.mz, C.mx, C.my, kchf(F.dim, F.my, F.mz), base, c, xc, out);
+                else k_prolong<T, 2, false><<<g, b, 0, s>>>(F.mx, F.my, F.mz, C.mx, C.my, kchf(F.dim, F.my, F.mz), base, c, xc, out);
             } else {
-                if (zg) k_prolong<T, 1, true><<<g, b, 0, s>>>(F.mx, F.my, F.mz, C.mx, C.my, KCH, base, c, xc, out);
-                else k_prolong<T, 1, false><<<g, b, 0, s>>>(F.mx, F.my, F.mz, C.mx, C.my, KCH, base, c, xc, out);
+                if (zg) k_prolong<T, 1, true><<<g, b, 0, s>>>(F.mx, F.my, F.mz, C.mx, C.my, kchf(F.dim, F.my, F.mz), base, c, xc, out);
+                else k_prolong<T, 1, false><<<g, b, 0, s>>>(F.mx, F.my, F.mz, C.mx, C.my, kchf(F.dim, F.my, F.mz), base, c, xc, out);
             }
         });
     }
@@ -576,12 +582,12 @@ struct Engine {
             const int gb = vblocks(n);
             if (normtype == NT_UNPRECOND) {
                 P4G_TRY(cx->launch(P4G_K_CG_UPDATE, true, 6.0 * n * sizeof(T), [&] {
-                    k_cg_update<T, 1><<<gb, 256, 0, st>>>(n, Sblk, S_ALPHA, x, pold, r, w, cx->part);
+                    k_cg_update<T, 1><<<gb, 256, 0, st>>>(n, Sblk, S_ALPHA, S_ONE, x, pold, r, w, cx->part);
                 }));
                 P4G_TRY(cx->finalize(gb, Sblk, FIN_RR, 0));
             } else {
                 P4G_TRY(cx->launch(P4G_K_CG_UPDATE, true, 6.0 * n * sizeof(T), [&] {
-                    k_cg_update<T, 0><<<gb, 256, 0, st>>>(n, Sblk, S_ALPHA, x, pold, r, w, cx->part);
+                    k_cg_update<T, 0><<<gb, 256, 0, st>>>(n, Sblk, S_ALPHA, S_ONE, x, pold, r, w, cx->part);
                 }));
             }
             P4G_TRY(pc_apply_dots(r, z, &nb));
@@ -690,8 +696,7 @@ struct p4g_solver_s {
     int normtype = NT_PRECOND, maxit = 10000;
     double rtol = 1e-5, atol = 1e-50, dtol = 1e5;
     bool mixed = false;
-    double inner_rtol = 1e-5;
-    int inner_maxit = 200;
+    double reliable_delta = 1e-2;
     // state
     bool ready = false;
     std::vector<LevelInfo> lv;
@@ -857,7 +862,7 @@ static int estimate_level(p4g_solver_s* s, int l) {
         ov.ax = L.ax; ov.ay = L.ay; ov.az = L.az; ov.jac = jac ? 1 : 0;
         P4G_TRY(get(&idv));
         k_stencil<double, 3, OpVar<double>, SrcZero<double>, OutIdiag>
-            <<<g, bk, 0, st>>>(L.mx, L.my, L.mz, KCH, ov, SrcZero<double>{}, OutIdiag{idv}, s->cx.part);
+            <<<g, bk, 0, st>>>(L.mx, L.my, L.mz, kchf(L.dim, L.my, L.mz), ov, SrcZero<double>{}, OutIdiag{idv}, s->cx.part);
         LAUNCH_CHECK();
     }
     const double idc = jac ? 1.0 / L.d : 1.0;
@@ -885,16 +890,16 @@ static int estimate_level(p4g_solver_s* s, int l) {
         HIP_TRY(hipMemcpyAsync(S + S_BCOEF, &s->cx.hS[16 + S_BCOEF], sizeof(double), hipMemcpyHostToDevice, st));
         if (L.var)
             k_stencil<double, 3, OpVar<double>, SrcDir<double>, OutApplyDot<double>><<<g, bk, 0, st>>>(
-                L.mx, L.my, L.mz, KCH, ov, SrcDir<double>{z, pold, pnew, S + S_BCOEF}, OutApplyDot<double>{w}, s->cx.part);
+                L.mx, L.my, L.mz, kchf(L.dim, L.my, L.mz), ov, SrcDir<double>{z, pold, pnew, S + S_BCOEF}, OutApplyDot<double>{w}, s->cx.part);
         else if (L.dim == 3)
             k_stencil<double, 3, OpConst<double>, SrcDir<double>, OutApplyDot<double>><<<g, bk, 0, st>>>(
-                L.mx, L.my, L.mz, KCH, oc, SrcDir<double>{z, pold, pnew, S + S_BCOEF}, OutApplyDot<double>{w}, s->cx.part);
+                L.mx, L.my, L.mz, kchf(L.dim, L.my, L.mz), oc, SrcDir<double>{z, pold, pnew, S + S_BCOEF}, OutApplyDot<double>{w}, s->cx.part);
         else if (L.dim == 2)
             k_stencil<double, 2, OpConst<double>, SrcDir<double>, OutApplyDot<double>><<<g, bk, 0, st>>>(
-                L.mx, L.my, L.mz, KCH, oc, SrcDir<double>{z, pold, pnew, S + S_BCOEF}, OutApplyDot<double>{w}, s->cx.part);
+                L.mx, L.my, L.mz, kchf(L.dim, L.my, L.mz), oc, SrcDir<double>{z, pold, pnew, S + S_BCOEF}, OutApplyDot<double>{w}, s->cx.part);
         else
             k_stencil<double, 1, OpConst<double>, SrcDir<double>, OutApplyDot<double>><<<g, bk, 0, st>>>(
-                L.mx, L.my, L.mz, KCH, oc, SrcDir<double>{z, pold, pnew, S + S_BCOEF}, OutApplyDot<double>{w}, s->cx.part);
+                L.mx, L.my, L.mz, kchf(L.dim, L.my, L.mz), oc, SrcDir<double>{z, pold, pnew, S + S_BCOEF}, OutApplyDot<double>{w}, s->cx.part);
         LAUNCH_CHECK();
         std::swap(pold, pnew);
         k_finalize<<<1, 256, 0, st>>>(nbs, s->cx.part, S, FIN_RAW, 0);
@@ -980,8 +985,9 @@ static int configure(p4g_solver_s* s) {
     const std::string pr = o.gets("p4g_precision", "double");
     if (pr == "mixed") s->mixed = true;
     else if (pr != "double") return set_error(PETSC_ERR_ARG_WRONG, "-p4g_precision %s", pr.c_str());
-    s->inner_rtol = o.getr("p4g_inner_rtol", 1e-5);
-    s->inner_maxit = o.geti("p4g_inner_max_it", 200);
+    s->reliable_delta = o.getr("p4g_reliable_delta", 1e-2);
+    if (!(s->reliable_delta > 0.0 && s->reliable_delta < 1.0))
+        return set_error(PETSC_ERR_ARG_OUTOFRANGE, "-p4g_reliable_delta must be in (0,1)");
     if (s->mixed && s->pctype != PCT_MG) return set_error(PETSC_ERR_SUP, "-p4g_precision mixed needs -pc_type mg");
     return 0;
 }
@@ -1086,7 +1092,9 @@ int solver_setup(p4g_solver_s* s) {
         HIP_TRY(hipMalloc(&s->cx.part, 2 * cap * sizeof(double)));
         s->cx.npart_cap = cap;
         HIP_TRY(hipMalloc(&s->cx.S, 32 * sizeof(double)));
-        HIP_TRY(hipMemset(s->cx.S, 0, 32 * sizeof(double)));
+        double init[32] = {0};
+        init[S_ONE] = init[16 + S_ONE] = 1.0;
+        HIP_TRY(hipMemcpy(s->cx.S, init, sizeof init, hipMemcpyHostToDevice));
         HIP_TRY(hipHostMalloc((void**)&s->cx.hS, 32 * sizeof(double), hipHostMallocDefault));
     }
     if (s->pctype == PCT_MG) {
@@ -1139,39 +1147,55 @@ static int fine64(p4g_solver_s* s, Src src, Out out, int cls, double bytes, long
         ov.kc = s->kc64; ov.cx = F.cx; ov.cy = F.cy; ov.cz = F.cz;
         ov.ax = F.ax; ov.ay = F.ay; ov.az = F.az; ov.jac = 0;
         return s->cx.launch(cls, true, bytes, [&] {
-            k_stencil<double, 3, OpVar<double>, Src, Out><<<g, bk, 0, st>>>(F.mx, F.my, F.mz, KCH, ov, src, out, part);
+            k_stencil<double, 3, OpVar<double>, Src, Out><<<g, bk, 0, st>>>(F.mx, F.my, F.mz, kchf(F.dim, F.my, F.mz), ov, src, out, part);
         });
     }
     OpConst<double> oc;
     oc.d = F.d; oc.db = F.db; oc.sx = F.sx; oc.sy = F.sy; oc.sz = F.sz; oc.id = oc.idb = 1.0;
     return s->cx.launch(cls, true, bytes, [&] {
         if (F.dim == 3)
-            k_stencil<double, 3, OpConst<double>, Src, Out><<<g, bk, 0, st>>>(F.mx, F.my, F.mz, KCH, oc, src, out, part);
+            k_stencil<double, 3, OpConst<double>, Src, Out><<<g, bk, 0, st>>>(F.mx, F.my, F.mz, kchf(F.dim, F.my, F.mz), oc, src, out, part);
         else if (F.dim == 2)
-            k_stencil<double, 2, OpConst<double>, Src, Out><<<g, bk, 0, st>>>(F.mx, F.my, F.mz, KCH, oc, src, out, part);
+            k_stencil<double, 2, OpConst<double>, Src, Out><<<g, bk, 0, st>>>(F.mx, F.my, F.mz, kchf(F.dim, F.my, F.mz), oc, src, out, part);
         else
-            k_stencil<double, 1, OpConst<double>, Src, Out><<<g, bk, 0, st>>>(F.mx, F.my, F.mz, KCH, oc, src, out, part);
+            k_stencil<double, 1, OpConst<double>, Src, Out><<<g, bk, 0, st>>>(F.mx, F.my, F.mz, kchf(F.dim, F.my, F.mz), oc, src, out, part);
     });
 }
 
-// FP64 iterative refinement around the FP32 PCG + FP32 V-cycle:
-//   r = b - A x (FP64, stored FP32), solve A e = r by FP32 PCG to
-//   inner_rtol, x += e fused into the next residual pass.
+// Mixed precision: FP32 PCG (FP32 vectors, FP32 V-cycle, FP64 coarse
+// solve and dot accumulation) with FP64 residual replacement ("reliable
+// updates", van der Vorst & Ye, SISC 22 (2000)): the FP32 iterate holds
+// the correction since the last replacement; when the iterated residual
+// has fallen by delta (or below the target), the correction is folded into
+// the FP64 iterate and r = b - A x is recomputed in FP64 in the same pass.
+// The Krylov recurrence continues across replacements; convergence is
+// decided on that FP64 residual: |b - A x|_2 <= max(rtol |b|_2, atol).
 static int mixed_solve(p4g_solver_s* s, const double* b, double* x) {
     Engine<float>& e = *s->e32;
     const LevelInfo& F = s->lv.back();
     const long n = F.n;
     hipStream_t st = s->cx.stream;
-    double* S = s->cx.S + 16;
+    Ctx* cx = &s->cx;
+    double* S = cx->S;          // engine scalars
+    double* So = cx->S + 16;    // outer scalars
+    const double* hs = cx->hS;
+    const double* ho = cx->hS + 16;
     const int gb = vblocks(n);
-    P4G_TRY(s->cx.launch(P4G_K_OTHER, true, 20.0 * n, [&] {
+    float* x32 = e.xbuf;        // correction since the last replacement
+    float* r32 = e.r;
+    P4G_TRY(cx->launch(P4G_K_OTHER, true, 20.0 * n, [&] {
         k_fill<double><<<gb, 256, 0, st>>>(n, 0.0, s->xa);
-        k_convert<double, float><<<gb, 256, 0, st>>>(n, b, s->r32);
-        k_dot2<double><<<gb, 256, 0, st>>>(n, b, b, s->cx.part);
+        k_fill<float><<<gb, 256, 0, st>>>(n, 0.0f, e.pa);
+        k_convert<double, float><<<gb, 256, 0, st>>>(n, b, r32);
+        k_dot2<double><<<gb, 256, 0, st>>>(n, b, b, cx->part);
     }));
-    P4G_TRY(s->cx.finalize(gb, S, FIN_RAW, 0));
-    P4G_TRY(s->cx.readS(s->cx.hS));
-    const double bnorm = std::sqrt(s->cx.hS[16 + S_TMP0]);
+    P4G_TRY(cx->finalize(gb, So, FIN_RAW, 0));
+    long nb = 0;
+    P4G_TRY(e.pc_apply_dots(r32, e.z, &nb));
+    P4G_TRY(cx->finalize(nb, S, FIN_BETA_PRECOND, 1));
+    P4G_TRY(cx->readS(cx->hS));
+    const double bnorm = std::sqrt(ho[S_TMP0]);
+    const double target = std::max(s->rtol * bnorm, s->atol);
     p4g_Report& R = s->rep;
     R.rnorm0 = bnorm;
     R.rnorm = bnorm;
@@ -1179,33 +1203,71 @@ static int mixed_solve(p4g_solver_s* s, const double* b, double* x) {
     R.its = 0;
     R.reason = 0;
     for (int q = 0; q < 64; ++q) R.history[q] = 0.0;
-    int hn = 0;
-    R.history[hn++] = bnorm;
+    R.history[0] = bnorm;
     double* xcur = s->xa;
     double* xnext = s->xb;
-    if (bnorm == 0.0) R.reason = P4G_CONVERGED_ATOL;
-    for (int it = 0; it < s->maxit && !R.reason; ++it) {
-        typename Engine<float>::CGOut co;
-        P4G_TRY(e.cg(s->r32, s->e32v, NT_PRECOND, s->inner_rtol, 0.0, 1e30, s->inner_maxit, co, s->cx.S));
-        R.inner_its += co.its;
-        long nbk = 0;
-        P4G_TRY(fine64(s, SrcSum<double, float>{xcur, s->e32v, xnext}, OutResidCvt<double, float>{b, s->r32},
-                       P4G_K_OTHER, 32.0 * n, &nbk));
-        std::swap(xcur, xnext);
-        P4G_TRY(s->cx.finalize(nbk, S, FIN_RAW, 0));
-        P4G_TRY(s->cx.readS(s->cx.hS));
-        const double rn = std::sqrt(s->cx.hS[16 + S_TMP0]);
-        R.its = it + 1;
-        R.rnorm = rn;
-        if (hn < 64) R.history[hn++] = rn;
-        if (std::isnan(rn) || std::isinf(rn)) { R.reason = P4G_DIVERGED_NANORINF; break; }
-        if (rn <= std::max(s->rtol * bnorm, s->atol)) {
-            R.reason = rn < s->atol ? P4G_CONVERGED_ATOL : P4G_CONVERGED_RTOL;
-            break;
+    if (bnorm <= s->atol) R.reason = P4G_CONVERGED_ATOL;
+    const double delta = s->reliable_delta;
+    double rr_last = bnorm * bnorm;   // |r|^2 at the last replacement
+    bool replace_now = false, folded = true;
+    int keep = S_ZERO;                // the first update sets x32 = alpha p
+    float* pold = e.pa;
+    float* pnew = e.pb;
+    double dpi_old = 0.0;
+    int i = 0, nrep = 0;
+    for (; i < s->maxit && !R.reason; ++i) {
+        R.its = i + 1;
+        if (hs[S_RZ] == 0.0) replace_now = true;   // exact breakdown: certify by replacement
+        long nbs = 0;
+        P4G_TRY(e.stencil(s->nlev - 1, SrcDir<float>{e.z, pold, pnew, S + S_BCOEF}, OutApplyDot<float>{e.w},
+                          P4G_K_STENCIL_DOT, 4, &nbs, false));
+        P4G_TRY(cx->finalize(nbs, S, FIN_ALPHA, 0));
+        std::swap(pold, pnew);
+        P4G_TRY(cx->launch(P4G_K_CG_UPDATE, true, 24.0 * n, [&] {
+            k_cg_update<float, 1><<<gb, 256, 0, st>>>(n, S, S_ALPHA, keep, x32, pold, r32, e.w, cx->part);
+        }));
+        P4G_TRY(cx->finalize(gb, S, FIN_RR, 0));
+        keep = S_ONE;
+        folded = false;
+        const bool replaced = replace_now;
+        if (replace_now) {
+            long nbk = 0;
+            P4G_TRY(fine64(s, SrcSum<double, float>{xcur, x32, xnext}, OutResidCvt<double, float>{b, r32},
+                           P4G_K_OTHER, 36.0 * n, &nbk));
+            P4G_TRY(cx->finalize(nbk, So, FIN_RAW, 0));
+            std::swap(xcur, xnext);
+            keep = S_ZERO;
+            folded = true;
+            replace_now = false;
+            ++nrep;
         }
+        P4G_TRY(e.pc_apply_dots(r32, e.z, &nb));
+        P4G_TRY(cx->finalize(nb, S, FIN_BETA_PRECOND, 0));
+        P4G_TRY(cx->readS(cx->hS));
+        const double dpi = hs[S_PW];
+        if (dpi == 0.0 || (i > 0 && ((dpi > 0) != (dpi_old > 0)))) { R.reason = P4G_DIVERGED_INDEFINITE_MAT; break; }
+        dpi_old = dpi;
+        double rn;
+        if (replaced) {
+            rn = std::sqrt(ho[S_TMP0]);
+            rr_last = rn * rn;
+        } else {
+            rn = std::sqrt(hs[S_RR]);
+        }
+        R.rnorm = rn;
+        if (i + 1 < 64) R.history[i + 1] = rn;
+        if (std::isnan(rn) || std::isinf(rn)) { R.reason = P4G_DIVERGED_NANORINF; break; }
+        if (replaced && rn <= target) { R.reason = rn < s->atol ? P4G_CONVERGED_ATOL : P4G_CONVERGED_RTOL; break; }
         if (rn >= s->dtol * bnorm) { R.reason = P4G_DIVERGED_DTOL; break; }
+        if (!replaced && (rn <= target || rn * rn <= delta * delta * rr_last)) replace_now = true;
     }
     if (!R.reason) R.reason = P4G_DIVERGED_ITS;
+    R.inner_its = nrep;   // number of FP64 residual replacements
+    if (!folded) {        // fold the last correction: x = x + x32
+        P4G_TRY(cx->launch(P4G_K_OTHER, true, 20.0 * n, [&] {
+            k_axpy_mixed<<<gb, 256, 0, st>>>(n, xcur, x32);
+        }));
+    }
     HIP_TRY(hipMemcpyAsync(x, xcur, n * sizeof(double), hipMemcpyDeviceToDevice, st));
     return 0;
 }
@@ -1461,7 +1523,8 @@ int p4g_mat_mult_device(p4g_Solver s, const double* d_x, double* d_y) {
     clear_error();
     if (!s || !d_x || !d_y) return set_error(PETSC_ERR_ARG_WRONG, "p4g_mat_mult_device: null argument");
     P4G_TRY(solver_setup(s));
-    P4G_TRY(fine64(s, SrcArr<double>{d_x}, OutApply<double>{d_y}, P4G_K_OTHER, 16.0 * s->lv.back().n, nullptr));
+    P4G_TRY(fine64(s, SrcArr<double>{d_x}, OutApply<double>{d_y}, P4G_K_APPLY, 16.0 * s->lv.back().n, nullptr));
+    s->cx.tm.collect();
     HIP_TRY(hipStreamSynchronize(s->cx.stream));
     return 0;
 }
@@ -1559,3 +1622,6 @@ int p4g_memcpy_dtoh(void* dst, const void* src, size_t bytes) {
 const char* p4g_version(void) { return "p4g 0.1 (gfx950)"; }
 
 }  // extern "C"
+
+// continuous-batching server (uses the types above)
+#include "p4g_serve_impl.hpp"
