@@ -237,6 +237,8 @@ static int blaunch(p4g_server_s* v, const BLaunch& L, const char* base) {
 static int upload(p4g_server_s* v, const Packer& pk, char** dbuf, char** hbuf, size_t* cap) {
     const size_t bytes = pk.buf.size();
     if (bytes == 0) return 0;
+    // work queued earlier may still read the device table or the staging buffer
+    HIP_TRY(hipStreamSynchronize(v->st));
     if (bytes > *cap) {
         if (*dbuf) hipFree(*dbuf);
         if (*hbuf) hipHostFree(*hbuf);
@@ -244,8 +246,6 @@ static int upload(p4g_server_s* v, const Packer& pk, char** dbuf, char** hbuf, s
         HIP_TRY(hipMalloc((void**)dbuf, *cap));
         HIP_TRY(hipHostMalloc((void**)hbuf, *cap, hipHostMallocDefault));
     }
-    // the previous upload from this staging buffer must have completed
-    HIP_TRY(hipStreamSynchronize(v->st));
     std::memcpy(*hbuf, pk.buf.data(), bytes);
     HIP_TRY(hipMemcpyAsync(*dbuf, *hbuf, bytes, hipMemcpyHostToDevice, v->st));
     return 0;
@@ -400,10 +400,10 @@ static int req_admit(p4g_server_s* v, SrvReq* r) {
     v->free_slots.pop_back();
     r->S = v->S + 32 * r->slot;
     // initial state: x = 0, p = 0, r32 = b, |b|^2 -> outer TMP0
-    double init[32] = {0};
-    init[S_ONE] = init[16 + S_ONE] = 1.0;
     hipStream_t st = v->st;
-    HIP_TRY(hipMemcpyAsync(r->S, init, sizeof init, hipMemcpyHostToDevice, st));
+    k_fill<double><<<1, 256, 0, st>>>(32, 0.0, r->S);
+    k_fill<double><<<1, 256, 0, st>>>(1, 1.0, r->S + S_ONE);
+    k_fill<double><<<1, 256, 0, st>>>(1, 1.0, r->S + 16 + S_ONE);
     HIP_TRY(hipMemcpyAsync(r->b64, r->hb, n * 8, hipMemcpyHostToDevice, st));
     const int gb = vblocks(n);
     k_fill<double><<<gb, 256, 0, st>>>(n, 0.0, r->xa);
@@ -593,7 +593,7 @@ static void serve_loop(p4g_server_s* v) {
         {
             std::unique_lock<std::mutex> lk(v->mu);
             v->cv.wait(lk, [&] { return v->stop || !v->queue.empty() || !v->active.empty(); });
-            if (v->stop && v->active.empty()) break;
+            if (v->stop && v->active.empty() && v->queue.empty()) break;
             // admission in arrival order, skipping requests that do not fit yet
             for (auto it = v->queue.begin(); it != v->queue.end();) {
                 SrvReq* r = *it;
