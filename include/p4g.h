@@ -27,13 +27,20 @@
 
 #include <stddef.h>
 #include <stdint.h>
+/* A program that also includes PETSc's own headers (a PCSHELL/MATSHELL
+ * binding inside a real PETSc build) defines P4G_HOSTED_BY_PETSC: the
+ * PETSc-shaped host surface is then left out and grids are p4g_DA handles;
+ * the callback registration needs that surface and is left out too. */
+#ifndef P4G_HOSTED_BY_PETSC
 #include "p4g_petsc.h"
+#endif
 
 #ifdef __cplusplus
 extern "C" {
 #endif
 
 typedef struct p4g_solver_s *p4g_Solver;
+typedef struct p4g_da_s *p4g_DA;   /* the DM of p4g_petsc.h */
 
 /* KSPConvergedReason numbering (PETSc) */
 #define P4G_CONVERGED_RTOL 2
@@ -64,23 +71,25 @@ typedef struct {
 /* DMDACreate{1,2,3}d(DM_BOUNDARY_NONE, STAR, dof 1, width 1) with a global
  * size of mx (x my (x mz)) vertices on [0,Lx]x[0,Ly]x[0,Lz]
  * (fish.c:197-220; the reference starts from 3 points per dimension). */
-int p4g_da_create(int dim, int mx, int my, int mz, double Lx, double Ly, double Lz, DM *da);
+int p4g_da_create(int dim, int mx, int my, int mz, double Lx, double Ly, double Lz, p4g_DA *da);
 /* DMSetFromOptions: -da_grid_x/y/z N, -da_refine R (m <- 2(m-1)+1, R
  * times); the refinement count is the default number of multigrid levels
  * minus one, as with PETSc's PCMG on a refined DMDA. */
-int p4g_da_set_from_options(DM da, const char *options);
-int p4g_da_get_sizes(DM da, int *dim, int *mx, int *my, int *mz, int *refine);
+int p4g_da_set_from_options(p4g_DA da, const char *options);
+int p4g_da_get_sizes(p4g_DA da, int *dim, int *mx, int *my, int *mz, int *refine);
+#ifndef P4G_HOSTED_BY_PETSC
 /* DMDASNESSetFunctionLocal(da, INSERT_VALUES, fn, ctx)  (fish.c:225-226) */
-int p4g_da_set_function_local(DM da, DMDASNESFunctionFn *fn, void *ctx);
+int p4g_da_set_function_local(p4g_DA da, DMDASNESFunctionFn *fn, void *ctx);
 /* DMDASNESSetJacobianLocal(da, fn, ctx)                 (fish.c:227-228) */
-int p4g_da_set_jacobian_local(DM da, DMDASNESJacobianFn *fn, void *ctx);
+int p4g_da_set_jacobian_local(p4g_DA da, DMDASNESJacobianFn *fn, void *ctx);
+#endif
 /* Variable-coefficient operator -div(k grad u) with k given per cell,
  * (mx-1)(my-1)(mz-1) values (3D only), face coefficient = mean of the 4
  * cells sharing the face's edge, coarse k = mean of the 8 fine cells.
  * Replaces the Jacobian callback's operator (the reference's callbacks
  * have constant cx,cy,cz only, poissonfunctions.h:44-47). */
-int p4g_da_set_cell_coefficient(DM da, const double *kcell);
-int p4g_da_destroy(DM *da);
+int p4g_da_set_cell_coefficient(p4g_DA da, const double *kcell);
+int p4g_da_destroy(p4g_DA *da);
 
 /* --------------------------------------------------------------- solver */
 
@@ -102,7 +111,7 @@ int p4g_da_destroy(DM *da);
  * Options of PETSc objects the device path does not implement (-pc_type
  * ilu|sor|lu, -mg_levels_pc_type sor, -pc_mg_galerkin, ...) are rejected
  * with PETSC_ERR_SUP. */
-int p4g_solver_create(DM da, const char *options, p4g_Solver *s);
+int p4g_solver_create(p4g_DA da, const char *options, p4g_Solver *s);
 /* PCSetUp_MG equivalent: Jacobian callbacks per level, stencil extraction,
  * device hierarchy, Chebyshev eigen-estimates, coarse Cholesky. */
 int p4g_solver_setup(p4g_Solver s);

@@ -83,6 +83,17 @@ struct OutChebR {
             a1 += double(v) * double(v);
         }
     }
+    __device__ __forceinline__ void point2(long q, T ua, T ub, T Aa, T Ab, T ia, T ib, double& a0,
+                                           double& a1) const {
+        const T2<T> bb = ld2(b + q);
+        const T va = c1 * ua + c2 * (ia * (bb.a - Aa));
+        const T vb = c1 * ub + c2 * (ib * (bb.b - Ab));
+        st2(y + q, va, vb);
+        if (dots) {
+            a0 += double(va) * double(bb.a) + double(vb) * double(bb.b);
+            a1 += double(va) * double(va) + double(vb) * double(vb);
+        }
+    }
 };
 
 // ---------------------------------------------------------------- transfers
@@ -196,27 +207,14 @@ struct FinItem {
     const double* flag;    // optional: skip when *flag == 0
 };
 
-__global__ void __launch_bounds__(256)
+// launch: <<<nitems, FT>>>
+__global__ void __launch_bounds__(FT)
 k_finalize_batch(const FinItem* __restrict__ items) {
     const FinItem& it = items[blockIdx.x];
     if (it.flag && *it.flag == 0.0) return;
-    __shared__ double s0[256], s1[256];
-    double a = 0.0, b = 0.0;
-    for (long q = threadIdx.x; q < it.nparts; q += 256) {
-        a += it.part[2 * q];
-        b += it.part[2 * q + 1];
-    }
-    s0[threadIdx.x] = a;
-    s1[threadIdx.x] = b;
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if (threadIdx.x < w) {
-            s0[threadIdx.x] += s0[threadIdx.x + w];
-            s1[threadIdx.x] += s1[threadIdx.x + w];
-        }
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) finalize_scalars(it.S, it.mode, it.first, s0[0], s1[0]);
+    double A, B;
+    sum_parts(it.part, it.nparts, A, B);
+    if (threadIdx.x == 0) finalize_scalars(it.S, it.mode, it.first, A, B);
 }
 
 }  // namespace p4g

@@ -27,6 +27,31 @@ constexpr int TX = 64;   // lanes along x: one wavefront
 constexpr int TY = 4;    // rows (3D) or march chunks (2D) per block
 constexpr int NT = TX * TY;
 
+// Two consecutive grid values, moved as one 2-element vector access.  The
+// address need only be aligned to one element (gfx950 global memory takes
+// unaligned vector accesses).
+template <typename T>
+struct T2 {
+    T a, b;
+};
+
+template <typename T>
+__device__ __forceinline__ T2<T> ld2(const T* __restrict__ p) {
+    typedef T vec __attribute__((ext_vector_type(2)));
+    vec v;
+    __builtin_memcpy(&v, p, sizeof(vec));
+    return T2<T>{v.x, v.y};
+}
+
+template <typename T>
+__device__ __forceinline__ void st2(T* __restrict__ p, T a, T b) {
+    typedef T vec __attribute__((ext_vector_type(2)));
+    vec v;
+    v.x = a;
+    v.y = b;
+    __builtin_memcpy(p, &v, sizeof(vec));
+}
+
 // ------------------------------------------------------------- operators
 
 // Constant-coefficient stencil.
@@ -96,13 +121,17 @@ template <typename T>
 struct SrcArr {                       // u = x
     const T* __restrict__ x;
     __device__ __forceinline__ T load(long q) const { return x[q]; }
+    __device__ __forceinline__ T2<T> load2(long q) const { return ld2(x + q); }
     __device__ __forceinline__ void put(long, T) const {}
+    __device__ __forceinline__ void put2(long, T, T) const {}
 };
 
 template <typename T>
 struct SrcZero {                      // u = 0 (first smoothing step from a zero guess)
     __device__ __forceinline__ T load(long) const { return T(0); }
+    __device__ __forceinline__ T2<T> load2(long) const { return T2<T>{T(0), T(0)}; }
     __device__ __forceinline__ void put(long, T) const {}
+    __device__ __forceinline__ void put2(long, T, T) const {}
 };
 
 template <typename T>
@@ -110,7 +139,12 @@ struct SrcScaled {                    // u = c b   (zero-guess smoothing step)
     const T* __restrict__ b;
     T c;
     __device__ __forceinline__ T load(long q) const { return c * b[q]; }
+    __device__ __forceinline__ T2<T> load2(long q) const {
+        const T2<T> v = ld2(b + q);
+        return T2<T>{c * v.a, c * v.b};
+    }
     __device__ __forceinline__ void put(long, T) const {}
+    __device__ __forceinline__ void put2(long, T, T) const {}
 };
 
 template <typename T>
@@ -119,7 +153,12 @@ struct SrcScaledPut {                 // u = c b, stored to x
     T c;
     T* __restrict__ x;
     __device__ __forceinline__ T load(long q) const { return c * b[q]; }
+    __device__ __forceinline__ T2<T> load2(long q) const {
+        const T2<T> v = ld2(b + q);
+        return T2<T>{c * v.a, c * v.b};
+    }
     __device__ __forceinline__ void put(long q, T v) const { x[q] = v; }
+    __device__ __forceinline__ void put2(long q, T a, T bb) const { st2(x + q, a, bb); }
 };
 
 template <typename T>
@@ -129,7 +168,13 @@ struct SrcDir {                       // p_new = z + beta p_old, stored (CG dire
     T* __restrict__ pnew;
     const double* __restrict__ beta;  // device scalar
     __device__ __forceinline__ T load(long q) const { return z[q] + T(*beta) * pold[q]; }
+    __device__ __forceinline__ T2<T> load2(long q) const {
+        const T bt = T(*beta);
+        const T2<T> zz = ld2(z + q), pp = ld2(pold + q);
+        return T2<T>{zz.a + bt * pp.a, zz.b + bt * pp.b};
+    }
     __device__ __forceinline__ void put(long q, T v) const { pnew[q] = v; }
+    __device__ __forceinline__ void put2(long q, T a, T b) const { st2(pnew + q, a, b); }
 };
 
 template <typename T, typename E>
@@ -138,7 +183,13 @@ struct SrcSum {                       // x_new = x + e (refinement update), stor
     const E* __restrict__ e;
     T* __restrict__ xnew;
     __device__ __forceinline__ T load(long q) const { return x[q] + T(e[q]); }
+    __device__ __forceinline__ T2<T> load2(long q) const {
+        const T2<T> xx = ld2(x + q);
+        const T2<E> ee = ld2(e + q);
+        return T2<T>{xx.a + T(ee.a), xx.b + T(ee.b)};
+    }
     __device__ __forceinline__ void put(long q, T v) const { xnew[q] = v; }
+    __device__ __forceinline__ void put2(long q, T a, T b) const { st2(xnew + q, a, b); }
 };
 
 // ------------------------------------------------------ output policies
@@ -151,6 +202,10 @@ struct OutApplyDot {                  // w = A u ; acc0 += u.w
         w[q] = Au;
         a0 += double(u) * double(Au);
     }
+    __device__ __forceinline__ void point2(long q, T ua, T ub, T Aa, T Ab, T, T, double& a0, double&) const {
+        st2(w + q, Aa, Ab);
+        a0 += double(ua) * double(Aa) + double(ub) * double(Ab);
+    }
 };
 
 template <typename T>
@@ -158,6 +213,9 @@ struct OutApply {                     // w = A u
     T* __restrict__ w;
     static constexpr int NDOT = 0;
     __device__ __forceinline__ void point(long q, T, T Au, T, double&, double&) const { w[q] = Au; }
+    __device__ __forceinline__ void point2(long q, T, T, T Aa, T Ab, T, T, double&, double&) const {
+        st2(w + q, Aa, Ab);
+    }
 };
 
 template <typename T, int NORM>
@@ -170,6 +228,12 @@ struct OutResid {                     // r = b - A u ; acc0 += r.r (NORM)
         r[q] = v;
         if (NORM) a0 += double(v) * double(v);
     }
+    __device__ __forceinline__ void point2(long q, T, T, T Aa, T Ab, T, T, double& a0, double&) const {
+        const T2<T> bb = ld2(b + q);
+        const T va = bb.a - Aa, vb = bb.b - Ab;
+        st2(r + q, va, vb);
+        if (NORM) a0 += double(va) * double(va) + double(vb) * double(vb);
+    }
 };
 
 template <typename T, typename R>
@@ -181,6 +245,12 @@ struct OutResidCvt {                  // r = b - A u in T, stored as R ; acc0 +=
         const T v = b[q] - Au;
         r[q] = R(v);
         a0 += double(v) * double(v);
+    }
+    __device__ __forceinline__ void point2(long q, T, T, T Aa, T Ab, T, T, double& a0, double&) const {
+        const T2<T> bb = ld2(b + q);
+        const T va = bb.a - Aa, vb = bb.b - Ab;
+        st2(r + q, R(va), R(vb));
+        a0 += double(va) * double(va) + double(vb) * double(vb);
     }
 };
 
@@ -201,6 +271,22 @@ struct OutCheb {
         if (DOTS) {
             a0 += double(v) * double(bq);
             a1 += double(v) * double(v);
+        }
+    }
+    __device__ __forceinline__ void point2(long q, T ua, T ub, T Aa, T Ab, T ia, T ib, double& a0,
+                                           double& a1) const {
+        const T2<T> bb = ld2(b + q);
+        T va = c1 * ua + c2 * (ia * (bb.a - Aa));
+        T vb = c1 * ub + c2 * (ib * (bb.b - Ab));
+        if (PREV) {
+            const T2<T> pp = ld2(prev + q);
+            va += c0 * pp.a;
+            vb += c0 * pp.b;
+        }
+        st2(y + q, va, vb);
+        if (DOTS) {
+            a0 += double(va) * double(bb.a) + double(vb) * double(bb.b);
+            a1 += double(va) * double(va) + double(vb) * double(vb);
         }
     }
 };
@@ -236,109 +322,143 @@ __device__ __forceinline__ void block_partials(double a0, double a1, double* __r
 
 // ------------------------------------------------------------- stencil
 
-// Generic marching stencil pass: for every vertex, u from Src, A u and the
-// Jacobi factor from Op, result written by Out.  Grid:
-//   3D: (ceil((mx-1)/64), ceil(my/4), ceil(mz/kch)), block (64,4)
-//   2D: (ceil((mx-1)/64), ceil(ceil(my/kch)/4), 1),  block (64,4)
-//   1D: (ceil((mx-1)/64), 1, 1),                     block (64,1)
+// Marching stencil pass over a 2D or 3D grid: for every vertex, u from Src,
+// A u and the Jacobi factor from Op, result written by Out.  A lane owns the
+// x pair (i0, i0+1), i0 = 1 + 2 (64 bx + lane), moved with 2-element vector
+// accesses; the west value of i0 and the east value of i0+1 come from the
+// neighbouring lanes (the wave's edge lanes load them).  The lane at i0 = 1
+// also owns the boundary point i = 0.  The march runs along the slowest
+// dimension (z in 3D, y in 2D) keeping the planes m-1, m, m+1 in registers;
+// 3D rows j+-1 are read through the cache.  Every lane of a wave runs the
+// same loop (loads are predicated) so the cross-lane exchanges see the whole
+// wave.  Grid:
+//   3D: (ceil((mx-1)/128), ceil(my/4), ceil(mz/kch)), block (64,4)
+//   2D: (ceil((mx-1)/128), ceil(ceil(my/kch)/4), 1),  block (64,4)
+template <typename T, int DIM, class Op, class Src, class Out>
+__device__ __forceinline__ void stencil_pairs(int bx, int by, int bz, int mx, int my, int mz, int kch,
+                                              const Op& op, const Src& src, const Out& out, double& a0,
+                                              double& a1) {
+    const int lane = threadIdx.x;
+    const int i0 = 1 + 2 * (bx * TX + lane), i1 = i0 + 1;
+    const T z0 = T(0);
+    const long sy = mx;
+    int j, m0, m1, M;
+    long sm, base;
+    if (DIM == 3) {
+        j = by * TY + threadIdx.y;
+        M = mz;
+        sm = (long)mx * my;
+        m0 = bz * kch;
+        m1 = min(m0 + kch, mz);
+        base = (long)j * sy;
+        if (j >= my) return;           // uniform per wave
+    } else {
+        j = 0;
+        M = my;
+        sm = sy;
+        m0 = (by * TY + threadIdx.y) * kch;
+        m1 = min(m0 + kch, my);
+        base = 0;
+    }
+    if (m0 >= m1) return;              // uniform per wave
+    const bool act = i0 < mx;
+    const bool two = i1 < mx;
+    const bool jb = DIM == 3 && (j == 0 || j == my - 1);
+    const bool ms = DIM == 3 && j - 1 > 0, mn = DIM == 3 && j + 1 < my - 1;
+    const bool wa = i0 - 1 > 0;                    // i0 couples to its west neighbour
+    const bool ea = i1 < mx - 1;                   // i0 couples to i1 (both interior)
+    const bool eb = i1 + 1 < mx - 1;               // i1 couples to its east neighbour
+    const bool xa = i0 == mx - 1, xb = i1 == mx - 1;   // x-boundary points
+    auto L2 = [&](long a) -> T2<T> {
+        if (two) return src.load2(a);
+        return T2<T>{src.load(a), z0};
+    };
+    long q = base + (long)m0 * sm + i0;
+    T2<T> ud{z0, z0}, uc{z0, z0};
+    if (act) {
+        if (m0 > 0) ud = L2(q - sm);
+        uc = L2(q);
+    }
+    for (int m = m0; m < m1; ++m, q += sm) {
+        T2<T> uu{z0, z0};
+        if (act && m + 1 < M) uu = L2(q + sm);
+        T wv = __shfl_up(uc.b, 1, TX);
+        T ev = __shfl_down(uc.a, 1, TX);
+        if (act) {
+            if (lane == 0 && wa) wv = src.load(q - 1);
+            if (lane == TX - 1 && eb) ev = src.load(q + 2);
+            const int jj = DIM == 3 ? j : m, kk = DIM == 3 ? m : 0;
+            if (two) src.put2(q, uc.a, uc.b);
+            else src.put(q, uc.a);
+            T Aa, Ab = z0, ia, ib = z0;
+            if (jb || m == 0 || m == M - 1) {
+                op.bnd(i0, jj, kk, uc.a, Aa, ia);
+                if (two) op.bnd(i1, jj, kk, uc.b, Ab, ib);
+            } else {
+                T2<T> sv{z0, z0}, nv{z0, z0};
+                if (DIM == 3) {
+                    if (ms) sv = L2(q - sy);
+                    if (mn) nv = L2(q + sy);
+                } else {
+                    if (m - 1 > 0) sv = ud;
+                    if (m + 1 < M - 1) nv = uu;
+                }
+                T2<T> dn{z0, z0}, up{z0, z0};
+                if (DIM == 3) {
+                    if (m - 1 > 0) dn = ud;
+                    if (m + 1 < M - 1) up = uu;
+                }
+                if (xa) op.bnd(i0, jj, kk, uc.a, Aa, ia);
+                else op.interior(i0, jj, kk, uc.a, wa ? wv : z0, ea ? uc.b : z0, sv.a, nv.a, dn.a, up.a, Aa, ia);
+                if (two) {
+                    if (xb) op.bnd(i1, jj, kk, uc.b, Ab, ib);
+                    else op.interior(i1, jj, kk, uc.b, uc.a, eb ? ev : z0, sv.b, nv.b, dn.b, up.b, Ab, ib);
+                }
+            }
+            if (two) out.point2(q, uc.a, uc.b, Aa, Ab, ia, ib, a0, a1);
+            else out.point(q, uc.a, Aa, ia, a0, a1);
+            if (i0 == 1) {
+                const T u0 = src.load(q - 1);
+                src.put(q - 1, u0);
+                op.bnd(0, jj, kk, u0, Aa, ia);
+                out.point(q - 1, u0, Aa, ia, a0, a1);
+            }
+        }
+        ud = uc;
+        uc = uu;
+    }
+}
+
+// Generic stencil pass (see stencil_pairs); 1D: thread per point from i = 1,
+// grid (ceil((mx-1)/64), 1, 1), block (64,1).
 template <typename T, int DIM, class Op, class Src, class Out>
 __device__ __forceinline__ void stencil_body(int bx, int by, int bz, int mx, int my, int mz, int kch,
                                              const Op& op, const Src& src, const Out& out,
                                              double& a0, double& a1) {
+    if (DIM != 1) {
+        stencil_pairs<T, DIM, Op, Src, Out>(bx, by, bz, mx, my, mz, kch, op, src, out, a0, a1);
+        return;
+    }
     const int i = 1 + bx * TX + threadIdx.x;
     const T z0 = T(0);
-    if (DIM == 3) {
-        const int j = by * TY + threadIdx.y;
-        const long sy = mx, sz = (long)mx * my;
-        const int k0 = bz * kch;
-        const int k1 = min(k0 + kch, mz);
-        if (i < mx && j < my && k0 < k1) {
-            const bool bij = (i == mx - 1) || (j == 0) || (j == my - 1);
-            const bool mw = i - 1 > 0, me = i + 1 < mx - 1, ms = j - 1 > 0, mn = j + 1 < my - 1;
-            long q = (long)k0 * sz + (long)j * sy + i;
-            T ud = (k0 > 0) ? src.load(q - sz) : z0;
-            T uc = src.load(q);
-            for (int k = k0; k < k1; ++k, q += sz) {
-                const T uu = (k + 1 < mz) ? src.load(q + sz) : z0;
-                src.put(q, uc);
-                T Au, idg;
-                if (bij || k == 0 || k == mz - 1) {
-                    op.bnd(i, j, k, uc, Au, idg);
-                } else {
-                    const T w = mw ? src.load(q - 1) : z0;
-                    const T e = me ? src.load(q + 1) : z0;
-                    const T s = ms ? src.load(q - sy) : z0;
-                    const T n = mn ? src.load(q + sy) : z0;
-                    const T dn = (k - 1 > 0) ? ud : z0;
-                    const T up = (k + 1 < mz - 1) ? uu : z0;
-                    op.interior(i, j, k, uc, w, e, s, n, dn, up, Au, idg);
-                }
-                out.point(q, uc, Au, idg, a0, a1);
-                if (i == 1) {
-                    const T u0 = src.load(q - 1);
-                    src.put(q - 1, u0);
-                    op.bnd(0, j, k, u0, Au, idg);
-                    out.point(q - 1, u0, Au, idg, a0, a1);
-                }
-                ud = uc;
-                uc = uu;
-            }
+    if (i < mx) {
+        const long q = i;
+        const T uc = src.load(q);
+        src.put(q, uc);
+        T Au, idg;
+        if (i == mx - 1) {
+            op.bnd(i, 0, 0, uc, Au, idg);
+        } else {
+            const T w = (i - 1 > 0) ? src.load(q - 1) : z0;
+            const T e = (i + 1 < mx - 1) ? src.load(q + 1) : z0;
+            op.interior(i, 0, 0, uc, w, e, z0, z0, z0, z0, Au, idg);
         }
-    } else if (DIM == 2) {
-        const int chunk = by * TY + threadIdx.y;
-        const long sy = mx;
-        const int j0 = chunk * kch;
-        const int j1 = min(j0 + kch, my);
-        if (i < mx && j0 < j1) {
-            const bool bi = (i == mx - 1);
-            const bool mw = i - 1 > 0, me = i + 1 < mx - 1;
-            long q = (long)j0 * sy + i;
-            T ud = (j0 > 0) ? src.load(q - sy) : z0;
-            T uc = src.load(q);
-            for (int j = j0; j < j1; ++j, q += sy) {
-                const T uu = (j + 1 < my) ? src.load(q + sy) : z0;
-                src.put(q, uc);
-                T Au, idg;
-                if (bi || j == 0 || j == my - 1) {
-                    op.bnd(i, j, 0, uc, Au, idg);
-                } else {
-                    const T w = mw ? src.load(q - 1) : z0;
-                    const T e = me ? src.load(q + 1) : z0;
-                    const T s = (j - 1 > 0) ? ud : z0;
-                    const T n = (j + 1 < my - 1) ? uu : z0;
-                    op.interior(i, j, 0, uc, w, e, s, n, z0, z0, Au, idg);
-                }
-                out.point(q, uc, Au, idg, a0, a1);
-                if (i == 1) {
-                    const T u0 = src.load(q - 1);
-                    src.put(q - 1, u0);
-                    op.bnd(0, j, 0, u0, Au, idg);
-                    out.point(q - 1, u0, Au, idg, a0, a1);
-                }
-                ud = uc;
-                uc = uu;
-            }
-        }
-    } else {
-        if (i < mx) {
-            const long q = i;
-            const T uc = src.load(q);
-            src.put(q, uc);
-            T Au, idg;
-            if (i == mx - 1) {
-                op.bnd(i, 0, 0, uc, Au, idg);
-            } else {
-                const T w = (i - 1 > 0) ? src.load(q - 1) : z0;
-                const T e = (i + 1 < mx - 1) ? src.load(q + 1) : z0;
-                op.interior(i, 0, 0, uc, w, e, z0, z0, z0, z0, Au, idg);
-            }
-            out.point(q, uc, Au, idg, a0, a1);
-            if (i == 1) {
-                const T u0 = src.load(0);
-                src.put(0, u0);
-                op.bnd(0, 0, 0, u0, Au, idg);
-                out.point(0, u0, Au, idg, a0, a1);
-            }
+        out.point(q, uc, Au, idg, a0, a1);
+        if (i == 1) {
+            const T u0 = src.load(0);
+            src.put(0, u0);
+            op.bnd(0, 0, 0, u0, Au, idg);
+            out.point(0, u0, Au, idg, a0, a1);
         }
     }
 }
@@ -356,37 +476,56 @@ k_stencil(int mx, int my, int mz, int kch, Op op, Src src, Out out, double* __re
 // ------------------------------------------------------------ transfers
 
 // b_c = P^T r_f, unscaled full weighting (weights 1, 1/2 per dimension).
-// Thread per coarse column (I,J), marching K; the in-plane weighted sum of
-// the fine plane 2K+1 is reused as plane 2(K+1)-1.
-// Grid: (ceil(mxc/64), ceil(myc/4) [3D] | ceil(ceil(myc/kch)/4) [2D], ceil(mzc/kch) [3D])
+// A lane owns coarse column I = 1 + 64 bx + lane (the lane at I = 1 also
+// owns I = 0) and marches K; on each fine row it reads the pair (2I-1, 2I)
+// and takes 2I+1 from the next lane.  The in-plane weighted sum of the fine
+// plane 2K+1 is reused as plane 2(K+1)-1.  Loops are uniform per wave.
+// Grid: (ceil((mxc-1)/64), ceil(myc/4) [3D] | ceil(ceil(myc/kch)/4) [2D] | 1,
+//        ceil(mzc/kch) [3D] | 1)
 template <typename T, int DIM>
 __device__ __forceinline__ void restrict_body(int bx, int by, int bz, int mxf, int myf, int mzf, int mxc,
                                               int myc, int mzc, int kch, const T* __restrict__ rf,
                                               T* __restrict__ bc) {
-    const int I = bx * TX + threadIdx.x;
-    if (I >= mxc) return;
-    const int i0 = 2 * I;
-    const bool hw = i0 - 1 >= 0, he = i0 + 1 < mxf;
-    auto row = [&](long base) -> T {   // x-weighted sum on a fine row
-        T s = rf[base + i0];
-        if (hw) s += T(0.5) * rf[base + i0 - 1];
-        if (he) s += T(0.5) * rf[base + i0 + 1];
-        return s;
+    const int lane = threadIdx.x;
+    const int I = 1 + bx * TX + lane;
+    const bool act = I < mxc;
+    const bool own0 = I == 1;
+    const int f = 2 * I;
+    const bool he = f + 1 < mxf;
+    const T h = T(0.5);
+    // x-weighted sums on a fine row: a = coarse I, b = coarse 0 (lane I = 1)
+    auto row = [&](long rb) -> T2<T> {
+        T2<T> v{T(0), T(0)};
+        if (act) v = ld2(rf + rb + f - 1);
+        T nx = __shfl_down(v.a, 1, TX);
+        if (act && lane == TX - 1 && he) nx = rf[rb + f + 1];
+        T2<T> o;
+        o.a = v.b + h * (v.a + (he ? nx : T(0)));
+        o.b = own0 ? rf[rb] + h * rf[rb + 1] : T(0);
+        return o;
     };
+    auto comb = [&](T2<T> mid, T2<T> lo, T2<T> hi) -> T2<T> {
+        return T2<T>{mid.a + h * (lo.a + hi.a), mid.b + h * (lo.b + hi.b)};
+    };
+    auto store = [&](long cb, T2<T> v) {
+        if (act) bc[cb + I] = v.a;
+        if (own0) bc[cb] = v.b;
+    };
+    const T2<T> zz{T(0), T(0)};
     if (DIM == 1) {
-        bc[I] = row(0);
+        store(0, row(0));
         return;
     }
     if (DIM == 2) {
         const int chunk = by * TY + threadIdx.y;
         const int J0 = chunk * kch, J1 = min(J0 + kch, myc);
         if (J0 >= J1) return;
-        T slo = (2 * J0 - 1 >= 0) ? row((long)(2 * J0 - 1) * mxf) : T(0);
+        T2<T> slo = (2 * J0 - 1 >= 0) ? row((long)(2 * J0 - 1) * mxf) : zz;
         for (int J = J0; J < J1; ++J) {
             const int j = 2 * J;
-            const T smid = row((long)j * mxf);
-            const T shi = (j + 1 < myf) ? row((long)(j + 1) * mxf) : T(0);
-            bc[(long)J * mxc + I] = smid + T(0.5) * (slo + shi);
+            const T2<T> smid = row((long)j * mxf);
+            const T2<T> shi = (j + 1 < myf) ? row((long)(j + 1) * mxf) : zz;
+            store((long)J * mxc, comb(smid, slo, shi));
             slo = shi;
         }
         return;
@@ -394,22 +533,22 @@ __device__ __forceinline__ void restrict_body(int bx, int by, int bz, int mxf, i
     const int J = by * TY + threadIdx.y;
     if (J >= myc) return;
     const int K0 = bz * kch, K1 = min(K0 + kch, mzc);
+    if (K0 >= K1) return;
     const int j0 = 2 * J;
     const long sxy = (long)mxf * myf;
-    auto plane = [&](int k) -> T {     // x,y-weighted sum on fine plane k
-        const long base = (long)k * sxy + (long)j0 * mxf;
-        T s = row(base);
-        if (j0 - 1 >= 0) s += T(0.5) * row(base - mxf);
-        if (j0 + 1 < myf) s += T(0.5) * row(base + mxf);
-        return s;
+    auto plane = [&](int k) -> T2<T> {     // x,y-weighted sums on fine plane k
+        const long rb = (long)k * sxy + (long)j0 * mxf;
+        const T2<T> mid = row(rb);
+        const T2<T> lo = (j0 - 1 >= 0) ? row(rb - mxf) : zz;
+        const T2<T> hi = (j0 + 1 < myf) ? row(rb + mxf) : zz;
+        return comb(mid, lo, hi);
     };
-    if (K0 >= K1) return;
-    T slo = (2 * K0 - 1 >= 0) ? plane(2 * K0 - 1) : T(0);
+    T2<T> slo = (2 * K0 - 1 >= 0) ? plane(2 * K0 - 1) : zz;
     for (int K = K0; K < K1; ++K) {
         const int k = 2 * K;
-        const T smid = plane(k);
-        const T shi = (k + 1 < mzf) ? plane(k + 1) : T(0);
-        bc[((long)K * myc + J) * mxc + I] = smid + T(0.5) * (slo + shi);
+        const T2<T> smid = plane(k);
+        const T2<T> shi = (k + 1 < mzf) ? plane(k + 1) : zz;
+        store(((long)K * myc + J) * mxc, comb(smid, slo, shi));
         slo = shi;
     }
 }
@@ -422,24 +561,41 @@ k_restrict(int mxf, int myf, int mzf, int mxc, int myc, int mzc, int kch, const 
 }
 
 // out = base + P x_c, base = c b (ZG: zero-guess smoothing result) or x.
-// Thread per fine column (i,j) marching k; the coarse in-plane interpolant
-// of plane K is computed once and reused for fine planes 2K-1, 2K, 2K+1.
-// Grid: (ceil(mxf/64), ceil(myf/4) [3D] | chunks [2D], ceil(mzf/kch) [3D])
+// A lane owns coarse column I = 1 + 64 bx + lane and writes the fine pair
+// (2I-1, 2I) (the lane at I = 1 also writes fine 0); coarse I-1 comes from
+// the previous lane.  Fine rows j (3D) march k; the coarse in-plane
+// interpolant of plane K is computed once and reused for fine planes 2K-1,
+// 2K, 2K+1.  Loops are uniform per wave.
+// Grid: (ceil((mxc-1)/64), ceil(myf/4) [3D] | ceil(ceil(myf/kch)/4) [2D] | 1,
+//        ceil(mzf/kch) [3D] | 1)
 template <typename T, int DIM, bool ZG>
 __device__ __forceinline__ void prolong_body(int bx, int by, int bz, int mxf, int myf, int mzf, int mxc,
                                              int myc, int kch, const T* __restrict__ xb, T c,
                                              const T* __restrict__ xc, T* __restrict__ out) {
-    const int i = bx * TX + threadIdx.x;
-    if (i >= mxf) return;
-    const int ia = i >> 1, ib = (i + 1) >> 1;          // equal when i is even
-    const bool iodd = i & 1;
-    auto crow = [&](long base) -> T {
-        const T a = xc[base + ia];
-        return iodd ? T(0.5) * (a + xc[base + ib]) : a;
+    const int lane = threadIdx.x;
+    const int I = 1 + bx * TX + lane;
+    const bool act = I < mxc;
+    const bool own0 = I == 1;
+    const int f = 2 * I;
+    const T h = T(0.5);
+    // coarse row values: a = coarse I-1, b = coarse I
+    auto crow = [&](long cb) -> T2<T> {
+        const T v = act ? xc[cb + I] : T(0);
+        T w = __shfl_up(v, 1, TX);
+        if (act && lane == 0) w = xc[cb + I - 1];
+        return T2<T>{w, v};
     };
-    auto basev = [&](long q) -> T { return ZG ? c * xb[q] : xb[q]; };
+    auto mix = [&](T2<T> lo, T2<T> hi) -> T2<T> { return T2<T>{h * (lo.a + hi.a), h * (lo.b + hi.b)}; };
+    auto emit = [&](long fb, T2<T> v) {          // fine row base fb, coarse-row values v
+        if (act) {
+            const T2<T> bb = ld2(xb + fb + f - 1);
+            const T b0 = ZG ? c * bb.a : bb.a, b1 = ZG ? c * bb.b : bb.b;
+            st2(out + fb + f - 1, b0 + h * (v.a + v.b), b1 + v.b);
+        }
+        if (own0) out[fb] = (ZG ? c * xb[fb] : xb[fb]) + v.a;
+    };
     if (DIM == 1) {
-        out[i] = basev(i) + crow(0);
+        emit(0, crow(0));
         return;
     }
     if (DIM == 2) {
@@ -447,10 +603,9 @@ __device__ __forceinline__ void prolong_body(int bx, int by, int bz, int mxf, in
         const int j0 = chunk * kch, j1 = min(j0 + kch, myf);
         for (int j = j0; j < j1; ++j) {
             const int ja = j >> 1, jb = (j + 1) >> 1;
-            T v = crow((long)ja * mxc);
-            if (j & 1) v = T(0.5) * (v + crow((long)jb * mxc));
-            const long q = (long)j * mxf + i;
-            out[q] = basev(q) + v;
+            T2<T> v = crow((long)ja * mxc);
+            if (j & 1) v = mix(v, crow((long)jb * mxc));
+            emit((long)j * mxf, v);
         }
         return;
     }
@@ -459,27 +614,26 @@ __device__ __forceinline__ void prolong_body(int bx, int by, int bz, int mxf, in
     const int ja = j >> 1, jb = (j + 1) >> 1;
     const bool jodd = j & 1;
     const long sxyc = (long)mxc * myc;
-    auto cplane = [&](int K) -> T {
-        const long base = (long)K * sxyc;
-        T v = crow(base + (long)ja * mxc);
-        return jodd ? T(0.5) * (v + crow(base + (long)jb * mxc)) : v;
+    auto cplane = [&](int K) -> T2<T> {
+        const long cb = (long)K * sxyc;
+        const T2<T> v = crow(cb + (long)ja * mxc);
+        return jodd ? mix(v, crow(cb + (long)jb * mxc)) : v;
     };
     const int k0 = bz * kch, k1 = min(k0 + kch, mzf);
     if (k0 >= k1) return;
     const long sxy = (long)mxf * myf;
+    const T2<T> zz{T(0), T(0)};
     int K = k0 >> 1;
-    T plo = cplane(K);
-    T phi = (2 * K + 1 < mzf) ? cplane(K + 1) : T(0);
+    T2<T> plo = cplane(K);
+    T2<T> phi = (2 * K + 1 < mzf) ? cplane(K + 1) : zz;
     for (int k = k0; k < k1; ++k) {
         const int Kk = k >> 1;
         if (Kk != K) {     // advance one coarse plane
             K = Kk;
             plo = phi;
-            phi = (2 * K + 1 < mzf) ? cplane(K + 1) : T(0);
+            phi = (2 * K + 1 < mzf) ? cplane(K + 1) : zz;
         }
-        const T v = (k & 1) ? T(0.5) * (plo + phi) : plo;
-        const long q = (long)k * sxy + (long)j * mxf + i;
-        out[q] = basev(q) + v;
+        emit((long)k * sxy + (long)j * mxf, (k & 1) ? mix(plo, phi) : plo);
     }
 }
 
@@ -676,23 +830,42 @@ __device__ __forceinline__ void finalize_scalars(double* __restrict__ S, int mod
     }
 }
 
-__global__ void __launch_bounds__(256)
+// Fixed-order sum of nparts (A, B) pairs by one block of FT threads: each
+// thread adds a strided subset, then waves and the block reduce in a fixed
+// pattern.  The result is on thread 0.
+constexpr int FT = 1024;
+
+__device__ __forceinline__ void sum_parts(const double* __restrict__ part, long nparts, double& A, double& B) {
+    __shared__ double s0[FT / 64], s1[FT / 64];
+    double a = 0.0, b = 0.0;
+    for (long q = threadIdx.x; q < nparts; q += FT) {
+        const T2<double> v = ld2(part + 2 * q);
+        a += v.a;
+        b += v.b;
+    }
+    a = wave_sum(a);
+    b = wave_sum(b);
+    if ((threadIdx.x & 63) == 0) {
+        s0[threadIdx.x >> 6] = a;
+        s1[threadIdx.x >> 6] = b;
+    }
+    __syncthreads();
+    A = 0.0;
+    B = 0.0;
+    if (threadIdx.x == 0)
+        for (int w = 0; w < FT / 64; ++w) {
+            A += s0[w];
+            B += s1[w];
+        }
+}
+
+// launch: <<<1, FT>>>
+__global__ void __launch_bounds__(FT)
 k_finalize(long nparts, const double* __restrict__ part, double* __restrict__ S, int mode,
            int first) {
-    __shared__ double s0[256], s1[256];
-    double a = 0.0, b = 0.0;
-    for (long q = threadIdx.x; q < nparts; q += 256) { a += part[2 * q]; b += part[2 * q + 1]; }
-    s0[threadIdx.x] = a;
-    s1[threadIdx.x] = b;
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if (threadIdx.x < w) {
-            s0[threadIdx.x] += s0[threadIdx.x + w];
-            s1[threadIdx.x] += s1[threadIdx.x + w];
-        }
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) finalize_scalars(S, mode, first, s0[0], s1[0]);
+    double A, B;
+    sum_parts(part, nparts, A, B);
+    if (threadIdx.x == 0) finalize_scalars(S, mode, first, A, B);
 }
 
 }  // namespace p4g

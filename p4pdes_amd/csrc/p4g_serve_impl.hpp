@@ -151,8 +151,9 @@ static GridRef gref_stencil(const LevelInfo& L, long start) {
     return r;
 }
 
-static GridRef gref_point(const LevelInfo& L, long start) {
-    const dim3 g = point_grid(L.dim, L.mx, L.my, L.mz);
+// transfer launch over the grid L with lanes over the coarse columns 1..mxc-1
+static GridRef gref_transfer(const LevelInfo& L, int mxc, long start) {
+    const dim3 g = transfer_grid(L.dim, mxc, L.my, L.mz);
     GridRef r;
     r.mx = L.mx; r.my = L.my; r.mz = L.mz; r.kch = kchf(L.dim, L.my, L.mz);
     r.gx = g.x; r.gy = g.y; r.gz = g.z; r.start = start;
@@ -224,7 +225,7 @@ static int blaunch(p4g_server_s* v, const BLaunch& L, const char* base) {
         k_update_batch<float><<<g, 256, 0, st>>>((const UpdateItem<float>*)items, L.nitems);
         break;
     case BK_FIN:
-        k_finalize_batch<<<dim3((unsigned)L.nitems), 256, 0, st>>>((const FinItem*)items);
+        k_finalize_batch<<<dim3((unsigned)L.nitems), FT, 0, st>>>((const FinItem*)items);
         break;
     case BK_COARSE:
         k_coarse_batch<float><<<dim3((unsigned)L.nitems), 256, L.lds, st>>>((const CoarseItem<float>*)items);
@@ -284,7 +285,7 @@ static int build_vcycle(p4g_server_s* v, Packer& pk, std::vector<BLaunch>& seq) 
             it.out = OutResid<float, 0>{bvec(q.r, q.l), q.r->lv[q.l].r};
             ri.push_back(it);
             RestrictItem<float> rt;
-            rt.g = gref_point(C, s2);
+            rt.g = gref_transfer(C, C.mx, s2);
             s2 += gblocks(rt.g);
             rt.mxf = L.mx; rt.myf = L.my; rt.mzf = L.mz;
             rt.rf = q.r->lv[q.l].r;
@@ -327,7 +328,7 @@ static int build_vcycle(p4g_server_s* v, Packer& pk, std::vector<BLaunch>& seq) 
             const LevelInfo& C = s->lv[q.l - 1];
             const double sc = level_scale(s, L);
             ProlongItem<float> p;
-            p.g = gref_point(L, s1);
+            p.g = gref_transfer(L, C.mx, s1);
             s1 += gblocks(p.g);
             p.mxc = C.mx; p.myc = C.my;
             p.xb = bvec(q.r, q.l);
@@ -410,7 +411,7 @@ static int req_admit(p4g_server_s* v, SrvReq* r) {
     k_fill<float><<<gb, 256, 0, st>>>(n, 0.0f, r->pa);
     k_convert<double, float><<<gb, 256, 0, st>>>(n, r->b64, r->r32);
     k_dot2<double><<<gb, 256, 0, st>>>(n, r->b64, r->b64, r->part);
-    k_finalize<<<1, 256, 0, st>>>(gb, r->part, r->S + 16, FIN_RAW, 0);
+    k_finalize<<<1, FT, 0, st>>>(gb, r->part, r->S + 16, FIN_RAW, 0);
     LAUNCH_CHECK();
     r->xcur = r->xa;
     r->xnext = r->xb;

@@ -145,7 +145,7 @@ struct Ctx {
     }
     int finalize(long nparts, double* Sblk, int mode, int first) {
         return launch(P4G_K_REDUCE, false, 16.0 * nparts, [&] {
-            k_finalize<<<1, 256, 0, stream>>>(nparts, part, Sblk, mode, first);
+            k_finalize<<<1, FT, 0, stream>>>(nparts, part, Sblk, mode, first);
         });
     }
     int readS(double* host) {
@@ -157,18 +157,19 @@ struct Ctx {
 
 static inline int vblocks(long n) { return (int)std::max(1L, std::min<long>(VBLOCKS, (n + 255) / 256)); }
 
-// launch geometry of the marching stencil kernels
+// launch geometry of the marching stencil kernels (2D/3D: two x points per lane)
 static dim3 stencil_grid(int dim, int mx, int my, int mz) {
-    const int gx = (mx - 1 + TX - 1) / TX;
+    if (dim == 1) return dim3((mx - 1 + TX - 1) / TX, 1, 1);
+    const int gx = (mx - 1 + 2 * TX - 1) / (2 * TX);
     if (dim == 3) return dim3(gx, (my + TY - 1) / TY, (mz + kchf(dim, my, mz) - 1) / kchf(dim, my, mz));
-    if (dim == 2) return dim3(gx, ((my + kchf(dim, my, mz) - 1) / kchf(dim, my, mz) + TY - 1) / TY, 1);
-    return dim3(gx, 1, 1);
+    return dim3(gx, ((my + kchf(dim, my, mz) - 1) / kchf(dim, my, mz) + TY - 1) / TY, 1);
 }
 static dim3 stencil_block(int dim) { return dim == 1 ? dim3(TX, 1, 1) : dim3(TX, TY, 1); }
 static long nblocks(dim3 g) { return (long)g.x * g.y * g.z; }
-// grid covering all points with lanes from i = 0 (transfers)
-static dim3 point_grid(int dim, int mx, int my, int mz) {
-    const int gx = (mx + TX - 1) / TX;
+// transfers: lanes over coarse columns I = 1..mxc-1, rows and march chunks
+// over the grid (my, mz) the kernel walks (coarse: restriction, fine: prolongation)
+static dim3 transfer_grid(int dim, int mxc, int my, int mz) {
+    const int gx = std::max(1, (mxc - 1 + TX - 1) / TX);
     if (dim == 3) return dim3(gx, (my + TY - 1) / TY, (mz + kchf(dim, my, mz) - 1) / kchf(dim, my, mz));
     if (dim == 2) return dim3(gx, ((my + kchf(dim, my, mz) - 1) / kchf(dim, my, mz) + TY - 1) / TY, 1);
     return dim3(gx, 1, 1);
@@ -305,7 +306,7 @@ struct Engine {
     int restrict_(int l, const T* rf, T* bc) {   // level l -> l-1
         const LevelInfo& F = (*lv)[l];
         const LevelInfo& C = (*lv)[l - 1];
-        const dim3 g = point_grid(C.dim, C.mx, C.my, C.mz), b = stencil_block(C.dim);
+        const dim3 g = transfer_grid(C.dim, C.mx, C.my, C.mz), b = stencil_block(C.dim);
         hipStream_t s = cx->stream;
         return cx->launch(P4G_K_RESTRICT, l == nlev - 1, double(F.n + C.n) * sizeof(T), [&] {
             if (F.dim == 3)
@@ -321,7 +322,7 @@ struct Engine {
     int prolong(int l, bool zg, const T* base, T c, const T* xc, T* out) {
         const LevelInfo& F = (*lv)[l];
         const LevelInfo& C = (*lv)[l - 1];
-        const dim3 g = point_grid(F.dim, F.mx, F.my, F.mz), b = stencil_block(F.dim);
+        const dim3 g = transfer_grid(F.dim, C.mx, F.my, F.mz), b = stencil_block(F.dim);
         hipStream_t s = cx->stream;
         return cx->launch(P4G_K_PROLONG, l == nlev - 1, double(2 * F.n + C.n) * sizeof(T), [&] {
             if (F.dim == 3) {
@@ -679,6 +680,10 @@ struct OutIdiag {   // stores the Jacobi factor of every row
     __device__ __forceinline__ void point(long q, double, double, double idg, double&, double&) const {
         id[q] = idg;
     }
+    __device__ __forceinline__ void point2(long q, double, double, double, double, double ia, double ib,
+                                           double&, double&) const {
+        st2(id + q, ia, ib);
+    }
 };
 
 }  // namespace p4g
@@ -875,7 +880,7 @@ static int estimate_level(p4g_solver_s* s, int l) {
     k_fill<double><<<gb, 256, 0, st>>>(n, 0.0, w);
     k_lanczos_update<<<gb, 256, 0, st>>>(n, 0.0, r, w, idv, idc, z, s->cx.part);
     LAUNCH_CHECK();
-    k_finalize<<<1, 256, 0, st>>>(gb, s->cx.part, S, FIN_RAW, 0);
+    k_finalize<<<1, FT, 0, st>>>(gb, s->cx.part, S, FIN_RAW, 0);
     P4G_TRY(s->cx.readS(s->cx.hS));
     double rz = s->cx.hS[16 + S_TMP0], rzold = 0.0;
     double al[16], be[16];
@@ -902,7 +907,7 @@ static int estimate_level(p4g_solver_s* s, int l) {
                 L.mx, L.my, L.mz, kchf(L.dim, L.my, L.mz), oc, SrcDir<double>{z, pold, pnew, S + S_BCOEF}, OutApplyDot<double>{w}, s->cx.part);
         LAUNCH_CHECK();
         std::swap(pold, pnew);
-        k_finalize<<<1, 256, 0, st>>>(nbs, s->cx.part, S, FIN_RAW, 0);
+        k_finalize<<<1, FT, 0, st>>>(nbs, s->cx.part, S, FIN_RAW, 0);
         P4G_TRY(s->cx.readS(s->cx.hS));
         const double pw = s->cx.hS[16 + S_TMP0];
         if (!(pw > 0.0)) break;
@@ -911,7 +916,7 @@ static int estimate_level(p4g_solver_s* s, int l) {
         m = it + 1;
         k_lanczos_update<<<gb, 256, 0, st>>>(n, alpha, r, w, idv, idc, z, s->cx.part);
         LAUNCH_CHECK();
-        k_finalize<<<1, 256, 0, st>>>(gb, s->cx.part, S, FIN_RAW, 0);
+        k_finalize<<<1, FT, 0, st>>>(gb, s->cx.part, S, FIN_RAW, 0);
         P4G_TRY(s->cx.readS(s->cx.hS));
         rzold = rz;
         rz = s->cx.hS[16 + S_TMP0];
