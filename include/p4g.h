@@ -101,13 +101,19 @@ int p4g_da_destroy(p4g_DA *da);
  *   -mg_levels_ksp_max_it n, -mg_levels_ksp_richardson_scale s,
  *   -mg_levels_ksp_chebyshev_esteig_steps n,
  *   -mg_coarse_ksp_type preonly, -mg_coarse_pc_type lu|cholesky|redundant,
- *   -p4g_precision double|mixed  (mixed: PCG with FP32 vectors and an FP32
- *       V-cycle, FP64 dot accumulation and coarsest Cholesky, and FP64
- *       residual replacement: the FP32 correction is folded into an FP64
- *       iterate and b - A x recomputed in FP64 whenever the iterated residual
- *       has fallen by -p4g_reliable_delta (default 1e-2); converged when the
- *       FP64 |b-Ax|_2 <= max(rtol |b|_2, atol)),
- *   -p4g_reliable_delta d.
+ *   -p4g_precision double|mixed  (mixed: PCG with FP32 direction, residual
+ *       and preconditioner vectors and an FP32 V-cycle, an FP64 iterate
+ *       (x += alpha p in FP64), FP64 dot accumulation and coarsest solve, and
+ *       FP64 residual replacement: b - A x is recomputed in FP64 whenever the
+ *       iterated residual has fallen by -p4g_reliable_delta (default 0.1);
+ *       converged when the FP64 |b-Ax|_2 <= max(rtol |b|_2, atol)),
+ *   -p4g_reliable_delta d,
+ *   -p4g_fp32_diff_form 1|0 (default 1): FP32 interior rows evaluated as
+ *       sums of differences s (u_c - u_nb), the same row when
+ *       d = 2(sx+sy+sz), with far less rounding on smooth vectors,
+ *   -p4g_mixed_x64 1|0 (default 1): 0 keeps the correction since the last
+ *       replacement in FP32 and folds it into the FP64 iterate at each
+ *       replacement.
  * Options of PETSc objects the device path does not implement (-pc_type
  * ilu|sor|lu, -mg_levels_pc_type sor, -pc_mg_galerkin, ...) are rejected
  * with PETSC_ERR_SUP. */

@@ -144,46 +144,43 @@ k_prolong_zg_batch(const ProlongItem<T>* __restrict__ items, int nitems) {
 template <typename T>
 struct CoarseItem {
     int n;
-    const double* Linv;
+    const double* Ainv;    // dense inverse of the coarsest operator
     const T* b;
     T* x;
 };
 
-// one block per item; dynamic LDS of 2 * max n doubles
+// one block of FT threads per item
 template <typename T>
-__global__ void __launch_bounds__(1024)
+__global__ void __launch_bounds__(FT)
 k_coarse_batch(const CoarseItem<T>* __restrict__ items) {
-    extern __shared__ double sh[];
     const CoarseItem<T>& it = items[blockIdx.x];
-    coarse_body<T>(it.n, it.Linv, it.b, it.x, sh);
+    coarse_rows<T>(it.n, it.Ainv, it.b, it.x, threadIdx.x >> 6, FT / 64);
 }
 
 // ---------------------------------------------------------- vector kernels
 
-// x = keep x + a p ; r -= a w ; acc0 += r.r  (a = S[S_ALPHA], keep = *keep)
-template <typename T>
+// x += a p in FP64 ; r -= a w in FP32 ; acc0 += r.r  (a = S[S_ALPHA])
 struct UpdateItem {
     GridRef g;             // only start and gx (= number of blocks) are used
     long n;
     const double* S;
-    const double* keep;
-    T *x, *r;
-    const T *p, *w;
+    double* x;
+    float* r;
+    const float *p, *w;
     double* part;
 };
 
-template <typename T>
 __global__ void __launch_bounds__(256)
-k_update_batch(const UpdateItem<T>* __restrict__ items, int nitems) {
+k_update_batch(const UpdateItem* __restrict__ items, int nitems) {
     const int t = find_item(items, nitems, (long)blockIdx.x);
-    const UpdateItem<T>& it = items[t];
+    const UpdateItem& it = items[t];
     const long lb = blockIdx.x - it.g.start;
-    const T a = T(it.S[S_ALPHA]);
-    const bool keep = *it.keep != 0.0;
+    const double a = it.S[S_ALPHA];
+    const float af = float(a);
     double acc = 0.0;
     for (long q = lb * 256 + threadIdx.x; q < it.n; q += (long)it.g.gx * 256) {
-        it.x[q] = keep ? it.x[q] + a * it.p[q] : a * it.p[q];
-        const T rv = it.r[q] - a * it.w[q];
+        it.x[q] += a * double(it.p[q]);
+        const float rv = it.r[q] - af * it.w[q];
         it.r[q] = rv;
         acc += double(rv) * double(rv);
     }

@@ -55,16 +55,23 @@ __device__ __forceinline__ void st2(T* __restrict__ p, T a, T b) {
 // ------------------------------------------------------------- operators
 
 // Constant-coefficient stencil.
+// With diff = 1 an interior row is evaluated as the sum of scaled
+// differences sx((uc-w)+(uc-e)) + sy(..) + sz(..), which equals
+// d uc - sx(w+e) - sy(s+n) - sz(dn+up) when d = 2(sx+sy+sz) (eliminated
+// boundary neighbours enter as 0) and rounds far less for smooth u: the
+// differences of close values are exact, where d uc - sum cancels.
 template <typename T>
 struct OpConst {
     T d, db, sx, sy, sz;   // interior diagonal, boundary diagonal, couplings
     T id, idb;             // Jacobi: 1/d, 1/db (or 1 when no Jacobi scaling)
+    int diff = 0;          // 1: difference form (requires d = 2(sx+sy+sz))
     __device__ __forceinline__ void bnd(int, int, int, T u, T& Au, T& idg) const {
         Au = db * u; idg = idb;
     }
     __device__ __forceinline__ void interior(int, int, int, T uc, T w, T e, T s, T n, T dn,
                                              T up, T& Au, T& idg) const {
-        Au = d * uc - sx * (w + e) - sy * (s + n) - sz * (dn + up);
+        if (diff) Au = sx * ((uc - w) + (uc - e)) + sy * ((uc - s) + (uc - n)) + sz * ((uc - dn) + (uc - up));
+        else Au = d * uc - sx * (w + e) - sy * (s + n) - sz * (dn + up);
         idg = id;
     }
 };
@@ -646,34 +653,28 @@ k_prolong(int mxf, int myf, int mzf, int mxc, int myc, int kch, const T* __restr
 
 // ------------------------------------------------------- coarse solve
 
-// x = L^-T (L^-1 b) with Linv = L^-1 (n x n, row major, lower triangular)
-// of the Cholesky factor of the coarsest operator, FP64.  One block.
+// x = Ainv b, Ainv = A_0^-1 (n x n, row major, FP64): the coarsest-grid
+// direct solve as one dense product.  A wave per row, lanes over columns
+// (coalesced), fixed-order wave sums; rows row0, row0 + rstride, ...
+// (row0 uniform per wave).
 template <typename T>
-__device__ __forceinline__ void coarse_body(int n, const double* __restrict__ Linv, const T* __restrict__ b,
-                                            T* __restrict__ x, double* __restrict__ sh) {
-    double* bb = sh;
-    double* yy = sh + n;
-    for (int q = threadIdx.x; q < n; q += blockDim.x) bb[q] = double(b[q]);
-    __syncthreads();
-    for (int r = threadIdx.x; r < n; r += blockDim.x) {
-        const double* L = Linv + (long)r * n;
+__device__ __forceinline__ void coarse_rows(int n, const double* __restrict__ Ainv, const T* __restrict__ b,
+                                            T* __restrict__ x, int row0, int rstride) {
+    const int lane = threadIdx.x & 63;
+    for (int r = row0; r < n; r += rstride) {
+        const double* __restrict__ A = Ainv + (long)r * n;
         double s = 0.0;
-        for (int c = 0; c <= r; ++c) s += L[c] * bb[c];
-        yy[r] = s;
-    }
-    __syncthreads();
-    for (int r = threadIdx.x; r < n; r += blockDim.x) {
-        double s = 0.0;
-        for (int c = r; c < n; ++c) s += Linv[(long)c * n + r] * yy[c];
-        x[r] = T(s);
+        for (int c = lane; c < n; c += 64) s += A[c] * double(b[c]);
+        s = wave_sum(s);
+        if (lane == 0) x[r] = T(s);
     }
 }
 
+// launch: <<<ceil(n/4), 256>>>
 template <typename T>
-__global__ void __launch_bounds__(1024)
-k_coarse_chol(int n, const double* __restrict__ Linv, const T* __restrict__ b, T* __restrict__ x) {
-    extern __shared__ double sh[];
-    coarse_body<T>(n, Linv, b, x, sh);
+__global__ void __launch_bounds__(256)
+k_coarse(int n, const double* __restrict__ Ainv, const T* __restrict__ b, T* __restrict__ x) {
+    coarse_rows<T>(n, Ainv, b, x, blockIdx.x * 4 + (threadIdx.x >> 6), gridDim.x * 4);
 }
 
 // ------------------------------------------------------- vector kernels
@@ -748,6 +749,30 @@ k_dot2(long n, const T* __restrict__ x, const T* __restrict__ y, double* __restr
     if (threadIdx.x == 0) {
         part[2 * blockIdx.x] = sm[0][0] + sm[0][1] + sm[0][2] + sm[0][3];
         part[2 * blockIdx.x + 1] = sm[1][0] + sm[1][1] + sm[1][2] + sm[1][3];
+    }
+}
+
+// x += a p in FP64 (x FP64, p FP32); r -= a w in FP32; acc0 += r.r
+__global__ void __launch_bounds__(256)
+k_cg_update_x64(long n, const double* __restrict__ S, int ialpha, double* __restrict__ x,
+                const float* __restrict__ p, float* __restrict__ r, const float* __restrict__ w,
+                double* __restrict__ part) {
+    const double a = S[ialpha];
+    const float af = float(a);
+    double acc = 0.0;
+    for (long q = blockIdx.x * 256L + threadIdx.x; q < n; q += (long)gridDim.x * 256) {
+        x[q] += a * double(p[q]);
+        const float rv = r[q] - af * w[q];
+        r[q] = rv;
+        acc += double(rv) * double(rv);
+    }
+    __shared__ double sm[4];
+    acc = wave_sum(acc);
+    if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = sm[0] + sm[1] + sm[2] + sm[3];
+        part[2 * blockIdx.x + 1] = 0.0;
     }
 }
 

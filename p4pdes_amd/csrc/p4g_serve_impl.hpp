@@ -3,7 +3,7 @@
 //
 // A worker thread owns one HIP stream.  Each round advances every active
 // request by one PCG iteration of the mixed-precision solver (FP32 vectors
-// and V-cycle, FP64 residual replacement; the same algorithm as
+// and V-cycle, FP64 iterate and residual replacement; the same algorithm as
 // -p4g_precision mixed): one ragged-batched launch per phase covers all
 // requests, and the V-cycle batches equal-sized levels of all requests
 // (descending by grid size, coarsest solves, ascending).  The V-cycle launch
@@ -39,8 +39,8 @@ struct SrvReq {
     long n = 0;
     size_t bytes = 0;
     int slot = -1;
-    double *b64 = nullptr, *xa = nullptr, *xb = nullptr;
-    float *r32 = nullptr, *z32 = nullptr, *pa = nullptr, *pb = nullptr, *w32 = nullptr, *x32 = nullptr;
+    double *b64 = nullptr, *x64 = nullptr;
+    float *r32 = nullptr, *z32 = nullptr, *pa = nullptr, *pb = nullptr, *w32 = nullptr;
     std::vector<SrvLevel> lv;
     double* part = nullptr;
     long npart = 0;
@@ -48,9 +48,8 @@ struct SrvReq {
     int state = RQ_QUEUED;
     double t_submit = 0, t_admit = 0, t_done = 0;
     int its = 0, nrep = 0, reason = 0;
-    bool replace_now = false, replaced = false, folded = true, keep_zero = true;
+    bool replace_now = false, replaced = false;
     double bnorm = 0, target = 0, rr_last = 0, dpi_old = 0, rn = 0;
-    double *xcur = nullptr, *xnext = nullptr;
     float *pold = nullptr, *pnew = nullptr;
     std::string err;
     double* S = nullptr;   // device scalar block (32 doubles) of the slot
@@ -99,7 +98,7 @@ static double wall_ms() {
 static size_t req_bytes(p4g_solver_s* s) {
     const LevelInfo& F = s->lv.back();
     const long n = F.n;
-    size_t b = 3 * n * sizeof(double) + 8 * n * sizeof(float);
+    size_t b = 2 * n * sizeof(double) + 7 * n * sizeof(float);   // + fine-level r, t0
     for (int l = 0; l + 1 < s->nlev; ++l) b += (l > 0 ? 4 : 2) * s->lv[l].n * sizeof(float);
     const long nb = std::max<long>(nblocks(stencil_grid(F.dim, F.mx, F.my, F.mz)), vblocks(n));
     b += 2 * nb * sizeof(double);
@@ -130,6 +129,7 @@ static OpConst<float> op32(p4g_solver_s* s, const LevelInfo& L) {
     o.d = float(L.d); o.db = float(L.db); o.sx = float(L.sx); o.sy = float(L.sy); o.sz = float(L.sz);
     o.id = s->jacobi ? float(1.0 / L.d) : 1.0f;
     o.idb = s->jacobi ? float(1.0 / L.db) : 1.0f;
+    o.diff = s->fp32_diff && std::fabs(L.d - 2.0 * (L.sx + L.sy + L.sz)) <= 1e-12 * std::fabs(L.d) ? 1 : 0;
     return o;
 }
 
@@ -139,8 +139,8 @@ static OpConst<double> op64(const LevelInfo& L) {
     return o;
 }
 
-static const double* solver_linv(p4g_solver_s* s) {
-    return s->e32 ? s->e32->Linv : (s->e64 ? s->e64->Linv : nullptr);
+static const double* solver_ainv(p4g_solver_s* s) {
+    return s->e32 ? s->e32->Ainv : (s->e64 ? s->e64->Ainv : nullptr);
 }
 
 static GridRef gref_stencil(const LevelInfo& L, long start) {
@@ -181,13 +181,12 @@ struct BLaunch {
     size_t off = 0;        // byte offset of the items in the device buffer
     int nitems = 0;
     long blocks = 0;
-    int lds = 0;
 };
 
 enum { BK_DIR = 0, BK_FIN, BK_UPD, BK_REP, BK_RESID, BK_RESTRICT, BK_PROLONG, BK_SMOOTH, BK_COARSE };
 
 typedef StencilItem<OpConst<float>, SrcDir<float>, OutApplyDot<float>> DirItem;
-typedef StencilItem<OpConst<double>, SrcSum<double, float>, OutResidCvt<double, float>> RepItem;
+typedef StencilItem<OpConst<double>, SrcArr<double>, OutResidCvt<double, float>> RepItem;
 typedef StencilItem<OpConst<float>, SrcScaled<float>, OutResid<float, 0>> ResidItem;
 typedef StencilItem<OpConst<float>, SrcArr<float>, OutChebR<float>> SmoothItem;
 
@@ -202,8 +201,8 @@ static int blaunch(p4g_server_s* v, const BLaunch& L, const char* base) {
         else k_stencil_batch<float, 2, OpConst<float>, SrcDir<float>, OutApplyDot<float>><<<g, dim3(TX, TY), 0, st>>>((const DirItem*)items, L.nitems);
         break;
     case BK_REP:
-        if (L.dim == 3) k_stencil_batch<double, 3, OpConst<double>, SrcSum<double, float>, OutResidCvt<double, float>><<<g, dim3(TX, TY), 0, st>>>((const RepItem*)items, L.nitems);
-        else k_stencil_batch<double, 2, OpConst<double>, SrcSum<double, float>, OutResidCvt<double, float>><<<g, dim3(TX, TY), 0, st>>>((const RepItem*)items, L.nitems);
+        if (L.dim == 3) k_stencil_batch<double, 3, OpConst<double>, SrcArr<double>, OutResidCvt<double, float>><<<g, dim3(TX, TY), 0, st>>>((const RepItem*)items, L.nitems);
+        else k_stencil_batch<double, 2, OpConst<double>, SrcArr<double>, OutResidCvt<double, float>><<<g, dim3(TX, TY), 0, st>>>((const RepItem*)items, L.nitems);
         break;
     case BK_RESID:
         if (L.dim == 3) k_stencil_batch<float, 3, OpConst<float>, SrcScaled<float>, OutResid<float, 0>><<<g, dim3(TX, TY), 0, st>>>((const ResidItem*)items, L.nitems);
@@ -222,13 +221,13 @@ static int blaunch(p4g_server_s* v, const BLaunch& L, const char* base) {
         else k_prolong_zg_batch<float, 2><<<g, dim3(TX, TY), 0, st>>>((const ProlongItem<float>*)items, L.nitems);
         break;
     case BK_UPD:
-        k_update_batch<float><<<g, 256, 0, st>>>((const UpdateItem<float>*)items, L.nitems);
+        k_update_batch<<<g, 256, 0, st>>>((const UpdateItem*)items, L.nitems);
         break;
     case BK_FIN:
         k_finalize_batch<<<dim3((unsigned)L.nitems), FT, 0, st>>>((const FinItem*)items);
         break;
     case BK_COARSE:
-        k_coarse_batch<float><<<dim3((unsigned)L.nitems), 256, L.lds, st>>>((const CoarseItem<float>*)items);
+        k_coarse_batch<float><<<dim3((unsigned)L.nitems), FT, 0, st>>>((const CoarseItem<float>*)items);
         break;
     }
     LAUNCH_CHECK();
@@ -260,7 +259,6 @@ static int upload(p4g_server_s* v, const Packer& pk, char** dbuf, char** hbuf, s
 static int build_vcycle(p4g_server_s* v, Packer& pk, std::vector<BLaunch>& seq) {
     struct LvRef { SrvReq* r; int l; };
     std::map<std::pair<int, int>, std::vector<LvRef>, std::greater<std::pair<int, int>>> groups;
-    int maxn0 = 0;
     for (SrvReq* r : v->active)
         for (int l = 1; l < r->nlev; ++l) groups[{r->sol->lv[l].mx, r->dim}].push_back({r, l});
     auto bvec = [](SrvReq* r, int l) -> float* { return l == r->nlev - 1 ? r->r32 : r->lv[l].b; };
@@ -305,14 +303,12 @@ static int build_vcycle(p4g_server_s* v, Packer& pk, std::vector<BLaunch>& seq) 
         for (SrvReq* r : v->active) {
             CoarseItem<float> it;
             it.n = (int)r->sol->lv[0].n;
-            it.Linv = solver_linv(r->sol);
+            it.Ainv = solver_ainv(r->sol);
             it.b = r->lv[0].b;
             it.x = r->lv[0].x;
-            maxn0 = std::max(maxn0, it.n);
             ci.push_back(it);
         }
         BLaunch c; c.kind = BK_COARSE; c.nitems = (int)ci.size(); c.blocks = c.nitems;
-        c.lds = 2 * maxn0 * (int)sizeof(double);
         c.off = pk.put(ci.data(), ci.size() * sizeof(CoarseItem<float>));
         seq.push_back(c);
     }
@@ -379,9 +375,8 @@ static int req_admit(p4g_server_s* v, SrvReq* r) {
     r->dim = F.dim;
     r->n = n;
     P4G_TRY(req_alloc(r, (void**)&r->b64, n * 8));
-    P4G_TRY(req_alloc(r, (void**)&r->xa, n * 8));
-    P4G_TRY(req_alloc(r, (void**)&r->xb, n * 8));
-    float** f32[6] = {&r->r32, &r->z32, &r->pa, &r->pb, &r->w32, &r->x32};
+    P4G_TRY(req_alloc(r, (void**)&r->x64, n * 8));
+    float** f32[5] = {&r->r32, &r->z32, &r->pa, &r->pb, &r->w32};
     for (float** q : f32) P4G_TRY(req_alloc(r, (void**)q, n * 4));
     r->lv.assign(r->nlev, SrvLevel());
     for (int l = 0; l < r->nlev; ++l) {
@@ -407,14 +402,12 @@ static int req_admit(p4g_server_s* v, SrvReq* r) {
     k_fill<double><<<1, 256, 0, st>>>(1, 1.0, r->S + 16 + S_ONE);
     HIP_TRY(hipMemcpyAsync(r->b64, r->hb, n * 8, hipMemcpyHostToDevice, st));
     const int gb = vblocks(n);
-    k_fill<double><<<gb, 256, 0, st>>>(n, 0.0, r->xa);
+    k_fill<double><<<gb, 256, 0, st>>>(n, 0.0, r->x64);
     k_fill<float><<<gb, 256, 0, st>>>(n, 0.0f, r->pa);
     k_convert<double, float><<<gb, 256, 0, st>>>(n, r->b64, r->r32);
     k_dot2<double><<<gb, 256, 0, st>>>(n, r->b64, r->b64, r->part);
     k_finalize<<<1, FT, 0, st>>>(gb, r->part, r->S + 16, FIN_RAW, 0);
     LAUNCH_CHECK();
-    r->xcur = r->xa;
-    r->xnext = r->xb;
     r->pold = r->pa;
     r->pnew = r->pb;
     r->state = RQ_INIT;
@@ -430,7 +423,7 @@ static int round_once(p4g_server_s* v) {
     // per-round CG phases (their item tables change every round)
     std::vector<DirItem> dir3, dir2;
     std::vector<FinItem> fa, frr, fraw, fbeta;
-    std::vector<UpdateItem<float>> up;
+    std::vector<UpdateItem> up;
     std::vector<RepItem> rep3, rep2;
     long sd3 = 0, sd2 = 0, su = 0, sr3 = 0, sr2 = 0;
     for (SrvReq* r : v->active) {
@@ -450,20 +443,17 @@ static int round_once(p4g_server_s* v) {
             (r->dim == 3 ? dir3 : dir2).push_back(d);
             fa.push_back(FinItem{r->part, gblocks(d.g), r->S, FIN_ALPHA, 0, nullptr});
             std::swap(r->pold, r->pnew);   // pold: the current direction
-            UpdateItem<float> u;
+            UpdateItem u;
             u.g.start = su;
             u.g.gx = vblocks(r->n);
             u.g.gy = u.g.gz = 1;
             su += u.g.gx;
             u.n = r->n;
             u.S = r->S;
-            u.keep = r->S + (r->keep_zero ? S_ZERO : S_ONE);
-            u.x = r->x32; u.r = r->r32; u.p = r->pold; u.w = r->w32;
+            u.x = r->x64; u.r = r->r32; u.p = r->pold; u.w = r->w32;
             u.part = r->part;
             up.push_back(u);
             frr.push_back(FinItem{r->part, (long)u.g.gx, r->S, FIN_RR, 0, nullptr});
-            r->keep_zero = false;
-            r->folded = false;
             if (r->replace_now) {
                 RepItem q;
                 long& sr = r->dim == 3 ? sr3 : sr2;
@@ -472,13 +462,10 @@ static int round_once(p4g_server_s* v) {
                 q.part = r->part;
                 q.flag = nullptr;
                 q.op = op64(F);
-                q.src = SrcSum<double, float>{r->xcur, r->x32, r->xnext};
+                q.src = SrcArr<double>{r->x64};
                 q.out = OutResidCvt<double, float>{r->b64, r->r32};
                 (r->dim == 3 ? rep3 : rep2).push_back(q);
                 fraw.push_back(FinItem{r->part, gblocks(q.g), r->S + 16, FIN_RAW, 0, nullptr});
-                std::swap(r->xcur, r->xnext);
-                r->keep_zero = true;
-                r->folded = true;
                 r->replace_now = false;
                 r->replaced = true;
                 r->nrep++;
@@ -498,7 +485,7 @@ static int round_once(p4g_server_s* v) {
     add(BK_DIR, 3, dir3.data(), sizeof(DirItem), dir3.size(), sd3);
     add(BK_DIR, 2, dir2.data(), sizeof(DirItem), dir2.size(), sd2);
     add(BK_FIN, 0, fa.data(), sizeof(FinItem), fa.size(), (long)fa.size());
-    add(BK_UPD, 0, up.data(), sizeof(UpdateItem<float>), up.size(), su);
+    add(BK_UPD, 0, up.data(), sizeof(UpdateItem), up.size(), su);
     add(BK_FIN, 0, frr.data(), sizeof(FinItem), frr.size(), (long)frr.size());
     add(BK_REP, 3, rep3.data(), sizeof(RepItem), rep3.size(), sr3);
     add(BK_REP, 2, rep2.data(), sizeof(RepItem), rep2.size(), sr2);
@@ -554,7 +541,6 @@ static bool req_step(p4g_server_s* v, SrvReq* r) {
         r->rr_last = r->bnorm * r->bnorm;
         r->rn = r->bnorm;
         r->state = RQ_ITER;
-        r->keep_zero = true;
         if (r->bnorm <= s->atol) { r->reason = P4G_CONVERGED_ATOL; return true; }
         return false;
     }
@@ -577,11 +563,7 @@ static bool req_step(p4g_server_s* v, SrvReq* r) {
 
 static int req_finish(p4g_server_s* v, SrvReq* r) {
     hipStream_t st = v->st;
-    if (!r->folded) {
-        k_axpy_mixed<<<vblocks(r->n), 256, 0, st>>>(r->n, r->xcur, r->x32);
-        LAUNCH_CHECK();
-    }
-    HIP_TRY(hipMemcpyAsync(r->hx, r->xcur, r->n * 8, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(r->hx, r->x64, r->n * 8, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
     r->t_done = wall_ms();
     return 0;

@@ -184,12 +184,13 @@ struct Engine {
     std::vector<LevelInfo>* lv = nullptr;
     int nlev = 0, cycles = 1, smoother = SM_CHEB, nu = 1, jacobi = 1, pctype = PCT_MG;
     double rich_scale = 1.0;
+    bool diff_form = false;   // constant-coefficient rows in difference form
     struct Dev {
         T *x = nullptr, *b = nullptr, *r = nullptr, *t[4] = {nullptr, nullptr, nullptr, nullptr};
         T* kc = nullptr;
     };
     std::vector<Dev> dv;
-    double* Linv = nullptr;
+    double* Ainv = nullptr;   // dense inverse of the coarsest operator
     int n0 = 0;
     // Krylov vectors on the finest level
     T *r = nullptr, *z = nullptr, *pa = nullptr, *pb = nullptr, *w = nullptr, *xbuf = nullptr,
@@ -213,7 +214,7 @@ struct Engine {
         return !L.var && L.d == L.db && (smoother == SM_CHEB || smoother == SM_RICH);
     }
 
-    int setup(Ctx* c, std::vector<LevelInfo>* levels, const std::vector<double>& Linv_host,
+    int setup(Ctx* c, std::vector<LevelInfo>* levels, const std::vector<double>& Ainv_host,
               int n0_) {
         cx = c;
         lv = levels;
@@ -247,10 +248,10 @@ struct Engine {
         if (pctype == PCT_MG) {
             n0 = n0_;
             void* q = nullptr;
-            HIP_TRY(hipMalloc(&q, Linv_host.size() * sizeof(double)));
+            HIP_TRY(hipMalloc(&q, Ainv_host.size() * sizeof(double)));
             allocs.push_back(q);
-            Linv = (double*)q;
-            HIP_TRY(hipMemcpy(Linv, Linv_host.data(), Linv_host.size() * sizeof(double),
+            Ainv = (double*)q;
+            HIP_TRY(hipMemcpy(Ainv, Ainv_host.data(), Ainv_host.size() * sizeof(double),
                               hipMemcpyHostToDevice));
         }
         return 0;
@@ -260,6 +261,8 @@ struct Engine {
         OpConst<T> o;
         o.d = T(L.d); o.db = T(L.db); o.sx = T(L.sx); o.sy = T(L.sy); o.sz = T(L.sz);
         o.id = jac ? T(1.0 / L.d) : T(1); o.idb = jac ? T(1.0 / L.db) : T(1);
+        // difference form only where it is the same row to working precision
+        o.diff = diff_form && std::fabs(L.d - 2.0 * (L.sx + L.sy + L.sz)) <= 1e-12 * std::fabs(L.d) ? 1 : 0;
         return o;
     }
     OpVar<T> opv(int l, bool jac) const {
@@ -340,9 +343,8 @@ struct Engine {
 
     int coarse(const T* b, T* x) {
         hipStream_t s = cx->stream;
-        const int nt = std::min(1024, ((n0 + 63) / 64) * 64);
         return cx->launch(P4G_K_COARSE, nlev == 1, double(n0) * n0 * 8.0, [&] {
-            k_coarse_chol<T><<<1, nt, 2 * n0 * sizeof(double), s>>>(n0, Linv, b, x);
+            k_coarse<T><<<(n0 + 3) / 4, 256, 0, s>>>(n0, Ainv, b, x);
         });
     }
 
@@ -701,11 +703,13 @@ struct p4g_solver_s {
     int normtype = NT_PRECOND, maxit = 10000;
     double rtol = 1e-5, atol = 1e-50, dtol = 1e5;
     bool mixed = false;
-    double reliable_delta = 1e-2;
+    double reliable_delta = 1e-1;
+    bool fp32_diff = true;    // FP32 engine: difference-form rows
+    bool mixed_x64 = true;    // mixed: accumulate the iterate in FP64 every iteration
     // state
     bool ready = false;
     std::vector<LevelInfo> lv;
-    std::vector<double> Linv;
+    std::vector<double> Ainv;   // dense inverse of the coarsest operator (host)
     int n0 = 0;
     Ctx cx;
     std::unique_ptr<Engine<double>> e64;
@@ -786,7 +790,7 @@ static void host_row(const LevelInfo& L, int i, int j, int k, double& dg, double
 }
 
 // dense Cholesky of the coarsest operator and the inverse of its factor
-static int coarse_factor(const LevelInfo& L, std::vector<double>& Linv) {
+static int coarse_factor(const LevelInfo& L, std::vector<double>& Ainv) {
     const int n = (int)L.n;
     if (n > MAX_COARSE)
         return set_error(PETSC_ERR_SUP, "coarsest grid has %d points (> %d); use more multigrid levels",
@@ -821,7 +825,7 @@ static int coarse_factor(const LevelInfo& L, std::vector<double>& Linv) {
         }
     }
     // Linv = L^-1 by forward substitution on the identity
-    Linv.assign((size_t)n * n, 0.0);
+    std::vector<double> Linv((size_t)n * n, 0.0);
     for (int c = 0; c < n; ++c) {
         for (int r = c; r < n; ++r) {
             double v = (r == c) ? 1.0 : 0.0;
@@ -829,6 +833,19 @@ static int coarse_factor(const LevelInfo& L, std::vector<double>& Linv) {
             Linv[(size_t)r * n + c] = v / A[(size_t)r * n + r];
         }
     }
+    // A^-1 = L^-T L^-1 (symmetric): entry (r,c) sums k >= max(r,c)
+    Ainv.assign((size_t)n * n, 0.0);
+    for (int k = 0; k < n; ++k) {               // rows of L^-1 are contiguous
+        const double* Lk = &Linv[(size_t)k * n];
+        for (int r = 0; r <= k; ++r) {
+            const double a = Lk[r];
+            if (a == 0.0) continue;
+            double* Ar = &Ainv[(size_t)r * n];
+            for (int c = 0; c <= r; ++c) Ar[c] += a * Lk[c];
+        }
+    }
+    for (int r = 0; r < n; ++r)
+        for (int c = 0; c < r; ++c) Ainv[(size_t)c * n + r] = Ainv[(size_t)r * n + c];
     return 0;
 }
 
@@ -990,7 +1007,9 @@ static int configure(p4g_solver_s* s) {
     const std::string pr = o.gets("p4g_precision", "double");
     if (pr == "mixed") s->mixed = true;
     else if (pr != "double") return set_error(PETSC_ERR_ARG_WRONG, "-p4g_precision %s", pr.c_str());
-    s->reliable_delta = o.getr("p4g_reliable_delta", 1e-2);
+    s->reliable_delta = o.getr("p4g_reliable_delta", 1e-1);
+    s->fp32_diff = o.getb("p4g_fp32_diff_form", true);
+    s->mixed_x64 = o.getb("p4g_mixed_x64", true);
     if (!(s->reliable_delta > 0.0 && s->reliable_delta < 1.0))
         return set_error(PETSC_ERR_ARG_OUTOFRANGE, "-p4g_reliable_delta must be in (0,1)");
     if (s->mixed && s->pctype != PCT_MG) return set_error(PETSC_ERR_SUP, "-p4g_precision mixed needs -pc_type mg");
@@ -1103,7 +1122,7 @@ int solver_setup(p4g_solver_s* s) {
         HIP_TRY(hipHostMalloc((void**)&s->cx.hS, 32 * sizeof(double), hipHostMallocDefault));
     }
     if (s->pctype == PCT_MG) {
-        P4G_TRY(coarse_factor(s->lv[0], s->Linv));
+        P4G_TRY(coarse_factor(s->lv[0], s->Ainv));
         s->n0 = (int)s->lv[0].n;
         if (s->smoother == SM_CHEB)
             for (int l = 1; l < s->nlev; ++l) P4G_TRY(estimate_level(s, l));
@@ -1120,7 +1139,8 @@ int solver_setup(p4g_solver_s* s) {
     if (s->mixed) {
         s->e32.reset(new Engine<float>);
         cfg(s->e32);
-        P4G_TRY(s->e32->setup(&s->cx, &s->lv, s->Linv, s->n0));
+        s->e32->diff_form = s->fp32_diff;
+        P4G_TRY(s->e32->setup(&s->cx, &s->lv, s->Ainv, s->n0));
         const long n = F.n;
         P4G_TRY(dalloc(s, (void**)&s->xa, n * sizeof(double)));
         P4G_TRY(dalloc(s, (void**)&s->xb, n * sizeof(double)));
@@ -1129,7 +1149,7 @@ int solver_setup(p4g_solver_s* s) {
     } else {
         s->e64.reset(new Engine<double>);
         cfg(s->e64);
-        P4G_TRY(s->e64->setup(&s->cx, &s->lv, s->Linv, s->n0));
+        P4G_TRY(s->e64->setup(&s->cx, &s->lv, s->Ainv, s->n0));
     }
     HIP_TRY(hipStreamSynchronize(s->cx.stream));
     s->ready = true;
@@ -1213,6 +1233,7 @@ static int mixed_solve(p4g_solver_s* s, const double* b, double* x) {
     double* xnext = s->xb;
     if (bnorm <= s->atol) R.reason = P4G_CONVERGED_ATOL;
     const double delta = s->reliable_delta;
+    const bool x64 = s->mixed_x64;    // FP64 iterate updated every iteration (no FP32 correction)
     double rr_last = bnorm * bnorm;   // |r|^2 at the last replacement
     bool replace_now = false, folded = true;
     int keep = S_ZERO;                // the first update sets x32 = alpha p
@@ -1228,21 +1249,32 @@ static int mixed_solve(p4g_solver_s* s, const double* b, double* x) {
                           P4G_K_STENCIL_DOT, 4, &nbs, false));
         P4G_TRY(cx->finalize(nbs, S, FIN_ALPHA, 0));
         std::swap(pold, pnew);
-        P4G_TRY(cx->launch(P4G_K_CG_UPDATE, true, 24.0 * n, [&] {
-            k_cg_update<float, 1><<<gb, 256, 0, st>>>(n, S, S_ALPHA, keep, x32, pold, r32, e.w, cx->part);
-        }));
+        if (x64) {
+            P4G_TRY(cx->launch(P4G_K_CG_UPDATE, true, 32.0 * n, [&] {
+                k_cg_update_x64<<<gb, 256, 0, st>>>(n, S, S_ALPHA, xcur, pold, r32, e.w, cx->part);
+            }));
+        } else {
+            P4G_TRY(cx->launch(P4G_K_CG_UPDATE, true, 24.0 * n, [&] {
+                k_cg_update<float, 1><<<gb, 256, 0, st>>>(n, S, S_ALPHA, keep, x32, pold, r32, e.w, cx->part);
+            }));
+            keep = S_ONE;
+            folded = false;
+        }
         P4G_TRY(cx->finalize(gb, S, FIN_RR, 0));
-        keep = S_ONE;
-        folded = false;
         const bool replaced = replace_now;
         if (replace_now) {
             long nbk = 0;
-            P4G_TRY(fine64(s, SrcSum<double, float>{xcur, x32, xnext}, OutResidCvt<double, float>{b, r32},
-                           P4G_K_OTHER, 36.0 * n, &nbk));
+            if (x64) {
+                P4G_TRY(fine64(s, SrcArr<double>{xcur}, OutResidCvt<double, float>{b, r32}, P4G_K_OTHER,
+                               20.0 * n, &nbk));
+            } else {
+                P4G_TRY(fine64(s, SrcSum<double, float>{xcur, x32, xnext}, OutResidCvt<double, float>{b, r32},
+                               P4G_K_OTHER, 36.0 * n, &nbk));
+                std::swap(xcur, xnext);
+                keep = S_ZERO;
+                folded = true;
+            }
             P4G_TRY(cx->finalize(nbk, So, FIN_RAW, 0));
-            std::swap(xcur, xnext);
-            keep = S_ZERO;
-            folded = true;
             replace_now = false;
             ++nrep;
         }
