@@ -24,7 +24,7 @@
 namespace p4g {
 
 constexpr int TX = 64;   // lanes along x: one wavefront
-constexpr int TY = 4;    // rows (3D) or march chunks (2D) per block
+constexpr int TY = 8;    // rows (3D) or march chunks (2D) per block
 constexpr int NT = TX * TY;
 
 // Two consecutive grid values, moved as one 2-element vector access.  The
@@ -339,8 +339,8 @@ __device__ __forceinline__ void block_partials(double a0, double a1, double* __r
 // 3D rows j+-1 are read through the cache.  Every lane of a wave runs the
 // same loop (loads are predicated) so the cross-lane exchanges see the whole
 // wave.  Grid:
-//   3D: (ceil((mx-1)/128), ceil(my/4), ceil(mz/kch)), block (64,4)
-//   2D: (ceil((mx-1)/128), ceil(ceil(my/kch)/4), 1),  block (64,4)
+//   3D: (ceil((mx-1)/128), ceil(my/TY), ceil(mz/kch)), block (64,TY)
+//   2D: (ceil((mx-1)/128), ceil(ceil(my/kch)/TY), 1),  block (64,TY)
 template <typename T, int DIM, class Op, class Src, class Out>
 __device__ __forceinline__ void stencil_pairs(int bx, int by, int bz, int mx, int my, int mz, int kch,
                                               const Op& op, const Src& src, const Out& out, double& a0,
@@ -487,7 +487,7 @@ k_stencil(int mx, int my, int mz, int kch, Op op, Src src, Out out, double* __re
 // owns I = 0) and marches K; on each fine row it reads the pair (2I-1, 2I)
 // and takes 2I+1 from the next lane.  The in-plane weighted sum of the fine
 // plane 2K+1 is reused as plane 2(K+1)-1.  Loops are uniform per wave.
-// Grid: (ceil((mxc-1)/64), ceil(myc/4) [3D] | ceil(ceil(myc/kch)/4) [2D] | 1,
+// Grid: (ceil((mxc-1)/64), ceil(myc/TY) [3D] | ceil(ceil(myc/kch)/TY) [2D] | 1,
 //        ceil(mzc/kch) [3D] | 1)
 template <typename T, int DIM>
 __device__ __forceinline__ void restrict_body(int bx, int by, int bz, int mxf, int myf, int mzf, int mxc,
@@ -573,7 +573,7 @@ k_restrict(int mxf, int myf, int mzf, int mxc, int myc, int mzc, int kch, const 
 // the previous lane.  Fine rows j (3D) march k; the coarse in-plane
 // interpolant of plane K is computed once and reused for fine planes 2K-1,
 // 2K, 2K+1.  Loops are uniform per wave.
-// Grid: (ceil((mxc-1)/64), ceil(myf/4) [3D] | ceil(ceil(myf/kch)/4) [2D] | 1,
+// Grid: (ceil((mxc-1)/64), ceil(myf/TY) [3D] | ceil(ceil(myf/kch)/TY) [2D] | 1,
 //        ceil(mzf/kch) [3D] | 1)
 template <typename T, int DIM, bool ZG>
 __device__ __forceinline__ void prolong_body(int bx, int by, int bz, int mxf, int myf, int mzf, int mxc,

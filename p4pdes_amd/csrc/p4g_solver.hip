@@ -54,9 +54,11 @@ enum { NT_PRECOND = 0, NT_UNPRECOND, NT_NATURAL };
 
 // planes per marching chunk: short chunks on small grids (latency-bound),
 // longer ones on large grids (fewer re-read halo planes)
+// marching chunk: one plane per thread on small grids (latency-bound, want
+// parallelism), longer chunks on large ones (fewer halo re-reads)
 static inline int kchf(int dim, int my, int mz) {
     const int slow = dim == 3 ? mz : (dim == 2 ? my : 1);
-    return slow >= 400 ? 8 : 4;
+    return slow >= 400 ? 8 : (slow >= 100 ? 4 : 1);
 }
 constexpr int VBLOCKS = 2048;      // blocks of the grid-stride vector kernels
 constexpr int MAX_COARSE = 2048;   // largest coarsest grid for the direct solve
