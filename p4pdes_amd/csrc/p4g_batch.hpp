@@ -117,26 +117,31 @@ k_restrict_batch(const RestrictItem<T>* __restrict__ items, int nitems) {
     restrict_body<T, DIM>(bx, by, bz, it.mxf, it.myf, it.mzf, it.g.mx, it.g.my, it.g.mz, it.g.kch, it.rf, it.bc);
 }
 
-template <typename T>
-struct ProlongItem {
-    GridRef g;             // fine grid
-    int mxc, myc;
-    const T* xb;
+// prolongation fused with the first smoothing step (zero-guess base c b)
+template <typename T, class Op, class Out>
+struct ProlongSmoothItem {
+    GridRef g;             // fine grid, stencil launch geometry
+    int mxc, myc;          // coarse grid
+    const T* b;            // base: t = c b + P x_c
     T c;
     const T* xc;
-    T* out;
+    double* part;
+    Op op;
+    Out out;
 };
 
-template <typename T, int DIM>
+template <typename T, int DIM, class Op, class Out>
 __global__ void __launch_bounds__(NT)
-k_prolong_zg_batch(const ProlongItem<T>* __restrict__ items, int nitems) {
+k_prolong_smooth_batch(const ProlongSmoothItem<T, Op, Out>* __restrict__ items, int nitems) {
     const int t = find_item(items, nitems, (long)blockIdx.x);
-    const ProlongItem<T>& it = items[t];
+    const ProlongSmoothItem<T, Op, Out>& it = items[t];
     int bx, by, bz;
     long lb;
     decode(it.g, blockIdx.x, bx, by, bz, lb);
-    prolong_body<T, DIM, true>(bx, by, bz, it.g.mx, it.g.my, it.g.mz, it.mxc, it.myc, it.g.kch, it.xb, it.c,
-                               it.xc, it.out);
+    double a0 = 0.0, a1 = 0.0;
+    prolong_smooth_body<T, DIM, Op, true, Out>(bx, by, bz, it.g.mx, it.g.my, it.g.mz, it.mxc, it.myc, it.g.kch,
+                                               it.op, it.b, it.c, it.xc, it.out, a0, a1);
+    block_partials<Out::NDOT>(a0, a1, it.part, lb);
 }
 
 // ------------------------------------------------------------- coarse solve

@@ -651,6 +651,217 @@ k_prolong(int mxf, int myf, int mzf, int mxc, int myc, int kch, const T* __restr
     prolong_body<T, DIM, ZG>(blockIdx.x, blockIdx.y, blockIdx.z, mxf, myf, mzf, mxc, myc, kch, xb, c, xc, out);
 }
 
+// ------------------------------------------- fused prolongation + smoothing
+
+// Values of one coarse row seen by a lane: w = coarse I-1, v = coarse I,
+// e = coarse I+1 (e is needed by the wave's last lane only).
+template <typename T>
+struct T3 {
+    T w, v, e;
+};
+
+template <typename T>
+__device__ __forceinline__ T3<T> mix3(const T3<T>& a, const T3<T>& b) {
+    const T h = T(0.5);
+    return T3<T>{h * (a.w + b.w), h * (a.v + b.v), h * (a.e + b.e)};
+}
+
+// Output wrapper that also stores the operand u (the prolonged iterate).
+template <typename T, class Out>
+struct OutStoreU {
+    T* __restrict__ t;
+    Out inner;
+    static constexpr int NDOT = Out::NDOT;
+    __device__ __forceinline__ void point(long q, T u, T Au, T idg, double& a0, double& a1) const {
+        t[q] = u;
+        inner.point(q, u, Au, idg, a0, a1);
+    }
+    __device__ __forceinline__ void point2(long q, T ua, T ub, T Aa, T Ab, T ia, T ib, double& a0,
+                                           double& a1) const {
+        st2(t + q, ua, ub);
+        inner.point2(q, ua, ub, Aa, Ab, ia, ib, a0, a1);
+    }
+};
+
+// One smoothing step applied to the prolonged iterate without storing it:
+//   t = base + P x_c   (base = c xb with ZG, else xb)
+//   Out(t, A t)        (e.g. y = t + s D^-1 (b - A t))
+// on a 2D/3D level of size (mx, my, mz) with coarse level (mxc, myc, .).
+// A lane owns coarse column I = 1 + 64 bx + lane and the fine pair
+// (2I-1, 2I) -- the lane mapping of stencil_pairs -- and the lane at I = 1
+// also fine 0.  The coarse values of I-1 and I+1 come from the neighbouring
+// lanes.  3D: a thread owns fine row j and marches fine planes k; t at rows
+// j-1, j, j+1 is interpolated from the coarse rows of the two coarse planes
+// k>>1 and (k>>1)+1, which stay in registers.  2D: a thread marches fine
+// rows.  Loops are uniform per wave.  Grid: that of stencil_pairs.
+template <typename T, int DIM, class Op, bool ZG, class Out>
+__device__ __forceinline__ void prolong_smooth_body(int bx, int by, int bz, int mx, int my, int mz, int mxc,
+                                                    int myc, int kch, const Op& op, const T* __restrict__ xb,
+                                                    T c, const T* __restrict__ xc, const Out& out, double& a0,
+                                                    double& a1) {
+    const int lane = threadIdx.x;
+    const int I = 1 + bx * TX + lane;
+    const bool act = I < mxc;
+    const int i0 = 2 * I - 1, i1 = 2 * I;
+    const T h = T(0.5), z0 = T(0);
+    const T3<T> zero3{z0, z0, z0};
+    const T2<T> zero2{z0, z0};
+    auto crow = [&](long cb) -> T3<T> {
+        const T v = act ? xc[cb + I] : z0;
+        T w = __shfl_up(v, 1, TX);
+        T e = __shfl_down(v, 1, TX);
+        if (act && lane == 0) w = xc[cb + I - 1];
+        if (act && lane == TX - 1 && I + 1 < mxc) e = xc[cb + I + 1];
+        return T3<T>{w, v, e};
+    };
+    auto basev = [&](long q) -> T { return ZG ? c * xb[q] : xb[q]; };
+    auto tpair = [&](const T3<T>& P, long q) -> T2<T> {   // t at (i0, i1), address q of i0
+        const T2<T> bb = ld2(xb + q);
+        const T b0 = ZG ? c * bb.a : bb.a, b1 = ZG ? c * bb.b : bb.b;
+        return T2<T>{b0 + h * (P.w + P.v), b1 + P.v};
+    };
+    // the stencil and output at the pair of row j, march index m
+    auto emit = [&](long q, int j, int m, int M, bool rowb, const T3<T>& Pc, const T2<T>& tc, const T2<T>& ts,
+                    const T2<T>& tn, const T2<T>& td, const T2<T>& tu, T wv, T ev) {
+        const int jj = DIM == 3 ? j : m, kk = DIM == 3 ? m : 0;
+        if (lane == 0 && i0 - 1 > 0) wv = basev(q - 1) + Pc.w;
+        if (lane == TX - 1 && i1 + 1 < mx - 1) ev = basev(q + 2) + h * (Pc.v + Pc.e);
+        T Aa, Ab, ia, ib;
+        if (rowb || m == 0 || m == M - 1) {
+            op.bnd(i0, jj, kk, tc.a, Aa, ia);
+            op.bnd(i1, jj, kk, tc.b, Ab, ib);
+        } else {
+            op.interior(i0, jj, kk, tc.a, i0 - 1 > 0 ? wv : z0, i1 < mx - 1 ? tc.b : z0, ts.a, tn.a, td.a, tu.a,
+                        Aa, ia);
+            if (i1 == mx - 1) op.bnd(i1, jj, kk, tc.b, Ab, ib);
+            else op.interior(i1, jj, kk, tc.b, tc.a, i1 + 1 < mx - 1 ? ev : z0, ts.b, tn.b, td.b, tu.b, Ab, ib);
+        }
+        out.point2(q, tc.a, tc.b, Aa, Ab, ia, ib, a0, a1);
+        if (I == 1) {      // fine 0: a boundary point
+            const T t0 = basev(q - 1) + Pc.w;
+            op.bnd(0, jj, kk, t0, Aa, ia);
+            out.point(q - 1, t0, Aa, ia, a0, a1);
+        }
+    };
+    if (DIM == 2) {
+        const int m0 = (by * TY + threadIdx.y) * kch, m1 = min(m0 + kch, my);
+        if (m0 >= m1) return;
+        const long sm = mx;
+        int Ka = m0 >> 1;
+        T3<T> Ya = crow((long)Ka * mxc);
+        T3<T> Yb = (Ka + 1 < myc) ? crow((long)(Ka + 1) * mxc) : zero3;
+        long q = (long)m0 * sm + i0;
+        T3<T> Pc = (m0 & 1) ? mix3(Ya, Yb) : Ya;
+        T2<T> tc = act ? tpair(Pc, q) : zero2, td = zero2;
+        if (m0 > 0) {
+            const T3<T> Pd = (m0 & 1) ? Ya : mix3(crow((long)(Ka - 1) * mxc), Ya);
+            if (act) td = tpair(Pd, q - sm);
+        }
+        for (int m = m0; m < m1; ++m, q += sm) {
+            if ((m >> 1) != Ka) {
+                Ka = m >> 1;
+                Ya = Yb;
+                Yb = (Ka + 1 < myc) ? crow((long)(Ka + 1) * mxc) : zero3;
+            }
+            T3<T> Pu = zero3;
+            T2<T> tu = zero2;
+            if (m + 1 < my) {
+                Pu = (m & 1) ? Yb : mix3(Ya, Yb);
+                if (act) tu = tpair(Pu, q + sm);
+            }
+            const T wv = __shfl_up(tc.b, 1, TX), ev = __shfl_down(tc.a, 1, TX);
+            if (act)
+                emit(q, 0, m, my, false, Pc, tc, (m - 1 > 0) ? td : zero2, (m + 1 < my - 1) ? tu : zero2, zero2,
+                     zero2, wv, ev);
+            td = tc;
+            tc = tu;
+            Pc = Pu;
+        }
+        return;
+    }
+    // 3D
+    const int j = by * TY + threadIdx.y;
+    if (j >= my) return;
+    const int k0 = bz * kch, k1 = min(k0 + kch, mz);
+    if (k0 >= k1) return;
+    const int mzc = (mz - 1) / 2 + 1;
+    const long sy = mx, sz = (long)mx * my, scz = (long)mxc * myc;
+    // coarse values of fine rows j-1, j, j+1 on coarse plane K
+    auto yplane = [&](int K, T3<T>* Y) {
+        const long pb = (long)K * scz;
+        const int J = j >> 1;
+        if ((j & 1) == 0) {
+            const T3<T> R0 = crow(pb + (long)J * mxc);
+            const T3<T> Rm = (J >= 1) ? crow(pb + (long)(J - 1) * mxc) : zero3;
+            const T3<T> Rp = (J + 1 < myc) ? crow(pb + (long)(J + 1) * mxc) : zero3;
+            Y[0] = mix3(Rm, R0);
+            Y[1] = R0;
+            Y[2] = mix3(R0, Rp);
+        } else {
+            const T3<T> R0 = crow(pb + (long)J * mxc), R1 = crow(pb + (long)(J + 1) * mxc);
+            Y[0] = R0;
+            Y[1] = mix3(R0, R1);
+            Y[2] = R1;
+        }
+    };
+    int Ka = k0 >> 1;
+    T3<T> Ya[3], Yb[3];
+    yplane(Ka, Ya);
+    if (Ka + 1 < mzc) yplane(Ka + 1, Yb);
+    else Yb[0] = Yb[1] = Yb[2] = zero3;
+    long q = (long)k0 * sz + (long)j * sy + i0;
+    T3<T> Pc = (k0 & 1) ? mix3(Ya[1], Yb[1]) : Ya[1];
+    T2<T> tc = act ? tpair(Pc, q) : zero2, td = zero2;
+    if (k0 > 0) {
+        T3<T> Pd;
+        if (k0 & 1) {
+            Pd = Ya[1];
+        } else {
+            T3<T> Yp[3];
+            yplane(Ka - 1, Yp);
+            Pd = mix3(Yp[1], Ya[1]);
+        }
+        if (act) td = tpair(Pd, q - sz);
+    }
+    const bool jb = (j == 0) || (j == my - 1);
+    for (int k = k0; k < k1; ++k, q += sz) {
+        if ((k >> 1) != Ka) {
+            Ka = k >> 1;
+            Ya[0] = Yb[0]; Ya[1] = Yb[1]; Ya[2] = Yb[2];
+            if (Ka + 1 < mzc) yplane(Ka + 1, Yb);
+            else Yb[0] = Yb[1] = Yb[2] = zero3;
+        }
+        T3<T> Pu = zero3;
+        T2<T> tu = zero2;
+        if (k + 1 < mz) {
+            Pu = (k & 1) ? Yb[1] : mix3(Ya[1], Yb[1]);
+            if (act) tu = tpair(Pu, q + sz);
+        }
+        T2<T> ts = zero2, tn = zero2;
+        if (!jb && k != 0 && k != mz - 1) {
+            if (j - 1 > 0 && act) ts = tpair((k & 1) ? mix3(Ya[0], Yb[0]) : Ya[0], q - sy);
+            if (j + 1 < my - 1 && act) tn = tpair((k & 1) ? mix3(Ya[2], Yb[2]) : Ya[2], q + sy);
+        }
+        const T wv = __shfl_up(tc.b, 1, TX), ev = __shfl_down(tc.a, 1, TX);
+        if (act)
+            emit(q, j, k, mz, jb, Pc, tc, ts, tn, (k - 1 > 0) ? td : zero2, (k + 1 < mz - 1) ? tu : zero2, wv, ev);
+        td = tc;
+        tc = tu;
+        Pc = Pu;
+    }
+}
+
+template <typename T, int DIM, class Op, bool ZG, class Out>
+__global__ void __launch_bounds__(NT)
+k_prolong_smooth(int mx, int my, int mz, int mxc, int myc, int kch, Op op, const T* __restrict__ xb, T c,
+                 const T* __restrict__ xc, Out out, double* __restrict__ part) {
+    double a0 = 0.0, a1 = 0.0;
+    prolong_smooth_body<T, DIM, Op, ZG, Out>(blockIdx.x, blockIdx.y, blockIdx.z, mx, my, mz, mxc, myc, kch, op, xb,
+                                             c, xc, out, a0, a1);
+    const long bid = blockIdx.x + (long)gridDim.x * (blockIdx.y + (long)gridDim.y * blockIdx.z);
+    block_partials<Out::NDOT>(a0, a1, part, bid);
+}
+
 // ------------------------------------------------------- coarse solve
 
 // x = Ainv b, Ainv = A_0^-1 (n x n, row major, FP64): the coarsest-grid

@@ -25,7 +25,7 @@
 namespace p4g {
 
 struct SrvLevel {
-    float *b = nullptr, *x = nullptr, *r = nullptr, *t0 = nullptr;
+    float *b = nullptr, *x = nullptr, *r = nullptr;
 };
 
 enum { RQ_QUEUED = 0, RQ_INIT = 1, RQ_ITER = 2, RQ_DONE = 3, RQ_FAILED = -1 };
@@ -98,8 +98,8 @@ static double wall_ms() {
 static size_t req_bytes(p4g_solver_s* s) {
     const LevelInfo& F = s->lv.back();
     const long n = F.n;
-    size_t b = 2 * n * sizeof(double) + 7 * n * sizeof(float);   // + fine-level r, t0
-    for (int l = 0; l + 1 < s->nlev; ++l) b += (l > 0 ? 4 : 2) * s->lv[l].n * sizeof(float);
+    size_t b = 2 * n * sizeof(double) + 6 * n * sizeof(float);   // 5 Krylov vectors + fine-level r
+    for (int l = 0; l + 1 < s->nlev; ++l) b += (l > 0 ? 3 : 2) * s->lv[l].n * sizeof(float);
     const long nb = std::max<long>(nblocks(stencil_grid(F.dim, F.mx, F.my, F.mz)), vblocks(n));
     b += 2 * nb * sizeof(double);
     return b;
@@ -183,12 +183,12 @@ struct BLaunch {
     long blocks = 0;
 };
 
-enum { BK_DIR = 0, BK_FIN, BK_UPD, BK_REP, BK_RESID, BK_RESTRICT, BK_PROLONG, BK_SMOOTH, BK_COARSE };
+enum { BK_DIR = 0, BK_FIN, BK_UPD, BK_REP, BK_RESID, BK_RESTRICT, BK_PSMOOTH, BK_COARSE };
 
 typedef StencilItem<OpConst<float>, SrcDir<float>, OutApplyDot<float>> DirItem;
 typedef StencilItem<OpConst<double>, SrcArr<double>, OutResidCvt<double, float>> RepItem;
 typedef StencilItem<OpConst<float>, SrcScaled<float>, OutResid<float, 0>> ResidItem;
-typedef StencilItem<OpConst<float>, SrcArr<float>, OutChebR<float>> SmoothItem;
+typedef ProlongSmoothItem<float, OpConst<float>, OutChebR<float>> PSmoothItem;
 
 static int blaunch(p4g_server_s* v, const BLaunch& L, const char* base) {
     if (L.nitems == 0 || L.blocks == 0) return 0;
@@ -208,17 +208,13 @@ static int blaunch(p4g_server_s* v, const BLaunch& L, const char* base) {
         if (L.dim == 3) k_stencil_batch<float, 3, OpConst<float>, SrcScaled<float>, OutResid<float, 0>><<<g, dim3(TX, TY), 0, st>>>((const ResidItem*)items, L.nitems);
         else k_stencil_batch<float, 2, OpConst<float>, SrcScaled<float>, OutResid<float, 0>><<<g, dim3(TX, TY), 0, st>>>((const ResidItem*)items, L.nitems);
         break;
-    case BK_SMOOTH:
-        if (L.dim == 3) k_stencil_batch<float, 3, OpConst<float>, SrcArr<float>, OutChebR<float>><<<g, dim3(TX, TY), 0, st>>>((const SmoothItem*)items, L.nitems);
-        else k_stencil_batch<float, 2, OpConst<float>, SrcArr<float>, OutChebR<float>><<<g, dim3(TX, TY), 0, st>>>((const SmoothItem*)items, L.nitems);
+    case BK_PSMOOTH:
+        if (L.dim == 3) k_prolong_smooth_batch<float, 3, OpConst<float>, OutChebR<float>><<<g, dim3(TX, TY), 0, st>>>((const PSmoothItem*)items, L.nitems);
+        else k_prolong_smooth_batch<float, 2, OpConst<float>, OutChebR<float>><<<g, dim3(TX, TY), 0, st>>>((const PSmoothItem*)items, L.nitems);
         break;
     case BK_RESTRICT:
         if (L.dim == 3) k_restrict_batch<float, 3><<<g, dim3(TX, TY), 0, st>>>((const RestrictItem<float>*)items, L.nitems);
         else k_restrict_batch<float, 2><<<g, dim3(TX, TY), 0, st>>>((const RestrictItem<float>*)items, L.nitems);
-        break;
-    case BK_PROLONG:
-        if (L.dim == 3) k_prolong_zg_batch<float, 3><<<g, dim3(TX, TY), 0, st>>>((const ProlongItem<float>*)items, L.nitems);
-        else k_prolong_zg_batch<float, 2><<<g, dim3(TX, TY), 0, st>>>((const ProlongItem<float>*)items, L.nitems);
         break;
     case BK_UPD:
         k_update_batch<<<g, 256, 0, st>>>((const UpdateItem*)items, L.nitems);
@@ -312,43 +308,33 @@ static int build_vcycle(p4g_server_s* v, Packer& pk, std::vector<BLaunch>& seq) 
         c.off = pk.put(ci.data(), ci.size() * sizeof(CoarseItem<float>));
         seq.push_back(c);
     }
-    // ascent
+    // ascent: prolongation fused with the post-smoothing step
     for (auto it = groups.rbegin(); it != groups.rend(); ++it) {
         const int dim = it->first.second;
-        std::vector<ProlongItem<float>> pi;
-        std::vector<SmoothItem> si;
-        long s1 = 0, s2 = 0;
+        std::vector<PSmoothItem> pi;
+        long s1 = 0;
         for (const LvRef& q : it->second) {
             p4g_solver_s* s = q.r->sol;
             const LevelInfo& L = s->lv[q.l];
             const LevelInfo& C = s->lv[q.l - 1];
             const double sc = level_scale(s, L);
-            ProlongItem<float> p;
-            p.g = gref_transfer(L, C.mx, s1);
-            s1 += gblocks(p.g);
-            p.mxc = C.mx; p.myc = C.my;
-            p.xb = bvec(q.r, q.l);
-            p.c = float(sc * (s->jacobi ? 1.0 / L.d : 1.0));
-            p.xc = q.r->lv[q.l - 1].x;
-            p.out = q.r->lv[q.l].t0;
-            pi.push_back(p);
-            SmoothItem m;
-            m.g = gref_stencil(L, s2);
-            s2 += gblocks(m.g);
-            m.part = q.r->part;
-            m.flag = nullptr;
-            m.op = op32(s, L);
-            m.src = SrcArr<float>{q.r->lv[q.l].t0};
             const bool fin = q.l == q.r->nlev - 1;
+            PSmoothItem m;
+            m.g = gref_stencil(L, s1);
+            s1 += gblocks(m.g);
+            m.mxc = C.mx;
+            m.myc = C.my;
+            m.b = bvec(q.r, q.l);
+            m.c = float(sc * (s->jacobi ? 1.0 / L.d : 1.0));
+            m.xc = q.r->lv[q.l - 1].x;
+            m.part = q.r->part;
+            m.op = op32(s, L);
             m.out = OutChebR<float>{bvec(q.r, q.l), fin ? q.r->z32 : q.r->lv[q.l].x, 1.0f, float(sc), fin ? 1 : 0};
-            si.push_back(m);
+            pi.push_back(m);
         }
-        BLaunch a; a.kind = BK_PROLONG; a.dim = dim; a.nitems = (int)pi.size(); a.blocks = s1;
-        a.off = pk.put(pi.data(), pi.size() * sizeof(ProlongItem<float>));
+        BLaunch a; a.kind = BK_PSMOOTH; a.dim = dim; a.nitems = (int)pi.size(); a.blocks = s1;
+        a.off = pk.put(pi.data(), pi.size() * sizeof(PSmoothItem));
         seq.push_back(a);
-        BLaunch b; b.kind = BK_SMOOTH; b.dim = dim; b.nitems = (int)si.size(); b.blocks = s2;
-        b.off = pk.put(si.data(), si.size() * sizeof(SmoothItem));
-        seq.push_back(b);
     }
     return 0;
 }
@@ -385,10 +371,7 @@ static int req_admit(p4g_server_s* v, SrvReq* r) {
             P4G_TRY(req_alloc(r, (void**)&r->lv[l].b, nl * 4));
             P4G_TRY(req_alloc(r, (void**)&r->lv[l].x, nl * 4));
         }
-        if (l > 0) {
-            P4G_TRY(req_alloc(r, (void**)&r->lv[l].r, nl * 4));
-            P4G_TRY(req_alloc(r, (void**)&r->lv[l].t0, nl * 4));
-        }
+        if (l > 0) P4G_TRY(req_alloc(r, (void**)&r->lv[l].r, nl * 4));
     }
     r->npart = std::max<long>(nblocks(stencil_grid(F.dim, F.mx, F.my, F.mz)), vblocks(n));
     P4G_TRY(req_alloc(r, (void**)&r->part, 2 * r->npart * 8));

@@ -187,6 +187,7 @@ struct Engine {
     int nlev = 0, cycles = 1, smoother = SM_CHEB, nu = 1, jacobi = 1, pctype = PCT_MG;
     double rich_scale = 1.0;
     bool diff_form = false;   // constant-coefficient rows in difference form
+    bool fuse_post = true;    // first post-smoothing step fused with the prolongation
     struct Dev {
         T *x = nullptr, *b = nullptr, *r = nullptr, *t[4] = {nullptr, nullptr, nullptr, nullptr};
         T* kc = nullptr;
@@ -323,6 +324,55 @@ struct Engine {
         });
     }
 
+    // One smoothing step (Out) applied to t = (zg ? c base : base) + P x_c
+    // on level l without storing t (Out may store it, OutStoreU).
+    template <bool ZG, class Out>
+    int prolong_smooth(int l, const T* base, T cz, const T* xc, Out out, double passes, long* nb, bool jac) {
+        const LevelInfo& L = (*lv)[l];
+        const LevelInfo& C = (*lv)[l - 1];
+        const dim3 g = stencil_grid(L.dim, L.mx, L.my, L.mz), b = stencil_block(L.dim);
+        if (nb) *nb = nblocks(g);
+        if (Out::NDOT && nblocks(g) > cx->npart_cap)
+            return set_error(PETSC_ERR_LIB, "internal: partial-sum buffer too small");
+        hipStream_t s = cx->stream;
+        const double bytes = passes * L.n * sizeof(T) + C.n * sizeof(T);
+        const int kch = kchf(L.dim, L.my, L.mz);
+        if (L.var) {
+            if (L.dim != 3) return set_error(PETSC_ERR_SUP, "variable coefficient needs 3D");
+            OpVar<T> o = opv(l, jac);
+            return cx->launch(P4G_K_SMOOTH, l == nlev - 1, bytes, [&] {
+                k_prolong_smooth<T, 3, OpVar<T>, ZG, Out><<<g, b, 0, s>>>(L.mx, L.my, L.mz, C.mx, C.my, kch, o, base,
+                                                                         cz, xc, out, cx->part);
+            });
+        }
+        OpConst<T> o = opc(L, jac);
+        return cx->launch(P4G_K_SMOOTH, l == nlev - 1, bytes, [&] {
+            if (L.dim == 3)
+                k_prolong_smooth<T, 3, OpConst<T>, ZG, Out><<<g, b, 0, s>>>(L.mx, L.my, L.mz, C.mx, C.my, kch, o, base,
+                                                                           cz, xc, out, cx->part);
+            else
+                k_prolong_smooth<T, 2, OpConst<T>, ZG, Out><<<g, b, 0, s>>>(L.mx, L.my, L.mz, C.mx, C.my, kch, o, base,
+                                                                           cz, xc, out, cx->part);
+        });
+    }
+
+    // the first post-smoothing step fused with the prolongation (2D/3D):
+    // y = t + s D^-1 (b - A t), t = pre + P x_c; with tstore, t is also stored
+    template <bool DOTS>
+    int post_first(int l, bool zg, const T* pre, T cz, const T* xc, const T* b, T* tstore, T* y, T s2, long* nb) {
+        const bool jac = jacobi != 0;
+        typedef OutCheb<T, false, DOTS ? 2 : 0> OC;
+        const OC oc{b, nullptr, y, T(0), T(1), s2};
+        const double passes = (zg ? 2.0 : 3.0) + (tstore ? 1.0 : 0.0);
+        if (tstore) {
+            const OutStoreU<T, OC> os{tstore, oc};
+            return zg ? prolong_smooth<true>(l, pre, cz, xc, os, passes, nb, jac)
+                      : prolong_smooth<false>(l, pre, cz, xc, os, passes, nb, jac);
+        }
+        return zg ? prolong_smooth<true>(l, pre, cz, xc, oc, passes, nb, jac)
+                  : prolong_smooth<false>(l, pre, cz, xc, oc, passes, nb, jac);
+    }
+
     // out = (zg ? c b : base) + P x_c   (level l-1 -> l)
     int prolong(int l, bool zg, const T* base, T c, const T* xc, T* out) {
         const LevelInfo& F = (*lv)[l];
@@ -369,7 +419,10 @@ struct Engine {
     // Chebyshev / Richardson smoothing with Jacobi on level l:
     //   x0 = nullptr means a zero initial guess.  The result lands in `out`.
     //   DOTS: fuse (out, b) and (out, out) partials into the final step.
-    int smooth(int l, const T* b, const T* x0, T* out, bool dots, long* nb) {
+    // mstart = 2: continue after a first step done elsewhere; x0 = iterate 1,
+    // pm2init = iterate 0.
+    int smooth(int l, const T* b, const T* x0, T* out, bool dots, long* nb, int mstart = 1,
+               const T* pm2init = nullptr) {
         const LevelInfo& L = (*lv)[l];
         const bool jac = jacobi != 0;
         const double idg = jac ? 1.0 / L.d : 1.0;
@@ -385,7 +438,7 @@ struct Engine {
             s = rich_scale;
         }
         // buffers: p[m] holds iterate m; choose temps distinct from inputs
-        const T* pm2 = nullptr;          // iterate m-2
+        const T* pm2 = pm2init;          // iterate m-2
         const T* pm1 = x0;               // iterate m-1 (nullptr: zero)
         bool pm1_implicit = (x0 == nullptr);   // iterate 1 of a zero guess kept implicit (c b)
         int used = -1;
@@ -396,7 +449,7 @@ struct Engine {
             }
             return nullptr;
         };
-        for (int m = 1; m <= nu; ++m) {
+        for (int m = mstart; m <= nu; ++m) {
             const bool last = (m == nu);
             T* dst = last ? out : pick(pm1, pm2);
             if (!dst) return set_error(PETSC_ERR_LIB, "internal: smoother buffers exhausted");
@@ -509,7 +562,30 @@ struct Engine {
         for (int q = 0; q < cycles; ++q) P4G_TRY(mcycle(l - 1, c.b, c.x, q == 0, false, nullptr));
         // x = pre + P x_c, then post-smoothing into x
         T* x0 = zgf ? d.t[0] : xpre;
-        P4G_TRY(prolong(l, zgf, zgf ? b : xpre, T(s * idg), c.x, x0));
+        const T* pre = zgf ? b : xpre;
+        if (L.dim >= 2 && fuse_post) {
+            // the first post-smoothing step reads the prolonged iterate on the fly
+            if (nu == 1) {
+                if (dots) P4G_TRY(post_first<true>(l, zgf, pre, T(s * idg), c.x, b, nullptr, x, T(s), nb));
+                else P4G_TRY(post_first<false>(l, zgf, pre, T(s * idg), c.x, b, nullptr, x, T(s), nb));
+            } else {
+                // t (iterate 0 of the recurrence) and iterate 1 go to temporaries
+                // distinct from pre, which other threads still read
+                T* tk = nullptr;
+                T* y1 = nullptr;
+                for (int q = 0; q < 4; ++q) {
+                    T* cand = d.t[q];
+                    if (!cand || cand == x || cand == b || cand == xpre) continue;
+                    if (!tk) tk = cand;
+                    else if (!y1) y1 = cand;
+                }
+                if (!tk || !y1) return set_error(PETSC_ERR_LIB, "internal: smoother buffers exhausted");
+                P4G_TRY(post_first<false>(l, zgf, pre, T(s * idg), c.x, b, tk, y1, T(s), nullptr));
+                P4G_TRY(smooth(l, b, y1, x, dots, nb, 2, tk));
+            }
+            return 0;
+        }
+        P4G_TRY(prolong(l, zgf, pre, T(s * idg), c.x, x0));
         P4G_TRY(smooth(l, b, x0, x, dots, nb));
         return 0;
     }
