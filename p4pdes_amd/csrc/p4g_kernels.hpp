@@ -184,6 +184,78 @@ struct SrcDir {                       // p_new = z + beta p_old, stored (CG dire
     __device__ __forceinline__ void put2(long q, T a, T b) const { st2(pnew + q, a, b); }
 };
 
+// p_new = z + beta p_old, stored; at the own point also the deferred CG
+// iterate update x += alpha p_old (alpha of the previous iteration; X may be
+// wider than T).
+template <typename T, typename X>
+struct SrcDirX {
+    const T* __restrict__ z;
+    const T* __restrict__ pold;
+    T* __restrict__ pnew;
+    const double* __restrict__ beta;  // device scalar
+    X* __restrict__ x;
+    const double* __restrict__ alpha; // device scalar
+    __device__ __forceinline__ T load(long q) const { return z[q] + T(*beta) * pold[q]; }
+    __device__ __forceinline__ T2<T> load2(long q) const {
+        const T bt = T(*beta);
+        const T2<T> zz = ld2(z + q), pp = ld2(pold + q);
+        return T2<T>{zz.a + bt * pp.a, zz.b + bt * pp.b};
+    }
+    __device__ __forceinline__ void put(long q, T v) const {
+        pnew[q] = v;
+        x[q] += X(*alpha) * X(pold[q]);
+    }
+    __device__ __forceinline__ void put2(long q, T a, T b) const {
+        st2(pnew + q, a, b);
+        const X al = X(*alpha);
+        const T2<T> pp = ld2(pold + q);
+        const T2<X> xx = ld2(x + q);
+        st2(x + q, xx.a + al * X(pp.a), xx.b + al * X(pp.b));
+    }
+};
+
+// u = c (r - alpha w): the CG residual update fused into the first kernel
+// that reads the residual; the new residual r - alpha w is stored (put).
+template <typename T>
+struct SrcScaledUpd {
+    const T* __restrict__ r;
+    const T* __restrict__ w;
+    const double* __restrict__ alpha;  // device scalar
+    T c;
+    T* __restrict__ rn;
+    __device__ __forceinline__ T load(long q) const { return c * (r[q] - T(*alpha) * w[q]); }
+    __device__ __forceinline__ T2<T> load2(long q) const {
+        const T al = T(*alpha);
+        const T2<T> rr = ld2(r + q), ww = ld2(w + q);
+        return T2<T>{c * (rr.a - al * ww.a), c * (rr.b - al * ww.b)};
+    }
+    __device__ __forceinline__ void put(long q, T) const { rn[q] = r[q] - T(*alpha) * w[q]; }
+    __device__ __forceinline__ void put2(long q, T, T) const {
+        const T al = T(*alpha);
+        const T2<T> rr = ld2(r + q), ww = ld2(w + q);
+        st2(rn + q, rr.a - al * ww.a, rr.b - al * ww.b);
+    }
+};
+
+// u = x + alpha p (X wide, P narrow), stored to xnew: a pending iterate
+// update folded into an FP64 residual pass.
+template <typename X, typename P>
+struct SrcAxpyPut {
+    const X* __restrict__ x;
+    const P* __restrict__ p;
+    const double* __restrict__ alpha;  // device scalar
+    X* __restrict__ xnew;
+    __device__ __forceinline__ X load(long q) const { return x[q] + X(*alpha) * X(p[q]); }
+    __device__ __forceinline__ T2<X> load2(long q) const {
+        const X al = X(*alpha);
+        const T2<X> xx = ld2(x + q);
+        const T2<P> pp = ld2(p + q);
+        return T2<X>{xx.a + al * X(pp.a), xx.b + al * X(pp.b)};
+    }
+    __device__ __forceinline__ void put(long q, X v) const { xnew[q] = v; }
+    __device__ __forceinline__ void put2(long q, X a, X b) const { st2(xnew + q, a, b); }
+};
+
 template <typename T, typename E>
 struct SrcSum {                       // x_new = x + e (refinement update), stored
     const T* __restrict__ x;
@@ -258,6 +330,27 @@ struct OutResidCvt {                  // r = b - A u in T, stored as R ; acc0 +=
         const T va = bb.a - Aa, vb = bb.b - Ab;
         st2(r + q, R(va), R(vb));
         a0 += double(va) * double(va) + double(vb) * double(vb);
+    }
+};
+
+// Wraps an output without dot products and adds acc0 += rn.rn at every
+// point (rn: the residual stored by SrcScaledUpd in the same thread).
+template <typename T, class Out>
+struct OutRR {
+    const T* __restrict__ rn;
+    Out inner;
+    static_assert(Out::NDOT == 0, "OutRR needs an output without dot products");
+    static constexpr int NDOT = 1;
+    __device__ __forceinline__ void point(long q, T u, T Au, T idg, double& a0, double& a1) const {
+        inner.point(q, u, Au, idg, a0, a1);
+        const double v = double(rn[q]);
+        a0 += v * v;
+    }
+    __device__ __forceinline__ void point2(long q, T ua, T ub, T Aa, T Ab, T ia, T ib, double& a0,
+                                           double& a1) const {
+        inner.point2(q, ua, ub, Aa, Ab, ia, ib, a0, a1);
+        const T2<T> v = ld2(rn + q);
+        a0 += double(v.a) * double(v.a) + double(v.b) * double(v.b);
     }
 };
 
@@ -961,6 +1054,38 @@ k_dot2(long n, const T* __restrict__ x, const T* __restrict__ y, double* __restr
         part[2 * blockIdx.x] = sm[0][0] + sm[0][1] + sm[0][2] + sm[0][3];
         part[2 * blockIdx.x + 1] = sm[1][0] + sm[1][1] + sm[1][2] + sm[1][3];
     }
+}
+
+// rn = r - a w (a = S[ialpha]); NORM: acc0 += rn.rn
+template <typename T, int NORM>
+__global__ void __launch_bounds__(256)
+k_r_update(long n, const double* __restrict__ S, int ialpha, const T* __restrict__ r, const T* __restrict__ w,
+           T* __restrict__ rn, double* __restrict__ part) {
+    const T a = T(S[ialpha]);
+    double acc = 0.0;
+    for (long q = blockIdx.x * 256L + threadIdx.x; q < n; q += (long)gridDim.x * 256) {
+        const T v = r[q] - a * w[q];
+        rn[q] = v;
+        if (NORM) acc += double(v) * double(v);
+    }
+    if (NORM) {
+        __shared__ double sm[4];
+        acc = wave_sum(acc);
+        if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = acc;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            part[2 * blockIdx.x] = sm[0] + sm[1] + sm[2] + sm[3];
+            part[2 * blockIdx.x + 1] = 0.0;
+        }
+    }
+}
+
+// x += a p (X wide, P narrow; a = S[ialpha])
+template <typename X, typename P>
+__global__ void __launch_bounds__(256)
+k_x_update(long n, const double* __restrict__ S, int ialpha, X* __restrict__ x, const P* __restrict__ p) {
+    const X a = X(S[ialpha]);
+    for (long q = blockIdx.x * 256L + threadIdx.x; q < n; q += (long)gridDim.x * 256) x[q] += a * X(p[q]);
 }
 
 // x += a p in FP64 (x FP64, p FP32); r -= a w in FP32; acc0 += r.r

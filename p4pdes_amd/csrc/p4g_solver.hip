@@ -122,6 +122,7 @@ struct Timing {
 struct Ctx {
     hipStream_t stream = nullptr;
     double* part = nullptr;       // block partial sums (2 per block)
+    double* part_rr = nullptr;    // second buffer: (r,r) of a fused residual update
     long npart_cap = 0;
     double* S = nullptr;          // scalar block (32 doubles: engine 0..15, outer 16..31)
     double* hS = nullptr;         // pinned mirror
@@ -145,9 +146,10 @@ struct Ctx {
         LAUNCH_CHECK();
         return 0;
     }
-    int finalize(long nparts, double* Sblk, int mode, int first) {
+    int finalize(long nparts, double* Sblk, int mode, int first, const double* src = nullptr) {
+        const double* pp = src ? src : part;
         return launch(P4G_K_REDUCE, false, 16.0 * nparts, [&] {
-            k_finalize<<<1, FT, 0, stream>>>(nparts, part, Sblk, mode, first);
+            k_finalize<<<1, FT, 0, stream>>>(nparts, pp, Sblk, mode, first);
         });
     }
     int readS(double* host) {
@@ -197,7 +199,19 @@ struct Engine {
     int n0 = 0;
     // Krylov vectors on the finest level
     T *r = nullptr, *z = nullptr, *pa = nullptr, *pb = nullptr, *w = nullptr, *xbuf = nullptr,
-      *bbuf = nullptr;
+      *bbuf = nullptr, *r2 = nullptr;
+    // A CG residual update r_new = r - alpha w pending for the next
+    // preconditioner application: the first fine-level kernel that reads the
+    // residual forms r_new on the fly and stores it (rr: also (r_new, r_new)
+    // into cx->part_rr, nparts blocks).
+    struct Pend {
+        const T* r = nullptr;
+        const T* w = nullptr;
+        const double* alpha = nullptr;
+        T* rn = nullptr;
+        bool rr = false, on = false;
+        long nparts = 0;
+    } pend;
     std::vector<void*> allocs;
 
     ~Engine() { for (void* p : allocs) hipFree(p); }
@@ -248,6 +262,7 @@ struct Engine {
         P4G_TRY(alloc(&pb, F.n));
         P4G_TRY(alloc(&w, F.n));
         P4G_TRY(alloc(&xbuf, F.n));
+        P4G_TRY(alloc(&r2, F.n));
         if (pctype == PCT_MG) {
             n0 = n0_;
             void* q = nullptr;
@@ -278,13 +293,14 @@ struct Engine {
 
     // one marching stencil pass on level l
     template <class Src, class Out>
-    int stencil(int l, Src src, Out out, int cls, double passes, long* nb, bool jac) {
+    int stencil(int l, Src src, Out out, int cls, double passes, long* nb, bool jac, double* pbuf = nullptr) {
         const LevelInfo& L = (*lv)[l];
         const dim3 g = stencil_grid(L.dim, L.mx, L.my, L.mz), b = stencil_block(L.dim);
         if (nb) *nb = nblocks(g);
         if (Out::NDOT && nblocks(g) > cx->npart_cap)
             return set_error(PETSC_ERR_LIB, "internal: partial-sum buffer too small");
         hipStream_t s = cx->stream;
+        double* part = pbuf ? pbuf : cx->part;
         const bool fine = (l == nlev - 1);
         const double bytes = passes * L.n * sizeof(T);
         if (L.var) {
@@ -292,20 +308,20 @@ struct Engine {
             OpVar<T> o = opv(l, jac);
             return cx->launch(cls, fine, bytes, [&] {
                 k_stencil<T, 3, OpVar<T>, Src, Out><<<g, b, 0, s>>>(L.mx, L.my, L.mz, kchf(L.dim, L.my, L.mz), o, src, out,
-                                                                 cx->part);
+                                                                 part);
             });
         }
         OpConst<T> o = opc(L, jac);
         return cx->launch(cls, fine, bytes, [&] {
             if (L.dim == 3)
                 k_stencil<T, 3, OpConst<T>, Src, Out><<<g, b, 0, s>>>(L.mx, L.my, L.mz, kchf(L.dim, L.my, L.mz), o, src,
-                                                                   out, cx->part);
+                                                                   out, part);
             else if (L.dim == 2)
                 k_stencil<T, 2, OpConst<T>, Src, Out><<<g, b, 0, s>>>(L.mx, L.my, L.mz, kchf(L.dim, L.my, L.mz), o, src,
-                                                                   out, cx->part);
+                                                                   out, part);
             else
                 k_stencil<T, 1, OpConst<T>, Src, Out><<<g, b, 0, s>>>(L.mx, L.my, L.mz, kchf(L.dim, L.my, L.mz), o, src,
-                                                                   out, cx->part);
+                                                                   out, part);
         });
     }
 
@@ -494,12 +510,24 @@ struct Engine {
                 const T cb = T(s * idg);
                 // m == 2 from zero guess: prev iterate is 0, so no prev term
                 if (nu >= 3 && smoother == SM_CHEB) return set_error(PETSC_ERR_LIB, "internal smoother path");
-                if (last && dots)
+                if (fine && pend.on && !(last && dots)) {   // b = r - alpha w, formed here and stored to b
+                    const SrcScaledUpd<T> su{pend.r, pend.w, pend.alpha, cb, pend.rn};
+                    const OutCheb<T, false, 0> oc{b, nullptr, dst, T(c0), T(c1), T(c2)};
+                    pend.on = false;
+                    if (pend.rr)
+                        P4G_TRY(stencil(l, su, OutRR<T, OutCheb<T, false, 0>>{b, oc}, P4G_K_SMOOTH, 4, &pend.nparts,
+                                        jac, cx->part_rr));
+                    else
+                        P4G_TRY(stencil(l, su, oc, P4G_K_SMOOTH, 4, &pend.nparts, jac));
+                } else if (last && dots) {
+                    P4G_TRY(pend_flush());
                     P4G_TRY(stencil(l, SrcScaled<T>{b, cb}, OutCheb<T, false, 2>{b, nullptr, dst, T(c0), T(c1), T(c2)},
                                     P4G_K_SMOOTH, 2, &nbk, jac));
-                else
+                } else {
+                    if (fine) P4G_TRY(pend_flush());
                     P4G_TRY(stencil(l, SrcScaled<T>{b, cb}, OutCheb<T, false, 0>{b, nullptr, dst, T(c0), T(c1), T(c2)},
                                     P4G_K_SMOOTH, 2, &nbk, jac));
+                }
                 pm1_implicit = false;
             } else if (prev) {
                 if (last && dots)
@@ -522,6 +550,25 @@ struct Engine {
             (void)used;
         }
         return 0;
+    }
+
+    // run a pending residual update as a kernel of its own
+    int pend_flush() {
+        if (!pend.on) return 0;
+        const long n = (*lv)[nlev - 1].n;
+        const int gb = vblocks(n);
+        hipStream_t s = cx->stream;
+        pend.on = false;
+        pend.nparts = gb;
+        return cx->launch(P4G_K_CG_UPDATE, true, 3.0 * n * sizeof(T), [&] {
+            if (pend.rr) k_r_update<T, 1><<<gb, 256, 0, s>>>(n, pend.alpha, 0, pend.r, pend.w, pend.rn, cx->part_rr);
+            else k_r_update<T, 0><<<gb, 256, 0, s>>>(n, pend.alpha, 0, pend.r, pend.w, pend.rn, cx->part_rr);
+        });
+    }
+    // whether the fine-level cycle starts with a kernel reading the residual
+    // through a source policy (then the pending update is fused into it)
+    bool pend_fusable(const LevelInfo& L) const {
+        return pctype == PCT_MG && nlev > 1 && zg_fusable(L) && (nu == 1 || !(nu >= 3 && smoother == SM_CHEB));
     }
 
     // PCMGMCycle: b, x of level l (x zero on entry when xzero).  On the
@@ -547,9 +594,21 @@ struct Engine {
         // pre-smoothing and residual
         bool zgf = false;
         T* xpre = nullptr;
+        if (l == nlev - 1 && pend.on && !(xzero && pend_fusable(L))) P4G_TRY(pend_flush());
         if (xzero && nu == 1 && zg_fusable(L)) {
             zgf = true;
-            P4G_TRY(stencil(l, SrcScaled<T>{b, T(s * idg)}, OutResid<T, 0>{b, d.r}, P4G_K_RESID_ZG, 2, nullptr, jac));
+            if (l == nlev - 1 && pend.on) {   // b = r - alpha w, formed here and stored to b
+                const SrcScaledUpd<T> su{pend.r, pend.w, pend.alpha, T(s * idg), pend.rn};
+                pend.on = false;
+                if (pend.rr)
+                    P4G_TRY(stencil(l, su, OutRR<T, OutResid<T, 0>>{b, OutResid<T, 0>{b, d.r}}, P4G_K_RESID_ZG, 4,
+                                    &pend.nparts, jac, cx->part_rr));
+                else
+                    P4G_TRY(stencil(l, su, OutResid<T, 0>{b, d.r}, P4G_K_RESID_ZG, 4, &pend.nparts, jac));
+            } else {
+                P4G_TRY(stencil(l, SrcScaled<T>{b, T(s * idg)}, OutResid<T, 0>{b, d.r}, P4G_K_RESID_ZG, 2, nullptr,
+                                jac));
+            }
         } else {
             xpre = d.t[3] ? d.t[3] : d.t[0];
             if (xpre == x && !xzero) return set_error(PETSC_ERR_LIB, "internal buffer aliasing");
@@ -593,6 +652,12 @@ struct Engine {
     // z = B r with fused (z,r), (z,z) partials; returns the block count
     int pc_apply_dots(const T* rr, T* zz, long* nb) {
         const LevelInfo& F = (*lv)[nlev - 1];
+        if (pctype == PCT_MG && nlev > 1) {
+            P4G_TRY(mcycle(nlev - 1, rr, zz, true, true, nb));
+            if (pend.on) return set_error(PETSC_ERR_LIB, "internal: pending residual update not consumed");
+            return 0;
+        }
+        P4G_TRY(pend_flush());
         if (pctype == PCT_MG) return mcycle(nlev - 1, rr, zz, true, true, nb);
         const double c = (pctype == PCT_JACOBI) ? 1.0 / F.d : 1.0;
         if (pctype == PCT_JACOBI && (F.var || F.d != F.db))
@@ -649,29 +714,33 @@ struct Engine {
         if (dp <= ttol) { o.reason = dp < atol ? P4G_CONVERGED_ATOL : P4G_CONVERGED_RTOL; return 0; }
         T* pold = pa;
         T* pnew = pb;
+        T* rc = r;      // current residual; the update r - alpha w goes to rx
+        T* rx = r2;
         double dpi_old = 0.0;
+        // x += alpha p is deferred into the next direction update (SrcDirX);
+        // alpha = 0 before the first iteration
+        k_fill<double><<<1, 256, 0, st>>>(1, 0.0, Sblk + S_ALPHA);
+        LAUNCH_CHECK();
         int i = 0;
         for (; i < maxit; ++i) {
             o.its = i + 1;
             if (hs[S_RZ] == 0.0) { o.reason = P4G_CONVERGED_ATOL; break; }
-            // p = z + (beta/betaold) p ; w = A p ; (p, w)
+            // p = z + (beta/betaold) p ; w = A p ; (p, w) ; x += alpha_prev p_prev
             long nbs = 0;
-            P4G_TRY(stencil(nlev - 1, SrcDir<T>{z, pold, pnew, Sblk + S_BCOEF}, OutApplyDot<T>{w},
-                            P4G_K_STENCIL_DOT, 4, &nbs, false));
+            P4G_TRY(stencil(nlev - 1, SrcDirX<T, T>{z, pold, pnew, Sblk + S_BCOEF, x, Sblk + S_ALPHA},
+                            OutApplyDot<T>{w}, P4G_K_STENCIL_DOT, 6, &nbs, false));
             P4G_TRY(cx->finalize(nbs, Sblk, FIN_ALPHA, 0));
             std::swap(pold, pnew);   // pold now holds the current direction
-            const int gb = vblocks(n);
-            if (normtype == NT_UNPRECOND) {
-                P4G_TRY(cx->launch(P4G_K_CG_UPDATE, true, 6.0 * n * sizeof(T), [&] {
-                    k_cg_update<T, 1><<<gb, 256, 0, st>>>(n, Sblk, S_ALPHA, S_ONE, x, pold, r, w, cx->part);
-                }));
-                P4G_TRY(cx->finalize(gb, Sblk, FIN_RR, 0));
-            } else {
-                P4G_TRY(cx->launch(P4G_K_CG_UPDATE, true, 6.0 * n * sizeof(T), [&] {
-                    k_cg_update<T, 0><<<gb, 256, 0, st>>>(n, Sblk, S_ALPHA, S_ONE, x, pold, r, w, cx->part);
-                }));
-            }
-            P4G_TRY(pc_apply_dots(r, z, &nb));
+            // r - alpha w, fused into the first kernel of the preconditioner
+            pend.r = rc;
+            pend.w = w;
+            pend.alpha = Sblk + S_ALPHA;
+            pend.rn = rx;
+            pend.rr = normtype == NT_UNPRECOND;
+            pend.on = true;
+            P4G_TRY(pc_apply_dots(rx, z, &nb));
+            std::swap(rc, rx);
+            if (normtype == NT_UNPRECOND) P4G_TRY(cx->finalize(pend.nparts, Sblk, FIN_RR, 0, cx->part_rr));
             P4G_TRY(cx->finalize(nb, Sblk, fmode, 0));
             P4G_TRY(cx->readS(cx->hS));
             const double dpi = hs[S_PW];
@@ -688,6 +757,10 @@ struct Engine {
             if (dp >= dtol * o.rnorm0) { o.reason = P4G_DIVERGED_DTOL; break; }
         }
         if (i >= maxit && !o.reason) o.reason = P4G_DIVERGED_ITS;
+        // the deferred update of the last iteration
+        P4G_TRY(cx->launch(P4G_K_CG_UPDATE, true, 3.0 * n * sizeof(T), [&] {
+            k_x_update<T, T><<<vblocks(n), 256, 0, st>>>(n, Sblk, S_ALPHA, x, pold);
+        }));
         return 0;
     }
 };
@@ -803,6 +876,7 @@ struct p4g_solver_s {
         e32.reset();
         for (void* p : allocs) hipFree(p);
         if (cx.part) hipFree(cx.part);
+        if (cx.part_rr) hipFree(cx.part_rr);
         if (cx.S) hipFree(cx.S);
         if (cx.hS) hipHostFree(cx.hS);
         if (cx.stream) hipStreamDestroy(cx.stream);
@@ -1192,6 +1266,7 @@ int solver_setup(p4g_solver_s* s) {
     long cap = std::max<long>(VBLOCKS, nblocks(stencil_grid(F.dim, F.mx, F.my, F.mz)));
     if (!s->cx.part) {
         HIP_TRY(hipMalloc(&s->cx.part, 2 * cap * sizeof(double)));
+        HIP_TRY(hipMalloc(&s->cx.part_rr, 2 * cap * sizeof(double)));
         s->cx.npart_cap = cap;
         HIP_TRY(hipMalloc(&s->cx.S, 32 * sizeof(double)));
         double init[32] = {0};
@@ -1319,44 +1394,67 @@ static int mixed_solve(p4g_solver_s* s, const double* b, double* x) {
     float* pnew = e.pb;
     double dpi_old = 0.0;
     int i = 0, nrep = 0;
+    float* r32n = e.r2;               // x64: the residual update goes to r32n
+    bool xpend = false;               // x64: x += alpha p of the last iteration not yet applied
     for (; i < s->maxit && !R.reason; ++i) {
         R.its = i + 1;
         if (hs[S_RZ] == 0.0) replace_now = true;   // exact breakdown: certify by replacement
         long nbs = 0;
-        P4G_TRY(e.stencil(s->nlev - 1, SrcDir<float>{e.z, pold, pnew, S + S_BCOEF}, OutApplyDot<float>{e.w},
-                          P4G_K_STENCIL_DOT, 4, &nbs, false));
+        if (x64 && xpend)   // direction update with the deferred FP64 iterate update
+            P4G_TRY(e.stencil(s->nlev - 1, SrcDirX<float, double>{e.z, pold, pnew, S + S_BCOEF, xcur, S + S_ALPHA},
+                              OutApplyDot<float>{e.w}, P4G_K_STENCIL_DOT, 8, &nbs, false));
+        else
+            P4G_TRY(e.stencil(s->nlev - 1, SrcDir<float>{e.z, pold, pnew, S + S_BCOEF}, OutApplyDot<float>{e.w},
+                              P4G_K_STENCIL_DOT, 4, &nbs, false));
         P4G_TRY(cx->finalize(nbs, S, FIN_ALPHA, 0));
         std::swap(pold, pnew);
+        const bool replaced = replace_now;
         if (x64) {
-            P4G_TRY(cx->launch(P4G_K_CG_UPDATE, true, 32.0 * n, [&] {
-                k_cg_update_x64<<<gb, 256, 0, st>>>(n, S, S_ALPHA, xcur, pold, r32, e.w, cx->part);
-            }));
+            if (replace_now) {
+                // x <- x + alpha p and r = b - A x in FP64 in one pass; the
+                // recursive residual update of this iteration is not needed
+                long nbk = 0;
+                P4G_TRY(fine64(s, SrcAxpyPut<double, float>{xcur, pold, S + S_ALPHA, xnext},
+                               OutResidCvt<double, float>{b, r32}, P4G_K_OTHER, 32.0 * n, &nbk));
+                P4G_TRY(cx->finalize(nbk, So, FIN_RAW, 0));
+                std::swap(xcur, xnext);
+                xpend = false;
+                replace_now = false;
+                ++nrep;
+                P4G_TRY(e.pc_apply_dots(r32, e.z, &nb));
+            } else {
+                // r - alpha w fused into the first kernel of the V-cycle
+                e.pend.r = r32;
+                e.pend.w = e.w;
+                e.pend.alpha = S + S_ALPHA;
+                e.pend.rn = r32n;
+                e.pend.rr = true;
+                e.pend.on = true;
+                xpend = true;
+                P4G_TRY(e.pc_apply_dots(r32n, e.z, &nb));
+                std::swap(r32, r32n);
+                P4G_TRY(cx->finalize(e.pend.nparts, S, FIN_RR, 0, cx->part_rr));
+            }
         } else {
             P4G_TRY(cx->launch(P4G_K_CG_UPDATE, true, 24.0 * n, [&] {
                 k_cg_update<float, 1><<<gb, 256, 0, st>>>(n, S, S_ALPHA, keep, x32, pold, r32, e.w, cx->part);
             }));
+            P4G_TRY(cx->finalize(gb, S, FIN_RR, 0));
             keep = S_ONE;
             folded = false;
-        }
-        P4G_TRY(cx->finalize(gb, S, FIN_RR, 0));
-        const bool replaced = replace_now;
-        if (replace_now) {
-            long nbk = 0;
-            if (x64) {
-                P4G_TRY(fine64(s, SrcArr<double>{xcur}, OutResidCvt<double, float>{b, r32}, P4G_K_OTHER,
-                               20.0 * n, &nbk));
-            } else {
+            if (replace_now) {
+                long nbk = 0;
                 P4G_TRY(fine64(s, SrcSum<double, float>{xcur, x32, xnext}, OutResidCvt<double, float>{b, r32},
                                P4G_K_OTHER, 36.0 * n, &nbk));
                 std::swap(xcur, xnext);
                 keep = S_ZERO;
                 folded = true;
+                P4G_TRY(cx->finalize(nbk, So, FIN_RAW, 0));
+                replace_now = false;
+                ++nrep;
             }
-            P4G_TRY(cx->finalize(nbk, So, FIN_RAW, 0));
-            replace_now = false;
-            ++nrep;
+            P4G_TRY(e.pc_apply_dots(r32, e.z, &nb));
         }
-        P4G_TRY(e.pc_apply_dots(r32, e.z, &nb));
         P4G_TRY(cx->finalize(nb, S, FIN_BETA_PRECOND, 0));
         P4G_TRY(cx->readS(cx->hS));
         const double dpi = hs[S_PW];
@@ -1375,6 +1473,11 @@ static int mixed_solve(p4g_solver_s* s, const double* b, double* x) {
         if (replaced && rn <= target) { R.reason = rn < s->atol ? P4G_CONVERGED_ATOL : P4G_CONVERGED_RTOL; break; }
         if (rn >= s->dtol * bnorm) { R.reason = P4G_DIVERGED_DTOL; break; }
         if (!replaced && (rn <= target || rn * rn <= delta * delta * rr_last)) replace_now = true;
+    }
+    if (x64 && xpend) {   // the deferred update of the last iteration
+        P4G_TRY(cx->launch(P4G_K_CG_UPDATE, true, 20.0 * n, [&] {
+            k_x_update<double, float><<<gb, 256, 0, st>>>(n, S, S_ALPHA, xcur, pold);
+        }));
     }
     if (!R.reason) R.reason = P4G_DIVERGED_ITS;
     R.inner_its = nrep;   // number of FP64 residual replacements
