@@ -209,6 +209,7 @@ void *p4g_device_malloc(size_t bytes);
 int   p4g_device_free(void *p);
 int   p4g_memcpy_htod(void *dst, const void *src, size_t bytes);
 int   p4g_memcpy_dtoh(void *dst, const void *src, size_t bytes);
+int   p4g_memcpy_dtod(void *dst, const void *src, size_t bytes);   /* device-to-device copy */
 const char *p4g_last_error(void);
 const char *p4g_version(void);
 

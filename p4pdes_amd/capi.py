@@ -68,6 +68,7 @@ def lib():
         L.p4g_device_free.argtypes = [vp]
         L.p4g_memcpy_htod.argtypes = [vp, vp, ctypes.c_size_t]
         L.p4g_memcpy_dtoh.argtypes = [vp, vp, ctypes.c_size_t]
+        L.p4g_memcpy_dtod.argtypes = [vp, vp, ctypes.c_size_t]
         L.p4g_da_create.argtypes = [i, i, i, i, d, d, d, ctypes.POINTER(vp)]
         L.p4g_da_set_from_options.argtypes = [vp, cp]
         L.p4g_da_set_cell_coefficient.argtypes = [vp, vp]

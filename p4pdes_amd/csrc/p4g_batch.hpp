@@ -83,16 +83,20 @@ struct OutChebR {
             a1 += double(v) * double(v);
         }
     }
-    __device__ __forceinline__ void point2(long q, T ua, T ub, T Aa, T Ab, T ia, T ib, double& a0,
-                                           double& a1) const {
-        const T2<T> bb = ld2(b + q);
-        const T va = c1 * ua + c2 * (ia * (bb.a - Aa));
-        const T vb = c1 * ub + c2 * (ib * (bb.b - Ab));
-        st2(y + q, va, vb);
-        if (dots) {
-            a0 += double(va) * double(bb.a) + double(vb) * double(bb.b);
-            a1 += double(va) * double(va) + double(vb) * double(vb);
-        }
+    template <int V>
+    __device__ __forceinline__ void pointv(long q, const TV<T, V>& u, const TV<T, V>& Au, const TV<T, V>& idg,
+                                           double& a0, double& a1) const {
+        const TV<T, V> bb = ldv<T, V>(b + q);
+        TV<T, V> v;
+#pragma unroll
+        for (int e = 0; e < V; ++e) v.v[e] = c1 * u.v[e] + c2 * (idg.v[e] * (bb.v[e] - Au.v[e]));
+        stv<T, V>(y + q, v);
+        if (dots)
+#pragma unroll
+            for (int e = 0; e < V; ++e) {
+                a0 += double(v.v[e]) * double(bb.v[e]);
+                a1 += double(v.v[e]) * double(v.v[e]);
+            }
     }
 };
 

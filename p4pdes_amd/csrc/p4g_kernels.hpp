@@ -52,6 +52,38 @@ __device__ __forceinline__ void st2(T* __restrict__ p, T a, T b) {
     __builtin_memcpy(p, &v, sizeof(vec));
 }
 
+// V consecutive grid values moved as one vector access (V * sizeof(T) <= 16).
+template <typename T, int V>
+struct TV {
+    T v[V];
+};
+
+template <typename T, int V>
+__device__ __forceinline__ TV<T, V> ldv(const T* __restrict__ p) {
+    typedef T vec __attribute__((ext_vector_type(V)));
+    vec t;
+    __builtin_memcpy(&t, p, sizeof(vec));
+    TV<T, V> r;
+#pragma unroll
+    for (int e = 0; e < V; ++e) r.v[e] = t[e];
+    return r;
+}
+
+template <typename T, int V>
+__device__ __forceinline__ void stv(T* __restrict__ p, const TV<T, V>& a) {
+    typedef T vec __attribute__((ext_vector_type(V)));
+    vec t;
+#pragma unroll
+    for (int e = 0; e < V; ++e) t[e] = a.v[e];
+    __builtin_memcpy(p, &t, sizeof(vec));
+}
+
+// points per lane of the stencil kernels: one 16-byte access
+template <typename T>
+struct VecOf {
+    static constexpr int V = 16 / sizeof(T);
+};
+
 // ------------------------------------------------------------- operators
 
 // Constant-coefficient stencil.
@@ -123,22 +155,35 @@ struct OpVar {
 };
 
 // ----------------------------------------------------- operand sources
+//
+// A source gives the stencil operand u at a point (load) or at V
+// consecutive points (loadv, one vector access), and may store the value
+// it formed at the kernel's own points (put / putv).
 
 template <typename T>
 struct SrcArr {                       // u = x
     const T* __restrict__ x;
     __device__ __forceinline__ T load(long q) const { return x[q]; }
-    __device__ __forceinline__ T2<T> load2(long q) const { return ld2(x + q); }
+    template <int V>
+    __device__ __forceinline__ TV<T, V> loadv(long q) const { return ldv<T, V>(x + q); }
     __device__ __forceinline__ void put(long, T) const {}
-    __device__ __forceinline__ void put2(long, T, T) const {}
+    template <int V>
+    __device__ __forceinline__ void putv(long, const TV<T, V>&) const {}
 };
 
 template <typename T>
 struct SrcZero {                      // u = 0 (first smoothing step from a zero guess)
     __device__ __forceinline__ T load(long) const { return T(0); }
-    __device__ __forceinline__ T2<T> load2(long) const { return T2<T>{T(0), T(0)}; }
+    template <int V>
+    __device__ __forceinline__ TV<T, V> loadv(long) const {
+        TV<T, V> r;
+#pragma unroll
+        for (int e = 0; e < V; ++e) r.v[e] = T(0);
+        return r;
+    }
     __device__ __forceinline__ void put(long, T) const {}
-    __device__ __forceinline__ void put2(long, T, T) const {}
+    template <int V>
+    __device__ __forceinline__ void putv(long, const TV<T, V>&) const {}
 };
 
 template <typename T>
@@ -146,12 +191,16 @@ struct SrcScaled {                    // u = c b   (zero-guess smoothing step)
     const T* __restrict__ b;
     T c;
     __device__ __forceinline__ T load(long q) const { return c * b[q]; }
-    __device__ __forceinline__ T2<T> load2(long q) const {
-        const T2<T> v = ld2(b + q);
-        return T2<T>{c * v.a, c * v.b};
+    template <int V>
+    __device__ __forceinline__ TV<T, V> loadv(long q) const {
+        TV<T, V> r = ldv<T, V>(b + q);
+#pragma unroll
+        for (int e = 0; e < V; ++e) r.v[e] *= c;
+        return r;
     }
     __device__ __forceinline__ void put(long, T) const {}
-    __device__ __forceinline__ void put2(long, T, T) const {}
+    template <int V>
+    __device__ __forceinline__ void putv(long, const TV<T, V>&) const {}
 };
 
 template <typename T>
@@ -160,12 +209,16 @@ struct SrcScaledPut {                 // u = c b, stored to x
     T c;
     T* __restrict__ x;
     __device__ __forceinline__ T load(long q) const { return c * b[q]; }
-    __device__ __forceinline__ T2<T> load2(long q) const {
-        const T2<T> v = ld2(b + q);
-        return T2<T>{c * v.a, c * v.b};
+    template <int V>
+    __device__ __forceinline__ TV<T, V> loadv(long q) const {
+        TV<T, V> r = ldv<T, V>(b + q);
+#pragma unroll
+        for (int e = 0; e < V; ++e) r.v[e] *= c;
+        return r;
     }
     __device__ __forceinline__ void put(long q, T v) const { x[q] = v; }
-    __device__ __forceinline__ void put2(long q, T a, T bb) const { st2(x + q, a, bb); }
+    template <int V>
+    __device__ __forceinline__ void putv(long q, const TV<T, V>& u) const { stv<T, V>(x + q, u); }
 };
 
 template <typename T>
@@ -175,13 +228,18 @@ struct SrcDir {                       // p_new = z + beta p_old, stored (CG dire
     T* __restrict__ pnew;
     const double* __restrict__ beta;  // device scalar
     __device__ __forceinline__ T load(long q) const { return z[q] + T(*beta) * pold[q]; }
-    __device__ __forceinline__ T2<T> load2(long q) const {
+    template <int V>
+    __device__ __forceinline__ TV<T, V> loadv(long q) const {
         const T bt = T(*beta);
-        const T2<T> zz = ld2(z + q), pp = ld2(pold + q);
-        return T2<T>{zz.a + bt * pp.a, zz.b + bt * pp.b};
+        TV<T, V> r = ldv<T, V>(z + q);
+        const TV<T, V> pp = ldv<T, V>(pold + q);
+#pragma unroll
+        for (int e = 0; e < V; ++e) r.v[e] += bt * pp.v[e];
+        return r;
     }
     __device__ __forceinline__ void put(long q, T v) const { pnew[q] = v; }
-    __device__ __forceinline__ void put2(long q, T a, T b) const { st2(pnew + q, a, b); }
+    template <int V>
+    __device__ __forceinline__ void putv(long q, const TV<T, V>& u) const { stv<T, V>(pnew + q, u); }
 };
 
 // p_new = z + beta p_old, stored; at the own point also the deferred CG
@@ -196,21 +254,33 @@ struct SrcDirX {
     X* __restrict__ x;
     const double* __restrict__ alpha; // device scalar
     __device__ __forceinline__ T load(long q) const { return z[q] + T(*beta) * pold[q]; }
-    __device__ __forceinline__ T2<T> load2(long q) const {
+    template <int V>
+    __device__ __forceinline__ TV<T, V> loadv(long q) const {
         const T bt = T(*beta);
-        const T2<T> zz = ld2(z + q), pp = ld2(pold + q);
-        return T2<T>{zz.a + bt * pp.a, zz.b + bt * pp.b};
+        TV<T, V> r = ldv<T, V>(z + q);
+        const TV<T, V> pp = ldv<T, V>(pold + q);
+#pragma unroll
+        for (int e = 0; e < V; ++e) r.v[e] += bt * pp.v[e];
+        return r;
     }
     __device__ __forceinline__ void put(long q, T v) const {
         pnew[q] = v;
         x[q] += X(*alpha) * X(pold[q]);
     }
-    __device__ __forceinline__ void put2(long q, T a, T b) const {
-        st2(pnew + q, a, b);
+    template <int V>
+    __device__ __forceinline__ void putv(long q, const TV<T, V>& u) const {
+        stv<T, V>(pnew + q, u);
         const X al = X(*alpha);
-        const T2<T> pp = ld2(pold + q);
-        const T2<X> xx = ld2(x + q);
-        st2(x + q, xx.a + al * X(pp.a), xx.b + al * X(pp.b));
+        const TV<T, V> pp = ldv<T, V>(pold + q);
+        // x may hold wider elements than p: V of them in one or more vector accesses
+        constexpr int VX = 16 / sizeof(X) < V ? 16 / sizeof(X) : V;
+#pragma unroll
+        for (int h = 0; h < V; h += VX) {
+            TV<X, VX> xx = ldv<X, VX>(x + q + h);
+#pragma unroll
+            for (int e = 0; e < VX; ++e) xx.v[e] += al * X(pp.v[h + e]);
+            stv<X, VX>(x + q + h, xx);
+        }
     }
 };
 
@@ -224,16 +294,24 @@ struct SrcScaledUpd {
     T c;
     T* __restrict__ rn;
     __device__ __forceinline__ T load(long q) const { return c * (r[q] - T(*alpha) * w[q]); }
-    __device__ __forceinline__ T2<T> load2(long q) const {
+    template <int V>
+    __device__ __forceinline__ TV<T, V> loadv(long q) const {
         const T al = T(*alpha);
-        const T2<T> rr = ld2(r + q), ww = ld2(w + q);
-        return T2<T>{c * (rr.a - al * ww.a), c * (rr.b - al * ww.b)};
+        TV<T, V> rr = ldv<T, V>(r + q);
+        const TV<T, V> ww = ldv<T, V>(w + q);
+#pragma unroll
+        for (int e = 0; e < V; ++e) rr.v[e] = c * (rr.v[e] - al * ww.v[e]);
+        return rr;
     }
     __device__ __forceinline__ void put(long q, T) const { rn[q] = r[q] - T(*alpha) * w[q]; }
-    __device__ __forceinline__ void put2(long q, T, T) const {
+    template <int V>
+    __device__ __forceinline__ void putv(long q, const TV<T, V>&) const {
         const T al = T(*alpha);
-        const T2<T> rr = ld2(r + q), ww = ld2(w + q);
-        st2(rn + q, rr.a - al * ww.a, rr.b - al * ww.b);
+        TV<T, V> rr = ldv<T, V>(r + q);
+        const TV<T, V> ww = ldv<T, V>(w + q);
+#pragma unroll
+        for (int e = 0; e < V; ++e) rr.v[e] = rr.v[e] - al * ww.v[e];
+        stv<T, V>(rn + q, rr);
     }
 };
 
@@ -246,14 +324,18 @@ struct SrcAxpyPut {
     const double* __restrict__ alpha;  // device scalar
     X* __restrict__ xnew;
     __device__ __forceinline__ X load(long q) const { return x[q] + X(*alpha) * X(p[q]); }
-    __device__ __forceinline__ T2<X> load2(long q) const {
+    template <int V>
+    __device__ __forceinline__ TV<X, V> loadv(long q) const {
         const X al = X(*alpha);
-        const T2<X> xx = ld2(x + q);
-        const T2<P> pp = ld2(p + q);
-        return T2<X>{xx.a + al * X(pp.a), xx.b + al * X(pp.b)};
+        TV<X, V> xx = ldv<X, V>(x + q);
+        const TV<P, V> pp = ldv<P, V>(p + q);
+#pragma unroll
+        for (int e = 0; e < V; ++e) xx.v[e] += al * X(pp.v[e]);
+        return xx;
     }
     __device__ __forceinline__ void put(long q, X v) const { xnew[q] = v; }
-    __device__ __forceinline__ void put2(long q, X a, X b) const { st2(xnew + q, a, b); }
+    template <int V>
+    __device__ __forceinline__ void putv(long q, const TV<X, V>& u) const { stv<X, V>(xnew + q, u); }
 };
 
 template <typename T, typename E>
@@ -262,16 +344,22 @@ struct SrcSum {                       // x_new = x + e (refinement update), stor
     const E* __restrict__ e;
     T* __restrict__ xnew;
     __device__ __forceinline__ T load(long q) const { return x[q] + T(e[q]); }
-    __device__ __forceinline__ T2<T> load2(long q) const {
-        const T2<T> xx = ld2(x + q);
-        const T2<E> ee = ld2(e + q);
-        return T2<T>{xx.a + T(ee.a), xx.b + T(ee.b)};
+    template <int V>
+    __device__ __forceinline__ TV<T, V> loadv(long q) const {
+        TV<T, V> xx = ldv<T, V>(x + q);
+#pragma unroll
+        for (int k = 0; k < V; ++k) xx.v[k] += T(e[q + k]);
+        return xx;
     }
     __device__ __forceinline__ void put(long q, T v) const { xnew[q] = v; }
-    __device__ __forceinline__ void put2(long q, T a, T b) const { st2(xnew + q, a, b); }
+    template <int V>
+    __device__ __forceinline__ void putv(long q, const TV<T, V>& u) const { stv<T, V>(xnew + q, u); }
 };
 
 // ------------------------------------------------------ output policies
+//
+// An output consumes u, A u and the Jacobi factor at a point (point) or at
+// V consecutive points (pointv) and accumulates up to NDOT dot products.
 
 template <typename T>
 struct OutApplyDot {                  // w = A u ; acc0 += u.w
@@ -281,9 +369,12 @@ struct OutApplyDot {                  // w = A u ; acc0 += u.w
         w[q] = Au;
         a0 += double(u) * double(Au);
     }
-    __device__ __forceinline__ void point2(long q, T ua, T ub, T Aa, T Ab, T, T, double& a0, double&) const {
-        st2(w + q, Aa, Ab);
-        a0 += double(ua) * double(Aa) + double(ub) * double(Ab);
+    template <int V>
+    __device__ __forceinline__ void pointv(long q, const TV<T, V>& u, const TV<T, V>& Au, const TV<T, V>&,
+                                           double& a0, double&) const {
+        stv<T, V>(w + q, Au);
+#pragma unroll
+        for (int e = 0; e < V; ++e) a0 += double(u.v[e]) * double(Au.v[e]);
     }
 };
 
@@ -292,8 +383,10 @@ struct OutApply {                     // w = A u
     T* __restrict__ w;
     static constexpr int NDOT = 0;
     __device__ __forceinline__ void point(long q, T, T Au, T, double&, double&) const { w[q] = Au; }
-    __device__ __forceinline__ void point2(long q, T, T, T Aa, T Ab, T, T, double&, double&) const {
-        st2(w + q, Aa, Ab);
+    template <int V>
+    __device__ __forceinline__ void pointv(long q, const TV<T, V>&, const TV<T, V>& Au, const TV<T, V>&, double&,
+                                           double&) const {
+        stv<T, V>(w + q, Au);
     }
 };
 
@@ -307,11 +400,16 @@ struct OutResid {                     // r = b - A u ; acc0 += r.r (NORM)
         r[q] = v;
         if (NORM) a0 += double(v) * double(v);
     }
-    __device__ __forceinline__ void point2(long q, T, T, T Aa, T Ab, T, T, double& a0, double&) const {
-        const T2<T> bb = ld2(b + q);
-        const T va = bb.a - Aa, vb = bb.b - Ab;
-        st2(r + q, va, vb);
-        if (NORM) a0 += double(va) * double(va) + double(vb) * double(vb);
+    template <int V>
+    __device__ __forceinline__ void pointv(long q, const TV<T, V>&, const TV<T, V>& Au, const TV<T, V>&,
+                                           double& a0, double&) const {
+        TV<T, V> v = ldv<T, V>(b + q);
+#pragma unroll
+        for (int e = 0; e < V; ++e) v.v[e] -= Au.v[e];
+        stv<T, V>(r + q, v);
+        if (NORM)
+#pragma unroll
+            for (int e = 0; e < V; ++e) a0 += double(v.v[e]) * double(v.v[e]);
     }
 };
 
@@ -325,11 +423,18 @@ struct OutResidCvt {                  // r = b - A u in T, stored as R ; acc0 +=
         r[q] = R(v);
         a0 += double(v) * double(v);
     }
-    __device__ __forceinline__ void point2(long q, T, T, T Aa, T Ab, T, T, double& a0, double&) const {
-        const T2<T> bb = ld2(b + q);
-        const T va = bb.a - Aa, vb = bb.b - Ab;
-        st2(r + q, R(va), R(vb));
-        a0 += double(va) * double(va) + double(vb) * double(vb);
+    template <int V>
+    __device__ __forceinline__ void pointv(long q, const TV<T, V>&, const TV<T, V>& Au, const TV<T, V>&,
+                                           double& a0, double&) const {
+        const TV<T, V> bb = ldv<T, V>(b + q);
+        TV<R, V> o;
+#pragma unroll
+        for (int e = 0; e < V; ++e) {
+            const T v = bb.v[e] - Au.v[e];
+            o.v[e] = R(v);
+            a0 += double(v) * double(v);
+        }
+        stv<R, V>(r + q, o);
     }
 };
 
@@ -346,11 +451,13 @@ struct OutRR {
         const double v = double(rn[q]);
         a0 += v * v;
     }
-    __device__ __forceinline__ void point2(long q, T ua, T ub, T Aa, T Ab, T ia, T ib, double& a0,
-                                           double& a1) const {
-        inner.point2(q, ua, ub, Aa, Ab, ia, ib, a0, a1);
-        const T2<T> v = ld2(rn + q);
-        a0 += double(v.a) * double(v.a) + double(v.b) * double(v.b);
+    template <int V>
+    __device__ __forceinline__ void pointv(long q, const TV<T, V>& u, const TV<T, V>& Au, const TV<T, V>& idg,
+                                           double& a0, double& a1) const {
+        inner.template pointv<V>(q, u, Au, idg, a0, a1);
+        const TV<T, V> v = ldv<T, V>(rn + q);
+#pragma unroll
+        for (int e = 0; e < V; ++e) a0 += double(v.v[e]) * double(v.v[e]);
     }
 };
 
@@ -373,21 +480,25 @@ struct OutCheb {
             a1 += double(v) * double(v);
         }
     }
-    __device__ __forceinline__ void point2(long q, T ua, T ub, T Aa, T Ab, T ia, T ib, double& a0,
-                                           double& a1) const {
-        const T2<T> bb = ld2(b + q);
-        T va = c1 * ua + c2 * (ia * (bb.a - Aa));
-        T vb = c1 * ub + c2 * (ib * (bb.b - Ab));
+    template <int V>
+    __device__ __forceinline__ void pointv(long q, const TV<T, V>& u, const TV<T, V>& Au, const TV<T, V>& idg,
+                                           double& a0, double& a1) const {
+        const TV<T, V> bb = ldv<T, V>(b + q);
+        TV<T, V> v;
+#pragma unroll
+        for (int e = 0; e < V; ++e) v.v[e] = c1 * u.v[e] + c2 * (idg.v[e] * (bb.v[e] - Au.v[e]));
         if (PREV) {
-            const T2<T> pp = ld2(prev + q);
-            va += c0 * pp.a;
-            vb += c0 * pp.b;
+            const TV<T, V> pp = ldv<T, V>(prev + q);
+#pragma unroll
+            for (int e = 0; e < V; ++e) v.v[e] += c0 * pp.v[e];
         }
-        st2(y + q, va, vb);
-        if (DOTS) {
-            a0 += double(va) * double(bb.a) + double(vb) * double(bb.b);
-            a1 += double(va) * double(va) + double(vb) * double(vb);
-        }
+        stv<T, V>(y + q, v);
+        if (DOTS)
+#pragma unroll
+            for (int e = 0; e < V; ++e) {
+                a0 += double(v.v[e]) * double(bb.v[e]);
+                a1 += double(v.v[e]) * double(v.v[e]);
+            }
     }
 };
 
@@ -423,23 +534,24 @@ __device__ __forceinline__ void block_partials(double a0, double a1, double* __r
 // ------------------------------------------------------------- stencil
 
 // Marching stencil pass over a 2D or 3D grid: for every vertex, u from Src,
-// A u and the Jacobi factor from Op, result written by Out.  A lane owns the
-// x pair (i0, i0+1), i0 = 1 + 2 (64 bx + lane), moved with 2-element vector
-// accesses; the west value of i0 and the east value of i0+1 come from the
-// neighbouring lanes (the wave's edge lanes load them).  The lane at i0 = 1
-// also owns the boundary point i = 0.  The march runs along the slowest
-// dimension (z in 3D, y in 2D) keeping the planes m-1, m, m+1 in registers;
-// 3D rows j+-1 are read through the cache.  Every lane of a wave runs the
-// same loop (loads are predicated) so the cross-lane exchanges see the whole
-// wave.  Grid:
-//   3D: (ceil((mx-1)/128), ceil(my/TY), ceil(mz/kch)), block (64,TY)
-//   2D: (ceil((mx-1)/128), ceil(ceil(my/kch)/TY), 1),  block (64,TY)
-template <typename T, int DIM, class Op, class Src, class Out>
-__device__ __forceinline__ void stencil_pairs(int bx, int by, int bz, int mx, int my, int mz, int kch,
-                                              const Op& op, const Src& src, const Out& out, double& a0,
-                                              double& a1) {
+// A u and the Jacobi factor from Op, result written by Out.  A lane owns V
+// consecutive x points i0 .. i0+V-1, i0 = 1 + V (64 bx + lane), moved with
+// one 16-byte vector access (V = 2 in FP64, 4 in FP32); the west value of
+// i0 and the east value of i0+V-1 come from the neighbouring lanes (the
+// wave's edge lanes load them).  The lane at i0 = 1 also owns the boundary
+// point i = 0; a lane at the end of a row whose points do not fill V uses
+// element accesses.  The march runs along the slowest dimension (z in 3D,
+// y in 2D) keeping the planes m-1, m, m+1 in registers; 3D rows j+-1 are
+// read through the cache.  Every lane of a wave runs the same loop (loads
+// are predicated) so the cross-lane exchanges see the whole wave.  Grid:
+//   3D: (ceil((mx-1)/(64V)), ceil(my/TY), ceil(mz/kch)), block (64,TY)
+//   2D: (ceil((mx-1)/(64V)), ceil(ceil(my/kch)/TY), 1),  block (64,TY)
+template <typename T, int DIM, int V, class Op, class Src, class Out>
+__device__ __forceinline__ void stencil_vec(int bx, int by, int bz, int mx, int my, int mz, int kch,
+                                            const Op& op, const Src& src, const Out& out, double& a0,
+                                            double& a1) {
     const int lane = threadIdx.x;
-    const int i0 = 1 + 2 * (bx * TX + lane), i1 = i0 + 1;
+    const int i0 = 1 + V * (bx * TX + lane);
     const T z0 = T(0);
     const long sy = mx;
     int j, m0, m1, M;
@@ -462,66 +574,87 @@ __device__ __forceinline__ void stencil_pairs(int bx, int by, int bz, int mx, in
     }
     if (m0 >= m1) return;              // uniform per wave
     const bool act = i0 < mx;
-    const bool two = i1 < mx;
+    const int ne = act ? min(V, mx - i0) : 0;     // points of this lane
+    const bool full = ne == V;
     const bool jb = DIM == 3 && (j == 0 || j == my - 1);
     const bool ms = DIM == 3 && j - 1 > 0, mn = DIM == 3 && j + 1 < my - 1;
-    const bool wa = i0 - 1 > 0;                    // i0 couples to its west neighbour
-    const bool ea = i1 < mx - 1;                   // i0 couples to i1 (both interior)
-    const bool eb = i1 + 1 < mx - 1;               // i1 couples to its east neighbour
-    const bool xa = i0 == mx - 1, xb = i1 == mx - 1;   // x-boundary points
-    auto L2 = [&](long a) -> T2<T> {
-        if (two) return src.load2(a);
-        return T2<T>{src.load(a), z0};
+    TV<T, V> zv;
+#pragma unroll
+    for (int e = 0; e < V; ++e) zv.v[e] = z0;
+    auto LV = [&](long a) -> TV<T, V> {
+        if (full) return src.template loadv<V>(a);
+        TV<T, V> r = zv;
+#pragma unroll
+        for (int e = 0; e < V; ++e)
+            if (e < ne) r.v[e] = src.load(a + e);
+        return r;
     };
     long q = base + (long)m0 * sm + i0;
-    T2<T> ud{z0, z0}, uc{z0, z0};
+    TV<T, V> ud = zv, uc = zv;
     if (act) {
-        if (m0 > 0) ud = L2(q - sm);
-        uc = L2(q);
+        if (m0 > 0) ud = LV(q - sm);
+        uc = LV(q);
     }
     for (int m = m0; m < m1; ++m, q += sm) {
-        T2<T> uu{z0, z0};
-        if (act && m + 1 < M) uu = L2(q + sm);
-        T wv = __shfl_up(uc.b, 1, TX);
-        T ev = __shfl_down(uc.a, 1, TX);
+        TV<T, V> uu = zv;
+        if (act && m + 1 < M) uu = LV(q + sm);
+        T wv = __shfl_up(uc.v[V - 1], 1, TX);
+        T ev = __shfl_down(uc.v[0], 1, TX);
         if (act) {
-            if (lane == 0 && wa) wv = src.load(q - 1);
-            if (lane == TX - 1 && eb) ev = src.load(q + 2);
+            if (lane == 0 && i0 - 1 > 0) wv = src.load(q - 1);
+            if (lane == TX - 1 && i0 + V < mx - 1) ev = src.load(q + V);
             const int jj = DIM == 3 ? j : m, kk = DIM == 3 ? m : 0;
-            if (two) src.put2(q, uc.a, uc.b);
-            else src.put(q, uc.a);
-            T Aa, Ab = z0, ia, ib = z0;
-            if (jb || m == 0 || m == M - 1) {
-                op.bnd(i0, jj, kk, uc.a, Aa, ia);
-                if (two) op.bnd(i1, jj, kk, uc.b, Ab, ib);
+            if (full) {
+                src.template putv<V>(q, uc);
             } else {
-                T2<T> sv{z0, z0}, nv{z0, z0};
+#pragma unroll
+                for (int e = 0; e < V; ++e)
+                    if (e < ne) src.put(q + e, uc.v[e]);
+            }
+            TV<T, V> Au = zv, idg = zv;
+            if (jb || m == 0 || m == M - 1) {
+#pragma unroll
+                for (int e = 0; e < V; ++e)
+                    if (e < ne) op.bnd(i0 + e, jj, kk, uc.v[e], Au.v[e], idg.v[e]);
+            } else {
+                TV<T, V> sv = zv, nv = zv, dn = zv, up = zv;
                 if (DIM == 3) {
-                    if (ms) sv = L2(q - sy);
-                    if (mn) nv = L2(q + sy);
+                    if (ms) sv = LV(q - sy);
+                    if (mn) nv = LV(q + sy);
+                    if (m - 1 > 0) dn = ud;
+                    if (m + 1 < M - 1) up = uu;
                 } else {
                     if (m - 1 > 0) sv = ud;
                     if (m + 1 < M - 1) nv = uu;
                 }
-                T2<T> dn{z0, z0}, up{z0, z0};
-                if (DIM == 3) {
-                    if (m - 1 > 0) dn = ud;
-                    if (m + 1 < M - 1) up = uu;
-                }
-                if (xa) op.bnd(i0, jj, kk, uc.a, Aa, ia);
-                else op.interior(i0, jj, kk, uc.a, wa ? wv : z0, ea ? uc.b : z0, sv.a, nv.a, dn.a, up.a, Aa, ia);
-                if (two) {
-                    if (xb) op.bnd(i1, jj, kk, uc.b, Ab, ib);
-                    else op.interior(i1, jj, kk, uc.b, uc.a, eb ? ev : z0, sv.b, nv.b, dn.b, up.b, Ab, ib);
+#pragma unroll
+                for (int e = 0; e < V; ++e) {
+                    const int i = i0 + e;
+                    if (e < ne) {
+                        if (i == mx - 1) {
+                            op.bnd(i, jj, kk, uc.v[e], Au.v[e], idg.v[e]);
+                        } else {
+                            const T w = e == 0 ? (i - 1 > 0 ? wv : z0) : uc.v[e > 0 ? e - 1 : 0];
+                            const T east = i + 1 < mx - 1 ? (e == V - 1 ? ev : uc.v[e < V - 1 ? e + 1 : 0]) : z0;
+                            op.interior(i, jj, kk, uc.v[e], w, east, sv.v[e], nv.v[e], dn.v[e], up.v[e], Au.v[e],
+                                        idg.v[e]);
+                        }
+                    }
                 }
             }
-            if (two) out.point2(q, uc.a, uc.b, Aa, Ab, ia, ib, a0, a1);
-            else out.point(q, uc.a, Aa, ia, a0, a1);
+            if (full) {
+                out.template pointv<V>(q, uc, Au, idg, a0, a1);
+            } else {
+#pragma unroll
+                for (int e = 0; e < V; ++e)
+                    if (e < ne) out.point(q + e, uc.v[e], Au.v[e], idg.v[e], a0, a1);
+            }
             if (i0 == 1) {
                 const T u0 = src.load(q - 1);
+                T A0, i00;
                 src.put(q - 1, u0);
-                op.bnd(0, jj, kk, u0, Aa, ia);
-                out.point(q - 1, u0, Aa, ia, a0, a1);
+                op.bnd(0, jj, kk, u0, A0, i00);
+                out.point(q - 1, u0, A0, i00, a0, a1);
             }
         }
         ud = uc;
@@ -529,14 +662,14 @@ __device__ __forceinline__ void stencil_pairs(int bx, int by, int bz, int mx, in
     }
 }
 
-// Generic stencil pass (see stencil_pairs); 1D: thread per point from i = 1,
+// Generic stencil pass (see stencil_vec); 1D: thread per point from i = 1,
 // grid (ceil((mx-1)/64), 1, 1), block (64,1).
 template <typename T, int DIM, class Op, class Src, class Out>
 __device__ __forceinline__ void stencil_body(int bx, int by, int bz, int mx, int my, int mz, int kch,
                                              const Op& op, const Src& src, const Out& out,
                                              double& a0, double& a1) {
     if (DIM != 1) {
-        stencil_pairs<T, DIM, Op, Src, Out>(bx, by, bz, mx, my, mz, kch, op, src, out, a0, a1);
+        stencil_vec<T, DIM, VecOf<T>::V, Op, Src, Out>(bx, by, bz, mx, my, mz, kch, op, src, out, a0, a1);
         return;
     }
     const int i = 1 + bx * TX + threadIdx.x;
@@ -769,10 +902,11 @@ struct OutStoreU {
         t[q] = u;
         inner.point(q, u, Au, idg, a0, a1);
     }
-    __device__ __forceinline__ void point2(long q, T ua, T ub, T Aa, T Ab, T ia, T ib, double& a0,
-                                           double& a1) const {
-        st2(t + q, ua, ub);
-        inner.point2(q, ua, ub, Aa, Ab, ia, ib, a0, a1);
+    template <int V>
+    __device__ __forceinline__ void pointv(long q, const TV<T, V>& u, const TV<T, V>& Au, const TV<T, V>& idg,
+                                           double& a0, double& a1) const {
+        stv<T, V>(t + q, u);
+        inner.template pointv<V>(q, u, Au, idg, a0, a1);
     }
 };
 
@@ -781,12 +915,12 @@ struct OutStoreU {
 //   Out(t, A t)        (e.g. y = t + s D^-1 (b - A t))
 // on a 2D/3D level of size (mx, my, mz) with coarse level (mxc, myc, .).
 // A lane owns coarse column I = 1 + 64 bx + lane and the fine pair
-// (2I-1, 2I) -- the lane mapping of stencil_pairs -- and the lane at I = 1
+// (2I-1, 2I) -- the lane mapping of stencil_vec with V = 2 -- and the lane at I = 1
 // also fine 0.  The coarse values of I-1 and I+1 come from the neighbouring
 // lanes.  3D: a thread owns fine row j and marches fine planes k; t at rows
 // j-1, j, j+1 is interpolated from the coarse rows of the two coarse planes
 // k>>1 and (k>>1)+1, which stay in registers.  2D: a thread marches fine
-// rows.  Loops are uniform per wave.  Grid: that of stencil_pairs.
+// rows.  Loops are uniform per wave.  Grid: that of stencil_vec with V = 2.
 template <typename T, int DIM, class Op, bool ZG, class Out>
 __device__ __forceinline__ void prolong_smooth_body(int bx, int by, int bz, int mx, int my, int mz, int mxc,
                                                     int myc, int kch, const Op& op, const T* __restrict__ xb,
@@ -829,7 +963,7 @@ __device__ __forceinline__ void prolong_smooth_body(int bx, int by, int bz, int 
             if (i1 == mx - 1) op.bnd(i1, jj, kk, tc.b, Ab, ib);
             else op.interior(i1, jj, kk, tc.b, tc.a, i1 + 1 < mx - 1 ? ev : z0, ts.b, tn.b, td.b, tu.b, Ab, ib);
         }
-        out.point2(q, tc.a, tc.b, Aa, Ab, ia, ib, a0, a1);
+        out.template pointv<2>(q, TV<T, 2>{{tc.a, tc.b}}, TV<T, 2>{{Aa, Ab}}, TV<T, 2>{{ia, ib}}, a0, a1);
         if (I == 1) {      // fine 0: a boundary point
             const T t0 = basev(q - 1) + Pc.w;
             op.bnd(0, jj, kk, t0, Aa, ia);

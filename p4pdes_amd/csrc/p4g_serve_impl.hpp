@@ -100,7 +100,7 @@ static size_t req_bytes(p4g_solver_s* s) {
     const long n = F.n;
     size_t b = 2 * n * sizeof(double) + 6 * n * sizeof(float);   // 5 Krylov vectors + fine-level r
     for (int l = 0; l + 1 < s->nlev; ++l) b += (l > 0 ? 3 : 2) * s->lv[l].n * sizeof(float);
-    const long nb = std::max<long>(nblocks(stencil_grid(F.dim, F.mx, F.my, F.mz)), vblocks(n));
+    const long nb = std::max<long>(nblocks(pair_grid(F.dim, F.mx, F.my, F.mz)), vblocks(n));
     b += 2 * nb * sizeof(double);
     return b;
 }
@@ -143,8 +143,10 @@ static const double* solver_ainv(p4g_solver_s* s) {
     return s->e32 ? s->e32->Ainv : (s->e64 ? s->e64->Ainv : nullptr);
 }
 
-static GridRef gref_stencil(const LevelInfo& L, long start) {
-    const dim3 g = stencil_grid(L.dim, L.mx, L.my, L.mz);
+// a stencil launch over level L with V x points per lane (V = 2: the pair
+// geometry of the fused prolongation + smoothing kernel)
+static GridRef gref_stencil(const LevelInfo& L, long start, int V) {
+    const dim3 g = stencil_grid_v(L.dim, L.mx, L.my, L.mz, V);
     GridRef r;
     r.mx = L.mx; r.my = L.my; r.mz = L.mz; r.kch = kchf(L.dim, L.my, L.mz);
     r.gx = g.x; r.gy = g.y; r.gz = g.z; r.start = start;
@@ -269,7 +271,7 @@ static int build_vcycle(p4g_server_s* v, Packer& pk, std::vector<BLaunch>& seq) 
             const LevelInfo& L = s->lv[q.l];
             const LevelInfo& C = s->lv[q.l - 1];
             ResidItem it;
-            it.g = gref_stencil(L, s1);
+            it.g = gref_stencil(L, s1, VecOf<float>::V);
             s1 += gblocks(it.g);
             it.part = nullptr;
             it.flag = nullptr;
@@ -320,7 +322,7 @@ static int build_vcycle(p4g_server_s* v, Packer& pk, std::vector<BLaunch>& seq) 
             const double sc = level_scale(s, L);
             const bool fin = q.l == q.r->nlev - 1;
             PSmoothItem m;
-            m.g = gref_stencil(L, s1);
+            m.g = gref_stencil(L, s1, 2);
             s1 += gblocks(m.g);
             m.mxc = C.mx;
             m.myc = C.my;
@@ -373,7 +375,7 @@ static int req_admit(p4g_server_s* v, SrvReq* r) {
         }
         if (l > 0) P4G_TRY(req_alloc(r, (void**)&r->lv[l].r, nl * 4));
     }
-    r->npart = std::max<long>(nblocks(stencil_grid(F.dim, F.mx, F.my, F.mz)), vblocks(n));
+    r->npart = std::max<long>(nblocks(pair_grid(F.dim, F.mx, F.my, F.mz)), vblocks(n));
     P4G_TRY(req_alloc(r, (void**)&r->part, 2 * r->npart * 8));
     r->slot = v->free_slots.back();
     v->free_slots.pop_back();
@@ -416,7 +418,7 @@ static int round_once(p4g_server_s* v) {
         if (r->state == RQ_ITER) {
             DirItem d;
             long& sd = r->dim == 3 ? sd3 : sd2;
-            d.g = gref_stencil(F, sd);
+            d.g = gref_stencil(F, sd, VecOf<float>::V);
             sd += gblocks(d.g);
             d.part = r->part;
             d.flag = nullptr;
@@ -440,7 +442,7 @@ static int round_once(p4g_server_s* v) {
             if (r->replace_now) {
                 RepItem q;
                 long& sr = r->dim == 3 ? sr3 : sr2;
-                q.g = gref_stencil(F, sr);
+                q.g = gref_stencil(F, sr, VecOf<double>::V);
                 sr += gblocks(q.g);
                 q.part = r->part;
                 q.flag = nullptr;
@@ -455,7 +457,7 @@ static int round_once(p4g_server_s* v) {
             }
         }
         const LevelInfo& Ff = s->lv.back();
-        fbeta.push_back(FinItem{r->part, nblocks(stencil_grid(Ff.dim, Ff.mx, Ff.my, Ff.mz)), r->S,
+        fbeta.push_back(FinItem{r->part, nblocks(pair_grid(Ff.dim, Ff.mx, Ff.my, Ff.mz)), r->S,
                                 FIN_BETA_PRECOND, r->state == RQ_INIT ? 1 : 0, nullptr});
     }
     auto add = [&](int kind, int dim, const void* data, size_t each, size_t cnt, long blocks) {

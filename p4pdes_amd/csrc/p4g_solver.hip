@@ -161,13 +161,18 @@ struct Ctx {
 
 static inline int vblocks(long n) { return (int)std::max(1L, std::min<long>(VBLOCKS, (n + 255) / 256)); }
 
-// launch geometry of the marching stencil kernels (2D/3D: two x points per lane)
-static dim3 stencil_grid(int dim, int mx, int my, int mz) {
+// launch geometry of the marching stencil kernels (2D/3D: V x points per lane)
+static dim3 stencil_grid_v(int dim, int mx, int my, int mz, int V) {
     if (dim == 1) return dim3((mx - 1 + TX - 1) / TX, 1, 1);
-    const int gx = (mx - 1 + 2 * TX - 1) / (2 * TX);
+    const int gx = (mx - 1 + V * TX - 1) / (V * TX);
     if (dim == 3) return dim3(gx, (my + TY - 1) / TY, (mz + kchf(dim, my, mz) - 1) / kchf(dim, my, mz));
     return dim3(gx, ((my + kchf(dim, my, mz) - 1) / kchf(dim, my, mz) + TY - 1) / TY, 1);
 }
+template <typename T>
+static dim3 stencil_grid(int dim, int mx, int my, int mz) { return stencil_grid_v(dim, mx, my, mz, VecOf<T>::V); }
+// two points per lane: the fused prolongation + smoothing kernel; the
+// largest stencil grid (partial-sum buffers are sized by it)
+static dim3 pair_grid(int dim, int mx, int my, int mz) { return stencil_grid_v(dim, mx, my, mz, 2); }
 static dim3 stencil_block(int dim) { return dim == 1 ? dim3(TX, 1, 1) : dim3(TX, TY, 1); }
 static long nblocks(dim3 g) { return (long)g.x * g.y * g.z; }
 // transfers: lanes over coarse columns I = 1..mxc-1, rows and march chunks
@@ -295,7 +300,7 @@ struct Engine {
     template <class Src, class Out>
     int stencil(int l, Src src, Out out, int cls, double passes, long* nb, bool jac, double* pbuf = nullptr) {
         const LevelInfo& L = (*lv)[l];
-        const dim3 g = stencil_grid(L.dim, L.mx, L.my, L.mz), b = stencil_block(L.dim);
+        const dim3 g = stencil_grid<T>(L.dim, L.mx, L.my, L.mz), b = stencil_block(L.dim);
         if (nb) *nb = nblocks(g);
         if (Out::NDOT && nblocks(g) > cx->npart_cap)
             return set_error(PETSC_ERR_LIB, "internal: partial-sum buffer too small");
@@ -346,7 +351,7 @@ struct Engine {
     int prolong_smooth(int l, const T* base, T cz, const T* xc, Out out, double passes, long* nb, bool jac) {
         const LevelInfo& L = (*lv)[l];
         const LevelInfo& C = (*lv)[l - 1];
-        const dim3 g = stencil_grid(L.dim, L.mx, L.my, L.mz), b = stencil_block(L.dim);
+        const dim3 g = pair_grid(L.dim, L.mx, L.my, L.mz), b = stencil_block(L.dim);
         if (nb) *nb = nblocks(g);
         if (Out::NDOT && nblocks(g) > cx->npart_cap)
             return set_error(PETSC_ERR_LIB, "internal: partial-sum buffer too small");
@@ -833,9 +838,10 @@ struct OutIdiag {   // stores the Jacobi factor of every row
     __device__ __forceinline__ void point(long q, double, double, double idg, double&, double&) const {
         id[q] = idg;
     }
-    __device__ __forceinline__ void point2(long q, double, double, double, double, double ia, double ib,
-                                           double&, double&) const {
-        st2(id + q, ia, ib);
+    template <int V>
+    __device__ __forceinline__ void pointv(long q, const TV<double, V>&, const TV<double, V>&,
+                                           const TV<double, V>& idg, double&, double&) const {
+        stv<double, V>(id + q, idg);
     }
 };
 
@@ -1020,7 +1026,7 @@ static int estimate_level(p4g_solver_s* s, int l) {
     const bool jac = s->jacobi != 0;
     OpConst<double> oc;
     OpVar<double> ov;
-    const dim3 g = stencil_grid(L.dim, L.mx, L.my, L.mz), bk = stencil_block(L.dim);
+    const dim3 g = stencil_grid<double>(L.dim, L.mx, L.my, L.mz), bk = stencil_block(L.dim);
     const long nbs = nblocks(g);
     if (!L.var) {
         oc.d = L.d; oc.db = L.db; oc.sx = L.sx; oc.sy = L.sy; oc.sz = L.sz;
@@ -1263,7 +1269,7 @@ int solver_setup(p4g_solver_s* s) {
     if (!s->cx.stream) HIP_TRY(hipStreamCreate(&s->cx.stream));
     // partial-sum capacity: the largest grid of blocks of any fused pass
     const LevelInfo& F = s->lv.back();
-    long cap = std::max<long>(VBLOCKS, nblocks(stencil_grid(F.dim, F.mx, F.my, F.mz)));
+    long cap = std::max<long>(VBLOCKS, nblocks(pair_grid(F.dim, F.mx, F.my, F.mz)));
     if (!s->cx.part) {
         HIP_TRY(hipMalloc(&s->cx.part, 2 * cap * sizeof(double)));
         HIP_TRY(hipMalloc(&s->cx.part_rr, 2 * cap * sizeof(double)));
@@ -1316,7 +1322,7 @@ int solver_setup(p4g_solver_s* s) {
 template <class Src, class Out>
 static int fine64(p4g_solver_s* s, Src src, Out out, int cls, double bytes, long* nb) {
     const LevelInfo& F = s->lv.back();
-    const dim3 g = stencil_grid(F.dim, F.mx, F.my, F.mz), bk = stencil_block(F.dim);
+    const dim3 g = stencil_grid<double>(F.dim, F.mx, F.my, F.mz), bk = stencil_block(F.dim);
     if (nb) *nb = nblocks(g);
     hipStream_t st = s->cx.stream;
     double* part = s->cx.part;
@@ -1834,6 +1840,11 @@ int p4g_memcpy_htod(void* dst, const void* src, size_t bytes) {
 
 int p4g_memcpy_dtoh(void* dst, const void* src, size_t bytes) {
     HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int p4g_memcpy_dtod(void* dst, const void* src, size_t bytes) {
+    HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
     return 0;
 }
 

@@ -50,6 +50,9 @@ CASES = {
     "2d_pc_jacobi": "-fsh_dim 2 -da_refine 3 -pc_type jacobi -ksp_rtol 1e-10",
     "2d_pc_none": "-fsh_dim 2 -da_refine 2 -pc_type none -ksp_rtol 1e-10",
     "3d_one_level": "-fsh_dim 3 -fsh_problem manusin -da_refine 2 -pc_type mg -pc_mg_levels 1",
+    "3d_richardson3": ("-fsh_dim 3 -fsh_problem manusin -da_refine 4 -pc_type mg -mg_levels_ksp_type "
+                       "richardson -mg_levels_pc_type jacobi -mg_levels_ksp_max_it 3 "
+                       "-mg_levels_ksp_richardson_scale 0.7 -ksp_rtol 1e-10"),
 }
 
 
@@ -163,3 +166,22 @@ def test_mixed_matches_fp64(refine):
     assert _rel(u, ou) < 1e-7
     print("mixed: outer its", res.its, "inner its", res.inner_its, "rel diff to FP64 oracle",
           _rel(u, ou))
+
+
+@pytest.mark.parametrize("variant", ["-p4g_mixed_x64 0", "-p4g_fp32_diff_form 0", "-p4g_reliable_delta 0.01",
+                                     "-fsh_problem varcoef"])
+def test_mixed_variants(variant):
+    # every mixed-precision variant converges on the FP64 true residual to the
+    # same discrete solution as the FP64 oracle
+    base = ("-fsh_dim 3 -fsh_problem manusin -fsh_noise_sigma 1e-3 -da_refine 4 %s -pc_mg_levels 4 "
+            "-mg_levels_ksp_max_it 2 -ksp_rtol 1e-10 %s" % (MG, variant))
+    print("case", variant, flush=True)
+    rc, text, res, u = capi.fish_run(base + " -p4g_precision mixed", want_u=True)
+    assert rc == 0, text
+    orc, otext, ores, ou = O.fish_run(base + " -ksp_norm_type unpreconditioned", want_u=True)
+    assert orc == 0
+    assert res.reason > 0
+    assert res.true_relres <= 1e-10 * 1.0001
+    assert abs(res.errinf - ores.errinf) < 1e-4 * ores.errinf
+    assert _rel(u, ou) < 1e-7
+    print("  its", res.its, "oracle its", ores.its, "replacements", res.inner_its, flush=True)
