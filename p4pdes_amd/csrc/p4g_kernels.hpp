@@ -284,35 +284,28 @@ struct SrcDirX {
     }
 };
 
-// u = c (r - alpha w): the CG residual update fused into the first kernel
-// that reads the residual; the new residual r - alpha w is stored (put).
+// u = r - alpha w, stored to rn: the CG residual update fused into the
+// first kernel that reads the residual (the scale of a zero-guess smoothing
+// step is applied by OutScaleU, so the stored value is the operand itself).
 template <typename T>
-struct SrcScaledUpd {
+struct SrcUpd {
     const T* __restrict__ r;
     const T* __restrict__ w;
     const double* __restrict__ alpha;  // device scalar
-    T c;
     T* __restrict__ rn;
-    __device__ __forceinline__ T load(long q) const { return c * (r[q] - T(*alpha) * w[q]); }
+    __device__ __forceinline__ T load(long q) const { return r[q] - T(*alpha) * w[q]; }
     template <int V>
     __device__ __forceinline__ TV<T, V> loadv(long q) const {
         const T al = T(*alpha);
         TV<T, V> rr = ldv<T, V>(r + q);
         const TV<T, V> ww = ldv<T, V>(w + q);
 #pragma unroll
-        for (int e = 0; e < V; ++e) rr.v[e] = c * (rr.v[e] - al * ww.v[e]);
+        for (int e = 0; e < V; ++e) rr.v[e] -= al * ww.v[e];
         return rr;
     }
-    __device__ __forceinline__ void put(long q, T) const { rn[q] = r[q] - T(*alpha) * w[q]; }
+    __device__ __forceinline__ void put(long q, T v) const { rn[q] = v; }
     template <int V>
-    __device__ __forceinline__ void putv(long q, const TV<T, V>&) const {
-        const T al = T(*alpha);
-        TV<T, V> rr = ldv<T, V>(r + q);
-        const TV<T, V> ww = ldv<T, V>(w + q);
-#pragma unroll
-        for (int e = 0; e < V; ++e) rr.v[e] = rr.v[e] - al * ww.v[e];
-        stv<T, V>(rn + q, rr);
-    }
+    __device__ __forceinline__ void putv(long q, const TV<T, V>& u) const { stv<T, V>(rn + q, u); }
 };
 
 // u = x + alpha p (X wide, P narrow), stored to xnew: a pending iterate
@@ -438,8 +431,31 @@ struct OutResidCvt {                  // r = b - A u in T, stored as R ; acc0 +=
     }
 };
 
+// The operand u was formed unscaled: the inner output sees c u and c A u
+// (the zero-guess smoothing iterate c b of an unscaled residual source).
+template <typename T, class Out>
+struct OutScaleU {
+    T c;
+    Out inner;
+    static constexpr int NDOT = Out::NDOT;
+    __device__ __forceinline__ void point(long q, T u, T Au, T idg, double& a0, double& a1) const {
+        inner.point(q, c * u, c * Au, idg, a0, a1);
+    }
+    template <int V>
+    __device__ __forceinline__ void pointv(long q, const TV<T, V>& u, const TV<T, V>& Au, const TV<T, V>& idg,
+                                           double& a0, double& a1) const {
+        TV<T, V> cu, cAu;
+#pragma unroll
+        for (int e = 0; e < V; ++e) {
+            cu.v[e] = c * u.v[e];
+            cAu.v[e] = c * Au.v[e];
+        }
+        inner.template pointv<V>(q, cu, cAu, idg, a0, a1);
+    }
+};
+
 // Wraps an output without dot products and adds acc0 += rn.rn at every
-// point (rn: the residual stored by SrcScaledUpd in the same thread).
+// point (rn: the residual stored by SrcUpd in the same thread).
 template <typename T, class Out>
 struct OutRR {
     const T* __restrict__ rn;
@@ -917,10 +933,11 @@ struct OutStoreU {
 // A lane owns coarse column I = 1 + 64 bx + lane and the fine pair
 // (2I-1, 2I) -- the lane mapping of stencil_vec with V = 2 -- and the lane at I = 1
 // also fine 0.  The coarse values of I-1 and I+1 come from the neighbouring
-// lanes.  3D: a thread owns fine row j and marches fine planes k; t at rows
-// j-1, j, j+1 is interpolated from the coarse rows of the two coarse planes
-// k>>1 and (k>>1)+1, which stay in registers.  2D: a thread marches fine
-// rows.  Loops are uniform per wave.  Grid: that of stencil_vec with V = 2.
+// lanes.  3D: a wave owns fine row j and marches fine planes k; t on its row
+// is interpolated from the coarse rows of the two coarse planes k>>1 and
+// (k>>1)+1, which stay in registers, and rows j-1, j+1 are exchanged through
+// LDS (one barrier per plane; the block-uniform loop is run by every wave).
+// 2D: a thread marches fine rows.  Grid: that of stencil_vec with V = 2.
 template <typename T, int DIM, class Op, bool ZG, class Out>
 __device__ __forceinline__ void prolong_smooth_body(int bx, int by, int bz, int mx, int my, int mz, int mxc,
                                                     int myc, int kch, const Op& op, const T* __restrict__ xb,
@@ -1006,24 +1023,30 @@ __device__ __forceinline__ void prolong_smooth_body(int bx, int by, int bz, int 
         }
         return;
     }
-    // 3D
-    const int j = by * TY + threadIdx.y;
-    if (j >= my) return;
+    // 3D: each wave forms t on its own row; rows j-1, j+1 of plane k come
+    // from the neighbouring waves through LDS (the block's first and last
+    // wave form the halo row outside the block themselves).  One barrier per
+    // plane, LDS double-buffered by plane parity; every wave of the block
+    // runs the same loop.
+    __shared__ T2<T> tbuf[2][TY + 2][TX];
+    const int ty = threadIdx.y;
+    const int j = by * TY + ty;
+    const bool rowok = j < my;           // uniform per wave
     const int k0 = bz * kch, k1 = min(k0 + kch, mz);
-    if (k0 >= k1) return;
+    if (k0 >= k1) return;                // uniform per block
     const int mzc = (mz - 1) / 2 + 1;
     const long sy = mx, sz = (long)mx * my, scz = (long)mxc * myc;
-    // coarse values of fine rows j-1, j, j+1 on coarse plane K
+    const bool lo_edge = ty == 0, hi_edge = ty == TY - 1;
+    // coarse values of fine rows j-1 (Y[0], edge waves), j (Y[1]), j+1 (Y[2],
+    // edge waves) on coarse plane K
     auto yplane = [&](int K, T3<T>* Y) {
         const long pb = (long)K * scz;
         const int J = j >> 1;
         if ((j & 1) == 0) {
             const T3<T> R0 = crow(pb + (long)J * mxc);
-            const T3<T> Rm = (J >= 1) ? crow(pb + (long)(J - 1) * mxc) : zero3;
-            const T3<T> Rp = (J + 1 < myc) ? crow(pb + (long)(J + 1) * mxc) : zero3;
-            Y[0] = mix3(Rm, R0);
             Y[1] = R0;
-            Y[2] = mix3(R0, Rp);
+            if (lo_edge) Y[0] = mix3((J >= 1) ? crow(pb + (long)(J - 1) * mxc) : zero3, R0);
+            if (hi_edge) Y[2] = mix3(R0, (J + 1 < myc) ? crow(pb + (long)(J + 1) * mxc) : zero3);
         } else {
             const T3<T> R0 = crow(pb + (long)J * mxc), R1 = crow(pb + (long)(J + 1) * mxc);
             Y[0] = R0;
@@ -1032,46 +1055,61 @@ __device__ __forceinline__ void prolong_smooth_body(int bx, int by, int bz, int 
         }
     };
     int Ka = k0 >> 1;
-    T3<T> Ya[3], Yb[3];
-    yplane(Ka, Ya);
-    if (Ka + 1 < mzc) yplane(Ka + 1, Yb);
-    else Yb[0] = Yb[1] = Yb[2] = zero3;
+    T3<T> Ya[3] = {zero3, zero3, zero3}, Yb[3] = {zero3, zero3, zero3};
     long q = (long)k0 * sz + (long)j * sy + i0;
-    T3<T> Pc = (k0 & 1) ? mix3(Ya[1], Yb[1]) : Ya[1];
-    T2<T> tc = act ? tpair(Pc, q) : zero2, td = zero2;
-    if (k0 > 0) {
-        T3<T> Pd;
-        if (k0 & 1) {
-            Pd = Ya[1];
-        } else {
-            T3<T> Yp[3];
-            yplane(Ka - 1, Yp);
-            Pd = mix3(Yp[1], Ya[1]);
+    T3<T> Pc = zero3;
+    T2<T> tc = zero2, td = zero2;
+    if (rowok) {
+        yplane(Ka, Ya);
+        if (Ka + 1 < mzc) yplane(Ka + 1, Yb);
+        Pc = (k0 & 1) ? mix3(Ya[1], Yb[1]) : Ya[1];
+        if (act) tc = tpair(Pc, q);
+        if (k0 > 0) {
+            T3<T> Pd;
+            if (k0 & 1) {
+                Pd = Ya[1];
+            } else {
+                T3<T> Yp[3] = {zero3, zero3, zero3};
+                yplane(Ka - 1, Yp);
+                Pd = mix3(Yp[1], Ya[1]);
+            }
+            if (act) td = tpair(Pd, q - sz);
         }
-        if (act) td = tpair(Pd, q - sz);
     }
     const bool jb = (j == 0) || (j == my - 1);
     for (int k = k0; k < k1; ++k, q += sz) {
-        if ((k >> 1) != Ka) {
-            Ka = k >> 1;
-            Ya[0] = Yb[0]; Ya[1] = Yb[1]; Ya[2] = Yb[2];
-            if (Ka + 1 < mzc) yplane(Ka + 1, Yb);
-            else Yb[0] = Yb[1] = Yb[2] = zero3;
-        }
         T3<T> Pu = zero3;
         T2<T> tu = zero2;
-        if (k + 1 < mz) {
-            Pu = (k & 1) ? Yb[1] : mix3(Ya[1], Yb[1]);
-            if (act) tu = tpair(Pu, q + sz);
+        const bool need_sn = rowok && !jb && k != 0 && k != mz - 1;
+        if (rowok) {
+            if ((k >> 1) != Ka) {
+                Ka = k >> 1;
+                Ya[0] = Yb[0]; Ya[1] = Yb[1]; Ya[2] = Yb[2];
+                if (Ka + 1 < mzc) yplane(Ka + 1, Yb);
+                else Yb[0] = Yb[1] = Yb[2] = zero3;
+            }
+            if (k + 1 < mz) {
+                Pu = (k & 1) ? Yb[1] : mix3(Ya[1], Yb[1]);
+                if (act) tu = tpair(Pu, q + sz);
+            }
+            T2<T>* row = tbuf[k & 1][0];
+            if (act) row[(ty + 1) * TX + lane] = tc;
+            if (act && need_sn && lo_edge && j - 1 > 0)
+                row[lane] = tpair((k & 1) ? mix3(Ya[0], Yb[0]) : Ya[0], q - sy);
+            if (act && need_sn && hi_edge && j + 1 < my - 1)
+                row[(TY + 1) * TX + lane] = tpair((k & 1) ? mix3(Ya[2], Yb[2]) : Ya[2], q + sy);
         }
-        T2<T> ts = zero2, tn = zero2;
-        if (!jb && k != 0 && k != mz - 1) {
-            if (j - 1 > 0 && act) ts = tpair((k & 1) ? mix3(Ya[0], Yb[0]) : Ya[0], q - sy);
-            if (j + 1 < my - 1 && act) tn = tpair((k & 1) ? mix3(Ya[2], Yb[2]) : Ya[2], q + sy);
+        __syncthreads();
+        if (rowok) {
+            const T wv = __shfl_up(tc.b, 1, TX), ev = __shfl_down(tc.a, 1, TX);
+            if (act) {
+                const T2<T>* row = tbuf[k & 1][0];
+                const T2<T> ts = (need_sn && j - 1 > 0) ? row[ty * TX + lane] : zero2;
+                const T2<T> tn = (need_sn && j + 1 < my - 1) ? row[(ty + 2) * TX + lane] : zero2;
+                emit(q, j, k, mz, jb, Pc, tc, ts, tn, (k - 1 > 0) ? td : zero2, (k + 1 < mz - 1) ? tu : zero2, wv,
+                     ev);
+            }
         }
-        const T wv = __shfl_up(tc.b, 1, TX), ev = __shfl_down(tc.a, 1, TX);
-        if (act)
-            emit(q, j, k, mz, jb, Pc, tc, ts, tn, (k - 1 > 0) ? td : zero2, (k + 1 < mz - 1) ? tu : zero2, wv, ev);
         td = tc;
         tc = tu;
         Pc = Pu;

@@ -516,12 +516,12 @@ struct Engine {
                 // m == 2 from zero guess: prev iterate is 0, so no prev term
                 if (nu >= 3 && smoother == SM_CHEB) return set_error(PETSC_ERR_LIB, "internal smoother path");
                 if (fine && pend.on && !(last && dots)) {   // b = r - alpha w, formed here and stored to b
-                    const SrcScaledUpd<T> su{pend.r, pend.w, pend.alpha, cb, pend.rn};
-                    const OutCheb<T, false, 0> oc{b, nullptr, dst, T(c0), T(c1), T(c2)};
+                    const SrcUpd<T> su{pend.r, pend.w, pend.alpha, pend.rn};
+                    typedef OutScaleU<T, OutCheb<T, false, 0>> OC;
+                    const OC oc{cb, OutCheb<T, false, 0>{b, nullptr, dst, T(c0), T(c1), T(c2)}};
                     pend.on = false;
                     if (pend.rr)
-                        P4G_TRY(stencil(l, su, OutRR<T, OutCheb<T, false, 0>>{b, oc}, P4G_K_SMOOTH, 4, &pend.nparts,
-                                        jac, cx->part_rr));
+                        P4G_TRY(stencil(l, su, OutRR<T, OC>{b, oc}, P4G_K_SMOOTH, 4, &pend.nparts, jac, cx->part_rr));
                     else
                         P4G_TRY(stencil(l, su, oc, P4G_K_SMOOTH, 4, &pend.nparts, jac));
                 } else if (last && dots) {
@@ -603,13 +603,14 @@ struct Engine {
         if (xzero && nu == 1 && zg_fusable(L)) {
             zgf = true;
             if (l == nlev - 1 && pend.on) {   // b = r - alpha w, formed here and stored to b
-                const SrcScaledUpd<T> su{pend.r, pend.w, pend.alpha, T(s * idg), pend.rn};
+                const SrcUpd<T> su{pend.r, pend.w, pend.alpha, pend.rn};
+                typedef OutScaleU<T, OutResid<T, 0>> OR;
+                const OR orr{T(s * idg), OutResid<T, 0>{b, d.r}};
                 pend.on = false;
                 if (pend.rr)
-                    P4G_TRY(stencil(l, su, OutRR<T, OutResid<T, 0>>{b, OutResid<T, 0>{b, d.r}}, P4G_K_RESID_ZG, 4,
-                                    &pend.nparts, jac, cx->part_rr));
+                    P4G_TRY(stencil(l, su, OutRR<T, OR>{b, orr}, P4G_K_RESID_ZG, 4, &pend.nparts, jac, cx->part_rr));
                 else
-                    P4G_TRY(stencil(l, su, OutResid<T, 0>{b, d.r}, P4G_K_RESID_ZG, 4, &pend.nparts, jac));
+                    P4G_TRY(stencil(l, su, orr, P4G_K_RESID_ZG, 4, &pend.nparts, jac));
             } else {
                 P4G_TRY(stencil(l, SrcScaled<T>{b, T(s * idg)}, OutResid<T, 0>{b, d.r}, P4G_K_RESID_ZG, 2, nullptr,
                                 jac));
