@@ -153,6 +153,15 @@ static GridRef gref_stencil(const LevelInfo& L, long start, int V) {
     return r;
 }
 
+// the fused prolongation + smoothing launch over level L
+static GridRef gref_fused(const LevelInfo& L, long start) {
+    const dim3 g = fused_grid(L.dim, L.mx, L.my, L.mz);
+    GridRef r;
+    r.mx = L.mx; r.my = L.my; r.mz = L.mz; r.kch = kch_fused(L.dim, L.my, L.mz);
+    r.gx = g.x; r.gy = g.y; r.gz = g.z; r.start = start;
+    return r;
+}
+
 // transfer launch over the grid L with lanes over the coarse columns 1..mxc-1
 static GridRef gref_transfer(const LevelInfo& L, int mxc, long start) {
     const dim3 g = transfer_grid(L.dim, mxc, L.my, L.mz);
@@ -322,7 +331,7 @@ static int build_vcycle(p4g_server_s* v, Packer& pk, std::vector<BLaunch>& seq) 
             const double sc = level_scale(s, L);
             const bool fin = q.l == q.r->nlev - 1;
             PSmoothItem m;
-            m.g = gref_stencil(L, s1, 2);
+            m.g = gref_fused(L, s1);
             s1 += gblocks(m.g);
             m.mxc = C.mx;
             m.myc = C.my;
@@ -457,7 +466,7 @@ static int round_once(p4g_server_s* v) {
             }
         }
         const LevelInfo& Ff = s->lv.back();
-        fbeta.push_back(FinItem{r->part, nblocks(pair_grid(Ff.dim, Ff.mx, Ff.my, Ff.mz)), r->S,
+        fbeta.push_back(FinItem{r->part, nblocks(fused_grid(Ff.dim, Ff.mx, Ff.my, Ff.mz)), r->S,
                                 FIN_BETA_PRECOND, r->state == RQ_INIT ? 1 : 0, nullptr});
     }
     auto add = [&](int kind, int dim, const void* data, size_t each, size_t cnt, long blocks) {

@@ -162,17 +162,29 @@ struct Ctx {
 static inline int vblocks(long n) { return (int)std::max(1L, std::min<long>(VBLOCKS, (n + 255) / 256)); }
 
 // launch geometry of the marching stencil kernels (2D/3D: V x points per lane)
-static dim3 stencil_grid_v(int dim, int mx, int my, int mz, int V) {
+static dim3 stencil_grid_k(int dim, int mx, int my, int mz, int V, int kch) {
     if (dim == 1) return dim3((mx - 1 + TX - 1) / TX, 1, 1);
     const int gx = (mx - 1 + V * TX - 1) / (V * TX);
-    if (dim == 3) return dim3(gx, (my + TY - 1) / TY, (mz + kchf(dim, my, mz) - 1) / kchf(dim, my, mz));
-    return dim3(gx, ((my + kchf(dim, my, mz) - 1) / kchf(dim, my, mz) + TY - 1) / TY, 1);
+    if (dim == 3) return dim3(gx, (my + TY - 1) / TY, (mz + kch - 1) / kch);
+    return dim3(gx, ((my + kch - 1) / kch + TY - 1) / TY, 1);
 }
+static dim3 stencil_grid_v(int dim, int mx, int my, int mz, int V) {
+    return stencil_grid_k(dim, mx, my, mz, V, kchf(dim, my, mz));
+}
+// marching chunk of the fused prolongation + smoothing kernel: its per-chunk
+// set-up (coarse planes, the first iterates) is longer, so chunks are twice
+// those of the stencil passes (257^3: 149 us at 8 planes, 153 us at 4;
+// 513^3: 1088 us at 16, 1144 us at 8)
+static inline int kch_fused(int dim, int my, int mz) { return std::min(16, 2 * kchf(dim, my, mz)); }
 template <typename T>
 static dim3 stencil_grid(int dim, int mx, int my, int mz) { return stencil_grid_v(dim, mx, my, mz, VecOf<T>::V); }
-// two points per lane: the fused prolongation + smoothing kernel; the
-// largest stencil grid (partial-sum buffers are sized by it)
+// two points per lane: the largest stencil grid (partial-sum buffers are
+// sized by it)
 static dim3 pair_grid(int dim, int mx, int my, int mz) { return stencil_grid_v(dim, mx, my, mz, 2); }
+// the fused prolongation + smoothing kernel (two points per lane)
+static dim3 fused_grid(int dim, int mx, int my, int mz) {
+    return stencil_grid_k(dim, mx, my, mz, 2, kch_fused(dim, my, mz));
+}
 static dim3 stencil_block(int dim) { return dim == 1 ? dim3(TX, 1, 1) : dim3(TX, TY, 1); }
 static long nblocks(dim3 g) { return (long)g.x * g.y * g.z; }
 // transfers: lanes over coarse columns I = 1..mxc-1, rows and march chunks
@@ -194,7 +206,10 @@ struct Engine {
     int nlev = 0, cycles = 1, smoother = SM_CHEB, nu = 1, jacobi = 1, pctype = PCT_MG;
     double rich_scale = 1.0;
     bool diff_form = false;   // constant-coefficient rows in difference form
-    bool fuse_post = true;    // first post-smoothing step fused with the prolongation
+    // V(1,1): the post-smoothing step fused with the prolongation (with more
+    // steps the prolonged iterate would have to be stored for the Chebyshev
+    // recurrence, and the fused kernel then loses to the separate passes)
+    bool fuse_post = true;
     struct Dev {
         T *x = nullptr, *b = nullptr, *r = nullptr, *t[4] = {nullptr, nullptr, nullptr, nullptr};
         T* kc = nullptr;
@@ -351,13 +366,13 @@ struct Engine {
     int prolong_smooth(int l, const T* base, T cz, const T* xc, Out out, double passes, long* nb, bool jac) {
         const LevelInfo& L = (*lv)[l];
         const LevelInfo& C = (*lv)[l - 1];
-        const dim3 g = pair_grid(L.dim, L.mx, L.my, L.mz), b = stencil_block(L.dim);
+        const dim3 g = fused_grid(L.dim, L.mx, L.my, L.mz), b = stencil_block(L.dim);
         if (nb) *nb = nblocks(g);
         if (Out::NDOT && nblocks(g) > cx->npart_cap)
             return set_error(PETSC_ERR_LIB, "internal: partial-sum buffer too small");
         hipStream_t s = cx->stream;
         const double bytes = passes * L.n * sizeof(T) + C.n * sizeof(T);
-        const int kch = kchf(L.dim, L.my, L.mz);
+        const int kch = kch_fused(L.dim, L.my, L.mz);
         if (L.var) {
             if (L.dim != 3) return set_error(PETSC_ERR_SUP, "variable coefficient needs 3D");
             OpVar<T> o = opv(l, jac);
@@ -628,7 +643,7 @@ struct Engine {
         // x = pre + P x_c, then post-smoothing into x
         T* x0 = zgf ? d.t[0] : xpre;
         const T* pre = zgf ? b : xpre;
-        if (L.dim >= 2 && fuse_post) {
+        if (L.dim >= 2 && fuse_post && nu == 1) {
             // the first post-smoothing step reads the prolonged iterate on the fly
             if (nu == 1) {
                 if (dots) P4G_TRY(post_first<true>(l, zgf, pre, T(s * idg), c.x, b, nullptr, x, T(s), nb));
