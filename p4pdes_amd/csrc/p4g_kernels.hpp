@@ -547,6 +547,33 @@ __device__ __forceinline__ void block_partials(double a0, double a1, double* __r
     }
 }
 
+// ------------------------------------------------------ block placement
+//
+// The dispatcher places workgroup b (x fastest, then y, z) on XCD b mod 8;
+// each XCD has its own L2.  Neighbouring tiles of a marching pass share halo
+// rows and planes, so the launch's tiles are renumbered: XCD x takes the
+// contiguous tile range [x q + min(x, r), ...) of q or q + 1 tiles
+// (q = n / 8, r = n mod 8), visited in dispatch order.  The map is a
+// bijection of [0, n) for every n.
+struct Tile {
+    int bx, by, bz;
+    long id;            // renumbered linear index (partial-sum slot)
+};
+
+__device__ __forceinline__ Tile xcd_tile() {
+    const long n = (long)gridDim.x * gridDim.y * gridDim.z;
+    const long b = blockIdx.x + (long)gridDim.x * (blockIdx.y + (long)gridDim.y * blockIdx.z);
+    const long q = n >> 3, r = n & 7, x = b & 7;
+    const long t = x * q + (x < r ? x : r) + (b >> 3);
+    Tile o;
+    o.id = t;
+    o.bx = (int)(t % gridDim.x);
+    const long u = t / gridDim.x;
+    o.by = (int)(u % gridDim.y);
+    o.bz = (int)(u / gridDim.y);
+    return o;
+}
+
 // ------------------------------------------------------------- stencil
 
 // Marching stencil pass over a 2D or 3D grid: for every vertex, u from Src,
@@ -716,10 +743,9 @@ template <typename T, int DIM, class Op, class Src, class Out>
 __global__ void __launch_bounds__(NT)
 k_stencil(int mx, int my, int mz, int kch, Op op, Src src, Out out, double* __restrict__ part) {
     double a0 = 0.0, a1 = 0.0;
-    stencil_body<T, DIM, Op, Src, Out>(blockIdx.x, blockIdx.y, blockIdx.z, mx, my, mz, kch, op, src, out,
-                                       a0, a1);
-    const long bid = blockIdx.x + (long)gridDim.x * (blockIdx.y + (long)gridDim.y * blockIdx.z);
-    block_partials<Out::NDOT>(a0, a1, part, bid);
+    const Tile tl = xcd_tile();
+    stencil_body<T, DIM, Op, Src, Out>(tl.bx, tl.by, tl.bz, mx, my, mz, kch, op, src, out, a0, a1);
+    block_partials<Out::NDOT>(a0, a1, part, tl.id);
 }
 
 // ------------------------------------------------------------ transfers
@@ -806,7 +832,8 @@ template <typename T, int DIM>
 __global__ void __launch_bounds__(NT)
 k_restrict(int mxf, int myf, int mzf, int mxc, int myc, int mzc, int kch, const T* __restrict__ rf,
            T* __restrict__ bc) {
-    restrict_body<T, DIM>(blockIdx.x, blockIdx.y, blockIdx.z, mxf, myf, mzf, mxc, myc, mzc, kch, rf, bc);
+    const Tile tl = xcd_tile();
+    restrict_body<T, DIM>(tl.bx, tl.by, tl.bz, mxf, myf, mzf, mxc, myc, mzc, kch, rf, bc);
 }
 
 // out = base + P x_c, base = c b (ZG: zero-guess smoothing result) or x.
@@ -890,7 +917,8 @@ template <typename T, int DIM, bool ZG>
 __global__ void __launch_bounds__(NT)
 k_prolong(int mxf, int myf, int mzf, int mxc, int myc, int kch, const T* __restrict__ xb, T c,
           const T* __restrict__ xc, T* __restrict__ out) {
-    prolong_body<T, DIM, ZG>(blockIdx.x, blockIdx.y, blockIdx.z, mxf, myf, mzf, mxc, myc, kch, xb, c, xc, out);
+    const Tile tl = xcd_tile();
+    prolong_body<T, DIM, ZG>(tl.bx, tl.by, tl.bz, mxf, myf, mzf, mxc, myc, kch, xb, c, xc, out);
 }
 
 // ------------------------------------------- fused prolongation + smoothing
@@ -1121,10 +1149,10 @@ __global__ void __launch_bounds__(NT)
 k_prolong_smooth(int mx, int my, int mz, int mxc, int myc, int kch, Op op, const T* __restrict__ xb, T c,
                  const T* __restrict__ xc, Out out, double* __restrict__ part) {
     double a0 = 0.0, a1 = 0.0;
-    prolong_smooth_body<T, DIM, Op, ZG, Out>(blockIdx.x, blockIdx.y, blockIdx.z, mx, my, mz, mxc, myc, kch, op, xb,
-                                             c, xc, out, a0, a1);
-    const long bid = blockIdx.x + (long)gridDim.x * (blockIdx.y + (long)gridDim.y * blockIdx.z);
-    block_partials<Out::NDOT>(a0, a1, part, bid);
+    const Tile tl = xcd_tile();
+    prolong_smooth_body<T, DIM, Op, ZG, Out>(tl.bx, tl.by, tl.bz, mx, my, mz, mxc, myc, kch, op, xb, c, xc, out,
+                                             a0, a1);
+    block_partials<Out::NDOT>(a0, a1, part, tl.id);
 }
 
 // ------------------------------------------------------- coarse solve
