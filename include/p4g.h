@@ -113,7 +113,12 @@ int p4g_da_destroy(p4g_DA *da);
  *       d = 2(sx+sy+sz), with far less rounding on smooth vectors,
  *   -p4g_mixed_x64 1|0 (default 1): 0 keeps the correction since the last
  *       replacement in FP32 and folds it into the FP64 iterate at each
- *       replacement.
+ *       replacement,
+ *   -p4g_fuse_cg auto|1|0 (default auto): the CG residual update fused into
+ *       the first kernel of the multigrid cycle and the direction update
+ *       with the operator apply; auto fuses when one fine-level vector is
+ *       larger than 256 MiB (below that the separate passes run from the
+ *       last-level cache and are faster).
  * Options of PETSc objects the device path does not implement (-pc_type
  * ilu|sor|lu, -mg_levels_pc_type sor, -pc_mg_galerkin, ...) are rejected
  * with PETSC_ERR_SUP. */

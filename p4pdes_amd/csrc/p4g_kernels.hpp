@@ -1256,6 +1256,21 @@ k_dot2(long n, const T* __restrict__ x, const T* __restrict__ y, double* __restr
     }
 }
 
+// p_new = z + beta p_old and the deferred iterate update x += alpha p_old
+// (beta, alpha device scalars; X may be wider than T)
+template <typename T, typename X>
+__global__ void __launch_bounds__(256)
+k_dir_x(long n, const double* __restrict__ beta, const double* __restrict__ alpha, const T* __restrict__ z,
+        const T* __restrict__ pold, T* __restrict__ pnew, X* __restrict__ x) {
+    const T bt = T(*beta);
+    const X al = X(*alpha);
+    for (long q = blockIdx.x * 256L + threadIdx.x; q < n; q += (long)gridDim.x * 256) {
+        const T p = pold[q];
+        pnew[q] = z[q] + bt * p;
+        x[q] += al * X(p);
+    }
+}
+
 // rn = r - a w (a = S[ialpha]); NORM: acc0 += rn.rn
 template <typename T, int NORM>
 __global__ void __launch_bounds__(256)

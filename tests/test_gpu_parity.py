@@ -50,6 +50,13 @@ CASES = {
     "2d_pc_jacobi": "-fsh_dim 2 -da_refine 3 -pc_type jacobi -ksp_rtol 1e-10",
     "2d_pc_none": "-fsh_dim 2 -da_refine 2 -pc_type none -ksp_rtol 1e-10",
     "3d_one_level": "-fsh_dim 3 -fsh_problem manusin -da_refine 2 -pc_type mg -pc_mg_levels 1",
+    "3d_cheb1_fused_cg": ("-fsh_dim 3 -fsh_problem manusin -da_refine 4 %s -mg_levels_ksp_max_it 1 -ksp_rtol 1e-10 "
+                          "-p4g_fuse_cg 1" % MG),
+    "3d_cheb2_fused_cg_unprecond": ("-fsh_dim 3 -fsh_problem manusin -fsh_noise_sigma 1e-3 -da_refine 4 %s "
+                                    "-ksp_norm_type unpreconditioned -ksp_rtol 1e-10 -p4g_fuse_cg 1" % MG),
+    "3d_richardson3_fused_cg": ("-fsh_dim 3 -fsh_problem manusin -da_refine 4 -pc_type mg -mg_levels_ksp_type "
+                                "richardson -mg_levels_pc_type jacobi -mg_levels_ksp_max_it 3 "
+                                "-mg_levels_ksp_richardson_scale 0.7 -ksp_rtol 1e-10 -p4g_fuse_cg 1"),
     "3d_richardson3": ("-fsh_dim 3 -fsh_problem manusin -da_refine 4 -pc_type mg -mg_levels_ksp_type "
                        "richardson -mg_levels_pc_type jacobi -mg_levels_ksp_max_it 3 "
                        "-mg_levels_ksp_richardson_scale 0.7 -ksp_rtol 1e-10"),
@@ -169,7 +176,7 @@ def test_mixed_matches_fp64(refine):
 
 
 @pytest.mark.parametrize("variant", ["-p4g_mixed_x64 0", "-p4g_fp32_diff_form 0", "-p4g_reliable_delta 0.01",
-                                     "-fsh_problem varcoef"])
+                                     "-fsh_problem varcoef", "-p4g_fuse_cg 1", "-p4g_fuse_cg 1 -fsh_problem varcoef"])
 def test_mixed_variants(variant):
     # every mixed-precision variant converges on the FP64 true residual to the
     # same discrete solution as the FP64 oracle
