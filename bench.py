@@ -263,7 +263,7 @@ def main():
                        key=lambda c: by_cls[c])
         breakdown = {capi.KERNEL_NAMES[c]: round(by_cls[c], 4) for c in by_cls}
         s.reset_stats()
-        s.set_timing(dominant, True)
+        s.set_timing(dominant, True, fine_only=True)   # one event pair per cycle
         print("bench: timing %d solves" % a.steps, file=sys.stderr, flush=True)
         total, lat, reps = time_solves(s, a.steps, a.warmup)
         s.set_timing(dominant, False)

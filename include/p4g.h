@@ -160,6 +160,8 @@ enum {
     P4G_K_OTHER,
     P4G_K_COUNT
 };
+/* on: 0 off, 1 every launch of the class, 2 only its launches on the finest
+ * level (one event pair per cycle instead of one per level) */
 int p4g_solver_set_timing(p4g_Solver s, int kernel_class, int on);
 /* launches, summed device ms, and algorithmic bytes moved, for the class
  * (optionally only launches on the finest level: fine_only = 1) */

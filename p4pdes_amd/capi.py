@@ -209,8 +209,10 @@ class Session:
         check(lib().p4g_solver_get_report(self.solver, ctypes.byref(rep)))
         return rep
 
-    def set_timing(self, cls, on=True):
-        check(lib().p4g_solver_set_timing(self.solver, cls, 1 if on else 0))
+    def set_timing(self, cls, on=True, fine_only=False):
+        """HIP-event timing of a kernel class: every launch, or with
+        fine_only only the launches on the finest level."""
+        check(lib().p4g_solver_set_timing(self.solver, cls, (2 if fine_only else 1) if on else 0))
 
     def reset_stats(self):
         check(lib().p4g_solver_reset_stats(self.solver))

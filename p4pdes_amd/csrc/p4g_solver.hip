@@ -89,7 +89,7 @@ struct KStat {
 };
 
 struct Timing {
-    bool on[P4G_K_COUNT] = {};
+    int on[P4G_K_COUNT] = {};   // 0 off, 1 every launch, 2 finest-level launches only
     KStat st[P4G_K_COUNT];
     struct Pending { hipEvent_t a, b; int cls; bool fine; };
     std::vector<Pending> pending;
@@ -134,7 +134,7 @@ struct Ctx {
         s.launches++;
         s.bytes += bytes;
         if (fine) { s.fine_launches++; s.fine_bytes += bytes; }
-        if (tm.on[cls]) {
+        if (tm.on[cls] == 1 || (tm.on[cls] == 2 && fine)) {
             hipEvent_t a = tm.get(), b = tm.get();
             hipEventRecord(a, stream);
             f();
@@ -155,6 +155,19 @@ struct Ctx {
     int readS(double* host) {
         HIP_TRY(hipMemcpyAsync(host, S, 32 * sizeof(double), hipMemcpyDeviceToHost, stream));
         HIP_TRY(hipStreamSynchronize(stream));
+        return 0;
+    }
+    // copy the scalar block to host and mark the point; work queued after
+    // readS_begin keeps the device busy while the host waits in readS_end
+    hipEvent_t sev = nullptr;
+    int readS_begin(double* host) {
+        if (!sev) HIP_TRY(hipEventCreateWithFlags(&sev, hipEventDisableTiming));
+        HIP_TRY(hipMemcpyAsync(host, S, 32 * sizeof(double), hipMemcpyDeviceToHost, stream));
+        HIP_TRY(hipEventRecord(sev, stream));
+        return 0;
+    }
+    int readS_end() {
+        HIP_TRY(hipEventSynchronize(sev));
         return 0;
     }
 };
@@ -754,23 +767,26 @@ struct Engine {
         const double ttol = std::max(rtol * dp, atol);
         if (std::isnan(dp) || std::isinf(dp)) { o.reason = P4G_DIVERGED_NANORINF; return 0; }
         if (dp <= ttol) { o.reason = dp < atol ? P4G_CONVERGED_ATOL : P4G_CONVERGED_RTOL; return 0; }
+        if (hs[S_RZ] == 0.0) { o.its = 1; o.reason = P4G_CONVERGED_ATOL; return 0; }
         T* pold = pa;
         T* pnew = pb;
         T* rc = r;      // current residual; the update r - alpha w goes to rx
         T* rx = r2;
         double dpi_old = 0.0;
-        // x += alpha p is deferred into the next direction update (SrcDirX);
-        // alpha = 0 before the first iteration
+        // x += alpha p is deferred into the next direction update; alpha = 0
+        // before the first iteration.  The direction update of iteration i+1
+        // needs no host decision (beta is formed on the device), so it is
+        // queued before the host waits for iteration i's scalars: the
+        // convergence test overlaps it, and on exit it has applied the last
+        // deferred update, so x is complete.
         k_fill<double><<<1, 256, 0, st>>>(1, 0.0, Sblk + S_ALPHA);
         LAUNCH_CHECK();
+        long nbs = 0;
+        P4G_TRY(direction<T>(z, pold, pnew, x, Sblk + S_BCOEF, Sblk + S_ALPHA, &nbs));
+        P4G_TRY(cx->finalize(nbs, Sblk, FIN_ALPHA, 0));
         int i = 0;
         for (; i < maxit; ++i) {
             o.its = i + 1;
-            if (hs[S_RZ] == 0.0) { o.reason = P4G_CONVERGED_ATOL; break; }
-            // p = z + (beta/betaold) p ; w = A p ; (p, w) ; x += alpha_prev p_prev
-            long nbs = 0;
-            P4G_TRY(direction<T>(z, pold, pnew, x, Sblk + S_BCOEF, Sblk + S_ALPHA, &nbs));
-            P4G_TRY(cx->finalize(nbs, Sblk, FIN_ALPHA, 0));
             std::swap(pold, pnew);   // pold now holds the current direction
             // r - alpha w, fused into the first kernel of the preconditioner
             pend.r = rc;
@@ -783,7 +799,11 @@ struct Engine {
             std::swap(rc, rx);
             if (normtype == NT_UNPRECOND) P4G_TRY(cx->finalize(pend.nparts, Sblk, FIN_RR, 0, cx->part_rr));
             P4G_TRY(cx->finalize(nb, Sblk, fmode, 0));
-            P4G_TRY(cx->readS(cx->hS));
+            P4G_TRY(cx->readS_begin(cx->hS));
+            // next direction (applies x += alpha_i p_i)
+            P4G_TRY(direction<T>(z, pold, pnew, x, Sblk + S_BCOEF, Sblk + S_ALPHA, &nbs));
+            P4G_TRY(cx->finalize(nbs, Sblk, FIN_ALPHA, 0));
+            P4G_TRY(cx->readS_end());
             const double dpi = hs[S_PW];
             if (dpi == 0.0 || (i > 0 && ((dpi > 0) != (dpi_old > 0)))) {
                 o.reason = P4G_DIVERGED_INDEFINITE_MAT;
@@ -796,12 +816,13 @@ struct Engine {
             if (std::isnan(dp) || std::isinf(dp)) { o.reason = P4G_DIVERGED_NANORINF; break; }
             if (dp <= ttol) { o.reason = dp < atol ? P4G_CONVERGED_ATOL : P4G_CONVERGED_RTOL; break; }
             if (dp >= dtol * o.rnorm0) { o.reason = P4G_DIVERGED_DTOL; break; }
+            if (hs[S_RZ] == 0.0 && i + 1 < maxit) {   // beta = 0: PETSc stops at the top of the next iteration
+                o.its = i + 2;
+                o.reason = P4G_CONVERGED_ATOL;
+                break;
+            }
         }
         if (i >= maxit && !o.reason) o.reason = P4G_DIVERGED_ITS;
-        // the deferred update of the last iteration
-        P4G_TRY(cx->launch(P4G_K_CG_UPDATE, true, 3.0 * n * sizeof(T), [&] {
-            k_x_update<T, T><<<vblocks(n), 256, 0, st>>>(n, Sblk, S_ALPHA, x, pold);
-        }));
         return 0;
     }
 };
@@ -1448,16 +1469,22 @@ static int mixed_solve(p4g_solver_s* s, const double* b, double* x) {
     int i = 0, nrep = 0;
     float* r32n = e.r2;               // x64: the residual update goes to r32n
     bool xpend = false;               // x64: x += alpha p of the last iteration not yet applied
-    for (; i < s->maxit && !R.reason; ++i) {
-        R.its = i + 1;
-        if (hs[S_RZ] == 0.0) replace_now = true;   // exact breakdown: certify by replacement
+    // direction update (with the deferred FP64 iterate update when one is
+    // pending); the next one is queued before the host waits for the
+    // scalars of the current iteration (it needs no host decision)
+    auto direction = [&]() -> int {
         long nbs = 0;
-        if (x64 && xpend)   // direction update with the deferred FP64 iterate update
+        if (x64 && xpend)
             P4G_TRY(e.direction<double>(e.z, pold, pnew, xcur, S + S_BCOEF, S + S_ALPHA, &nbs));
         else
             P4G_TRY(e.stencil(s->nlev - 1, SrcDir<float>{e.z, pold, pnew, S + S_BCOEF}, OutApplyDot<float>{e.w},
                               P4G_K_STENCIL_DOT, 4, &nbs, false));
-        P4G_TRY(cx->finalize(nbs, S, FIN_ALPHA, 0));
+        return cx->finalize(nbs, S, FIN_ALPHA, 0);
+    };
+    if (!R.reason) P4G_TRY(direction());
+    for (; i < s->maxit && !R.reason; ++i) {
+        R.its = i + 1;
+        if (hs[S_RZ] == 0.0) replace_now = true;   // exact breakdown: certify by replacement
         std::swap(pold, pnew);
         const bool replaced = replace_now;
         if (x64) {
@@ -1507,7 +1534,9 @@ static int mixed_solve(p4g_solver_s* s, const double* b, double* x) {
             P4G_TRY(e.pc_apply_dots(r32, e.z, &nb));
         }
         P4G_TRY(cx->finalize(nb, S, FIN_BETA_PRECOND, 0));
-        P4G_TRY(cx->readS(cx->hS));
+        P4G_TRY(cx->readS_begin(cx->hS));
+        P4G_TRY(direction());   // applies a pending x += alpha p: x is complete on every exit
+        P4G_TRY(cx->readS_end());
         const double dpi = hs[S_PW];
         if (dpi == 0.0 || (i > 0 && ((dpi > 0) != (dpi_old > 0)))) { R.reason = P4G_DIVERGED_INDEFINITE_MAT; break; }
         dpi_old = dpi;
@@ -1524,11 +1553,6 @@ static int mixed_solve(p4g_solver_s* s, const double* b, double* x) {
         if (replaced && rn <= target) { R.reason = rn < s->atol ? P4G_CONVERGED_ATOL : P4G_CONVERGED_RTOL; break; }
         if (rn >= s->dtol * bnorm) { R.reason = P4G_DIVERGED_DTOL; break; }
         if (!replaced && (rn <= target || rn * rn <= delta * delta * rr_last)) replace_now = true;
-    }
-    if (x64 && xpend) {   // the deferred update of the last iteration
-        P4G_TRY(cx->launch(P4G_K_CG_UPDATE, true, 20.0 * n, [&] {
-            k_x_update<double, float><<<gb, 256, 0, st>>>(n, S, S_ALPHA, xcur, pold);
-        }));
     }
     if (!R.reason) R.reason = P4G_DIVERGED_ITS;
     R.inner_its = nrep;   // number of FP64 residual replacements
@@ -1832,7 +1856,8 @@ int p4g_solver_destroy(p4g_Solver* s) {
 
 int p4g_solver_set_timing(p4g_Solver s, int cls, int on) {
     if (!s || cls < 0 || cls >= P4G_K_COUNT) return set_error(PETSC_ERR_ARG_WRONG, "p4g_solver_set_timing");
-    s->cx.tm.on[cls] = on != 0;
+    if (on < 0 || on > 2) return set_error(PETSC_ERR_ARG_OUTOFRANGE, "p4g_solver_set_timing: on must be 0, 1 or 2");
+    s->cx.tm.on[cls] = on;
     return 0;
 }
 
