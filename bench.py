@@ -58,6 +58,20 @@ def time_solves(sess, steps, warmup):
     return total, lat, reps
 
 
+TRAFFIC_FILE = "profiles/hbm_traffic_fp64_257cubed.json"
+
+
+def measured_traffic(cls_name):
+    """HBM bytes per finest-level launch of a kernel class on the 257^3
+    problem, from the calibrated rocprofv3 FETCH_SIZE / WRITE_SIZE passes
+    recorded in TRAFFIC_FILE (None when absent)."""
+    path = os.path.join(ROOT, TRAFFIC_FILE)
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f).get("kernels", {}).get(cls_name)
+
+
 def kernel_roofline(sess, cls):
     """Average duration and algorithmic bytes per launch of kernel class
     `cls` on the finest level, from HIP events recorded around each launch
@@ -282,10 +296,13 @@ def main():
                      "grid": [257, 257, 257], "levels": 7, "its": rep.its,
                      "true_relres": rep.true_relres, "errinf": einf,
                      "errinf_closed_form": 1.254995e-05, "parallelism": "single GPU"}
+    traffic = measured_traffic(capi.KERNEL_NAMES[dominant])
     out["roofline"] = {"kernel": capi.KERNEL_NAMES[dominant], "bound": "hbm",
                        "achieved": roof["gbs"] if roof else None, "peak": HBM_PEAK_GBS,
                        "unit": "GB/s", "frac": (roof["gbs"] / HBM_PEAK_GBS) if roof else None,
-                       "traffic": None,
+                       "traffic": traffic["traffic_bytes"] if traffic else None,
+                       "traffic_kernel": traffic["kernel"] if traffic else None,
+                       "traffic_source": TRAFFIC_FILE if traffic else None,
                        "avg_ms": roof["avg_ms"] if roof else None,
                        "algorithmic_bytes_per_launch": roof["bytes_per_launch"] if roof else None,
                        "launches": roof["launches"] if roof else None}
