@@ -28,6 +28,10 @@ C2 = ("-fsh_dim 3 -fsh_problem manusin -da_refine 7 -pc_mg_levels 7 %s "
 # precisions converge on the true FP64 residual |b - A x|_2 / |b|_2 <= 1e-10
 C3 = ("-fsh_dim 3 -fsh_problem manusin -fsh_noise_sigma 1e-3 -fsh_noise_seed 0 -da_refine 8 "
       "-pc_mg_levels 8 %s -mg_levels_ksp_max_it 2 -ksp_rtol 1e-10" % MG)
+# C4: 257^3 vertices, cell coefficient k = exp(G), G a seeded Gaussian random field (random
+# Fourier features, sigma 1, correlation length 0.1); manufactured solution sin sin sin
+C4 = ("-fsh_dim 3 -fsh_problem varcoef -da_refine 7 -pc_mg_levels 7 %s "
+      "-mg_levels_ksp_max_it 1 -ksp_rtol 1e-10" % MG)
 HBM_PEAK_GBS = 8000.0
 METRIC = "3D Poisson DOF/s to rtol 1e-10, mixed vs FP64 MG-CG; p50/p99 solve latency"
 
@@ -271,6 +275,7 @@ def main():
             s.set_timing(c, True)
         s.solve()
         by_cls = {c: s.kernel_stats(c)[1] for c in range(capi.K_COUNT)}
+        solve_bytes = sum(s.kernel_stats(c)[2] for c in range(capi.K_COUNT))   # algorithmic, one solve
         for c in range(capi.K_COUNT):
             s.set_timing(c, False)
         dominant = max((c for c in by_cls if c not in (capi.K_OTHER, capi.K_REDUCE)),
@@ -307,6 +312,9 @@ def main():
                        "algorithmic_bytes_per_launch": roof["bytes_per_launch"] if roof else None,
                        "launches": roof["launches"] if roof else None}
     out["kernel_ms_per_solve"] = breakdown
+    # whole solve: algorithmic bytes of every launch (all levels) / solve time
+    out["solve_algorithmic_gb"] = solve_bytes / 1e9
+    out["solve_effective_gbs"] = solve_bytes / (ms * 1e-3) / 1e9
     out["stencil_apply_257"] = apply257
 
     # ---- C3: 513^3 mixed vs FP64 -----------------------------------------
@@ -336,6 +344,25 @@ def main():
                           "FP64 iterative refinement around FP32 PCG + FP32 V-cycle; both to "
                           "|b-Ax|_2/|b|_2 <= 1e-10")
         out["mixed_vs_fp64_513"] = c3
+
+    # ---- C4: 257^3 variable coefficient, FP64 and mixed ---------------------
+    if not a.no_513:
+        c4 = {}
+        for prec in ("double", "mixed"):
+            extra = " -ksp_norm_type unpreconditioned" if prec == "double" else " -p4g_precision mixed"
+            print("bench: 257^3 variable coefficient %s" % prec, file=sys.stderr, flush=True)
+            with capi.Session(C4 + extra) as s:
+                tot, l4, r4 = time_solves(s, 3, 1)
+                m4 = tot / 3 * 1e3
+                e4, _ = s.errors()
+                c4[prec] = {"dof_per_s": s.n / (m4 * 1e-3), "ms_per_solve": m4, "its": r4[-1].its,
+                            "true_relres": r4[-1].true_relres, "errinf_vs_exact": e4}
+        c4["speedup_mixed_over_fp64"] = c4["mixed"]["dof_per_s"] / c4["double"]["dof_per_s"]
+        c4["workload"] = ("3D -div(k grad u) = f on 257^3 vertices (256^3 cells), k = exp(G), G a seeded "
+                          "Gaussian random field (random Fourier features, sigma 1, correlation length 0.1), "
+                          "face coefficient = mean of the 4 cells on its edge, coarse k by averaging; "
+                          "7-level V(1,1) Chebyshev/Jacobi; both to |b-Ax|_2/|b|_2 <= 1e-10")
+        out["varcoef_257"] = c4
 
     # ---- C5: serving stream ------------------------------------------------
     if not a.no_serve:

@@ -192,3 +192,24 @@ def test_mixed_variants(variant):
     assert abs(res.errinf - ores.errinf) < 1e-4 * ores.errinf
     assert _rel(u, ou) < 1e-7
     print("  its", res.its, "oracle its", ores.its, "replacements", res.inner_its, flush=True)
+
+
+def test_c4_varcoef_257_mixed_and_fp64():
+    # C4 at full size: both precisions reach the FP64 true residual, agree with
+    # each other, and the discretization error falls ~4x from 129^3 (O(h^2))
+    base = ("-fsh_dim 3 -fsh_problem varcoef -da_refine %d -pc_mg_levels %d %s -mg_levels_ksp_max_it 1 "
+            "-ksp_rtol 1e-10")
+    res = {}
+    for r in (6, 7):
+        for prec in ("-ksp_norm_type unpreconditioned", "-p4g_precision mixed"):
+            rc, text, fr, u = capi.fish_run(base % (r, r, MG) + " " + prec, want_u=True)
+            assert rc == 0, text
+            assert fr.reason > 0 and fr.true_relres <= 1e-10 * 1.0001, (r, prec, fr.true_relres)
+            res[(r, prec)] = (fr, u)
+            print("refine", r, prec, "its", fr.its, "errinf", fr.errinf, flush=True)
+    for r in (6, 7):
+        (f64, u64), (fm, um) = res[(r, "-ksp_norm_type unpreconditioned")], res[(r, "-p4g_precision mixed")]
+        assert _rel(um, u64) < 1e-7
+        assert abs(fm.errinf - f64.errinf) < 1e-4 * f64.errinf
+    ratio = res[(6, "-ksp_norm_type unpreconditioned")][0].errinf / res[(7, "-ksp_norm_type unpreconditioned")][0].errinf
+    assert 3.0 < ratio < 5.0, ratio
