@@ -83,6 +83,28 @@ struct OutChebR {
             a1 += double(v) * double(v);
         }
     }
+    __device__ __forceinline__ void pointb(long q, T u, T Au, T idg, T bq, double& a0, double& a1) const {
+        const T v = c1 * u + c2 * (idg * (bq - Au));
+        y[q] = v;
+        if (dots) {
+            a0 += double(v) * double(bq);
+            a1 += double(v) * double(v);
+        }
+    }
+    template <int V>
+    __device__ __forceinline__ void pointvb(long q, const TV<T, V>& u, const TV<T, V>& Au, const TV<T, V>& idg,
+                                            const TV<T, V>& bb, double& a0, double& a1) const {
+        TV<T, V> v;
+#pragma unroll
+        for (int e = 0; e < V; ++e) v.v[e] = c1 * u.v[e] + c2 * (idg.v[e] * (bb.v[e] - Au.v[e]));
+        stv<T, V>(y + q, v);
+        if (dots)
+#pragma unroll
+            for (int e = 0; e < V; ++e) {
+                a0 += double(v.v[e]) * double(bb.v[e]);
+                a1 += double(v.v[e]) * double(v.v[e]);
+            }
+    }
     template <int V>
     __device__ __forceinline__ void pointv(long q, const TV<T, V>& u, const TV<T, V>& Au, const TV<T, V>& idg,
                                            double& a0, double& a1) const {

@@ -516,6 +516,35 @@ struct OutCheb {
                 a1 += double(v.v[e]) * double(v.v[e]);
             }
     }
+    // the same with b given by the caller (already loaded)
+    __device__ __forceinline__ void pointb(long q, T u, T Au, T idg, T bq, double& a0, double& a1) const {
+        T v = c1 * u + c2 * (idg * (bq - Au));
+        if (PREV) v += c0 * prev[q];
+        y[q] = v;
+        if (DOTS) {
+            a0 += double(v) * double(bq);
+            a1 += double(v) * double(v);
+        }
+    }
+    template <int V>
+    __device__ __forceinline__ void pointvb(long q, const TV<T, V>& u, const TV<T, V>& Au, const TV<T, V>& idg,
+                                            const TV<T, V>& bb, double& a0, double& a1) const {
+        TV<T, V> v;
+#pragma unroll
+        for (int e = 0; e < V; ++e) v.v[e] = c1 * u.v[e] + c2 * (idg.v[e] * (bb.v[e] - Au.v[e]));
+        if (PREV) {
+            const TV<T, V> pp = ldv<T, V>(prev + q);
+#pragma unroll
+            for (int e = 0; e < V; ++e) v.v[e] += c0 * pp.v[e];
+        }
+        stv<T, V>(y + q, v);
+        if (DOTS)
+#pragma unroll
+            for (int e = 0; e < V; ++e) {
+                a0 += double(v.v[e]) * double(bb.v[e]);
+                a1 += double(v.v[e]) * double(v.v[e]);
+            }
+    }
 };
 
 // ------------------------------------------------------------ reductions
@@ -936,24 +965,6 @@ __device__ __forceinline__ T3<T> mix3(const T3<T>& a, const T3<T>& b) {
     return T3<T>{h * (a.w + b.w), h * (a.v + b.v), h * (a.e + b.e)};
 }
 
-// Output wrapper that also stores the operand u (the prolonged iterate).
-template <typename T, class Out>
-struct OutStoreU {
-    T* __restrict__ t;
-    Out inner;
-    static constexpr int NDOT = Out::NDOT;
-    __device__ __forceinline__ void point(long q, T u, T Au, T idg, double& a0, double& a1) const {
-        t[q] = u;
-        inner.point(q, u, Au, idg, a0, a1);
-    }
-    template <int V>
-    __device__ __forceinline__ void pointv(long q, const TV<T, V>& u, const TV<T, V>& Au, const TV<T, V>& idg,
-                                           double& a0, double& a1) const {
-        stv<T, V>(t + q, u);
-        inner.template pointv<V>(q, u, Au, idg, a0, a1);
-    }
-};
-
 // One smoothing step applied to the prolonged iterate without storing it:
 //   t = base + P x_c   (base = c xb with ZG, else xb)
 //   Out(t, A t)        (e.g. y = t + s D^-1 (b - A t))
@@ -1082,16 +1093,41 @@ __device__ __forceinline__ void prolong_smooth_body(int bx, int by, int bz, int 
             Y[2] = R1;
         }
     };
+    // base values on the centre row of one plane: the lane's pair and the
+    // edge lanes' extra value (lane 0: i0-1, last lane: i1+1); loaded two
+    // planes ahead, and handed to the output as b (ZG) instead of re-read
+    struct BP {
+        T2<T> p;
+        T x;
+    };
+    const bool la = act && rowok;
+    auto bload = [&](long qq) -> BP {
+        BP r{zero2, z0};
+        if (la) {
+            r.p = ld2(xb + qq);
+            if (lane == 0) r.x = xb[qq - 1];
+            else if (lane == TX - 1 && i1 + 1 < mx) r.x = xb[qq + 2];
+        }
+        return r;
+    };
+    auto tpb = [&](const T3<T>& P, const T2<T>& bb) -> T2<T> {
+        const T b0 = ZG ? c * bb.a : bb.a, b1 = ZG ? c * bb.b : bb.b;
+        return T2<T>{b0 + h * (P.w + P.v), b1 + P.v};
+    };
+    auto scb = [&](T v) -> T { return ZG ? c * v : v; };
     int Ka = k0 >> 1;
     T3<T> Ya[3] = {zero3, zero3, zero3}, Yb[3] = {zero3, zero3, zero3};
     long q = (long)k0 * sz + (long)j * sy + i0;
     T3<T> Pc = zero3;
     T2<T> tc = zero2, td = zero2;
+    BP bp0{zero2, z0}, bp1{zero2, z0};
     if (rowok) {
         yplane(Ka, Ya);
         if (Ka + 1 < mzc) yplane(Ka + 1, Yb);
         Pc = (k0 & 1) ? mix3(Ya[1], Yb[1]) : Ya[1];
-        if (act) tc = tpair(Pc, q);
+        bp0 = bload(q);
+        if (k0 + 1 < mz) bp1 = bload(q + sz);
+        tc = tpb(Pc, bp0.p);
         if (k0 > 0) {
             T3<T> Pd;
             if (k0 & 1) {
@@ -1108,8 +1144,10 @@ __device__ __forceinline__ void prolong_smooth_body(int bx, int by, int bz, int 
     for (int k = k0; k < k1; ++k, q += sz) {
         T3<T> Pu = zero3;
         T2<T> tu = zero2;
+        BP bp2{zero2, z0};
         const bool need_sn = rowok && !jb && k != 0 && k != mz - 1;
         if (rowok) {
+            if (k + 2 < mz) bp2 = bload(q + 2 * sz);
             if ((k >> 1) != Ka) {
                 Ka = k >> 1;
                 Ya[0] = Yb[0]; Ya[1] = Yb[1]; Ya[2] = Yb[2];
@@ -1118,7 +1156,7 @@ __device__ __forceinline__ void prolong_smooth_body(int bx, int by, int bz, int 
             }
             if (k + 1 < mz) {
                 Pu = (k & 1) ? Yb[1] : mix3(Ya[1], Yb[1]);
-                if (act) tu = tpair(Pu, q + sz);
+                tu = tpb(Pu, bp1.p);
             }
             T2<T>* row = tbuf[k & 1][0];
             if (act) row[(ty + 1) * TX + lane] = tc;
@@ -1134,13 +1172,37 @@ __device__ __forceinline__ void prolong_smooth_body(int bx, int by, int bz, int 
                 const T2<T>* row = tbuf[k & 1][0];
                 const T2<T> ts = (need_sn && j - 1 > 0) ? row[ty * TX + lane] : zero2;
                 const T2<T> tn = (need_sn && j + 1 < my - 1) ? row[(ty + 2) * TX + lane] : zero2;
-                emit(q, j, k, mz, jb, Pc, tc, ts, tn, (k - 1 > 0) ? td : zero2, (k + 1 < mz - 1) ? tu : zero2, wv,
-                     ev);
+                const T2<T> tdd = (k - 1 > 0) ? td : zero2, tuu = (k + 1 < mz - 1) ? tu : zero2;
+                T wv2 = wv, ev2 = ev;
+                if (lane == 0 && i0 - 1 > 0) wv2 = scb(bp0.x) + Pc.w;
+                if (lane == TX - 1 && i1 + 1 < mx - 1) ev2 = scb(bp0.x) + h * (Pc.v + Pc.e);
+                T Aa, Ab, ia, ib;
+                if (jb || k == 0 || k == mz - 1) {
+                    op.bnd(i0, j, k, tc.a, Aa, ia);
+                    op.bnd(i1, j, k, tc.b, Ab, ib);
+                } else {
+                    op.interior(i0, j, k, tc.a, i0 - 1 > 0 ? wv2 : z0, i1 < mx - 1 ? tc.b : z0, ts.a, tn.a, tdd.a,
+                                tuu.a, Aa, ia);
+                    if (i1 == mx - 1) op.bnd(i1, j, k, tc.b, Ab, ib);
+                    else op.interior(i1, j, k, tc.b, tc.a, i1 + 1 < mx - 1 ? ev2 : z0, ts.b, tn.b, tdd.b, tuu.b, Ab,
+                                     ib);
+                }
+                const TV<T, 2> uu{{tc.a, tc.b}}, AA{{Aa, Ab}}, II{{ia, ib}};
+                if constexpr (ZG) out.template pointvb<2>(q, uu, AA, II, TV<T, 2>{{bp0.p.a, bp0.p.b}}, a0, a1);
+                else out.template pointv<2>(q, uu, AA, II, a0, a1);
+                if (I == 1) {      // fine 0: a boundary point
+                    const T t0 = scb(bp0.x) + Pc.w;
+                    op.bnd(0, j, k, t0, Aa, ia);
+                    if constexpr (ZG) out.pointb(q - 1, t0, Aa, ia, bp0.x, a0, a1);
+                    else out.point(q - 1, t0, Aa, ia, a0, a1);
+                }
             }
         }
         td = tc;
         tc = tu;
         Pc = Pu;
+        bp0 = bp1;
+        bp1 = bp2;
     }
 }
 

@@ -381,7 +381,7 @@ struct Engine {
     }
 
     // One smoothing step (Out) applied to t = (zg ? c base : base) + P x_c
-    // on level l without storing t (Out may store it, OutStoreU).
+    // on level l without storing t.
     template <bool ZG, class Out>
     int prolong_smooth(int l, const T* base, T cz, const T* xc, Out out, double passes, long* nb, bool jac) {
         const LevelInfo& L = (*lv)[l];
@@ -413,18 +413,13 @@ struct Engine {
     }
 
     // the first post-smoothing step fused with the prolongation (2D/3D):
-    // y = t + s D^-1 (b - A t), t = pre + P x_c; with tstore, t is also stored
+    // y = t + s D^-1 (b - A t), t = pre + P x_c, t not stored
     template <bool DOTS>
-    int post_first(int l, bool zg, const T* pre, T cz, const T* xc, const T* b, T* tstore, T* y, T s2, long* nb) {
+    int post_first(int l, bool zg, const T* pre, T cz, const T* xc, const T* b, T* y, T s2, long* nb) {
         const bool jac = jacobi != 0;
         typedef OutCheb<T, false, DOTS ? 2 : 0> OC;
         const OC oc{b, nullptr, y, T(0), T(1), s2};
-        const double passes = (zg ? 2.0 : 3.0) + (tstore ? 1.0 : 0.0);
-        if (tstore) {
-            const OutStoreU<T, OC> os{tstore, oc};
-            return zg ? prolong_smooth<true>(l, pre, cz, xc, os, passes, nb, jac)
-                      : prolong_smooth<false>(l, pre, cz, xc, os, passes, nb, jac);
-        }
+        const double passes = zg ? 2.0 : 3.0;
         return zg ? prolong_smooth<true>(l, pre, cz, xc, oc, passes, nb, jac)
                   : prolong_smooth<false>(l, pre, cz, xc, oc, passes, nb, jac);
     }
@@ -678,25 +673,9 @@ struct Engine {
         T* x0 = zgf ? d.t[0] : xpre;
         const T* pre = zgf ? b : xpre;
         if (L.dim >= 2 && fuse_post && nu == 1) {
-            // the first post-smoothing step reads the prolonged iterate on the fly
-            if (nu == 1) {
-                if (dots) P4G_TRY(post_first<true>(l, zgf, pre, T(s * idg), c.x, b, nullptr, x, T(s), nb));
-                else P4G_TRY(post_first<false>(l, zgf, pre, T(s * idg), c.x, b, nullptr, x, T(s), nb));
-            } else {
-                // t (iterate 0 of the recurrence) and iterate 1 go to temporaries
-                // distinct from pre, which other threads still read
-                T* tk = nullptr;
-                T* y1 = nullptr;
-                for (int q = 0; q < 4; ++q) {
-                    T* cand = d.t[q];
-                    if (!cand || cand == x || cand == b || cand == xpre) continue;
-                    if (!tk) tk = cand;
-                    else if (!y1) y1 = cand;
-                }
-                if (!tk || !y1) return set_error(PETSC_ERR_LIB, "internal: smoother buffers exhausted");
-                P4G_TRY(post_first<false>(l, zgf, pre, T(s * idg), c.x, b, tk, y1, T(s), nullptr));
-                P4G_TRY(smooth(l, b, y1, x, dots, nb, 2, tk));
-            }
+            // the post-smoothing step reads the prolonged iterate on the fly
+            if (dots) P4G_TRY(post_first<true>(l, zgf, pre, T(s * idg), c.x, b, x, T(s), nb));
+            else P4G_TRY(post_first<false>(l, zgf, pre, T(s * idg), c.x, b, x, T(s), nb));
             return 0;
         }
         P4G_TRY(prolong(l, zgf, pre, T(s * idg), c.x, x0));
