@@ -73,52 +73,43 @@ struct OutChebR {
     T* __restrict__ y;
     T c1, c2;
     int dots;
-    static constexpr int NDOT = 2;
-    __device__ __forceinline__ void point(long q, T u, T Au, T idg, double& a0, double& a1) const {
-        const T bq = b[q];
-        const T v = c1 * u + c2 * (idg * (bq - Au));
+    static constexpr int NDOT = 2, NPRE = 1;
+    template <int V>
+    __device__ __forceinline__ void pre(long q, TV<T, V>* p) const { p[0] = ldv<T, V>(b + q); }
+    __device__ __forceinline__ void pre1(long q, T* p) const { p[0] = b[q]; }
+    __device__ __forceinline__ void pointp(long q, T u, T Au, T idg, const T* p, double& a0, double& a1) const {
+        const T v = c1 * u + c2 * (idg * (p[0] - Au));
         y[q] = v;
         if (dots) {
-            a0 += double(v) * double(bq);
-            a1 += double(v) * double(v);
-        }
-    }
-    __device__ __forceinline__ void pointb(long q, T u, T Au, T idg, T bq, double& a0, double& a1) const {
-        const T v = c1 * u + c2 * (idg * (bq - Au));
-        y[q] = v;
-        if (dots) {
-            a0 += double(v) * double(bq);
+            a0 += double(v) * double(p[0]);
             a1 += double(v) * double(v);
         }
     }
     template <int V>
-    __device__ __forceinline__ void pointvb(long q, const TV<T, V>& u, const TV<T, V>& Au, const TV<T, V>& idg,
-                                            const TV<T, V>& bb, double& a0, double& a1) const {
+    __device__ __forceinline__ void pointvp(long q, const TV<T, V>& u, const TV<T, V>& Au, const TV<T, V>& idg,
+                                            const TV<T, V>* p, double& a0, double& a1) const {
         TV<T, V> v;
 #pragma unroll
-        for (int e = 0; e < V; ++e) v.v[e] = c1 * u.v[e] + c2 * (idg.v[e] * (bb.v[e] - Au.v[e]));
+        for (int e = 0; e < V; ++e) v.v[e] = c1 * u.v[e] + c2 * (idg.v[e] * (p[0].v[e] - Au.v[e]));
         stv<T, V>(y + q, v);
         if (dots)
 #pragma unroll
             for (int e = 0; e < V; ++e) {
-                a0 += double(v.v[e]) * double(bb.v[e]);
+                a0 += double(v.v[e]) * double(p[0].v[e]);
                 a1 += double(v.v[e]) * double(v.v[e]);
             }
+    }
+    __device__ __forceinline__ void point(long q, T u, T Au, T idg, double& a0, double& a1) const {
+        T p[1];
+        pre1(q, p);
+        pointp(q, u, Au, idg, p, a0, a1);
     }
     template <int V>
     __device__ __forceinline__ void pointv(long q, const TV<T, V>& u, const TV<T, V>& Au, const TV<T, V>& idg,
                                            double& a0, double& a1) const {
-        const TV<T, V> bb = ldv<T, V>(b + q);
-        TV<T, V> v;
-#pragma unroll
-        for (int e = 0; e < V; ++e) v.v[e] = c1 * u.v[e] + c2 * (idg.v[e] * (bb.v[e] - Au.v[e]));
-        stv<T, V>(y + q, v);
-        if (dots)
-#pragma unroll
-            for (int e = 0; e < V; ++e) {
-                a0 += double(v.v[e]) * double(bb.v[e]);
-                a1 += double(v.v[e]) * double(v.v[e]);
-            }
+        TV<T, V> p[1];
+        pre<V>(q, p);
+        pointvp<V>(q, u, Au, idg, p, a0, a1);
     }
 };
 

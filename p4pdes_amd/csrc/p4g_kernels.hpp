@@ -187,41 +187,6 @@ struct SrcZero {                      // u = 0 (first smoothing step from a zero
 };
 
 template <typename T>
-struct SrcScaled {                    // u = c b   (zero-guess smoothing step)
-    const T* __restrict__ b;
-    T c;
-    __device__ __forceinline__ T load(long q) const { return c * b[q]; }
-    template <int V>
-    __device__ __forceinline__ TV<T, V> loadv(long q) const {
-        TV<T, V> r = ldv<T, V>(b + q);
-#pragma unroll
-        for (int e = 0; e < V; ++e) r.v[e] *= c;
-        return r;
-    }
-    __device__ __forceinline__ void put(long, T) const {}
-    template <int V>
-    __device__ __forceinline__ void putv(long, const TV<T, V>&) const {}
-};
-
-template <typename T>
-struct SrcScaledPut {                 // u = c b, stored to x
-    const T* __restrict__ b;
-    T c;
-    T* __restrict__ x;
-    __device__ __forceinline__ T load(long q) const { return c * b[q]; }
-    template <int V>
-    __device__ __forceinline__ TV<T, V> loadv(long q) const {
-        TV<T, V> r = ldv<T, V>(b + q);
-#pragma unroll
-        for (int e = 0; e < V; ++e) r.v[e] *= c;
-        return r;
-    }
-    __device__ __forceinline__ void put(long q, T v) const { x[q] = v; }
-    template <int V>
-    __device__ __forceinline__ void putv(long q, const TV<T, V>& u) const { stv<T, V>(x + q, u); }
-};
-
-template <typename T>
 struct SrcDir {                       // p_new = z + beta p_old, stored (CG direction)
     const T* __restrict__ z;
     const T* __restrict__ pold;
@@ -353,11 +318,15 @@ struct SrcSum {                       // x_new = x + e (refinement update), stor
 //
 // An output consumes u, A u and the Jacobi factor at a point (point) or at
 // V consecutive points (pointv) and accumulates up to NDOT dot products.
+// An output that reads NPRE per-point arrays besides its result lets the
+// marching pass load them ahead: pre / pre1 load them at q (V points or
+// one), pointvp / pointp consume them.  Those arrays must not be written by
+// the pass's source.
 
 template <typename T>
 struct OutApplyDot {                  // w = A u ; acc0 += u.w
     T* __restrict__ w;
-    static constexpr int NDOT = 1;
+    static constexpr int NDOT = 1, NPRE = 0;
     __device__ __forceinline__ void point(long q, T u, T Au, T, double& a0, double&) const {
         w[q] = Au;
         a0 += double(u) * double(Au);
@@ -374,7 +343,7 @@ struct OutApplyDot {                  // w = A u ; acc0 += u.w
 template <typename T>
 struct OutApply {                     // w = A u
     T* __restrict__ w;
-    static constexpr int NDOT = 0;
+    static constexpr int NDOT = 0, NPRE = 0;
     __device__ __forceinline__ void point(long q, T, T Au, T, double&, double&) const { w[q] = Au; }
     template <int V>
     __device__ __forceinline__ void pointv(long q, const TV<T, V>&, const TV<T, V>& Au, const TV<T, V>&, double&,
@@ -387,22 +356,37 @@ template <typename T, int NORM>
 struct OutResid {                     // r = b - A u ; acc0 += r.r (NORM)
     const T* __restrict__ b;
     T* __restrict__ r;
-    static constexpr int NDOT = NORM;
-    __device__ __forceinline__ void point(long q, T, T Au, T, double& a0, double&) const {
-        const T v = b[q] - Au;
+    static constexpr int NDOT = NORM, NPRE = 1;
+    template <int V>
+    __device__ __forceinline__ void pre(long q, TV<T, V>* p) const { p[0] = ldv<T, V>(b + q); }
+    __device__ __forceinline__ void pre1(long q, T* p) const { p[0] = b[q]; }
+    __device__ __forceinline__ void pointp(long q, T, T Au, T, const T* p, double& a0, double&) const {
+        const T v = p[0] - Au;
         r[q] = v;
         if (NORM) a0 += double(v) * double(v);
     }
     template <int V>
-    __device__ __forceinline__ void pointv(long q, const TV<T, V>&, const TV<T, V>& Au, const TV<T, V>&,
-                                           double& a0, double&) const {
-        TV<T, V> v = ldv<T, V>(b + q);
+    __device__ __forceinline__ void pointvp(long q, const TV<T, V>&, const TV<T, V>& Au, const TV<T, V>&,
+                                            const TV<T, V>* p, double& a0, double&) const {
+        TV<T, V> v;
 #pragma unroll
-        for (int e = 0; e < V; ++e) v.v[e] -= Au.v[e];
+        for (int e = 0; e < V; ++e) v.v[e] = p[0].v[e] - Au.v[e];
         stv<T, V>(r + q, v);
         if (NORM)
 #pragma unroll
             for (int e = 0; e < V; ++e) a0 += double(v.v[e]) * double(v.v[e]);
+    }
+    __device__ __forceinline__ void point(long q, T u, T Au, T idg, double& a0, double& a1) const {
+        T p[1];
+        pre1(q, p);
+        pointp(q, u, Au, idg, p, a0, a1);
+    }
+    template <int V>
+    __device__ __forceinline__ void pointv(long q, const TV<T, V>& u, const TV<T, V>& Au, const TV<T, V>& idg,
+                                           double& a0, double& a1) const {
+        TV<T, V> p[1];
+        pre<V>(q, p);
+        pointvp<V>(q, u, Au, idg, p, a0, a1);
     }
 };
 
@@ -410,36 +394,52 @@ template <typename T, typename R>
 struct OutResidCvt {                  // r = b - A u in T, stored as R ; acc0 += r.r
     const T* __restrict__ b;
     R* __restrict__ r;
-    static constexpr int NDOT = 1;
-    __device__ __forceinline__ void point(long q, T, T Au, T, double& a0, double&) const {
-        const T v = b[q] - Au;
+    static constexpr int NDOT = 1, NPRE = 1;
+    template <int V>
+    __device__ __forceinline__ void pre(long q, TV<T, V>* p) const { p[0] = ldv<T, V>(b + q); }
+    __device__ __forceinline__ void pre1(long q, T* p) const { p[0] = b[q]; }
+    __device__ __forceinline__ void pointp(long q, T, T Au, T, const T* p, double& a0, double&) const {
+        const T v = p[0] - Au;
         r[q] = R(v);
         a0 += double(v) * double(v);
     }
     template <int V>
-    __device__ __forceinline__ void pointv(long q, const TV<T, V>&, const TV<T, V>& Au, const TV<T, V>&,
-                                           double& a0, double&) const {
-        const TV<T, V> bb = ldv<T, V>(b + q);
+    __device__ __forceinline__ void pointvp(long q, const TV<T, V>&, const TV<T, V>& Au, const TV<T, V>&,
+                                            const TV<T, V>* p, double& a0, double&) const {
         TV<R, V> o;
 #pragma unroll
         for (int e = 0; e < V; ++e) {
-            const T v = bb.v[e] - Au.v[e];
+            const T v = p[0].v[e] - Au.v[e];
             o.v[e] = R(v);
             a0 += double(v) * double(v);
         }
         stv<R, V>(r + q, o);
     }
+    __device__ __forceinline__ void point(long q, T u, T Au, T idg, double& a0, double& a1) const {
+        T p[1];
+        pre1(q, p);
+        pointp(q, u, Au, idg, p, a0, a1);
+    }
+    template <int V>
+    __device__ __forceinline__ void pointv(long q, const TV<T, V>& u, const TV<T, V>& Au, const TV<T, V>& idg,
+                                           double& a0, double& a1) const {
+        TV<T, V> p[1];
+        pre<V>(q, p);
+        pointvp<V>(q, u, Au, idg, p, a0, a1);
+    }
 };
 
-// The operand u was formed unscaled: the inner output sees c u and c A u
-// (the zero-guess smoothing iterate c b of an unscaled residual source).
+// Zero-guess smoothing step c b on a level whose right-hand side b is the
+// operand u itself (read by the source, or formed and stored as b by it):
+// the inner output sees c u, c A u, and u as its b (its one per-point array).
 template <typename T, class Out>
 struct OutScaleU {
     T c;
     Out inner;
-    static constexpr int NDOT = Out::NDOT;
+    static_assert(Out::NPRE == 1, "OutScaleU supplies the inner output's b");
+    static constexpr int NDOT = Out::NDOT, NPRE = 0;
     __device__ __forceinline__ void point(long q, T u, T Au, T idg, double& a0, double& a1) const {
-        inner.point(q, c * u, c * Au, idg, a0, a1);
+        inner.pointp(q, c * u, c * Au, idg, &u, a0, a1);
     }
     template <int V>
     __device__ __forceinline__ void pointv(long q, const TV<T, V>& u, const TV<T, V>& Au, const TV<T, V>& idg,
@@ -450,30 +450,27 @@ struct OutScaleU {
             cu.v[e] = c * u.v[e];
             cAu.v[e] = c * Au.v[e];
         }
-        inner.template pointv<V>(q, cu, cAu, idg, a0, a1);
+        inner.template pointvp<V>(q, cu, cAu, idg, &u, a0, a1);
     }
 };
 
-// Wraps an output without dot products and adds acc0 += rn.rn at every
-// point (rn: the residual stored by SrcUpd in the same thread).
+// Wraps an output without dot products and adds acc0 += u.u at every point
+// (u: the residual formed and stored by SrcUpd).
 template <typename T, class Out>
 struct OutRR {
-    const T* __restrict__ rn;
     Out inner;
-    static_assert(Out::NDOT == 0, "OutRR needs an output without dot products");
-    static constexpr int NDOT = 1;
+    static_assert(Out::NDOT == 0 && Out::NPRE == 0, "OutRR wraps an output without dots or operands");
+    static constexpr int NDOT = 1, NPRE = 0;
     __device__ __forceinline__ void point(long q, T u, T Au, T idg, double& a0, double& a1) const {
         inner.point(q, u, Au, idg, a0, a1);
-        const double v = double(rn[q]);
-        a0 += v * v;
+        a0 += double(u) * double(u);
     }
     template <int V>
     __device__ __forceinline__ void pointv(long q, const TV<T, V>& u, const TV<T, V>& Au, const TV<T, V>& idg,
                                            double& a0, double& a1) const {
         inner.template pointv<V>(q, u, Au, idg, a0, a1);
-        const TV<T, V> v = ldv<T, V>(rn + q);
 #pragma unroll
-        for (int e = 0; e < V; ++e) a0 += double(v.v[e]) * double(v.v[e]);
+        for (int e = 0; e < V; ++e) a0 += double(u.v[e]) * double(u.v[e]);
     }
 };
 
@@ -484,66 +481,53 @@ struct OutCheb {
     const T* __restrict__ prev;
     T* __restrict__ y;
     T c0, c1, c2;
-    static constexpr int NDOT = DOTS;
-    __device__ __forceinline__ void point(long q, T u, T Au, T idg, double& a0,
-                                          double& a1) const {
-        const T bq = b[q];
-        T v = c1 * u + c2 * (idg * (bq - Au));
-        if (PREV) v += c0 * prev[q];
+    static constexpr int NDOT = DOTS, NPRE = PREV ? 2 : 1;   // b, prev
+    template <int V>
+    __device__ __forceinline__ void pre(long q, TV<T, V>* p) const {
+        p[0] = ldv<T, V>(b + q);
+        if (PREV) p[NPRE - 1] = ldv<T, V>(prev + q);
+    }
+    __device__ __forceinline__ void pre1(long q, T* p) const {
+        p[0] = b[q];
+        if (PREV) p[NPRE - 1] = prev[q];
+    }
+    __device__ __forceinline__ void pointp(long q, T u, T Au, T idg, const T* p, double& a0, double& a1) const {
+        T v = c1 * u + c2 * (idg * (p[0] - Au));
+        if (PREV) v += c0 * p[NPRE - 1];
         y[q] = v;
         if (DOTS) {
-            a0 += double(v) * double(bq);
+            a0 += double(v) * double(p[0]);
             a1 += double(v) * double(v);
         }
+    }
+    template <int V>
+    __device__ __forceinline__ void pointvp(long q, const TV<T, V>& u, const TV<T, V>& Au, const TV<T, V>& idg,
+                                            const TV<T, V>* p, double& a0, double& a1) const {
+        TV<T, V> v;
+#pragma unroll
+        for (int e = 0; e < V; ++e) v.v[e] = c1 * u.v[e] + c2 * (idg.v[e] * (p[0].v[e] - Au.v[e]));
+        if (PREV)
+#pragma unroll
+            for (int e = 0; e < V; ++e) v.v[e] += c0 * p[NPRE - 1].v[e];
+        stv<T, V>(y + q, v);
+        if (DOTS)
+#pragma unroll
+            for (int e = 0; e < V; ++e) {
+                a0 += double(v.v[e]) * double(p[0].v[e]);
+                a1 += double(v.v[e]) * double(v.v[e]);
+            }
+    }
+    __device__ __forceinline__ void point(long q, T u, T Au, T idg, double& a0, double& a1) const {
+        T p[NPRE];
+        pre1(q, p);
+        pointp(q, u, Au, idg, p, a0, a1);
     }
     template <int V>
     __device__ __forceinline__ void pointv(long q, const TV<T, V>& u, const TV<T, V>& Au, const TV<T, V>& idg,
                                            double& a0, double& a1) const {
-        const TV<T, V> bb = ldv<T, V>(b + q);
-        TV<T, V> v;
-#pragma unroll
-        for (int e = 0; e < V; ++e) v.v[e] = c1 * u.v[e] + c2 * (idg.v[e] * (bb.v[e] - Au.v[e]));
-        if (PREV) {
-            const TV<T, V> pp = ldv<T, V>(prev + q);
-#pragma unroll
-            for (int e = 0; e < V; ++e) v.v[e] += c0 * pp.v[e];
-        }
-        stv<T, V>(y + q, v);
-        if (DOTS)
-#pragma unroll
-            for (int e = 0; e < V; ++e) {
-                a0 += double(v.v[e]) * double(bb.v[e]);
-                a1 += double(v.v[e]) * double(v.v[e]);
-            }
-    }
-    // the same with b given by the caller (already loaded)
-    __device__ __forceinline__ void pointb(long q, T u, T Au, T idg, T bq, double& a0, double& a1) const {
-        T v = c1 * u + c2 * (idg * (bq - Au));
-        if (PREV) v += c0 * prev[q];
-        y[q] = v;
-        if (DOTS) {
-            a0 += double(v) * double(bq);
-            a1 += double(v) * double(v);
-        }
-    }
-    template <int V>
-    __device__ __forceinline__ void pointvb(long q, const TV<T, V>& u, const TV<T, V>& Au, const TV<T, V>& idg,
-                                            const TV<T, V>& bb, double& a0, double& a1) const {
-        TV<T, V> v;
-#pragma unroll
-        for (int e = 0; e < V; ++e) v.v[e] = c1 * u.v[e] + c2 * (idg.v[e] * (bb.v[e] - Au.v[e]));
-        if (PREV) {
-            const TV<T, V> pp = ldv<T, V>(prev + q);
-#pragma unroll
-            for (int e = 0; e < V; ++e) v.v[e] += c0 * pp.v[e];
-        }
-        stv<T, V>(y + q, v);
-        if (DOTS)
-#pragma unroll
-            for (int e = 0; e < V; ++e) {
-                a0 += double(v.v[e]) * double(bb.v[e]);
-                a1 += double(v.v[e]) * double(v.v[e]);
-            }
+        TV<T, V> p[NPRE];
+        pre<V>(q, p);
+        pointvp<V>(q, u, Au, idg, p, a0, a1);
     }
 };
 
@@ -615,13 +599,21 @@ __device__ __forceinline__ Tile xcd_tile() {
 // element accesses.  The march runs along the slowest dimension (z in 3D,
 // y in 2D) keeping the planes m-1, m, m+1 in registers; 3D rows j+-1 are
 // read through the cache.  Every lane of a wave runs the same loop (loads
-// are predicated) so the cross-lane exchanges see the whole wave.  Grid:
+// are predicated) so the cross-lane exchanges see the whole wave.
+// Latency hiding: the centre row of plane m+PD is loaded while plane m is
+// computed (a ring of PD plane records: the row, the edge lanes' extra x
+// value with PF bit 1, the output's per-point operands); PF bit 0 loads rows
+// j+-1 of plane m+1 one plane early.
+// Grid:
 //   3D: (ceil((mx-1)/(64V)), ceil(my/TY), ceil(mz/kch)), block (64,TY)
 //   2D: (ceil((mx-1)/(64V)), ceil(ceil(my/kch)/TY), 1),  block (64,TY)
-template <typename T, int DIM, int V, class Op, class Src, class Out>
+template <typename T, int DIM, int V, class Op, class Src, class Out, int PD = 1, int PF = 3>
 __device__ __forceinline__ void stencil_vec(int bx, int by, int bz, int mx, int my, int mz, int kch,
                                             const Op& op, const Src& src, const Out& out, double& a0,
                                             double& a1) {
+    static_assert(PD >= 1 && PD <= 4, "prefetch distance");
+    constexpr bool PSN = (PF & 1) != 0 && DIM == 3, PEX = (PF & 2) != 0;
+    constexpr int NP = Out::NPRE, NPA = NP > 0 ? NP : 1;
     const int lane = threadIdx.x;
     const int i0 = 1 + V * (bx * TX + lane);
     const T z0 = T(0);
@@ -650,31 +642,88 @@ __device__ __forceinline__ void stencil_vec(int bx, int by, int bz, int mx, int 
     const bool full = ne == V;
     const bool jb = DIM == 3 && (j == 0 || j == my - 1);
     const bool ms = DIM == 3 && j - 1 > 0, mn = DIM == 3 && j + 1 < my - 1;
+    const bool wl = lane == 0 && i0 - 1 > 0, el = lane == TX - 1 && i0 + V < mx - 1;
     TV<T, V> zv;
 #pragma unroll
     for (int e = 0; e < V; ++e) zv.v[e] = z0;
+    // (active lanes only: ne >= 1, so element 0 of a partial lane is in range)
     auto LV = [&](long a) -> TV<T, V> {
         if (full) return src.template loadv<V>(a);
-        TV<T, V> r = zv;
+        TV<T, V> r;
 #pragma unroll
-        for (int e = 0; e < V; ++e)
-            if (e < ne) r.v[e] = src.load(a + e);
+        for (int e = 0; e < V; ++e) {
+            const T v = src.load(a + (e < ne ? e : 0));
+            r.v[e] = e < ne ? v : z0;
+        }
         return r;
     };
+    // a plane's values on the centre row: the operand (u), the edge lanes'
+    // extra value (x: lane 0 i0-1, last lane i0+V when interior; PEX), the
+    // output's per-point operands (o: full lanes; ox: at i0-1 = 0 for the
+    // lane with i0 = 1).  c*: plane m; r*[d]: plane m+1+d.
+    TV<T, V> cu = zv, ru[PD], co[NPA], ro[PD][NPA];
+    T cx = z0, rx[PD], cox[NPA], rox[PD][NPA];
+#pragma unroll
+    for (int k = 0; k < NPA; ++k) {
+        co[k] = zv;
+        cox[k] = z0;
+    }
+#pragma unroll
+    for (int d = 0; d < PD; ++d) {
+        ru[d] = zv;
+        rx[d] = z0;
+#pragma unroll
+        for (int k = 0; k < NPA; ++k) {
+            ro[d][k] = zv;
+            rox[d][k] = z0;
+        }
+    }
+    // load the values of the plane at a (own: the plane is computed here)
+    auto LR = [&](long a, bool own, TV<T, V>& u, T& x, TV<T, V>* o, T* ox) {
+        u = LV(a);
+        if (PEX) x = lane == 0 ? src.load(a - 1) : (el ? src.load(a + V) : z0);
+        if constexpr (NP > 0) {
+            if (own && full) out.template pre<V>(a, o);
+            if (own && i0 == 1) out.pre1(a - 1, ox);
+        }
+    };
+    // rows j-1, j+1 of plane m are read by interior rows of interior planes
+    auto snok = [&](int m) -> bool { return PSN && act && !jb && m > 0 && m < M - 1; };
+    const int mlast = min(m1, M - 1);  // last plane whose centre row is read
     long q = base + (long)m0 * sm + i0;
-    TV<T, V> ud = zv, uc = zv;
+    TV<T, V> ud = zv;
+    TV<T, V> sc = zv, nc = zv;
     if (act) {
         if (m0 > 0) ud = LV(q - sm);
-        uc = LV(q);
+        LR(q, true, cu, cx, co, cox);
+#pragma unroll
+        for (int d = 0; d + 1 < PD; ++d)
+            if (m0 + 1 + d <= mlast) LR(q + (1 + d) * sm, m0 + 1 + d < m1, ru[d], rx[d], ro[d], rox[d]);
+        if (snok(m0)) {
+            if (ms) sc = LV(q - sy);
+            if (mn) nc = LV(q + sy);
+        }
     }
     for (int m = m0; m < m1; ++m, q += sm) {
-        TV<T, V> uu = zv;
-        if (act && m + 1 < M) uu = LV(q + sm);
+        if (act && m + PD <= mlast)
+            LR(q + PD * sm, m + PD < m1, ru[PD - 1], rx[PD - 1], ro[PD - 1], rox[PD - 1]);
+        TV<T, V> sn = zv, nn = zv;
+        if (m + 1 < m1 && snok(m + 1)) {
+            if (ms) sn = LV(q + sm - sy);
+            if (mn) nn = LV(q + sm + sy);
+        }
+        const TV<T, V> uc = cu;
+        const TV<T, V> uu = ru[0];     // plane m+1 (read only when m+1 < M)
         T wv = __shfl_up(uc.v[V - 1], 1, TX);
         T ev = __shfl_down(uc.v[0], 1, TX);
         if (act) {
-            if (lane == 0 && i0 - 1 > 0) wv = src.load(q - 1);
-            if (lane == TX - 1 && i0 + V < mx - 1) ev = src.load(q + V);
+            if (PEX) {
+                if (wl) wv = cx;
+                if (el) ev = cx;
+            } else {
+                if (wl) wv = src.load(q - 1);
+                if (el) ev = src.load(q + V);
+            }
             const int jj = DIM == 3 ? j : m, kk = DIM == 3 ? m : 0;
             if (full) {
                 src.template putv<V>(q, uc);
@@ -691,8 +740,13 @@ __device__ __forceinline__ void stencil_vec(int bx, int by, int bz, int mx, int 
             } else {
                 TV<T, V> sv = zv, nv = zv, dn = zv, up = zv;
                 if (DIM == 3) {
-                    if (ms) sv = LV(q - sy);
-                    if (mn) nv = LV(q + sy);
+                    if (PSN) {
+                        sv = sc;
+                        nv = nc;
+                    } else {
+                        if (ms) sv = LV(q - sy);
+                        if (mn) nv = LV(q + sy);
+                    }
                     if (m - 1 > 0) dn = ud;
                     if (m + 1 < M - 1) up = uu;
                 } else {
@@ -715,22 +769,42 @@ __device__ __forceinline__ void stencil_vec(int bx, int by, int bz, int mx, int 
                 }
             }
             if (full) {
-                out.template pointv<V>(q, uc, Au, idg, a0, a1);
+                if constexpr (NP > 0) out.template pointvp<V>(q, uc, Au, idg, co, a0, a1);
+                else out.template pointv<V>(q, uc, Au, idg, a0, a1);
             } else {
 #pragma unroll
                 for (int e = 0; e < V; ++e)
                     if (e < ne) out.point(q + e, uc.v[e], Au.v[e], idg.v[e], a0, a1);
             }
             if (i0 == 1) {
-                const T u0 = src.load(q - 1);
+                const T u0 = PEX ? cx : src.load(q - 1);
                 T A0, i00;
                 src.put(q - 1, u0);
                 op.bnd(0, jj, kk, u0, A0, i00);
-                out.point(q - 1, u0, A0, i00, a0, a1);
+                if constexpr (NP > 0) out.pointp(q - 1, u0, A0, i00, cox, a0, a1);
+                else out.point(q - 1, u0, A0, i00, a0, a1);
             }
         }
-        ud = uc;
-        uc = uu;
+        ud = cu;
+        cu = ru[0];
+        cx = rx[0];
+#pragma unroll
+        for (int k = 0; k < NPA; ++k) {
+            co[k] = ro[0][k];
+            cox[k] = rox[0][k];
+        }
+#pragma unroll
+        for (int d = 0; d + 1 < PD; ++d) {
+            ru[d] = ru[d + 1];
+            rx[d] = rx[d + 1];
+#pragma unroll
+            for (int k = 0; k < NPA; ++k) {
+                ro[d][k] = ro[d + 1][k];
+                rox[d][k] = rox[d + 1][k];
+            }
+        }
+        sc = sn;
+        nc = nn;
     }
 }
 
@@ -1188,12 +1262,18 @@ __device__ __forceinline__ void prolong_smooth_body(int bx, int by, int bz, int 
                                      ib);
                 }
                 const TV<T, 2> uu{{tc.a, tc.b}}, AA{{Aa, Ab}}, II{{ia, ib}};
-                if constexpr (ZG) out.template pointvb<2>(q, uu, AA, II, TV<T, 2>{{bp0.p.a, bp0.p.b}}, a0, a1);
-                else out.template pointv<2>(q, uu, AA, II, a0, a1);
+                // with ZG the base is the output's b, its one per-point array
+                if constexpr (ZG) {
+                    static_assert(Out::NPRE == 1, "the base is the output's b");
+                    const TV<T, 2> bb{{bp0.p.a, bp0.p.b}};
+                    out.template pointvp<2>(q, uu, AA, II, &bb, a0, a1);
+                } else {
+                    out.template pointv<2>(q, uu, AA, II, a0, a1);
+                }
                 if (I == 1) {      // fine 0: a boundary point
                     const T t0 = scb(bp0.x) + Pc.w;
                     op.bnd(0, j, k, t0, Aa, ia);
-                    if constexpr (ZG) out.pointb(q - 1, t0, Aa, ia, bp0.x, a0, a1);
+                    if constexpr (ZG) out.pointp(q - 1, t0, Aa, ia, &bp0.x, a0, a1);
                     else out.point(q - 1, t0, Aa, ia, a0, a1);
                 }
             }

@@ -198,7 +198,7 @@ enum { BK_DIR = 0, BK_FIN, BK_UPD, BK_REP, BK_RESID, BK_RESTRICT, BK_PSMOOTH, BK
 
 typedef StencilItem<OpConst<float>, SrcDir<float>, OutApplyDot<float>> DirItem;
 typedef StencilItem<OpConst<double>, SrcArr<double>, OutResidCvt<double, float>> RepItem;
-typedef StencilItem<OpConst<float>, SrcScaled<float>, OutResid<float, 0>> ResidItem;
+typedef StencilItem<OpConst<float>, SrcArr<float>, OutScaleU<float, OutResid<float, 0>>> ResidItem;
 typedef ProlongSmoothItem<float, OpConst<float>, OutChebR<float>> PSmoothItem;
 
 static int blaunch(p4g_server_s* v, const BLaunch& L, const char* base) {
@@ -216,8 +216,8 @@ static int blaunch(p4g_server_s* v, const BLaunch& L, const char* base) {
         else k_stencil_batch<double, 2, OpConst<double>, SrcArr<double>, OutResidCvt<double, float>><<<g, dim3(TX, TY), 0, st>>>((const RepItem*)items, L.nitems);
         break;
     case BK_RESID:
-        if (L.dim == 3) k_stencil_batch<float, 3, OpConst<float>, SrcScaled<float>, OutResid<float, 0>><<<g, dim3(TX, TY), 0, st>>>((const ResidItem*)items, L.nitems);
-        else k_stencil_batch<float, 2, OpConst<float>, SrcScaled<float>, OutResid<float, 0>><<<g, dim3(TX, TY), 0, st>>>((const ResidItem*)items, L.nitems);
+        if (L.dim == 3) k_stencil_batch<float, 3, OpConst<float>, SrcArr<float>, OutScaleU<float, OutResid<float, 0>>><<<g, dim3(TX, TY), 0, st>>>((const ResidItem*)items, L.nitems);
+        else k_stencil_batch<float, 2, OpConst<float>, SrcArr<float>, OutScaleU<float, OutResid<float, 0>>><<<g, dim3(TX, TY), 0, st>>>((const ResidItem*)items, L.nitems);
         break;
     case BK_PSMOOTH:
         if (L.dim == 3) k_prolong_smooth_batch<float, 3, OpConst<float>, OutChebR<float>><<<g, dim3(TX, TY), 0, st>>>((const PSmoothItem*)items, L.nitems);
@@ -286,8 +286,8 @@ static int build_vcycle(p4g_server_s* v, Packer& pk, std::vector<BLaunch>& seq) 
             it.flag = nullptr;
             it.op = op32(s, L);
             const float c = float(level_scale(s, L) * (s->jacobi ? 1.0 / L.d : 1.0));
-            it.src = SrcScaled<float>{bvec(q.r, q.l), c};
-            it.out = OutResid<float, 0>{bvec(q.r, q.l), q.r->lv[q.l].r};
+            it.src = SrcArr<float>{bvec(q.r, q.l)};
+            it.out = OutScaleU<float, OutResid<float, 0>>{c, OutResid<float, 0>{bvec(q.r, q.l), q.r->lv[q.l].r}};
             ri.push_back(it);
             RestrictItem<float> rt;
             rt.g = gref_transfer(C, C.mx, s2);

@@ -546,21 +546,24 @@ struct Engine {
                 // m == 2 from zero guess: prev iterate is 0, so no prev term
                 if (nu >= 3 && smoother == SM_CHEB) return set_error(PETSC_ERR_LIB, "internal smoother path");
                 if (fine && pend.on && !(last && dots)) {   // b = r - alpha w, formed here and stored to b
+                    if (pend.rn != b) return set_error(PETSC_ERR_LIB, "internal: pending update not into b");
                     const SrcUpd<T> su{pend.r, pend.w, pend.alpha, pend.rn};
                     typedef OutScaleU<T, OutCheb<T, false, 0>> OC;
                     const OC oc{cb, OutCheb<T, false, 0>{b, nullptr, dst, T(c0), T(c1), T(c2)}};
                     pend.on = false;
                     if (pend.rr)
-                        P4G_TRY(stencil(l, su, OutRR<T, OC>{b, oc}, P4G_K_SMOOTH, 4, &pend.nparts, jac, cx->part_rr));
+                        P4G_TRY(stencil(l, su, OutRR<T, OC>{oc}, P4G_K_SMOOTH, 4, &pend.nparts, jac, cx->part_rr));
                     else
                         P4G_TRY(stencil(l, su, oc, P4G_K_SMOOTH, 4, &pend.nparts, jac));
                 } else if (last && dots) {
                     P4G_TRY(pend_flush());
-                    P4G_TRY(stencil(l, SrcScaled<T>{b, cb}, OutCheb<T, false, 2>{b, nullptr, dst, T(c0), T(c1), T(c2)},
+                    typedef OutScaleU<T, OutCheb<T, false, 2>> OC;
+                    P4G_TRY(stencil(l, SrcArr<T>{b}, OC{cb, OutCheb<T, false, 2>{b, nullptr, dst, T(c0), T(c1), T(c2)}},
                                     P4G_K_SMOOTH, 2, &nbk, jac));
                 } else {
                     if (fine) P4G_TRY(pend_flush());
-                    P4G_TRY(stencil(l, SrcScaled<T>{b, cb}, OutCheb<T, false, 0>{b, nullptr, dst, T(c0), T(c1), T(c2)},
+                    typedef OutScaleU<T, OutCheb<T, false, 0>> OC;
+                    P4G_TRY(stencil(l, SrcArr<T>{b}, OC{cb, OutCheb<T, false, 0>{b, nullptr, dst, T(c0), T(c1), T(c2)}},
                                     P4G_K_SMOOTH, 2, &nbk, jac));
                 }
                 pm1_implicit = false;
@@ -647,16 +650,18 @@ struct Engine {
         if (xzero && nu == 1 && zg_fusable(L)) {
             zgf = true;
             if (l == nlev - 1 && pend.on) {   // b = r - alpha w, formed here and stored to b
+                if (pend.rn != b) return set_error(PETSC_ERR_LIB, "internal: pending update not into b");
                 const SrcUpd<T> su{pend.r, pend.w, pend.alpha, pend.rn};
                 typedef OutScaleU<T, OutResid<T, 0>> OR;
                 const OR orr{T(s * idg), OutResid<T, 0>{b, d.r}};
                 pend.on = false;
                 if (pend.rr)
-                    P4G_TRY(stencil(l, su, OutRR<T, OR>{b, orr}, P4G_K_RESID_ZG, 4, &pend.nparts, jac, cx->part_rr));
+                    P4G_TRY(stencil(l, su, OutRR<T, OR>{orr}, P4G_K_RESID_ZG, 4, &pend.nparts, jac, cx->part_rr));
                 else
                     P4G_TRY(stencil(l, su, orr, P4G_K_RESID_ZG, 4, &pend.nparts, jac));
             } else {
-                P4G_TRY(stencil(l, SrcScaled<T>{b, T(s * idg)}, OutResid<T, 0>{b, d.r}, P4G_K_RESID_ZG, 2, nullptr,
+                typedef OutScaleU<T, OutResid<T, 0>> OR;
+                P4G_TRY(stencil(l, SrcArr<T>{b}, OR{T(s * idg), OutResid<T, 0>{b, d.r}}, P4G_K_RESID_ZG, 2, nullptr,
                                 jac));
             }
         } else {
@@ -870,7 +875,7 @@ k_lanczos_update(long n, double a, double* __restrict__ r, const double* __restr
 
 struct OutIdiag {   // stores the Jacobi factor of every row
     double* __restrict__ id;
-    static constexpr int NDOT = 0;
+    static constexpr int NDOT = 0, NPRE = 0;
     __device__ __forceinline__ void point(long q, double, double, double idg, double&, double&) const {
         id[q] = idg;
     }
