@@ -149,7 +149,7 @@ int p4g_solver_destroy(p4g_Solver *s);
  * the selected class. */
 enum {
     P4G_K_STENCIL_DOT = 0, /* w = A p fused with (p,w), the CG operator apply  */
-    P4G_K_CG_UPDATE,       /* x += a p, r -= a w                               */
+    P4G_K_CG_UPDATE,       /* p = z + b p, x += a p (or x and r together)      */
     P4G_K_RESID_ZG,        /* r = b - A (c b): zero-guess smoothing + residual */
     P4G_K_RESTRICT,        /* b_c = P^T r                                      */
     P4G_K_PROLONG,         /* x = c b + P x_c                                  */
@@ -157,6 +157,7 @@ enum {
     P4G_K_COARSE,          /* coarsest-grid Cholesky solve                     */
     P4G_K_REDUCE,          /* finalisation of fused dot products               */
     P4G_K_APPLY,           /* y = A x, plain operator apply (MatMult)          */
+    P4G_K_CG_RESID,        /* r -= a w, the CG residual update on its own      */
     P4G_K_OTHER,
     P4G_K_COUNT
 };

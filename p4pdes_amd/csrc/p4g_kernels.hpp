@@ -26,6 +26,8 @@ namespace p4g {
 constexpr int TX = 64;   // lanes along x: one wavefront
 constexpr int TY = 8;    // rows (3D) or march chunks (2D) per block
 constexpr int NT = TX * TY;
+constexpr int TYF = 6;   // output rows per block of the fused 3D prolongation + smoothing
+static_assert(TX * (TYF + 2) == NT, "the fused 3D block (64, TYF+2) has the stencil block's shape");
 
 // Two consecutive grid values, moved as one 2-element vector access.  The
 // address need only be aligned to one element (gfx950 global memory takes
@@ -540,11 +542,12 @@ __device__ __forceinline__ double wave_sum(double v) {
 }
 
 // fixed-order block sum of up to two values; thread 0 writes part[2*bid+q]
-template <int NDOT>
+// (blocks of at most NW waves)
+template <int NDOT, int NW = NT / 64>
 __device__ __forceinline__ void block_partials(double a0, double a1, double* __restrict__ part,
                                                long bid) {
     if (NDOT == 0) return;
-    __shared__ double sm[2][NT / 64];
+    __shared__ double sm[2][NW];
     const int lin = threadIdx.x + blockDim.x * threadIdx.y;
     const int wv = lin >> 6, ln = lin & 63;
     const int nw = (blockDim.x * blockDim.y) >> 6;
@@ -1046,16 +1049,195 @@ __device__ __forceinline__ T3<T> mix3(const T3<T>& a, const T3<T>& b) {
 // A lane owns coarse column I = 1 + 64 bx + lane and the fine pair
 // (2I-1, 2I) -- the lane mapping of stencil_vec with V = 2 -- and the lane at I = 1
 // also fine 0.  The coarse values of I-1 and I+1 come from the neighbouring
-// lanes.  3D: a wave owns fine row j and marches fine planes k; t on its row
-// is interpolated from the coarse rows of the two coarse planes k>>1 and
-// (k>>1)+1, which stay in registers, and rows j-1, j+1 are exchanged through
-// LDS (one barrier per plane; the block-uniform loop is run by every wave).
-// 2D: a thread marches fine rows.  Grid: that of stencil_vec with V = 2.
+// lanes; t on a fine row is interpolated from the coarse rows of the two
+// coarse planes k>>1 and (k>>1)+1.
+//
+// 3D form of the fused step.  A block of R+2 waves covers the R fine rows
+// j0 .. j0+R-1 (j0 = R by) and planes k0 .. k1-1 (k0 = kch bz); wave w forms
+// t on row j0-1+w: the first and last wave form only the rows beside the
+// block (read by the stencil of its edge rows), the others also apply the
+// stencil to their row.  t of plane k goes through LDS (double-buffered by
+// plane parity, one barrier per plane; every wave runs the same loop); t
+// of planes k-1, k, k+1 of the own row stays in registers.  Loads run ahead
+// of use: the base row two planes ahead, the coarse row of the next coarse
+// plane one coarse plane ahead.  Grid (ceil((mx-1)/128), ceil(my/R),
+// ceil(mz/kch)), block (64, R+2).
+template <typename T, class Op, bool ZG, class Out, int R>
+__device__ __forceinline__ void prolong_smooth_3d(int bx, int by, int bz, int mx, int my, int mz, int mxc,
+                                                  int myc, int kch, const Op& op, const T* __restrict__ xb,
+                                                  T c, const T* __restrict__ xc, const Out& out, double& a0,
+                                                  double& a1) {
+    __shared__ T2<T> tbuf[2][R + 2][TX];
+    const int lane = threadIdx.x, w = threadIdx.y;
+    const int I = 1 + bx * TX + lane;
+    const bool act = I < mxc;
+    const int i0 = 2 * I - 1, i1 = 2 * I;
+    const T h = T(0.5), z0 = T(0);
+    const T3<T> zero3{z0, z0, z0};
+    const T2<T> zero2{z0, z0};
+    const int j = by * R + w - 1;
+    const bool halo = w == 0 || w == R + 1;
+    // rows this wave forms t on: an output row inside the grid; a halo row
+    // only where a stencil reads it (interior rows)
+    const bool rowok = halo ? (j > 0 && j < my - 1) : (j < my);   // uniform per wave
+    const int k0 = bz * kch, k1 = min(k0 + kch, mz);
+    if (k0 >= k1) return;                // uniform per block
+    const int mzc = (mz - 1) / 2 + 1;
+    const long sy = mx, sz = (long)mx * my, scz = (long)mxc * myc;
+    const bool la = act && rowok;
+    const int J = rowok ? j >> 1 : 0;
+    const bool odd = (j & 1) != 0;
+    // coarse values of the own row on one coarse plane, as loaded: v = coarse
+    // column I, x = the edge lane's extra column (lane 0: I-1, last lane:
+    // I+1); second row (J+1) for odd j
+    struct CR {
+        T v0, x0, v1, x1;
+    };
+    const CR cr0{z0, z0, z0, z0};
+    auto craw = [&](int K) -> CR {
+        CR r = cr0;
+        if (la) {
+            const long pb = (long)K * scz + (long)J * mxc + I;
+            r.v0 = xc[pb];
+            if (lane == 0) r.x0 = xc[pb - 1];
+            else if (lane == TX - 1 && I + 1 < mxc) r.x0 = xc[pb + 1];
+            if (odd) {
+                r.v1 = xc[pb + mxc];
+                if (lane == 0) r.x1 = xc[pb + mxc - 1];
+                else if (lane == TX - 1 && I + 1 < mxc) r.x1 = xc[pb + mxc + 1];
+            }
+        }
+        return r;
+    };
+    // the row's coarse values (I-1, I, I+1) from a loaded CR (every lane of
+    // the wave takes part in the exchange)
+    auto fin1 = [&](T v, T x) -> T3<T> {
+        T wv = __shfl_up(v, 1, TX);
+        T ev = __shfl_down(v, 1, TX);
+        if (act && lane == 0) wv = x;
+        if (act && lane == TX - 1 && I + 1 < mxc) ev = x;
+        return T3<T>{wv, v, ev};
+    };
+    auto cfin = [&](const CR& r) -> T3<T> {
+        const T3<T> a = fin1(r.v0, r.x0);
+        if (!odd) return a;
+        return mix3(a, fin1(r.v1, r.x1));
+    };
+    struct BP {
+        T2<T> p;
+        T x;
+    };
+    // base values on the own row: the lane's pair and the edge lane's extra value
+    auto bload = [&](long qq) -> BP {
+        BP r{zero2, z0};
+        if (la) {
+            r.p = ld2(xb + qq);
+            if (lane == 0) r.x = xb[qq - 1];
+            else if (lane == TX - 1 && i1 + 1 < mx) r.x = xb[qq + 2];
+        }
+        return r;
+    };
+    auto tpb = [&](const T3<T>& P, const T2<T>& bb) -> T2<T> {
+        const T b0 = ZG ? c * bb.a : bb.a, b1 = ZG ? c * bb.b : bb.b;
+        return T2<T>{b0 + h * (P.w + P.v), b1 + P.v};
+    };
+    auto scb = [&](T v) -> T { return ZG ? c * v : v; };
+    int Ka = k0 >> 1;
+    long q = (long)k0 * sz + (long)(rowok ? j : 0) * sy + i0;
+    T3<T> Ya = zero3, Yb = zero3, Pc = zero3;
+    CR cn = cr0;
+    T2<T> tc = zero2, td = zero2;
+    BP bp0{zero2, z0}, bp1{zero2, z0};
+    if (rowok) {
+        Ya = cfin(craw(Ka));
+        if (Ka + 1 < mzc) Yb = cfin(craw(Ka + 1));
+        if (Ka + 2 < mzc) cn = craw(Ka + 2);
+        Pc = (k0 & 1) ? mix3(Ya, Yb) : Ya;
+        bp0 = bload(q);
+        if (k0 + 1 < mz) bp1 = bload(q + sz);
+        tc = tpb(Pc, bp0.p);
+        if (!halo && k0 > 0) {
+            const T3<T> Pd = (k0 & 1) ? Ya : mix3(cfin(craw(Ka - 1)), Ya);
+            const BP bd = bload(q - sz);
+            td = tpb(Pd, bd.p);
+        }
+    }
+    const bool jb = (j == 0) || (j == my - 1);
+    for (int k = k0; k < k1; ++k, q += sz) {
+        T3<T> Pu = zero3;
+        T2<T> tu = zero2;
+        BP bp2{zero2, z0};
+        if (rowok) {
+            if (k + 2 < mz) bp2 = bload(q + 2 * sz);
+            if ((k >> 1) != Ka) {
+                Ka = k >> 1;
+                Ya = Yb;
+                Yb = (Ka + 1 < mzc) ? cfin(cn) : zero3;
+                cn = (Ka + 2 < mzc) ? craw(Ka + 2) : cr0;
+            }
+            if (k + 1 < mz) {
+                Pu = (k & 1) ? Yb : mix3(Ya, Yb);
+                tu = tpb(Pu, bp1.p);
+            }
+            if (act) tbuf[k & 1][w][lane] = tc;
+        }
+        __syncthreads();
+        T wv = z0, ev = z0;
+        if (!halo && rowok) {            // uniform per wave: the whole wave exchanges
+            wv = __shfl_up(tc.b, 1, TX);
+            ev = __shfl_down(tc.a, 1, TX);
+        }
+        if (!halo && rowok && act) {
+            const bool need_sn = !jb && k != 0 && k != mz - 1;
+            const T2<T> ts = (need_sn && j - 1 > 0) ? tbuf[k & 1][w - 1][lane] : zero2;
+            const T2<T> tn = (need_sn && j + 1 < my - 1) ? tbuf[k & 1][w + 1][lane] : zero2;
+            const T2<T> tdd = (k - 1 > 0) ? td : zero2, tuu = (k + 1 < mz - 1) ? tu : zero2;
+            if (lane == 0 && i0 - 1 > 0) wv = scb(bp0.x) + Pc.w;
+            if (lane == TX - 1 && i1 + 1 < mx - 1) ev = scb(bp0.x) + h * (Pc.v + Pc.e);
+            T Aa, Ab, ia, ib;
+            if (jb || k == 0 || k == mz - 1) {
+                op.bnd(i0, j, k, tc.a, Aa, ia);
+                op.bnd(i1, j, k, tc.b, Ab, ib);
+            } else {
+                op.interior(i0, j, k, tc.a, i0 - 1 > 0 ? wv : z0, i1 < mx - 1 ? tc.b : z0, ts.a, tn.a, tdd.a, tuu.a,
+                            Aa, ia);
+                if (i1 == mx - 1) op.bnd(i1, j, k, tc.b, Ab, ib);
+                else op.interior(i1, j, k, tc.b, tc.a, i1 + 1 < mx - 1 ? ev : z0, ts.b, tn.b, tdd.b, tuu.b, Ab, ib);
+            }
+            const TV<T, 2> uu{{tc.a, tc.b}}, AA{{Aa, Ab}}, II{{ia, ib}};
+            // with ZG the base is the output's b, its one per-point array
+            if constexpr (ZG) {
+                static_assert(Out::NPRE == 1, "the base is the output's b");
+                const TV<T, 2> bb{{bp0.p.a, bp0.p.b}};
+                out.template pointvp<2>(q, uu, AA, II, &bb, a0, a1);
+            } else {
+                out.template pointv<2>(q, uu, AA, II, a0, a1);
+            }
+            if (I == 1) {      // fine 0: a boundary point
+                const T t0 = scb(bp0.x) + Pc.w;
+                op.bnd(0, j, k, t0, Aa, ia);
+                if constexpr (ZG) out.pointp(q - 1, t0, Aa, ia, &bp0.x, a0, a1);
+                else out.point(q - 1, t0, Aa, ia, a0, a1);
+            }
+        }
+        td = tc;
+        tc = tu;
+        Pc = Pu;
+        bp0 = bp1;
+        bp1 = bp2;
+    }
+}
+
 template <typename T, int DIM, class Op, bool ZG, class Out>
 __device__ __forceinline__ void prolong_smooth_body(int bx, int by, int bz, int mx, int my, int mz, int mxc,
                                                     int myc, int kch, const Op& op, const T* __restrict__ xb,
                                                     T c, const T* __restrict__ xc, const Out& out, double& a0,
                                                     double& a1) {
+    // 2D: a thread marches fine rows (grid: that of stencil_vec with V = 2)
+    if constexpr (DIM == 3) {
+        prolong_smooth_3d<T, Op, ZG, Out, TYF>(bx, by, bz, mx, my, mz, mxc, myc, kch, op, xb, c, xc, out, a0, a1);
+        return;
+    }
     const int lane = threadIdx.x;
     const int I = 1 + bx * TX + lane;
     const bool act = I < mxc;
@@ -1135,154 +1317,6 @@ __device__ __forceinline__ void prolong_smooth_body(int bx, int by, int bz, int 
             Pc = Pu;
         }
         return;
-    }
-    // 3D: each wave forms t on its own row; rows j-1, j+1 of plane k come
-    // from the neighbouring waves through LDS (the block's first and last
-    // wave form the halo row outside the block themselves).  One barrier per
-    // plane, LDS double-buffered by plane parity; every wave of the block
-    // runs the same loop.
-    __shared__ T2<T> tbuf[2][TY + 2][TX];
-    const int ty = threadIdx.y;
-    const int j = by * TY + ty;
-    const bool rowok = j < my;           // uniform per wave
-    const int k0 = bz * kch, k1 = min(k0 + kch, mz);
-    if (k0 >= k1) return;                // uniform per block
-    const int mzc = (mz - 1) / 2 + 1;
-    const long sy = mx, sz = (long)mx * my, scz = (long)mxc * myc;
-    const bool lo_edge = ty == 0, hi_edge = ty == TY - 1;
-    // coarse values of fine rows j-1 (Y[0], edge waves), j (Y[1]), j+1 (Y[2],
-    // edge waves) on coarse plane K
-    auto yplane = [&](int K, T3<T>* Y) {
-        const long pb = (long)K * scz;
-        const int J = j >> 1;
-        if ((j & 1) == 0) {
-            const T3<T> R0 = crow(pb + (long)J * mxc);
-            Y[1] = R0;
-            if (lo_edge) Y[0] = mix3((J >= 1) ? crow(pb + (long)(J - 1) * mxc) : zero3, R0);
-            if (hi_edge) Y[2] = mix3(R0, (J + 1 < myc) ? crow(pb + (long)(J + 1) * mxc) : zero3);
-        } else {
-            const T3<T> R0 = crow(pb + (long)J * mxc), R1 = crow(pb + (long)(J + 1) * mxc);
-            Y[0] = R0;
-            Y[1] = mix3(R0, R1);
-            Y[2] = R1;
-        }
-    };
-    // base values on the centre row of one plane: the lane's pair and the
-    // edge lanes' extra value (lane 0: i0-1, last lane: i1+1); loaded two
-    // planes ahead, and handed to the output as b (ZG) instead of re-read
-    struct BP {
-        T2<T> p;
-        T x;
-    };
-    const bool la = act && rowok;
-    auto bload = [&](long qq) -> BP {
-        BP r{zero2, z0};
-        if (la) {
-            r.p = ld2(xb + qq);
-            if (lane == 0) r.x = xb[qq - 1];
-            else if (lane == TX - 1 && i1 + 1 < mx) r.x = xb[qq + 2];
-        }
-        return r;
-    };
-    auto tpb = [&](const T3<T>& P, const T2<T>& bb) -> T2<T> {
-        const T b0 = ZG ? c * bb.a : bb.a, b1 = ZG ? c * bb.b : bb.b;
-        return T2<T>{b0 + h * (P.w + P.v), b1 + P.v};
-    };
-    auto scb = [&](T v) -> T { return ZG ? c * v : v; };
-    int Ka = k0 >> 1;
-    T3<T> Ya[3] = {zero3, zero3, zero3}, Yb[3] = {zero3, zero3, zero3};
-    long q = (long)k0 * sz + (long)j * sy + i0;
-    T3<T> Pc = zero3;
-    T2<T> tc = zero2, td = zero2;
-    BP bp0{zero2, z0}, bp1{zero2, z0};
-    if (rowok) {
-        yplane(Ka, Ya);
-        if (Ka + 1 < mzc) yplane(Ka + 1, Yb);
-        Pc = (k0 & 1) ? mix3(Ya[1], Yb[1]) : Ya[1];
-        bp0 = bload(q);
-        if (k0 + 1 < mz) bp1 = bload(q + sz);
-        tc = tpb(Pc, bp0.p);
-        if (k0 > 0) {
-            T3<T> Pd;
-            if (k0 & 1) {
-                Pd = Ya[1];
-            } else {
-                T3<T> Yp[3] = {zero3, zero3, zero3};
-                yplane(Ka - 1, Yp);
-                Pd = mix3(Yp[1], Ya[1]);
-            }
-            if (act) td = tpair(Pd, q - sz);
-        }
-    }
-    const bool jb = (j == 0) || (j == my - 1);
-    for (int k = k0; k < k1; ++k, q += sz) {
-        T3<T> Pu = zero3;
-        T2<T> tu = zero2;
-        BP bp2{zero2, z0};
-        const bool need_sn = rowok && !jb && k != 0 && k != mz - 1;
-        if (rowok) {
-            if (k + 2 < mz) bp2 = bload(q + 2 * sz);
-            if ((k >> 1) != Ka) {
-                Ka = k >> 1;
-                Ya[0] = Yb[0]; Ya[1] = Yb[1]; Ya[2] = Yb[2];
-                if (Ka + 1 < mzc) yplane(Ka + 1, Yb);
-                else Yb[0] = Yb[1] = Yb[2] = zero3;
-            }
-            if (k + 1 < mz) {
-                Pu = (k & 1) ? Yb[1] : mix3(Ya[1], Yb[1]);
-                tu = tpb(Pu, bp1.p);
-            }
-            T2<T>* row = tbuf[k & 1][0];
-            if (act) row[(ty + 1) * TX + lane] = tc;
-            if (act && need_sn && lo_edge && j - 1 > 0)
-                row[lane] = tpair((k & 1) ? mix3(Ya[0], Yb[0]) : Ya[0], q - sy);
-            if (act && need_sn && hi_edge && j + 1 < my - 1)
-                row[(TY + 1) * TX + lane] = tpair((k & 1) ? mix3(Ya[2], Yb[2]) : Ya[2], q + sy);
-        }
-        __syncthreads();
-        if (rowok) {
-            const T wv = __shfl_up(tc.b, 1, TX), ev = __shfl_down(tc.a, 1, TX);
-            if (act) {
-                const T2<T>* row = tbuf[k & 1][0];
-                const T2<T> ts = (need_sn && j - 1 > 0) ? row[ty * TX + lane] : zero2;
-                const T2<T> tn = (need_sn && j + 1 < my - 1) ? row[(ty + 2) * TX + lane] : zero2;
-                const T2<T> tdd = (k - 1 > 0) ? td : zero2, tuu = (k + 1 < mz - 1) ? tu : zero2;
-                T wv2 = wv, ev2 = ev;
-                if (lane == 0 && i0 - 1 > 0) wv2 = scb(bp0.x) + Pc.w;
-                if (lane == TX - 1 && i1 + 1 < mx - 1) ev2 = scb(bp0.x) + h * (Pc.v + Pc.e);
-                T Aa, Ab, ia, ib;
-                if (jb || k == 0 || k == mz - 1) {
-                    op.bnd(i0, j, k, tc.a, Aa, ia);
-                    op.bnd(i1, j, k, tc.b, Ab, ib);
-                } else {
-                    op.interior(i0, j, k, tc.a, i0 - 1 > 0 ? wv2 : z0, i1 < mx - 1 ? tc.b : z0, ts.a, tn.a, tdd.a,
-                                tuu.a, Aa, ia);
-                    if (i1 == mx - 1) op.bnd(i1, j, k, tc.b, Ab, ib);
-                    else op.interior(i1, j, k, tc.b, tc.a, i1 + 1 < mx - 1 ? ev2 : z0, ts.b, tn.b, tdd.b, tuu.b, Ab,
-                                     ib);
-                }
-                const TV<T, 2> uu{{tc.a, tc.b}}, AA{{Aa, Ab}}, II{{ia, ib}};
-                // with ZG the base is the output's b, its one per-point array
-                if constexpr (ZG) {
-                    static_assert(Out::NPRE == 1, "the base is the output's b");
-                    const TV<T, 2> bb{{bp0.p.a, bp0.p.b}};
-                    out.template pointvp<2>(q, uu, AA, II, &bb, a0, a1);
-                } else {
-                    out.template pointv<2>(q, uu, AA, II, a0, a1);
-                }
-                if (I == 1) {      // fine 0: a boundary point
-                    const T t0 = scb(bp0.x) + Pc.w;
-                    op.bnd(0, j, k, t0, Aa, ia);
-                    if constexpr (ZG) out.pointp(q - 1, t0, Aa, ia, &bp0.x, a0, a1);
-                    else out.point(q - 1, t0, Aa, ia, a0, a1);
-                }
-            }
-        }
-        td = tc;
-        tc = tu;
-        Pc = Pu;
-        bp0 = bp1;
-        bp1 = bp2;
     }
 }
 

@@ -100,7 +100,7 @@ static size_t req_bytes(p4g_solver_s* s) {
     const long n = F.n;
     size_t b = 2 * n * sizeof(double) + 6 * n * sizeof(float);   // 5 Krylov vectors + fine-level r
     for (int l = 0; l + 1 < s->nlev; ++l) b += (l > 0 ? 3 : 2) * s->lv[l].n * sizeof(float);
-    const long nb = std::max<long>(nblocks(pair_grid(F.dim, F.mx, F.my, F.mz)), vblocks(n));
+    const long nb = std::max<long>(max_part_blocks(s->lv), vblocks(n));
     b += 2 * nb * sizeof(double);
     return b;
 }
@@ -384,7 +384,7 @@ static int req_admit(p4g_server_s* v, SrvReq* r) {
         }
         if (l > 0) P4G_TRY(req_alloc(r, (void**)&r->lv[l].r, nl * 4));
     }
-    r->npart = std::max<long>(nblocks(pair_grid(F.dim, F.mx, F.my, F.mz)), vblocks(n));
+    r->npart = std::max<long>(max_part_blocks(s->lv), vblocks(n));
     P4G_TRY(req_alloc(r, (void**)&r->part, 2 * r->npart * 8));
     r->slot = v->free_slots.back();
     v->free_slots.pop_back();

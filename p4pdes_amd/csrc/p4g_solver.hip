@@ -185,21 +185,33 @@ static dim3 stencil_grid_v(int dim, int mx, int my, int mz, int V) {
     return stencil_grid_k(dim, mx, my, mz, V, kchf(dim, my, mz));
 }
 // marching chunk of the fused prolongation + smoothing kernel: its per-chunk
-// set-up (coarse planes, the first iterates) is longer, so chunks are twice
-// those of the stencil passes (257^3: 149 us at 8 planes, 153 us at 4;
-// 513^3: 1088 us at 16, 1144 us at 8)
-static inline int kch_fused(int dim, int my, int mz) { return std::min(16, 2 * kchf(dim, my, mz)); }
+// set-up (coarse planes, the first iterates, the halo rows) is longer, so
+// chunks are longer than those of the stencil passes (3D, 6-row blocks:
+// 257^3 120 us at 16 planes, 134 us at 8; 513^3 855 us at 16, 939 us at 8)
+static inline int kch_fused(int dim, int my, int mz) {
+    const int slow = dim == 3 ? mz : my;
+    return slow >= 200 ? 16 : std::min(16, 2 * kchf(dim, my, mz));
+}
 template <typename T>
 static dim3 stencil_grid(int dim, int mx, int my, int mz) { return stencil_grid_v(dim, mx, my, mz, VecOf<T>::V); }
-// two points per lane: the largest stencil grid (partial-sum buffers are
-// sized by it)
+// two points per lane: the largest stencil grid
 static dim3 pair_grid(int dim, int mx, int my, int mz) { return stencil_grid_v(dim, mx, my, mz, 2); }
 // the fused prolongation + smoothing kernel (two points per lane)
 static dim3 fused_grid(int dim, int mx, int my, int mz) {
-    return stencil_grid_k(dim, mx, my, mz, 2, kch_fused(dim, my, mz));
+    const int kch = kch_fused(dim, my, mz);
+    if (dim == 3) return dim3((mx - 1 + 2 * TX - 1) / (2 * TX), (my + TYF - 1) / TYF, (mz + kch - 1) / kch);
+    return stencil_grid_k(dim, mx, my, mz, 2, kch);
 }
 static dim3 stencil_block(int dim) { return dim == 1 ? dim3(TX, 1, 1) : dim3(TX, TY, 1); }
 static long nblocks(dim3 g) { return (long)g.x * g.y * g.z; }
+// the most blocks of any pass that writes block partial sums, over the
+// levels (the marching chunks depend on the size, so not always the finest)
+static long max_part_blocks(const std::vector<LevelInfo>& lv) {
+    long m = 0;
+    for (const LevelInfo& L : lv)
+        m = std::max(m, std::max(nblocks(pair_grid(L.dim, L.mx, L.my, L.mz)), nblocks(fused_grid(L.dim, L.mx, L.my, L.mz))));
+    return m;
+}
 // transfers: lanes over coarse columns I = 1..mxc-1, rows and march chunks
 // over the grid (my, mz) the kernel walks (coarse: restriction, fine: prolongation)
 static dim3 transfer_grid(int dim, int mxc, int my, int mz) {
@@ -598,7 +610,7 @@ struct Engine {
         hipStream_t s = cx->stream;
         pend.on = false;
         pend.nparts = gb;
-        return cx->launch(P4G_K_CG_UPDATE, true, 3.0 * n * sizeof(T), [&] {
+        return cx->launch(P4G_K_CG_RESID, true, 3.0 * n * sizeof(T), [&] {
             if (pend.rr) k_r_update<T, 1><<<gb, 256, 0, s>>>(n, pend.alpha, 0, pend.r, pend.w, pend.rn, cx->part_rr);
             else k_r_update<T, 0><<<gb, 256, 0, s>>>(n, pend.alpha, 0, pend.r, pend.w, pend.rn, cx->part_rr);
         });
@@ -1318,7 +1330,7 @@ int solver_setup(p4g_solver_s* s) {
     if (!s->cx.stream) HIP_TRY(hipStreamCreate(&s->cx.stream));
     // partial-sum capacity: the largest grid of blocks of any fused pass
     const LevelInfo& F = s->lv.back();
-    long cap = std::max<long>(VBLOCKS, nblocks(pair_grid(F.dim, F.mx, F.my, F.mz)));
+    long cap = std::max<long>(VBLOCKS, max_part_blocks(s->lv));
     if (!s->cx.part) {
         HIP_TRY(hipMalloc(&s->cx.part, 2 * cap * sizeof(double)));
         HIP_TRY(hipMalloc(&s->cx.part_rr, 2 * cap * sizeof(double)));
