@@ -911,27 +911,62 @@ __device__ __forceinline__ void restrict_body(int bx, int by, int bz, int mxf, i
         }
         return;
     }
+    // 3D: march the fine planes 2 K0 - 1 .. 2 K1 - 1 of the chunk; the three
+    // rows of plane k+1 are loaded while plane k is summed
     const int J = by * TY + threadIdx.y;
     if (J >= myc) return;
     const int K0 = bz * kch, K1 = min(K0 + kch, mzc);
     if (K0 >= K1) return;
     const int j0 = 2 * J;
     const long sxy = (long)mxf * myf;
-    auto plane = [&](int k) -> T2<T> {     // x,y-weighted sums on fine plane k
-        const long rb = (long)k * sxy + (long)j0 * mxf;
-        const T2<T> mid = row(rb);
-        const T2<T> lo = (j0 - 1 >= 0) ? row(rb - mxf) : zz;
-        const T2<T> hi = (j0 + 1 < myf) ? row(rb + mxf) : zz;
-        return comb(mid, lo, hi);
+    const bool hlo = j0 - 1 >= 0, hhi = j0 + 1 < myf;
+    // a fine row as loaded: the lane's pair (f-1, f) and one extra value
+    // (last lane: f+1; the lane at I = 1: fine 0, with fine 1 = its pair's first)
+    struct RR {
+        T2<T> v;
+        T x;
     };
-    T2<T> slo = (2 * K0 - 1 >= 0) ? plane(2 * K0 - 1) : zz;
-    for (int K = K0; K < K1; ++K) {
-        const int k = 2 * K;
-        const T2<T> smid = plane(k);
-        const T2<T> shi = (k + 1 < mzf) ? plane(k + 1) : zz;
-        store(((long)K * myc + J) * mxc, comb(smid, slo, shi));
-        slo = shi;
+    auto rload = [&](long rb) -> RR {
+        RR r{zz, T(0)};
+        if (act) {
+            r.v = ld2(rf + rb + f - 1);
+            if (own0) r.x = rf[rb];
+            else if (lane == TX - 1 && he) r.x = rf[rb + f + 1];
+        }
+        return r;
+    };
+    auto rsum = [&](const RR& r) -> T2<T> {   // every lane of the wave takes part
+        T nx = __shfl_down(r.v.a, 1, TX);
+        if (act && lane == TX - 1 && he) nx = r.x;
+        T2<T> o;
+        o.a = r.v.b + h * (r.v.a + (he ? nx : T(0)));
+        o.b = own0 ? r.x + h * r.v.a : T(0);
+        return o;
+    };
+    RR c0{zz, T(0)}, c1{zz, T(0)}, c2{zz, T(0)}, n0{zz, T(0)}, n1{zz, T(0)}, n2{zz, T(0)};
+    auto pload = [&](int k, RR& lo, RR& mid, RR& hi) {
+        const long rb = (long)k * sxy + (long)j0 * mxf;
+        mid = rload(rb);
+        if (hlo) lo = rload(rb - mxf);
+        if (hhi) hi = rload(rb + mxf);
+    };
+    const int kf0 = max(2 * K0 - 1, 0), kf1 = min(2 * K1 - 1, mzf - 1);
+    pload(kf0, c0, c1, c2);
+    T2<T> slo = zz, smid = zz;
+    for (int k = kf0; k <= kf1; ++k) {
+        if (k + 1 <= kf1) pload(k + 1, n0, n1, n2);
+        const T2<T> s = comb(rsum(c1), hlo ? rsum(c0) : zz, hhi ? rsum(c2) : zz);
+        if (k & 1) {                       // plane 2K+1: the last of K, the first of K+1
+            if (k > 2 * K0) store(((long)(k >> 1) * myc + J) * mxc, comb(smid, slo, s));
+            slo = s;
+        } else {
+            smid = s;
+        }
+        c0 = n0;
+        c1 = n1;
+        c2 = n2;
     }
+    if (!(kf1 & 1)) store(((long)(kf1 >> 1) * myc + J) * mxc, comb(smid, slo, zz));   // the last fine plane
 }
 
 template <typename T, int DIM>

@@ -162,11 +162,12 @@ static GridRef gref_fused(const LevelInfo& L, long start) {
     return r;
 }
 
-// transfer launch over the grid L with lanes over the coarse columns 1..mxc-1
-static GridRef gref_transfer(const LevelInfo& L, int mxc, long start) {
-    const dim3 g = transfer_grid(L.dim, mxc, L.my, L.mz);
+// restriction launch over the coarse grid L (lanes over its columns 1..mx-1)
+static GridRef gref_restrict(const LevelInfo& L, long start) {
+    const int kch = kch_restrict(L.dim, L.my, L.mz);
+    const dim3 g = transfer_grid(L.dim, L.mx, L.my, L.mz, kch);
     GridRef r;
-    r.mx = L.mx; r.my = L.my; r.mz = L.mz; r.kch = kchf(L.dim, L.my, L.mz);
+    r.mx = L.mx; r.my = L.my; r.mz = L.mz; r.kch = kch;
     r.gx = g.x; r.gy = g.y; r.gz = g.z; r.start = start;
     return r;
 }
@@ -290,7 +291,7 @@ static int build_vcycle(p4g_server_s* v, Packer& pk, std::vector<BLaunch>& seq) 
             it.out = OutScaleU<float, OutResid<float, 0>>{c, OutResid<float, 0>{bvec(q.r, q.l), q.r->lv[q.l].r}};
             ri.push_back(it);
             RestrictItem<float> rt;
-            rt.g = gref_transfer(C, C.mx, s2);
+            rt.g = gref_restrict(C, s2);
             s2 += gblocks(rt.g);
             rt.mxf = L.mx; rt.myf = L.my; rt.mzf = L.mz;
             rt.rf = q.r->lv[q.l].r;

@@ -214,11 +214,18 @@ static long max_part_blocks(const std::vector<LevelInfo>& lv) {
 }
 // transfers: lanes over coarse columns I = 1..mxc-1, rows and march chunks
 // over the grid (my, mz) the kernel walks (coarse: restriction, fine: prolongation)
-static dim3 transfer_grid(int dim, int mxc, int my, int mz) {
+static dim3 transfer_grid(int dim, int mxc, int my, int mz, int kch) {
     const int gx = std::max(1, (mxc - 1 + TX - 1) / TX);
-    if (dim == 3) return dim3(gx, (my + TY - 1) / TY, (mz + kchf(dim, my, mz) - 1) / kchf(dim, my, mz));
-    if (dim == 2) return dim3(gx, ((my + kchf(dim, my, mz) - 1) / kchf(dim, my, mz) + TY - 1) / TY, 1);
+    if (dim == 3) return dim3(gx, (my + TY - 1) / TY, (mz + kch - 1) / kch);
+    if (dim == 2) return dim3(gx, ((my + kch - 1) / kch + TY - 1) / TY, 1);
     return dim3(gx, 1, 1);
+}
+// marching chunk of the restriction, in coarse planes (rows in 2D) of the
+// coarse grid (my, mz): each is two fine planes read (fine 257^3: 45 us at
+// 1-2, 48 us at 4; fine 513^3: 293 us at 4, 303 us at 2 or 8)
+static inline int kch_restrict(int dim, int my, int mz) {
+    const int slow = dim == 3 ? mz : (dim == 2 ? my : 1);
+    return slow >= 200 ? 4 : (slow >= 100 ? 2 : 1);
 }
 
 // ------------------------------------------------------------- engine
@@ -380,15 +387,16 @@ struct Engine {
     int restrict_(int l, const T* rf, T* bc) {   // level l -> l-1
         const LevelInfo& F = (*lv)[l];
         const LevelInfo& C = (*lv)[l - 1];
-        const dim3 g = transfer_grid(C.dim, C.mx, C.my, C.mz), b = stencil_block(C.dim);
+        const int kch = kch_restrict(C.dim, C.my, C.mz);
+        const dim3 g = transfer_grid(C.dim, C.mx, C.my, C.mz, kch), b = stencil_block(C.dim);
         hipStream_t s = cx->stream;
         return cx->launch(P4G_K_RESTRICT, l == nlev - 1, double(F.n + C.n) * sizeof(T), [&] {
             if (F.dim == 3)
-                k_restrict<T, 3><<<g, b, 0, s>>>(F.mx, F.my, F.mz, C.mx, C.my, C.mz, kchf(C.dim, C.my, C.mz), rf, bc);
+                k_restrict<T, 3><<<g, b, 0, s>>>(F.mx, F.my, F.mz, C.mx, C.my, C.mz, kch, rf, bc);
             else if (F.dim == 2)
-                k_restrict<T, 2><<<g, b, 0, s>>>(F.mx, F.my, F.mz, C.mx, C.my, C.mz, kchf(C.dim, C.my, C.mz), rf, bc);
+                k_restrict<T, 2><<<g, b, 0, s>>>(F.mx, F.my, F.mz, C.mx, C.my, C.mz, kch, rf, bc);
             else
-                k_restrict<T, 1><<<g, b, 0, s>>>(F.mx, F.my, F.mz, C.mx, C.my, C.mz, kchf(C.dim, C.my, C.mz), rf, bc);
+                k_restrict<T, 1><<<g, b, 0, s>>>(F.mx, F.my, F.mz, C.mx, C.my, C.mz, kch, rf, bc);
         });
     }
 
@@ -440,7 +448,7 @@ struct Engine {
     int prolong(int l, bool zg, const T* base, T c, const T* xc, T* out) {
         const LevelInfo& F = (*lv)[l];
         const LevelInfo& C = (*lv)[l - 1];
-        const dim3 g = transfer_grid(F.dim, C.mx, F.my, F.mz), b = stencil_block(F.dim);
+        const dim3 g = transfer_grid(F.dim, C.mx, F.my, F.mz, kchf(F.dim, F.my, F.mz)), b = stencil_block(F.dim);
         hipStream_t s = cx->stream;
         return cx->launch(P4G_K_PROLONG, l == nlev - 1, double(2 * F.n + C.n) * sizeof(T), [&] {
             if (F.dim == 3) {
