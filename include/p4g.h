@@ -117,8 +117,9 @@ int p4g_da_destroy(p4g_DA *da);
  *   -p4g_fuse_cg auto|1|0 (default auto): the CG residual update fused into
  *       the first kernel of the multigrid cycle and the direction update
  *       with the operator apply; auto fuses when one fine-level vector is
- *       larger than 256 MiB (below that the separate passes run from the
- *       last-level cache and are faster).
+ *       larger than 768 MiB (below that the separate streaming passes are
+ *       faster); -p4g_fuse_r and -p4g_fuse_dir (auto|1|0, default: the
+ *       value of -p4g_fuse_cg) set the two fusions separately.
  * Options of PETSc objects the device path does not implement (-pc_type
  * ilu|sor|lu, -mg_levels_pc_type sor, -pc_mg_galerkin, ...) are rejected
  * with PETSC_ERR_SUP. */

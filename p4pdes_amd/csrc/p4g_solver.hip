@@ -242,13 +242,10 @@ struct Engine {
     // steps the prolonged iterate would have to be stored for the Chebyshev
     // recurrence, and the fused kernel then loses to the separate passes)
     bool fuse_post = true;
-    // CG updates fused into stencil passes (the residual update into the
-    // first kernel of the cycle, the direction update with the operator
-    // apply) pay off once a fine vector no longer fits the 256 MB last-level
-    // cache; below that the split passes re-read from the cache and are
-    // faster (257^3 FP64: residual 151 us split / 180 fused, direction
-    // 203 / 232; 513^3: 1388 / 1329 and 1778 / 1773)
-    bool fuse_cg = true;
+    // CG updates fused into stencil passes: the residual update into the
+    // first kernel of the cycle (fuse_r), the direction update with the
+    // operator apply (fuse_dir)
+    bool fuse_r = true, fuse_dir = true;
     struct Dev {
         T *x = nullptr, *b = nullptr, *r = nullptr, *t[4] = {nullptr, nullptr, nullptr, nullptr};
         T* kc = nullptr;
@@ -626,14 +623,14 @@ struct Engine {
     // whether the fine-level cycle starts with a kernel reading the residual
     // through a source policy (then the pending update is fused into it)
     bool pend_fusable(const LevelInfo& L) const {
-        return fuse_cg && pctype == PCT_MG && nlev > 1 && zg_fusable(L) &&
+        return fuse_r && pctype == PCT_MG && nlev > 1 && zg_fusable(L) &&
                (nu == 1 || !(nu >= 3 && smoother == SM_CHEB));
     }
     // p_new = z + beta p_old ; w = A p_new ; (p_new, w) ; x += alpha p_old
     template <typename X>
     int direction(const T* zz, const T* pold, T* pnew, X* xx, const double* beta, const double* alpha, long* nbs) {
         const long n = (*lv)[nlev - 1].n;
-        if (fuse_cg)
+        if (fuse_dir)
             return stencil(nlev - 1, SrcDirX<T, X>{zz, pold, pnew, beta, xx, alpha}, OutApplyDot<T>{w},
                            P4G_K_STENCIL_DOT, 4.0 + 2.0 * sizeof(X) / sizeof(T), nbs, false);
         hipStream_t s = cx->stream;
@@ -924,7 +921,7 @@ struct p4g_solver_s {
     double reliable_delta = 1e-1;
     bool fp32_diff = true;    // FP32 engine: difference-form rows
     bool mixed_x64 = true;    // mixed: accumulate the iterate in FP64 every iteration
-    int fuse_cg = -1;         // CG updates fused into stencil passes: 1 on, 0 off, -1 by vector size
+    int fuse_r = -1, fuse_dir = -1;   // CG updates fused into stencil passes: 1 on, 0 off, -1 by rule
     // state
     bool ready = false;
     std::vector<LevelInfo> lv;
@@ -1231,11 +1228,17 @@ static int configure(p4g_solver_s* s) {
     s->fp32_diff = o.getb("p4g_fp32_diff_form", true);
     s->mixed_x64 = o.getb("p4g_mixed_x64", true);
     {
-        const std::string f = o.gets("p4g_fuse_cg", "auto");
-        if (f == "auto") s->fuse_cg = -1;
-        else if (f == "1" || f == "true" || f == "yes") s->fuse_cg = 1;
-        else if (f == "0" || f == "false" || f == "no") s->fuse_cg = 0;
-        else return set_error(PETSC_ERR_ARG_OUTOFRANGE, "-p4g_fuse_cg must be auto, 1 or 0");
+        auto tri = [&](const std::string& f, const char* name, int& v) -> int {
+            if (f == "auto") v = -1;
+            else if (f == "1" || f == "true" || f == "yes") v = 1;
+            else if (f == "0" || f == "false" || f == "no") v = 0;
+            else return set_error(PETSC_ERR_ARG_OUTOFRANGE, "-%s must be auto, 1 or 0", name);
+            return 0;
+        };
+        const std::string both = o.gets("p4g_fuse_cg", "auto");
+        P4G_TRY(tri(both, "p4g_fuse_cg", s->fuse_r));
+        P4G_TRY(tri(o.gets("p4g_fuse_r", both), "p4g_fuse_r", s->fuse_r));
+        P4G_TRY(tri(o.gets("p4g_fuse_dir", both), "p4g_fuse_dir", s->fuse_dir));
     }
     if (!(s->reliable_delta > 0.0 && s->reliable_delta < 1.0))
         return set_error(PETSC_ERR_ARG_OUTOFRANGE, "-p4g_reliable_delta must be in (0,1)");
@@ -1331,6 +1334,20 @@ static int build_levels(p4g_solver_s* s) {
     return 0;
 }
 
+// Which CG updates run inside stencil passes (the residual update in the
+// first kernel of the cycle, the direction update with the operator apply).
+// The fused passes read two arrays at stencil neighbours; the split passes
+// stream, and re-read what is still in the 256 MB last-level cache.
+// Measured per solve: 257^3 FP64 7.58 ms split / 7.70 residual fused /
+// 7.90 both; 513^3 FP32 (mixed) 53.6 / 54.0 / 54.1; 513^3 FP64 94.3 split
+// residual / 93.7 both.  Both are fused above 768 MiB per fine vector.
+template <typename T>
+static void fuse_rules(const p4g_solver_s* s, Engine<T>* e, double vec_bytes) {
+    const bool big = vec_bytes > 768.0 * 1048576.0;
+    e->fuse_r = s->fuse_r >= 0 ? s->fuse_r == 1 : big;
+    e->fuse_dir = s->fuse_dir >= 0 ? s->fuse_dir == 1 : big;
+}
+
 int solver_setup(p4g_solver_s* s) {
     if (s->ready) return 0;
     const double t0 = now_ms();
@@ -1368,7 +1385,7 @@ int solver_setup(p4g_solver_s* s) {
         s->e32.reset(new Engine<float>);
         cfg(s->e32);
         s->e32->diff_form = s->fp32_diff;
-        s->e32->fuse_cg = s->fuse_cg >= 0 ? s->fuse_cg == 1 : (double)F.n * sizeof(float) > 256.0 * 1048576.0;
+        fuse_rules(s, s->e32.get(), F.n * sizeof(float));
         P4G_TRY(s->e32->setup(&s->cx, &s->lv, s->Ainv, s->n0));
         const long n = F.n;
         P4G_TRY(dalloc(s, (void**)&s->xa, n * sizeof(double)));
@@ -1378,7 +1395,7 @@ int solver_setup(p4g_solver_s* s) {
     } else {
         s->e64.reset(new Engine<double>);
         cfg(s->e64);
-        s->e64->fuse_cg = s->fuse_cg >= 0 ? s->fuse_cg == 1 : (double)F.n * sizeof(double) > 256.0 * 1048576.0;
+        fuse_rules(s, s->e64.get(), F.n * sizeof(double));
         P4G_TRY(s->e64->setup(&s->cx, &s->lv, s->Ainv, s->n0));
     }
     HIP_TRY(hipStreamSynchronize(s->cx.stream));

@@ -79,6 +79,18 @@ def test_unsupported_options_rejected(opts, fragment):
     L.p4g_da_destroy(ctypes.byref(da))
 
 
+@pytest.mark.parametrize("opts", ["-p4g_fuse_cg maybe", "-p4g_fuse_r 2", "-p4g_fuse_dir on"])
+def test_fusion_options_validated(opts):
+    L = capi.lib()
+    da = ctypes.c_void_p()
+    capi.check(L.p4g_da_create(3, 17, 17, 17, 1.0, 1.0, 1.0, ctypes.byref(da)))
+    s = ctypes.c_void_p()
+    rc = L.p4g_solver_create(da, ("-pc_type mg -mg_levels_pc_type jacobi " + opts).encode(), ctypes.byref(s))
+    assert rc == 63   # PETSC_ERR_ARG_OUTOFRANGE
+    assert "must be auto, 1 or 0" in capi.last_error()
+    L.p4g_da_destroy(ctypes.byref(da))
+
+
 @pytest.mark.parametrize("args,code", [
     ("-fsh_cx -1 -pc_type mg", 2), ("-fsh_problem manuexp -fsh_cx 2 -pc_type mg", 3),
     ("-fsh_dim 4 -pc_type mg", 1)])

@@ -57,6 +57,10 @@ CASES = {
     "3d_richardson3_fused_cg": ("-fsh_dim 3 -fsh_problem manusin -da_refine 4 -pc_type mg -mg_levels_ksp_type "
                                 "richardson -mg_levels_pc_type jacobi -mg_levels_ksp_max_it 3 "
                                 "-mg_levels_ksp_richardson_scale 0.7 -ksp_rtol 1e-10 -p4g_fuse_cg 1"),
+    "3d_cheb1_fused_r_only": ("-fsh_dim 3 -fsh_problem manusin -da_refine 4 %s -mg_levels_ksp_max_it 1 "
+                              "-ksp_rtol 1e-10 -p4g_fuse_r 1 -p4g_fuse_dir 0" % MG),
+    "3d_cheb2_fused_dir_only": ("-fsh_dim 3 -fsh_problem manusin -fsh_noise_sigma 1e-3 -da_refine 4 %s "
+                                "-ksp_rtol 1e-10 -p4g_fuse_r 0 -p4g_fuse_dir 1" % MG),
     "3d_richardson3": ("-fsh_dim 3 -fsh_problem manusin -da_refine 4 -pc_type mg -mg_levels_ksp_type "
                        "richardson -mg_levels_pc_type jacobi -mg_levels_ksp_max_it 3 "
                        "-mg_levels_ksp_richardson_scale 0.7 -ksp_rtol 1e-10"),
@@ -176,7 +180,8 @@ def test_mixed_matches_fp64(refine):
 
 
 @pytest.mark.parametrize("variant", ["-p4g_mixed_x64 0", "-p4g_fp32_diff_form 0", "-p4g_reliable_delta 0.01",
-                                     "-fsh_problem varcoef", "-p4g_fuse_cg 1", "-p4g_fuse_cg 1 -fsh_problem varcoef"])
+                                     "-fsh_problem varcoef", "-p4g_fuse_cg 1", "-p4g_fuse_cg 1 -fsh_problem varcoef",
+                                     "-p4g_fuse_r 1 -p4g_fuse_dir 0", "-p4g_fuse_r 0 -p4g_fuse_dir 1"])
 def test_mixed_variants(variant):
     # every mixed-precision variant converges on the FP64 true residual to the
     # same discrete solution as the FP64 oracle
