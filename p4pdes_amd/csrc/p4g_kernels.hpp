@@ -854,6 +854,128 @@ k_stencil(int mx, int my, int mz, int kch, Op op, Src src, Out out, double* __re
     block_partials<Out::NDOT>(a0, a1, part, tl.id);
 }
 
+// ------------------------------------------- Galerkin coarse operators
+
+// A level operator in 27-point stencil form: plane c = 9 (oz+1) + 3 (oy+1)
+// + (ox+1) of g holds, for every point q, the coefficient of row q toward
+// the point at offset (ox, oy, oz) (zero where that point is outside the
+// grid).  Coarse operators A_c = P^T A P (-pc_mg_galerkin) are of this form.
+template <typename T>
+struct Op27 {
+    const double* __restrict__ g;
+    long n;
+    int mx, my, mz;
+    int jac;                     // Jacobi factor 1 / A_qq, else 1
+};
+
+// Generic pass on a 27-point level: a thread per point (grid-stride), the
+// operand and its neighbours from Src, the row sum in column order.
+// Launch: <<<blocks, 256>>>, one partial-sum slot per block.
+template <typename T, class Src, class Out>
+__global__ void __launch_bounds__(256)
+k_stencil27(Op27<T> op, Src src, Out out, double* __restrict__ part) {
+    double a0 = 0.0, a1 = 0.0;
+    const long sxy = (long)op.mx * op.my;
+    for (long q = blockIdx.x * 256L + threadIdx.x; q < op.n; q += (long)gridDim.x * 256L) {
+        const int i = (int)(q % op.mx);
+        const long r = q / op.mx;
+        const int j = (int)(r % op.my), k = (int)(r / op.my);
+        T Au = T(0);
+        for (int oz = -1; oz <= 1; ++oz) {
+            if (k + oz < 0 || k + oz >= op.mz) continue;
+            for (int oy = -1; oy <= 1; ++oy) {
+                if (j + oy < 0 || j + oy >= op.my) continue;
+                for (int ox = -1; ox <= 1; ++ox) {
+                    if (i + ox < 0 || i + ox >= op.mx) continue;
+                    const int c = 9 * (oz + 1) + 3 * (oy + 1) + (ox + 1);
+                    const T a = T(op.g[c * op.n + q]);
+                    if (a != T(0)) Au += a * src.load(q + oz * sxy + oy * (long)op.mx + ox);
+                }
+            }
+        }
+        const T uc = src.load(q);
+        src.put(q, uc);
+        const T idg = op.jac ? T(1) / T(op.g[13 * op.n + q]) : T(1);
+        out.point(q, uc, Au, idg, a0, a1);
+    }
+    block_partials<Out::NDOT, 4>(a0, a1, part, blockIdx.x);
+}
+
+// The fine operator of a Galerkin product: the rediscretized constant
+// 5/7-point operator (g == nullptr: Dirichlet rows diagonal-only, interior
+// rows coupled to interior neighbours) or a 27-point level.
+struct GalFine {
+    int dim, mx, my, mz;
+    double d, db, sx, sy, sz;
+    const double* g;
+    long n;
+};
+
+__device__ __forceinline__ bool gal_bnd(const GalFine& f, int i, int j, int k) {
+    return i == 0 || i == f.mx - 1 || (f.dim >= 2 && (j == 0 || j == f.my - 1)) ||
+           (f.dim == 3 && (k == 0 || k == f.mz - 1));
+}
+
+// coefficient of the fine operator in row (i, j, k) toward offset (ox, oy, oz)
+__device__ __forceinline__ double gal_fine(const GalFine& f, int i, int j, int k, int ox, int oy, int oz) {
+    if (f.g) return f.g[(long)(9 * (oz + 1) + 3 * (oy + 1) + (ox + 1)) * f.n + ((long)k * f.my + j) * f.mx + i];
+    const bool bnd = gal_bnd(f, i, j, k);
+    if (ox == 0 && oy == 0 && oz == 0) return bnd ? f.db : f.d;
+    if (bnd || (ox != 0) + (oy != 0) + (oz != 0) != 1) return 0.0;
+    if (gal_bnd(f, i + ox, j + oy, k + oz)) return 0.0;
+    return ox ? -f.sx : (oy ? -f.sy : -f.sz);
+}
+
+// A_c = P^T A_f P in 27-point form, a thread per coarse point (grid-stride).
+// P is the Q1 interpolation on all vertices (weights 1, 1/2 per axis, fine
+// 2I <-> coarse I).  Coefficient (I, I+o) = sum over fine i near 2I and
+// fine j = i + s near 2(I+o) of P(i,I) A_f(i,j) P(j,I+o).
+__global__ void __launch_bounds__(256)
+k_galerkin(GalFine f, int mxc, int myc, int mzc, double* __restrict__ gc) {
+    const long nc = (long)mxc * myc * mzc;
+    auto wt = [](int e) -> double { return e == 0 ? 1.0 : (e == 1 || e == -1 ? 0.5 : 0.0); };
+    for (long q = blockIdx.x * 256L + threadIdx.x; q < nc; q += (long)gridDim.x * 256L) {
+        const int I = (int)(q % mxc);
+        const long r = q / mxc;
+        const int J = (int)(r % myc), K = (int)(r / myc);
+        for (int c = 0; c < 27; ++c) {
+            const int oz = c / 9 - 1, oy = (c / 3) % 3 - 1, ox = c % 3 - 1;
+            const int I2 = I + ox, J2 = J + oy, K2 = K + oz;
+            double acc = 0.0;
+            if (I2 >= 0 && I2 < mxc && J2 >= 0 && J2 < myc && K2 >= 0 && K2 < mzc) {
+                for (int dz = -1; dz <= 1; ++dz) {
+                    const int fk = 2 * K + dz;
+                    if (fk < 0 || fk >= f.mz) continue;
+                    for (int dy = -1; dy <= 1; ++dy) {
+                        const int fj = 2 * J + dy;
+                        if (fj < 0 || fj >= f.my) continue;
+                        for (int dx = -1; dx <= 1; ++dx) {
+                            const int fi = 2 * I + dx;
+                            if (fi < 0 || fi >= f.mx) continue;
+                            const double wi = wt(dx) * wt(dy) * wt(dz);
+                            for (int sz = -1; sz <= 1; ++sz) {
+                                const int gk = fk + sz, ez = gk - 2 * K2;
+                                if (gk < 0 || gk >= f.mz || ez < -1 || ez > 1) continue;
+                                for (int sy = -1; sy <= 1; ++sy) {
+                                    const int gj = fj + sy, ey = gj - 2 * J2;
+                                    if (gj < 0 || gj >= f.my || ey < -1 || ey > 1) continue;
+                                    for (int sx = -1; sx <= 1; ++sx) {
+                                        const int gi = fi + sx, ex = gi - 2 * I2;
+                                        if (gi < 0 || gi >= f.mx || ex < -1 || ex > 1) continue;
+                                        const double a = gal_fine(f, fi, fj, fk, sx, sy, sz);
+                                        if (a != 0.0) acc += wi * a * (wt(ex) * wt(ey) * wt(ez));
+                                    }
+                                }
+                            }
+                        }
+                    }
+                }
+            }
+            gc[c * nc + q] = acc;
+        }
+    }
+}
+
 // ------------------------------------------------------------ transfers
 
 // b_c = P^T r_f, unscaled full weighting (weights 1, 1/2 per dimension).

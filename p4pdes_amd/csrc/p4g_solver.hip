@@ -79,6 +79,7 @@ struct LevelInfo {
     int cx = 0, cy = 0, cz = 0;
     double ax = 0, ay = 0, az = 0;
     double emin = 0, emax = 0;                       // Chebyshev bounds
+    bool gal = false;                                // Galerkin operator P^T A P in 27-point form
 };
 
 // ------------------------------------------------------------- timing
@@ -246,6 +247,8 @@ struct Engine {
     // first kernel of the cycle (fuse_r), the direction update with the
     // operator apply (fuse_dir)
     bool fuse_r = true, fuse_dir = true;
+    // per level: 27-point Galerkin coefficients (FP64, device) or null
+    std::vector<const double*> g27;
     struct Dev {
         T *x = nullptr, *b = nullptr, *r = nullptr, *t[4] = {nullptr, nullptr, nullptr, nullptr};
         T* kc = nullptr;
@@ -284,7 +287,7 @@ struct Engine {
     }
 
     bool zg_fusable(const LevelInfo& L) const {
-        return !L.var && L.d == L.db && (smoother == SM_CHEB || smoother == SM_RICH);
+        return !L.var && !L.gal && L.d == L.db && (smoother == SM_CHEB || smoother == SM_RICH);
     }
 
     int setup(Ctx* c, std::vector<LevelInfo>* levels, const std::vector<double>& Ainv_host,
@@ -351,6 +354,16 @@ struct Engine {
     template <class Src, class Out>
     int stencil(int l, Src src, Out out, int cls, double passes, long* nb, bool jac, double* pbuf = nullptr) {
         const LevelInfo& L = (*lv)[l];
+        if (L.gal) {   // a Galerkin level: the 27-point pass
+            const int gb = vblocks(L.n);
+            if (nb) *nb = gb;
+            const Op27<T> o{g27[l], L.n, L.mx, L.my, L.mz, jac ? 1 : 0};
+            hipStream_t s = cx->stream;
+            double* part = pbuf ? pbuf : cx->part;
+            const double bytes = passes * L.n * sizeof(T) + 27.0 * L.n * sizeof(double);
+            return cx->launch(cls, l == nlev - 1, bytes,
+                              [&] { k_stencil27<T, Src, Out><<<gb, 256, 0, s>>>(o, src, out, part); });
+        }
         const dim3 g = stencil_grid<T>(L.dim, L.mx, L.my, L.mz), b = stencil_block(L.dim);
         if (nb) *nb = nblocks(g);
         if (Out::NDOT && nblocks(g) > cx->npart_cap)
@@ -538,7 +551,7 @@ struct Engine {
                     pm1_implicit = true;
                     continue;
                 }
-                if (L.var) {
+                if (L.var || L.gal) {
                     // variable diagonal: a stencil pass with u = 0 gives
                     // y = c2 D^-1 b
                     if (last && dots)
@@ -694,7 +707,7 @@ struct Engine {
         // x = pre + P x_c, then post-smoothing into x
         T* x0 = zgf ? d.t[0] : xpre;
         const T* pre = zgf ? b : xpre;
-        if (L.dim >= 2 && fuse_post && nu == 1) {
+        if (L.dim >= 2 && fuse_post && nu == 1 && !L.gal) {
             // the post-smoothing step reads the prolonged iterate on the fly
             if (dots) P4G_TRY(post_first<true>(l, zgf, pre, T(s * idg), c.x, b, x, T(s), nb));
             else P4G_TRY(post_first<false>(l, zgf, pre, T(s * idg), c.x, b, x, T(s), nb));
@@ -922,10 +935,12 @@ struct p4g_solver_s {
     bool fp32_diff = true;    // FP32 engine: difference-form rows
     bool mixed_x64 = true;    // mixed: accumulate the iterate in FP64 every iteration
     int fuse_r = -1, fuse_dir = -1;   // CG updates fused into stencil passes: 1 on, 0 off, -1 by rule
+    bool galerkin = false;            // coarse operators P^T A P (-pc_mg_galerkin)
     // state
     bool ready = false;
     std::vector<LevelInfo> lv;
     std::vector<double> Ainv;   // dense inverse of the coarsest operator (host)
+    std::vector<double*> g27;   // per level: 27-point Galerkin coefficients (device, FP64) or null
     int n0 = 0;
     Ctx cx;
     std::unique_ptr<Engine<double>> e64;
@@ -1006,15 +1021,31 @@ static void host_row(const LevelInfo& L, int i, int j, int k, double& dg, double
     s[4] = (k - 1 > 0) ? kd : 0; s[5] = (k + 1 < L.mz - 1) ? ku : 0;
 }
 
-// dense Cholesky of the coarsest operator and the inverse of its factor
-static int coarse_factor(const LevelInfo& L, std::vector<double>& Ainv) {
+// dense Cholesky of the coarsest operator and the inverse of its factor;
+// g27: the operator's 27-point coefficients (device), else its 5/7-point rows
+static int coarse_factor(const LevelInfo& L, const double* g27, std::vector<double>& Ainv) {
     const int n = (int)L.n;
     if (n > MAX_COARSE)
         return set_error(PETSC_ERR_SUP, "coarsest grid has %d points (> %d); use more multigrid levels",
                          n, MAX_COARSE);
     std::vector<double> A((size_t)n * n, 0.0);
     const int mx = L.mx, my = L.my;
-    for (int k = 0; k < L.mz; ++k)
+    if (g27) {
+        std::vector<double> g((size_t)27 * n);
+        HIP_TRY(hipMemcpy(g.data(), g27, g.size() * sizeof(double), hipMemcpyDeviceToHost));
+        for (int k = 0; k < L.mz; ++k)
+            for (int j = 0; j < L.my; ++j)
+                for (int i = 0; i < L.mx; ++i) {
+                    const long q = ((long)k * my + j) * mx + i;
+                    for (int c = 0; c < 27; ++c) {
+                        const int oz = c / 9 - 1, oy = (c / 3) % 3 - 1, ox = c % 3 - 1;
+                        if (i + ox < 0 || i + ox >= mx || j + oy < 0 || j + oy >= my || k + oz < 0 || k + oz >= L.mz)
+                            continue;
+                        A[q * n + q + ((long)oz * my + oy) * mx + ox] = g[(size_t)c * n + q];
+                    }
+                }
+    }
+    for (int k = 0; k < (g27 ? 0 : L.mz); ++k)
         for (int j = 0; j < L.my; ++j)
             for (int i = 0; i < L.mx; ++i) {
                 const long q = ((long)k * my + j) * mx + i;
@@ -1066,6 +1097,31 @@ static int coarse_factor(const LevelInfo& L, std::vector<double>& Ainv) {
     return 0;
 }
 
+// -pc_mg_galerkin: the coarse operators P^T A P, formed on the device in
+// 27-point form level by level from the finest (rediscretized) operator
+static int galerkin_levels(p4g_solver_s* s) {
+    s->g27.assign(s->nlev, nullptr);
+    if (!s->galerkin) return 0;
+    if (s->lv.back().var)
+        return set_error(PETSC_ERR_SUP, "-pc_mg_galerkin with a variable coefficient is not implemented on the device");
+    hipStream_t st = s->cx.stream;
+    for (int l = s->nlev - 2; l >= 0; --l) {
+        LevelInfo& C = s->lv[l];
+        const LevelInfo& L = s->lv[l + 1];
+        P4G_TRY(dalloc(s, (void**)&s->g27[l], 27 * C.n * sizeof(double)));
+        GalFine f;
+        f.dim = L.dim; f.mx = L.mx; f.my = L.my; f.mz = L.mz;
+        f.d = L.d; f.db = L.db; f.sx = L.sx; f.sy = L.sy; f.sz = L.sz;
+        f.g = s->g27[l + 1];   // null on the finest level
+        f.n = L.n;
+        k_galerkin<<<vblocks(C.n), 256, 0, st>>>(f, C.mx, C.my, C.mz, s->g27[l]);
+        LAUNCH_CHECK();
+        C.gal = true;
+    }
+    HIP_TRY(hipStreamSynchronize(st));
+    return 0;
+}
+
 static int estimate_level(p4g_solver_s* s, int l) {
     LevelInfo& L = s->lv[l];
     const long n = L.n;
@@ -1086,8 +1142,14 @@ static int estimate_level(p4g_solver_s* s, int l) {
     OpConst<double> oc;
     OpVar<double> ov;
     const dim3 g = stencil_grid<double>(L.dim, L.mx, L.my, L.mz), bk = stencil_block(L.dim);
-    const long nbs = nblocks(g);
-    if (!L.var) {
+    const long nbs = L.gal ? vblocks(n) : nblocks(g);
+    const Op27<double> og{L.gal ? s->g27[l] : nullptr, n, L.mx, L.my, L.mz, jac ? 1 : 0};
+    if (L.gal) {
+        P4G_TRY(get(&idv));
+        k_stencil27<double, SrcZero<double>, OutIdiag><<<vblocks(n), 256, 0, st>>>(og, SrcZero<double>{}, OutIdiag{idv},
+                                                                                 s->cx.part);
+        LAUNCH_CHECK();
+    } else if (!L.var) {
         oc.d = L.d; oc.db = L.db; oc.sx = L.sx; oc.sy = L.sy; oc.sz = L.sz;
         oc.id = jac ? 1.0 / L.d : 1.0; oc.idb = jac ? 1.0 / L.db : 1.0;
     } else {
@@ -1127,7 +1189,10 @@ static int estimate_level(p4g_solver_s* s, int l) {
         if (it > 0) { beta = rz / rzold; be[it - 1] = beta; }
         s->cx.hS[16 + S_BCOEF] = beta;
         HIP_TRY(hipMemcpyAsync(S + S_BCOEF, &s->cx.hS[16 + S_BCOEF], sizeof(double), hipMemcpyHostToDevice, st));
-        if (L.var)
+        if (L.gal)
+            k_stencil27<double, SrcDir<double>, OutApplyDot<double>><<<vblocks(n), 256, 0, st>>>(
+                og, SrcDir<double>{z, pold, pnew, S + S_BCOEF}, OutApplyDot<double>{w}, s->cx.part);
+        else if (L.var)
             k_stencil<double, 3, OpVar<double>, SrcDir<double>, OutApplyDot<double>><<<g, bk, 0, st>>>(
                 L.mx, L.my, L.mz, kchf(L.dim, L.my, L.mz), ov, SrcDir<double>{z, pold, pnew, S + S_BCOEF}, OutApplyDot<double>{w}, s->cx.part);
         else if (L.dim == 3)
@@ -1168,8 +1233,8 @@ static int estimate_level(p4g_solver_s* s, int l) {
     return 0;
 }
 
-static const char* kUnsupported[] = {"pc_mg_galerkin", "pc_mg_type", "pc_mg_multiplicative_cycles",
-                                     "mg_levels_ksp_norm_type", nullptr};
+static const char* kUnsupported[] = {"pc_mg_type", "pc_mg_multiplicative_cycles", "mg_levels_ksp_norm_type",
+                                     nullptr};
 
 static int configure(p4g_solver_s* s) {
     Options& o = s->opt;
@@ -1243,6 +1308,15 @@ static int configure(p4g_solver_s* s) {
     if (!(s->reliable_delta > 0.0 && s->reliable_delta < 1.0))
         return set_error(PETSC_ERR_ARG_OUTOFRANGE, "-p4g_reliable_delta must be in (0,1)");
     if (s->mixed && s->pctype != PCT_MG) return set_error(PETSC_ERR_SUP, "-p4g_precision mixed needs -pc_type mg");
+    if (o.has("pc_mg_galerkin")) {   // PETSc: both | pmat | mat | none; the bare flag means both
+        const std::string g = o.gets("pc_mg_galerkin", "");
+        if (g.empty() || g == "both" || g == "pmat" || g == "mat" || g == "true" || g == "1")
+            s->galerkin = s->pctype == PCT_MG && s->nlev > 1;
+        else if (g != "none" && g != "false" && g != "0")
+            return set_error(PETSC_ERR_ARG_WRONG, "-pc_mg_galerkin %s", g.c_str());
+    }
+    if (s->galerkin && s->mixed)
+        return set_error(PETSC_ERR_SUP, "-pc_mg_galerkin with -p4g_precision mixed is not implemented on the device");
     return 0;
 }
 
@@ -1367,7 +1441,8 @@ int solver_setup(p4g_solver_s* s) {
         HIP_TRY(hipHostMalloc((void**)&s->cx.hS, 32 * sizeof(double), hipHostMallocDefault));
     }
     if (s->pctype == PCT_MG) {
-        P4G_TRY(coarse_factor(s->lv[0], s->Ainv));
+        P4G_TRY(galerkin_levels(s));
+        P4G_TRY(coarse_factor(s->lv[0], s->g27[0], s->Ainv));
         s->n0 = (int)s->lv[0].n;
         if (s->smoother == SM_CHEB)
             for (int l = 1; l < s->nlev; ++l) P4G_TRY(estimate_level(s, l));
@@ -1375,6 +1450,7 @@ int solver_setup(p4g_solver_s* s) {
     auto cfg = [&](auto& e) {
         e->pctype = s->pctype; e->cycles = s->cycles; e->smoother = s->smoother; e->nu = s->nu;
         e->jacobi = s->jacobi; e->rich_scale = s->rich_scale;
+        e->g27.assign(s->g27.begin(), s->g27.end());
     };
     if (F.var) {
         const size_t nc = F.kcell.size();

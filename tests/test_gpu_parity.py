@@ -61,6 +61,18 @@ CASES = {
                               "-ksp_rtol 1e-10 -p4g_fuse_r 1 -p4g_fuse_dir 0" % MG),
     "3d_cheb2_fused_dir_only": ("-fsh_dim 3 -fsh_problem manusin -fsh_noise_sigma 1e-3 -da_refine 4 %s "
                                 "-ksp_rtol 1e-10 -p4g_fuse_r 0 -p4g_fuse_dir 1" % MG),
+    "3d_galerkin_cheb2": ("-fsh_dim 3 -fsh_problem manusin -da_refine 4 %s -pc_mg_galerkin "
+                          "-ksp_rtol 1e-10" % MG),
+    "3d_galerkin_cheb1_unprecond": ("-fsh_dim 3 -fsh_problem manusin -fsh_noise_sigma 1e-3 -da_refine 4 %s "
+                                    "-pc_mg_galerkin -mg_levels_ksp_max_it 1 -ksp_norm_type unpreconditioned "
+                                    "-ksp_rtol 1e-10" % MG),
+    "3d_galerkin_test7_problem_jacobi": ("-fsh_dim 3 -fsh_problem manupoly -ksp_rtol 1.0e-12 -pc_type mg "
+                                         "-pc_mg_galerkin -da_refine 2 -mg_levels_pc_type jacobi"),
+    "2d_galerkin_w_richardson": ("-fsh_dim 2 -fsh_problem manupoly -fsh_cx 0.3 -da_refine 5 -pc_type mg "
+                                 "-pc_mg_galerkin both -pc_mg_cycle_type w -mg_levels_ksp_type richardson "
+                                 "-mg_levels_pc_type jacobi -mg_levels_ksp_richardson_scale 0.6 "
+                                 "-mg_levels_ksp_max_it 2 -ksp_rtol 1e-10"),
+    "1d_galerkin": "-fsh_dim 1 -fsh_problem manupoly -da_refine 6 %s -pc_mg_galerkin -ksp_rtol 1e-10" % MG,
     "3d_richardson3": ("-fsh_dim 3 -fsh_problem manusin -da_refine 4 -pc_type mg -mg_levels_ksp_type "
                        "richardson -mg_levels_pc_type jacobi -mg_levels_ksp_max_it 3 "
                        "-mg_levels_ksp_richardson_scale 0.7 -ksp_rtol 1e-10"),
@@ -86,6 +98,32 @@ def test_fish_parity(name):
     assert abs(res.errinf - ores.errinf) <= float(np.max(np.abs(u - ou))) * (1 + 1e-9) + 1e-15
     # printed error line: same text as the oracle's (%.3e)
     assert text.splitlines()[-1] == otext.splitlines()[-1]
+
+
+@pytest.mark.parametrize("name", sorted(n for n in CASES if "galerkin" in n))
+def test_galerkin_trajectory(name):
+    # coarse operators P^T A P formed on the device in 27-point form against
+    # the oracle's sparse triple product: the same iteration count and the
+    # same final residual reduction (the coarse operators steer the whole
+    # residual history, not only the count)
+    args = CASES[name]
+    rc, text, res, _ = capi.fish_run(args)
+    assert rc == 0, text
+    orc, otext, ores, _ = O.fish_run(args)
+    assert orc == 0, otext
+    assert res.its == ores.its, (res.its, ores.its)
+    red, ored = res.rnorm / res.rnorm0, ores.rnorm / ores.rnorm0
+    print(name, "its", res.its, "reduction", red, "oracle", ored, flush=True)
+    assert abs(red - ored) <= 1e-8 * ored
+
+
+def test_galerkin_off_is_rediscretization():
+    # -pc_mg_galerkin none leaves the rediscretized coarse operators
+    base = "-fsh_dim 3 -fsh_problem manusin -da_refine 3 %s -ksp_rtol 1e-10" % MG
+    rc0, t0, r0, u0 = capi.fish_run(base, want_u=True)
+    rc1, t1, r1, u1 = capi.fish_run(base + " -pc_mg_galerkin none", want_u=True)
+    assert rc0 == 0 and rc1 == 0
+    assert r0.its == r1.its and np.array_equal(u0, u1)
 
 
 def test_golden_lines_with_jacobi_smoother():
