@@ -152,6 +152,7 @@ void oksp_estimate_eigs(oksp *k);
 /* multigrid (PCMG) */
 struct omg {
     int     nlev, cycles, galerkin;
+    int     full;               /* -pc_mg_type full: one F-cycle per apply */
     ogrid  *g;                  /* g[0] coarsest ... g[nlev-1] finest   */
     ocsr   *A;                  /* per level operators                  */
     oksp   *smooth;             /* per level smoother (0: coarse solve) */

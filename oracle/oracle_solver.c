@@ -7,7 +7,10 @@
  *   KSPCG (preconditioned-norm PCG, default rtol 1e-5, atol 1e-50, dtol 1e5),
  *   PCMG multiplicative cycles with one KSP per level used for pre- and
  *   post-smoothing (nonzero initial guess), x_c = 0 before recursion,
- *   restriction = P^T, interpolation-add = P,
+ *   restriction = P^T, interpolation-add = P; PCMG type full (PETSc's
+ *   PCMGFCycle_Private: the right-hand side restricted to the coarsest
+ *   level, then upward a cycle per level started from the interpolated
+ *   coarser iterate, x_{l+1} = P x_l),
  *   KSPCHEBYSHEV (first step x += s B r, then the three-term recurrence;
  *   bounds [0.1, 1.1] x estimated lambda_max from 10 Krylov steps),
  *   KSPRICHARDSON, PCJACOBI, PCSOR (symmetric local sweeps, omega 1),
@@ -590,7 +593,8 @@ int omg_setup(omg *mg, const ogrid *fine, const oprob *prob, const ocsr *Afine,
     mg->cycles = (ct && !strcmp(ct, "w")) ? 2 : 1;
     mg->galerkin = oopts_bool(o, "pc_mg_galerkin", 0);
     const char *mgtype = oopts_get(o, "pc_mg_type");
-    if (mgtype && strcmp(mgtype, "multiplicative")) {
+    mg->full = mgtype && !strcmp(mgtype, "full");
+    if (mgtype && strcmp(mgtype, "multiplicative") && !mg->full) {
         snprintf(err, errlen, "oracle: -pc_mg_type %s not supported", mgtype);
         return 56;
     }
@@ -707,13 +711,29 @@ static void mcycle(omg *mg, int l) {
     oksp_solve(&mg->smooth[l], mg->b[l], mg->x[l], &reason, &r0, &rn);
 }
 
-/* PCApply_MG: x = 0, one multiplicative cycle */
+/* F-cycle: b restricted through every level to the coarsest; the coarsest
+ * solved from zero; then for l = 1 .. L: x_l = P x_{l-1} (overwritten, not
+ * added) and one cycle on level l from that iterate */
+static void fcycle(omg *mg) {
+    int L = mg->nlev - 1;
+    for (int l = L; l > 0; l--) oq1_restrict(&mg->g[l], &mg->g[l - 1], mg->b[l], mg->b[l - 1]);
+    ovset(mg->g[0].n, 0.0, mg->x[0]);
+    mcycle(mg, 0);
+    for (int l = 1; l <= L; l++) {
+        ovset(mg->g[l].n, 0.0, mg->x[l]);
+        oq1_interp_add(&mg->g[l - 1], &mg->g[l], mg->x[l - 1], mg->x[l]);
+        mcycle(mg, l);
+    }
+}
+
+/* PCApply_MG: multiplicative: x = 0, one cycle; full: one F-cycle */
 void omg_apply(omg *mg, const double *b, double *x) {
     int L = mg->nlev - 1;
     mg->b[L] = (double *)b;
     mg->x[L] = x;
     ovset(mg->g[L].n, 0.0, x);
-    mcycle(mg, L);
+    if (mg->full && L > 0) fcycle(mg);
+    else mcycle(mg, L);
     mg->b[L] = NULL; mg->x[L] = NULL;
 }
 

@@ -249,6 +249,7 @@ struct Engine {
     bool fuse_r = true, fuse_dir = true;
     // per level: 27-point Galerkin coefficients (FP64, device) or null
     std::vector<const double*> g27;
+    bool fmg = false;          // full multigrid: an F-cycle per apply
     struct Dev {
         T *x = nullptr, *b = nullptr, *r = nullptr, *t[4] = {nullptr, nullptr, nullptr, nullptr};
         T* kc = nullptr;
@@ -718,9 +719,28 @@ struct Engine {
         return 0;
     }
 
+    // PCMG type full (PETSc's PCMGFCycle_Private): b restricted through every
+    // level, the coarsest solved from zero, then upward x_l = P x_{l-1}
+    // (overwritten) and one cycle on level l from that iterate
+    int fcycle(const T* b, T* x, bool dots, long* nb) {
+        P4G_TRY(pend_flush());
+        const int L = nlev - 1;
+        P4G_TRY(restrict_(L, b, dv[L - 1].b));
+        for (int l = L - 1; l > 0; --l) P4G_TRY(restrict_(l, dv[l].b, dv[l - 1].b));
+        P4G_TRY(mcycle(0, dv[0].b, dv[0].x, true, false, nullptr));
+        for (int l = 1; l <= L; ++l) {
+            const T* bl = l == L ? b : dv[l].b;
+            T* xl = l == L ? x : dv[l].x;
+            P4G_TRY(prolong(l, true, bl, T(0), dv[l - 1].x, xl));   // 0 b + P x_{l-1}
+            P4G_TRY(mcycle(l, bl, xl, false, l == L && dots, l == L ? nb : nullptr));
+        }
+        return 0;
+    }
+
     // z = B r with fused (z,r), (z,z) partials; returns the block count
     int pc_apply_dots(const T* rr, T* zz, long* nb) {
         const LevelInfo& F = (*lv)[nlev - 1];
+        if (pctype == PCT_MG && nlev > 1 && fmg) return fcycle(rr, zz, true, nb);
         if (pctype == PCT_MG && nlev > 1) {
             P4G_TRY(mcycle(nlev - 1, rr, zz, true, true, nb));
             if (pend.on) return set_error(PETSC_ERR_LIB, "internal: pending residual update not consumed");
@@ -936,6 +956,7 @@ struct p4g_solver_s {
     bool mixed_x64 = true;    // mixed: accumulate the iterate in FP64 every iteration
     int fuse_r = -1, fuse_dir = -1;   // CG updates fused into stencil passes: 1 on, 0 off, -1 by rule
     bool galerkin = false;            // coarse operators P^T A P (-pc_mg_galerkin)
+    bool fmg = false;                 // -pc_mg_type full: one F-cycle per preconditioner apply
     // state
     bool ready = false;
     std::vector<LevelInfo> lv;
@@ -1233,8 +1254,7 @@ static int estimate_level(p4g_solver_s* s, int l) {
     return 0;
 }
 
-static const char* kUnsupported[] = {"pc_mg_type", "pc_mg_multiplicative_cycles", "mg_levels_ksp_norm_type",
-                                     nullptr};
+static const char* kUnsupported[] = {"pc_mg_multiplicative_cycles", "mg_levels_ksp_norm_type", nullptr};
 
 static int configure(p4g_solver_s* s) {
     Options& o = s->opt;
@@ -1249,10 +1269,12 @@ static int configure(p4g_solver_s* s) {
                                         "default ILU is not implemented");
     else return set_error(PETSC_ERR_SUP, "-pc_type %s is not implemented on the device", pt.c_str());
     for (const char** u = kUnsupported; *u; ++u)
-        if (o.has(*u)) {
-            if (std::string(*u) == "pc_mg_type" && o.gets(*u, "") == "multiplicative") continue;
-            return set_error(PETSC_ERR_SUP, "-%s is not implemented on the device", *u);
-        }
+        if (o.has(*u)) return set_error(PETSC_ERR_SUP, "-%s is not implemented on the device", *u);
+    const std::string mt = o.gets("pc_mg_type", "multiplicative");
+    if (mt == "full") s->fmg = true;
+    else if (mt != "multiplicative")
+        return set_error(PETSC_ERR_SUP, "-pc_mg_type %s is not implemented on the device (multiplicative, full)",
+                         mt.c_str());
     s->rtol = o.getr("ksp_rtol", 1e-5);
     s->atol = o.getr("ksp_atol", 1e-50);
     s->dtol = o.getr("ksp_divtol", 1e5);
@@ -1451,6 +1473,7 @@ int solver_setup(p4g_solver_s* s) {
         e->pctype = s->pctype; e->cycles = s->cycles; e->smoother = s->smoother; e->nu = s->nu;
         e->jacobi = s->jacobi; e->rich_scale = s->rich_scale;
         e->g27.assign(s->g27.begin(), s->g27.end());
+        e->fmg = s->fmg;
     };
     if (F.var) {
         const size_t nc = F.kcell.size();

@@ -66,6 +66,7 @@ def test_da_from_options():
     ("-pc_type mg -pc_mg_levels 3", "sor"),
     ("-pc_type mg -pc_mg_levels 3 -mg_levels_pc_type jacobi -pc_mg_galerkin -p4g_precision mixed", "galerkin"),
     ("-ksp_type gmres -pc_type mg", "gmres"),
+    ("-pc_type mg -pc_mg_levels 3 -mg_levels_pc_type jacobi -pc_mg_type additive", "additive"),
     ("", "pc_type mg"),
 ])
 def test_unsupported_options_rejected(opts, fragment):

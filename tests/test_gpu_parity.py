@@ -73,6 +73,13 @@ CASES = {
                                  "-mg_levels_pc_type jacobi -mg_levels_ksp_richardson_scale 0.6 "
                                  "-mg_levels_ksp_max_it 2 -ksp_rtol 1e-10"),
     "1d_galerkin": "-fsh_dim 1 -fsh_problem manupoly -da_refine 6 %s -pc_mg_galerkin -ksp_rtol 1e-10" % MG,
+    "3d_fcycle_cheb1": ("-fsh_dim 3 -fsh_problem manusin -da_refine 4 %s -pc_mg_type full "
+                        "-mg_levels_ksp_max_it 1 -ksp_rtol 1e-10" % MG),
+    "2d_fcycle_w_cheb2": ("-fsh_dim 2 -fsh_problem manupoly -da_refine 5 %s -pc_mg_type full "
+                          "-pc_mg_cycle_type w -ksp_rtol 1e-10" % MG),
+    "3d_fcycle_galerkin": ("-fsh_dim 3 -fsh_problem manusin -da_refine 3 %s -pc_mg_type full -pc_mg_galerkin "
+                           "-ksp_rtol 1e-10" % MG),
+    "1d_fcycle": "-fsh_dim 1 -fsh_problem manupoly -da_refine 6 %s -pc_mg_type full -ksp_rtol 1e-10" % MG,
     "3d_richardson3": ("-fsh_dim 3 -fsh_problem manusin -da_refine 4 -pc_type mg -mg_levels_ksp_type "
                        "richardson -mg_levels_pc_type jacobi -mg_levels_ksp_max_it 3 "
                        "-mg_levels_ksp_richardson_scale 0.7 -ksp_rtol 1e-10"),
@@ -168,6 +175,26 @@ def test_operator_and_vcycle_parity(args):
         assert abs(z @ w - x @ bw) < 1e-11 * abs(z @ w)
 
 
+@pytest.mark.parametrize("args", [
+    "-fsh_dim 3 -fsh_problem manusin -da_refine 4 %s -pc_mg_type full -mg_levels_ksp_max_it 1" % MG,
+    "-fsh_dim 3 -fsh_problem manusin -da_refine 3 %s -pc_mg_galerkin" % MG,
+    "-fsh_dim 2 -fsh_problem manupoly -da_refine 5 %s -pc_mg_galerkin -pc_mg_type full "
+    "-pc_mg_cycle_type w" % MG], ids=["3d_fcycle", "3d_galerkin", "2d_galerkin_fcycle_w"])
+def test_fcycle_and_galerkin_apply_parity(args):
+    # one preconditioner application against the oracle's, and for the
+    # (symmetric) Galerkin V-cycle the symmetry of B
+    print("case", args, flush=True)
+    with capi.Session(args) as s:
+        sv = capi.Solver(s)
+        rng = np.random.default_rng(7)
+        x = rng.standard_normal(s.n)
+        z = sv.pc_apply(x)
+        assert _rel(z, O.pc_apply(args, x)) < 1e-11
+        if "full" not in args:
+            w = rng.standard_normal(s.n)
+            assert abs(z @ w - x @ sv.pc_apply(w)) < 1e-11 * abs(z @ w)
+
+
 def test_rhs_parity():
     args = "-fsh_dim 3 -fsh_problem manusin -fsh_noise_sigma 1e-3 -da_refine 3 %s" % MG
     with capi.Session(args) as s:
@@ -219,7 +246,8 @@ def test_mixed_matches_fp64(refine):
 
 @pytest.mark.parametrize("variant", ["-p4g_mixed_x64 0", "-p4g_fp32_diff_form 0", "-p4g_reliable_delta 0.01",
                                      "-fsh_problem varcoef", "-p4g_fuse_cg 1", "-p4g_fuse_cg 1 -fsh_problem varcoef",
-                                     "-p4g_fuse_r 1 -p4g_fuse_dir 0", "-p4g_fuse_r 0 -p4g_fuse_dir 1"])
+                                     "-p4g_fuse_r 1 -p4g_fuse_dir 0", "-p4g_fuse_r 0 -p4g_fuse_dir 1",
+                                     "-pc_mg_type full"])
 def test_mixed_variants(variant):
     # every mixed-precision variant converges on the FP64 true residual to the
     # same discrete solution as the FP64 oracle
