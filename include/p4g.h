@@ -210,6 +210,29 @@ int p4g_server_wait(p4g_Server srv, long id, p4g_Report *rep);
 int p4g_server_stats(p4g_Server srv, p4g_ServerStats *st);
 int p4g_server_destroy(p4g_Server *srv);
 
+/* ------------------------------------------------ PETSc binary files */
+
+/* Vectors and the operator in PETSc's binary format (what VecView, VecLoad
+ * and MatView write and read through a binary PetscViewer: big-endian
+ * int32 class id, sizes, then the data), to exchange right-hand sides,
+ * solutions and the matrix with PETSc programs such as c/ch2/loadsolve.c
+ * (VecLoad/MatLoad, loadsolve.c:48-70) or the dump of c/ch12/obstacle.c:
+ * 137-146 (VecView of u, then of the lower bound, into one file).  A file
+ * holds its objects one after another; append 0 starts a new file.
+ * Error codes: 65 cannot open, 66 short read, 67 write failed, 79 not a
+ * PETSc Vec where one was expected (PETSC_ERR_FILE_*).
+ * The solver options -ksp_view_mat, -ksp_view_rhs and -ksp_view_solution
+ * binary:<file> write the operator, the right-hand side and the solution
+ * after every KSP solve, as KSPSolve's viewers do. */
+int p4g_vec_view_binary(const char *path, long n, const double *x, int append);
+/* the index-th object of the file (0-based), which must be a Vec; x may be
+ * NULL to query its length */
+int p4g_vec_load_binary(const char *path, int index, long *n, double *x);
+/* the operator the Jacobian callback (or the cell coefficient) defines on
+ * this DMDA, as an AIJ matrix: sorted column indices per row, the entries
+ * the callback inserts (MatView of the assembled Jacobian); host only */
+int p4g_da_mat_view_binary(p4g_DA da, const char *path, int append);
+
 /* ---------------------------------------------------- device utilities */
 
 int   p4g_device_count(int *n);

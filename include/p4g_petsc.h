@@ -43,6 +43,10 @@ typedef int    PetscEnum;
 #define PETSC_ERR_ARG_SIZ   60
 #define PETSC_ERR_ARG_OUTOFRANGE 63
 #define PETSC_ERR_LIB       76
+#define PETSC_ERR_FILE_OPEN 65
+#define PETSC_ERR_FILE_READ 66
+#define PETSC_ERR_FILE_WRITE 67
+#define PETSC_ERR_FILE_UNEXPECTED 79
 #define PETSC_ERR_NOT_CONVERGED 91
 
 typedef struct p4g_da_s     *DM;     /* a DMDA: structured vertex grid   */

@@ -88,6 +88,9 @@ def lib():
                                               ctypes.POINTER(d), ctypes.POINTER(d)]
         L.p4g_solver_reset_stats.argtypes = [vp]
         L.p4g_device_count.argtypes = [ctypes.POINTER(i)]
+        L.p4g_vec_view_binary.argtypes = [cp, ctypes.c_long, vp, i]
+        L.p4g_vec_load_binary.argtypes = [cp, i, ctypes.POINTER(ctypes.c_long), vp]
+        L.p4g_da_mat_view_binary.argtypes = [vp, cp, i]
         _p4g = L
     return _p4g
 
@@ -101,6 +104,7 @@ def fishlib():
         L = ctypes.CDLL(LIBFISH)
         vp, i, d = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
         L.fish_run.argtypes = [i, vp, ctypes.POINTER(FishResult), vp, ctypes.c_char_p, i]
+        L.fish_write_operator.argtypes = [i, vp, ctypes.c_char_p]
         L.fish_session_create.argtypes = [i, vp, ctypes.POINTER(vp)]
         L.fish_session_solve.argtypes = [vp, ctypes.POINTER(Report)]
         L.fish_session_errors.argtypes = [vp, ctypes.POINTER(d), ctypes.POINTER(d)]
@@ -158,6 +162,27 @@ def grid_shape(args):
     if dim < 2:
         m[1] = 1
     return dim, m[0], m[1], m[2]
+
+
+def vec_view_binary(path, x, append=False):
+    """Write x as a PETSc binary Vec (VecView on a binary viewer)."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    check(lib().p4g_vec_view_binary(str(path).encode(), x.size, x.ctypes.data, 1 if append else 0))
+
+
+def vec_load_binary(path, index=0):
+    """The index-th object of a PETSc binary file, which must be a Vec."""
+    n = ctypes.c_long(0)
+    check(lib().p4g_vec_load_binary(str(path).encode(), index, ctypes.byref(n), None))
+    x = np.empty(n.value)
+    check(lib().p4g_vec_load_binary(str(path).encode(), index, ctypes.byref(n), x.ctypes.data))
+    return x
+
+
+def write_operator(args, path):
+    """Set up fish for `args` (host only) and write its operator as a PETSc binary AIJ matrix."""
+    argc, argv = _argv(args)
+    return fishlib().fish_write_operator(argc, argv, str(path).encode())
 
 
 def fish_run(args, want_u=False):

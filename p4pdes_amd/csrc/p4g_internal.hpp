@@ -3,6 +3,7 @@
 
 #include <cstdarg>
 #include <cstdint>
+#include <cstdio>
 #include <string>
 #include <vector>
 
@@ -95,4 +96,23 @@ namespace p4g {
 void* make_view(p4g_da_s* da, double* base, std::vector<double*>& rows,
                 std::vector<double**>& planes);
 void  info_of(p4g_da_s* da, DMDALocalInfo* info);
+
+// PETSc binary files: big-endian int32 / IEEE double, buffered writes
+extern const int32_t kVecClassId, kMatClassId;
+struct BinaryFile {
+    FILE* fp = nullptr;
+    std::vector<unsigned char> buf;
+    bool failed = false;
+    ~BinaryFile();
+    bool open(const char* path, const char* mode);
+    bool close();                       // flush and close; false on any write error
+    bool flush();
+    void bytes(const void* p, size_t k);
+    void i32(int32_t v);
+    void f64(double v);
+    void f64s(const double* x, long n);
+    bool read_i32(int32_t& v);
+    bool read_f64s(double* x, long n);
+    bool skip(long k);
+};
 }  // namespace p4g

@@ -1,6 +1,7 @@
 // p4g_shim.cpp -- the PETSc-shaped host surface (p4g_petsc.h): grid queries,
 // the recording matrix, host vectors, errors and options.  Set-up only;
 // nothing here runs inside a solve.
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -106,6 +107,67 @@ void* make_view(p4g_da_s* da, double* base, std::vector<double*>& rows,
     return planes.data();
 }
 
+// big-endian writer / reader of PETSc binary files
+const int32_t kVecClassId = 1211214, kMatClassId = 1211216;
+
+BinaryFile::~BinaryFile() {
+    if (fp) fclose(fp);
+}
+bool BinaryFile::open(const char* path, const char* mode) {
+    fp = fopen(path, mode);
+    return fp != nullptr;
+}
+bool BinaryFile::close() {
+    bool ok = flush();
+    if (fp && fclose(fp) != 0) ok = false;
+    fp = nullptr;
+    return ok && !failed;
+}
+bool BinaryFile::flush() {
+    if (!fp || buf.empty()) return true;
+    if (fwrite(buf.data(), 1, buf.size(), fp) != buf.size()) failed = true;
+    buf.clear();
+    return !failed;
+}
+void BinaryFile::bytes(const void* p, size_t k) {
+    const unsigned char* c = (const unsigned char*)p;
+    buf.insert(buf.end(), c, c + k);
+    if (buf.size() >= (1u << 20)) flush();
+}
+void BinaryFile::i32(int32_t v) {
+    const uint32_t u = __builtin_bswap32((uint32_t)v);
+    bytes(&u, 4);
+}
+void BinaryFile::f64(double v) {
+    uint64_t u;
+    std::memcpy(&u, &v, 8);
+    u = __builtin_bswap64(u);
+    bytes(&u, 8);
+}
+void BinaryFile::f64s(const double* x, long n) {
+    for (long q = 0; q < n; ++q) f64(x[q]);
+}
+bool BinaryFile::read_i32(int32_t& v) {
+    uint32_t u;
+    if (fread(&u, 4, 1, fp) != 1) return false;
+    v = (int32_t)__builtin_bswap32(u);
+    return true;
+}
+bool BinaryFile::read_f64s(double* x, long n) {
+    std::vector<uint64_t> tmp(1 << 16);
+    for (long q = 0; q < n;) {
+        const long k = std::min<long>((long)tmp.size(), n - q);
+        if (fread(tmp.data(), 8, (size_t)k, fp) != (size_t)k) return false;
+        for (long e = 0; e < k; ++e) {
+            const uint64_t u = __builtin_bswap64(tmp[e]);
+            std::memcpy(&x[q + e], &u, 8);
+        }
+        q += k;
+    }
+    return true;
+}
+bool BinaryFile::skip(long k) { return fseek(fp, k, SEEK_CUR) == 0; }
+
 }  // namespace p4g
 
 using namespace p4g;
@@ -122,6 +184,50 @@ PetscErrorCode p4g_seterr(PetscErrorCode code, const char* fmt, ...) {
 }
 
 const char* p4g_last_error(void) { return g_last_error.c_str(); }
+
+// ------------------------------------------------------ PETSc binary vectors
+
+int p4g_vec_view_binary(const char* path, long n, const double* x, int append) {
+    clear_error();
+    if (!path || n < 0 || (n > 0 && !x)) return set_error(PETSC_ERR_ARG_WRONG, "p4g_vec_view_binary: bad argument");
+    if (n > 2147483647L) return set_error(PETSC_ERR_ARG_OUTOFRANGE, "p4g_vec_view_binary: length exceeds int32");
+    BinaryFile f;
+    if (!f.open(path, append ? "ab" : "wb")) return set_error(PETSC_ERR_FILE_OPEN, "cannot open %s", path);
+    f.i32(kVecClassId);
+    f.i32((int32_t)n);
+    f.f64s(x, n);
+    if (!f.close()) return set_error(PETSC_ERR_FILE_WRITE, "write to %s failed", path);
+    return 0;
+}
+
+int p4g_vec_load_binary(const char* path, int index, long* n, double* x) {
+    clear_error();
+    if (!path || index < 0 || !n) return set_error(PETSC_ERR_ARG_WRONG, "p4g_vec_load_binary: bad argument");
+    BinaryFile f;
+    if (!f.open(path, "rb")) return set_error(PETSC_ERR_FILE_OPEN, "cannot open %s", path);
+    for (int k = 0;; ++k) {
+        int32_t id = 0, len = 0;
+        if (!f.read_i32(id)) return set_error(PETSC_ERR_FILE_READ, "%s: object %d not found", path, index);
+        if (id != kVecClassId) {
+            if (k == index) return set_error(PETSC_ERR_FILE_UNEXPECTED, "%s: object %d is not a PETSc Vec", path, index);
+            // skip a matrix: sizes, row lengths, column indices, values
+            if (id != kMatClassId) return set_error(PETSC_ERR_FILE_UNEXPECTED, "%s: unknown class id %d", path, id);
+            int32_t m = 0, nc = 0, nz = 0;
+            if (!f.read_i32(m) || !f.read_i32(nc) || !f.read_i32(nz) || m < 0 || nz < 0 ||
+                !f.skip(4L * m + 4L * nz + 8L * nz))
+                return set_error(PETSC_ERR_FILE_READ, "%s: truncated matrix", path);
+            continue;
+        }
+        if (!f.read_i32(len) || len < 0) return set_error(PETSC_ERR_FILE_READ, "%s: truncated vector header", path);
+        if (k < index) {
+            if (!f.skip(8L * len)) return set_error(PETSC_ERR_FILE_READ, "%s: truncated vector", path);
+            continue;
+        }
+        *n = len;
+        if (x && !f.read_f64s(x, len)) return set_error(PETSC_ERR_FILE_READ, "%s: truncated vector", path);
+        return 0;
+    }
+}
 
 // ---------------------------------------------------------------- DM queries
 

@@ -1337,6 +1337,13 @@ static int configure(p4g_solver_s* s) {
         else if (g != "none" && g != "false" && g != "0")
             return set_error(PETSC_ERR_ARG_WRONG, "-pc_mg_galerkin %s", g.c_str());
     }
+    for (const char* vw : {"ksp_view_mat", "ksp_view_rhs", "ksp_view_solution"})
+        if (o.has(vw)) {
+            const std::string v = o.gets(vw, "");
+            if (v.compare(0, 7, "binary:") != 0 || v.size() == 7)
+                return set_error(PETSC_ERR_SUP, "-%s %s: the device path writes binary:<file> viewers only", vw,
+                                 v.c_str());
+        }
     if (s->galerkin && s->mixed)
         return set_error(PETSC_ERR_SUP, "-pc_mg_galerkin with -p4g_precision mixed is not implemented on the device");
     return 0;
@@ -1703,6 +1710,77 @@ static int true_residual(p4g_solver_s* s, const double* b, const double* x, doub
     return 0;
 }
 
+// a level operator in PETSc's binary AIJ format: class id, rows, columns,
+// nonzeros, the row lengths, the column indices (ascending per row) and
+// the values, written in three passes over the rows
+static int mat_view_binary(const LevelInfo& L, const char* path, bool append) {
+    if (L.n > 2147483647L) return set_error(PETSC_ERR_ARG_OUTOFRANGE, "matrix size exceeds int32");
+    const long sxy = (long)L.mx * L.my;
+    // entries of row q in ascending column order: -s[4] (k-1), -s[2] (j-1),
+    // -s[0] (i-1), the diagonal, -s[1], -s[3], -s[5]
+    auto row = [&](long q, long* col, double* val) -> int {
+        const int i = (int)(q % L.mx), j = (int)((q / L.mx) % L.my), k = (int)(q / sxy);
+        double dg, c[6];
+        host_row(L, i, j, k, dg, c);
+        const long off[7] = {-sxy, -(long)L.mx, -1, 0, 1, (long)L.mx, sxy};
+        const double v[7] = {-c[4], -c[2], -c[0], dg, -c[1], -c[3], -c[5]};
+        int m = 0;
+        for (int e = 0; e < 7; ++e)
+            if (e == 3 || v[e] != 0.0) {
+                col[m] = q + off[e];
+                val[m] = v[e];
+                ++m;
+            }
+        return m;
+    };
+    long nz = 0, col[7];
+    double val[7];
+    for (long q = 0; q < L.n; ++q) nz += row(q, col, val);
+    if (nz > 2147483647L) return set_error(PETSC_ERR_ARG_OUTOFRANGE, "nonzero count exceeds int32");
+    BinaryFile f;
+    if (!f.open(path, append ? "ab" : "wb")) return set_error(PETSC_ERR_FILE_OPEN, "cannot open %s", path);
+    f.i32(kMatClassId);
+    f.i32((int32_t)L.n);
+    f.i32((int32_t)L.n);
+    f.i32((int32_t)nz);
+    for (long q = 0; q < L.n; ++q) f.i32(row(q, col, val));
+    for (long q = 0; q < L.n; ++q) {
+        const int m = row(q, col, val);
+        for (int e = 0; e < m; ++e) f.i32((int32_t)col[e]);
+    }
+    for (long q = 0; q < L.n; ++q) {
+        const int m = row(q, col, val);
+        for (int e = 0; e < m; ++e) f.f64(val[e]);
+    }
+    if (!f.close()) return set_error(PETSC_ERR_FILE_WRITE, "write to %s failed", path);
+    return 0;
+}
+
+// KSPSolve's viewers: -ksp_view_mat, -ksp_view_rhs, -ksp_view_solution
+// binary:<file>, after a solve (b, x on the device); a file named by
+// several of them receives the objects in that order
+static int ksp_views(p4g_solver_s* s, const double* b, double* x) {
+    static const char* names[3] = {"ksp_view_mat", "ksp_view_rhs", "ksp_view_solution"};
+    std::vector<std::string> written;
+    std::vector<double> h;
+    for (int w = 0; w < 3; ++w) {
+        if (!s->opt.has(names[w])) continue;
+        const std::string path = s->opt.gets(names[w], "").substr(7);   // after "binary:"
+        bool append = false;
+        for (const std::string& p : written) append = append || p == path;
+        written.push_back(path);
+        if (w == 0) {
+            P4G_TRY(mat_view_binary(s->lv.back(), path.c_str(), append));
+            continue;
+        }
+        const long n = s->lv.back().n;
+        h.resize(n);
+        HIP_TRY(hipMemcpy(h.data(), w == 1 ? b : x, n * sizeof(double), hipMemcpyDeviceToHost));
+        P4G_TRY(p4g_vec_view_binary(path.c_str(), n, h.data(), append ? 1 : 0));
+    }
+    return 0;
+}
+
 int ksp_solve_device(p4g_solver_s* s, const double* b, double* x) {
     P4G_TRY(solver_setup(s));
     hipStream_t st = s->cx.stream;
@@ -1731,7 +1809,7 @@ int ksp_solve_device(p4g_solver_s* s, const double* b, double* x) {
     double rel = 0.0;
     P4G_TRY(true_residual(s, b, x, s->mixed ? s->xb : s->e64->w, &rel));
     R.true_relres = rel;
-    return 0;
+    return ksp_views(s, b, x);
 }
 
 }  // namespace p4g
@@ -1836,6 +1914,16 @@ int p4g_solver_setup(p4g_Solver s) {
     clear_error();
     if (!s) return set_error(PETSC_ERR_ARG_WRONG, "p4g_solver_setup: null solver");
     return solver_setup(s);
+}
+
+int p4g_da_mat_view_binary(DM da, const char* path, int append) {
+    clear_error();
+    if (!da || !path) return set_error(PETSC_ERR_ARG_WRONG, "p4g_da_mat_view_binary: null argument");
+    p4g_solver_s tmp;   // one level: the rows of the callbacks on this grid (no device work)
+    tmp.da = da;
+    tmp.nlev = 1;
+    P4G_TRY(build_levels(&tmp));
+    return mat_view_binary(tmp.lv.back(), path, append != 0);
 }
 
 int p4g_ksp_solve_device(p4g_Solver s, const double* d_b, double* d_x, p4g_Report* rep) {

@@ -109,7 +109,7 @@ static char *join(int argc, const char **argv) {
     return s;
 }
 
-static int setup(int argc, const char **argv, Setup *S, char *err, int errlen) {
+static int setup(int argc, const char **argv, Setup *S, char *err, int errlen, int want_solver) {
     memset(S, 0, sizeof *S);
     Args a = {argc, argv};
     FishCtx *c = &S->ctx;
@@ -191,7 +191,7 @@ static int setup(int argc, const char **argv, Setup *S, char *err, int errlen) {
         e = p4g_da_set_function_local(S->da, residual_ptr[p->dim - 1], &c->pc);
         if (!e) e = p4g_da_set_jacobian_local(S->da, jacobian_ptr[p->dim - 1], &c->pc);
     }
-    if (!e) e = p4g_solver_create(S->da, S->optstr, &S->solver);
+    if (!e && want_solver) e = p4g_solver_create(S->da, S->optstr, &S->solver);
     if (e) { snprintf(err, errlen, "%s\n", p4g_last_error()); return e; }
     return 0;
 }
@@ -244,7 +244,7 @@ int fish_run(int argc, const char **argv, fish_result *res, double *u_out, char 
     memset(res, 0, sizeof *res);
     Setup S;
     char err[1200];
-    int e = setup(argc, argv, &S, err, sizeof err);
+    int e = setup(argc, argv, &S, err, sizeof err, 1);
     if (e) {
         outf(&o, "[0]PETSC ERROR: %s", err);
         setup_free(&S);
@@ -311,10 +311,23 @@ struct fish_session_s {
     long    n;
 };
 
+int fish_write_operator(int argc, const char **argv, const char *path) {
+    Setup S;
+    char err[1200];
+    int e = setup(argc, argv, &S, err, sizeof err, 0);
+    if (e) {
+        setup_free(&S);
+        return p4g_seterr(e, "%s", err);
+    }
+    e = p4g_da_mat_view_binary(S.da, path, 0);
+    setup_free(&S);
+    return e;
+}
+
 int fish_session_create(int argc, const char **argv, fish_session *out) {
     struct fish_session_s *fs = (struct fish_session_s *)calloc(1, sizeof *fs);
     char err[1200];
-    int e = setup(argc, argv, &fs->S, err, sizeof err);
+    int e = setup(argc, argv, &fs->S, err, sizeof err, 1);
     if (e) {
         setup_free(&fs->S);
         free(fs);

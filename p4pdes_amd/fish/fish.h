@@ -41,6 +41,10 @@ p4g_Solver fish_session_solver(fish_session s);
 int  fish_session_sizes(fish_session s, int *dim, int *mx, int *my, int *mz);
 const double *fish_session_rhs_device(fish_session s);
 int  fish_session_destroy(fish_session *s);
+/* host only: set up the problem of the command line and write its operator
+ * (the finest level, the rows the Jacobian callback inserts) to path in
+ * PETSc's binary AIJ format, as -mat_view binary:<file> does */
+int fish_write_operator(int argc, const char **argv, const char *path);
 
 #ifdef __cplusplus
 }

@@ -195,6 +195,30 @@ def test_fcycle_and_galerkin_apply_parity(args):
             assert abs(z @ w - x @ sv.pc_apply(w)) < 1e-11 * abs(z @ w)
 
 
+def test_ksp_viewer_files(tmp_path):
+    # -ksp_view_mat/-ksp_view_rhs/-ksp_view_solution binary:<file> after the
+    # solve: the operator and F(u0) equal the oracle's, and the solution
+    # solves the system to the requested tolerance (PETSc binary layout)
+    from test_binary_io import read_petsc, _csr_apply
+    fA, fb = tmp_path / "A.dat", tmp_path / "by.dat"
+    args = ("-fsh_dim 3 -fsh_problem manusin -da_refine 3 %s -ksp_rtol 1e-10 -ksp_norm_type unpreconditioned "
+            "-ksp_view_mat binary:%s -ksp_view_rhs binary:%s -ksp_view_solution binary:%s" % (MG, fA, fb, fb))
+    rc, text, res, _ = capi.fish_run(args)
+    assert rc == 0, text
+    (_, (M, N, rowptr, cols, vals)), = read_petsc(fA)
+    (k1, b), (k2, y) = read_petsc(fb)
+    assert k1 == k2 == "vec" and b.size == y.size == M
+    plain = " ".join(a for a in args.split() if not a.startswith("binary:") and not a.startswith("-ksp_view"))
+    rng = np.random.default_rng(4)
+    x = rng.standard_normal(M)
+    ax = _csr_apply(M, rowptr, cols, vals, x)
+    assert np.max(np.abs(ax - O.mat_mult(plain, x))) <= 1e-12 * np.max(np.abs(ax))
+    assert _rel(b, O.rhs(plain)) < 1e-14
+    r = b - _csr_apply(M, rowptr, cols, vals, y)
+    assert np.linalg.norm(r) <= 1.5e-10 * np.linalg.norm(b)   # recursive vs true residual
+    assert np.array_equal(capi.vec_load_binary(fb, 1), y)
+
+
 def test_rhs_parity():
     args = "-fsh_dim 3 -fsh_problem manusin -fsh_noise_sigma 1e-3 -da_refine 3 %s" % MG
     with capi.Session(args) as s:
