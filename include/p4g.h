@@ -141,6 +141,10 @@ int p4g_pc_apply_device(p4g_Solver s, const double *d_r, double *d_z);
 int p4g_mat_mult(p4g_Solver s, const double *x, double *y);
 int p4g_mat_mult_device(p4g_Solver s, const double *d_x, double *d_y);
 int p4g_solver_get_report(p4g_Solver s, p4g_Report *rep);
+/* the residual norms of the last solve's convergence test, iteration 0 on
+ * (what -ksp_monitor prints; KSPGetResidualHistory): the first min(cap, *n)
+ * into h, their number into *n */
+int p4g_solver_get_history(p4g_Solver s, double *h, int cap, int *n);
 int p4g_solver_destroy(p4g_Solver *s);
 
 /* --------------------------------------------------------- measurement */

@@ -961,6 +961,7 @@ struct p4g_solver_s {
     bool ready = false;
     std::vector<LevelInfo> lv;
     std::vector<double> Ainv;   // dense inverse of the coarsest operator (host)
+    std::vector<double> hist;   // norms of the convergence test of the last solve, from iteration 0
     std::vector<double*> g27;   // per level: 27-point Galerkin coefficients (device, FP64) or null
     int n0 = 0;
     Ctx cx;
@@ -1582,6 +1583,7 @@ static int mixed_solve(p4g_solver_s* s, const double* b, double* x) {
     R.reason = 0;
     for (int q = 0; q < 64; ++q) R.history[q] = 0.0;
     R.history[0] = bnorm;
+    s->hist.assign(1, bnorm);
     double* xcur = s->xa;
     double* xnext = s->xb;
     if (bnorm <= s->atol) R.reason = P4G_CONVERGED_ATOL;
@@ -1676,6 +1678,7 @@ static int mixed_solve(p4g_solver_s* s, const double* b, double* x) {
         }
         R.rnorm = rn;
         if (i + 1 < 64) R.history[i + 1] = rn;
+        s->hist.push_back(rn);
         if (std::isnan(rn) || std::isinf(rn)) { R.reason = P4G_DIVERGED_NANORINF; break; }
         if (replaced && rn <= target) { R.reason = rn < s->atol ? P4G_CONVERGED_ATOL : P4G_CONVERGED_RTOL; break; }
         if (rn >= s->dtol * bnorm) { R.reason = P4G_DIVERGED_DTOL; break; }
@@ -1797,6 +1800,7 @@ int ksp_solve_device(p4g_solver_s* s, const double* b, double* x) {
         R.its = co.its; R.reason = co.reason; R.rnorm0 = co.rnorm0; R.rnorm = co.rnorm;
         R.inner_its = 0;
         for (size_t q = 0; q < 64; ++q) R.history[q] = q < co.hist.size() ? co.hist[q] : 0.0;
+        s->hist = co.hist;
     }
     HIP_TRY(hipEventRecord(e1, st));
     HIP_TRY(hipEventSynchronize(e1));
@@ -2045,6 +2049,13 @@ int p4g_mat_mult(p4g_Solver s, const double* x, double* y) {
     hipFree(dx);
     hipFree(dy);
     return e;
+}
+
+int p4g_solver_get_history(p4g_Solver s, double* h, int cap, int* n) {
+    if (!s || !n || (cap > 0 && !h)) return set_error(PETSC_ERR_ARG_WRONG, "p4g_solver_get_history: bad argument");
+    *n = (int)s->hist.size();
+    for (int q = 0; q < std::min(cap, *n); ++q) h[q] = s->hist[q];
+    return 0;
 }
 
 int p4g_solver_get_report(p4g_Solver s, p4g_Report* rep) {

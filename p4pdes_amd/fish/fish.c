@@ -275,6 +275,14 @@ int fish_run(int argc, const char **argv, fish_result *res, double *u_out, char 
         outf(&o, "Matrix is symmetric (tolerance %g)\n", tol);
         outf(&o, "Matrix is symmetric (tolerance %g)\n", tol);
     }
+    if (opt_get(&a, "ksp_monitor")) {   /* KSPMonitorResidual, at the KSP's tab level under SNES */
+        int nh = 0;
+        p4g_solver_get_history(S.solver, NULL, 0, &nh);
+        double *h = (double *)malloc(sizeof(double) * (nh > 0 ? nh : 1));
+        p4g_solver_get_history(S.solver, h, nh, &nh);
+        for (int q = 0; q < nh; q++) outf(&o, "    %3d KSP Residual norm %14.12e \n", q, h[q]);
+        free(h);
+    }
     if (opt_get(&a, "ksp_converged_reason")) {
         if (rep.reason > 0)
             outf(&o, "    Linear solve converged due to %s iterations %d\n", reason_name(rep.reason), rep.its);

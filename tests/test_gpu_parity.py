@@ -9,6 +9,7 @@ of the sine eigen-modes, the true FP64 residual, and agreement of the
 mixed-precision and FP64 device solutions.
 """
 import math
+import re
 
 import numpy as np
 import pytest
@@ -217,6 +218,32 @@ def test_ksp_viewer_files(tmp_path):
     r = b - _csr_apply(M, rowptr, cols, vals, y)
     assert np.linalg.norm(r) <= 1.5e-10 * np.linalg.norm(b)   # recursive vs true residual
     assert np.array_equal(capi.vec_load_binary(fb, 1), y)
+
+
+@pytest.mark.parametrize("args", [
+    "-fsh_dim 3 -fsh_problem manusin -da_refine 4 %s -ksp_rtol 1e-10" % MG,
+    "-fsh_dim 2 -fsh_problem manupoly -da_refine 5 %s -ksp_norm_type unpreconditioned" % MG,
+    "-fsh_dim 3 -fsh_problem manusin -da_refine 3 %s -ksp_norm_type natural -pc_mg_galerkin" % MG,
+], ids=["3d_precond", "2d_unprecond", "3d_natural_galerkin"])
+def test_ksp_monitor_lines(args):
+    # -ksp_monitor: PETSc's KSPMonitorResidual lines, before the converged
+    # reason, with the oracle's residual norms
+    a = args + " -ksp_monitor -ksp_converged_reason -snes_monitor_short"
+    rc, text, res, _ = capi.fish_run(a)
+    assert rc == 0, text
+    orc, otext, ores, _ = O.fish_run(a)
+    g = [l for l in text.splitlines() if "KSP Residual norm" in l]
+    o = [l for l in otext.splitlines() if "KSP Residual norm" in l]
+    assert len(g) == len(o) == res.its + 1, (len(g), len(o), res.its)
+    pat = re.compile(r"^     *\d+ KSP Residual norm \d\.\d{12}e[+-]\d\d $")
+    h0 = float(o[0].split()[-1])
+    for q, (lg, lo) in enumerate(zip(g, o)):
+        assert pat.match(lg), lg
+        assert lg.split("KSP")[0] == lo.split("KSP")[0]
+        vg, vo = float(lg.split()[-1]), float(lo.split()[-1])
+        assert abs(vg - vo) <= 1e-6 * abs(vo) + 1e-12 * h0, (q, vg, vo)
+    lines = text.splitlines()
+    assert lines.index(g[-1]) + 1 == next(i for i, l in enumerate(lines) if "Linear solve" in l)
 
 
 def test_rhs_parity():
