@@ -191,6 +191,42 @@ int p4g_solver_reset_stats(p4g_Solver s);
  * finishes.  b and x are host arrays owned by the caller; both must stay
  * valid until the request has finished.
  * Options: -p4g_serve_graph 1|0, -p4g_serve_max_slots n (default 64). */
+/* ------------------------------------------- variational inequalities */
+
+/* SNESSolve with SNESVINEWTONRSLS and SNESVISetComputeVariableBounds
+ * (c/ch12/obstacle.c:118-121,130-134): the reduced-space Newton method for
+ * xl <= x <= xu, F(x) of the grid's residual callback, the Jacobian of its
+ * Jacobian callback.  Per Newton step the active set {x <= xl + 1e-8,
+ * F > 0} u {x >= xu - 1e-8, F < 0}, the inactive-set system J_II y = F_I
+ * by the solver's KSP (CG; -pc_type mg uses Galerkin coarse operators on
+ * coarse inactive sets, as PETSc does under RSLS), the backtracking (bt,
+ * alpha 0) or basic line search with trial points projected onto the
+ * bounds, and SNESConvergedDefault on |F| over the inactive set.  Options
+ * (from p4g_solver_create): -snes_rtol 1e-8, -snes_atol 1e-50, -snes_stol
+ * 1e-8, -snes_max_it 50, -snes_divergence_tolerance 1e4,
+ * -snes_linesearch_type bt|basic, -snes_type vinewtonrsls.  xu may be NULL
+ * (+infinity).  x: initial iterate in, solution out; F (may be NULL): the
+ * residual at the solution (SNESGetFunction).  Host arrays of the grid's
+ * size.  The solver's fine operator stays the last reduced one afterwards:
+ * use a solver for VI solves only. */
+typedef struct {
+    int    its;            /* Newton iterations (snes->iter)                   */
+    int    reason;         /* SNESConvergedReason (3 CONVERGED_FNORM_RELATIVE) */
+    int    last_ksp_its;   /* iterations of the last linear solve             */
+    int    ksp_its_total;
+    long   active;         /* size of the last active set                      */
+    double fnorm0, fnorm;  /* |F| on the inactive set, first and last          */
+    double solve_ms;       /* host wall time of the solve                      */
+} p4g_VIReport;
+int p4g_vi_solve(p4g_Solver s, const double *xl, const double *xu, double *x, double *F,
+                 p4g_VIReport *rep);
+/* *nf norms fnorm[i], i = 0..its (the SNES monitor values), and *nk
+ * linear solves, Newton step i's ksp_its[i] and ksp_reason[i] (a step that
+ * ended the solve in failure has a linear solve but no norm).  Copies the
+ * first min(cap, count) entries of each; arrays may be NULL. */
+int p4g_vi_get_history(p4g_Solver s, double *fnorm, int *ksp_its, int *ksp_reason, int cap, int *nf,
+                       int *nk);
+
 typedef struct p4g_server_s *p4g_Server;
 
 typedef struct {

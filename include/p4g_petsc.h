@@ -42,6 +42,7 @@ typedef int    PetscEnum;
 #define PETSC_ERR_ARG_WRONG 62
 #define PETSC_ERR_ARG_SIZ   60
 #define PETSC_ERR_ARG_OUTOFRANGE 63
+#define PETSC_ERR_ORDER     73
 #define PETSC_ERR_LIB       76
 #define PETSC_ERR_FILE_OPEN 65
 #define PETSC_ERR_FILE_READ 66
@@ -96,6 +97,10 @@ PetscErrorCode p4g_seterr(PetscErrorCode code, const char *fmt, ...);
 
 /* grid (DMDA) queries */
 PetscErrorCode DMGetBoundingBox(DM da, PetscReal gmin[], PetscReal gmax[]);
+/* the grid's box [min, max] per dimension (default [0, L] from p4g_da_create);
+ * replaces DMDASetUniformCoordinates (c/ch12/obstacle.c:95) */
+PetscErrorCode DMDASetUniformCoordinates(DM da, PetscReal xmin, PetscReal xmax, PetscReal ymin,
+                                         PetscReal ymax, PetscReal zmin, PetscReal zmax);
 PetscErrorCode DMDAGetLocalInfo(DM da, DMDALocalInfo *info);
 PetscErrorCode DMSetApplicationContext(DM da, void *ctx);
 PetscErrorCode DMGetApplicationContext(DM da, void *ctx);

@@ -4,7 +4,8 @@
  * A plain-C restatement of the reference's structured-grid Poisson path
  * (c/ch6/fish.c with c/ch6/poissonfunctions.c) together with the PETSc
  * solver semantics that path reaches through its options (KSPCG, PCMG,
- * KSPCHEBYSHEV, KSPRICHARDSON, PCJACOBI, PCSOR, PCILU, PCLU).  It exists to
+ * KSPCHEBYSHEV, KSPRICHARDSON, PCJACOBI, PCSOR, PCILU, PCLU), and of the
+ * obstacle problem of c/ch12/obstacle.c under SNESVINEWTONRSLS.  It exists to
  * check the HIP product library and to time a CPU baseline; nothing in the
  * product links, loads or calls it.
  *
@@ -57,6 +58,27 @@ typedef struct {
  * Returns 0 on success, a nonzero PETSc-like error code otherwise. */
 int oracle_fish_run(int argc, const char **argv, fish_result *res,
                     double *u_out, char *outbuf, int outlen);
+
+/* Result of one obstacle run (c/ch12/obstacle.c); the same layout as
+ * obstacle_result in the product's obstacle driver. */
+typedef struct {
+    int    mx, my;
+    int    its;              /* SNES iterations on the final grid          */
+    int    reason;           /* SNESConvergedReason numbering              */
+    int    last_ksp_its;     /* KSPGetIterationNumber after the last solve */
+    int    ksp_its_total;    /* final grid                                 */
+    int    stages;           /* 1 + -snes_grid_sequence                    */
+    int    stage_its[8];     /* SNES iterations per grid                   */
+    long   active;           /* GetActiveSet count (obstacle.c:201-232)    */
+    double fnorm0, fnorm;    /* inactive-set |F|, final grid               */
+    double error1, errorinf, areaerr;   /* obstacle.c:180-196             */
+    double setup_ms, solve_ms;
+} obstacle_result;
+
+/* Run the oracle as `obstacle` would be run with argv; text to outbuf,
+ * the final solution (mx*my doubles) to u_out; both may be NULL. */
+int oracle_obstacle_run(int argc, const char **argv, obstacle_result *res,
+                        double *u_out, char *outbuf, int outlen);
 
 /* Time `nsolves` solves (after `warmup`) of the linear system fish builds
  * for argv, with `nthreads` OpenMP threads.  Setup is excluded.  Fills

@@ -16,8 +16,6 @@
 #endif
 #include "oracle_internal.h"
 
-int oracle_parse_ksp(const char *s, int def);
-int oracle_parse_pc(const char *s, int def);
 
 static double now_ms(void) {
     struct timespec ts;

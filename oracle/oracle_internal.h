@@ -95,7 +95,7 @@ int         oopts_bool(oopts *o, const char *key, int def);
 
 /* ---- solvers (oracle_solver.c) ---- */
 enum { KSP_CG = 0, KSP_RICHARDSON, KSP_CHEBYSHEV, KSP_PREONLY };
-enum { PC_NONE = 0, PC_JACOBI, PC_SOR, PC_ILU, PC_LU, PC_MG };
+enum { PC_NONE = 0, PC_JACOBI, PC_SOR, PC_ILU, PC_LU, PC_MG, PC_SHELL };
 enum { NORM_NONE = 0, NORM_PRECONDITIONED, NORM_UNPRECONDITIONED, NORM_NATURAL };
 
 /* KSPConvergedReason values */
@@ -121,6 +121,8 @@ typedef struct opc {
     ocsr         F;  int64_t *dpos;   /* ilu: factors and diagonal offsets */
     double      *band; int bw;        /* lu: banded factor                 */
     omg         *mg;
+    void       (*shell)(void *ctx, const double *r, double *z);   /* PCSHELL */
+    void        *shell_ctx;
 } opc;
 void opc_setup(opc *pc, int type, const ocsr *A);
 void opc_apply(opc *pc, const double *r, double *z);
@@ -136,12 +138,15 @@ typedef struct {
     int     est_lanczos;     /* 1: Lanczos(CG), 0: GMRES Hessenberg     */
     int     esteig_steps;
     double  est_tf[4];       /* emin = tf0*min+tf1*max, emax = tf2*min+tf3*max */
+    const long *gidx;        /* estimate start vector: noisy(gidx[i]), else noisy(i) */
     opc     pc;
     const ocsr *A;
     double *w[5];
     FILE   *mon; char *mbuf; int mlen; int *mpos; const char *prefix;
 } oksp;
 void oksp_init(oksp *k, int type, const ocsr *A, int pctype);
+int  oracle_parse_ksp(const char *s, int def);   /* -1 when unknown */
+int  oracle_parse_pc(const char *s, int def);
 void oksp_free(oksp *k);
 /* Solve A x = b; returns its; sets *reason, *rnorm0, *rnorm. */
 int  oksp_solve(oksp *k, const double *b, double *x, int *reason, double *rnorm0,
@@ -165,6 +170,9 @@ int  omg_setup(omg *mg, const ogrid *fine, const oprob *prob, const ocsr *Afine,
                const double *kcell_fine, int nlev, oopts *o, char *err, int errlen);
 void omg_apply(omg *mg, const double *b, double *x);
 void omg_free(omg *mg);
+
+/* A_c = P^T A P (CSR triple product, sorted columns) */
+void oracle_galerkin(const ocsr *A, const ocsr *P, long nc, ocsr *Ac);
 
 /* emit text like PetscPrintf into a growing buffer and/or stdout */
 void oprintf(char *buf, int len, int *pos, const char *fmt, ...);

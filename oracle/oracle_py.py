@@ -166,3 +166,44 @@ def rhs(args):
     if rc:
         raise RuntimeError("oracle_rhs failed: %d" % rc)
     return F
+
+
+class ObstacleResult(ctypes.Structure):
+    """Mirror of obstacle_result in oracle/oracle.h."""
+    _fields_ = [(n, ctypes.c_int) for n in
+                ("mx", "my", "its", "reason", "last_ksp_its", "ksp_its_total", "stages")] + \
+               [("stage_its", ctypes.c_int * 8), ("active", ctypes.c_long)] + \
+               [(n, ctypes.c_double) for n in
+                ("fnorm0", "fnorm", "error1", "errorinf", "areaerr", "setup_ms", "solve_ms")]
+
+    def as_dict(self):
+        d = {n: getattr(self, n) for n, _ in self._fields_}
+        d["stage_its"] = list(self.stage_its)[:self.stages]
+        return d
+
+
+def obstacle_shape(args):
+    """(mx, my) of the final grid obstacle builds for these options."""
+    if isinstance(args, str):
+        args = args.split()
+    def opt(k, d):
+        return int(args[args.index(k) + 1]) if k in args else d
+    r = opt("-da_refine", 0) + opt("-snes_grid_sequence", 0)
+    return (opt("-da_grid_x", 3) - 1) * 2 ** r + 1, (opt("-da_grid_y", 3) - 1) * 2 ** r + 1
+
+
+def obstacle_run(args, want_u=False):
+    """Run the oracle as c/ch12/obstacle.c would run with `args`.
+
+    Returns (rc, text, ObstacleResult, u or None)."""
+    argc, argv = _argv(args)
+    res = ObstacleResult()
+    buf = ctypes.create_string_buffer(1 << 18)
+    u = None
+    uptr = None
+    if want_u:
+        mx, my = obstacle_shape(args)
+        u = np.zeros(mx * my)
+        uptr = u.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    rc = lib().oracle_obstacle_run(argc, argv, ctypes.byref(res), uptr, buf, len(buf))
+    return rc, buf.value.decode(), res, u

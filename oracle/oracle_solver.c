@@ -247,6 +247,7 @@ void opc_apply(opc *pc, const double *r, double *z) {
     }
     case PC_LU: band_solve(pc, r, z); break;
     case PC_MG: omg_apply(pc->mg, r, z); break;
+    case PC_SHELL: pc->shell(pc->shell_ctx, r, z); break;
     }
 }
 
@@ -321,7 +322,7 @@ void oksp_estimate_eigs(oksp *k) {
     long n = A->n;
     double *r = (double *)malloc(sizeof(double) * n), *z = (double *)malloc(sizeof(double) * n);
     double *p = (double *)malloc(sizeof(double) * n), *w = (double *)malloc(sizeof(double) * n);
-    for (long i = 0; i < n; i++) r[i] = oracle_noisy(i);
+    for (long i = 0; i < n; i++) r[i] = oracle_noisy(k->gidx ? k->gidx[i] : i);
     double al[16], be[16];
     int m = 0;
     opc_apply(&k->pc, r, z);
@@ -497,7 +498,7 @@ int oksp_solve(oksp *k, const double *b, double *x, int *reason, double *rnorm0,
 /* ------------------------------------------------------------------- MG */
 
 /* Galerkin coarse operator A_c = P^T A P */
-static void galerkin(const ocsr *A, const ocsr *P, long nc, ocsr *Ac) {
+void oracle_galerkin(const ocsr *A, const ocsr *P, long nc, ocsr *Ac) {
     /* transpose of P */
     long nf = P->n;
     int64_t *tc = (int64_t *)calloc(nc + 1, sizeof(int64_t));
@@ -637,7 +638,7 @@ int omg_setup(omg *mg, const ogrid *fine, const oprob *prob, const ocsr *Afine,
     for (int l = nlev - 1; l > 0; l--) {
         if (mg->galerkin) {
             oq1_build_P(&mg->g[l - 1], &mg->g[l], &mg->P[l]);
-            galerkin(&mg->A[l], &mg->P[l], mg->g[l - 1].n, &mg->A[l - 1]);
+            oracle_galerkin(&mg->A[l], &mg->P[l], mg->g[l - 1].n, &mg->A[l - 1]);
         } else if (kfine) {
             const ogrid *gf = &mg->g[l];
             size_t nc = (size_t)((gf->mx - 1) / 2) * ((gf->my - 1) / 2) * ((gf->mz - 1) / 2);

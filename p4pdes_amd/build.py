@@ -57,6 +57,23 @@ def build_fish():
     return out
 
 
+def build_obstacle():
+    """libp4g_obstacle.so + the `obstacle` driver (c/ch12/obstacle.c on the ABI),
+    linked with the ch6 callbacks as the reference links ../ch6/poissonfunctions.o."""
+    os.makedirs(LIB, exist_ok=True)
+    obs = os.path.join(PKG, "obstacle")
+    srcs = [os.path.join(FISH, "poissonfunctions.c"), os.path.join(obs, "obstacle.c")]
+    out = os.path.join(LIB, "libp4g_obstacle.so")
+    _run(["gcc"] + C_FLAGS + ["-I" + FISH, "-shared", "-o", out + ".tmp"] + srcs +
+         ["-L" + LIB, "-lp4g", "-Wl,-rpath,$ORIGIN", "-lm"])
+    os.replace(out + ".tmp", out)
+    exe = os.path.join(LIB, "obstacle")
+    _run(["gcc"] + C_FLAGS + ["-o", exe + ".tmp", os.path.join(obs, "obstacle_main.c"),
+                              "-L" + LIB, "-lp4g_obstacle", "-lp4g", "-Wl,-rpath,$ORIGIN", "-lm"])
+    os.replace(exe + ".tmp", exe)
+    return out
+
+
 def build_oracle():
     _run(["make", "-s", "-C", os.path.join(ROOT, "oracle")])
 
@@ -66,6 +83,7 @@ def build_all():
         raise RuntimeError("hipcc not found: the gfx950 library cannot be built")
     build_lib()
     build_fish()
+    build_obstacle()
     build_oracle()
 
 

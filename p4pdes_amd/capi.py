@@ -1,5 +1,5 @@
-"""ctypes binding of the p4g C ABI (include/p4g.h) and of the fish driver
-(p4pdes_amd/fish/fish.h).  The compute path is the HIP library; there is no
+"""ctypes binding of the p4g C ABI (include/p4g.h) and of the fish and
+obstacle drivers (p4pdes_amd/fish/fish.h, p4pdes_amd/obstacle/obstacle.h).  The compute path is the HIP library; there is no
 CPU fallback: loading fails loudly if the built libraries are missing.
 """
 import ctypes
@@ -11,6 +11,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIBDIR = os.path.join(_HERE, "lib")
 LIBP4G = os.path.join(LIBDIR, "libp4g.so")
 LIBFISH = os.path.join(LIBDIR, "libp4g_fish.so")
+LIBOBST = os.path.join(LIBDIR, "libp4g_obstacle.so")
 
 # kernel classes (enum in p4g.h)
 K_STENCIL_DOT, K_CG_UPDATE, K_RESID_ZG, K_RESTRICT, K_PROLONG, K_SMOOTH, K_COARSE, \
@@ -49,7 +50,32 @@ class FishResult(ctypes.Structure):
 
 
 _p4g = None
+class VIReport(ctypes.Structure):
+    """p4g_VIReport of include/p4g.h."""
+    _fields_ = [(n, ctypes.c_int) for n in ("its", "reason", "last_ksp_its", "ksp_its_total")] + \
+               [("active", ctypes.c_long)] + \
+               [(n, ctypes.c_double) for n in ("fnorm0", "fnorm", "solve_ms")]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+class ObstacleResult(ctypes.Structure):
+    """obstacle_result (same layout as the oracle's)."""
+    _fields_ = [(n, ctypes.c_int) for n in
+                ("mx", "my", "its", "reason", "last_ksp_its", "ksp_its_total", "stages")] + \
+               [("stage_its", ctypes.c_int * 8), ("active", ctypes.c_long)] + \
+               [(n, ctypes.c_double) for n in
+                ("fnorm0", "fnorm", "error1", "errorinf", "areaerr", "setup_ms", "solve_ms")]
+
+    def as_dict(self):
+        d = {n: getattr(self, n) for n, _ in self._fields_}
+        d["stage_its"] = list(self.stage_its)[:self.stages]
+        return d
+
+
 _fish = None
+_obst = None
 
 
 def lib():
@@ -91,6 +117,9 @@ def lib():
         L.p4g_vec_view_binary.argtypes = [cp, ctypes.c_long, vp, i]
         L.p4g_vec_load_binary.argtypes = [cp, i, ctypes.POINTER(ctypes.c_long), vp]
         L.p4g_da_mat_view_binary.argtypes = [vp, cp, i]
+        L.p4g_vi_solve.argtypes = [vp, vp, vp, vp, vp, ctypes.POINTER(VIReport)]
+        L.p4g_vi_get_history.argtypes = [vp, vp, vp, vp, i, ctypes.POINTER(i), ctypes.POINTER(i)]
+        L.DMDASetUniformCoordinates.argtypes = [vp] + [d] * 6
         _p4g = L
     return _p4g
 
@@ -117,6 +146,44 @@ def fishlib():
         L.fish_session_destroy.argtypes = [ctypes.POINTER(vp)]
         _fish = L
     return _fish
+
+
+def obstaclelib():
+    global _obst
+    if _obst is None:
+        lib()
+        if not os.path.exists(LIBOBST):
+            raise RuntimeError("libp4g_obstacle.so is not built")
+        L = ctypes.CDLL(LIBOBST)
+        L.obstacle_run.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.POINTER(ObstacleResult),
+                                   ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int]
+        _obst = L
+    return _obst
+
+
+def obstacle_shape(args):
+    """(mx, my) of the final grid obstacle builds for these options."""
+    if isinstance(args, str):
+        args = args.split()
+    def opt(k, dflt):
+        return int(args[args.index(k) + 1]) if k in args else dflt
+    r = opt("-da_refine", 0) + opt("-snes_grid_sequence", 0)
+    return (opt("-da_grid_x", 3) - 1) * 2 ** r + 1, (opt("-da_grid_y", 3) - 1) * 2 ** r + 1
+
+
+def obstacle_run(args, want_u=False):
+    """Run obstacle on the GPU.  Returns (rc, text, ObstacleResult, u or None)."""
+    argc, argv = _argv(args)
+    res = ObstacleResult()
+    buf = ctypes.create_string_buffer(1 << 18)
+    u = None
+    uptr = None
+    if want_u:
+        mx, my = obstacle_shape(args)
+        u = np.zeros(mx * my)
+        uptr = u.ctypes.data
+    rc = obstaclelib().obstacle_run(argc, argv, ctypes.byref(res), uptr, buf, len(buf))
+    return rc, buf.value.decode(), res, u
 
 
 def last_error():

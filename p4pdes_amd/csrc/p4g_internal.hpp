@@ -44,6 +44,7 @@ struct Options {
 struct p4g_da_s {
     int    dim = 2, mx = 3, my = 3, mz = 3;
     double L[3] = {1.0, 1.0, 1.0};
+    double lo[3] = {0.0, 0.0, 0.0};   // bounding box: [lo, lo + L] per dimension
     int    refine = 0;
     DMDASNESFunctionFn* fn = nullptr;
     void*  fnctx = nullptr;

@@ -895,7 +895,10 @@ k_stencil27(Op27<T> op, Src src, Out out, double* __restrict__ part) {
         }
         const T uc = src.load(q);
         src.put(q, uc);
-        const T idg = op.jac ? T(1) / T(op.g[13 * op.n + q]) : T(1);
+        // a zero diagonal marks a null row (an active vertex of a reduced-
+        // space VI step): its Jacobi factor is 0, so smoothing leaves it
+        const T dq = T(op.g[13 * op.n + q]);
+        const T idg = op.jac ? (dq != T(0) ? T(1) / dq : T(0)) : T(1);
         out.point(q, uc, Au, idg, a0, a1);
     }
     block_partials<Out::NDOT, 4>(a0, a1, part, blockIdx.x);
@@ -1775,6 +1778,159 @@ k_finalize(long nparts, const double* __restrict__ part, double* __restrict__ S,
     double A, B;
     sum_parts(part, nparts, A, B);
     if (threadIdx.x == 0) finalize_scalars(S, mode, first, A, B);
+}
+
+
+// ------------------------------------------------ reduced-space VI Newton
+// (PETSc's SNESVINEWTONRSLS over the constant 3/5/7-point operator).  A
+// reduced Newton step is solved in full-size vectors: the inactive block
+// keeps the operator's rows and their couplings to inactive vertices; an
+// active vertex becomes a null row (all coefficients zero).  With a right-
+// hand side that is zero on the active set, a Krylov iterate, a Galerkin
+// coarse level (P^T A P of a null-row operator has null rows exactly where
+// no inactive fine vertex restricts) and a Jacobi-smoothed cycle stay equal
+// to the reduced-space ones; the only place a value reaches an active
+// vertex is the prolongation onto the finest level, which is masked.
+
+// the grid and the constant operator of a level (d, db: interior and
+// boundary diagonal; sx, sy, sz: couplings between interior vertices)
+struct ViGrid {
+    int dim, mx, my, mz;
+    long n;
+    double d, db, sx, sy, sz;
+};
+
+__device__ __forceinline__ bool vi_bnd(const ViGrid& g, int i, int j, int k) {
+    return i == 0 || i == g.mx - 1 || (g.dim >= 2 && (j == 0 || j == g.my - 1)) ||
+           (g.dim == 3 && (k == 0 || k == g.mz - 1));
+}
+
+__device__ __forceinline__ void vi_ijk(const ViGrid& g, long q, int& i, int& j, int& k) {
+    i = (int)(q % g.mx);
+    const long r = q / g.mx;
+    j = (int)(r % g.my);
+    k = (int)(r / g.my);
+}
+
+// (A x)_q of the constant operator: boundary rows db x; interior rows
+// d x - s * (interior neighbours)
+__device__ __forceinline__ double vi_apply(const ViGrid& g, const double* __restrict__ x, long q, int i, int j,
+                                           int k) {
+    if (vi_bnd(g, i, j, k)) return g.db * x[q];
+    const long sy = g.mx, sz = (long)g.mx * g.my;
+    double nb = 0.0;
+    if (!vi_bnd(g, i - 1, j, k)) nb += g.sx * x[q - 1];
+    if (!vi_bnd(g, i + 1, j, k)) nb += g.sx * x[q + 1];
+    if (g.dim >= 2) {
+        if (!vi_bnd(g, i, j - 1, k)) nb += g.sy * x[q - sy];
+        if (!vi_bnd(g, i, j + 1, k)) nb += g.sy * x[q + sy];
+    }
+    if (g.dim == 3) {
+        if (!vi_bnd(g, i, j, k - 1)) nb += g.sz * x[q - sz];
+        if (!vi_bnd(g, i, j, k + 1)) nb += g.sz * x[q + sz];
+    }
+    return g.d * x[q] - nb;
+}
+
+// the active-set predicate of SNESVIGetActiveSetIS (bounds: xl, xu or none)
+__device__ __forceinline__ bool vi_is_active(double x, double f, double xl, const double* __restrict__ xu, long q) {
+    return (x <= xl + 1.0e-8 && f > 0.0) || (xu && x >= xu[q] - 1.0e-8 && f < 0.0);
+}
+
+// F = A x + F0 (F0 null: F = A x)
+__global__ void __launch_bounds__(256)
+k_vi_fun(ViGrid g, const double* __restrict__ x, const double* __restrict__ F0, double* __restrict__ F) {
+    for (long q = blockIdx.x * 256L + threadIdx.x; q < g.n; q += (long)gridDim.x * 256L) {
+        int i, j, k;
+        vi_ijk(g, q, i, j, k);
+        const double a = vi_apply(g, x, q, i, j, k);
+        F[q] = F0 ? a + F0[q] : a;
+    }
+}
+
+// x = clamp(x0 - lambda y, xl, xu) (SNESVIProjectOntoBounds of a trial point;
+// y null: project x0)
+__global__ void __launch_bounds__(256)
+k_vi_trial(long n, const double* __restrict__ x0, const double* __restrict__ y, double lambda,
+           const double* __restrict__ xl, const double* __restrict__ xu, double* __restrict__ x) {
+    for (long q = blockIdx.x * 256L + threadIdx.x; q < n; q += (long)gridDim.x * 256L) {
+        double v = y ? x0[q] - lambda * y[q] : x0[q];
+        if (v < xl[q]) v = xl[q];
+        else if (xu && v > xu[q]) v = xu[q];
+        x[q] = v;
+    }
+}
+
+// partials: a0 = sum of F^2 over the inactive set (SNESVIComputeInactiveSet-
+// Fnorm), a1 = |x|^2.  With act non-null also the active flags and the
+// reduced right-hand side Fm (F on the inactive set, 0 on the active set).
+__global__ void __launch_bounds__(256)
+k_vi_sets(long n, const double* __restrict__ x, const double* __restrict__ F, const double* __restrict__ xl,
+          const double* __restrict__ xu, unsigned char* __restrict__ act, double* __restrict__ Fm,
+          double* __restrict__ part) {
+    double a0 = 0.0, a1 = 0.0;
+    for (long q = blockIdx.x * 256L + threadIdx.x; q < n; q += (long)gridDim.x * 256L) {
+        const double xv = x[q], fv = F[q];
+        const bool a = vi_is_active(xv, fv, xl[q], xu, q);
+        if (!a) a0 += fv * fv;
+        a1 += xv * xv;
+        if (act) {
+            act[q] = a ? 1 : 0;
+            Fm[q] = a ? 0.0 : fv;
+        }
+    }
+    block_partials<2, 4>(a0, a1, part, blockIdx.x);
+}
+
+// partials: a0 = (x, y), a1 = (y, y)
+__global__ void __launch_bounds__(256)
+k_vi_dot(long n, const double* __restrict__ x, const double* __restrict__ y, double* __restrict__ part) {
+    double a0 = 0.0, a1 = 0.0;
+    for (long q = blockIdx.x * 256L + threadIdx.x; q < n; q += (long)gridDim.x * 256L) {
+        const double yv = y[q];
+        a0 += x[q] * yv;
+        a1 += yv * yv;
+    }
+    block_partials<2, 4>(a0, a1, part, blockIdx.x);
+}
+
+// the reduced operator in 27-point form (coefficient c = 9(oz+1) + 3(oy+1)
+// + (ox+1) of row q at g27[c n + q]): inactive rows keep the diagonal and
+// the couplings to inactive interior neighbours; active rows are null
+__global__ void __launch_bounds__(256)
+k_vi_nullrow27(ViGrid g, const unsigned char* __restrict__ act, double* __restrict__ g27) {
+    for (long q = blockIdx.x * 256L + threadIdx.x; q < g.n; q += (long)gridDim.x * 256L) {
+        int i, j, k;
+        vi_ijk(g, q, i, j, k);
+        double c[27];
+        for (int t = 0; t < 27; ++t) c[t] = 0.0;
+        if (!act[q]) {
+            const bool bnd = vi_bnd(g, i, j, k);
+            c[13] = bnd ? g.db : g.d;
+            if (!bnd) {
+                const long sy = g.mx, sz = (long)g.mx * g.my;
+                if (!vi_bnd(g, i - 1, j, k) && !act[q - 1]) c[12] = -g.sx;
+                if (!vi_bnd(g, i + 1, j, k) && !act[q + 1]) c[14] = -g.sx;
+                if (g.dim >= 2) {
+                    if (!vi_bnd(g, i, j - 1, k) && !act[q - sy]) c[10] = -g.sy;
+                    if (!vi_bnd(g, i, j + 1, k) && !act[q + sy]) c[16] = -g.sy;
+                }
+                if (g.dim == 3) {
+                    if (!vi_bnd(g, i, j, k - 1) && !act[q - sz]) c[4] = -g.sz;
+                    if (!vi_bnd(g, i, j, k + 1) && !act[q + sz]) c[22] = -g.sz;
+                }
+            }
+        }
+        for (int t = 0; t < 27; ++t) g27[(long)t * g.n + q] = c[t];
+    }
+}
+
+// x_q = 0 where the operator's diagonal is zero (an active vertex)
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_vi_mask(long n, const double* __restrict__ diag, T* __restrict__ x) {
+    for (long q = blockIdx.x * 256L + threadIdx.x; q < n; q += (long)gridDim.x * 256L)
+        if (diag[q] == 0.0) x[q] = T(0);
 }
 
 }  // namespace p4g

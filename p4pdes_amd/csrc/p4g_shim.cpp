@@ -233,7 +233,20 @@ int p4g_vec_load_binary(const char* path, int index, long* n, double* x) {
 
 PetscErrorCode DMGetBoundingBox(DM da, PetscReal gmin[], PetscReal gmax[]) {
     if (!da) return set_error(PETSC_ERR_ARG_WRONG, "DMGetBoundingBox: null DM");
-    for (int d = 0; d < da->dim; ++d) { gmin[d] = 0.0; gmax[d] = da->L[d]; }
+    for (int d = 0; d < da->dim; ++d) { gmin[d] = da->lo[d]; gmax[d] = da->lo[d] + da->L[d]; }
+    return 0;
+}
+
+PetscErrorCode DMDASetUniformCoordinates(DM da, PetscReal xmin, PetscReal xmax, PetscReal ymin, PetscReal ymax,
+                                         PetscReal zmin, PetscReal zmax) {
+    if (!da) return set_error(PETSC_ERR_ARG_WRONG, "DMDASetUniformCoordinates: null DM");
+    const double a[3] = {xmin, ymin, zmin}, b[3] = {xmax, ymax, zmax};
+    for (int d = 0; d < da->dim; ++d) {
+        if (!(b[d] > a[d]))
+            return set_error(PETSC_ERR_ARG_OUTOFRANGE, "DMDASetUniformCoordinates: empty interval in dimension %d", d);
+        da->lo[d] = a[d];
+        da->L[d] = b[d] - a[d];
+    }
     return 0;
 }
 

@@ -257,6 +257,10 @@ struct Engine {
     std::vector<Dev> dv;
     double* Ainv = nullptr;   // dense inverse of the coarsest operator
     int n0 = 0;
+    // reduced-space VI step: diagonal of the finest (null-row) operator,
+    // zero at active vertices; null otherwise
+    const double* vidiag = nullptr;
+    bool vi = false;   // levels may become null-row Galerkin levels later (full temporaries)
     // Krylov vectors on the finest level
     T *r = nullptr, *z = nullptr, *pa = nullptr, *pb = nullptr, *w = nullptr, *xbuf = nullptr,
       *bbuf = nullptr, *r2 = nullptr;
@@ -305,7 +309,7 @@ struct Engine {
                 if (l < nlev - 1) { P4G_TRY(alloc(&d.x, L.n)); P4G_TRY(alloc(&d.b, L.n)); }
                 if (l > 0) {
                     P4G_TRY(alloc(&d.r, L.n));
-                    const int nt = (nu == 1 && cycles == 1 && zg_fusable(L)) ? 1 : 4;
+                    const int nt = (!vi && nu == 1 && cycles == 1 && zg_fusable(L)) ? 1 : 4;
                     for (int q = 0; q < nt; ++q) P4G_TRY(alloc(&d.t[q], L.n));
                 }
             }
@@ -332,6 +336,16 @@ struct Engine {
             HIP_TRY(hipMemcpy(Ainv, Ainv_host.data(), Ainv_host.size() * sizeof(double),
                               hipMemcpyHostToDevice));
         }
+        return 0;
+    }
+
+    // a new coarsest operator of the same size (reduced-space VI steps)
+    int set_coarse_inverse(const std::vector<double>& Ainv_host) {
+        if (!Ainv || (long)Ainv_host.size() != (long)n0 * n0)
+            return set_error(PETSC_ERR_LIB, "internal: coarse inverse of a different size");
+        HIP_TRY(hipMemcpyAsync(Ainv, Ainv_host.data(), Ainv_host.size() * sizeof(double), hipMemcpyHostToDevice,
+                               cx->stream));
+        HIP_TRY(hipStreamSynchronize(cx->stream));
         return 0;
     }
 
@@ -455,8 +469,19 @@ struct Engine {
                   : prolong_smooth<false>(l, pre, cz, xc, oc, passes, nb, jac);
     }
 
-    // out = (zg ? c b : base) + P x_c   (level l-1 -> l)
+    // out = (zg ? c b : base) + P x_c   (level l-1 -> l); on the finest level
+    // of a reduced-space VI step the active vertices of out are zeroed
     int prolong(int l, bool zg, const T* base, T c, const T* xc, T* out) {
+        P4G_TRY(prolong_q1(l, zg, base, c, xc, out));
+        if (l != nlev - 1 || !vidiag) return 0;
+        const long n = (*lv)[l].n;
+        hipStream_t s = cx->stream;
+        return cx->launch(P4G_K_PROLONG, true, double(n) * (8.0 + sizeof(T)), [&] {
+            k_vi_mask<T><<<vblocks(n), 256, 0, s>>>(n, vidiag, out);
+        });
+    }
+
+    int prolong_q1(int l, bool zg, const T* base, T c, const T* xc, T* out) {
         const LevelInfo& F = (*lv)[l];
         const LevelInfo& C = (*lv)[l - 1];
         const dim3 g = transfer_grid(F.dim, C.mx, F.my, F.mz, kchf(F.dim, F.my, F.mz)), b = stencil_block(F.dim);
@@ -957,6 +982,16 @@ struct p4g_solver_s {
     int fuse_r = -1, fuse_dir = -1;   // CG updates fused into stencil passes: 1 on, 0 off, -1 by rule
     bool galerkin = false;            // coarse operators P^T A P (-pc_mg_galerkin)
     bool fmg = false;                 // -pc_mg_type full: one F-cycle per preconditioner apply
+    // reduced-space VI Newton (p4g_vi_solve): SNES options and state
+    double snes_rtol = 1e-8, snes_atol = 1e-50, snes_stol = 1e-8, snes_divtol = 1e4;
+    int snes_maxit = 50;
+    std::string snes_type, ls_type;
+    bool vi_mode = false;                 // set up for p4g_vi_solve
+    double* vi_buf = nullptr;             // device: F0 X F Y W G Fm xl xu, n doubles each
+    unsigned char* vi_act = nullptr;      // device: active flags
+    double* vi_g27 = nullptr;             // device: the finest null-row operator (27 planes)
+    std::vector<double> vi_fhist;         // |F| on the inactive set, iterations 0..its
+    std::vector<int> vi_kits, vi_kreason; // per Newton step: KSP iterations and reason
     // state
     bool ready = false;
     std::vector<LevelInfo> lv;
@@ -1066,6 +1101,10 @@ static int coarse_factor(const LevelInfo& L, const double* g27, std::vector<doub
                         A[q * n + q + ((long)oz * my + oy) * mx + ox] = g[(size_t)c * n + q];
                     }
                 }
+        // a null row (an active vertex of a reduced-space VI step) has a zero
+        // row and column: factor it as an identity row, so x_q = b_q
+        for (int q = 0; q < n; ++q)
+            if (A[(size_t)q * n + q] == 0.0) A[(size_t)q * n + q] = 1.0;
     }
     for (int k = 0; k < (g27 ? 0 : L.mz); ++k)
         for (int j = 0; j < L.my; ++j)
@@ -1122,7 +1161,7 @@ static int coarse_factor(const LevelInfo& L, const double* g27, std::vector<doub
 // -pc_mg_galerkin: the coarse operators P^T A P, formed on the device in
 // 27-point form level by level from the finest (rediscretized) operator
 static int galerkin_levels(p4g_solver_s* s) {
-    s->g27.assign(s->nlev, nullptr);
+    if ((int)s->g27.size() != s->nlev) s->g27.assign(s->nlev, nullptr);
     if (!s->galerkin) return 0;
     if (s->lv.back().var)
         return set_error(PETSC_ERR_SUP, "-pc_mg_galerkin with a variable coefficient is not implemented on the device");
@@ -1130,11 +1169,11 @@ static int galerkin_levels(p4g_solver_s* s) {
     for (int l = s->nlev - 2; l >= 0; --l) {
         LevelInfo& C = s->lv[l];
         const LevelInfo& L = s->lv[l + 1];
-        P4G_TRY(dalloc(s, (void**)&s->g27[l], 27 * C.n * sizeof(double)));
+        if (!s->g27[l]) P4G_TRY(dalloc(s, (void**)&s->g27[l], 27 * C.n * sizeof(double)));
         GalFine f;
         f.dim = L.dim; f.mx = L.mx; f.my = L.my; f.mz = L.mz;
         f.d = L.d; f.db = L.db; f.sx = L.sx; f.sy = L.sy; f.sz = L.sz;
-        f.g = s->g27[l + 1];   // null on the finest level
+        f.g = s->g27[l + 1];   // null on a rediscretized finest level
         f.n = L.n;
         k_galerkin<<<vblocks(C.n), 256, 0, st>>>(f, C.mx, C.my, C.mz, s->g27[l]);
         LAUNCH_CHECK();
@@ -1276,6 +1315,13 @@ static int configure(p4g_solver_s* s) {
     else if (mt != "multiplicative")
         return set_error(PETSC_ERR_SUP, "-pc_mg_type %s is not implemented on the device (multiplicative, full)",
                          mt.c_str());
+    s->snes_rtol = o.getr("snes_rtol", 1e-8);
+    s->snes_atol = o.getr("snes_atol", 1e-50);
+    s->snes_stol = o.getr("snes_stol", 1e-8);
+    s->snes_divtol = o.getr("snes_divergence_tolerance", 1e4);
+    s->snes_maxit = o.geti("snes_max_it", 50);
+    s->snes_type = o.gets("snes_type", "");
+    s->ls_type = o.gets("snes_linesearch_type", "bt");
     s->rtol = o.getr("ksp_rtol", 1e-5);
     s->atol = o.getr("ksp_atol", 1e-50);
     s->dtol = o.getr("ksp_divtol", 1e5);
@@ -1482,6 +1528,7 @@ int solver_setup(p4g_solver_s* s) {
         e->jacobi = s->jacobi; e->rich_scale = s->rich_scale;
         e->g27.assign(s->g27.begin(), s->g27.end());
         e->fmg = s->fmg;
+        e->vi = s->vi_mode;
     };
     if (F.var) {
         const size_t nc = F.kcell.size();
@@ -1503,6 +1550,7 @@ int solver_setup(p4g_solver_s* s) {
         s->e64.reset(new Engine<double>);
         cfg(s->e64);
         fuse_rules(s, s->e64.get(), F.n * sizeof(double));
+        if (s->vi_mode) s->e64->fuse_r = s->e64->fuse_dir = false;
         P4G_TRY(s->e64->setup(&s->cx, &s->lv, s->Ainv, s->n0));
     }
     HIP_TRY(hipStreamSynchronize(s->cx.stream));
@@ -2143,3 +2191,5 @@ const char* p4g_version(void) { return "p4g 0.1 (gfx950)"; }
 
 // continuous-batching server (uses the types above)
 #include "p4g_serve_impl.hpp"
+// reduced-space VI Newton (SNESVINEWTONRSLS) over the solver's levels
+#include "p4g_vi_impl.hpp"

@@ -1,0 +1,315 @@
+// Reduced-space Newton for variational inequalities on the device: PETSc's
+// SNESSolve_VINEWTONRSLS with SNESLineSearchApply_BT (alpha 0) or _Basic and
+// SNESConvergedDefault, as c/ch12/obstacle.c:118-121 selects it.  The
+// scalar decisions (convergence tests, the line-search model) run on the
+// host; every vector pass runs on the device.  A Newton step solves the
+// inactive-set system in full-size vectors with the active vertices as
+// null rows (k_vi_nullrow27); under -pc_type mg the coarse operators are
+// the Galerkin products of that operator, which is what PETSc's
+// PCMGSetGalerkin under RSLS computes on its coarse inactive sets.
+// Included once, at the end of p4g_solver.hip.
+
+namespace p4g {
+
+struct ViRun {
+    p4g_solver_s* s = nullptr;
+    ViGrid g;
+    long n = 0;
+    int nb = 1;
+    hipStream_t st = nullptr;
+    double *F0 = nullptr, *X = nullptr, *F = nullptr, *Y = nullptr, *W = nullptr, *G = nullptr,
+           *Fm = nullptr, *xl = nullptr, *xu = nullptr;
+    double* S = nullptr;   // scalar block: the second half of cx.S
+
+    int sums(double& a, double& b) {
+        P4G_TRY(s->cx.finalize(nb, S, FIN_RAW, 0));
+        P4G_TRY(s->cx.readS(s->cx.hS));
+        a = s->cx.hS[16 + S_TMP0];
+        b = s->cx.hS[16 + S_TMP1];
+        return 0;
+    }
+    // f = A x + F0 (rhs) or A x
+    int fun(const double* x, double* f, bool rhs) {
+        k_vi_fun<<<nb, 256, 0, st>>>(g, x, rhs ? F0 : nullptr, f);
+        LAUNCH_CHECK();
+        return 0;
+    }
+    // |f| on the inactive set of (x, f), and |x|
+    int norms(const double* x, const double* f, double& fn, double& xn) {
+        k_vi_sets<<<nb, 256, 0, st>>>(n, x, f, xl, xu, nullptr, nullptr, s->cx.part);
+        LAUNCH_CHECK();
+        double a, b;
+        P4G_TRY(sums(a, b));
+        fn = std::sqrt(a);
+        xn = std::sqrt(b);
+        return 0;
+    }
+    // (x, y) and |y|
+    int dot(const double* x, const double* y, double& xy, double& yn) {
+        k_vi_dot<<<nb, 256, 0, st>>>(n, x, y, s->cx.part);
+        LAUNCH_CHECK();
+        double a, b;
+        P4G_TRY(sums(a, b));
+        xy = a;
+        yn = std::sqrt(b);
+        return 0;
+    }
+    // w = proj(x0 - lambda y), gv = F(w), gn = |gv| on w's inactive set
+    int trial(const double* x0, const double* y, double lambda, double* w, double* gv, double& gn) {
+        k_vi_trial<<<nb, 256, 0, st>>>(n, x0, y, lambda, xl, xu, w);
+        LAUNCH_CHECK();
+        P4G_TRY(fun(w, gv, true));
+        double xn;
+        return norms(w, gv, gn, xn);
+    }
+};
+
+// the finest operator of the current active set and, under -pc_type mg, its
+// Galerkin coarse levels, coarse factor and Chebyshev bounds
+static int vi_operator(p4g_solver_s* s, ViRun& R) {
+    const int L = s->nlev - 1;
+    Engine<double>& e = *s->e64;
+    // per-level coefficient tables exist only under -pc_type mg -pc_mg_galerkin
+    if ((int)s->g27.size() != s->nlev) s->g27.assign(s->nlev, nullptr);
+    if ((int)e.g27.size() != s->nlev) e.g27.assign(s->nlev, nullptr);
+    k_vi_nullrow27<<<R.nb, 256, 0, R.st>>>(R.g, s->vi_act, s->vi_g27);
+    LAUNCH_CHECK();
+    s->lv[L].gal = true;
+    s->g27[L] = s->vi_g27;
+    e.g27[L] = s->vi_g27;
+    e.vidiag = s->vi_g27 + 13 * R.n;
+    if (s->pctype != PCT_MG) return 0;
+    P4G_TRY(galerkin_levels(s));
+    for (int l = 0; l < L; ++l) e.g27[l] = s->g27[l];
+    std::vector<double> Ainv;
+    P4G_TRY(coarse_factor(s->lv[0], s->g27[0], Ainv));
+    P4G_TRY(e.set_coarse_inverse(Ainv));
+    if (s->smoother == SM_CHEB)
+        for (int l = 1; l <= L; ++l) P4G_TRY(estimate_level(s, l));
+    return 0;
+}
+
+// SNESConvergedDefault (SNESConvergedReason numbering)
+static int vi_converged(const p4g_solver_s* s, int it, double xnorm, double snorm, double fnorm, double& ttol,
+                        double& rnorm0) {
+    if (it == 0) { ttol = fnorm * s->snes_rtol; rnorm0 = fnorm; }
+    if (std::isnan(fnorm) || std::isinf(fnorm)) return -4;   // DIVERGED_FNORM_NAN
+    if (fnorm < s->snes_atol) return 2;                      // CONVERGED_FNORM_ABS
+    if (it) {
+        if (fnorm <= ttol) return 3;                         // CONVERGED_FNORM_RELATIVE
+        if (snorm < s->snes_stol * xnorm) return 4;          // CONVERGED_SNORM_RELATIVE
+        if (s->snes_divtol > 0 && fnorm > s->snes_divtol * rnorm0) return -9;   // DIVERGED_DTOL
+    }
+    return 0;
+}
+
+// SNESLineSearchApply_BT with alpha = 0 (or _Basic).  On return X, F hold
+// the accepted point (pointers swapped with the trial buffers), fnorm its
+// norm, ynorm = lambda |Y|; returns 1 when the search failed.
+static int vi_line_search(p4g_solver_s* s, ViRun& R, double& fnorm, double& xnorm, double& ynorm, int* fail) {
+    const double alpha = 0.0, steptol = 1e-12;
+    double lambda = 1.0, ydot;
+    *fail = 0;
+    P4G_TRY(R.dot(R.X, R.X, ydot, xnorm));
+    P4G_TRY(R.dot(R.Y, R.Y, ydot, ynorm));
+    double gnorm = 0.0;
+    if (s->ls_type == "basic") {
+        P4G_TRY(R.trial(R.X, R.Y, 1.0, R.W, R.G, gnorm));
+    } else {
+        if (ynorm == 0.0) { *fail = 1; return 0; }
+        // initial slope of |F|^2/2 along -Y: -(F, J Y)
+        P4G_TRY(R.fun(R.Y, R.W, false));
+        double initslope, wn;
+        P4G_TRY(R.dot(R.F, R.W, initslope, wn));
+        if (initslope > 0.0) initslope = -initslope;
+        if (initslope == 0.0) initslope = -1.0;
+        const double f = fnorm * fnorm;
+        P4G_TRY(R.trial(R.X, R.Y, lambda, R.W, R.G, gnorm));
+        double g = gnorm * gnorm;
+        bool ok = 0.5 * g <= 0.5 * f + lambda * alpha * initslope;
+        if (!ok) {
+            // rellength = max |Y_i| / max(|X_i|, 1), on the host
+            std::vector<double> hx(R.n), hy(R.n);
+            HIP_TRY(hipStreamSynchronize(R.st));
+            HIP_TRY(hipMemcpy(hx.data(), R.X, R.n * sizeof(double), hipMemcpyDeviceToHost));
+            HIP_TRY(hipMemcpy(hy.data(), R.Y, R.n * sizeof(double), hipMemcpyDeviceToHost));
+            double rellength = 0.0;
+            for (long q = 0; q < R.n; ++q)
+                rellength = std::max(rellength, std::fabs(hy[q]) / std::max(std::fabs(hx[q]), 1.0));
+            const double minlambda = steptol / rellength;
+            // quadratic model
+            double lt = -initslope / (g - f - 2.0 * lambda * initslope);
+            double lprev = lambda, gprev = g;
+            if (lt > 0.5 * lambda) lt = 0.5 * lambda;
+            lambda = lt <= 0.1 * lambda ? 0.1 * lambda : lt;
+            P4G_TRY(R.trial(R.X, R.Y, lambda, R.W, R.G, gnorm));
+            g = gnorm * gnorm;
+            ok = 0.5 * g < 0.5 * f + lambda * alpha * initslope;
+            // cubic model
+            while (!ok) {
+                if (lambda <= minlambda) break;
+                const double t1 = 0.5 * (g - f) - lambda * initslope;
+                const double t2 = 0.5 * (gprev - f) - lprev * initslope;
+                const double a = (t1 / (lambda * lambda) - t2 / (lprev * lprev)) / (lambda - lprev);
+                const double b = (-lprev * t1 / (lambda * lambda) + lambda * t2 / (lprev * lprev)) / (lambda - lprev);
+                double d = b * b - 3.0 * a * initslope;
+                if (d < 0.0) d = 0.0;
+                lt = (a == 0.0) ? -initslope / (2.0 * b) : (-b + std::sqrt(d)) / (3.0 * a);
+                lprev = lambda;
+                gprev = g;
+                if (lt > 0.5 * lambda) lt = 0.5 * lambda;
+                lambda = lt <= 0.1 * lambda ? 0.1 * lambda : lt;
+                P4G_TRY(R.trial(R.X, R.Y, lambda, R.W, R.G, gnorm));
+                g = gnorm * gnorm;
+                ok = 0.5 * g <= 0.5 * f + lambda * alpha * initslope;
+            }
+        }
+        *fail = ok ? 0 : 1;
+    }
+    std::swap(R.X, R.W);
+    std::swap(R.F, R.G);
+    fnorm = gnorm;
+    double xx;
+    P4G_TRY(R.dot(R.X, R.X, xx, xnorm));
+    ynorm *= lambda;
+    return 0;
+}
+
+static int vi_solve(p4g_solver_s* s, const double* xl, const double* xu, double* x, double* Fout,
+                    p4g_VIReport* rep) {
+    const double t0 = now_ms();
+    p4g_da_s* da = s->da;
+    if (!da->fn) return set_error(PETSC_ERR_ARG_WRONG, "no residual callback: call p4g_da_set_function_local");
+    if (!s->snes_type.empty() && s->snes_type != "vinewtonrsls")
+        return set_error(PETSC_ERR_SUP, "-snes_type %s: p4g_vi_solve implements vinewtonrsls", s->snes_type.c_str());
+    if (s->ls_type != "bt" && s->ls_type != "basic")
+        return set_error(PETSC_ERR_SUP, "-snes_linesearch_type %s is not implemented (bt, basic)", s->ls_type.c_str());
+    if (s->mixed) return set_error(PETSC_ERR_SUP, "p4g_vi_solve: mixed precision is not implemented");
+    if (!da->kcell.empty()) return set_error(PETSC_ERR_SUP, "p4g_vi_solve: variable coefficients are not implemented");
+    if (s->fmg) return set_error(PETSC_ERR_SUP, "p4g_vi_solve: -pc_mg_type full is not implemented");
+    if (!s->vi_mode) {
+        if (s->ready)
+            return set_error(PETSC_ERR_ORDER, "p4g_vi_solve: the solver was set up for linear solves; create a "
+                                              "new solver for VI solves");
+        s->vi_mode = true;
+        if (s->pctype == PCT_MG) s->galerkin = true;   // PCMGSetGalerkin under RSLS
+    }
+    P4G_TRY(solver_setup(s));
+    const int L = s->nlev - 1;
+    const LevelInfo& FL = s->lv[L];
+    const long n = FL.n;
+    ViRun R;
+    R.s = s;
+    R.n = n;
+    R.nb = vblocks(n);
+    R.st = s->cx.stream;
+    R.S = s->cx.S + 16;
+    R.g.dim = FL.dim; R.g.mx = FL.mx; R.g.my = FL.my; R.g.mz = FL.mz; R.g.n = n;
+    R.g.d = FL.d; R.g.db = FL.db; R.g.sx = FL.sx; R.g.sy = FL.sy; R.g.sz = FL.sz;
+    if (!s->vi_buf) {
+        P4G_TRY(dalloc(s, (void**)&s->vi_buf, 9 * n * sizeof(double)));
+        P4G_TRY(dalloc(s, (void**)&s->vi_act, n));
+        P4G_TRY(dalloc(s, (void**)&s->vi_g27, 27 * n * sizeof(double)));
+    }
+    double* b = s->vi_buf;
+    R.F0 = b; R.X = b + n; R.F = b + 2 * n; R.Y = b + 3 * n; R.W = b + 4 * n; R.G = b + 5 * n;
+    R.Fm = b + 6 * n; R.xl = b + 7 * n; R.xu = xu ? b + 8 * n : nullptr;
+    {   // F(0) by the residual callback: F(x) = J x + F(0) for this linear residual
+        std::vector<double> z(n, 0.0), F0(n, 0.0);
+        std::vector<double*> r1, r2;
+        std::vector<double**> p1, p2;
+        void* au = make_view(da, z.data(), r1, p1);
+        void* aF = make_view(da, F0.data(), r2, p2);
+        DMDALocalInfo info;
+        info_of(da, &info);
+        P4G_TRY(da->fn(&info, au, aF, da->fnctx));
+        HIP_TRY(hipMemcpy(R.F0, F0.data(), n * sizeof(double), hipMemcpyHostToDevice));
+    }
+    HIP_TRY(hipMemcpy(R.xl, xl, n * sizeof(double), hipMemcpyHostToDevice));
+    if (xu) HIP_TRY(hipMemcpy(R.xu, xu, n * sizeof(double), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(R.W, x, n * sizeof(double), hipMemcpyHostToDevice));
+    s->vi_fhist.clear();
+    s->vi_kits.clear();
+    s->vi_kreason.clear();
+    // SNESVIProjectOntoBounds, F(X), the inactive-set norm
+    k_vi_trial<<<R.nb, 256, 0, R.st>>>(n, R.W, nullptr, 0.0, R.xl, R.xu, R.X);
+    LAUNCH_CHECK();
+    P4G_TRY(R.fun(R.X, R.F, true));
+    double fnorm, xnorm, ynorm = 0.0, ttol = 0.0, rnorm0 = 0.0;
+    P4G_TRY(R.norms(R.X, R.F, fnorm, xnorm));
+    p4g_VIReport out;
+    std::memset(&out, 0, sizeof out);
+    out.fnorm0 = fnorm;
+    s->vi_fhist.push_back(fnorm);
+    int reason = vi_converged(s, 0, 0.0, 0.0, fnorm, ttol, rnorm0);
+    int i = 0, iter = 0;
+    for (; !reason && i < s->snes_maxit; ++i) {
+        // active set, reduced right-hand side, reduced operator
+        k_vi_sets<<<R.nb, 256, 0, R.st>>>(n, R.X, R.F, R.xl, R.xu, s->vi_act, R.Fm, s->cx.part);
+        LAUNCH_CHECK();
+        P4G_TRY(vi_operator(s, R));
+        typename Engine<double>::CGOut co;
+        P4G_TRY(s->e64->cg(R.Fm, R.Y, s->normtype, s->rtol, s->atol, s->dtol, s->maxit, co, s->cx.S));
+        HIP_TRY(hipStreamSynchronize(R.st));
+        s->vi_kits.push_back(co.its);
+        s->vi_kreason.push_back(co.reason);
+        s->hist = co.hist;
+        out.last_ksp_its = co.its;
+        out.ksp_its_total += co.its;
+        if (co.reason < 0) { reason = -3; break; }   // DIVERGED_LINEAR_SOLVE (maxLinearSolveFailures 1)
+        int fail = 0;
+        P4G_TRY(vi_line_search(s, R, fnorm, xnorm, ynorm, &fail));
+        if (fail) {                                  // maxFailures 1
+            reason = (s->snes_stol * xnorm > ynorm) ? 4 : -6;
+            break;
+        }
+        iter = i + 1;
+        s->vi_fhist.push_back(fnorm);
+        reason = vi_converged(s, iter, xnorm, ynorm, fnorm, ttol, rnorm0);
+    }
+    if (!reason && i >= s->snes_maxit) reason = -5;   // DIVERGED_MAX_IT
+    // the size of the last active set
+    {
+        std::vector<unsigned char> act(n);
+        HIP_TRY(hipStreamSynchronize(R.st));
+        if (!s->vi_kits.empty()) {
+            HIP_TRY(hipMemcpy(act.data(), s->vi_act, n, hipMemcpyDeviceToHost));
+            for (long q = 0; q < n; ++q) out.active += act[q];
+        }
+    }
+    HIP_TRY(hipMemcpy(x, R.X, n * sizeof(double), hipMemcpyDeviceToHost));
+    if (Fout) HIP_TRY(hipMemcpy(Fout, R.F, n * sizeof(double), hipMemcpyDeviceToHost));
+    s->cx.tm.collect();
+    out.its = iter;
+    out.reason = reason;
+    out.fnorm = fnorm;
+    out.solve_ms = now_ms() - t0;
+    if (rep) *rep = out;
+    return 0;
+}
+
+}  // namespace p4g
+
+extern "C" {
+
+int p4g_vi_solve(p4g_Solver s, const double* xl, const double* xu, double* x, double* F, p4g_VIReport* rep) {
+    clear_error();
+    if (!s || !xl || !x) return set_error(PETSC_ERR_ARG_WRONG, "p4g_vi_solve: null argument");
+    return p4g::vi_solve(s, xl, xu, x, F, rep);
+}
+
+int p4g_vi_get_history(p4g_Solver s, double* fnorm, int* ksp_its, int* ksp_reason, int cap, int* nf, int* nk) {
+    clear_error();
+    if (!s || !nf || !nk) return set_error(PETSC_ERR_ARG_WRONG, "p4g_vi_get_history: null argument");
+    *nf = (int)s->vi_fhist.size();
+    *nk = (int)s->vi_kits.size();
+    for (int q = 0; q < cap && q < *nf; ++q)
+        if (fnorm) fnorm[q] = s->vi_fhist[q];
+    for (int q = 0; q < cap && q < *nk; ++q) {
+        if (ksp_its) ksp_its[q] = s->vi_kits[q];
+        if (ksp_reason) ksp_reason[q] = s->vi_kreason[q];
+    }
+    return 0;
+}
+
+}  // extern "C"

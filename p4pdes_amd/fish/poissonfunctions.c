@@ -1,8 +1,8 @@
 /*
  * poissonfunctions.c -- residual/Jacobian callbacks (see poissonfunctions.h).
  *
- * Discretization of c/ch6/poissonfunctions.c:4-258: vertex grid with
- * spacing h = L/(m-1); rows scaled by the cell measure (1D: 1/h, 2D: the
+ * Discretization of c/ch6/poissonfunctions.c:4-258: vertex grid over the
+ * DM's bounding box, x_i = xmin + i h with h = (xmax-xmin)/(m-1); rows scaled by the cell measure (1D: 1/h, 2D: the
  * hx/hy ratios, 3D: dvol/h^2), so the diagonal is constant, boundary rows
  * included:  F = scdiag (u - g) on the boundary; in the interior
  * F = scdiag u - sum_dir sc_dir (u_- + u_+) - measure f, where a neighbour
@@ -14,6 +14,7 @@
 
 typedef struct {
     double h[3];
+    double lo[3];     /* lower corner of the box        */
     double sc[3];     /* coupling per direction        */
     double scdiag;    /* diagonal of every row          */
     double measure;   /* multiplies f in the residual   */
@@ -25,6 +26,7 @@ static PetscErrorCode scales(DMDALocalInfo *info, PoissonCtx *user, Scales *s) {
     const int m[3] = {info->mx, info->my, info->mz};
     const double c[3] = {user->cx, user->cy, user->cz};
     for (int d = 0; d < 3; d++) s->h[d] = d < info->dim ? (hi[d] - lo[d]) / (m[d] - 1) : 1.0;
+    for (int d = 0; d < 3; d++) s->lo[d] = d < info->dim ? lo[d] : 0.0;
     if (info->dim == 1) {
         s->sc[0] = c[0] / s->h[0];
         s->sc[1] = s->sc[2] = 0.0;
@@ -54,7 +56,7 @@ PetscErrorCode Poisson1DFunctionLocal(DMDALocalInfo *info, PetscReal *au, PetscR
     PetscCall(scales(info, user, &s));
     const double h = s.h[0];
     for (int i = info->xs; i < info->xs + info->xm; i++) {
-        const double x = i * h;
+        const double x = s.lo[0] + i * h;
         if (i == 0 || i == info->mx - 1) {
             aF[i] = s.scdiag * (au[i] - user->g_bdry(x, 0.0, 0.0, user));
             continue;
@@ -73,9 +75,9 @@ PetscErrorCode Poisson2DFunctionLocal(DMDALocalInfo *info, PetscReal **au, Petsc
     PetscCall(scales(info, user, &s));
     const double hx = s.h[0], hy = s.h[1];
     for (int j = info->ys; j < info->ys + info->ym; j++) {
-        const double y = j * hy;
+        const double y = s.lo[1] + j * hy;
         for (int i = info->xs; i < info->xs + info->xm; i++) {
-            const double x = i * hx;
+            const double x = s.lo[0] + i * hx;
             if (i == 0 || i == info->mx - 1 || j == 0 || j == info->my - 1) {
                 aF[j][i] = s.scdiag * (au[j][i] - user->g_bdry(x, y, 0.0, user));
                 continue;
@@ -98,11 +100,11 @@ PetscErrorCode Poisson3DFunctionLocal(DMDALocalInfo *info, PetscReal ***au, Pets
     PetscCall(scales(info, user, &s));
     const double hx = s.h[0], hy = s.h[1], hz = s.h[2];
     for (int k = info->zs; k < info->zs + info->zm; k++) {
-        const double z = k * hz;
+        const double z = s.lo[2] + k * hz;
         for (int j = info->ys; j < info->ys + info->ym; j++) {
-            const double y = j * hy;
+            const double y = s.lo[1] + j * hy;
             for (int i = info->xs; i < info->xs + info->xm; i++) {
-                const double x = i * hx;
+                const double x = s.lo[0] + i * hx;
                 if (i == 0 || i == info->mx - 1 || j == 0 || j == info->my - 1 || k == 0 ||
                     k == info->mz - 1) {
                     aF[k][j][i] = s.scdiag * (au[k][j][i] - user->g_bdry(x, y, z, user));
@@ -215,7 +217,8 @@ PetscErrorCode InitialState(DM da, InitialType it, PetscBool gbdry, Vec u, Poiss
                 if (info.dim > 2) b = b || k == 0 || k == mz - 1;
                 if (b)
                     a[((long)k * my + j) * mx + i] =
-                        user->g_bdry(i * h[0], info.dim > 1 ? j * h[1] : 0.0, info.dim > 2 ? k * h[2] : 0.0, user);
+                        user->g_bdry(lo[0] + i * h[0], info.dim > 1 ? lo[1] + j * h[1] : 0.0,
+                                     info.dim > 2 ? lo[2] + k * h[2] : 0.0, user);
             }
     PetscCall(VecRestoreArray(u, &a));
     return 0;
@@ -236,7 +239,7 @@ PetscErrorCode VarPoisson3DFunctionLocal(DMDALocalInfo *info, PetscReal ***au, P
     for (int k = info->zs; k < info->zs + info->zm; k++)
         for (int j = info->ys; j < info->ys + info->ym; j++)
             for (int i = info->xs; i < info->xs + info->xm; i++) {
-                const double x = i * hx, y = j * hy, z = k * hz;
+                const double x = s.lo[0] + i * hx, y = s.lo[1] + j * hy, z = s.lo[2] + k * hz;
                 if (i == 0 || i == info->mx - 1 || j == 0 || j == info->my - 1 || k == 0 ||
                     k == info->mz - 1) {
                     double sum = 0.0;
