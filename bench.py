@@ -234,6 +234,41 @@ def profile_workload():
     print("profile workload done", flush=True)
 
 
+OBSTACLE = "-da_refine 2 -snes_grid_sequence 8 -pc_type mg -mg_levels_pc_type jacobi"
+
+
+def obstacle(with_oracle):
+    """The obstacle problem from 9x9 through 8 grid refinements to 2049x2049:
+    reduced-space VI Newton per grid, CG + Galerkin V(2,2) Chebyshev/Jacobi on
+    the inactive sets.  Host wall time of the whole sequence (set-up, callbacks
+    and transfers included; the iterate crosses PCIe once per grid), and the
+    oracle's time for the same run."""
+    from p4pdes_amd import capi
+    capi.obstacle_run(OBSTACLE.replace("-snes_grid_sequence 8", "-snes_grid_sequence 3"))   # warm-up
+    walls = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        rc, text, res, _ = capi.obstacle_run(OBSTACLE)
+        walls.append(time.perf_counter() - t0)
+        if rc:
+            return {"error": text}
+    wall = sorted(walls)[1]
+    dof = res.mx * res.my
+    o = {"workload": "obstacle problem of c/ch12/obstacle.c, grid-sequenced 9^2 -> 2049^2 (%d DOF on the "
+                     "final grid), SNESVINEWTONRSLS per grid, CG + Galerkin multigrid on the inactive sets "
+                     "(V(2,2) Chebyshev/Jacobi)" % dof,
+         "options": OBSTACLE, "wall_s": wall, "dof_per_s": dof / wall, "stage_its": res.as_dict()["stage_its"],
+         "ksp_its_final_grid": res.ksp_its_total, "report": text.splitlines()[-2:]}
+    if with_oracle:
+        from oracle import oracle_py
+        t0 = time.perf_counter()
+        rco, texto, reso, _ = oracle_py.obstacle_run(OBSTACLE)
+        o["oracle_wall_s"] = time.perf_counter() - t0
+        o["oracle_threads"] = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        o["oracle_report_equal"] = texto.splitlines()[-2:] == o["report"]
+    return o
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -242,6 +277,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-513", action="store_true", help="skip the 513^3 mixed/FP64 runs")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving stream")
+    ap.add_argument("--no-obstacle", action="store_true", help="skip the obstacle (VI) solve")
     ap.add_argument("--profile-workload", action="store_true",
                     help="only run the counter-collection workload: two 257^3 solves and a 1 GiB "
                          "device copy (the calibration copy of known size); prints no JSON")
@@ -368,6 +404,11 @@ def main():
     if not a.no_serve:
         print("bench: serving stream", file=sys.stderr, flush=True)
         out["serving"] = serving()
+
+    # ---- obstacle problem (c/ch12/obstacle.c), grid-sequenced to 2049^2 ---
+    if not a.no_obstacle:
+        print("bench: obstacle 2049^2", file=sys.stderr, flush=True)
+        out["obstacle_2049"] = obstacle(not a.no_cpu)
 
     # ---- CPU baseline ------------------------------------------------------
     if not a.no_cpu:

@@ -216,7 +216,8 @@ typedef struct {
     int    ksp_its_total;
     long   active;         /* size of the last active set                      */
     double fnorm0, fnorm;  /* |F| on the inactive set, first and last          */
-    double solve_ms;       /* host wall time of the solve                      */
+    double setup_ms;       /* the solver's set-up (levels from the callbacks), first call */
+    double solve_ms;       /* host wall time of the solve without the set-up   */
 } p4g_VIReport;
 int p4g_vi_solve(p4g_Solver s, const double *xl, const double *xu, double *x, double *F,
                  p4g_VIReport *rep);

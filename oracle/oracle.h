@@ -68,7 +68,7 @@ typedef struct {
     int    last_ksp_its;     /* KSPGetIterationNumber after the last solve */
     int    ksp_its_total;    /* final grid                                 */
     int    stages;           /* 1 + -snes_grid_sequence                    */
-    int    stage_its[8];     /* SNES iterations per grid                   */
+    int    stage_its[16];    /* SNES iterations per grid (first 16)      */
     long   active;           /* GetActiveSet count (obstacle.c:201-232)    */
     double fnorm0, fnorm;    /* inactive-set |F|, final grid               */
     double error1, errorinf, areaerr;   /* obstacle.c:180-196             */

@@ -593,6 +593,10 @@ int oracle_obstacle_run(int argc, const char **argv, obstacle_result *res, doubl
     /* grid (obstacle.c:86-95): 3x3 base, -da_grid_x/y, -da_refine */
     int mx = oopts_int(&o, "da_grid_x", 3), my = oopts_int(&o, "da_grid_y", 3);
     const int refine = oopts_int(&o, "da_refine", 0), seq = oopts_int(&o, "snes_grid_sequence", 0);
+    if (seq < 0 || seq > 12) {
+        oprintf(out, outlen, &pos, "[0]PETSC ERROR: -snes_grid_sequence %d out of range\n", seq);
+        return 63;
+    }
     for (int r = 0; r < refine; r++) { mx = 2 * (mx - 1) + 1; my = 2 * (my - 1) + 1; }
     const int lev_opt = oopts_int(&o, "pc_mg_levels", -1);
     ogrid g;
@@ -610,7 +614,7 @@ int oracle_obstacle_run(int argc, const char **argv, obstacle_result *res, doubl
         const int ind = 2 * (seq - stage);
         e = rsls(&c, &g, X, F, xl, NULL, ind, &sg, out, outlen, &pos, err, sizeof err);
         if (e) { free(xl); free(F); break; }
-        if (stage < 8) res->stage_its[stage] = sg.its;
+        if (stage < 16) res->stage_its[stage] = sg.its;
         if (snes_reason) {
             if (sg.reason > 0)
                 oprintf(out, outlen, &pos, "%*s  Nonlinear solve converged due to %s iterations %d\n", ind, "",

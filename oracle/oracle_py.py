@@ -172,7 +172,7 @@ class ObstacleResult(ctypes.Structure):
     """Mirror of obstacle_result in oracle/oracle.h."""
     _fields_ = [(n, ctypes.c_int) for n in
                 ("mx", "my", "its", "reason", "last_ksp_its", "ksp_its_total", "stages")] + \
-               [("stage_its", ctypes.c_int * 8), ("active", ctypes.c_long)] + \
+               [("stage_its", ctypes.c_int * 16), ("active", ctypes.c_long)] + \
                [(n, ctypes.c_double) for n in
                 ("fnorm0", "fnorm", "error1", "errorinf", "areaerr", "setup_ms", "solve_ms")]
 

@@ -54,7 +54,7 @@ class VIReport(ctypes.Structure):
     """p4g_VIReport of include/p4g.h."""
     _fields_ = [(n, ctypes.c_int) for n in ("its", "reason", "last_ksp_its", "ksp_its_total")] + \
                [("active", ctypes.c_long)] + \
-               [(n, ctypes.c_double) for n in ("fnorm0", "fnorm", "solve_ms")]
+               [(n, ctypes.c_double) for n in ("fnorm0", "fnorm", "setup_ms", "solve_ms")]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}
@@ -64,7 +64,7 @@ class ObstacleResult(ctypes.Structure):
     """obstacle_result (same layout as the oracle's)."""
     _fields_ = [(n, ctypes.c_int) for n in
                 ("mx", "my", "its", "reason", "last_ksp_its", "ksp_its_total", "stages")] + \
-               [("stage_its", ctypes.c_int * 8), ("active", ctypes.c_long)] + \
+               [("stage_its", ctypes.c_int * 16), ("active", ctypes.c_long)] + \
                [(n, ctypes.c_double) for n in
                 ("fnorm0", "fnorm", "error1", "errorinf", "areaerr", "setup_ms", "solve_ms")]
 

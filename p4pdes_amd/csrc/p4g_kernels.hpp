@@ -1921,7 +1921,15 @@ k_vi_nullrow27(ViGrid g, const unsigned char* __restrict__ act, double* __restri
                 }
             }
         }
-        for (int t = 0; t < 27; ++t) g27[(long)t * g.n + q] = c[t];
+        // only the planes of in-grid offsets are ever read (9 in 2D, 3 in 1D)
+        const int z0 = g.dim == 3 ? 0 : 1, z1 = g.dim == 3 ? 2 : 1;
+        const int y0 = g.dim >= 2 ? 0 : 1, y1 = g.dim >= 2 ? 2 : 1;
+        for (int oz = z0; oz <= z1; ++oz)
+            for (int oy = y0; oy <= y1; ++oy)
+                for (int ox = 0; ox < 3; ++ox) {
+                    const int t = 9 * oz + 3 * oy + ox;
+                    g27[(long)t * g.n + q] = c[t];
+                }
     }
 }
 

@@ -987,6 +987,8 @@ struct p4g_solver_s {
     int snes_maxit = 50;
     std::string snes_type, ls_type;
     bool vi_mode = false;                 // set up for p4g_vi_solve
+    double* est_scratch = nullptr;        // device: eigen-estimate vectors, est_cap doubles (VI solves)
+    long est_cap = 0;
     double* vi_buf = nullptr;             // device: F0 X F Y W G Fm xl xu, n doubles each
     unsigned char* vi_act = nullptr;      // device: active flags
     double* vi_g27 = nullptr;             // device: the finest null-row operator (27 planes)
@@ -1188,7 +1190,13 @@ static int estimate_level(p4g_solver_s* s, int l) {
     const long n = L.n;
     hipStream_t st = s->cx.stream;
     std::vector<void*> tmp;
+    long used = 0;   // slices of the solver's estimate scratch when it has one (VI solves)
     auto get = [&](double** p) -> int {
+        if (s->est_scratch && used + n <= s->est_cap) {
+            *p = s->est_scratch + used;
+            used += n;
+            return 0;
+        }
         void* q = nullptr;
         hipError_t e = hipMalloc(&q, n * sizeof(double));
         if (e != hipSuccess) return set_error(PETSC_ERR_MEM, "hipMalloc (estimate): %s", hipGetErrorString(e));
@@ -1516,7 +1524,13 @@ int solver_setup(p4g_solver_s* s) {
         HIP_TRY(hipMemcpy(s->cx.S, init, sizeof init, hipMemcpyHostToDevice));
         HIP_TRY(hipHostMalloc((void**)&s->cx.hS, 32 * sizeof(double), hipHostMallocDefault));
     }
-    if (s->pctype == PCT_MG) {
+    if (s->pctype == PCT_MG && s->vi_mode) {
+        // VI solves form the operators, coarse factor and bounds per Newton
+        // step from the active set; here only the sizes
+        s->g27.assign(s->nlev, nullptr);
+        s->n0 = (int)s->lv[0].n;
+        s->Ainv.assign((size_t)s->n0 * s->n0, 0.0);
+    } else if (s->pctype == PCT_MG) {
         P4G_TRY(galerkin_levels(s));
         P4G_TRY(coarse_factor(s->lv[0], s->g27[0], s->Ainv));
         s->n0 = (int)s->lv[0].n;

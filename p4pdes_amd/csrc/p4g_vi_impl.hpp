@@ -194,7 +194,10 @@ static int vi_solve(p4g_solver_s* s, const double* xl, const double* xu, double*
         s->vi_mode = true;
         if (s->pctype == PCT_MG) s->galerkin = true;   // PCMGSetGalerkin under RSLS
     }
+    const double ts = now_ms();
+    const bool fresh = !s->ready;
     P4G_TRY(solver_setup(s));
+    const double setup_ms = fresh ? now_ms() - ts : 0.0;
     const int L = s->nlev - 1;
     const LevelInfo& FL = s->lv[L];
     const long n = FL.n;
@@ -210,6 +213,12 @@ static int vi_solve(p4g_solver_s* s, const double* xl, const double* xu, double*
         P4G_TRY(dalloc(s, (void**)&s->vi_buf, 9 * n * sizeof(double)));
         P4G_TRY(dalloc(s, (void**)&s->vi_act, n));
         P4G_TRY(dalloc(s, (void**)&s->vi_g27, 27 * n * sizeof(double)));
+        // the Chebyshev bounds are re-estimated every Newton step: keep their
+        // vectors (6 of the finest size) instead of allocating per level
+        if (s->pctype == PCT_MG && s->smoother == SM_CHEB) {
+            s->est_cap = 6 * n;
+            P4G_TRY(dalloc(s, (void**)&s->est_scratch, s->est_cap * sizeof(double)));
+        }
     }
     double* b = s->vi_buf;
     R.F0 = b; R.X = b + n; R.F = b + 2 * n; R.Y = b + 3 * n; R.W = b + 4 * n; R.G = b + 5 * n;
@@ -283,7 +292,8 @@ static int vi_solve(p4g_solver_s* s, const double* xl, const double* xu, double*
     out.its = iter;
     out.reason = reason;
     out.fnorm = fnorm;
-    out.solve_ms = now_ms() - t0;
+    out.setup_ms = setup_ms;
+    out.solve_ms = now_ms() - t0 - setup_ms;
     if (rep) *rep = out;
     return 0;
 }

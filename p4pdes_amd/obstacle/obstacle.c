@@ -156,7 +156,7 @@ int obstacle_run(int argc, const char **argv, obstacle_result *res, double *u_ou
     memset(res, 0, sizeof *res);
     Args a = {argc, argv};
     const int refine = opt_int(&a, "da_refine", 0), seq = opt_int(&a, "snes_grid_sequence", 0);
-    if (seq < 0 || seq > 7) {
+    if (seq < 0 || seq > 12) {
         outf(&o, "[0]PETSC ERROR: -snes_grid_sequence %d out of range\n", seq);
         return PETSC_ERR_ARG_OUTOFRANGE;
     }
@@ -229,7 +229,7 @@ int obstacle_run(int argc, const char **argv, obstacle_result *res, double *u_ou
                     if (monitor && q + 1 < nh) snes_line(&o, ind, q + 1, fh[q + 1]);
                 }
                 free(fh); free(ki); free(kr);
-                if (stage < 8) res->stage_its[stage] = rep.its;
+                if (stage < 16) res->stage_its[stage] = rep.its;
                 if (sreason) {
                     if (rep.reason > 0)
                         outf(&o, "%*s  Nonlinear solve converged due to %s iterations %d\n", ind, "",
@@ -238,6 +238,7 @@ int obstacle_run(int argc, const char **argv, obstacle_result *res, double *u_ou
                         outf(&o, "%*s  Nonlinear solve did not converge due to %s iterations %d\n", ind, "",
                              snes_reason_name(rep.reason), rep.its);
                 }
+                res->setup_ms += rep.setup_ms;
                 res->solve_ms += rep.solve_ms;
                 if (stage == seq) {   /* report (obstacle.c:160-198) */
                     res->mx = mx; res->my = my;

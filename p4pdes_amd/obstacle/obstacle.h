@@ -18,7 +18,7 @@ extern "C" {
 typedef struct {
     int    mx, my;
     int    its, reason, last_ksp_its, ksp_its_total, stages;
-    int    stage_its[8];
+    int    stage_its[16];
     long   active;
     double fnorm0, fnorm, error1, errorinf, areaerr;
     double setup_ms, solve_ms;
