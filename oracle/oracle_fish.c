@@ -127,7 +127,11 @@ static int session_setup(osession *s, int argc, const char **argv, char *out, in
     k->maxit = oopts_int(o, "ksp_max_it", 10000);
     k->rich_scale = oopts_real(o, "ksp_richardson_scale", 1.0);
     const char *nt = oopts_get(o, "ksp_norm_type");
-    if (ktype == KSP_RICHARDSON || ktype == KSP_CHEBYSHEV) k->normtype = NORM_UNPRECONDITIONED;
+    /* KSPRICHARDSON keeps CG's preconditioned default (the reference's study
+     * scripts always pass -ksp_norm_type unpreconditioned, so the default is
+     * unpinned); the outer Chebyshev loop here computes no norms */
+    if (ktype == KSP_CHEBYSHEV) k->normtype = NORM_UNPRECONDITIONED;
+    if (ktype == KSP_RICHARDSON) k->normtype = NORM_PRECONDITIONED;
     if (nt) {
         if (!strcmp(nt, "preconditioned")) k->normtype = NORM_PRECONDITIONED;
         else if (!strcmp(nt, "unpreconditioned")) k->normtype = NORM_UNPRECONDITIONED;

@@ -158,6 +158,10 @@ void oksp_estimate_eigs(oksp *k);
 struct omg {
     int     nlev, cycles, galerkin;
     int     full;               /* -pc_mg_type full: one F-cycle per apply */
+    int     kaskade;            /* -pc_mg_type kaskade                     */
+    int     mcycles;            /* -pc_mg_multiplicative_cycles            */
+    double  ttol, atol;         /* PCApplyRichardson_MG: stop test after the */
+    int    *rreason;            /*   finest pre-smoothing (NULL: off)     */
     ogrid  *g;                  /* g[0] coarsest ... g[nlev-1] finest   */
     ocsr   *A;                  /* per level operators                  */
     oksp   *smooth;             /* per level smoother (0: coarse solve) */

@@ -484,12 +484,81 @@ static int rich_solve(oksp *k, const double *b, double *x) {
     return k->maxit;
 }
 
+static void mcycle(omg *mg, int l);
+
+/* PCApplyRichardson_MG: up to maxit multiplicative cycles from x = 0, each
+ * stopping after the finest pre-smoothing once |b - A x| <= ttol, ttol =
+ * max(rtol |b|, atol); returns the number of completed cycles */
+static int rich_mg(oksp *k, const double *b, double *x, int *reason) {
+    omg *mg = k->pc.mg;
+    const int L = mg->nlev - 1;
+    const long n = k->A->n;
+    double ttol = 0.0;
+    if (k->rtol > 0.0) {
+        double rn = ovnorm2(n, b);
+        ttol = k->rtol * rn > k->atol ? k->rtol * rn : k->atol;
+    } else if (k->atol > 0.0) ttol = k->atol;
+    int rr = 0;
+    mg->ttol = ttol; mg->atol = k->atol; mg->rreason = &rr;
+    ovset(n, 0.0, x);
+    mg->b[L] = (double *)b; mg->x[L] = x;
+    int i = 0;
+    for (; i < k->maxit; i++) {
+        mcycle(mg, L);
+        if (rr) break;
+    }
+    mg->rreason = NULL; mg->b[L] = NULL; mg->x[L] = NULL;
+    *reason = rr ? rr : R_CONVERGED_ITS;
+    return i;
+}
+
+/* KSPSolve_Richardson, the generic loop from x = 0 (PETSc takes it when a
+ * monitor is set or the PC has no PCApplyRichardson): norm, monitor and
+ * stopping test of r (unpreconditioned) or B r (preconditioned), x += s B r,
+ * r = b - A x; after maxit updates one more norm and test */
+static int rich_outer(oksp *k, const double *b, double *x, int *reason, double *rnorm0, double *rnorm) {
+    if (k->pc.type == PC_MG && k->pc.mg && !k->monitor) return rich_mg(k, b, x, reason);
+    const ocsr *A = k->A;
+    const long n = A->n;
+    double *r = k->w[0], *z = k->w[1], ttol = 0.0, rn = 0.0;
+    ovset(n, 0.0, x);
+    ovcopy(n, b, r);
+    int i = 0, its = 0;
+    *reason = 0;
+    for (; i < k->maxit; i++) {
+        if (k->normtype == NORM_UNPRECONDITIONED) rn = ovnorm2(n, r);
+        else { opc_apply(&k->pc, r, z); rn = ovnorm2(n, z); }
+        *rnorm = rn;
+        ksp_monitor(k, i, rn);
+        *reason = converged(k, i, rn, rnorm0, &ttol);
+        if (*reason) break;
+        if (k->normtype == NORM_UNPRECONDITIONED) opc_apply(&k->pc, r, z);
+        for (long q = 0; q < n; q++) x[q] += k->rich_scale * z[q];
+        its++;
+        ocsr_resid(A, b, x, r);
+    }
+    if (!*reason) {
+        if (k->normtype == NORM_UNPRECONDITIONED) rn = ovnorm2(n, r);
+        else { opc_apply(&k->pc, r, z); rn = ovnorm2(n, z); }
+        *rnorm = rn;
+        ksp_monitor(k, i, rn);
+        if (its >= k->maxit) {
+            *reason = converged(k, i, rn, rnorm0, &ttol);
+            if (!*reason) *reason = R_DIVERGED_ITS;
+        }
+    }
+    return its;
+}
+
 int oksp_solve(oksp *k, const double *b, double *x, int *reason, double *rnorm0, double *rnorm) {
     *reason = 0; *rnorm0 = 0.0; *rnorm = 0.0;
     switch (k->type) {
     case KSP_CG: return cg_solve(k, b, x, reason, rnorm0, rnorm);
     case KSP_CHEBYSHEV: *reason = R_CONVERGED_ITS; return cheb_solve(k, b, x);
-    case KSP_RICHARDSON: *reason = R_CONVERGED_ITS; return rich_solve(k, b, x);
+    case KSP_RICHARDSON:
+        if (k->normtype != NORM_NONE) return rich_outer(k, b, x, reason, rnorm0, rnorm);
+        *reason = R_CONVERGED_ITS;
+        return rich_solve(k, b, x);
     case KSP_PREONLY: opc_apply(&k->pc, b, x); *reason = R_CONVERGED_ITS; return 1;
     }
     return 0;
@@ -595,7 +664,10 @@ int omg_setup(omg *mg, const ogrid *fine, const oprob *prob, const ocsr *Afine,
     mg->galerkin = oopts_bool(o, "pc_mg_galerkin", 0);
     const char *mgtype = oopts_get(o, "pc_mg_type");
     mg->full = mgtype && !strcmp(mgtype, "full");
-    if (mgtype && strcmp(mgtype, "multiplicative") && !mg->full) {
+    mg->kaskade = mgtype && !strcmp(mgtype, "kaskade");
+    mg->mcycles = oopts_int(o, "pc_mg_multiplicative_cycles", 1);
+    if (mg->mcycles < 1) mg->mcycles = 1;
+    if (mgtype && strcmp(mgtype, "multiplicative") && !mg->full && !mg->kaskade) {
         snprintf(err, errlen, "oracle: -pc_mg_type %s not supported", mgtype);
         return 56;
     }
@@ -705,6 +777,13 @@ static void mcycle(omg *mg, int l) {
     }
     oksp_solve(&mg->smooth[l], mg->b[l], mg->x[l], &reason, &r0, &rn);
     ocsr_resid(&mg->A[l], mg->b[l], mg->x[l], mg->r[l]);
+    if (l == mg->nlev - 1 && mg->rreason && mg->ttol > 0.0) {   /* PCApplyRichardson_MG */
+        const double rr = ovnorm2(mg->g[l].n, mg->r[l]);
+        if (rr <= mg->ttol) {
+            *mg->rreason = rr < mg->atol ? R_CONVERGED_ATOL : R_CONVERGED_RTOL;
+            return;
+        }
+    }
     oq1_restrict(&mg->g[l], &mg->g[l - 1], mg->r[l], mg->b[l - 1]);
     ovset(mg->g[l - 1].n, 0.0, mg->x[l - 1]);
     for (int c = 0; c < mg->cycles; c++) mcycle(mg, l - 1);
@@ -727,14 +806,35 @@ static void fcycle(omg *mg) {
     }
 }
 
-/* PCApply_MG: multiplicative: x = 0, one cycle; full: one F-cycle */
+/* Kaskade (PETSc's PCMGKCycle_Private): b restricted through every level,
+ * the coarsest solved from zero, then upward x_l = P x_{l-1} (overwritten)
+ * and one smoother solve on level l from that iterate (no coarse
+ * correction) */
+static void kcycle(omg *mg) {
+    int reason; double r0, rn;
+    int L = mg->nlev - 1;
+    for (int l = L; l > 0; l--) oq1_restrict(&mg->g[l], &mg->g[l - 1], mg->b[l], mg->b[l - 1]);
+    ovset(mg->g[0].n, 0.0, mg->x[0]);
+    mcycle(mg, 0);
+    for (int l = 1; l <= L; l++) {
+        ovset(mg->g[l].n, 0.0, mg->x[l]);
+        oq1_interp_add(&mg->g[l - 1], &mg->g[l], mg->x[l - 1], mg->x[l]);
+        oksp_solve(&mg->smooth[l], mg->b[l], mg->x[l], &reason, &r0, &rn);
+    }
+}
+
+/* PCApply_MG: x = 0, then multiplicative: -pc_mg_multiplicative_cycles
+ * cycles, each from the last iterate; full: one F-cycle; kaskade: one
+ * K-cycle */
 void omg_apply(omg *mg, const double *b, double *x) {
     int L = mg->nlev - 1;
     mg->b[L] = (double *)b;
     mg->x[L] = x;
     ovset(mg->g[L].n, 0.0, x);
     if (mg->full && L > 0) fcycle(mg);
-    else mcycle(mg, L);
+    else if (mg->kaskade && L > 0) kcycle(mg);
+    else
+        for (int c = 0; c < mg->mcycles; c++) mcycle(mg, L);
     mg->b[L] = NULL; mg->x[L] = NULL;
 }
 

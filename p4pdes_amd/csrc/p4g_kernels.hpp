@@ -1781,6 +1781,12 @@ k_finalize(long nparts, const double* __restrict__ part, double* __restrict__ S,
 }
 
 
+// y += a x
+template <typename T>
+__global__ void __launch_bounds__(256) k_axpy(long n, T a, const T* __restrict__ x, T* __restrict__ y) {
+    for (long q = blockIdx.x * 256L + threadIdx.x; q < n; q += (long)gridDim.x * 256L) y[q] += a * x[q];
+}
+
 // ------------------------------------------------ reduced-space VI Newton
 // (PETSc's SNESVINEWTONRSLS over the constant 3/5/7-point operator).  A
 // reduced Newton step is solved in full-size vectors: the inactive block

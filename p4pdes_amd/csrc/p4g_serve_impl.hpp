@@ -117,6 +117,11 @@ static int serve_supported(p4g_solver_s* s) {
             return set_error(PETSC_ERR_SUP, "serving supports constant-coefficient stencils");
     if (s->smoother != SM_CHEB && s->smoother != SM_RICH)
         return set_error(PETSC_ERR_SUP, "serving needs a Chebyshev or Richardson smoother");
+    if (s->galerkin || s->fmg || s->kaskade || s->mcycles != 1)
+        return set_error(PETSC_ERR_SUP, "serving runs rediscretized multiplicative V-cycles (no -pc_mg_galerkin, "
+                                        "-pc_mg_type full|kaskade or -pc_mg_multiplicative_cycles)");
+    if (s->ksp_rich || s->vi_mode)
+        return set_error(PETSC_ERR_SUP, "serving runs CG solves (no -ksp_type richardson or VI solver)");
     return 0;
 }
 

@@ -261,6 +261,9 @@ struct Engine {
     // zero at active vertices; null otherwise
     const double* vidiag = nullptr;
     bool vi = false;   // levels may become null-row Galerkin levels later (full temporaries)
+    int mcycles = 1;        // -pc_mg_multiplicative_cycles
+    bool kaskade = false;   // -pc_mg_type kaskade
+    bool full_temps = false;   // four smoother temporaries on every level
     // Krylov vectors on the finest level
     T *r = nullptr, *z = nullptr, *pa = nullptr, *pb = nullptr, *w = nullptr, *xbuf = nullptr,
       *bbuf = nullptr, *r2 = nullptr;
@@ -309,7 +312,7 @@ struct Engine {
                 if (l < nlev - 1) { P4G_TRY(alloc(&d.x, L.n)); P4G_TRY(alloc(&d.b, L.n)); }
                 if (l > 0) {
                     P4G_TRY(alloc(&d.r, L.n));
-                    const int nt = (!vi && nu == 1 && cycles == 1 && zg_fusable(L)) ? 1 : 4;
+                    const int nt = (!vi && !full_temps && nu == 1 && cycles == 1 && zg_fusable(L)) ? 1 : 4;
                     for (int q = 0; q < nt; ++q) P4G_TRY(alloc(&d.t[q], L.n));
                 }
             }
@@ -762,10 +765,65 @@ struct Engine {
         return 0;
     }
 
+    // PCMG type kaskade (PETSc's PCMGKCycle_Private): b restricted through
+    // every level, the coarsest solved from zero, then upward x_l = P x_{l-1}
+    // (overwritten) and one smoother solve on level l from that iterate
+    int kcycle(const T* b, T* x, bool dots, long* nb) {
+        P4G_TRY(pend_flush());
+        const int L = nlev - 1;
+        P4G_TRY(restrict_(L, b, dv[L - 1].b));
+        for (int l = L - 1; l > 0; --l) P4G_TRY(restrict_(l, dv[l].b, dv[l - 1].b));
+        P4G_TRY(coarse(dv[0].b, dv[0].x));
+        for (int l = 1; l <= L; ++l) {
+            const T* bl = l == L ? b : dv[l].b;
+            T* xl = l == L ? x : dv[l].x;
+            T* t = dv[l].t[0];
+            P4G_TRY(prolong(l, true, bl, T(0), dv[l - 1].x, t));   // 0 b + P x_{l-1}
+            P4G_TRY(smooth(l, bl, t, xl, l == L && dots, l == L ? nb : nullptr));
+        }
+        return 0;
+    }
+
+    // one cycle on the finest level that stops after the pre-smoothing when
+    // |b - A x| <= ttol (PCApplyRichardson_MG); *conv: the reason, or 0
+    int mcycle_check(const T* b, T* x, bool xzero, double ttol, double atol, double* Sblk, int* conv) {
+        const int l = nlev - 1;
+        const LevelInfo& L = (*lv)[l];
+        Dev& d = dv[l];
+        Dev& c = dv[l - 1];
+        const bool jac = jacobi != 0;
+        hipStream_t st = cx->stream;
+        *conv = 0;
+        T* xpre = d.t[3];
+        if (!xpre) return set_error(PETSC_ERR_LIB, "internal: no smoother temporaries");
+        P4G_TRY(smooth(l, b, xzero ? nullptr : x, xpre, false, nullptr));
+        long nbr = 0;
+        P4G_TRY(stencil(l, SrcArr<T>{xpre}, OutResid<T, 1>{b, d.r}, P4G_K_RESID_ZG, 3, &nbr, jac));
+        P4G_TRY(cx->finalize(nbr, Sblk, FIN_RAW, 0));
+        P4G_TRY(cx->readS(cx->hS));
+        const double rn = std::sqrt(cx->hS[(Sblk - cx->S) + S_TMP0]);
+        if (ttol > 0.0 && rn <= ttol) {
+            HIP_TRY(hipMemcpyAsync(x, xpre, L.n * sizeof(T), hipMemcpyDeviceToDevice, st));
+            *conv = rn < atol ? P4G_CONVERGED_ATOL : P4G_CONVERGED_RTOL;
+            return 0;
+        }
+        P4G_TRY(restrict_(l, d.r, c.b));
+        for (int q = 0; q < cycles; ++q) P4G_TRY(mcycle(l - 1, c.b, c.x, q == 0, false, nullptr));
+        P4G_TRY(prolong(l, false, xpre, T(0), c.x, xpre));
+        return smooth(l, b, xpre, x, false, nullptr);
+    }
+
     // z = B r with fused (z,r), (z,z) partials; returns the block count
     int pc_apply_dots(const T* rr, T* zz, long* nb) {
         const LevelInfo& F = (*lv)[nlev - 1];
         if (pctype == PCT_MG && nlev > 1 && fmg) return fcycle(rr, zz, true, nb);
+        if (pctype == PCT_MG && nlev > 1 && kaskade) return kcycle(rr, zz, true, nb);
+        if (pctype == PCT_MG && nlev > 1 && mcycles > 1) {   // each cycle from the last iterate
+            P4G_TRY(pend_flush());
+            for (int c = 0; c < mcycles; ++c)
+                P4G_TRY(mcycle(nlev - 1, rr, zz, c == 0, c == mcycles - 1, c == mcycles - 1 ? nb : nullptr));
+            return 0;
+        }
         if (pctype == PCT_MG && nlev > 1) {
             P4G_TRY(mcycle(nlev - 1, rr, zz, true, true, nb));
             if (pend.on) return set_error(PETSC_ERR_LIB, "internal: pending residual update not consumed");
@@ -884,6 +942,97 @@ struct Engine {
         if (i >= maxit && !o.reason) o.reason = P4G_DIVERGED_ITS;
         return 0;
     }
+
+    // KSPSolve_Richardson from x = 0.  monitor: PETSc's generic loop (taken
+    // when a monitor is set, or the PC is not multigrid): the norm of r
+    // (unpreconditioned) or B r (preconditioned) is monitored and tested,
+    // then x += scale B r, r = b - A x; after maxit updates one more norm and
+    // test.  Otherwise, with multigrid, PCApplyRichardson_MG: up to maxit
+    // cycles, each stopping after the finest pre-smoothing once
+    // |b - A x| <= max(rtol |b|, atol); its = the completed cycles.
+    int richardson(const T* b, T* x, int normtype, double rtol, double atol, double dtol, int maxit,
+                   double scale, bool monitor, CGOut& o, double* Sblk) {
+        const LevelInfo& F = (*lv)[nlev - 1];
+        const long n = F.n;
+        hipStream_t st = cx->stream;
+        const int gb = vblocks(n);
+        const double* hs = cx->hS + (Sblk - cx->S);
+        o = CGOut();
+        P4G_TRY(fill(n, T(0), x));
+        auto norm2 = [&](const T* v, double& nv) -> int {
+            P4G_TRY(cx->launch(P4G_K_OTHER, true, double(n) * sizeof(T), [&] {
+                k_dot2<T><<<gb, 256, 0, st>>>(n, v, v, cx->part);
+            }));
+            P4G_TRY(cx->finalize(gb, Sblk, FIN_RAW, 0));
+            P4G_TRY(cx->readS(cx->hS));
+            nv = std::sqrt(hs[S_TMP0]);
+            return 0;
+        };
+        if (pctype == PCT_MG && nlev > 1 && !monitor) {
+            double ttol = 0.0;
+            if (rtol > 0.0) {
+                double bn;
+                P4G_TRY(norm2(b, bn));
+                ttol = std::max(rtol * bn, atol);
+            } else if (atol > 0.0) {
+                ttol = atol;
+            }
+            int i = 0, conv = 0;
+            for (; i < maxit && !conv; ++i) {
+                P4G_TRY(mcycle_check(b, x, i == 0, ttol, atol, Sblk, &conv));
+                if (conv) break;
+            }
+            o.its = i;
+            o.reason = conv ? conv : P4G_CONVERGED_ITS;
+            return 0;
+        }
+        HIP_TRY(hipMemcpyAsync(r, b, n * sizeof(T), hipMemcpyDeviceToDevice, st));
+        auto pcz = [&](double& zn) -> int {   // z = B r and |z|
+            long nb = 0;
+            P4G_TRY(pc_apply_dots(r, z, &nb));
+            P4G_TRY(cx->finalize(nb, Sblk, FIN_RAW, 0));
+            P4G_TRY(cx->readS(cx->hS));
+            zn = std::sqrt(hs[S_TMP1]);
+            return 0;
+        };
+        double ttol = 0.0, rn = 0.0;
+        auto test = [&](int it) -> int {   // KSPConvergedDefault
+            if (it == 0) { o.rnorm0 = rn; ttol = std::max(rtol * rn, atol); }
+            if (std::isnan(rn) || std::isinf(rn)) return P4G_DIVERGED_NANORINF;
+            if (rn <= ttol) return rn < atol ? P4G_CONVERGED_ATOL : P4G_CONVERGED_RTOL;
+            if (it > 0 && rn >= dtol * o.rnorm0) return P4G_DIVERGED_DTOL;
+            return 0;
+        };
+        const bool unp = normtype == NT_UNPRECOND;
+        int i = 0, its = 0;
+        for (; i < maxit; ++i) {
+            if (unp) P4G_TRY(norm2(r, rn));
+            else P4G_TRY(pcz(rn));
+            o.rnorm = rn;
+            o.hist.push_back(rn);
+            o.reason = test(i);
+            if (o.reason) break;
+            double zn;
+            if (unp) P4G_TRY(pcz(zn));
+            P4G_TRY(cx->launch(P4G_K_CG_UPDATE, true, 3.0 * n * sizeof(T), [&] {
+                k_axpy<T><<<gb, 256, 0, st>>>(n, T(scale), z, x);
+            }));
+            ++its;
+            P4G_TRY(stencil(nlev - 1, SrcArr<T>{x}, OutResid<T, 0>{b, r}, P4G_K_CG_RESID, 3, nullptr, false));
+        }
+        if (!o.reason) {
+            if (unp) P4G_TRY(norm2(r, rn));
+            else P4G_TRY(pcz(rn));
+            o.rnorm = rn;
+            o.hist.push_back(rn);
+            if (its >= maxit) {
+                o.reason = test(i);
+                if (!o.reason) o.reason = P4G_DIVERGED_ITS;
+            }
+        }
+        o.its = its;
+        return 0;
+    }
 };
 
 // FP64 Lanczos (PCG) estimate of the extreme eigenvalues of D^-1 A on one
@@ -982,6 +1131,11 @@ struct p4g_solver_s {
     int fuse_r = -1, fuse_dir = -1;   // CG updates fused into stencil passes: 1 on, 0 off, -1 by rule
     bool galerkin = false;            // coarse operators P^T A P (-pc_mg_galerkin)
     bool fmg = false;                 // -pc_mg_type full: one F-cycle per preconditioner apply
+    bool kaskade = false;             // -pc_mg_type kaskade
+    int mcycles = 1;                  // -pc_mg_multiplicative_cycles
+    bool ksp_rich = false;            // -ksp_type richardson
+    double rich_outer_scale = 1.0;    // -ksp_richardson_scale
+    bool monitor = false;             // -ksp_monitor (PETSc's generic Richardson loop)
     // reduced-space VI Newton (p4g_vi_solve): SNES options and state
     double snes_rtol = 1e-8, snes_atol = 1e-50, snes_stol = 1e-8, snes_divtol = 1e4;
     int snes_maxit = 50;
@@ -1302,12 +1456,16 @@ static int estimate_level(p4g_solver_s* s, int l) {
     return 0;
 }
 
-static const char* kUnsupported[] = {"pc_mg_multiplicative_cycles", "mg_levels_ksp_norm_type", nullptr};
+static const char* kUnsupported[] = {"mg_levels_ksp_norm_type", nullptr};
 
 static int configure(p4g_solver_s* s) {
     Options& o = s->opt;
     const std::string kt = o.gets("ksp_type", "cg");
-    if (kt != "cg") return set_error(PETSC_ERR_SUP, "-ksp_type %s: the device path implements cg", kt.c_str());
+    if (kt == "richardson") s->ksp_rich = true;
+    else if (kt != "cg")
+        return set_error(PETSC_ERR_SUP, "-ksp_type %s: the device path implements cg and richardson", kt.c_str());
+    s->rich_outer_scale = o.getr("ksp_richardson_scale", 1.0);
+    s->monitor = o.has("ksp_monitor");
     const std::string pt = o.gets("pc_type", "");
     if (pt == "mg") s->pctype = PCT_MG;
     else if (pt == "jacobi") s->pctype = PCT_JACOBI;
@@ -1320,9 +1478,12 @@ static int configure(p4g_solver_s* s) {
         if (o.has(*u)) return set_error(PETSC_ERR_SUP, "-%s is not implemented on the device", *u);
     const std::string mt = o.gets("pc_mg_type", "multiplicative");
     if (mt == "full") s->fmg = true;
+    else if (mt == "kaskade") s->kaskade = true;
     else if (mt != "multiplicative")
-        return set_error(PETSC_ERR_SUP, "-pc_mg_type %s is not implemented on the device (multiplicative, full)",
-                         mt.c_str());
+        return set_error(PETSC_ERR_SUP,
+                         "-pc_mg_type %s is not implemented on the device (multiplicative, full, kaskade)", mt.c_str());
+    s->mcycles = o.geti("pc_mg_multiplicative_cycles", 1);
+    if (s->mcycles < 1) return set_error(PETSC_ERR_ARG_OUTOFRANGE, "-pc_mg_multiplicative_cycles must be >= 1");
     s->snes_rtol = o.getr("snes_rtol", 1e-8);
     s->snes_atol = o.getr("snes_atol", 1e-50);
     s->snes_stol = o.getr("snes_stol", 1e-8);
@@ -1401,6 +1562,13 @@ static int configure(p4g_solver_s* s) {
         }
     if (s->galerkin && s->mixed)
         return set_error(PETSC_ERR_SUP, "-pc_mg_galerkin with -p4g_precision mixed is not implemented on the device");
+    if (s->ksp_rich && s->mixed)
+        return set_error(PETSC_ERR_SUP, "-ksp_type richardson with -p4g_precision mixed is not implemented on the device");
+    if (s->ksp_rich && s->normtype == NT_NATURAL)
+        return set_error(PETSC_ERR_SUP, "-ksp_norm_type natural is not supported by -ksp_type richardson");
+    if ((s->kaskade || s->mcycles > 1) && s->mixed)
+        return set_error(PETSC_ERR_SUP, "-pc_mg_type kaskade and -pc_mg_multiplicative_cycles with -p4g_precision "
+                                        "mixed are not implemented on the device");
     return 0;
 }
 
@@ -1543,6 +1711,9 @@ int solver_setup(p4g_solver_s* s) {
         e->g27.assign(s->g27.begin(), s->g27.end());
         e->fmg = s->fmg;
         e->vi = s->vi_mode;
+        e->kaskade = s->kaskade;
+        e->mcycles = s->mcycles;
+        e->full_temps = s->kaskade || s->mcycles > 1 || (s->ksp_rich && s->pctype == PCT_MG);
     };
     if (F.var) {
         const size_t nc = F.kcell.size();
@@ -1858,7 +2029,11 @@ int ksp_solve_device(p4g_solver_s* s, const double* b, double* x) {
         P4G_TRY(mixed_solve(s, b, x));
     } else {
         typename Engine<double>::CGOut co;
-        P4G_TRY(s->e64->cg(b, x, s->normtype, s->rtol, s->atol, s->dtol, s->maxit, co, s->cx.S));
+        if (s->ksp_rich)
+            P4G_TRY(s->e64->richardson(b, x, s->normtype, s->rtol, s->atol, s->dtol, s->maxit, s->rich_outer_scale,
+                                       s->monitor, co, s->cx.S));
+        else
+            P4G_TRY(s->e64->cg(b, x, s->normtype, s->rtol, s->atol, s->dtol, s->maxit, co, s->cx.S));
         R.its = co.its; R.reason = co.reason; R.rnorm0 = co.rnorm0; R.rnorm = co.rnorm;
         R.inner_its = 0;
         for (size_t q = 0; q < 64; ++q) R.history[q] = q < co.hist.size() ? co.hist[q] : 0.0;

@@ -84,6 +84,22 @@ CASES = {
     "3d_richardson3": ("-fsh_dim 3 -fsh_problem manusin -da_refine 4 -pc_type mg -mg_levels_ksp_type "
                        "richardson -mg_levels_pc_type jacobi -mg_levels_ksp_max_it 3 "
                        "-mg_levels_ksp_richardson_scale 0.7 -ksp_rtol 1e-10"),
+    # outer Richardson: the multigrid shortcut (no monitor) and PETSc's
+    # generic loop (with -ksp_monitor), cycle variants of c/ch6/study
+    "3d_outer_richardson_mg_shortcut": "-fsh_dim 3 -fsh_problem manusin -da_refine 4 %s -ksp_type richardson "
+                                       "-ksp_rtol 1e-10" % MG,
+    "2d_outer_richardson_unprecond_w": ("-fsh_dim 2 -fsh_problem manupoly -da_refine 5 %s -ksp_type richardson "
+                                        "-ksp_norm_type unpreconditioned -ksp_monitor -pc_mg_cycle_type w "
+                                        "-ksp_rtol 1e-10" % MG),
+    "3d_outer_richardson_scaled_jacobi": ("-fsh_dim 3 -fsh_problem manusin -da_refine 3 -pc_type jacobi "
+                                          "-ksp_type richardson -ksp_richardson_scale 0.9 -ksp_rtol 1e-3 "
+                                          "-ksp_max_it 20000"),
+    "3d_mcycles2_cg": "-fsh_dim 3 -fsh_problem manusin -da_refine 4 %s -pc_mg_multiplicative_cycles 2 "
+                      "-ksp_rtol 1e-10" % MG,
+    "3d_kaskade_cg": "-fsh_dim 3 -fsh_problem manusin -da_refine 4 %s -pc_mg_type kaskade "
+                     "-mg_levels_ksp_max_it 4 -ksp_rtol 1e-10" % MG,
+    "3d_kaskade_outer_richardson": ("-fsh_dim 3 -fsh_problem manusin -da_refine 4 %s -pc_mg_type kaskade "
+                                    "-mg_levels_ksp_max_it 4 -ksp_type richardson -ksp_monitor -ksp_rtol 1e-10" % MG),
 }
 
 
@@ -180,7 +196,10 @@ def test_operator_and_vcycle_parity(args):
     "-fsh_dim 3 -fsh_problem manusin -da_refine 4 %s -pc_mg_type full -mg_levels_ksp_max_it 1" % MG,
     "-fsh_dim 3 -fsh_problem manusin -da_refine 3 %s -pc_mg_galerkin" % MG,
     "-fsh_dim 2 -fsh_problem manupoly -da_refine 5 %s -pc_mg_galerkin -pc_mg_type full "
-    "-pc_mg_cycle_type w" % MG], ids=["3d_fcycle", "3d_galerkin", "2d_galerkin_fcycle_w"])
+    "-pc_mg_cycle_type w" % MG,
+    "-fsh_dim 3 -fsh_problem manusin -da_refine 4 %s -pc_mg_type kaskade -mg_levels_ksp_max_it 3" % MG,
+    "-fsh_dim 2 -fsh_problem manupoly -da_refine 5 %s -pc_mg_multiplicative_cycles 3 -pc_mg_cycle_type w" % MG],
+    ids=["3d_fcycle", "3d_galerkin", "2d_galerkin_fcycle_w", "3d_kaskade", "2d_mcycles3_w"])
 def test_fcycle_and_galerkin_apply_parity(args):
     # one preconditioner application against the oracle's, and for the
     # (symmetric) Galerkin V-cycle the symmetry of B
@@ -191,7 +210,7 @@ def test_fcycle_and_galerkin_apply_parity(args):
         x = rng.standard_normal(s.n)
         z = sv.pc_apply(x)
         assert _rel(z, O.pc_apply(args, x)) < 1e-11
-        if "full" not in args:
+        if "full" not in args and "kaskade" not in args:
             w = rng.standard_normal(s.n)
             assert abs(z @ w - x @ sv.pc_apply(w)) < 1e-11 * abs(z @ w)
 
@@ -224,7 +243,17 @@ def test_ksp_viewer_files(tmp_path):
     "-fsh_dim 3 -fsh_problem manusin -da_refine 4 %s -ksp_rtol 1e-10" % MG,
     "-fsh_dim 2 -fsh_problem manupoly -da_refine 5 %s -ksp_norm_type unpreconditioned" % MG,
     "-fsh_dim 3 -fsh_problem manusin -da_refine 3 %s -ksp_norm_type natural -pc_mg_galerkin" % MG,
-], ids=["3d_precond", "2d_unprecond", "3d_natural_galerkin"])
+    # c/ch6/study/cyclereductions.sh: one Richardson iteration per cycle type
+    "-fsh_dim 3 -da_refine 4 %s -ksp_type richardson -ksp_max_it 1 -ksp_norm_type unpreconditioned "
+    "-mg_levels_ksp_max_it 2 -pc_mg_cycle_type w" % MG,
+    "-fsh_dim 3 -da_refine 4 %s -ksp_type richardson -ksp_max_it 1 -ksp_norm_type unpreconditioned "
+    "-mg_levels_ksp_max_it 1 -pc_mg_multiplicative_cycles 2 -fsh_initial_type random" % MG,
+    "-fsh_dim 3 -da_refine 4 %s -ksp_type richardson -ksp_max_it 1 -ksp_norm_type unpreconditioned "
+    "-mg_levels_ksp_max_it 2 -pc_mg_type full" % MG,
+    "-fsh_dim 3 -da_refine 4 %s -ksp_type richardson -ksp_max_it 1 -ksp_norm_type unpreconditioned "
+    "-mg_levels_ksp_max_it 4 -pc_mg_type kaskade" % MG,
+], ids=["3d_precond", "2d_unprecond", "3d_natural_galerkin", "study_w", "study_mcycles2_random",
+        "study_full", "study_kaskade"])
 def test_ksp_monitor_lines(args):
     # -ksp_monitor: PETSc's KSPMonitorResidual lines, before the converged
     # reason, with the oracle's residual norms
