@@ -1781,6 +1781,115 @@ k_finalize(long nparts, const double* __restrict__ part, double* __restrict__ S,
 }
 
 
+// ------------------------------------------------------ SOR smoothing
+// PETSc's MatSOR with omega = 1 and a zero initial guess, symmetric sweep
+// (the PCSOR default): forward x_q = (b_q - sum_{j<q} a_qj x_j) / a_qq, then
+// backward x_q = (b_q - sum_{j!=q} a_qj x_j) / a_qq with the newest values,
+// in lexicographic (x fastest) order.  A vertex depends only on neighbours
+// earlier in that order, so every vertex of the hyperplane
+// wi*i + wj*j + wk*k = p (weights making every earlier neighbour's plane
+// smaller: 1,1,1 for the 3/5/7-point operator, 1,2,4 for 27 points) is
+// updated in parallel, plane after plane, in exactly the serial arithmetic.
+// Local sweeps of nblk ranks (blocks of the slowest dimension starting at
+// blk[0..nblk-1]) drop the couplings between blocks, as PETSc's
+// SOR_LOCAL_SYMMETRIC_SWEEP does on several processes.
+struct SorOp {
+    int dim, mx, my, mz;
+    long n;
+    double d, db, sx, sy, sz;     // constant operator (g == nullptr)
+    const double* g;              // 27-point coefficients (Galerkin or VI levels)
+    int nblk;
+    int blk[17];
+};
+
+__device__ __forceinline__ void sor_block(const SorOp& o, int s, int& lo, int& hi) {
+    lo = 0;
+    hi = o.dim == 3 ? o.mz : (o.dim == 2 ? o.my : o.mx);
+    for (int q = 1; q < o.nblk; ++q) {
+        if (o.blk[q] <= s) lo = o.blk[q];
+        else { hi = o.blk[q]; break; }
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_sor_plane(SorOp o, int p, int backward, const T* __restrict__ b, T* __restrict__ x) {
+    const long t = blockIdx.x * 256L + threadIdx.x;
+    const int wj = o.g ? 2 : 1, wk = o.g ? 4 : 1;
+    int i, j, k;
+    if (o.dim == 3) {
+        if (t >= (long)o.my * o.mz) return;
+        j = (int)(t % o.my);
+        k = (int)(t / o.my);
+    } else if (o.dim == 2) {
+        if (t >= o.my) return;
+        j = (int)t;
+        k = 0;
+    } else {
+        if (t > 0) return;
+        j = 0;
+        k = 0;
+    }
+    i = p - wj * j - wk * k;
+    if (i < 0 || i >= o.mx) return;
+    const int sdim = o.dim == 3 ? k : (o.dim == 2 ? j : i);
+    int lo, hi;
+    sor_block(o, sdim, lo, hi);
+    const long sy = o.mx, sz = (long)o.mx * o.my;
+    const long q = ((long)k * o.my + j) * o.mx + i;
+    double s = double(b[q]), dq;
+    if (o.g) {
+        dq = o.g[13 * o.n + q];
+        for (int c = 0; c < 27; ++c) {
+            if (c == 13 || (!backward && c > 13)) continue;
+            const int oz = c / 9 - 1, oy = (c / 3) % 3 - 1, ox = c % 3 - 1;
+            const int ii = i + ox, jj = j + oy, kk = k + oz;
+            if (ii < 0 || ii >= o.mx || jj < 0 || jj >= o.my || kk < 0 || kk >= o.mz) continue;
+            const int sn = o.dim == 3 ? kk : (o.dim == 2 ? jj : ii);
+            if (sn < lo || sn >= hi) continue;
+            const double a = o.g[(long)c * o.n + q];
+            if (a != 0.0) s -= a * double(x[q + oz * sz + oy * sy + ox]);
+        }
+        x[q] = dq != 0.0 ? T(s * (1.0 / dq)) : T(0);   // a null row stays zero
+        return;
+    }
+    const bool bnd = i == 0 || i == o.mx - 1 || (o.dim >= 2 && (j == 0 || j == o.my - 1)) ||
+                     (o.dim == 3 && (k == 0 || k == o.mz - 1));
+    if (bnd) {
+        x[q] = T(s * (1.0 / o.db));
+        return;
+    }
+    // interior neighbours only, in column order: k-1, j-1, i-1 | i+1, j+1, k+1
+    auto in = [&](int ii, int jj, int kk) {
+        const bool nb = ii == 0 || ii == o.mx - 1 || (o.dim >= 2 && (jj == 0 || jj == o.my - 1)) ||
+                        (o.dim == 3 && (kk == 0 || kk == o.mz - 1));
+        const int sn = o.dim == 3 ? kk : (o.dim == 2 ? jj : ii);
+        return !nb && sn >= lo && sn < hi;
+    };
+    if (o.dim == 3 && in(i, j, k - 1)) s += o.sz * double(x[q - sz]);
+    if (o.dim >= 2 && in(i, j - 1, k)) s += o.sy * double(x[q - sy]);
+    if (in(i - 1, j, k)) s += o.sx * double(x[q - 1]);
+    if (backward) {
+        if (in(i + 1, j, k)) s += o.sx * double(x[q + 1]);
+        if (o.dim >= 2 && in(i, j + 1, k)) s += o.sy * double(x[q + sy]);
+        if (o.dim == 3 && in(i, j, k + 1)) s += o.sz * double(x[q + sz]);
+    }
+    x[q] = T(s * (1.0 / o.d));
+}
+
+// y = a p + c q + e z (p or q may be null: 0)
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_comb3(long n, double a, const T* __restrict__ p, double c, const T* __restrict__ q, double e,
+        const T* __restrict__ z, T* __restrict__ y) {
+    for (long m = blockIdx.x * 256L + threadIdx.x; m < n; m += (long)gridDim.x * 256L) {
+        double v = e * double(z[m]);
+        if (p) v += a * double(p[m]);
+        if (q) v += c * double(q[m]);
+        y[m] = T(v);
+    }
+}
+
 // y += a x
 template <typename T>
 __global__ void __launch_bounds__(256) k_axpy(long n, T a, const T* __restrict__ x, T* __restrict__ y) {

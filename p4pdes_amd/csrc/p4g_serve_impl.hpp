@@ -120,6 +120,7 @@ static int serve_supported(p4g_solver_s* s) {
     if (s->galerkin || s->fmg || s->kaskade || s->mcycles != 1)
         return set_error(PETSC_ERR_SUP, "serving runs rediscretized multiplicative V-cycles (no -pc_mg_galerkin, "
                                         "-pc_mg_type full|kaskade or -pc_mg_multiplicative_cycles)");
+    if (s->sor) return set_error(PETSC_ERR_SUP, "serving runs Jacobi-preconditioned smoothers (no SOR)");
     if (s->ksp_rich || s->vi_mode)
         return set_error(PETSC_ERR_SUP, "serving runs CG solves (no -ksp_type richardson or VI solver)");
     return 0;
@@ -657,7 +658,11 @@ int p4g_server_create(size_t hbm_budget_bytes, const char* options, p4g_Server* 
     v->max_slots = std::max(1, std::min(1024, o.geti("p4g_serve_max_slots", 64)));
     hipGetDevice(&v->device);
     int e = 0;
-    if (hipStreamCreate(&v->st) != hipSuccess) e = set_error(PETSC_ERR_LIB, "p4g_server_create: stream");
+    // non-blocking (default priority): while the worker captures a graph on
+    // it, synchronous copies on the legacy stream from other threads do not
+    // depend on it (the server orders its own work with explicit syncs)
+    if (hipStreamCreateWithFlags(&v->st, hipStreamNonBlocking) != hipSuccess)
+        e = set_error(PETSC_ERR_LIB, "p4g_server_create: stream");
     if (!e && hipMalloc(&v->S, sizeof(double) * 32 * v->max_slots) != hipSuccess)
         e = set_error(PETSC_ERR_MEM, "p4g_server_create: scalar blocks");
     if (!e && hipHostMalloc((void**)&v->hS, sizeof(double) * 32 * v->max_slots, hipHostMallocDefault) != hipSuccess)

@@ -80,7 +80,31 @@ struct LevelInfo {
     double ax = 0, ay = 0, az = 0;
     double emin = 0, emax = 0;                       // Chebyshev bounds
     bool gal = false;                                // Galerkin operator P^T A P in 27-point form
+    int sor_nblk = 1;                                // SOR local sweeps: blocks of the slowest dimension
+    int sor_blk[17] = {0};                           //   and their first indices
 };
+
+// one symmetric SOR sweep z = SSOR(r) on a level (omega 1, zero initial
+// guess): the hyperplanes forward then backward (k_sor_plane)
+template <typename T>
+static int sor_run(const LevelInfo& L, const double* g27, const T* r, T* z, hipStream_t st) {
+    SorOp o;
+    o.dim = L.dim; o.mx = L.mx; o.my = L.my; o.mz = L.mz; o.n = L.n;
+    o.d = L.d; o.db = L.db; o.sx = L.sx; o.sy = L.sy; o.sz = L.sz;
+    o.g = L.gal ? g27 : nullptr;
+    if (L.var) return set_error(PETSC_ERR_SUP, "SOR smoothing of a variable-coefficient level is not implemented");
+    if (L.gal && !g27) return set_error(PETSC_ERR_LIB, "internal: SOR on a Galerkin level without coefficients");
+    o.nblk = L.sor_nblk;
+    for (int q = 0; q < 17; ++q) o.blk[q] = L.sor_blk[q];
+    const int wj = o.g ? 2 : 1, wk = o.g ? 4 : 1;
+    const int np = (L.mx - 1) + wj * (L.dim >= 2 ? L.my - 1 : 0) + wk * (L.dim == 3 ? L.mz - 1 : 0) + 1;
+    const long nt = L.dim == 3 ? (long)L.my * L.mz : (L.dim == 2 ? (long)L.my : 1L);
+    const int gb = (int)((nt + 255) / 256);
+    for (int p = 0; p < np; ++p) k_sor_plane<T><<<gb, 256, 0, st>>>(o, p, 0, r, z);
+    for (int p = np - 1; p >= 0; --p) k_sor_plane<T><<<gb, 256, 0, st>>>(o, p, 1, r, z);
+    LAUNCH_CHECK();
+    return 0;
+}
 
 // ------------------------------------------------------------- timing
 
@@ -251,7 +275,7 @@ struct Engine {
     std::vector<const double*> g27;
     bool fmg = false;          // full multigrid: an F-cycle per apply
     struct Dev {
-        T *x = nullptr, *b = nullptr, *r = nullptr, *t[4] = {nullptr, nullptr, nullptr, nullptr};
+        T *x = nullptr, *b = nullptr, *r = nullptr, *t[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
         T* kc = nullptr;
     };
     std::vector<Dev> dv;
@@ -264,6 +288,7 @@ struct Engine {
     int mcycles = 1;        // -pc_mg_multiplicative_cycles
     bool kaskade = false;   // -pc_mg_type kaskade
     bool full_temps = false;   // four smoother temporaries on every level
+    bool sor = false;          // level smoother preconditioned by SSOR (-mg_levels_pc_type sor)
     // Krylov vectors on the finest level
     T *r = nullptr, *z = nullptr, *pa = nullptr, *pb = nullptr, *w = nullptr, *xbuf = nullptr,
       *bbuf = nullptr, *r2 = nullptr;
@@ -295,7 +320,7 @@ struct Engine {
     }
 
     bool zg_fusable(const LevelInfo& L) const {
-        return !L.var && !L.gal && L.d == L.db && (smoother == SM_CHEB || smoother == SM_RICH);
+        return !sor && !L.var && !L.gal && L.d == L.db && (smoother == SM_CHEB || smoother == SM_RICH);
     }
 
     int setup(Ctx* c, std::vector<LevelInfo>* levels, const std::vector<double>& Ainv_host,
@@ -312,7 +337,7 @@ struct Engine {
                 if (l < nlev - 1) { P4G_TRY(alloc(&d.x, L.n)); P4G_TRY(alloc(&d.b, L.n)); }
                 if (l > 0) {
                     P4G_TRY(alloc(&d.r, L.n));
-                    const int nt = (!vi && !full_temps && nu == 1 && cycles == 1 && zg_fusable(L)) ? 1 : 4;
+                    const int nt = sor ? 5 : (!vi && !full_temps && nu == 1 && cycles == 1 && zg_fusable(L)) ? 1 : 4;
                     for (int q = 0; q < nt; ++q) P4G_TRY(alloc(&d.t[q], L.n));
                 }
             }
@@ -531,8 +556,73 @@ struct Engine {
     //   DOTS: fuse (out, b) and (out, out) partials into the final step.
     // mstart = 2: continue after a first step done elsewhere; x0 = iterate 1,
     // pm2init = iterate 0.
+    // Chebyshev / Richardson smoothing preconditioned by SSOR: each step
+    // forms r = b - A p explicitly, z = SSOR(r), then the recurrence (the
+    // oracle's KSPSolve_Chebyshev / _Richardson with PCSOR)
+    int smooth_sor(int l, const T* b, const T* x0, T* out, bool dots, long* nb) {
+        const LevelInfo& L = (*lv)[l];
+        const long n = L.n;
+        Dev& d = dv[l];
+        hipStream_t st = cx->stream;
+        T* tt[5];
+        int nt = 0;
+        for (int q = 0; q < 5; ++q)
+            if (d.t[q] && d.t[q] != b && d.t[q] != x0 && d.t[q] != out) tt[nt++] = d.t[q];
+        if (nt < 4 || !d.r || d.r == b || d.r == out)
+            return set_error(PETSC_ERR_LIB, "internal: SOR smoother temporaries");
+        T* r = d.r;
+        T* z = tt[0];
+        double s, mu = 0.0, omegaprod = 0.0, c0 = 1.0, c1 = 0.0;
+        if (smoother == SM_CHEB) {
+            s = 2.0 / (L.emax + L.emin);
+            const double alpha = 1.0 - s * L.emin;
+            mu = 1.0 / alpha;
+            omegaprod = 2.0 / alpha;
+            c1 = mu;
+        } else {
+            s = rich_scale;
+        }
+        const T* pm1 = x0;          // iterate m-1 (nullptr: zero)
+        const T* pm2 = nullptr;     // iterate m-2
+        const int gb = vblocks(n);
+        for (int m = 1; m <= nu; ++m) {
+            const T* rr = b;
+            if (pm1) {
+                P4G_TRY(stencil(l, SrcArr<T>{pm1}, OutResid<T, 0>{b, r}, P4G_K_SMOOTH, 3, nullptr, false));
+                rr = r;
+            }
+            P4G_TRY(sor_run<T>(L, g27.size() > (size_t)l ? g27[l] : nullptr, rr, z, st));
+            T* dst = out;
+            if (m < nu)
+                for (int q = 1; q < 4; ++q)
+                    if (tt[q] != pm1 && tt[q] != pm2) { dst = tt[q]; break; }
+            double ca = 0.0, cb = 1.0, cc = s;
+            if (smoother == SM_CHEB && m >= 2) {
+                const double ckp1 = 2.0 * mu * c1 - c0;
+                const double omega = omegaprod * c1 / ckp1;
+                c0 = c1;
+                c1 = ckp1;
+                ca = 1.0 - omega; cb = omega; cc = omega * s;
+            }
+            const T* pa = (smoother == SM_CHEB && m >= 2) ? pm2 : nullptr;
+            P4G_TRY(cx->launch(P4G_K_SMOOTH, l == nlev - 1, 3.0 * n * sizeof(T), [&] {
+                k_comb3<T><<<gb, 256, 0, st>>>(n, ca, pa, cb, pm1, cc, z, dst);
+            }));
+            pm2 = pm1;
+            pm1 = dst;
+        }
+        if (dots) {
+            if (nb) *nb = gb;
+            P4G_TRY(cx->launch(P4G_K_SMOOTH, l == nlev - 1, 2.0 * n * sizeof(T), [&] {
+                k_dot2<T><<<gb, 256, 0, st>>>(n, out, b, cx->part);
+            }));
+        }
+        return 0;
+    }
+
     int smooth(int l, const T* b, const T* x0, T* out, bool dots, long* nb, int mstart = 1,
                const T* pm2init = nullptr) {
+        if (sor) return smooth_sor(l, b, x0, out, dots, nb);
         const LevelInfo& L = (*lv)[l];
         const bool jac = jacobi != 0;
         const double idg = jac ? 1.0 / L.d : 1.0;
@@ -736,7 +826,7 @@ struct Engine {
         // x = pre + P x_c, then post-smoothing into x
         T* x0 = zgf ? d.t[0] : xpre;
         const T* pre = zgf ? b : xpre;
-        if (L.dim >= 2 && fuse_post && nu == 1 && !L.gal) {
+        if (L.dim >= 2 && fuse_post && nu == 1 && !L.gal && !sor) {
             // the post-smoothing step reads the prolonged iterate on the fly
             if (dots) P4G_TRY(post_first<true>(l, zgf, pre, T(s * idg), c.x, b, x, T(s), nb));
             else P4G_TRY(post_first<false>(l, zgf, pre, T(s * idg), c.x, b, x, T(s), nb));
@@ -1136,6 +1226,8 @@ struct p4g_solver_s {
     bool ksp_rich = false;            // -ksp_type richardson
     double rich_outer_scale = 1.0;    // -ksp_richardson_scale
     bool monitor = false;             // -ksp_monitor (PETSc's generic Richardson loop)
+    bool sor = false;                 // -mg_levels_pc_type sor: SSOR-preconditioned level smoothers
+    int sor_blocks = 1;               // -p4g_sor_blocks: local sweeps of that many ranks
     // reduced-space VI Newton (p4g_vi_solve): SNES options and state
     double snes_rtol = 1e-8, snes_atol = 1e-50, snes_stol = 1e-8, snes_divtol = 1e4;
     int snes_maxit = 50;
@@ -1397,8 +1489,19 @@ static int estimate_level(p4g_solver_s* s, int l) {
     k_fill<double><<<gb, 256, 0, st>>>(n, 0.0, pa);
     HIP_TRY(hipMemsetAsync(S, 0, 16 * sizeof(double), st));
     k_fill<double><<<gb, 256, 0, st>>>(n, 0.0, w);
-    k_lanczos_update<<<gb, 256, 0, st>>>(n, 0.0, r, w, idv, idc, z, s->cx.part);
-    LAUNCH_CHECK();
+    // r -= alpha w, z = B r (B = D^-1, or one SSOR sweep), partial (r, z)
+    auto lupdate = [&](double alpha) -> int {
+        if (s->sor) {
+            if (alpha != 0.0) k_axpy<double><<<gb, 256, 0, st>>>(n, -alpha, w, r);
+            P4G_TRY(sor_run<double>(L, s->g27.size() > (size_t)l ? s->g27[l] : nullptr, r, z, st));
+            k_dot2<double><<<gb, 256, 0, st>>>(n, z, r, s->cx.part);
+        } else {
+            k_lanczos_update<<<gb, 256, 0, st>>>(n, alpha, r, w, idv, idc, z, s->cx.part);
+        }
+        LAUNCH_CHECK();
+        return 0;
+    };
+    P4G_TRY(lupdate(0.0));
     k_finalize<<<1, FT, 0, st>>>(gb, s->cx.part, S, FIN_RAW, 0);
     P4G_TRY(s->cx.readS(s->cx.hS));
     double rz = s->cx.hS[16 + S_TMP0], rzold = 0.0;
@@ -1436,8 +1539,7 @@ static int estimate_level(p4g_solver_s* s, int l) {
         const double alpha = rz / pw;
         al[it] = alpha;
         m = it + 1;
-        k_lanczos_update<<<gb, 256, 0, st>>>(n, alpha, r, w, idv, idc, z, s->cx.part);
-        LAUNCH_CHECK();
+        P4G_TRY(lupdate(alpha));
         k_finalize<<<1, FT, 0, st>>>(gb, s->cx.part, S, FIN_RAW, 0);
         P4G_TRY(s->cx.readS(s->cx.hS));
         rzold = rz;
@@ -1512,6 +1614,19 @@ static int configure(p4g_solver_s* s) {
     const std::string lp = o.gets("mg_levels_pc_type", s->pctype == PCT_MG ? "sor" : "jacobi");
     if (lp == "jacobi") s->jacobi = 1;
     else if (lp == "none") s->jacobi = 0;
+    else if (lp == "sor" && s->pctype == PCT_MG) {   // PETSc's default: symmetric sweep, omega 1
+        s->sor = true;
+        s->jacobi = 1;
+        for (const char* k : {"mg_levels_pc_sor_forward", "mg_levels_pc_sor_backward",
+                              "mg_levels_pc_sor_local_forward", "mg_levels_pc_sor_local_backward"})
+            if (o.has(k)) return set_error(PETSC_ERR_SUP, "-%s: the device SOR smoother sweeps symmetrically", k);
+        if (o.getr("mg_levels_pc_sor_omega", 1.0) != 1.0 || o.geti("mg_levels_pc_sor_its", 1) != 1 ||
+            o.geti("mg_levels_pc_sor_lits", 1) != 1)
+            return set_error(PETSC_ERR_SUP, "the device SOR smoother has omega 1 and one sweep");
+        s->sor_blocks = o.geti("p4g_sor_blocks", 1);
+        if (s->sor_blocks < 1 || s->sor_blocks > 16)
+            return set_error(PETSC_ERR_ARG_OUTOFRANGE, "-p4g_sor_blocks must be in 1..16");
+    }
     else if (s->nlev == 1) s->jacobi = 1;   /* one level: no smoother is used */
     else
         return set_error(PETSC_ERR_SUP, "-mg_levels_pc_type %s is not implemented on the device (PETSc's "
@@ -1562,6 +1677,9 @@ static int configure(p4g_solver_s* s) {
         }
     if (s->galerkin && s->mixed)
         return set_error(PETSC_ERR_SUP, "-pc_mg_galerkin with -p4g_precision mixed is not implemented on the device");
+    if (s->sor && (s->mixed || !s->da->kcell.empty()))
+        return set_error(PETSC_ERR_SUP, "-mg_levels_pc_type sor needs FP64 and a constant-coefficient operator "
+                                        "on the device");
     if (s->ksp_rich && s->mixed)
         return set_error(PETSC_ERR_SUP, "-ksp_type richardson with -p4g_precision mixed is not implemented on the device");
     if (s->ksp_rich && s->normtype == NT_NATURAL)
@@ -1674,10 +1792,38 @@ static void fuse_rules(const p4g_solver_s* s, Engine<T>* e, double vec_bytes) {
     e->fuse_dir = s->fuse_dir >= 0 ? s->fuse_dir == 1 : big;
 }
 
+// local SOR sweeps of several ranks: the slowest dimension of the finest
+// grid split as M/np + (M mod np > q); a coarse index I belongs to the rank
+// owning fine index 2I (PETSc's DMDA ownership under refinement)
+static void sor_splits(p4g_solver_s* s) {
+    const int np = s->sor_blocks;
+    for (LevelInfo& L : s->lv) { L.sor_nblk = 1; L.sor_blk[0] = 0; }
+    if (np <= 1) return;
+    auto slow = [](const LevelInfo& L) { return L.dim == 3 ? L.mz : (L.dim == 2 ? L.my : L.mx); };
+    const int F = s->nlev - 1;
+    std::vector<std::vector<int>> sp(s->nlev, std::vector<int>(np + 1, 0));
+    const int M = slow(s->lv[F]);
+    for (int q = 0; q < np; ++q) sp[F][q + 1] = sp[F][q] + M / np + (M % np > q ? 1 : 0);
+    for (int l = F; l > 0; --l) {
+        const int mc = slow(s->lv[l - 1]);
+        for (int q = 0; q < np; ++q) {
+            int cnt = 0;
+            for (int I = 0; I < mc; ++I)
+                if (2 * I >= sp[l][q] && 2 * I < sp[l][q + 1]) ++cnt;
+            sp[l - 1][q + 1] = sp[l - 1][q] + cnt;
+        }
+    }
+    for (int l = 0; l <= F; ++l) {
+        s->lv[l].sor_nblk = np;
+        for (int q = 0; q < np; ++q) s->lv[l].sor_blk[q] = sp[l][q];
+    }
+}
+
 int solver_setup(p4g_solver_s* s) {
     if (s->ready) return 0;
     const double t0 = now_ms();
     P4G_TRY(build_levels(s));
+    sor_splits(s);
     if (!s->cx.stream) HIP_TRY(hipStreamCreate(&s->cx.stream));
     // partial-sum capacity: the largest grid of blocks of any fused pass
     const LevelInfo& F = s->lv.back();
@@ -1714,6 +1860,7 @@ int solver_setup(p4g_solver_s* s) {
         e->kaskade = s->kaskade;
         e->mcycles = s->mcycles;
         e->full_temps = s->kaskade || s->mcycles > 1 || (s->ksp_rich && s->pctype == PCT_MG);
+        e->sor = s->sor;
     };
     if (F.var) {
         const size_t nc = F.kcell.size();

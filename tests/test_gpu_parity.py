@@ -166,6 +166,55 @@ def test_golden_lines_with_jacobi_smoother():
     assert otext == text
 
 
+def _golden_fish():
+    import json
+    import os
+    here = os.path.dirname(os.path.abspath(__file__))
+    return {t["test"]: t for t in json.load(open(os.path.join(here, "golden", "fish_golden.json")))["tests"]}
+
+
+@pytest.mark.parametrize("num", [1, 3, 4, 7])
+def test_golden_fish_with_default_sor_smoother(num):
+    # the reference's own commands (PETSc's default Chebyshev/SOR level
+    # smoother), byte for byte against c/ch6/output/fish.testN; the 2-rank
+    # tests with the local SOR sweeps of two processes
+    t = _golden_fish()[num]
+    args = t["options"] + (" -p4g_sor_blocks %d" % t["nprocs"] if t["nprocs"] > 1 else "")
+    print("golden fish", num, args, flush=True)
+    rc, text, res, _ = capi.fish_run(args)
+    assert rc == 0, text
+    assert text == t["expected"]
+
+
+SOR_CASES = {
+    "3d_cheb_sor": "-fsh_dim 3 -fsh_problem manusin -da_refine 4 -pc_type mg -ksp_rtol 1e-10",
+    "2d_w_rich_sor_2blocks": ("-fsh_dim 2 -fsh_problem manupoly -da_refine 5 -pc_type mg -pc_mg_cycle_type w "
+                              "-mg_levels_ksp_type richardson -mg_levels_ksp_max_it 1 -ksp_rtol 1e-10 "
+                              "-p4g_sor_blocks 2 -oracle_nranks 2"),
+    "3d_galerkin_sor_3blocks": ("-fsh_dim 3 -fsh_problem manupoly -da_refine 3 -pc_type mg -pc_mg_galerkin "
+                                "-ksp_rtol 1e-10 -p4g_sor_blocks 3 -oracle_nranks 3"),
+    "2d_fcycle_sor": "-fsh_dim 2 -fsh_problem manusin -da_refine 5 -pc_type mg -pc_mg_type full -ksp_rtol 1e-10",
+    "3d_kaskade_richardson_sor": ("-fsh_dim 3 -fsh_problem manusin -da_refine 4 -pc_type mg -pc_mg_type kaskade "
+                                  "-mg_levels_ksp_max_it 4 -ksp_type richardson -ksp_monitor -ksp_rtol 1e-8"),
+    "1d_sor": "-fsh_dim 1 -fsh_problem manupoly -da_refine 7 -pc_type mg -ksp_rtol 1e-12",
+}
+
+
+@pytest.mark.parametrize("name", sorted(SOR_CASES))
+def test_sor_smoother_parity(name):
+    args = SOR_CASES[name]
+    # the device reads -p4g_sor_blocks, the oracle -oracle_nranks
+    dev = re.sub(r"-oracle_nranks \d+", "", args)
+    print("sor case", name, dev, flush=True)
+    rc, text, res, u = capi.fish_run(dev, want_u=True)
+    assert rc == 0, text
+    orc, otext, ores, ou = O.fish_run(args, want_u=True)
+    assert orc == 0, otext
+    assert res.reason > 0 and abs(res.its - ores.its) <= 1, (res.its, ores.its)
+    assert _rel(u, ou) < 1e-9
+    assert text.splitlines()[-1] == otext.splitlines()[-1]
+
+
 OPS = [
     "-fsh_dim 3 -fsh_problem manusin -da_refine 4 %s -ksp_rtol 1e-10" % MG,
     "-fsh_dim 2 -fsh_problem manupoly -fsh_cx 0.3 -da_refine 5 %s -mg_levels_ksp_max_it 1" % MG,

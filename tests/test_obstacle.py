@@ -196,6 +196,8 @@ CASES = {
     # Newton step: 513x513 from u = 0 reaches -snes_max_it; sequence instead
     "seq5_to_513": "-da_refine 3 -snes_grid_sequence 5 -pc_type mg -mg_levels_pc_type jacobi",
     "129_max_it": "-da_refine 6 -pc_type mg -mg_levels_pc_type jacobi -snes_max_it 5",
+    "65_sor": "-da_refine 5 -pc_type mg",
+    "seq4_sor_w": "-snes_grid_sequence 4 -pc_type mg -pc_mg_cycle_type w",
 }
 SHOW = " -snes_monitor_short -ksp_converged_reason -snes_converged_reason"
 
@@ -239,6 +241,21 @@ def test_obstacle_device_parity(case):
     assert np.max(np.abs(u - uo)) <= 1e-9 * np.max(np.abs(uo))
     assert res.active == reso.active
     assert text.splitlines()[-1] == texto.splitlines()[-1]   # the error line
+
+
+@pytest.mark.gpu
+def test_obstacle_device_golden_test3_default_smoother():
+    # obstacle.test3 as the reference runs it (-pc_type mg with PETSc's
+    # default Chebyshev/SOR smoother on the inactive-set Galerkin levels):
+    # every line but the 3x3 one (see the module docstring)
+    from p4pdes_amd import capi
+    t = GOLDEN[3]
+    print("obstacle golden 3 default smoother", flush=True)
+    rc, text, res, _ = capi.obstacle_run(t["options"])
+    assert rc == 0, text
+    assert text.splitlines()[1:] == t["expected"].splitlines()[1:]
+    rco, texto, _, _ = O.obstacle_run(t["options"])
+    assert text == texto
 
 
 @pytest.mark.gpu
