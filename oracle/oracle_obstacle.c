@@ -25,13 +25,9 @@
  * Pinning: obstacle.test1 (|F| history 3.86571, 1.3323, < 1e-11, 11 CG/ILU
  * iterations per Newton step, the error line) and obstacle.test3 (grid-
  * sequenced SNES iteration counts 1, 2, 2, 3, last KSP count 4 and the
- * error line).  On test3's 3x3 grid one Newton step solves a diagonal
- * reduced system; the residual after it is exactly zero here (reason
- * CONVERGED_FNORM_ABS) and at rounding level in PETSc (FNORM_RELATIVE), a
- * rounding-level difference no test can pin.  A one-level PCMG is the
- * coarse direct solve here, as in the fish oracle.  The backtracking
- * branch of the line search is not reached by those tests: parity
- * unpinned there.
+ * error line).  A one-level PCMG (test3's 3x3 grid) applies the level
+ * smoother, as PETSc's PCMGSetLevels makes it.  The backtracking branch of
+ * the line search is not reached by those tests: parity unpinned there.
  */
 #include <math.h>
 #include <stdlib.h>
@@ -234,7 +230,7 @@ static int vimg_setup(vimg *m, const ogrid *fine, int nlev, long nk, const long 
     }
     for (int l = 0; l < nlev; l++) {
         const long n = m->n[l] ? m->n[l] : 1;
-        if (l > 0) {
+        if (l > 0 || nlev == 1) {   /* PCMGSetLevels: a single level is a smoother level */
             oksp_init(&m->sm[l], ltype, &m->A[l], lpc);
             m->sm[l].maxit = lits;
             m->sm[l].rich_scale = lscale;

@@ -704,10 +704,27 @@ int ocsr_is_symmetric(const ocsr *A, double tol) {
 
 /* ----------------------------------------------------------------- vectors */
 
+/* deterministic: fixed chunks of 4096 summed in order, whatever the
+ * thread count (an OpenMP reduction combines partials in no fixed order) */
+#define OV_CHUNK 4096L
 double ovdot(long n, const double *x, const double *y) {
+    const long nc = (n + OV_CHUNK - 1) / OV_CHUNK;
+    if (nc <= 1) {
+        double s = 0.0;
+        for (long i = 0; i < n; i++) s += x[i] * y[i];
+        return s;
+    }
+    double *part = (double *)malloc(sizeof(double) * nc);
+#pragma omp parallel for schedule(static)
+    for (long c = 0; c < nc; c++) {
+        const long lo = c * OV_CHUNK, hi = lo + OV_CHUNK < n ? lo + OV_CHUNK : n;
+        double s = 0.0;
+        for (long i = lo; i < hi; i++) s += x[i] * y[i];
+        part[c] = s;
+    }
     double s = 0.0;
-#pragma omp parallel for reduction(+ : s) schedule(static)
-    for (long i = 0; i < n; i++) s += x[i] * y[i];
+    for (long c = 0; c < nc; c++) s += part[c];
+    free(part);
     return s;
 }
 double ovnorm2(long n, const double *x) { return sqrt(ovdot(n, x, x)); }

@@ -737,7 +737,7 @@ int omg_setup(omg *mg, const ogrid *fine, const oprob *prob, const ocsr *Afine,
     }
     for (int l = 0; l < nlev; l++) {
         long n = mg->g[l].n;
-        if (l > 0) {
+        if (l > 0 || nlev == 1) {   /* PCMGSetLevels: a single level is a smoother level */
             oksp_init(&mg->smooth[l], ltype, &mg->A[l], lpc);
             mg->smooth[l].maxit = lits;
             mg->smooth[l].rich_scale = lscale;
@@ -755,7 +755,7 @@ int omg_setup(omg *mg, const ogrid *fine, const oprob *prob, const ocsr *Afine,
             pc->nblk = mg->nranks;
             for (int q = 0; q <= mg->nranks; q++) pc->blk[q] = (long)mg->split[l][q] * plane;
         }
-        if (l > 0 && ltype == KSP_CHEBYSHEV) oksp_estimate_eigs(&mg->smooth[l]);
+        if ((l > 0 || nlev == 1) && ltype == KSP_CHEBYSHEV) oksp_estimate_eigs(&mg->smooth[l]);
         mg->smooth[l].guess_nonzero = 1;   /* smoothd == smoothu */
         mg->smooth[l].normtype = NORM_NONE;
         if (l < nlev - 1) {

@@ -335,7 +335,7 @@ struct Engine {
             Dev& d = dv[l];
             if (pctype == PCT_MG) {
                 if (l < nlev - 1) { P4G_TRY(alloc(&d.x, L.n)); P4G_TRY(alloc(&d.b, L.n)); }
-                if (l > 0) {
+                if (l > 0 || nlev == 1) {   // smoothing levels (one level: the smoother itself)
                     P4G_TRY(alloc(&d.r, L.n));
                     const int nt = sor ? 5 : (!vi && !full_temps && nu == 1 && cycles == 1 && zg_fusable(L)) ? 1 : 4;
                     for (int q = 0; q < nt; ++q) P4G_TRY(alloc(&d.t[q], L.n));
@@ -920,7 +920,8 @@ struct Engine {
             return 0;
         }
         P4G_TRY(pend_flush());
-        if (pctype == PCT_MG) return mcycle(nlev - 1, rr, zz, true, true, nb);
+        // one level: PCMGSetLevels makes it a smoother level, applied once
+        if (pctype == PCT_MG) return smooth(0, rr, nullptr, zz, true, nb);
         const double c = (pctype == PCT_JACOBI) ? 1.0 / F.d : 1.0;
         if (pctype == PCT_JACOBI && (F.var || F.d != F.db))
             return set_error(PETSC_ERR_SUP, "-pc_type jacobi on the device needs a constant diagonal");
@@ -1842,14 +1843,19 @@ int solver_setup(p4g_solver_s* s) {
         // VI solves form the operators, coarse factor and bounds per Newton
         // step from the active set; here only the sizes
         s->g27.assign(s->nlev, nullptr);
-        s->n0 = (int)s->lv[0].n;
+        s->n0 = s->nlev > 1 ? (int)s->lv[0].n : 1;
         s->Ainv.assign((size_t)s->n0 * s->n0, 0.0);
     } else if (s->pctype == PCT_MG) {
         P4G_TRY(galerkin_levels(s));
-        P4G_TRY(coarse_factor(s->lv[0], s->g27[0], s->Ainv));
-        s->n0 = (int)s->lv[0].n;
+        if (s->nlev > 1) {
+            P4G_TRY(coarse_factor(s->lv[0], s->g27[0], s->Ainv));
+            s->n0 = (int)s->lv[0].n;
+        } else {   // one level: a smoother level, no coarse solve
+            s->n0 = 1;
+            s->Ainv.assign(1, 0.0);
+        }
         if (s->smoother == SM_CHEB)
-            for (int l = 1; l < s->nlev; ++l) P4G_TRY(estimate_level(s, l));
+            for (int l = s->nlev > 1 ? 1 : 0; l < s->nlev; ++l) P4G_TRY(estimate_level(s, l));
     }
     auto cfg = [&](auto& e) {
         e->pctype = s->pctype; e->cycles = s->cycles; e->smoother = s->smoother; e->nu = s->nu;
@@ -2506,19 +2512,32 @@ int p4g_device_free(void* p) {
     return 0;
 }
 
-int p4g_memcpy_htod(void* dst, const void* src, size_t bytes) {
-    HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+// Copies on a library-owned non-blocking stream, synchronized before return:
+// never on the legacy stream, which would implicitly depend on (and, while a
+// server thread captures a graph, fail against) the other streams.  They are
+// ordered after everything the library's calls have completed.
+static int copy_sync(void* dst, const void* src, size_t bytes, hipMemcpyKind kind) {
+    static hipStream_t cs = [] {
+        hipStream_t t = nullptr;
+        if (hipStreamCreateWithFlags(&t, hipStreamNonBlocking) != hipSuccess) t = nullptr;
+        return t;
+    }();
+    if (!cs) return set_error(PETSC_ERR_LIB, "p4g_memcpy: no copy stream");
+    HIP_TRY(hipMemcpyAsync(dst, src, bytes, kind, cs));
+    HIP_TRY(hipStreamSynchronize(cs));
     return 0;
+}
+
+int p4g_memcpy_htod(void* dst, const void* src, size_t bytes) {
+    return copy_sync(dst, src, bytes, hipMemcpyHostToDevice);
 }
 
 int p4g_memcpy_dtoh(void* dst, const void* src, size_t bytes) {
-    HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
-    return 0;
+    return copy_sync(dst, src, bytes, hipMemcpyDeviceToHost);
 }
 
 int p4g_memcpy_dtod(void* dst, const void* src, size_t bytes) {
-    HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
-    return 0;
+    return copy_sync(dst, src, bytes, hipMemcpyDeviceToDevice);
 }
 
 const char* p4g_version(void) { return "p4g 0.1 (gfx950)"; }

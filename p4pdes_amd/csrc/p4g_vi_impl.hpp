@@ -81,11 +81,13 @@ static int vi_operator(p4g_solver_s* s, ViRun& R) {
     if (s->pctype != PCT_MG) return 0;
     P4G_TRY(galerkin_levels(s));
     for (int l = 0; l < L; ++l) e.g27[l] = s->g27[l];
-    std::vector<double> Ainv;
-    P4G_TRY(coarse_factor(s->lv[0], s->g27[0], Ainv));
-    P4G_TRY(e.set_coarse_inverse(Ainv));
-    if (s->smoother == SM_CHEB)
-        for (int l = 1; l <= L; ++l) P4G_TRY(estimate_level(s, l));
+    if (L > 0) {
+        std::vector<double> Ainv;
+        P4G_TRY(coarse_factor(s->lv[0], s->g27[0], Ainv));
+        P4G_TRY(e.set_coarse_inverse(Ainv));
+    }
+    if (s->smoother == SM_CHEB)   // one level: the level smoother is the preconditioner
+        for (int l = L > 0 ? 1 : 0; l <= L; ++l) P4G_TRY(estimate_level(s, l));
     return 0;
 }
 

@@ -267,13 +267,23 @@ def test_fcycle_reaches_discretization_accuracy_in_one_cycle():
     assert ef < 0.3 * ev
 
 
-def test_fcycle_one_level_is_the_coarse_solve():
-    # with one level the F-cycle is the direct solve
-    args = ("-fsh_dim 3 -fsh_problem manusin -da_refine 2 -pc_type mg -pc_mg_levels 1 "
-            "-pc_mg_type full")
+def test_one_level_multigrid_is_the_smoother():
+    # PCMGSetLevels makes a single level a smoother level (not the coarse
+    # direct solve): every cycle type applies the level smoother once, a
+    # polynomial in B A that does not invert A
+    base = "-fsh_dim 3 -fsh_problem manusin -da_refine 2 -pc_type mg -pc_mg_levels 1 -mg_levels_pc_type jacobi"
     u = _smooth_field(9, 3)
-    x = O.pc_apply(args, O.mat_mult(args, u))
-    assert np.max(np.abs(x - u)) <= 1e-12 * np.max(np.abs(u))
+    b = O.mat_mult(base, u)
+    xs = [O.pc_apply(base + t, b) for t in ("", " -pc_mg_type full", " -pc_mg_type kaskade")]
+    for x in xs[1:]:
+        assert np.max(np.abs(x - xs[0])) <= 1e-14 * np.max(np.abs(xs[0]))
+    assert np.max(np.abs(xs[0] - u)) > 1e-3 * np.max(np.abs(u))
+    # two Chebyshev(2)/Jacobi steps from zero: x = p(D^-1 A) D^-1 b with the
+    # smoother's bounds; the one-step case is the scaled Jacobi step
+    one = O.pc_apply(base + " -mg_levels_ksp_max_it 1", b)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        ratio = one / O.pc_apply(base + " -mg_levels_ksp_type richardson -mg_levels_ksp_max_it 1", b)
+    assert np.allclose(ratio[np.isfinite(ratio)], ratio[np.isfinite(ratio)][0], rtol=1e-12)
 
 
 def test_fcycle_matches_vcycle_on_two_levels_after_the_coarse_start():
