@@ -640,7 +640,8 @@ static int parse_pc(const char *s, int def) {
     if (!strcmp(s, "none")) return PC_NONE;
     if (!strcmp(s, "jacobi")) return PC_JACOBI;
     if (!strcmp(s, "sor")) return PC_SOR;
-    if (!strcmp(s, "ilu")) return PC_ILU;
+    /* ICC(0) of this symmetric operator is ILU(0) (U = D L^T) up to rounding */
+    if (!strcmp(s, "ilu") || !strcmp(s, "icc")) return PC_ILU;
     if (!strcmp(s, "lu") || !strcmp(s, "cholesky") || !strcmp(s, "redundant")) return PC_LU;
     if (!strcmp(s, "mg")) return PC_MG;
     return -1;

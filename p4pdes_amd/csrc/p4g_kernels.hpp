@@ -1877,6 +1877,59 @@ k_sor_plane(SorOp o, int p, int backward, const T* __restrict__ b, T* __restrict
     x[q] = T(s * (1.0 / o.d));
 }
 
+// ILU(0) of the constant 3/5/7-point operator (PETSc's PCILU / PCICC with
+// no fill): the factor keeps A's pattern and A's off-diagonal entries; only
+// the pivots u change (computed on the host in lexicographic order).  The
+// solves run on the same hyperplanes as the SOR sweeps: forward
+// y_q = r_q - sum_{lower m} (a_qm / u_m) y_m, backward
+// z_q = (y_q - sum_{upper m} a_qm z_m) / u_q, each sum in column order.
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_ilu_plane(SorOp o, const double* __restrict__ u, int p, int backward, const T* __restrict__ r,
+            T* __restrict__ z) {
+    const long t = blockIdx.x * 256L + threadIdx.x;
+    int i, j, k;
+    if (o.dim == 3) {
+        if (t >= (long)o.my * o.mz) return;
+        j = (int)(t % o.my);
+        k = (int)(t / o.my);
+    } else if (o.dim == 2) {
+        if (t >= o.my) return;
+        j = (int)t;
+        k = 0;
+    } else {
+        if (t > 0) return;
+        j = 0;
+        k = 0;
+    }
+    i = p - j - k;
+    if (i < 0 || i >= o.mx) return;
+    const long sy = o.mx, sz = (long)o.mx * o.my;
+    const long q = ((long)k * o.my + j) * o.mx + i;
+    auto bnd = [&](int ii, int jj, int kk) {
+        return ii == 0 || ii == o.mx - 1 || (o.dim >= 2 && (jj == 0 || jj == o.my - 1)) ||
+               (o.dim == 3 && (kk == 0 || kk == o.mz - 1));
+    };
+    const bool bq = bnd(i, j, k);
+    if (!backward) {
+        double s = double(r[q]);
+        if (!bq) {   // a = -s toward interior neighbours
+            if (o.dim == 3 && !bnd(i, j, k - 1)) s -= ((-o.sz) / u[q - sz]) * double(z[q - sz]);
+            if (o.dim >= 2 && !bnd(i, j - 1, k)) s -= ((-o.sy) / u[q - sy]) * double(z[q - sy]);
+            if (!bnd(i - 1, j, k)) s -= ((-o.sx) / u[q - 1]) * double(z[q - 1]);
+        }
+        z[q] = T(s);
+        return;
+    }
+    double s = double(z[q]);
+    if (!bq) {
+        if (!bnd(i + 1, j, k)) s -= (-o.sx) * double(z[q + 1]);
+        if (o.dim >= 2 && !bnd(i, j + 1, k)) s -= (-o.sy) * double(z[q + sy]);
+        if (o.dim == 3 && !bnd(i, j, k + 1)) s -= (-o.sz) * double(z[q + sz]);
+    }
+    z[q] = T(s / u[q]);
+}
+
 // y = a p + c q + e z (p or q may be null: 0)
 template <typename T>
 __global__ void __launch_bounds__(256)

@@ -48,7 +48,7 @@ using namespace p4g;
 
 namespace p4g {
 
-enum { PCT_MG = 0, PCT_JACOBI, PCT_NONE };
+enum { PCT_MG = 0, PCT_JACOBI, PCT_NONE, PCT_ILU };
 enum { SM_CHEB = 0, SM_RICH };
 enum { NT_PRECOND = 0, NT_UNPRECOND, NT_NATURAL };
 
@@ -102,6 +102,48 @@ static int sor_run(const LevelInfo& L, const double* g27, const T* r, T* z, hipS
     const int gb = (int)((nt + 255) / 256);
     for (int p = 0; p < np; ++p) k_sor_plane<T><<<gb, 256, 0, st>>>(o, p, 0, r, z);
     for (int p = np - 1; p >= 0; --p) k_sor_plane<T><<<gb, 256, 0, st>>>(o, p, 1, r, z);
+    LAUNCH_CHECK();
+    return 0;
+}
+
+// ILU(0) pivots of a constant-operator level in lexicographic order, the
+// IKJ arithmetic of a row: u_q = a_qq - l_qm a_mq over the lower interior
+// neighbours m in column order, l_qm = a_qm / u_m
+static void ilu_pivots(const LevelInfo& L, std::vector<double>& u) {
+    u.assign(L.n, 0.0);
+    const long sy = L.mx, sz = (long)L.mx * L.my;
+    auto bnd = [&](int i, int j, int k) {
+        return i == 0 || i == L.mx - 1 || (L.dim >= 2 && (j == 0 || j == L.my - 1)) ||
+               (L.dim == 3 && (k == 0 || k == L.mz - 1));
+    };
+    for (int k = 0; k < L.mz; ++k)
+        for (int j = 0; j < L.my; ++j)
+            for (int i = 0; i < L.mx; ++i) {
+                const long q = ((long)k * L.my + j) * L.mx + i;
+                if (bnd(i, j, k)) { u[q] = L.db; continue; }
+                double v = L.d;
+                if (L.dim == 3 && !bnd(i, j, k - 1)) v -= ((-L.sz) / u[q - sz]) * (-L.sz);
+                if (L.dim >= 2 && !bnd(i, j - 1, k)) v -= ((-L.sy) / u[q - sy]) * (-L.sy);
+                if (!bnd(i - 1, j, k)) v -= ((-L.sx) / u[q - 1]) * (-L.sx);
+                u[q] = v;
+            }
+}
+
+// z = (L U)^-1 r with the ILU(0) factor of a constant-operator level
+template <typename T>
+static int ilu_run(const LevelInfo& L, const double* u, const T* r, T* z, hipStream_t st) {
+    if (L.var || L.gal || !u) return set_error(PETSC_ERR_SUP, "ILU(0) is implemented for constant 3/5/7-point levels");
+    SorOp o;
+    o.dim = L.dim; o.mx = L.mx; o.my = L.my; o.mz = L.mz; o.n = L.n;
+    o.d = L.d; o.db = L.db; o.sx = L.sx; o.sy = L.sy; o.sz = L.sz;
+    o.g = nullptr;
+    o.nblk = 1;
+    o.blk[0] = 0;
+    const int np = (L.mx - 1) + (L.dim >= 2 ? L.my - 1 : 0) + (L.dim == 3 ? L.mz - 1 : 0) + 1;
+    const long nt = L.dim == 3 ? (long)L.my * L.mz : (L.dim == 2 ? (long)L.my : 1L);
+    const int gb = (int)((nt + 255) / 256);
+    for (int p = 0; p < np; ++p) k_ilu_plane<T><<<gb, 256, 0, st>>>(o, u, p, 0, r, z);
+    for (int p = np - 1; p >= 0; --p) k_ilu_plane<T><<<gb, 256, 0, st>>>(o, u, p, 1, r, z);
     LAUNCH_CHECK();
     return 0;
 }
@@ -288,7 +330,9 @@ struct Engine {
     int mcycles = 1;        // -pc_mg_multiplicative_cycles
     bool kaskade = false;   // -pc_mg_type kaskade
     bool full_temps = false;   // four smoother temporaries on every level
-    bool sor = false;          // level smoother preconditioned by SSOR (-mg_levels_pc_type sor)
+    bool sor = false;          // explicit smoother path: SSOR or ILU(0) level preconditioner
+    bool lilu = false;         // the level preconditioner is ILU(0) (-mg_levels_pc_type ilu|icc)
+    std::vector<const double*> ilu_u;   // per level: ILU(0) pivots (device) or null
     // Krylov vectors on the finest level
     T *r = nullptr, *z = nullptr, *pa = nullptr, *pb = nullptr, *w = nullptr, *xbuf = nullptr,
       *bbuf = nullptr, *r2 = nullptr;
@@ -591,7 +635,8 @@ struct Engine {
                 P4G_TRY(stencil(l, SrcArr<T>{pm1}, OutResid<T, 0>{b, r}, P4G_K_SMOOTH, 3, nullptr, false));
                 rr = r;
             }
-            P4G_TRY(sor_run<T>(L, g27.size() > (size_t)l ? g27[l] : nullptr, rr, z, st));
+            if (lilu) P4G_TRY(ilu_run<T>(L, ilu_u.size() > (size_t)l ? ilu_u[l] : nullptr, rr, z, st));
+            else P4G_TRY(sor_run<T>(L, g27.size() > (size_t)l ? g27[l] : nullptr, rr, z, st));
             T* dst = out;
             if (m < nu)
                 for (int q = 1; q < 4; ++q)
@@ -922,6 +967,15 @@ struct Engine {
         P4G_TRY(pend_flush());
         // one level: PCMGSetLevels makes it a smoother level, applied once
         if (pctype == PCT_MG) return smooth(0, rr, nullptr, zz, true, nb);
+        if (pctype == PCT_ILU) {
+            const long n = F.n;
+            hipStream_t s = cx->stream;
+            P4G_TRY(ilu_run<T>(F, ilu_u.size() > (size_t)(nlev - 1) ? ilu_u[nlev - 1] : nullptr, rr, zz, s));
+            *nb = vblocks(n);
+            return cx->launch(P4G_K_OTHER, true, 2.0 * n * sizeof(T), [&] {
+                k_dot2<T><<<vblocks(n), 256, 0, s>>>(n, zz, rr, cx->part);
+            });
+        }
         const double c = (pctype == PCT_JACOBI) ? 1.0 / F.d : 1.0;
         if (pctype == PCT_JACOBI && (F.var || F.d != F.db))
             return set_error(PETSC_ERR_SUP, "-pc_type jacobi on the device needs a constant diagonal");
@@ -1227,7 +1281,9 @@ struct p4g_solver_s {
     bool ksp_rich = false;            // -ksp_type richardson
     double rich_outer_scale = 1.0;    // -ksp_richardson_scale
     bool monitor = false;             // -ksp_monitor (PETSc's generic Richardson loop)
-    bool sor = false;                 // -mg_levels_pc_type sor: SSOR-preconditioned level smoothers
+    bool sor = false;                 // -mg_levels_pc_type sor|ilu|icc: explicit level smoothers
+    bool lilu = false;                //   ... with ILU(0) instead of SSOR
+    std::vector<double*> ilu_u;       // per level: ILU(0) pivots (device) or null
     int sor_blocks = 1;               // -p4g_sor_blocks: local sweeps of that many ranks
     // reduced-space VI Newton (p4g_vi_solve): SNES options and state
     double snes_rtol = 1e-8, snes_atol = 1e-50, snes_stol = 1e-8, snes_divtol = 1e4;
@@ -1494,7 +1550,8 @@ static int estimate_level(p4g_solver_s* s, int l) {
     auto lupdate = [&](double alpha) -> int {
         if (s->sor) {
             if (alpha != 0.0) k_axpy<double><<<gb, 256, 0, st>>>(n, -alpha, w, r);
-            P4G_TRY(sor_run<double>(L, s->g27.size() > (size_t)l ? s->g27[l] : nullptr, r, z, st));
+            if (s->lilu) P4G_TRY(ilu_run<double>(L, s->ilu_u.size() > (size_t)l ? s->ilu_u[l] : nullptr, r, z, st));
+            else P4G_TRY(sor_run<double>(L, s->g27.size() > (size_t)l ? s->g27[l] : nullptr, r, z, st));
             k_dot2<double><<<gb, 256, 0, st>>>(n, z, r, s->cx.part);
         } else {
             k_lanczos_update<<<gb, 256, 0, st>>>(n, alpha, r, w, idv, idc, z, s->cx.part);
@@ -1573,9 +1630,7 @@ static int configure(p4g_solver_s* s) {
     if (pt == "mg") s->pctype = PCT_MG;
     else if (pt == "jacobi") s->pctype = PCT_JACOBI;
     else if (pt == "none") s->pctype = PCT_NONE;
-    else if (pt.empty())
-        return set_error(PETSC_ERR_SUP, "the device path needs -pc_type mg (or jacobi/none); PETSc's "
-                                        "default ILU is not implemented");
+    else if (pt.empty() || pt == "ilu" || pt == "icc") s->pctype = PCT_ILU;   // PETSc's serial default
     else return set_error(PETSC_ERR_SUP, "-pc_type %s is not implemented on the device", pt.c_str());
     for (const char** u = kUnsupported; *u; ++u)
         if (o.has(*u)) return set_error(PETSC_ERR_SUP, "-%s is not implemented on the device", *u);
@@ -1615,6 +1670,11 @@ static int configure(p4g_solver_s* s) {
     const std::string lp = o.gets("mg_levels_pc_type", s->pctype == PCT_MG ? "sor" : "jacobi");
     if (lp == "jacobi") s->jacobi = 1;
     else if (lp == "none") s->jacobi = 0;
+    else if ((lp == "ilu" || lp == "icc") && s->pctype == PCT_MG) {   // ILU(0) = IC(0) for this operator
+        s->sor = true;
+        s->lilu = true;
+        s->jacobi = 1;
+    }
     else if (lp == "sor" && s->pctype == PCT_MG) {   // PETSc's default: symmetric sweep, omega 1
         s->sor = true;
         s->jacobi = 1;
@@ -1678,6 +1738,12 @@ static int configure(p4g_solver_s* s) {
         }
     if (s->galerkin && s->mixed)
         return set_error(PETSC_ERR_SUP, "-pc_mg_galerkin with -p4g_precision mixed is not implemented on the device");
+    if ((s->pctype == PCT_ILU || s->lilu) && o.geti("pc_factor_levels", 0) + o.geti("mg_levels_pc_factor_levels", 0) != 0)
+        return set_error(PETSC_ERR_SUP, "the device ILU is ILU(0): -pc_factor_levels 0");
+    if ((s->pctype == PCT_ILU || s->lilu) && s->galerkin)
+        return set_error(PETSC_ERR_SUP, "ILU(0) of Galerkin coarse levels is not implemented on the device");
+    if (s->pctype == PCT_ILU && (s->mixed || !s->da->kcell.empty()))
+        return set_error(PETSC_ERR_SUP, "-pc_type ilu needs FP64 and a constant-coefficient operator on the device");
     if (s->sor && (s->mixed || !s->da->kcell.empty()))
         return set_error(PETSC_ERR_SUP, "-mg_levels_pc_type sor needs FP64 and a constant-coefficient operator "
                                         "on the device");
@@ -1825,6 +1891,20 @@ int solver_setup(p4g_solver_s* s) {
     const double t0 = now_ms();
     P4G_TRY(build_levels(s));
     sor_splits(s);
+    // ILU(0) pivots: the finest level for -pc_type ilu, the smoothing levels
+    // for -mg_levels_pc_type ilu (computed once on the host, in the factor's
+    // own row order)
+    s->ilu_u.assign(s->nlev, nullptr);
+    for (int l = 0; l < s->nlev; ++l) {
+        const bool need = (s->pctype == PCT_ILU && l == s->nlev - 1) || (s->lilu && (l > 0 || s->nlev == 1));
+        if (!need) continue;
+        const LevelInfo& L = s->lv[l];
+        if (L.var || L.gal) return set_error(PETSC_ERR_SUP, "ILU(0) is implemented for constant 3/5/7-point levels");
+        std::vector<double> u;
+        ilu_pivots(L, u);
+        P4G_TRY(dalloc(s, (void**)&s->ilu_u[l], L.n * sizeof(double)));
+        HIP_TRY(hipMemcpy(s->ilu_u[l], u.data(), L.n * sizeof(double), hipMemcpyHostToDevice));
+    }
     if (!s->cx.stream) HIP_TRY(hipStreamCreate(&s->cx.stream));
     // partial-sum capacity: the largest grid of blocks of any fused pass
     const LevelInfo& F = s->lv.back();
@@ -1867,6 +1947,8 @@ int solver_setup(p4g_solver_s* s) {
         e->mcycles = s->mcycles;
         e->full_temps = s->kaskade || s->mcycles > 1 || (s->ksp_rich && s->pctype == PCT_MG);
         e->sor = s->sor;
+        e->lilu = s->lilu;
+        e->ilu_u.assign(s->ilu_u.begin(), s->ilu_u.end());
     };
     if (F.var) {
         const size_t nc = F.kcell.size();

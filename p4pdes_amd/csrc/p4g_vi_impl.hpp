@@ -189,6 +189,9 @@ static int vi_solve(p4g_solver_s* s, const double* xl, const double* xu, double*
     if (s->mixed) return set_error(PETSC_ERR_SUP, "p4g_vi_solve: mixed precision is not implemented");
     if (!da->kcell.empty()) return set_error(PETSC_ERR_SUP, "p4g_vi_solve: variable coefficients are not implemented");
     if (s->fmg) return set_error(PETSC_ERR_SUP, "p4g_vi_solve: -pc_mg_type full is not implemented");
+    if (s->pctype == PCT_ILU || s->lilu)
+        return set_error(PETSC_ERR_SUP, "p4g_vi_solve: ILU of the reduced operator is not implemented (PETSc's "
+                                        "default PC here); use -pc_type mg");
     if (!s->vi_mode) {
         if (s->ready)
             return set_error(PETSC_ERR_ORDER, "p4g_vi_solve: the solver was set up for linear solves; create a "

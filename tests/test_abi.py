@@ -61,15 +61,16 @@ def test_da_from_options():
 
 
 @pytest.mark.parametrize("opts,fragment", [
-    ("-pc_type ilu", "ilu"),
+    ("-pc_type ilu -pc_factor_levels 1", "ILU(0)"),
+    ("-pc_type lu", "lu"),
+    ("-pc_type mg -pc_mg_levels 3 -mg_levels_pc_type ilu -pc_mg_galerkin", "Galerkin"),
     ("-pc_type mg -pc_mg_levels 3 -mg_levels_pc_sor_omega 1.5", "omega"),
     ("-pc_type mg -pc_mg_levels 3 -mg_levels_pc_sor_forward", "sor_forward"),
     ("-pc_type mg -pc_mg_levels 3 -p4g_precision mixed", "sor"),
-    ("-pc_type mg -pc_mg_levels 3 -mg_levels_pc_type icc", "icc"),
+    ("-pc_type mg -pc_mg_levels 3 -mg_levels_pc_type asm", "asm"),
     ("-pc_type mg -pc_mg_levels 3 -mg_levels_pc_type jacobi -pc_mg_galerkin -p4g_precision mixed", "galerkin"),
     ("-ksp_type gmres -pc_type mg", "gmres"),
     ("-pc_type mg -pc_mg_levels 3 -mg_levels_pc_type jacobi -pc_mg_type additive", "additive"),
-    ("", "pc_type mg"),
 ])
 def test_unsupported_options_rejected(opts, fragment):
     L = capi.lib()

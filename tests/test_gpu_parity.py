@@ -173,11 +173,12 @@ def _golden_fish():
     return {t["test"]: t for t in json.load(open(os.path.join(here, "golden", "fish_golden.json")))["tests"]}
 
 
-@pytest.mark.parametrize("num", [1, 3, 4, 7])
-def test_golden_fish_with_default_sor_smoother(num):
-    # the reference's own commands (PETSc's default Chebyshev/SOR level
-    # smoother), byte for byte against c/ch6/output/fish.testN; the 2-rank
-    # tests with the local SOR sweeps of two processes
+@pytest.mark.parametrize("num", [1, 2, 3, 4, 5, 6, 7, 8])
+def test_golden_fish_on_device(num):
+    # the reference's own commands (PETSc's defaults: ILU(0) as the KSP's PC,
+    # Chebyshev/SOR level smoothing), byte for byte against
+    # c/ch6/output/fish.testN; the 2-rank tests with the local SOR sweeps of
+    # two processes
     t = _golden_fish()[num]
     args = t["options"] + (" -p4g_sor_blocks %d" % t["nprocs"] if t["nprocs"] > 1 else "")
     print("golden fish", num, args, flush=True)
@@ -198,6 +199,16 @@ SOR_CASES = {
                                   "-mg_levels_ksp_max_it 4 -ksp_type richardson -ksp_monitor -ksp_rtol 1e-8"),
     "1d_sor": "-fsh_dim 1 -fsh_problem manupoly -da_refine 7 -pc_type mg -ksp_rtol 1e-12",
 }
+
+
+SOR_CASES.update({
+    "3d_cg_ilu": "-fsh_dim 3 -fsh_problem manusin -da_refine 4 -ksp_rtol 1e-10",
+    "2d_cg_icc_aniso": "-fsh_dim 2 -fsh_problem manupoly -fsh_cx 100 -da_refine 6 -pc_type icc -ksp_rtol 1e-10",
+    "2d_mg_cheb_icc_aniso": ("-fsh_dim 2 -fsh_problem manupoly -fsh_cy 1000 -da_refine 6 -pc_type mg "
+                             "-mg_levels_pc_type icc -ksp_rtol 1e-10"),
+    "3d_mg_rich_ilu": ("-fsh_dim 3 -fsh_problem manusin -da_refine 4 -pc_type mg -mg_levels_ksp_type richardson "
+                       "-mg_levels_pc_type ilu -mg_levels_ksp_max_it 1 -ksp_rtol 1e-10"),
+})
 
 
 @pytest.mark.parametrize("name", sorted(SOR_CASES))
