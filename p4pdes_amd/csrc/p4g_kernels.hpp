@@ -1930,6 +1930,70 @@ k_ilu_plane(SorOp o, const double* __restrict__ u, int p, int backward, const T*
     z[q] = T(s / u[q]);
 }
 
+// ILU(0) of an operator in 27-point form whose nonzeros lie on the 7-point
+// pattern (the null-row operator of a reduced VI step): no fill, so the
+// factor is A's pattern with new pivots.  Pivots, plane by plane (i+j+k):
+// u_q = a_qq - sum_{lower m} (a_qm / u_m) a_mq in column order; a null row
+// keeps u_q = 0 and its solves give 0.
+__device__ __forceinline__ bool ilu27_site(const SorOp& o, long t, int p, int& i, int& j, int& k) {
+    if (o.dim == 3) {
+        if (t >= (long)o.my * o.mz) return false;
+        j = (int)(t % o.my);
+        k = (int)(t / o.my);
+    } else if (o.dim == 2) {
+        if (t >= o.my) return false;
+        j = (int)t;
+        k = 0;
+    } else {
+        if (t > 0) return false;
+        j = 0;
+        k = 0;
+    }
+    i = p - j - k;
+    return i >= 0 && i < o.mx;
+}
+
+__global__ void __launch_bounds__(256) k_ilu27_pivot_plane(SorOp o, double* __restrict__ u, int p) {
+    int i, j, k;
+    if (!ilu27_site(o, blockIdx.x * 256L + threadIdx.x, p, i, j, k)) return;
+    const long sy = o.mx, sz = (long)o.mx * o.my;
+    const long q = ((long)k * o.my + j) * o.mx + i;
+    double v = o.g[13 * o.n + q];
+    if (v != 0.0) {
+        // lower neighbours in column order: k-1 (c 4), j-1 (c 10), i-1 (c 12);
+        // a_mq is row m's coefficient toward q (c 22, 16, 14)
+        if (k > 0 && o.g[4 * o.n + q] != 0.0) v -= (o.g[4 * o.n + q] / u[q - sz]) * o.g[22 * o.n + q - sz];
+        if (j > 0 && o.g[10 * o.n + q] != 0.0) v -= (o.g[10 * o.n + q] / u[q - sy]) * o.g[16 * o.n + q - sy];
+        if (i > 0 && o.g[12 * o.n + q] != 0.0) v -= (o.g[12 * o.n + q] / u[q - 1]) * o.g[14 * o.n + q - 1];
+    }
+    u[q] = v;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_ilu27_plane(SorOp o, const double* __restrict__ u, int p, int backward, const T* __restrict__ r,
+              T* __restrict__ z) {
+    int i, j, k;
+    if (!ilu27_site(o, blockIdx.x * 256L + threadIdx.x, p, i, j, k)) return;
+    const long sy = o.mx, sz = (long)o.mx * o.my;
+    const long q = ((long)k * o.my + j) * o.mx + i;
+    const double uq = u[q];
+    if (uq == 0.0) { z[q] = T(0); return; }
+    if (!backward) {
+        double s = double(r[q]);
+        if (k > 0 && o.g[4 * o.n + q] != 0.0) s -= (o.g[4 * o.n + q] / u[q - sz]) * double(z[q - sz]);
+        if (j > 0 && o.g[10 * o.n + q] != 0.0) s -= (o.g[10 * o.n + q] / u[q - sy]) * double(z[q - sy]);
+        if (i > 0 && o.g[12 * o.n + q] != 0.0) s -= (o.g[12 * o.n + q] / u[q - 1]) * double(z[q - 1]);
+        z[q] = T(s);
+        return;
+    }
+    double s = double(z[q]);
+    if (i + 1 < o.mx && o.g[14 * o.n + q] != 0.0) s -= o.g[14 * o.n + q] * double(z[q + 1]);
+    if (j + 1 < o.my && o.g[16 * o.n + q] != 0.0) s -= o.g[16 * o.n + q] * double(z[q + sy]);
+    if (k + 1 < o.mz && o.g[22 * o.n + q] != 0.0) s -= o.g[22 * o.n + q] * double(z[q + sz]);
+    z[q] = T(s / uq);
+}
+
 // y = a p + c q + e z (p or q may be null: 0)
 template <typename T>
 __global__ void __launch_bounds__(256)

@@ -129,6 +129,39 @@ static void ilu_pivots(const LevelInfo& L, std::vector<double>& u) {
             }
 }
 
+// ILU(0) of a 27-point-form level with 7-point nonzeros (reduced VI steps):
+// pivots into u, then z = (L U)^-1 r
+static SorOp sor_op27(const LevelInfo& L, const double* g27) {
+    SorOp o;
+    o.dim = L.dim; o.mx = L.mx; o.my = L.my; o.mz = L.mz; o.n = L.n;
+    o.d = o.db = o.sx = o.sy = o.sz = 0.0;
+    o.g = g27;
+    o.nblk = 1;
+    o.blk[0] = 0;
+    return o;
+}
+static int ilu27_pivots(const LevelInfo& L, const double* g27, double* u, hipStream_t st) {
+    const SorOp o = sor_op27(L, g27);
+    const int np = (L.mx - 1) + (L.dim >= 2 ? L.my - 1 : 0) + (L.dim == 3 ? L.mz - 1 : 0) + 1;
+    const long nt = L.dim == 3 ? (long)L.my * L.mz : (L.dim == 2 ? (long)L.my : 1L);
+    const int gb = (int)((nt + 255) / 256);
+    for (int p = 0; p < np; ++p) k_ilu27_pivot_plane<<<gb, 256, 0, st>>>(o, u, p);
+    LAUNCH_CHECK();
+    return 0;
+}
+template <typename T>
+static int ilu27_run(const LevelInfo& L, const double* g27, const double* u, const T* r, T* z, hipStream_t st) {
+    if (!g27 || !u) return set_error(PETSC_ERR_LIB, "internal: ILU(0) of a 27-point level without data");
+    const SorOp o = sor_op27(L, g27);
+    const int np = (L.mx - 1) + (L.dim >= 2 ? L.my - 1 : 0) + (L.dim == 3 ? L.mz - 1 : 0) + 1;
+    const long nt = L.dim == 3 ? (long)L.my * L.mz : (L.dim == 2 ? (long)L.my : 1L);
+    const int gb = (int)((nt + 255) / 256);
+    for (int p = 0; p < np; ++p) k_ilu27_plane<T><<<gb, 256, 0, st>>>(o, u, p, 0, r, z);
+    for (int p = np - 1; p >= 0; --p) k_ilu27_plane<T><<<gb, 256, 0, st>>>(o, u, p, 1, r, z);
+    LAUNCH_CHECK();
+    return 0;
+}
+
 // z = (L U)^-1 r with the ILU(0) factor of a constant-operator level
 template <typename T>
 static int ilu_run(const LevelInfo& L, const double* u, const T* r, T* z, hipStream_t st) {
@@ -970,7 +1003,9 @@ struct Engine {
         if (pctype == PCT_ILU) {
             const long n = F.n;
             hipStream_t s = cx->stream;
-            P4G_TRY(ilu_run<T>(F, ilu_u.size() > (size_t)(nlev - 1) ? ilu_u[nlev - 1] : nullptr, rr, zz, s));
+            const double* uf = ilu_u.size() > (size_t)(nlev - 1) ? ilu_u[nlev - 1] : nullptr;
+            if (F.gal) P4G_TRY(ilu27_run<T>(F, g27.size() > (size_t)(nlev - 1) ? g27[nlev - 1] : nullptr, uf, rr, zz, s));
+            else P4G_TRY(ilu_run<T>(F, uf, rr, zz, s));
             *nb = vblocks(n);
             return cx->launch(P4G_K_OTHER, true, 2.0 * n * sizeof(T), [&] {
                 k_dot2<T><<<vblocks(n), 256, 0, s>>>(n, zz, rr, cx->part);

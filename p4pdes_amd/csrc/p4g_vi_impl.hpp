@@ -78,6 +78,7 @@ static int vi_operator(p4g_solver_s* s, ViRun& R) {
     s->g27[L] = s->vi_g27;
     e.g27[L] = s->vi_g27;
     e.vidiag = s->vi_g27 + 13 * R.n;
+    if (s->pctype == PCT_ILU) return ilu27_pivots(s->lv[L], s->vi_g27, s->ilu_u[L], R.st);   // PETSc's default PC
     if (s->pctype != PCT_MG) return 0;
     P4G_TRY(galerkin_levels(s));
     for (int l = 0; l < L; ++l) e.g27[l] = s->g27[l];
@@ -189,9 +190,8 @@ static int vi_solve(p4g_solver_s* s, const double* xl, const double* xu, double*
     if (s->mixed) return set_error(PETSC_ERR_SUP, "p4g_vi_solve: mixed precision is not implemented");
     if (!da->kcell.empty()) return set_error(PETSC_ERR_SUP, "p4g_vi_solve: variable coefficients are not implemented");
     if (s->fmg) return set_error(PETSC_ERR_SUP, "p4g_vi_solve: -pc_mg_type full is not implemented");
-    if (s->pctype == PCT_ILU || s->lilu)
-        return set_error(PETSC_ERR_SUP, "p4g_vi_solve: ILU of the reduced operator is not implemented (PETSc's "
-                                        "default PC here); use -pc_type mg");
+    if (s->lilu)
+        return set_error(PETSC_ERR_SUP, "p4g_vi_solve: ILU smoothing of the Galerkin levels is not implemented");
     if (!s->vi_mode) {
         if (s->ready)
             return set_error(PETSC_ERR_ORDER, "p4g_vi_solve: the solver was set up for linear solves; create a "

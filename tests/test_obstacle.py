@@ -194,6 +194,8 @@ CASES = {
     "seq5_to_513": "-da_refine 3 -snes_grid_sequence 5 -pc_type mg -mg_levels_pc_type jacobi",
     "129_max_it": "-da_refine 6 -pc_type mg -mg_levels_pc_type jacobi -snes_max_it 5",
     "65_sor": "-da_refine 5 -pc_type mg",
+    "65_ilu": "-da_refine 5 -ksp_rtol 1e-8",
+    "seq3_ilu": "-snes_grid_sequence 3",
     "seq4_sor_w": "-snes_grid_sequence 4 -pc_type mg -pc_mg_cycle_type w",
 }
 SHOW = " -snes_monitor_short -ksp_converged_reason -snes_converged_reason"
@@ -238,6 +240,18 @@ def test_obstacle_device_parity(case):
     assert np.max(np.abs(u - uo)) <= 1e-9 * np.max(np.abs(uo))
     assert res.active == reso.active
     assert text.splitlines()[-1] == texto.splitlines()[-1]   # the error line
+
+
+@pytest.mark.gpu
+def test_obstacle_device_golden_test1_default_ilu():
+    # obstacle.test1 as the reference runs it: CG with PETSc's default ILU(0)
+    # on each reduced Newton system, byte for byte
+    from p4pdes_amd import capi
+    t = GOLDEN[1]
+    print("obstacle golden 1 default ILU", flush=True)
+    rc, text, res, _ = capi.obstacle_run(t["options"])
+    assert rc == 0, text
+    assert text == t["expected"]
 
 
 @pytest.mark.gpu
