@@ -1994,6 +1994,21 @@ k_ilu27_plane(SorOp o, const double* __restrict__ u, int p, int backward, const 
     z[q] = T(s / uq);
 }
 
+// z = D^-1 r of a constant-operator level (boundary rows db, interior d)
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_jacobi_const(int dim, int mx, int my, int mz, double d, double db, const T* __restrict__ r, T* __restrict__ z) {
+    const long n = (long)mx * my * mz;
+    for (long q = blockIdx.x * 256L + threadIdx.x; q < n; q += (long)gridDim.x * 256L) {
+        const int i = (int)(q % mx);
+        const long t = q / mx;
+        const int j = (int)(t % my), k = (int)(t / my);
+        const bool bnd = i == 0 || i == mx - 1 || (dim >= 2 && (j == 0 || j == my - 1)) ||
+                         (dim == 3 && (k == 0 || k == mz - 1));
+        z[q] = T(double(r[q]) * (1.0 / (bnd ? db : d)));
+    }
+}
+
 // y = a p + c q + e z (p or q may be null: 0)
 template <typename T>
 __global__ void __launch_bounds__(256)

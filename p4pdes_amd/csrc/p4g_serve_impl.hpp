@@ -121,6 +121,8 @@ static int serve_supported(p4g_solver_s* s) {
         return set_error(PETSC_ERR_SUP, "serving runs rediscretized multiplicative V-cycles (no -pc_mg_galerkin, "
                                         "-pc_mg_type full|kaskade or -pc_mg_multiplicative_cycles)");
     if (s->sor) return set_error(PETSC_ERR_SUP, "serving runs Jacobi-preconditioned smoothers (no SOR)");
+    if (s->coarse_its > 0)
+        return set_error(PETSC_ERR_SUP, "serving solves the coarsest level directly (no -mg_coarse_ksp_type cg)");
     if (s->ksp_rich || s->vi_mode)
         return set_error(PETSC_ERR_SUP, "serving runs CG solves (no -ksp_type richardson or VI solver)");
     return 0;

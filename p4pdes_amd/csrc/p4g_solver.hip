@@ -365,6 +365,8 @@ struct Engine {
     bool full_temps = false;   // four smoother temporaries on every level
     bool sor = false;          // explicit smoother path: SSOR or ILU(0) level preconditioner
     bool lilu = false;         // the level preconditioner is ILU(0) (-mg_levels_pc_type ilu|icc)
+    int coarse_its = 0;        // coarsest level: >0 PCG iterations (-mg_coarse_ksp_type cg), else the dense solve
+    int coarse_pc = 0;         //   its PC: 0 none, 1 Jacobi, 2 ILU(0)
     std::vector<const double*> ilu_u;   // per level: ILU(0) pivots (device) or null
     // Krylov vectors on the finest level
     T *r = nullptr, *z = nullptr, *pa = nullptr, *pb = nullptr, *w = nullptr, *xbuf = nullptr,
@@ -412,9 +414,10 @@ struct Engine {
             Dev& d = dv[l];
             if (pctype == PCT_MG) {
                 if (l < nlev - 1) { P4G_TRY(alloc(&d.x, L.n)); P4G_TRY(alloc(&d.b, L.n)); }
-                if (l > 0 || nlev == 1) {   // smoothing levels (one level: the smoother itself)
+                if (l > 0 || nlev == 1 || coarse_its > 0) {   // smoothing levels, an iterative coarse solve
                     P4G_TRY(alloc(&d.r, L.n));
-                    const int nt = sor ? 5 : (!vi && !full_temps && nu == 1 && cycles == 1 && zg_fusable(L)) ? 1 : 4;
+                    int nt = sor ? 5 : (!vi && !full_temps && nu == 1 && cycles == 1 && zg_fusable(L)) ? 1 : 4;
+                    if (l == 0 && nlev > 1) nt = 3;   // the coarse CG's z, p, w
                     for (int q = 0; q < nt; ++q) P4G_TRY(alloc(&d.t[q], L.n));
                 }
             }
@@ -605,7 +608,62 @@ struct Engine {
         });
     }
 
-    int coarse(const T* b, T* x) {
+    // the coarsest level's KSP CG with a fixed iteration count (convergence
+    // test skip), from x (xzero: from 0): the oracle's KSPSolve_CG without norms
+    int coarse_cg(const T* b, T* x, bool xzero) {
+        const LevelInfo& L = (*lv)[0];
+        const long n = L.n;
+        Dev& d = dv[0];
+        T *r = d.r, *z = d.t[0], *p = d.t[1], *w = d.t[2];
+        if (!r || !z || !p || !w) return set_error(PETSC_ERR_LIB, "internal: coarse CG vectors");
+        hipStream_t st = cx->stream;
+        const int gb = vblocks(n);
+        double* Sb = cx->S + 16;            // the second scalar block (the outer KSP uses the first)
+        const double* hs = cx->hS + 16;
+        if (xzero) {
+            P4G_TRY(fill(n, T(0), x));
+            HIP_TRY(hipMemcpyAsync(r, b, n * sizeof(T), hipMemcpyDeviceToDevice, st));
+        } else {
+            P4G_TRY(stencil(0, SrcArr<T>{x}, OutResid<T, 0>{b, r}, P4G_K_COARSE, 3, nullptr, false));
+        }
+        auto zr = [&](double& beta) -> int {   // z = B r, beta = (z, r)
+            if (coarse_pc == 2) P4G_TRY(ilu_run<T>(L, ilu_u.size() > 0 ? ilu_u[0] : nullptr, r, z, st));
+            else if (coarse_pc == 1) k_jacobi_const<T><<<gb, 256, 0, st>>>(L.dim, L.mx, L.my, L.mz, L.d, L.db, r, z);
+            else HIP_TRY(hipMemcpyAsync(z, r, n * sizeof(T), hipMemcpyDeviceToDevice, st));
+            k_dot2<T><<<gb, 256, 0, st>>>(n, z, r, cx->part);
+            LAUNCH_CHECK();
+            P4G_TRY(cx->finalize(gb, Sb, FIN_RAW, 0));
+            P4G_TRY(cx->readS(cx->hS));
+            beta = hs[S_TMP0];
+            return 0;
+        };
+        double beta = 0.0, betaold = 0.0, dpi = 0.0, dpiold = 0.0;
+        P4G_TRY(zr(beta));
+        for (int i = 0; i < coarse_its; ++i) {
+            if (beta == 0.0) break;
+            const double bb = i == 0 ? 0.0 : beta / betaold;
+            k_comb3<T><<<gb, 256, 0, st>>>(n, 0.0, nullptr, bb, i == 0 ? nullptr : p, 1.0, z, p);
+            long nbs = 0;
+            P4G_TRY(stencil(0, SrcArr<T>{p}, OutApplyDot<T>{w}, P4G_K_COARSE, 2, &nbs, false));
+            P4G_TRY(cx->finalize(nbs, Sb, FIN_RAW, 0));
+            P4G_TRY(cx->readS(cx->hS));
+            dpiold = dpi;
+            dpi = hs[S_TMP0];
+            betaold = beta;
+            if (dpi == 0.0 || (i > 0 && ((dpi > 0) != (dpiold > 0)))) break;
+            const double a = beta / dpi;
+            k_axpy<T><<<gb, 256, 0, st>>>(n, T(a), p, x);
+            LAUNCH_CHECK();
+            if (i + 1 == coarse_its) break;   // the last B r would go unused
+            k_axpy<T><<<gb, 256, 0, st>>>(n, T(-a), w, r);
+            LAUNCH_CHECK();
+            P4G_TRY(zr(beta));
+        }
+        return 0;
+    }
+
+    int coarse(const T* b, T* x, bool xzero = true) {
+        if (coarse_its > 0) return coarse_cg(b, x, xzero);
         hipStream_t s = cx->stream;
         return cx->launch(P4G_K_COARSE, nlev == 1, double(n0) * n0 * 8.0, [&] {
             k_coarse<T><<<(n0 + 3) / 4, 256, 0, s>>>(n0, Ainv, b, x);
@@ -854,7 +912,7 @@ struct Engine {
     // finest level with dots, the last kernel fuses (x,b) and (x,x).
     int mcycle(int l, const T* b, T* x, bool xzero, bool dots, long* nb) {
         if (l == 0) {
-            P4G_TRY(coarse(b, x));
+            P4G_TRY(coarse(b, x, xzero));
             if (dots) {   // one-level multigrid: (x,b), (x,x) in a separate pass
                 const long n = (*lv)[0].n;
                 hipStream_t s = cx->stream;
@@ -1319,6 +1377,7 @@ struct p4g_solver_s {
     bool sor = false;                 // -mg_levels_pc_type sor|ilu|icc: explicit level smoothers
     bool lilu = false;                //   ... with ILU(0) instead of SSOR
     std::vector<double*> ilu_u;       // per level: ILU(0) pivots (device) or null
+    int coarse_its = 0, coarse_pc = 0;   // -mg_coarse_ksp_type cg: iterations and PC (0 none, 1 jacobi, 2 ilu)
     int sor_blocks = 1;               // -p4g_sor_blocks: local sweeps of that many ranks
     // reduced-space VI Newton (p4g_vi_solve): SNES options and state
     double snes_rtol = 1e-8, snes_atol = 1e-50, snes_stol = 1e-8, snes_divtol = 1e4;
@@ -1733,8 +1792,20 @@ static int configure(p4g_solver_s* s) {
     s->esteig_steps = std::max(1, std::min(15, o.geti("mg_levels_ksp_chebyshev_esteig_steps", 10)));
     const std::string cks = o.gets("mg_coarse_ksp_type", "preonly");
     const std::string cpc = o.gets("mg_coarse_pc_type", "lu");
-    if (cks != "preonly" || (cpc != "lu" && cpc != "cholesky" && cpc != "redundant"))
-        return set_error(PETSC_ERR_SUP, "coarse solver must be preonly + lu/cholesky/redundant");
+    if (cks == "cg" && s->pctype == PCT_MG) {   // c/ch6/study/coarseperformance.sh: fixed-count PCG
+        if (o.gets("mg_coarse_ksp_convergence_test", "") != "skip")
+            return set_error(PETSC_ERR_SUP, "-mg_coarse_ksp_type cg runs a fixed count on the device: add "
+                                            "-mg_coarse_ksp_convergence_test skip");
+        s->coarse_its = o.geti("mg_coarse_ksp_max_it", 10000);
+        if (s->coarse_its < 1) return set_error(PETSC_ERR_ARG_OUTOFRANGE, "-mg_coarse_ksp_max_it must be >= 1");
+        if (cpc == "icc" || cpc == "ilu") s->coarse_pc = 2;
+        else if (cpc == "jacobi") s->coarse_pc = 1;
+        else if (cpc == "none") s->coarse_pc = 0;
+        else return set_error(PETSC_ERR_SUP, "-mg_coarse_pc_type %s with a CG coarse solve (icc, ilu, jacobi, none)",
+                              cpc.c_str());
+    } else if (cks != "preonly" || (cpc != "lu" && cpc != "cholesky" && cpc != "redundant")) {
+        return set_error(PETSC_ERR_SUP, "coarse solver must be preonly + lu/cholesky/redundant or cg (fixed count)");
+    }
     const std::string pr = o.gets("p4g_precision", "double");
     if (pr == "mixed") s->mixed = true;
     else if (pr != "double") return set_error(PETSC_ERR_ARG_WRONG, "-p4g_precision %s", pr.c_str());
@@ -1779,6 +1850,9 @@ static int configure(p4g_solver_s* s) {
         return set_error(PETSC_ERR_SUP, "ILU(0) of Galerkin coarse levels is not implemented on the device");
     if (s->pctype == PCT_ILU && (s->mixed || !s->da->kcell.empty()))
         return set_error(PETSC_ERR_SUP, "-pc_type ilu needs FP64 and a constant-coefficient operator on the device");
+    if (s->coarse_its > 0 && (s->mixed || s->galerkin || !s->da->kcell.empty() || s->nlev < 2))
+        return set_error(PETSC_ERR_SUP, "-mg_coarse_ksp_type cg needs FP64, a rediscretized constant-coefficient "
+                                        "coarsest level and at least two levels on the device");
     if (s->sor && (s->mixed || !s->da->kcell.empty()))
         return set_error(PETSC_ERR_SUP, "-mg_levels_pc_type sor needs FP64 and a constant-coefficient operator "
                                         "on the device");
@@ -1931,7 +2005,8 @@ int solver_setup(p4g_solver_s* s) {
     // own row order)
     s->ilu_u.assign(s->nlev, nullptr);
     for (int l = 0; l < s->nlev; ++l) {
-        const bool need = (s->pctype == PCT_ILU && l == s->nlev - 1) || (s->lilu && (l > 0 || s->nlev == 1));
+        const bool need = (s->pctype == PCT_ILU && l == s->nlev - 1) || (s->lilu && (l > 0 || s->nlev == 1)) ||
+                          (l == 0 && s->coarse_its > 0 && s->coarse_pc == 2);
         if (!need) continue;
         const LevelInfo& L = s->lv[l];
         if (L.var || L.gal) return set_error(PETSC_ERR_SUP, "ILU(0) is implemented for constant 3/5/7-point levels");
@@ -1962,10 +2037,10 @@ int solver_setup(p4g_solver_s* s) {
         s->Ainv.assign((size_t)s->n0 * s->n0, 0.0);
     } else if (s->pctype == PCT_MG) {
         P4G_TRY(galerkin_levels(s));
-        if (s->nlev > 1) {
+        if (s->nlev > 1 && s->coarse_its == 0) {
             P4G_TRY(coarse_factor(s->lv[0], s->g27[0], s->Ainv));
             s->n0 = (int)s->lv[0].n;
-        } else {   // one level: a smoother level, no coarse solve
+        } else {   // one level (a smoother level) or an iterative coarse solve: no dense factor
             s->n0 = 1;
             s->Ainv.assign(1, 0.0);
         }
@@ -1983,6 +2058,8 @@ int solver_setup(p4g_solver_s* s) {
         e->full_temps = s->kaskade || s->mcycles > 1 || (s->ksp_rich && s->pctype == PCT_MG);
         e->sor = s->sor;
         e->lilu = s->lilu;
+        e->coarse_its = s->coarse_its;
+        e->coarse_pc = s->coarse_pc;
         e->ilu_u.assign(s->ilu_u.begin(), s->ilu_u.end());
     };
     if (F.var) {

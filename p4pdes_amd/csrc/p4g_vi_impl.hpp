@@ -190,6 +190,9 @@ static int vi_solve(p4g_solver_s* s, const double* xl, const double* xu, double*
     if (s->mixed) return set_error(PETSC_ERR_SUP, "p4g_vi_solve: mixed precision is not implemented");
     if (!da->kcell.empty()) return set_error(PETSC_ERR_SUP, "p4g_vi_solve: variable coefficients are not implemented");
     if (s->fmg) return set_error(PETSC_ERR_SUP, "p4g_vi_solve: -pc_mg_type full is not implemented");
+    if (s->coarse_its > 0)
+        return set_error(PETSC_ERR_SUP, "p4g_vi_solve: the coarsest reduced system is solved directly (no "
+                                        "-mg_coarse_ksp_type cg)");
     if (s->lilu)
         return set_error(PETSC_ERR_SUP, "p4g_vi_solve: ILU smoothing of the Galerkin levels is not implemented");
     if (!s->vi_mode) {

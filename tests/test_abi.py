@@ -71,6 +71,12 @@ def test_da_from_options():
     ("-pc_type mg -pc_mg_levels 3 -mg_levels_pc_type jacobi -pc_mg_galerkin -p4g_precision mixed", "galerkin"),
     ("-ksp_type gmres -pc_type mg", "gmres"),
     ("-pc_type mg -pc_mg_levels 3 -mg_levels_pc_type jacobi -pc_mg_type additive", "additive"),
+    ("-pc_type mg -pc_mg_levels 3 -mg_coarse_ksp_type cg", "convergence_test skip"),
+    ("-pc_type mg -pc_mg_levels 3 -mg_coarse_ksp_type cg -mg_coarse_ksp_convergence_test skip "
+     "-mg_coarse_pc_type sor", "mg_coarse_pc_type sor"),
+    ("-pc_type mg -pc_mg_levels 3 -mg_levels_pc_type jacobi -mg_coarse_ksp_type cg "
+     "-mg_coarse_ksp_convergence_test skip -mg_coarse_pc_type jacobi -p4g_precision mixed", "FP64"),
+    ("-pc_type mg -pc_mg_levels 3 -mg_coarse_ksp_type gmres", "fixed count"),
 ])
 def test_unsupported_options_rejected(opts, fragment):
     L = capi.lib()

@@ -211,6 +211,24 @@ SOR_CASES.update({
 })
 
 
+# c/ch6/study/coarseperformance.sh: a fixed count of coarse PCG iterations
+# (-mg_coarse_ksp_type cg -mg_coarse_ksp_convergence_test skip) in place of
+# the coarse LU; the W-cycle restarts the coarse CG from a nonzero iterate
+_CCG = "-mg_coarse_ksp_type cg -mg_coarse_ksp_convergence_test skip"
+SOR_CASES.update({
+    "3d_coarse_cg_icc": ("-fsh_dim 3 -fsh_problem manusin -da_refine 4 -pc_type mg -pc_mg_levels 3 %s "
+                         "-mg_coarse_pc_type icc -mg_coarse_ksp_max_it 2 -ksp_rtol 1e-10" % _CCG),
+    "2d_coarse_cg_jacobi_w": ("-fsh_dim 2 -fsh_problem manupoly -da_refine 6 -pc_type mg -pc_mg_levels 3 "
+                              "-pc_mg_cycle_type w -mg_levels_pc_type jacobi %s -mg_coarse_pc_type jacobi "
+                              "-mg_coarse_ksp_max_it 5 -ksp_rtol 1e-10" % _CCG),
+    "1d_coarse_cg_none": ("-fsh_dim 1 -fsh_problem manupoly -da_refine 8 -pc_type mg -pc_mg_levels 4 %s "
+                          "-mg_coarse_pc_type none -mg_coarse_ksp_max_it 3 -ksp_rtol 1e-12" % _CCG),
+    "3d_coarse_cg_fcycle": ("-fsh_dim 3 -fsh_problem manupoly -da_refine 4 -pc_type mg -pc_mg_type full "
+                            "-pc_mg_levels 3 -mg_levels_pc_type jacobi %s -mg_coarse_pc_type ilu "
+                            "-mg_coarse_ksp_max_it 3 -ksp_rtol 1e-10" % _CCG),
+})
+
+
 @pytest.mark.parametrize("name", sorted(SOR_CASES))
 def test_sor_smoother_parity(name):
     args = SOR_CASES[name]
@@ -222,6 +240,24 @@ def test_sor_smoother_parity(name):
     orc, otext, ores, ou = O.fish_run(args, want_u=True)
     assert orc == 0, otext
     assert res.reason > 0 and abs(res.its - ores.its) <= 1, (res.its, ores.its)
+    assert _rel(u, ou) < 1e-9
+    assert text.splitlines()[-1] == otext.splitlines()[-1]
+
+
+@pytest.mark.parametrize("depth", [4, 3])
+def test_coarse_cg_depth_study(depth):
+    # c/ch6/study/coarseperformance.sh at 129^3: two ICC-preconditioned CG
+    # iterations on the coarsest grid (33^3 at depth 3) raise the V-cycle
+    # count when the cycle is shallow
+    args = ("-fsh_dim 3 -ksp_rtol 1.0e-10 -ksp_converged_reason -pc_type mg -da_refine 6 -pc_mg_levels %d "
+            "-mg_coarse_ksp_type cg -mg_coarse_pc_type icc -mg_coarse_ksp_max_it 2 "
+            "-mg_coarse_ksp_convergence_test skip" % depth)
+    print("coarse study", depth, flush=True)
+    rc, text, res, u = capi.fish_run(args, want_u=True)
+    assert rc == 0, text
+    orc, otext, ores, ou = O.fish_run(args, want_u=True)
+    assert orc == 0, otext
+    assert res.reason == 2 and abs(res.its - ores.its) <= 1, (res.its, ores.its)
     assert _rel(u, ou) < 1e-9
     assert text.splitlines()[-1] == otext.splitlines()[-1]
 
