@@ -80,6 +80,33 @@ typedef struct {
 int oracle_obstacle_run(int argc, const char **argv, obstacle_result *res,
                         double *u_out, char *outbuf, int outlen);
 
+/* Result of one minimal-surface run (c/ch7/minimal.c); the same layout as
+ * minimal_result in the product's minimal driver. */
+typedef struct {
+    int    mx, my;
+    int    its;              /* SNES iterations on the final grid          */
+    int    reason;           /* SNESConvergedReason                        */
+    int    last_ksp_its, ksp_its_total, stages;
+    int    stage_its[16];
+    double fnorm0, fnorm;    /* |F|, final grid                            */
+    double errinf;           /* catenoid, q = -1/2: |u - u_exact|_inf      */
+    double area;             /* MSEMonitor's area of the final iterate     */
+    double setup_ms, solve_ms;
+} minimal_result;
+
+/* Run the oracle as `minimal` would be run with argv; text to outbuf,
+ * the final solution (mx*my doubles) to u_out; both may be NULL. */
+int oracle_minimal_run(int argc, const char **argv, minimal_result *res,
+                       double *u_out, char *outbuf, int outlen);
+
+/* The minimal-surface residual and its colored finite-difference Jacobian
+ * (9 coefficients per row) on an mx x my grid of (0,1)^2; problem 0 tent,
+ * 1 catenoid. */
+void oracle_minimal_residual(int mx, int my, int problem, double q, double H, double c,
+                             const double *u, double *F);
+void oracle_minimal_fdjac(int mx, int my, int problem, double q, double H, double c,
+                          const double *u, double *J9);
+
 /* Time `nsolves` solves (after `warmup`) of the linear system fish builds
  * for argv, with `nthreads` OpenMP threads.  Setup is excluded.  Fills
  * times_ms[nsolves] and the result of the last solve. */

@@ -94,7 +94,7 @@ double      oopts_real(oopts *o, const char *key, double def);
 int         oopts_bool(oopts *o, const char *key, int def);
 
 /* ---- solvers (oracle_solver.c) ---- */
-enum { KSP_CG = 0, KSP_RICHARDSON, KSP_CHEBYSHEV, KSP_PREONLY };
+enum { KSP_CG = 0, KSP_RICHARDSON, KSP_CHEBYSHEV, KSP_PREONLY, KSP_GMRES };
 enum { PC_NONE = 0, PC_JACOBI, PC_SOR, PC_ILU, PC_LU, PC_MG, PC_SHELL };
 enum { NORM_NONE = 0, NORM_PRECONDITIONED, NORM_UNPRECONDITIONED, NORM_NATURAL };
 
@@ -102,6 +102,7 @@ enum { NORM_NONE = 0, NORM_PRECONDITIONED, NORM_UNPRECONDITIONED, NORM_NATURAL }
 #define R_CONVERGED_RTOL 2
 #define R_CONVERGED_ATOL 3
 #define R_CONVERGED_ITS 4
+#define R_DIVERGED_NULL (-2)
 #define R_DIVERGED_ITS (-3)
 #define R_DIVERGED_DTOL (-4)
 #define R_DIVERGED_BREAKDOWN (-5)
@@ -132,6 +133,7 @@ void opc_sor_richardson(opc *pc, const double *b, double *x, int its);
 
 typedef struct {
     int     type, pctype, maxit, normtype, monitor, guess_nonzero;
+    int     restart;         /* gmres: Krylov dimension per cycle (30)  */
     double  rtol, atol, dtol, rich_scale;
     double  emin, emax;      /* chebyshev, after transform              */
     int     est_done;

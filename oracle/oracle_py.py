@@ -182,6 +182,56 @@ class ObstacleResult(ctypes.Structure):
         return d
 
 
+class MinimalResult(ctypes.Structure):
+    """Mirror of minimal_result in oracle/oracle.h."""
+    _fields_ = [(n, ctypes.c_int) for n in
+                ("mx", "my", "its", "reason", "last_ksp_its", "ksp_its_total", "stages")] + \
+               [("stage_its", ctypes.c_int * 16)] + \
+               [(n, ctypes.c_double) for n in ("fnorm0", "fnorm", "errinf", "area", "setup_ms", "solve_ms")]
+
+    def as_dict(self):
+        d = {n: getattr(self, n) for n, _ in self._fields_}
+        d["stage_its"] = list(self.stage_its)[:self.stages]
+        return d
+
+
+def minimal_run(args, want_u=False):
+    """Run the oracle as c/ch7/minimal.c would run with `args`.
+
+    Returns (rc, text, MinimalResult, u or None)."""
+    argc, argv = _argv(args)
+    res = MinimalResult()
+    buf = ctypes.create_string_buffer(1 << 18)
+    u = None
+    uptr = None
+    if want_u:
+        mx, my = obstacle_shape(args)
+        u = np.zeros(mx * my)
+        uptr = u.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    rc = lib().oracle_minimal_run(argc, argv, ctypes.byref(res), uptr, buf, len(buf))
+    return rc, buf.value.decode(), res, u
+
+
+def minimal_residual(mx, my, problem, q, H, c, u):
+    """F(u) of c/ch7/minimal.c (problem 0 tent, 1 catenoid)."""
+    F = np.zeros(mx * my)
+    dp = ctypes.POINTER(ctypes.c_double)
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    lib().oracle_minimal_residual(mx, my, problem, ctypes.c_double(q), ctypes.c_double(H), ctypes.c_double(c),
+                                  u.ctypes.data_as(dp), F.ctypes.data_as(dp))
+    return F
+
+
+def minimal_fdjac(mx, my, problem, q, H, c, u):
+    """The colored finite-difference Jacobian, shape (mx*my, 9)."""
+    J = np.zeros((mx * my, 9))
+    dp = ctypes.POINTER(ctypes.c_double)
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    lib().oracle_minimal_fdjac(mx, my, problem, ctypes.c_double(q), ctypes.c_double(H), ctypes.c_double(c),
+                               u.ctypes.data_as(dp), J.ctypes.data_as(dp))
+    return J
+
+
 def obstacle_shape(args):
     """(mx, my) of the final grid obstacle builds for these options."""
     if isinstance(args, str):

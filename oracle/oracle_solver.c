@@ -26,6 +26,7 @@ const char *oracle_reason_name(int r) {
     case R_CONVERGED_RTOL: return "CONVERGED_RTOL";
     case R_CONVERGED_ATOL: return "CONVERGED_ATOL";
     case R_CONVERGED_ITS: return "CONVERGED_ITS";
+    case R_DIVERGED_NULL: return "DIVERGED_NULL";
     case R_DIVERGED_ITS: return "DIVERGED_ITS";
     case R_DIVERGED_DTOL: return "DIVERGED_DTOL";
     case R_DIVERGED_BREAKDOWN: return "DIVERGED_BREAKDOWN";
@@ -261,7 +262,8 @@ void oksp_init(oksp *k, int type, const ocsr *A, int pctype) {
     k->rtol = 1e-5; k->atol = 1e-50; k->dtol = 1e5;
     k->maxit = 10000;
     k->rich_scale = 1.0;
-    k->normtype = (type == KSP_CG) ? NORM_PRECONDITIONED : NORM_NONE;
+    k->normtype = (type == KSP_CG || type == KSP_GMRES) ? NORM_PRECONDITIONED : NORM_NONE;
+    k->restart = 30;
     k->esteig_steps = 10;
     k->est_tf[0] = 0.0; k->est_tf[1] = 0.1; k->est_tf[2] = 0.0; k->est_tf[3] = 1.1;
     k->est_lanczos = 1;
@@ -429,6 +431,119 @@ static int cg_solve(oksp *k, const double *b, double *x, int *reason, double *rn
     return its;
 }
 
+/* KSPSolve_GMRES (PETSc's default KSP for SNES): restarted GMRES(m), left
+ * preconditioning, classical Gram-Schmidt without refinement, the residual
+ * norm of the preconditioned system from the Givens-rotated Hessenberg
+ * least-squares problem.  A cycle starts from the true residual B (b - A x);
+ * the convergence test runs at the start of each cycle (ksp its so far) and
+ * after each Arnoldi step; the happy breakdown tolerance is 1e-30.  H is the
+ * rotated Hessenberg column-major with leading dimension m + 2.            */
+static int gmres_solve(oksp *k, const double *b, double *x, int *reason, double *rnorm0, double *rnorm) {
+    const ocsr *A = k->A;
+    const long n = A->n;
+    const int m = k->restart > 0 ? k->restart : 30;
+    const int ld = m + 2;
+    double **V = (double **)malloc(sizeof(double *) * (m + 1));
+    for (int j = 0; j <= m; j++) V[j] = (double *)malloc(sizeof(double) * (n ? n : 1));
+    double *H = (double *)calloc((size_t)ld * (m + 1), sizeof(double));
+    double *cc = (double *)calloc(m + 1, sizeof(double)), *ss = (double *)calloc(m + 1, sizeof(double));
+    double *grs = (double *)calloc(m + 2, sizeof(double)), *h = (double *)malloc(sizeof(double) * (m + 1));
+    double *t = k->w[0], *aw = k->w[1], *corr = k->w[2];
+    double ttol = 0.0, rn = -1.0;
+    int its = 0, first = 1;
+    *reason = 0;
+#define HH(a, c) H[(size_t)(c) * ld + (a)]
+    while (!*reason) {
+        /* KSPInitialResidual: V0 = B (b - A x), or B b from a zero guess */
+        if (first && !k->guess_nonzero) { ovset(n, 0.0, x); opc_apply(&k->pc, b, V[0]); }
+        else { ocsr_resid(A, b, x, t); opc_apply(&k->pc, t, V[0]); }
+        /* KSPGMRESCycle */
+        double res = ovnorm2(n, V[0]);
+        if (res != 0.0) { const double s = 1.0 / res; for (long i = 0; i < n; i++) V[0][i] *= s; }
+        grs[0] = res;
+        rn = res;
+        ksp_monitor(k, its, res);
+        if (res == 0.0) { *reason = R_CONVERGED_ATOL; if (first) *rnorm0 = 0.0; break; }
+        {
+            const int c = converged(k, its, res, rnorm0, &ttol);
+            *reason = c;
+        }
+        first = 0;
+        int it = 0, hapend = 0;
+        while (!*reason && it < m && its < k->maxit) {
+            if (it) ksp_monitor(k, its, res);
+            /* V[it+1] = B A V[it] */
+            ocsr_mult(A, V[it], aw);
+            opc_apply(&k->pc, aw, V[it + 1]);
+            /* classical Gram-Schmidt: h = V^T v, v -= V h */
+            for (int j = 0; j <= it; j++) h[j] = ovdot(n, V[it + 1], V[j]);
+            for (long i = 0; i < n; i++) {
+                double s = V[it + 1][i];
+                for (int j = 0; j <= it; j++) s -= h[j] * V[j][i];
+                V[it + 1][i] = s;
+            }
+            for (int j = 0; j <= it; j++) HH(j, it) = h[j];
+            double tt = ovnorm2(n, V[it + 1]);
+            if (tt != 0.0) { const double s = 1.0 / tt; for (long i = 0; i < n; i++) V[it + 1][i] *= s; }
+            HH(it + 1, it) = tt;
+            double hapbnd = fabs(tt / grs[it]);
+            if (hapbnd > 1e-30) hapbnd = 1e-30;
+            if (tt < hapbnd) hapend = 1;
+            /* KSPGMRESUpdateHessenberg: the earlier rotations, then a new one */
+            for (int j = 1; j <= it; j++) {
+                const double a0 = HH(j - 1, it), a1 = HH(j, it);
+                HH(j - 1, it) = cc[j - 1] * a0 + ss[j - 1] * a1;
+                HH(j, it) = cc[j - 1] * a1 - ss[j - 1] * a0;
+            }
+            if (!hapend) {
+                const double a0 = HH(it, it), a1 = HH(it + 1, it);
+                const double r = sqrt(a0 * a0 + a1 * a1);
+                if (r == 0.0) { *reason = R_DIVERGED_NULL; break; }
+                cc[it] = a0 / r;
+                ss[it] = a1 / r;
+                grs[it + 1] = -(ss[it] * grs[it]);
+                grs[it] = cc[it] * grs[it];
+                HH(it, it) = cc[it] * a0 + ss[it] * a1;
+                res = fabs(grs[it + 1]);
+            } else {
+                res = 0.0;
+            }
+            it++;
+            its++;
+            rn = res;
+            *reason = converged(k, its, res, rnorm0, &ttol);
+            if (hapend && !*reason) { *reason = R_DIVERGED_BREAKDOWN; break; }
+        }
+        if (it && (*reason || its >= k->maxit)) ksp_monitor(k, its, res);
+        /* KSPGMRESBuildSoln: back substitution on the rotated system, x += V y */
+        if (it > 0) {
+            const int top = it - 1;
+            double *y = grs;
+            if (HH(top, top) != 0.0) y[top] = grs[top] / HH(top, top);
+            else { *reason = R_DIVERGED_BREAKDOWN; y[top] = 0.0; }
+            for (int ii = 1; ii <= top; ii++) {
+                const int kk = top - ii;
+                double s = grs[kk];
+                for (int j = kk + 1; j <= top; j++) s -= HH(kk, j) * y[j];
+                if (HH(kk, kk) == 0.0) { *reason = R_DIVERGED_BREAKDOWN; break; }
+                y[kk] = s / HH(kk, kk);
+            }
+            for (long i = 0; i < n; i++) {
+                double s = 0.0;
+                for (int j = 0; j <= top; j++) s += y[j] * V[j][i];
+                corr[i] = s;
+            }
+            for (long i = 0; i < n; i++) x[i] += corr[i];
+        }
+        if (its >= k->maxit && !*reason) *reason = R_DIVERGED_ITS;
+    }
+#undef HH
+    *rnorm = rn;
+    for (int j = 0; j <= m; j++) free(V[j]);
+    free(V); free(H); free(cc); free(ss); free(grs); free(h);
+    return its;
+}
+
 /* KSPSolve_Chebyshev: p[km1] = x, p[k] = x + s B (b - A x), then
  * p[kp1] = (1-w) p[km1] + w p[k] + w s B (b - A p[k])               */
 static int cheb_solve(oksp *k, const double *b, double *x) {
@@ -560,6 +675,7 @@ int oksp_solve(oksp *k, const double *b, double *x, int *reason, double *rnorm0,
         *reason = R_CONVERGED_ITS;
         return rich_solve(k, b, x);
     case KSP_PREONLY: opc_apply(&k->pc, b, x); *reason = R_CONVERGED_ITS; return 1;
+    case KSP_GMRES: return gmres_solve(k, b, x, reason, rnorm0, rnorm);
     }
     return 0;
 }
@@ -633,6 +749,7 @@ static int parse_ksp(const char *s, int def) {
     if (!strcmp(s, "richardson")) return KSP_RICHARDSON;
     if (!strcmp(s, "chebyshev")) return KSP_CHEBYSHEV;
     if (!strcmp(s, "preonly")) return KSP_PREONLY;
+    if (!strcmp(s, "gmres")) return KSP_GMRES;
     return -1;
 }
 static int parse_pc(const char *s, int def) {
