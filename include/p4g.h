@@ -46,6 +46,7 @@ typedef struct p4g_da_s *p4g_DA;   /* the DM of p4g_petsc.h */
 #define P4G_CONVERGED_RTOL 2
 #define P4G_CONVERGED_ATOL 3
 #define P4G_CONVERGED_ITS 4
+#define P4G_DIVERGED_NULL (-2)
 #define P4G_DIVERGED_ITS (-3)
 #define P4G_DIVERGED_DTOL (-4)
 #define P4G_DIVERGED_BREAKDOWN (-5)
@@ -227,6 +228,40 @@ int p4g_vi_solve(p4g_Solver s, const double *xl, const double *xu, double *x, do
  * first min(cap, count) entries of each; arrays may be NULL. */
 int p4g_vi_get_history(p4g_Solver s, double *fnorm, int *ksp_its, int *ksp_reason, int cap, int *nf,
                        int *nk);
+
+/* ------------------------------------------------- minimal surface */
+
+/* SNESSolve with SNESNEWTONLS and -snes_fd_color for c/ch7/minimal.c's
+ * FormFunctionLocal (minimal.c:230-301): the minimal-surface equation
+ * -div((1 + |grad u|^2)^q grad u) = 0 on a 2D grid of the unit square
+ * (DMDASetUniformCoordinates(0,1,0,1)), u = g on the boundary.  The device
+ * evaluates that residual itself; gb holds g at the grid's nodes (only the
+ * boundary entries are read), and when the grid has a residual callback
+ * (p4g_da_set_function_local) it must agree with the device residual at
+ * the initial iterate (else PETSC_ERR_ARG_WRONG).  Per Newton step the
+ * Jacobian by colored finite differences (PETSc's "wp" step), the solver's
+ * KSP (GMRES unless -ksp_type cg; -ksp_gmres_restart) with PC ILU(0)
+ * (default), mg (each coarser level: the finite-difference Jacobian at the
+ * injected iterate; coarse band LU) or none, the bt (alpha 1e-4) or basic
+ * line search, SNESConvergedDefault.  SNES options as for p4g_vi_solve but
+ * -snes_type newtonls; -mat_is_symmetric tol checks each Jacobian.  x:
+ * initial iterate in, solution out (host, the grid's size).  mon (may be
+ * NULL) is called with the host iterate after the initial residual and
+ * after each Newton step, as SNESMonitorSet's monitors are; a nonzero
+ * return ends the solve with that error.  Replaces the DMDASNESSetFunction-
+ * Local + SNESSolve pair of minimal.c:150-177. */
+typedef struct {
+    int    its, reason, last_ksp_its, ksp_its_total;
+    double fnorm0, fnorm;
+    double setup_ms, solve_ms;
+} p4g_MSReport;
+typedef int (*p4g_MSMonitorFn)(void *ctx, int it, double fnorm, const double *x);
+int p4g_ms_solve(p4g_Solver s, double q, const double *gb, double *x, p4g_MSMonitorFn mon, void *monctx,
+                 p4g_MSReport *rep);
+/* As p4g_vi_get_history, plus jac_sym[i]: MatIsSymmetric of Newton step
+ * i's Jacobian under -mat_is_symmetric (1 symmetric, 0 not, -1 unchecked). */
+int p4g_ms_get_history(p4g_Solver s, double *fnorm, int *ksp_its, int *ksp_reason, int *jac_sym, int cap,
+                       int *nf, int *nk);
 
 typedef struct p4g_server_s *p4g_Server;
 

@@ -126,6 +126,7 @@ static int session_setup(osession *s, int argc, const char **argv, char *out, in
     k->dtol = oopts_real(o, "ksp_divtol", 1e5);
     k->maxit = oopts_int(o, "ksp_max_it", 10000);
     k->rich_scale = oopts_real(o, "ksp_richardson_scale", 1.0);
+    k->restart = oopts_int(o, "ksp_gmres_restart", 30);
     const char *nt = oopts_get(o, "ksp_norm_type");
     /* KSPRICHARDSON keeps CG's preconditioned default (the reference's study
      * scripts always pass -ksp_norm_type unpreconditioned, so the default is

@@ -74,6 +74,22 @@ def build_obstacle():
     return out
 
 
+def build_minimal():
+    """libp4g_minimal.so + the `minimal` driver (c/ch7/minimal.c on the ABI)."""
+    os.makedirs(LIB, exist_ok=True)
+    mdir = os.path.join(PKG, "minimal")
+    out = os.path.join(LIB, "libp4g_minimal.so")
+    _run(["gcc"] + C_FLAGS + ["-I" + FISH, "-shared", "-o", out + ".tmp", os.path.join(FISH, "poissonfunctions.c"),
+                              os.path.join(mdir, "minimal.c"),
+                              "-L" + LIB, "-lp4g", "-Wl,-rpath,$ORIGIN", "-lm"])
+    os.replace(out + ".tmp", out)
+    exe = os.path.join(LIB, "minimal")
+    _run(["gcc"] + C_FLAGS + ["-o", exe + ".tmp", os.path.join(mdir, "minimal_main.c"),
+                              "-L" + LIB, "-lp4g_minimal", "-lp4g", "-Wl,-rpath,$ORIGIN", "-lm"])
+    os.replace(exe + ".tmp", exe)
+    return out
+
+
 def build_oracle():
     _run(["make", "-s", "-C", os.path.join(ROOT, "oracle")])
 
@@ -84,6 +100,7 @@ def build_all():
     build_lib()
     build_fish()
     build_obstacle()
+    build_minimal()
     build_oracle()
 
 

@@ -12,6 +12,7 @@ LIBDIR = os.path.join(_HERE, "lib")
 LIBP4G = os.path.join(LIBDIR, "libp4g.so")
 LIBFISH = os.path.join(LIBDIR, "libp4g_fish.so")
 LIBOBST = os.path.join(LIBDIR, "libp4g_obstacle.so")
+LIBMIN = os.path.join(LIBDIR, "libp4g_minimal.so")
 
 # kernel classes (enum in p4g.h)
 K_STENCIL_DOT, K_CG_UPDATE, K_RESID_ZG, K_RESTRICT, K_PROLONG, K_SMOOTH, K_COARSE, \
@@ -74,8 +75,22 @@ class ObstacleResult(ctypes.Structure):
         return d
 
 
+class MinimalResult(ctypes.Structure):
+    """minimal_result (same layout as the oracle's)."""
+    _fields_ = [(n, ctypes.c_int) for n in
+                ("mx", "my", "its", "reason", "last_ksp_its", "ksp_its_total", "stages")] + \
+               [("stage_its", ctypes.c_int * 16)] + \
+               [(n, ctypes.c_double) for n in ("fnorm0", "fnorm", "errinf", "area", "setup_ms", "solve_ms")]
+
+    def as_dict(self):
+        d = {n: getattr(self, n) for n, _ in self._fields_}
+        d["stage_its"] = list(self.stage_its)[:self.stages]
+        return d
+
+
 _fish = None
 _obst = None
+_min = None
 
 
 def lib():
@@ -159,6 +174,34 @@ def obstaclelib():
                                    ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int]
         _obst = L
     return _obst
+
+
+def minimallib():
+    global _min
+    if _min is None:
+        lib()
+        if not os.path.exists(LIBMIN):
+            raise RuntimeError("libp4g_minimal.so is not built")
+        L = ctypes.CDLL(LIBMIN)
+        L.minimal_run.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.POINTER(MinimalResult),
+                                  ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int]
+        _min = L
+    return _min
+
+
+def minimal_run(args, want_u=False):
+    """Run minimal on the GPU.  Returns (rc, text, MinimalResult, u or None)."""
+    argc, argv = _argv(args)
+    res = MinimalResult()
+    buf = ctypes.create_string_buffer(1 << 18)
+    u = None
+    uptr = None
+    if want_u:
+        mx, my = obstacle_shape(args)
+        u = np.zeros(mx * my)
+        uptr = u.ctypes.data
+    rc = minimallib().minimal_run(argc, argv, ctypes.byref(res), uptr, buf, len(buf))
+    return rc, buf.value.decode(), res, u
 
 
 def obstacle_shape(args):

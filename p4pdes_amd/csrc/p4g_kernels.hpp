@@ -2102,7 +2102,7 @@ k_vi_trial(long n, const double* __restrict__ x0, const double* __restrict__ y, 
            const double* __restrict__ xl, const double* __restrict__ xu, double* __restrict__ x) {
     for (long q = blockIdx.x * 256L + threadIdx.x; q < n; q += (long)gridDim.x * 256L) {
         double v = y ? x0[q] - lambda * y[q] : x0[q];
-        if (v < xl[q]) v = xl[q];
+        if (xl && v < xl[q]) v = xl[q];
         else if (xu && v > xu[q]) v = xu[q];
         x[q] = v;
     }
@@ -2188,4 +2188,173 @@ k_vi_mask(long n, const double* __restrict__ diag, T* __restrict__ x) {
         if (diag[q] == 0.0) x[q] = T(0);
 }
 
+
+// ------------------------------------------------------ minimal surface
+//
+// c/ch7/minimal.c:230-301 on the unit square (box stencil, 2D).  G holds
+// the boundary values g at the boundary nodes (a node's own value, and the
+// value minimal.c evaluates at a neighbour on the boundary: the same point
+// for the power-of-two spacings of a refined DMDA).  No FMA contraction, so
+// that the finite-difference Jacobian, which divides rounding by its step,
+// follows the reference's arithmetic.
+
+struct MsGrid {
+    int mx, my;
+    long n;
+    double hx, hy, q;
+};
+
+// F at node (i, j) with u at node rp replaced by up (rp < 0: none)
+__device__ __forceinline__ double ms_point(const MsGrid& g, const double* __restrict__ u,
+                                           const double* __restrict__ G, int i, int j, long rp, double up) {
+#pragma clang fp contract(off)
+    const int mx = g.mx;
+    const long q = (long)j * mx + i;
+    auto U = [&](long p) { return p == rp ? up : u[p]; };
+    if (j == 0 || i == 0 || i == mx - 1 || j == g.my - 1) return U(q) - G[q];
+    const bool E = i + 1 == mx - 1, W = i - 1 == 0, N = j + 1 == g.my - 1, S = j - 1 == 0;
+    const double uc = U(q);
+    const double ue = E ? G[q + 1] : U(q + 1);
+    const double uw = W ? G[q - 1] : U(q - 1);
+    const double un = N ? G[q + mx] : U(q + mx);
+    const double us = S ? G[q - mx] : U(q - mx);
+    const double une = (E || N) ? G[q + mx + 1] : U(q + mx + 1);
+    const double unw = (W || N) ? G[q + mx - 1] : U(q + mx - 1);
+    const double use = (E || S) ? G[q - mx + 1] : U(q - mx + 1);
+    const double usw = (W || S) ? G[q - mx - 1] : U(q - mx - 1);
+    const double hx = g.hx, hy = g.hy;
+    double dux, duy;
+    dux = (ue - uc) / hx;
+    duy = (un + une - us - use) / (4.0 * hy);
+    const double De = pow(1.0 + (dux * dux + duy * duy), g.q);
+    dux = (uc - uw) / hx;
+    duy = (unw + un - usw - us) / (4.0 * hy);
+    const double Dw = pow(1.0 + (dux * dux + duy * duy), g.q);
+    dux = (ue + une - uw - unw) / (4.0 * hx);
+    duy = (un - uc) / hy;
+    const double Dn = pow(1.0 + (dux * dux + duy * duy), g.q);
+    dux = (ue + use - uw - usw) / (4.0 * hx);
+    duy = (uc - us) / hy;
+    const double Ds = pow(1.0 + (dux * dux + duy * duy), g.q);
+    return -(hy / hx) * (De * (ue - uc) - Dw * (uc - uw)) - (hx / hy) * (Dn * (un - uc) - Ds * (uc - us));
+}
+
+__global__ void __launch_bounds__(256)
+k_ms_residual(MsGrid g, const double* __restrict__ u, const double* __restrict__ G, double* __restrict__ F) {
+    for (long q = blockIdx.x * 256L + threadIdx.x; q < g.n; q += (long)gridDim.x * 256L)
+        F[q] = ms_point(g, u, G, (int)(q % g.mx), (int)(q / g.mx), -1, 0.0);
+}
+
+// MatFDColoringApply with the "wp" step h = 1/d on the DMDA box coloring:
+// each row's entry for a neighbour p is (F_q(u + h e_p) - F_q(u)) d, which
+// is what the colored sweep computes (no other column of p's color lies in
+// q's box).  A thread per row evaluates its nine perturbed residuals and
+// writes the 2D stencil planes 9..17 of the 27-plane form.
+__global__ void __launch_bounds__(256)
+k_ms_fdjac(MsGrid g, const double* __restrict__ u, const double* __restrict__ G, const double* __restrict__ F0,
+           double d, double h, double* __restrict__ g27) {
+#pragma clang fp contract(off)
+    const long n = g.n;
+    for (long q = blockIdx.x * 256L + threadIdx.x; q < n; q += (long)gridDim.x * 256L) {
+        const int i = (int)(q % g.mx), j = (int)(q / g.mx);
+        const double f0 = F0[q];
+        for (int oy = -1; oy <= 1; ++oy)
+            for (int ox = -1; ox <= 1; ++ox) {
+                const int c = 9 + (oy + 1) * 3 + (ox + 1);
+                double v = 0.0;
+                if (i + ox >= 0 && i + ox < g.mx && j + oy >= 0 && j + oy < g.my) {
+                    const long p = q + (long)oy * g.mx + ox;
+                    v = (ms_point(g, u, G, i, j, p, u[p] + h) - f0) * d;
+                }
+                g27[(long)c * n + q] = v;
+            }
+    }
+}
+
+// coarse (I, J) <- fine (2I, 2J)
+__global__ void __launch_bounds__(256)
+k_inject2(int mxc, int myc, int mxf, const double* __restrict__ xf, double* __restrict__ xc) {
+    const long nc = (long)mxc * myc;
+    for (long q = blockIdx.x * 256L + threadIdx.x; q < nc; q += (long)gridDim.x * 256L) {
+        const int I = (int)(q % mxc), J = (int)(q / mxc);
+        xc[q] = xf[(long)(2 * J) * mxf + 2 * I];
+    }
+}
+
+// per block: max |a(q, p) - a(p, q)| over the 2D box entries (MatIsSymmetric)
+__global__ void __launch_bounds__(256)
+k_sym27(int mx, int my, const double* __restrict__ g27, double* __restrict__ part) {
+    const long n = (long)mx * my;
+    double m = 0.0;
+    for (long q = blockIdx.x * 256L + threadIdx.x; q < n; q += (long)gridDim.x * 256L) {
+        const int i = (int)(q % mx), j = (int)(q / mx);
+        for (int oy = -1; oy <= 1; ++oy)
+            for (int ox = -1; ox <= 1; ++ox) {
+                if (i + ox < 0 || i + ox >= mx || j + oy < 0 || j + oy >= my) continue;
+                const long p = q + (long)oy * mx + ox;
+                const double a = g27[(long)(9 + (oy + 1) * 3 + (ox + 1)) * n + q];
+                const double b = g27[(long)(9 + (1 - oy) * 3 + (1 - ox)) * n + p];
+                m = fmax(m, fabs(a - b));
+            }
+    }
+    __shared__ double sm[256];
+    sm[threadIdx.x] = m;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) sm[threadIdx.x] = fmax(sm[threadIdx.x], sm[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = sm[0];
+}
+
+// ------------------------------------------------------------ GMRES
+//
+// V holds the Krylov basis as m contiguous vectors of length n.
+
+// per block: (V_j, w) for j < m, written to part[j * gridDim.x + block]
+__global__ void __launch_bounds__(256)
+k_mdot(long n, int m, const double* __restrict__ V, const double* __restrict__ w, double* __restrict__ part) {
+    __shared__ double sm[4];
+    for (int j = 0; j < m; ++j) {
+        const double* v = V + (long)j * n;
+        double a = 0.0;
+        for (long q = blockIdx.x * 256L + threadIdx.x; q < n; q += (long)gridDim.x * 256L) a += v[q] * w[q];
+        a = wave_sum(a);
+        if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = a;
+        __syncthreads();
+        if (threadIdx.x == 0) part[(long)j * gridDim.x + blockIdx.x] = sm[0] + sm[1] + sm[2] + sm[3];
+        __syncthreads();
+    }
+}
+
+// out[j] = sum_b part[j * nb + b], one block per j, a fixed order
+__global__ void __launch_bounds__(256)
+k_mfinal(int nb, const double* __restrict__ part, double* __restrict__ out) {
+    const double* p = part + (long)blockIdx.x * nb;
+    double a = 0.0;
+    for (int b = threadIdx.x; b < nb; b += 256) a += p[b];
+    __shared__ double sm[4];
+    a = wave_sum(a);
+    if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = a;
+    __syncthreads();
+    if (threadIdx.x == 0) out[blockIdx.x] = sm[0] + sm[1] + sm[2] + sm[3];
+}
+
+// w += s * sum_{j < m} c_j V_j   (c on the device)
+__global__ void __launch_bounds__(256)
+k_maxpy(long n, int m, double s, const double* __restrict__ c, const double* __restrict__ V, double* __restrict__ w) {
+    for (long q = blockIdx.x * 256L + threadIdx.x; q < n; q += (long)gridDim.x * 256L) {
+        double a = 0.0;
+        for (int j = 0; j < m; ++j) a += c[j] * V[(long)j * n + q];
+        w[q] += s * a;
+    }
+}
+
+// v = w / sqrt(*nn) (nn: a device scalar holding |w|^2), or a copy when 0
+__global__ void __launch_bounds__(256)
+k_normalize(long n, const double* __restrict__ nn, const double* __restrict__ w, double* __restrict__ v) {
+    const double t = sqrt(*nn);
+    const double s = t != 0.0 ? 1.0 / t : 1.0;
+    for (long q = blockIdx.x * 256L + threadIdx.x; q < n; q += (long)gridDim.x * 256L) v[q] = w[q] * s;
+}
 }  // namespace p4g

@@ -123,8 +123,8 @@ static int serve_supported(p4g_solver_s* s) {
     if (s->sor) return set_error(PETSC_ERR_SUP, "serving runs Jacobi-preconditioned smoothers (no SOR)");
     if (s->coarse_its > 0)
         return set_error(PETSC_ERR_SUP, "serving solves the coarsest level directly (no -mg_coarse_ksp_type cg)");
-    if (s->ksp_rich || s->vi_mode)
-        return set_error(PETSC_ERR_SUP, "serving runs CG solves (no -ksp_type richardson or VI solver)");
+    if (s->ksp_rich || s->ksp_gmres || s->vi_mode)
+        return set_error(PETSC_ERR_SUP, "serving runs CG solves (no -ksp_type richardson|gmres or VI solver)");
     return 0;
 }
 

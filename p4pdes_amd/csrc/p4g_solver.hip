@@ -1181,6 +1181,146 @@ struct Engine {
         return 0;
     }
 
+    // KSPSolve_GMRES from x = 0: restarted GMRES(m) with left preconditioning,
+    // classical Gram-Schmidt without refinement and the preconditioned
+    // residual norm of the Givens-rotated Hessenberg least-squares problem.
+    // A cycle starts from the true residual B (b - A x); the convergence test
+    // runs at each cycle start and after each Arnoldi step; hist records the
+    // norms where PETSc's KSPGMRESCycle calls its monitor.  The basis, its
+    // dots and the updates stay on the device; the host forms the rotations
+    // from the it + 2 scalars of each step.
+    T* gm_V = nullptr;
+    double *gm_c = nullptr, *gm_part = nullptr;
+    int gm_m = 0;
+    int gmres(const T* b, T* x, double rtol, double atol, double dtol, int maxit, int m, CGOut& o, double* Sblk) {
+        const LevelInfo& F = (*lv)[nlev - 1];
+        const long n = F.n;
+        hipStream_t st = cx->stream;
+        const int nb = vblocks(n);
+        o = CGOut();
+        if (m < 1) return set_error(PETSC_ERR_ARG_OUTOFRANGE, "-ksp_gmres_restart must be >= 1");
+        if (!gm_V || gm_m < m) {
+            P4G_TRY(alloc(&gm_V, (long)(m + 1) * n));
+            double* c = nullptr;
+            HIP_TRY(hipMalloc((void**)&c, (size_t)(m + 1) * sizeof(double)));
+            allocs.push_back(c);
+            gm_c = c;
+            HIP_TRY(hipMalloc((void**)&c, (size_t)(m + 1) * nb * sizeof(double)));
+            allocs.push_back(c);
+            gm_part = c;
+            gm_m = m;
+        }
+        T* V = gm_V;
+        const int soff = (int)(Sblk - cx->S);
+        const double* hs = cx->hS + soff;
+        const int ld = m + 2;
+        std::vector<double> H((size_t)ld * (m + 1), 0.0), cc(m + 1), ss(m + 1), grs(m + 2), hh(m + 2);
+        auto HH = [&](int a, int c) -> double& { return H[(size_t)c * ld + a]; };
+        // |v|^2 into Sblk[S_TMP0] and v /= |v| (VecNormalize)
+        auto normalize = [&](T* v) -> int {
+            k_dot2<T><<<nb, 256, 0, st>>>(n, v, v, cx->part);
+            LAUNCH_CHECK();
+            P4G_TRY(cx->finalize(nb, Sblk, FIN_RAW, 0));
+            k_normalize<<<nb, 256, 0, st>>>(n, Sblk + S_TMP0, v, v);
+            LAUNCH_CHECK();
+            return 0;
+        };
+        P4G_TRY(fill(n, T(0), x));
+        double ttol = 0.0, res = 0.0;
+        int its = 0, reason = 0;
+        bool first = true;
+        while (!reason) {
+            if (first) {
+                P4G_TRY(pc_apply(b, V));
+            } else {
+                P4G_TRY(stencil(nlev - 1, SrcArr<T>{x}, OutResid<T, 0>{b, r}, P4G_K_OTHER, 3, nullptr, false));
+                P4G_TRY(pc_apply(r, V));
+            }
+            P4G_TRY(normalize(V));
+            P4G_TRY(cx->readS(cx->hS));
+            res = std::sqrt(hs[S_TMP0]);
+            grs[0] = res;
+            o.hist.push_back(res);
+            if (first) { o.rnorm0 = res; ttol = std::max(rtol * res, atol); }
+            first = false;
+            if (res == 0.0) { reason = P4G_CONVERGED_ATOL; break; }
+            auto test = [&](int k, double rn) -> int {
+                if (std::isnan(rn) || std::isinf(rn)) return P4G_DIVERGED_NANORINF;
+                if (rn <= ttol) return rn < atol ? P4G_CONVERGED_ATOL : P4G_CONVERGED_RTOL;
+                if (k > 0 && rn >= dtol * o.rnorm0) return P4G_DIVERGED_DTOL;
+                return 0;
+            };
+            reason = test(its, res);
+            int it = 0;
+            bool hapend = false;
+            while (!reason && it < m && its < maxit) {
+                if (it) o.hist.push_back(res);
+                T* vi = V + (long)it * n;
+                T* vn = V + (long)(it + 1) * n;
+                P4G_TRY(mat_mult(vi, w));
+                P4G_TRY(pc_apply(w, vn));
+                k_mdot<<<nb, 256, 0, st>>>(n, it + 1, V, vn, gm_part);
+                k_mfinal<<<it + 1, 256, 0, st>>>(nb, gm_part, gm_c);
+                k_maxpy<<<nb, 256, 0, st>>>(n, it + 1, -1.0, gm_c, V, vn);
+                LAUNCH_CHECK();
+                HIP_TRY(hipMemcpyAsync(hh.data(), gm_c, (it + 1) * sizeof(double), hipMemcpyDeviceToHost, st));
+                P4G_TRY(normalize(vn));
+                P4G_TRY(cx->readS(cx->hS));
+                const double tt = std::sqrt(hs[S_TMP0]);
+                for (int j = 0; j <= it; ++j) HH(j, it) = hh[j];
+                HH(it + 1, it) = tt;
+                double hapbnd = std::fabs(tt / grs[it]);
+                if (hapbnd > 1e-30) hapbnd = 1e-30;
+                if (tt < hapbnd) hapend = true;
+                for (int j = 1; j <= it; ++j) {   // the earlier rotations
+                    const double a0 = HH(j - 1, it), a1 = HH(j, it);
+                    HH(j - 1, it) = cc[j - 1] * a0 + ss[j - 1] * a1;
+                    HH(j, it) = cc[j - 1] * a1 - ss[j - 1] * a0;
+                }
+                if (!hapend) {
+                    const double a0 = HH(it, it), a1 = HH(it + 1, it);
+                    const double rr = std::sqrt(a0 * a0 + a1 * a1);
+                    if (rr == 0.0) { reason = P4G_DIVERGED_NULL; break; }
+                    cc[it] = a0 / rr;
+                    ss[it] = a1 / rr;
+                    grs[it + 1] = -(ss[it] * grs[it]);
+                    grs[it] = cc[it] * grs[it];
+                    HH(it, it) = cc[it] * a0 + ss[it] * a1;
+                    res = std::fabs(grs[it + 1]);
+                } else {
+                    res = 0.0;
+                }
+                ++it;
+                ++its;
+                reason = test(its, res);
+                if (hapend && !reason) { reason = P4G_DIVERGED_BREAKDOWN; break; }
+            }
+            if (it && (reason || its >= maxit)) o.hist.push_back(res);
+            if (it > 0) {   // KSPGMRESBuildSoln: back substitution, x += V y
+                const int top = it - 1;
+                std::vector<double> y(grs.begin(), grs.begin() + it);
+                if (HH(top, top) != 0.0) y[top] = grs[top] / HH(top, top);
+                else { reason = P4G_DIVERGED_BREAKDOWN; y[top] = 0.0; }
+                for (int ii = 1; ii <= top; ++ii) {
+                    const int kk = top - ii;
+                    double sum = grs[kk];
+                    for (int j = kk + 1; j <= top; ++j) sum -= HH(kk, j) * y[j];
+                    if (HH(kk, kk) == 0.0) { reason = P4G_DIVERGED_BREAKDOWN; break; }
+                    y[kk] = sum / HH(kk, kk);
+                }
+                HIP_TRY(hipMemcpyAsync(gm_c, y.data(), it * sizeof(double), hipMemcpyHostToDevice, st));
+                k_maxpy<<<nb, 256, 0, st>>>(n, it, 1.0, gm_c, V, x);
+                LAUNCH_CHECK();
+                HIP_TRY(hipStreamSynchronize(st));   // y lives on the host stack
+            }
+            if (its >= maxit && !reason) reason = P4G_DIVERGED_ITS;
+        }
+        o.its = its;
+        o.reason = reason;
+        o.rnorm = res;
+        return 0;
+    }
+
     // KSPSolve_Richardson from x = 0.  monitor: PETSc's generic loop (taken
     // when a monitor is set, or the PC is not multigrid): the norm of r
     // (unpreconditioned) or B r (preconditioned) is monitored and tested,
@@ -1372,6 +1512,8 @@ struct p4g_solver_s {
     bool kaskade = false;             // -pc_mg_type kaskade
     int mcycles = 1;                  // -pc_mg_multiplicative_cycles
     bool ksp_rich = false;            // -ksp_type richardson
+    bool ksp_gmres = false;           // -ksp_type gmres
+    int gmres_restart = 30;           // -ksp_gmres_restart
     double rich_outer_scale = 1.0;    // -ksp_richardson_scale
     bool monitor = false;             // -ksp_monitor (PETSc's generic Richardson loop)
     bool sor = false;                 // -mg_levels_pc_type sor|ilu|icc: explicit level smoothers
@@ -1383,7 +1525,11 @@ struct p4g_solver_s {
     double snes_rtol = 1e-8, snes_atol = 1e-50, snes_stol = 1e-8, snes_divtol = 1e4;
     int snes_maxit = 50;
     std::string snes_type, ls_type;
-    bool vi_mode = false;                 // set up for p4g_vi_solve
+    bool vi_mode = false;                 // set up for p4g_vi_solve (or p4g_ms_solve: operators per Newton step)
+    bool ms_mode = false;                 // set up for p4g_ms_solve
+    double* ms_buf = nullptr;             // its vectors
+    std::vector<double> ms_fhist;         // |F| per Newton iterate
+    std::vector<int> ms_kits, ms_kreason, ms_sym;   // per Newton step
     double* est_scratch = nullptr;        // device: eigen-estimate vectors, est_cap doubles (VI solves)
     long est_cap = 0;
     double* vi_buf = nullptr;             // device: F0 X F Y W G Fm xl xu, n doubles each
@@ -1716,8 +1862,12 @@ static int configure(p4g_solver_s* s) {
     Options& o = s->opt;
     const std::string kt = o.gets("ksp_type", "cg");
     if (kt == "richardson") s->ksp_rich = true;
+    else if (kt == "gmres") s->ksp_gmres = true;
     else if (kt != "cg")
-        return set_error(PETSC_ERR_SUP, "-ksp_type %s: the device path implements cg and richardson", kt.c_str());
+        return set_error(PETSC_ERR_SUP, "-ksp_type %s: the device path implements cg, gmres and richardson",
+                         kt.c_str());
+    s->gmres_restart = o.geti("ksp_gmres_restart", 30);
+    if (s->gmres_restart < 1) return set_error(PETSC_ERR_ARG_OUTOFRANGE, "-ksp_gmres_restart must be >= 1");
     s->rich_outer_scale = o.getr("ksp_richardson_scale", 1.0);
     s->monitor = o.has("ksp_monitor");
     const std::string pt = o.gets("pc_type", "");
@@ -1856,6 +2006,9 @@ static int configure(p4g_solver_s* s) {
     if (s->sor && (s->mixed || !s->da->kcell.empty()))
         return set_error(PETSC_ERR_SUP, "-mg_levels_pc_type sor needs FP64 and a constant-coefficient operator "
                                         "on the device");
+    if (s->ksp_gmres && (s->mixed || s->normtype != NT_PRECOND))
+        return set_error(PETSC_ERR_SUP, "-ksp_type gmres runs in FP64 with the preconditioned norm (left "
+                                        "preconditioning) on the device");
     if (s->ksp_rich && s->mixed)
         return set_error(PETSC_ERR_SUP, "-ksp_type richardson with -p4g_precision mixed is not implemented on the device");
     if (s->ksp_rich && s->normtype == NT_NATURAL)
@@ -2379,6 +2532,8 @@ int ksp_solve_device(p4g_solver_s* s, const double* b, double* x) {
         if (s->ksp_rich)
             P4G_TRY(s->e64->richardson(b, x, s->normtype, s->rtol, s->atol, s->dtol, s->maxit, s->rich_outer_scale,
                                        s->monitor, co, s->cx.S));
+        else if (s->ksp_gmres)
+            P4G_TRY(s->e64->gmres(b, x, s->rtol, s->atol, s->dtol, s->maxit, s->gmres_restart, co, s->cx.S));
         else
             P4G_TRY(s->e64->cg(b, x, s->normtype, s->rtol, s->atol, s->dtol, s->maxit, co, s->cx.S));
         R.its = co.its; R.reason = co.reason; R.rnorm0 = co.rnorm0; R.rnorm = co.rnorm;
@@ -2742,3 +2897,4 @@ const char* p4g_version(void) { return "p4g 0.1 (gfx950)"; }
 #include "p4g_serve_impl.hpp"
 // reduced-space VI Newton (SNESVINEWTONRSLS) over the solver's levels
 #include "p4g_vi_impl.hpp"
+#include "p4g_ms_impl.hpp"

@@ -16,7 +16,7 @@ def _declared(header):
     text = open(os.path.join(ROOT, "include", header)).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
     text = re.sub(r"^#define[^\n]*(\\\n[^\n]*)*", "", text, flags=re.M)   # macros
-    names = re.findall(r"^(?:const\s+)?[A-Za-z_][A-Za-z_0-9]*(?:\s+|\s*\*+\s*)([A-Za-z_][A-Za-z_0-9]*)\s*\(",
+    names = re.findall(r"^(?!typedef\b)(?:const\s+)?[A-Za-z_][A-Za-z_0-9]*(?:\s+|\s*\*+\s*)([A-Za-z_][A-Za-z_0-9]*)\s*\(",
                        text, flags=re.M)
     return sorted(set(n for n in names if n not in ("if", "while", "return")))
 
@@ -69,7 +69,8 @@ def test_da_from_options():
     ("-pc_type mg -pc_mg_levels 3 -p4g_precision mixed", "sor"),
     ("-pc_type mg -pc_mg_levels 3 -mg_levels_pc_type asm", "asm"),
     ("-pc_type mg -pc_mg_levels 3 -mg_levels_pc_type jacobi -pc_mg_galerkin -p4g_precision mixed", "galerkin"),
-    ("-ksp_type gmres -pc_type mg", "gmres"),
+    ("-ksp_type bcgs -pc_type mg", "bcgs"),
+    ("-ksp_type gmres -ksp_norm_type unpreconditioned", "left"),
     ("-pc_type mg -pc_mg_levels 3 -mg_levels_pc_type jacobi -pc_mg_type additive", "additive"),
     ("-pc_type mg -pc_mg_levels 3 -mg_coarse_ksp_type cg", "convergence_test skip"),
     ("-pc_type mg -pc_mg_levels 3 -mg_coarse_ksp_type cg -mg_coarse_ksp_convergence_test skip "

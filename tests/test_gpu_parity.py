@@ -211,6 +211,18 @@ SOR_CASES.update({
 })
 
 
+# -ksp_type gmres: restarted GMRES(m), left preconditioning (PETSc's default
+# KSP; the oracle's restatement is pinned in tests/test_minimal.py)
+SOR_CASES.update({
+    "2d_gmres_mg_jacobi": ("-fsh_dim 2 -fsh_problem manupoly -da_refine 5 -ksp_type gmres -pc_type mg "
+                           "-mg_levels_pc_type jacobi -ksp_rtol 1e-10"),
+    "3d_gmres_ilu_restart5": "-fsh_dim 3 -fsh_problem manusin -da_refine 3 -ksp_type gmres -ksp_gmres_restart 5 "
+                             "-ksp_rtol 1e-10",
+    "2d_gmres_none_restarts": "-fsh_dim 2 -fsh_problem manupoly -fsh_cx 10 -da_refine 4 -ksp_type gmres -pc_type none "
+                              "-ksp_rtol 1e-8",
+})
+
+
 # c/ch6/study/coarseperformance.sh: a fixed count of coarse PCG iterations
 # (-mg_coarse_ksp_type cg -mg_coarse_ksp_convergence_test skip) in place of
 # the coarse LU; the W-cycle restarts the coarse CG from a nonzero iterate
