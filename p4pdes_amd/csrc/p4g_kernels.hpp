@@ -1994,6 +1994,88 @@ k_ilu27_plane(SorOp o, const double* __restrict__ u, int p, int backward, const 
     z[q] = T(s / uq);
 }
 
+// ILU(0) of a 2D operator on the full 9-point box pattern (the DMDA box
+// stencil's matrix), natural row order, the IKJ elimination of PETSc's
+// MatLUFactorNumeric over the ILU(0) pattern.  Row q = (i, j) eliminates its
+// lower neighbours SW, S, SE, W in column order; each updates the entries of
+// row q that its own upper part reaches.  Row q depends on rows on the
+// hyperplanes i + 2j - 1, -2, -3, so a plane p = i + 2j is one launch.  The
+// factor f holds 9 planes: L multipliers SW, S, SE, W (0..3), then U's
+// diagonal, E, NW, N, NE (4..8).  The operator is g27's 2D planes 9..17.
+__global__ void __launch_bounds__(256)
+k_ilu9_factor_plane(int mx, int my, const double* __restrict__ g, double* __restrict__ f, int p) {
+    const int j = blockIdx.x * 256 + threadIdx.x;
+    if (j >= my) return;
+    const int i = p - 2 * j;
+    if (i < 0 || i >= mx) return;
+    const long n = (long)mx * my, q = (long)j * mx + i;
+    auto A = [&](int c) { return g[(long)c * n + q]; };
+    auto F = [&](int c, long r) { return f[(long)c * n + r]; };
+    const bool hw = i > 0, he = i + 1 < mx, hs = j > 0, hn = j + 1 < my;
+    double aS = hs ? A(10) : 0.0, aSE = (hs && he) ? A(11) : 0.0, aW = hw ? A(12) : 0.0, aC = A(13);
+    double aE = he ? A(14) : 0.0, aNW = (hn && hw) ? A(15) : 0.0, aN = hn ? A(16) : 0.0;
+    const double aNE = (hn && he) ? A(17) : 0.0;
+    double lSW = 0.0, lS = 0.0, lSE = 0.0, lW = 0.0;
+    if (hs && hw) {   // SW: its E is S, its N is W, its NE is q
+        const long k = q - mx - 1;
+        lSW = A(9) / F(4, k);
+        aS -= lSW * F(5, k);
+        aW -= lSW * F(7, k);
+        aC -= lSW * F(8, k);
+    }
+    if (hs) {         // S: its E is SE, its NW is W, its N is q, its NE is E
+        const long k = q - mx;
+        lS = aS / F(4, k);
+        if (he) aSE -= lS * F(5, k);
+        if (hw) aW -= lS * F(6, k);
+        aC -= lS * F(7, k);
+        if (he) aE -= lS * F(8, k);
+    }
+    if (hs && he) {   // SE: its NW is q, its N is E
+        const long k = q - mx + 1;
+        lSE = aSE / F(4, k);
+        aC -= lSE * F(6, k);
+        aE -= lSE * F(7, k);
+    }
+    if (hw) {         // W: its E is q, its N is NW, its NE is N
+        const long k = q - 1;
+        lW = aW / F(4, k);
+        aC -= lW * F(5, k);
+        if (hn) aNW -= lW * F(7, k);
+        if (hn) aN -= lW * F(8, k);
+    }
+    f[0 * n + q] = lSW; f[1 * n + q] = lS; f[2 * n + q] = lSE; f[3 * n + q] = lW;
+    f[4 * n + q] = aC; f[5 * n + q] = aE; f[6 * n + q] = aNW; f[7 * n + q] = aN; f[8 * n + q] = aNE;
+}
+
+// z = (LU)^-1 r: forward on planes p ascending, backward descending
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_ilu9_plane(int mx, int my, const double* __restrict__ f, int p, int backward, const T* __restrict__ r,
+             T* __restrict__ z) {
+    const int j = blockIdx.x * 256 + threadIdx.x;
+    if (j >= my) return;
+    const int i = p - 2 * j;
+    if (i < 0 || i >= mx) return;
+    const long n = (long)mx * my, q = (long)j * mx + i;
+    const bool hw = i > 0, he = i + 1 < mx, hs = j > 0, hn = j + 1 < my;
+    if (!backward) {
+        double s = double(r[q]);
+        if (hs && hw) s -= f[q] * double(z[q - mx - 1]);
+        if (hs) s -= f[n + q] * double(z[q - mx]);
+        if (hs && he) s -= f[2 * n + q] * double(z[q - mx + 1]);
+        if (hw) s -= f[3 * n + q] * double(z[q - 1]);
+        z[q] = T(s);
+        return;
+    }
+    double s = double(z[q]);
+    if (he) s -= f[5 * n + q] * double(z[q + 1]);
+    if (hn && hw) s -= f[6 * n + q] * double(z[q + mx - 1]);
+    if (hn) s -= f[7 * n + q] * double(z[q + mx]);
+    if (hn && he) s -= f[8 * n + q] * double(z[q + mx + 1]);
+    z[q] = T(s / f[4 * n + q]);
+}
+
 // z = D^-1 r of a constant-operator level (boundary rows db, interior d)
 template <typename T>
 __global__ void __launch_bounds__(256)

@@ -130,7 +130,7 @@ static int ms_operator(p4g_solver_s* s, MsRun& R) {
     const int L = s->nlev - 1;
     Engine<double>& e = *s->e64;
     P4G_TRY(ms_fdjac_level(s, R, L, R.X, R.F));
-    if (s->pctype == PCT_ILU) return ilu27_pivots(s->lv[L], s->g27[L], s->ilu_u[L], R.st);
+    if (s->pctype == PCT_ILU) return ilu9_factor(s->lv[L], s->g27[L], s->ms_ilu9, R.st);
     if (s->pctype != PCT_MG) return 0;
     for (int l = L - 1; l >= 0; --l) {
         const MsGrid &gc = R.grid[l], &gf = R.grid[l + 1];
@@ -296,6 +296,10 @@ static int ms_solve(p4g_solver_s* s, double q, const double* gb, double* x, p4g_
             HIP_TRY(hipMemset(s->g27[l], 0, bytes));   // the 2D operator uses planes 9..17 only
             s->lv[l].gal = true;
             s->e64->g27[l] = s->g27[l];
+        }
+        if (s->pctype == PCT_ILU) {
+            P4G_TRY(dalloc(s, (void**)&s->ms_ilu9, 9 * n * sizeof(double)));
+            s->e64->ilu9 = s->ms_ilu9;
         }
         if (s->pctype == PCT_MG && s->smoother == SM_CHEB) {
             s->est_cap = 6 * n;

