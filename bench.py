@@ -269,6 +269,48 @@ def obstacle(with_oracle):
     return o
 
 
+MINIMAL = ("-da_grid_x 33 -da_grid_y 33 -snes_grid_sequence 6 -snes_fd_color -snes_converged_reason "
+           "-ksp_converged_reason -pc_type mg")
+
+
+def minimal(with_oracle):
+    """The minimal-surface equation as c/ch8/cluster.sh:66-70 runs it: catenoid
+    boundary values, 33^2 base grid, 6 grid refinements to 2049^2, Newton
+    with colored finite-difference Jacobians per grid, GMRES + multigrid over
+    the finite-difference Jacobians of every level.  Host wall time of the
+    whole sequence (set-up, callbacks and transfers included), and the
+    oracle's time for the same run through 1025^2 (5 refinements: about
+    15 s of one core; the full run takes the oracle about 4 times that)."""
+    from p4pdes_amd import capi
+    capi.minimal_run(MINIMAL.replace("-snes_grid_sequence 6", "-snes_grid_sequence 2"))   # warm-up
+    walls = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        rc, text, res, _ = capi.minimal_run(MINIMAL)
+        walls.append(time.perf_counter() - t0)
+        if rc:
+            return {"error": text}
+    wall = sorted(walls)[1]
+    dof = res.mx * res.my
+    o = {"workload": "minimal-surface equation of c/ch7/minimal.c (catenoid), grid-sequenced 33^2 -> 2049^2 "
+                     "(%d DOF on the final grid), Newton per grid with colored finite-difference Jacobians, "
+                     "GMRES + multigrid over the level Jacobians (Chebyshev/SOR)" % dof,
+         "options": MINIMAL, "wall_s": wall, "dof_per_s": dof / wall, "stage_its": res.as_dict()["stage_its"],
+         "ksp_its_total": res.ksp_its_total, "report": text.splitlines()[-1]}
+    if with_oracle:
+        from oracle import oracle_py
+        sample = MINIMAL.replace("-snes_grid_sequence 6", "-snes_grid_sequence 5")
+        rc5, text5, res5, _ = capi.minimal_run(sample)
+        t0 = time.perf_counter()
+        rco, texto, reso, _ = oracle_py.minimal_run(sample)
+        o["oracle_sample"] = "the same options through 1025^2 (-snes_grid_sequence 5)"
+        o["oracle_sample_wall_s"] = time.perf_counter() - t0
+        o["oracle_threads"] = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        o["oracle_report_equal"] = texto.splitlines()[-1] == text5.splitlines()[-1]
+        o["oracle_stage_its_equal"] = reso.as_dict()["stage_its"] == res5.as_dict()["stage_its"]
+    return o
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -278,6 +320,7 @@ def main():
     ap.add_argument("--no-513", action="store_true", help="skip the 513^3 mixed/FP64 runs")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving stream")
     ap.add_argument("--no-obstacle", action="store_true", help="skip the obstacle (VI) solve")
+    ap.add_argument("--no-minimal", action="store_true", help="skip the minimal-surface solve")
     ap.add_argument("--profile-workload", action="store_true",
                     help="only run the counter-collection workload: two 257^3 solves and a 1 GiB "
                          "device copy (the calibration copy of known size); prints no JSON")
@@ -409,6 +452,8 @@ def main():
     if not a.no_obstacle:
         print("bench: obstacle 2049^2", file=sys.stderr, flush=True)
         out["obstacle_2049"] = obstacle(not a.no_cpu)
+    if not a.no_minimal:
+        out["minimal_2049"] = minimal(not a.no_cpu)
 
     # ---- CPU baseline ------------------------------------------------------
     if not a.no_cpu:

@@ -1994,6 +1994,167 @@ k_ilu27_plane(SorOp o, const double* __restrict__ u, int p, int backward, const 
     z[q] = T(s / uq);
 }
 
+// One directional sweep of PETSc's SOR (omega 1, zero initial guess) on a
+// 2D 27-plane level (planes 9..17), as strips of 64 rows, one wave each.
+// Lane r owns row j of its strip and at step t takes the point c = t - 2r
+// along its row (sweep order), so the row below (forward) or above
+// (backward) is one lane over and two steps ahead: its three values reach
+// lane r by shuffles of each lane's last three results.  Lane 0 takes that
+// row from the previous strip through global memory, after that strip's
+// last lane has published (release) how many points of its row are done;
+// a strip starts SOR9_SLACK points behind, so in steady state it rarely
+// polls.  The arithmetic is k_sor_plane's: forward x = (b - sum over
+// c = 9..12 of a_c x_c) / a_13; backward the same over c = 9..12, 14..17,
+// the lower neighbours holding forward values; a null row stays zero.
+// flag: per strip, base + points done (bases grow from sweep to sweep); a
+// wait longer than 2^24 polls sets *err and ends the waiting (no hang).
+constexpr int SOR9_SLACK = 48;
+
+__device__ __forceinline__ double ld_agent(const double* p) {
+    const unsigned long long v =
+        __hip_atomic_load((const unsigned long long*)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return __longlong_as_double((long long)v);
+}
+
+// the per-point record of k_sor9_strips: the 2D planes c = 9..17 with the
+// diagonal replaced by its reciprocal (0 for a null row), and a pad
+__global__ void __launch_bounds__(256)
+k_sor9_pack(long n, const double* __restrict__ g, double* __restrict__ pk) {
+    for (long q = blockIdx.x * 256L + threadIdx.x; q < n; q += (long)gridDim.x * 256L) {
+        double* o = pk + q * 10;
+#pragma unroll
+        for (int c = 0; c < 9; ++c) o[c] = g[(long)(9 + c) * n + q];
+        const double dq = g[13 * n + q];
+        o[4] = dq != 0.0 ? 1.0 / dq : 0.0;
+        o[9] = dq;
+    }
+}
+
+__global__ void __launch_bounds__(64)
+k_sor9_strips(int mx, int my, const double* __restrict__ pk, const double* __restrict__ b, double* x,
+              unsigned long long* flag, unsigned long long base, int backward, int* err) {
+    const int s = blockIdx.x, r = threadIdx.x;
+    const int row0 = 64 * s;
+    const int rows = min(64, my - row0);
+    const int j = backward ? my - 1 - (row0 + r) : row0 + r;
+    const bool live = r < rows;
+    const bool hs = j > 0, hn = j + 1 < my;
+    const int nsteps = mx + 2 * (rows - 1);
+    unsigned long long* prev = flag + s - 1;
+    unsigned long long known = 0;
+    bool dead = false;
+    double h1 = 0.0, h2 = 0.0, h3 = 0.0, xo = 0.0;
+    // the operands of the next step, loaded one step ahead
+    double nb = 0.0, na[10], nx[4];
+    auto fetch = [&](int t) {
+        const int c = t - 2 * r;
+        if (!(live && c >= 0 && c < mx)) return;
+        const int i = backward ? mx - 1 - c : c;
+        const long q = (long)j * mx + i;
+        nb = b[q];
+        const double* rec = pk + q * 10;
+#pragma unroll
+        for (int k = 0; k < 10; ++k) na[k] = rec[k];
+        if (backward) {   // forward values below and to the left
+            nx[0] = (hs && i > 0) ? x[q - mx - 1] : 0.0;
+            nx[1] = hs ? x[q - mx] : 0.0;
+            nx[2] = (hs && i + 1 < mx) ? x[q - mx + 1] : 0.0;
+            nx[3] = i > 0 ? x[q - 1] : 0.0;
+        }
+    };
+    fetch(0);
+    for (int t = 0; t < nsteps; ++t) {
+        const double u1 = __shfl_up(h1, 1, 64), u2 = __shfl_up(h2, 1, 64), u3 = __shfl_up(h3, 1, 64);
+        const int c = t - 2 * r;
+        const double cb = nb;
+        double a[10], cx[4];
+#pragma unroll
+        for (int k = 0; k < 10; ++k) a[k] = na[k];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) cx[k] = nx[k];
+        double res = 0.0;
+        if (live && c >= 0 && c < mx) {
+            const int i = backward ? mx - 1 - c : c;
+            const long q = (long)j * mx + i;
+            const bool hv = backward ? hn : hs;   // the row toward the sweep's origin
+            double v_im1 = 0.0, v_i = 0.0, v_ip1 = 0.0;
+            if (hv) {
+                if (r > 0) {
+                    if (!backward) { v_im1 = u3; v_i = u2; v_ip1 = u1; }
+                    else { v_im1 = u1; v_i = u2; v_ip1 = u3; }
+                } else {
+                    const unsigned long long need = base + (unsigned long long)min(c == 0 ? SOR9_SLACK : c + 2, mx);
+                    if (!dead && known < need) {
+                        long spins = 0;
+                        while ((known = __hip_atomic_load(prev, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT)) < need) {
+                            if (++spins > (1L << 24)) {
+                                atomicExch(err, 1);
+                                dead = true;
+                                break;
+                            }
+                            __builtin_amdgcn_s_sleep(1);
+                        }
+                    }
+                    const long qv = backward ? q + mx : q - mx;
+                    if (i > 0) v_im1 = ld_agent(x + qv - 1);
+                    v_i = ld_agent(x + qv);
+                    if (i + 1 < mx) v_ip1 = ld_agent(x + qv + 1);
+                }
+            }
+            const bool hw = i > 0, he = i + 1 < mx;
+            double sum = cb;
+            if (!backward) {
+                if (hs && hw && a[0] != 0.0) sum -= a[0] * v_im1;
+                if (hs && a[1] != 0.0) sum -= a[1] * v_i;
+                if (hs && he && a[2] != 0.0) sum -= a[2] * v_ip1;
+                if (hw && a[3] != 0.0) sum -= a[3] * xo;
+            } else {
+                if (hs && hw && a[0] != 0.0) sum -= a[0] * cx[0];
+                if (hs && a[1] != 0.0) sum -= a[1] * cx[1];
+                if (hs && he && a[2] != 0.0) sum -= a[2] * cx[2];
+                if (hw && a[3] != 0.0) sum -= a[3] * cx[3];
+                if (he && a[5] != 0.0) sum -= a[5] * xo;
+                if (hn && hw && a[6] != 0.0) sum -= a[6] * v_im1;
+                if (hn && a[7] != 0.0) sum -= a[7] * v_i;
+                if (hn && he && a[8] != 0.0) sum -= a[8] * v_ip1;
+            }
+            res = a[9] != 0.0 ? sum * a[4] : 0.0;
+            x[q] = res;
+            xo = res;
+            if (r == 63) {   // a full strip publishes its last row's progress
+                __threadfence();
+                __hip_atomic_store(flag + s, base + (unsigned long long)(c + 1), __ATOMIC_RELEASE,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+            }
+        }
+        fetch(t + 1);
+        h3 = h2;
+        h2 = h1;
+        h1 = res;
+    }
+}
+
+// The dense inverse of a band LU factor (unit lower L and upper U in band
+// storage, row i at B[i * w + (col - i + bw)], w = 2 bw + 1): thread c
+// solves L U z = e_c by band substitution and writes column c of the
+// row-major inverse, which is also its workspace (coalesced across c).
+__global__ void __launch_bounds__(256)
+k_band_inverse(int n, int bw, const double* __restrict__ B, double* __restrict__ Ainv) {
+    const int c = blockIdx.x * 256 + threadIdx.x;
+    if (c >= n) return;
+    const int w = 2 * bw + 1;
+    for (int i = 0; i < n; ++i) {
+        double sum = (i == c) ? 1.0 : 0.0;
+        for (int j = max(0, i - bw); j < i; ++j) sum -= B[(long)i * w + (j - i + bw)] * Ainv[(long)j * n + c];
+        Ainv[(long)i * n + c] = sum;
+    }
+    for (int i = n - 1; i >= 0; --i) {
+        double sum = Ainv[(long)i * n + c];
+        for (int j = i + 1; j <= min(i + bw, n - 1); ++j) sum -= B[(long)i * w + (j - i + bw)] * Ainv[(long)j * n + c];
+        Ainv[(long)i * n + c] = sum / B[(long)i * w + bw];
+    }
+}
+
 // ILU(0) of a 2D operator on the full 9-point box pattern (the DMDA box
 // stencil's matrix), natural row order, the IKJ elimination of PETSc's
 // MatLUFactorNumeric over the ILU(0) pattern.  Row q = (i, j) eliminates its

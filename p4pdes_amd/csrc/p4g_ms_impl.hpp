@@ -12,13 +12,16 @@
 namespace p4g {
 
 // The coarsest level of a nonsymmetric 27-plane operator: band LU without
-// pivoting (PETSc's serial coarse LU factors these diagonally dominant
-// Jacobians without row exchanges) and the dense inverse by band solves.
-static int coarse_factor_lu(const LevelInfo& L, const double* g27, std::vector<double>& Ainv) {
+// pivoting on the host (PETSc's serial coarse LU factors these diagonally
+// dominant Jacobians without row exchanges), then the engine's dense
+// inverse on the device, a column per thread (k_band_inverse).
+static int coarse_factor_lu(p4g_solver_s* s, const LevelInfo& L, const double* g27, Engine<double>& e,
+                            hipStream_t st) {
     const int n = (int)L.n;
     if (n > MAX_COARSE)
         return set_error(PETSC_ERR_SUP, "coarsest grid has %d points (> %d); use more multigrid levels", n,
                          MAX_COARSE);
+    if (!e.Ainv || e.n0 != n) return set_error(PETSC_ERR_LIB, "internal: coarse inverse of a different size");
     const int mx = L.mx, my = L.my;
     const int bw = L.dim == 3 ? mx * my + mx + 1 : (L.dim == 2 ? mx + 1 : 1);
     const int w = 2 * bw + 1;
@@ -48,21 +51,14 @@ static int coarse_factor_lu(const LevelInfo& L, const double* g27, std::vector<d
             for (long j = kk + 1; j <= iend; ++j) at(i, j) -= l * at(kk, j);
         }
     }
-    Ainv.assign((size_t)n * n, 0.0);
-    std::vector<double> z(n);
-    for (int c = 0; c < n; ++c) {
-        for (long i = 0; i < n; ++i) {
-            double sum = (i == c) ? 1.0 : 0.0;
-            for (long j = std::max<long>(0, i - bw); j < i; ++j) sum -= at(i, j) * z[j];
-            z[i] = sum;
-        }
-        for (long i = n - 1; i >= 0; --i) {
-            double sum = z[i];
-            for (long j = i + 1; j <= std::min<long>(i + bw, n - 1); ++j) sum -= at(i, j) * z[j];
-            z[i] = sum / at(i, i);
-        }
-        for (long i = 0; i < n; ++i) Ainv[(size_t)i * n + c] = z[i];
+    if (!s->ms_band || s->ms_band_cap < (long)B.size()) {
+        P4G_TRY(dalloc(s, (void**)&s->ms_band, B.size() * sizeof(double)));
+        s->ms_band_cap = (long)B.size();
     }
+    HIP_TRY(hipMemcpyAsync(s->ms_band, B.data(), B.size() * sizeof(double), hipMemcpyHostToDevice, st));
+    k_band_inverse<<<(n + 255) / 256, 256, 0, st>>>(n, bw, s->ms_band, e.Ainv);
+    LAUNCH_CHECK();
+    HIP_TRY(hipStreamSynchronize(st));   // B lives on the host stack
     return 0;
 }
 
@@ -121,6 +117,7 @@ static int ms_fdjac_level(p4g_solver_s* s, MsRun& R, int l, const double* x, con
     const double unorm = std::sqrt(s->cx.hS[16 + S_TMP1]);
     const double d = 1.0 / (std::sqrt(1.0 + unorm) * 1.490116119384766e-08);   // "wp": PETSC_SQRT_MACHINE_EPSILON
     k_ms_fdjac<<<nb, 256, 0, R.st>>>(g, x, R.gb[l], F, d, 1.0 / d, s->g27[l]);
+    if (s->lv[l].pipe) k_sor9_pack<<<nb, 256, 0, R.st>>>(g.n, s->g27[l], s->lv[l].pipe->pk);
     LAUNCH_CHECK();
     return 0;
 }
@@ -141,10 +138,8 @@ static int ms_operator(p4g_solver_s* s, MsRun& R) {
         P4G_TRY(ms_fdjac_level(s, R, l, R.xl[l], R.fl[l]));
     }
     if (L > 0) {
-        std::vector<double> Ainv;
         HIP_TRY(hipStreamSynchronize(R.st));
-        P4G_TRY(coarse_factor_lu(s->lv[0], s->g27[0], Ainv));
-        P4G_TRY(e.set_coarse_inverse(Ainv));
+        P4G_TRY(coarse_factor_lu(s, s->lv[0], s->g27[0], e, R.st));
     }
     if (s->smoother == SM_CHEB)   // one level: the level smoother is the preconditioner
         for (int l = L > 0 ? 1 : 0; l <= L; ++l) P4G_TRY(estimate_level(s, l));
@@ -297,6 +292,19 @@ static int ms_solve(p4g_solver_s* s, double q, const double* gb, double* x, p4g_
             s->lv[l].gal = true;
             s->e64->g27[l] = s->g27[l];
         }
+        if (s->sor && s->pctype == PCT_MG) {   // pipelined strip sweeps on every level
+            s->ms_pipes.resize(s->nlev);
+            for (int l = l0; l <= L; ++l) {
+                if (s->lv[l].mx >= (1 << 22)) continue;
+                const int ns = (s->lv[l].my + 63) / 64;
+                SorPipe& P = s->ms_pipes[l];
+                P4G_TRY(dalloc(s, (void**)&P.flags, (ns + 1) * sizeof(unsigned long long) + sizeof(int)));
+                HIP_TRY(hipMemset(P.flags, 0, (ns + 1) * sizeof(unsigned long long) + sizeof(int)));
+                P.err = (int*)(P.flags + ns + 1);
+                P4G_TRY(dalloc(s, (void**)&P.pk, 10 * s->lv[l].n * sizeof(double)));
+                s->lv[l].pipe = &P;
+            }
+        }
         if (s->pctype == PCT_ILU) {
             P4G_TRY(dalloc(s, (void**)&s->ms_ilu9, 9 * n * sizeof(double)));
             s->e64->ilu9 = s->ms_ilu9;
@@ -400,6 +408,12 @@ static int ms_solve(p4g_solver_s* s, double q, const double* gb, double* x, p4g_
     }
     if (!reason && i >= s->snes_maxit) reason = -5;   // DIVERGED_MAX_IT
     HIP_TRY(hipMemcpy(x, R.X, n * sizeof(double), hipMemcpyDeviceToHost));
+    for (const SorPipe& P : s->ms_pipes) {
+        if (!P.err) continue;
+        int e = 0;
+        HIP_TRY(hipMemcpy(&e, P.err, sizeof(int), hipMemcpyDeviceToHost));
+        if (e) return set_error(PETSC_ERR_LIB, "p4g_ms_solve: a pipelined SOR sweep stalled");
+    }
     s->cx.tm.collect();
     out.its = iter;
     out.reason = reason;

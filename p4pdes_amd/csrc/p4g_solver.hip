@@ -23,6 +23,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <type_traits>
 #include <vector>
 
 #include "p4g_internal.hpp"
@@ -69,6 +70,15 @@ static double now_ms() {
 }
 
 // precision-independent description of one multigrid level
+// the progress flags of a level's pipelined 2D SOR sweeps (k_sor9_strips):
+// one word per 64-row strip, a stall word, and the host's sweep counter
+struct SorPipe {
+    double* pk = nullptr;                 // the operator as k_sor9_pack records (refreshed with it)
+    unsigned long long* flags = nullptr;
+    int* err = nullptr;
+    unsigned long long epoch = 0;
+};
+
 struct LevelInfo {
     int  dim = 3, mx = 0, my = 1, mz = 1;
     long n = 0;
@@ -82,12 +92,25 @@ struct LevelInfo {
     bool gal = false;                                // Galerkin operator P^T A P in 27-point form
     int sor_nblk = 1;                                // SOR local sweeps: blocks of the slowest dimension
     int sor_blk[17] = {0};                           //   and their first indices
+    SorPipe* pipe = nullptr;                         // 2D 27-plane SOR by pipelined strips, when set
 };
 
 // one symmetric SOR sweep z = SSOR(r) on a level (omega 1, zero initial
 // guess): the hyperplanes forward then backward (k_sor_plane)
 template <typename T>
 static int sor_run(const LevelInfo& L, const double* g27, const T* r, T* z, hipStream_t st) {
+    if constexpr (std::is_same<T, double>::value) {
+        if (L.gal && L.dim == 2 && L.pipe && L.pipe->pk && L.sor_nblk <= 1 && g27) {   // one launch per direction
+            const int ns = (L.my + 63) / 64;
+            SorPipe* P = L.pipe;
+            const unsigned long long b0 = (++P->epoch) << 22;
+            k_sor9_strips<<<ns, 64, 0, st>>>(L.mx, L.my, P->pk, r, z, P->flags, b0, 0, P->err);
+            const unsigned long long b1 = (++P->epoch) << 22;
+            k_sor9_strips<<<ns, 64, 0, st>>>(L.mx, L.my, P->pk, r, z, P->flags, b1, 1, P->err);
+            LAUNCH_CHECK();
+            return 0;
+        }
+    }
     SorOp o;
     o.dim = L.dim; o.mx = L.mx; o.my = L.my; o.mz = L.mz; o.n = L.n;
     o.d = L.d; o.db = L.db; o.sx = L.sx; o.sy = L.sy; o.sz = L.sz;
@@ -1550,6 +1573,9 @@ struct p4g_solver_s {
     bool ms_mode = false;                 // set up for p4g_ms_solve
     double* ms_buf = nullptr;             // its vectors
     double* ms_ilu9 = nullptr;            // its box-pattern ILU(0) factor (-pc_type ilu)
+    std::vector<SorPipe> ms_pipes;        // its levels' pipelined SOR flags (-mg_levels_pc_type sor)
+    double* ms_band = nullptr;            // the coarse band LU factor, uploaded for the inverse
+    long ms_band_cap = 0;
     std::vector<double> ms_fhist;         // |F| per Newton iterate
     std::vector<int> ms_kits, ms_kreason, ms_sym;   // per Newton step
     double* est_scratch = nullptr;        // device: eigen-estimate vectors, est_cap doubles (VI solves)
