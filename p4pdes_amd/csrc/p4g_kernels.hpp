@@ -2008,7 +2008,7 @@ k_ilu27_plane(SorOp o, const double* __restrict__ u, int p, int backward, const 
 // the lower neighbours holding forward values; a null row stays zero.
 // flag: per strip, base + points done (bases grow from sweep to sweep); a
 // wait longer than 2^24 polls sets *err and ends the waiting (no hang).
-constexpr int SOR9_SLACK = 48;
+constexpr int SOR9_SLACK = 64;
 
 __device__ __forceinline__ double ld_agent(const double* p) {
     const unsigned long long v =
@@ -2040,11 +2040,14 @@ k_sor9_strips(int mx, int my, const double* __restrict__ pk, const double* __res
     const bool live = r < rows;
     const bool hs = j > 0, hn = j + 1 < my;
     const int nsteps = mx + 2 * (rows - 1);
-    unsigned long long* prev = flag + s - 1;
-    unsigned long long known = 0;
+    // the previous strip's last row, copied into an LDS ring by the wave in
+    // coalesced batches once that strip has published it (lane 0's row
+    // neighbour); positions are along the sweep: p -> column p or mx-1-p
+    __shared__ double ring[256];
+    const int jn = backward ? my - row0 : row0 - 1;   // lane 0's neighbour row (s > 0), the same for all lanes
+    int copied = 0;                             // positions in the ring (wave-uniform)
     bool dead = false;
     double h1 = 0.0, h2 = 0.0, h3 = 0.0, xo = 0.0;
-    // the operands of the next step, loaded one step ahead
     double nb = 0.0, na[10], nx[4];
     auto fetch = [&](int t) {
         const int c = t - 2 * r;
@@ -2064,6 +2067,31 @@ k_sor9_strips(int mx, int my, const double* __restrict__ pk, const double* __res
     };
     fetch(0);
     for (int t = 0; t < nsteps; ++t) {
+        // lane 0 is at position t: it needs positions t-1..t+1 of the neighbour row
+        if (s > 0 && t < mx && copied < min(t + 2, mx)) {   // uniform
+            unsigned long long kn = 0;
+            if (r == 0) {
+                const unsigned long long need = base + (unsigned long long)min(t == 0 ? SOR9_SLACK : t + 2, mx);
+                long spins = 0;
+                while (!dead && (kn = __hip_atomic_load(flag + s - 1, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT)) < need) {
+                    if (++spins > (1L << 24)) {
+                        atomicExch(err, 1);
+                        dead = true;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                }
+                if (dead) kn = base + (unsigned long long)mx;
+            }
+            kn = __shfl(kn, 0, 64);
+            const int avail = (int)min((unsigned long long)mx, kn - base);
+            const int upto = min(avail, t + 130);
+            for (int p = copied + r; p < upto; p += 64) {
+                const int col = backward ? mx - 1 - p : p;
+                ring[p & 255] = ld_agent(x + (long)jn * mx + col);
+            }
+            __syncthreads();
+            copied = upto;
+        }
         const double u1 = __shfl_up(h1, 1, 64), u2 = __shfl_up(h2, 1, 64), u3 = __shfl_up(h3, 1, 64);
         const int c = t - 2 * r;
         const double cb = nb;
@@ -2082,23 +2110,12 @@ k_sor9_strips(int mx, int my, const double* __restrict__ pk, const double* __res
                 if (r > 0) {
                     if (!backward) { v_im1 = u3; v_i = u2; v_ip1 = u1; }
                     else { v_im1 = u1; v_i = u2; v_ip1 = u3; }
-                } else {
-                    const unsigned long long need = base + (unsigned long long)min(c == 0 ? SOR9_SLACK : c + 2, mx);
-                    if (!dead && known < need) {
-                        long spins = 0;
-                        while ((known = __hip_atomic_load(prev, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT)) < need) {
-                            if (++spins > (1L << 24)) {
-                                atomicExch(err, 1);
-                                dead = true;
-                                break;
-                            }
-                            __builtin_amdgcn_s_sleep(1);
-                        }
-                    }
-                    const long qv = backward ? q + mx : q - mx;
-                    if (i > 0) v_im1 = ld_agent(x + qv - 1);
-                    v_i = ld_agent(x + qv);
-                    if (i + 1 < mx) v_ip1 = ld_agent(x + qv + 1);
+                } else {   // positions c-1, c, c+1 of the ring
+                    const double pm = c > 0 ? ring[(c - 1) & 255] : 0.0;
+                    const double pp = c + 1 < mx ? ring[(c + 1) & 255] : 0.0;
+                    v_i = ring[c & 255];
+                    if (!backward) { v_im1 = pm; v_ip1 = pp; }
+                    else { v_im1 = pp; v_ip1 = pm; }
                 }
             }
             const bool hw = i > 0, he = i + 1 < mx;
@@ -2121,7 +2138,8 @@ k_sor9_strips(int mx, int my, const double* __restrict__ pk, const double* __res
             res = a[9] != 0.0 ? sum * a[4] : 0.0;
             x[q] = res;
             xo = res;
-            if (r == 63) {   // a full strip publishes its last row's progress
+            // a full strip publishes its last row's progress every 16 points
+            if (r == 63 && ((c & 15) == 15 || c == mx - 1)) {
                 __threadfence();
                 __hip_atomic_store(flag + s, base + (unsigned long long)(c + 1), __ATOMIC_RELEASE,
                                    __HIP_MEMORY_SCOPE_AGENT);

@@ -134,6 +134,11 @@ MS_CASES = {
     "tent_seq4_mg": "-ms_problem tent -snes_grid_sequence 4 -snes_fd_color -pc_type mg",
     "catenoid_c101_seq2_restart5": ("-ms_catenoid_c 1.01 -snes_grid_sequence 2 -da_refine 1 -ksp_gmres_restart 5 "
                                     "-snes_fd_color"),
+    # several 64-row strips of the pipelined SOR sweeps (65^2: a full strip
+    # and a one-row strip; 129^2: two full strips and a one-row strip)
+    "tent_65_mg_sor_2strips": "-da_refine 5 -ms_problem tent -snes_fd_color -pc_type mg",
+    "tent_129_mg_sor_3strips": "-da_refine 6 -ms_problem tent -snes_fd_color -pc_type mg",
+    "catenoid_seq6_mg_sor_3strips": "-snes_grid_sequence 6 -snes_fd_color -pc_type mg",
     "tent_H10_basic_monitor": ("-da_refine 2 -ms_problem tent -ms_tent_H 10 -snes_linesearch_type basic "
                                "-snes_fd_color -ms_monitor -ms_quaddegree 2"),
 }
