@@ -269,6 +269,41 @@ def obstacle(with_oracle):
     return o
 
 
+OBSTACLE_CLUSTER = ("-da_grid_x 33 -da_grid_y 33 -snes_grid_sequence 6 -snes_converged_reason -ksp_converged_reason "
+                    "-pc_type mg")
+
+
+def obstacle_cluster(with_oracle):
+    """The obstacle problem as c/ch8/cluster.sh:78-82 runs it: 33^2 base grid, six
+    refinements to 2049^2, PETSc's default Chebyshev/SOR level smoothing (the
+    SOR sweeps as pipelined strips).  Host wall time of the whole sequence,
+    and the oracle's time for the same run."""
+    from p4pdes_amd import capi
+    capi.obstacle_run(OBSTACLE_CLUSTER.replace("-snes_grid_sequence 6", "-snes_grid_sequence 1"))   # warm-up
+    walls = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        rc, text, res, _ = capi.obstacle_run(OBSTACLE_CLUSTER)
+        walls.append(time.perf_counter() - t0)
+        if rc:
+            return {"error": text}
+    wall = sorted(walls)[1]
+    dof = res.mx * res.my
+    o = {"workload": "obstacle problem of c/ch12/obstacle.c as c/ch8/cluster.sh runs it, grid-sequenced 33^2 -> "
+                     "2049^2 (%d DOF on the final grid), CG + Galerkin multigrid on the inactive sets with "
+                     "Chebyshev/SOR smoothing" % dof,
+         "options": OBSTACLE_CLUSTER, "wall_s": wall, "dof_per_s": dof / wall,
+         "stage_its": res.as_dict()["stage_its"], "report": text.splitlines()[-2:]}
+    if with_oracle:
+        from oracle import oracle_py
+        t0 = time.perf_counter()
+        rco, texto, reso, _ = oracle_py.obstacle_run(OBSTACLE_CLUSTER)
+        o["oracle_wall_s"] = time.perf_counter() - t0
+        o["oracle_threads"] = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        o["oracle_report_equal"] = texto.splitlines()[-2:] == o["report"]
+    return o
+
+
 MINIMAL = ("-da_grid_x 33 -da_grid_y 33 -snes_grid_sequence 6 -snes_fd_color -snes_converged_reason "
            "-ksp_converged_reason -pc_type mg")
 
@@ -452,6 +487,7 @@ def main():
     if not a.no_obstacle:
         print("bench: obstacle 2049^2", file=sys.stderr, flush=True)
         out["obstacle_2049"] = obstacle(not a.no_cpu)
+        out["obstacle_cluster_2049"] = obstacle_cluster(not a.no_cpu)
     if not a.no_minimal:
         out["minimal_2049"] = minimal(not a.no_cpu)
 

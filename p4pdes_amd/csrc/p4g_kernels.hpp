@@ -2010,6 +2010,15 @@ k_ilu27_plane(SorOp o, const double* __restrict__ u, int p, int backward, const 
 // wait longer than 2^24 polls sets *err and ends the waiting (no hang).
 constexpr int SOR9_SLACK = 64;
 
+// lane i receives lane i-1's value (DPP wave_shr:1; lane 0 keeps its own)
+__device__ __forceinline__ double wave_shr1(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = (int)(b & 0xffffffffLL), hi = (int)(b >> 32);
+    const int rlo = __builtin_amdgcn_update_dpp(lo, lo, 0x138, 0xf, 0xf, false);
+    const int rhi = __builtin_amdgcn_update_dpp(hi, hi, 0x138, 0xf, 0xf, false);
+    return __longlong_as_double(((long long)rhi << 32) | (unsigned int)rlo);
+}
+
 __device__ __forceinline__ double ld_agent(const double* p) {
     const unsigned long long v =
         __hip_atomic_load((const unsigned long long*)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -2030,6 +2039,12 @@ k_sor9_pack(long n, const double* __restrict__ g, double* __restrict__ pk) {
     }
 }
 
+// the operands of one step of k_sor9_strips, loaded SOR9_AHEAD steps early
+struct Sor9Ops {
+    double b, a[10], x[4];
+};
+constexpr int SOR9_AHEAD = 4;
+
 __global__ void __launch_bounds__(64)
 k_sor9_strips(int mx, int my, const double* __restrict__ pk, const double* __restrict__ b, double* x,
               unsigned long long* flag, unsigned long long base, int backward, int* err) {
@@ -2045,28 +2060,27 @@ k_sor9_strips(int mx, int my, const double* __restrict__ pk, const double* __res
     // neighbour); positions are along the sweep: p -> column p or mx-1-p
     __shared__ double ring[256];
     const int jn = backward ? my - row0 : row0 - 1;   // lane 0's neighbour row (s > 0), the same for all lanes
-    int copied = 0;                             // positions in the ring (wave-uniform)
+    int copied = 0;                                    // positions in the ring (wave-uniform)
     bool dead = false;
     double h1 = 0.0, h2 = 0.0, h3 = 0.0, xo = 0.0;
-    double nb = 0.0, na[10], nx[4];
-    auto fetch = [&](int t) {
+    Sor9Ops pf[SOR9_AHEAD];   // a ring of prefetched operands, indexed by step mod SOR9_AHEAD
+    auto fetch = [&](int t, Sor9Ops& o) {
         const int c = t - 2 * r;
         if (!(live && c >= 0 && c < mx)) return;
         const int i = backward ? mx - 1 - c : c;
         const long q = (long)j * mx + i;
-        nb = b[q];
+        o.b = b[q];
         const double* rec = pk + q * 10;
 #pragma unroll
-        for (int k = 0; k < 10; ++k) na[k] = rec[k];
+        for (int k = 0; k < 10; ++k) o.a[k] = rec[k];
         if (backward) {   // forward values below and to the left
-            nx[0] = (hs && i > 0) ? x[q - mx - 1] : 0.0;
-            nx[1] = hs ? x[q - mx] : 0.0;
-            nx[2] = (hs && i + 1 < mx) ? x[q - mx + 1] : 0.0;
-            nx[3] = i > 0 ? x[q - 1] : 0.0;
+            o.x[0] = (hs && i > 0) ? x[q - mx - 1] : 0.0;
+            o.x[1] = hs ? x[q - mx] : 0.0;
+            o.x[2] = (hs && i + 1 < mx) ? x[q - mx + 1] : 0.0;
+            o.x[3] = i > 0 ? x[q - 1] : 0.0;
         }
     };
-    fetch(0);
-    for (int t = 0; t < nsteps; ++t) {
+    auto step = [&](int t, Sor9Ops& cur) {
         // lane 0 is at position t: it needs positions t-1..t+1 of the neighbour row
         if (s > 0 && t < mx && copied < min(t + 2, mx)) {   // uniform
             unsigned long long kn = 0;
@@ -2092,16 +2106,11 @@ k_sor9_strips(int mx, int my, const double* __restrict__ pk, const double* __res
             __syncthreads();
             copied = upto;
         }
-        const double u1 = __shfl_up(h1, 1, 64), u2 = __shfl_up(h2, 1, 64), u3 = __shfl_up(h3, 1, 64);
+        const double u1 = wave_shr1(h1), u2 = wave_shr1(h2), u3 = wave_shr1(h3);
         const int c = t - 2 * r;
-        const double cb = nb;
-        double a[10], cx[4];
-#pragma unroll
-        for (int k = 0; k < 10; ++k) a[k] = na[k];
-#pragma unroll
-        for (int k = 0; k < 4; ++k) cx[k] = nx[k];
         double res = 0.0;
         if (live && c >= 0 && c < mx) {
+            const double* a = cur.a;
             const int i = backward ? mx - 1 - c : c;
             const long q = (long)j * mx + i;
             const bool hv = backward ? hn : hs;   // the row toward the sweep's origin
@@ -2119,17 +2128,17 @@ k_sor9_strips(int mx, int my, const double* __restrict__ pk, const double* __res
                 }
             }
             const bool hw = i > 0, he = i + 1 < mx;
-            double sum = cb;
+            double sum = cur.b;
             if (!backward) {
                 if (hs && hw && a[0] != 0.0) sum -= a[0] * v_im1;
                 if (hs && a[1] != 0.0) sum -= a[1] * v_i;
                 if (hs && he && a[2] != 0.0) sum -= a[2] * v_ip1;
                 if (hw && a[3] != 0.0) sum -= a[3] * xo;
             } else {
-                if (hs && hw && a[0] != 0.0) sum -= a[0] * cx[0];
-                if (hs && a[1] != 0.0) sum -= a[1] * cx[1];
-                if (hs && he && a[2] != 0.0) sum -= a[2] * cx[2];
-                if (hw && a[3] != 0.0) sum -= a[3] * cx[3];
+                if (hs && hw && a[0] != 0.0) sum -= a[0] * cur.x[0];
+                if (hs && a[1] != 0.0) sum -= a[1] * cur.x[1];
+                if (hs && he && a[2] != 0.0) sum -= a[2] * cur.x[2];
+                if (hw && a[3] != 0.0) sum -= a[3] * cur.x[3];
                 if (he && a[5] != 0.0) sum -= a[5] * xo;
                 if (hn && hw && a[6] != 0.0) sum -= a[6] * v_im1;
                 if (hn && a[7] != 0.0) sum -= a[7] * v_i;
@@ -2138,17 +2147,24 @@ k_sor9_strips(int mx, int my, const double* __restrict__ pk, const double* __res
             res = a[9] != 0.0 ? sum * a[4] : 0.0;
             x[q] = res;
             xo = res;
-            // a full strip publishes its last row's progress every 16 points
-            if (r == 63 && ((c & 15) == 15 || c == mx - 1)) {
+            // a full strip publishes its last row's progress every 32 points
+            if (r == 63 && ((c & 31) == 31 || c == mx - 1)) {
                 __threadfence();
                 __hip_atomic_store(flag + s, base + (unsigned long long)(c + 1), __ATOMIC_RELEASE,
                                    __HIP_MEMORY_SCOPE_AGENT);
             }
         }
-        fetch(t + 1);
+        fetch(t + SOR9_AHEAD, cur);   // this slot now holds step t + SOR9_AHEAD
         h3 = h2;
         h2 = h1;
         h1 = res;
+    };
+#pragma unroll
+    for (int k = 0; k < SOR9_AHEAD; ++k) fetch(k, pf[k]);
+    for (int t0 = 0; t0 < nsteps; t0 += SOR9_AHEAD) {
+#pragma unroll
+        for (int k = 0; k < SOR9_AHEAD; ++k)
+            if (t0 + k < nsteps) step(t0 + k, pf[k]);
     }
 }
 

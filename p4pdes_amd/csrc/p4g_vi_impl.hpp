@@ -82,6 +82,10 @@ static int vi_operator(p4g_solver_s* s, ViRun& R) {
     if (s->pctype != PCT_MG) return 0;
     P4G_TRY(galerkin_levels(s));
     for (int l = 0; l < L; ++l) e.g27[l] = s->g27[l];
+    for (int l = 0; l <= L; ++l)   // the pipelined SOR sweeps read the operator as packed records
+        if (s->lv[l].pipe)
+            k_sor9_pack<<<vblocks(s->lv[l].n), 256, 0, R.st>>>(s->lv[l].n, s->g27[l], s->lv[l].pipe->pk);
+    LAUNCH_CHECK();
     if (L > 0) {
         std::vector<double> Ainv;
         P4G_TRY(coarse_factor(s->lv[0], s->g27[0], Ainv));
@@ -223,6 +227,19 @@ static int vi_solve(p4g_solver_s* s, const double* xl, const double* xu, double*
         P4G_TRY(dalloc(s, (void**)&s->vi_g27, 27 * n * sizeof(double)));
         // the Chebyshev bounds are re-estimated every Newton step: keep their
         // vectors (6 of the finest size) instead of allocating per level
+        if (s->sor && s->pctype == PCT_MG && s->lv[L].dim == 2) {   // pipelined strip sweeps on every level
+            s->ms_pipes.resize(s->nlev);
+            for (int l = 0; l <= L; ++l) {
+                if (s->lv[l].mx >= (1 << 22)) continue;
+                const int ns = (s->lv[l].my + 63) / 64;
+                SorPipe& P = s->ms_pipes[l];
+                P4G_TRY(dalloc(s, (void**)&P.flags, (ns + 1) * sizeof(unsigned long long) + sizeof(int)));
+                HIP_TRY(hipMemset(P.flags, 0, (ns + 1) * sizeof(unsigned long long) + sizeof(int)));
+                P.err = (int*)(P.flags + ns + 1);
+                P4G_TRY(dalloc(s, (void**)&P.pk, 10 * s->lv[l].n * sizeof(double)));
+                s->lv[l].pipe = &P;
+            }
+        }
         if (s->pctype == PCT_MG && s->smoother == SM_CHEB) {
             s->est_cap = 6 * n;
             P4G_TRY(dalloc(s, (void**)&s->est_scratch, s->est_cap * sizeof(double)));
@@ -296,6 +313,12 @@ static int vi_solve(p4g_solver_s* s, const double* xl, const double* xu, double*
     }
     HIP_TRY(hipMemcpy(x, R.X, n * sizeof(double), hipMemcpyDeviceToHost));
     if (Fout) HIP_TRY(hipMemcpy(Fout, R.F, n * sizeof(double), hipMemcpyDeviceToHost));
+    for (const SorPipe& P : s->ms_pipes) {
+        if (!P.err) continue;
+        int e = 0;
+        HIP_TRY(hipMemcpy(&e, P.err, sizeof(int), hipMemcpyDeviceToHost));
+        if (e) return set_error(PETSC_ERR_LIB, "p4g_vi_solve: a pipelined SOR sweep stalled");
+    }
     s->cx.tm.collect();
     out.its = iter;
     out.reason = reason;
