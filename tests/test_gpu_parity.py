@@ -360,8 +360,11 @@ def test_ksp_viewer_files(tmp_path):
     "-mg_levels_ksp_max_it 2 -pc_mg_type full" % MG,
     "-fsh_dim 3 -da_refine 4 %s -ksp_type richardson -ksp_max_it 1 -ksp_norm_type unpreconditioned "
     "-mg_levels_ksp_max_it 4 -pc_mg_type kaskade" % MG,
+    # GMRES: the Hessenberg estimates at PETSc's monitor points, restarts included
+    "-fsh_dim 3 -fsh_problem manusin -da_refine 3 -ksp_type gmres -ksp_gmres_restart 5 -ksp_rtol 1e-10",
+    "-fsh_dim 2 -fsh_problem manupoly -da_refine 5 -ksp_type gmres %s -ksp_rtol 1e-10" % MG,
 ], ids=["3d_precond", "2d_unprecond", "3d_natural_galerkin", "study_w", "study_mcycles2_random",
-        "study_full", "study_kaskade"])
+        "study_full", "study_kaskade", "gmres_ilu_restart5", "gmres_mg"])
 def test_ksp_monitor_lines(args):
     # -ksp_monitor: PETSc's KSPMonitorResidual lines, before the converged
     # reason, with the oracle's residual norms
