@@ -192,6 +192,7 @@ static int vi_solve(p4g_solver_s* s, const double* xl, const double* xu, double*
     if (s->ls_type != "bt" && s->ls_type != "basic")
         return set_error(PETSC_ERR_SUP, "-snes_linesearch_type %s is not implemented (bt, basic)", s->ls_type.c_str());
     if (s->mixed) return set_error(PETSC_ERR_SUP, "p4g_vi_solve: mixed precision is not implemented");
+    if (s->ksp_rich) return set_error(PETSC_ERR_SUP, "p4g_vi_solve: -ksp_type cg (the default) or gmres");
     if (!da->kcell.empty()) return set_error(PETSC_ERR_SUP, "p4g_vi_solve: variable coefficients are not implemented");
     if (s->fmg) return set_error(PETSC_ERR_SUP, "p4g_vi_solve: -pc_mg_type full is not implemented");
     if (s->coarse_its > 0)
@@ -283,7 +284,10 @@ static int vi_solve(p4g_solver_s* s, const double* xl, const double* xu, double*
         LAUNCH_CHECK();
         P4G_TRY(vi_operator(s, R));
         typename Engine<double>::CGOut co;
-        P4G_TRY(s->e64->cg(R.Fm, R.Y, s->normtype, s->rtol, s->atol, s->dtol, s->maxit, co, s->cx.S));
+        if (s->ksp_gmres)
+            P4G_TRY(s->e64->gmres(R.Fm, R.Y, s->rtol, s->atol, s->dtol, s->maxit, s->gmres_restart, co, s->cx.S));
+        else
+            P4G_TRY(s->e64->cg(R.Fm, R.Y, s->normtype, s->rtol, s->atol, s->dtol, s->maxit, co, s->cx.S));
         HIP_TRY(hipStreamSynchronize(R.st));
         s->vi_kits.push_back(co.its);
         s->vi_kreason.push_back(co.reason);

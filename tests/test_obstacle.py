@@ -176,6 +176,8 @@ def test_null_row_galerkin_identity():
 # ---------------------------------------------------------- device parity
 
 CASES = {
+    "65_gmres_ilu": "-da_refine 5 -ksp_type gmres",
+    "seq3_gmres_mg": "-snes_grid_sequence 3 -ksp_type gmres -pc_type mg -mg_levels_pc_type jacobi",
     "9_mg": "-da_refine 2 -pc_type mg -mg_levels_pc_type jacobi",
     "33_mg": "-da_refine 4 -pc_type mg -mg_levels_pc_type jacobi",
     "129_mg": "-da_refine 6 -pc_type mg -mg_levels_pc_type jacobi",
