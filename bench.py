@@ -1,14 +1,17 @@
 """Benchmark: 3D Poisson DOF/s to rtol 1e-10 with CG + geometric multigrid on
 one MI355X, mixed precision vs FP64, p50/p99 solve latency.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-513]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-extra]
 
-A step is one KSP solve (CG + V(1,1) Chebyshev/Jacobi multigrid, from x = 0)
-of the 257^3 FP64 system (16,974,593 DOF = mx*my*mz, the reference's global
-Vec size), with F(u0) already resident in HBM; set-up (callbacks,
-hierarchy, eigen-estimates, coarse factorization) is outside the timed
-region, as KSPSolve is timed apart from SNESSolve in the reference's
-studies.  Prints ONE JSON line.
+A step is one KSP solve (from x = 0) of the 513^3 system (135,005,697 DOF =
+mx*my*mz, the reference's global Vec size): f = manufactured sine right-hand
+side + N(0, 1e-3) noise, CG + 8-level V(2,2) Chebyshev/Jacobi multigrid, to a
+true FP64 relative residual |b - A x|_2 / |b|_2 <= 1e-10.  `value` is the FP64
+MG-CG path; the mixed-precision path (FP32 V-cycle and Krylov vectors inside
+FP64 iterative refinement) is timed the same way beside it.  F(u0) is already
+resident in HBM; set-up (callbacks, hierarchy, eigen-estimates, coarse
+factorization) is outside the timed region, as KSPSolve is timed apart from
+SNESSolve in c/ch7/study/optimal3.sh:22-23.  Prints ONE JSON line.
 """
 import argparse
 import json
@@ -62,18 +65,27 @@ def time_solves(sess, steps, warmup):
     return total, lat, reps
 
 
-TRAFFIC_FILE = "profiles/hbm_traffic_fp64_257cubed.json"
+TRAFFIC_FILES = {513: "profiles/hbm_traffic_513cubed.json",
+                 257: "profiles/hbm_traffic_fp64_257cubed.json"}
 
 
-def measured_traffic(cls_name):
-    """HBM bytes per finest-level launch of a kernel class on the 257^3
-    problem, from the calibrated rocprofv3 FETCH_SIZE / WRITE_SIZE passes
-    recorded in TRAFFIC_FILE (None when absent)."""
-    path = os.path.join(ROOT, TRAFFIC_FILE)
-    if not os.path.exists(path):
+def measured_traffic(grid, prec, cls_name):
+    """HBM bytes per finest-level launch of a kernel class on the grid^3
+    problem in precision prec ("double" or "mixed"), from the calibrated
+    rocprofv3 FETCH_SIZE / WRITE_SIZE passes recorded in TRAFFIC_FILES[grid]
+    (None when absent)."""
+    rel = TRAFFIC_FILES.get(grid)
+    path = os.path.join(ROOT, rel) if rel else None
+    if not path or not os.path.exists(path):
         return None
     with open(path) as f:
-        return json.load(f).get("kernels", {}).get(cls_name)
+        d = json.load(f)
+    k = d.get("kernels", {})
+    k = k.get(prec, k) if prec in k else k
+    t = k.get(cls_name)
+    if t is not None:
+        t = dict(t, source=rel)
+    return t
 
 
 def kernel_roofline(sess, cls):
@@ -109,21 +121,57 @@ def stencil_apply_roofline(sess, reps=20):
     return r
 
 
-def cpu_baseline(args, nsolves=16):
-    """The oracle (C restatement, CSR operator as PETSc's AIJ) on the host
-    cores: nsolves KSP solves of the same system after one warm-up (about
-    10 s of CPU work at 0.6 s per solve); the median solve time."""
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def cpu_baseline(args, label, nsolves=3, sample_its=2):
+    """The oracle (C restatement, CSR operator as PETSc's AIJ), compiled here
+    with -march=native, on the host cores given to this job: one warm-up,
+    then nsolves KSP solves of the same system, median; then one solve on a
+    single thread capped at sample_its iterations, scaled to the full
+    iteration count (k iterations apply the preconditioner k + 1 times), as
+    the 1-core figure.  Set-up is timed apart."""
+    import tempfile
     from oracle import oracle_py
-    ncores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
-    threads = max(1, min(16, ncores))
-    times, res = oracle_py.fish_time(args, nthreads=threads, warmup=1, nsolves=nsolves)
+    nproc = os.cpu_count()
+    affinity = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else nproc
+    omp = os.environ.get("OMP_NUM_THREADS")
+    threads = affinity
+    if omp and omp.isdigit() and int(omp) > 0:
+        threads = min(threads, int(omp))
+    try:
+        libpath = oracle_py.build_native(tempfile.mkdtemp(prefix="oracle_native_"))
+        build = "gcc -O3 -march=native (built on this host)"
+    except Exception as e:      # noqa: BLE001 - fall back to the shipped x86-64-v3 build
+        libpath = None
+        build = "gcc -O3 -march=x86-64-v3 (native build failed: %s)" % e
+    r = oracle_py.fish_time_sample(args, threads, 1, nsolves, sample_its, libpath=libpath)
     _, mx, my, mz = oracle_py.grid_shape(args)
     n = mx * my * mz
+    times = r["times_ms"]
     med = sorted(times)[len(times) // 2]
+    its = r["res"].its
+    # a PCG solve of k iterations applies the preconditioner k + 1 times
+    one_solve_ms = r["sample_ms"] / (r["sample_its"] + 1) * (its + 1)
     return {"value": n / (med * 1e-3), "unit": "DOF/s", "cores": threads, "kind": "port",
-            "sample": "%d KSP solves of the full 257^3 system after one warm-up (median %.2f s, total "
-                      "%.1f s), oracle CSR CG + V(1,1) Chebyshev/Jacobi multigrid, %d OpenMP threads, "
-                      "%d its" % (nsolves, med * 1e-3, sum(times) * 1e-3, threads, res.its)}
+            "sample": "%d KSP solves of the full %s system after one warm-up (median %.2f s, total %.1f s; "
+                      "set-up %.1f s apart), oracle CSR CG + multigrid, %d OpenMP threads, %d its"
+                      % (nsolves, label, med * 1e-3, sum(times) * 1e-3, r["setup_ms"] * 1e-3, threads, its),
+            "median_solve_s": med * 1e-3, "its": its,
+            "one_core": {"value": n / (one_solve_ms * 1e-3), "unit": "DOF/s", "cores": 1,
+                         "sample": "one solve capped at %d iterations on 1 thread (%.2f s), scaled by "
+                                   "preconditioner applications to the %d-iteration solve"
+                                   % (r["sample_its"], r["sample_ms"] * 1e-3, its)},
+            "host": {"nproc": nproc, "affinity_cpus": affinity, "omp_num_threads": omp,
+                     "model": _cpu_model(), "oracle_build": build}}
 
 
 SERVE_SHAPES = [(3, 4), (3, 5), (3, 6), (3, 7), (2, 7), (2, 9), (2, 10)]   # (dim, refine)
@@ -211,16 +259,23 @@ def serving(rate=40.0, nreq=80, seed=2026, budget=64 << 30):
 
 
 COPY_BYTES = 1 << 30
+C3_FP64 = C3 + " -ksp_norm_type unpreconditioned"
+C3_MIXED = C3 + " -p4g_precision mixed"
 
 
-def profile_workload():
-    """Workload for hardware-counter passes: the 257^3 solve, then a device
-    copy of COPY_BYTES (read COPY_BYTES, write COPY_BYTES) whose counter
-    readings calibrate the counters' bytes."""
+def profile_workload(which):
+    """Workload for hardware-counter and trace passes: two solves of the
+    513^3 system in FP64 and two in mixed precision ("513"), or two 257^3
+    FP64 solves ("257"); then a device copy of COPY_BYTES (read COPY_BYTES,
+    write COPY_BYTES) whose counter readings calibrate the counters' bytes."""
     from p4pdes_amd import capi
-    with capi.Session(C2) as s:
-        for _ in range(2):
-            s.solve()
+    cases = [C2] if which == "257" else [C3_FP64, C3_MIXED]
+    for opts in cases:
+        print("profile workload: %s" % opts, flush=True)
+        with capi.Session(opts) as s:
+            for _ in range(2):
+                s.solve()
+            capi.check(capi.lib().p4g_device_synchronize())
     L = capi.lib()
     src = L.p4g_device_malloc(COPY_BYTES)
     dst = L.p4g_device_malloc(COPY_BYTES)
@@ -232,6 +287,61 @@ def profile_workload():
     L.p4g_device_free(src)
     L.p4g_device_free(dst)
     print("profile workload done", flush=True)
+
+
+def timed_config(opts, steps, warmup, grid, prec, roofline=True):
+    """Set up one fish session and time `steps` solves after `warmup`; with
+    roofline, first time every kernel class once (one untimed solve) to find
+    the dominant class, then event-time its finest-level launches during the
+    timed solves (one event pair per launch on the solver's stream)."""
+    from p4pdes_amd import capi
+    out = {}
+    with capi.Session(opts) as s:
+        s.solve()
+        dominant = None
+        if roofline:
+            s.reset_stats()
+            for c in range(capi.K_COUNT):
+                s.set_timing(c, True)
+            s.solve()
+            by_cls = {c: s.kernel_stats(c)[1] for c in range(capi.K_COUNT)}
+            solve_bytes = sum(s.kernel_stats(c)[2] for c in range(capi.K_COUNT))
+            for c in range(capi.K_COUNT):
+                s.set_timing(c, False)
+            dominant = max((c for c in by_cls if c not in (capi.K_OTHER, capi.K_REDUCE)),
+                           key=lambda c: by_cls[c])
+            out["kernel_ms_per_solve_evented"] = {capi.KERNEL_NAMES[c]: round(by_cls[c], 4) for c in by_cls}
+            out["solve_algorithmic_gb"] = solve_bytes / 1e9
+            s.reset_stats()
+            s.set_timing(dominant, True, fine_only=True)
+        total, lat, reps = time_solves(s, steps, warmup)
+        if roofline:
+            s.set_timing(dominant, False)
+            roof = kernel_roofline(s, dominant)
+            name = capi.KERNEL_NAMES[dominant]
+            traffic = measured_traffic(grid, prec, name)
+            out["roofline"] = {"kernel": name, "bound": "hbm",
+                               "achieved": roof["gbs"] if roof else None, "peak": HBM_PEAK_GBS,
+                               "unit": "GB/s", "frac": (roof["gbs"] / HBM_PEAK_GBS) if roof else None,
+                               "traffic": traffic["traffic_bytes"] if traffic else None,
+                               "traffic_kernel": traffic["kernel"] if traffic else None,
+                               "traffic_source": traffic["source"] if traffic else None,
+                               "avg_ms": roof["avg_ms"] if roof else None,
+                               "algorithmic_bytes_per_launch": roof["bytes_per_launch"] if roof else None,
+                               "launches": roof["launches"] if roof else None}
+        einf, _ = s.errors()
+        ms = total / steps * 1e3
+        rep = reps[-1]
+        out.update({"dof_per_s": s.n / (ms * 1e-3), "ms_per_solve": ms,
+                    "latency_ms": {"p50": pct(lat, 0.5), "p99": pct(lat, 0.99)},
+                    "its": rep.its, "inner_its": rep.inner_its, "true_relres": rep.true_relres,
+                    "errinf_vs_exact": einf, "n": s.n})
+        if roofline:
+            out["solve_effective_gbs"] = out["solve_algorithmic_gb"] / (ms * 1e-3)
+        if prec == "double":
+            out["stencil_apply"] = stencil_apply_roofline(s)
+        out["_solution"] = s.solution() if grid == 513 else None
+    return out
 
 
 OBSTACLE = "-da_refine 2 -snes_grid_sequence 8 -pc_type mg -mg_levels_pc_type jacobi"
@@ -352,13 +462,16 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--no-513", action="store_true", help="skip the 513^3 mixed/FP64 runs")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="only the 513^3 headline (no 257^3, variable-coefficient, serving, "
+                         "obstacle or minimal-surface legs)")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving stream")
     ap.add_argument("--no-obstacle", action="store_true", help="skip the obstacle (VI) solve")
     ap.add_argument("--no-minimal", action="store_true", help="skip the minimal-surface solve")
-    ap.add_argument("--profile-workload", action="store_true",
-                    help="only run the counter-collection workload: two 257^3 solves and a 1 GiB "
-                         "device copy (the calibration copy of known size); prints no JSON")
+    ap.add_argument("--profile-workload", choices=("513", "257"), default=None,
+                    help="only run the counter-collection workload: two solves of the 513^3 system "
+                         "in each precision (or two 257^3 FP64 solves) and a 1 GiB device copy (the "
+                         "calibration copy of known size); prints no JSON")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU only; --gpus %d is not supported "
@@ -370,107 +483,67 @@ def main():
         print("bench.py: no HIP device", file=sys.stderr)
         sys.exit(1)
     if a.profile_workload:
-        profile_workload()
+        profile_workload(a.profile_workload)
         return
+    import numpy as np
 
     out = {"metric": METRIC, "unit": "DOF/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
-           "data": "synthetic (manufactured sine right-hand side, closed-form solution)"}
+           "data": "synthetic (manufactured sine right-hand side plus seeded N(0,1e-3) noise; "
+                   "closed-form noise-free solution)"}
 
-    # ---- headline: C2 257^3 FP64 ----------------------------------------
-    print("bench: setting up 257^3", file=sys.stderr, flush=True)
-    with capi.Session(C2) as s:
-        n = s.n
-        s.solve()
-        # roofline of the dominant kernel: time every class once (untimed run
-        # is unaffected), then event-time the dominant class in the timed run
-        s.reset_stats()
-        for c in range(capi.K_COUNT):
-            s.set_timing(c, True)
-        s.solve()
-        by_cls = {c: s.kernel_stats(c)[1] for c in range(capi.K_COUNT)}
-        solve_bytes = sum(s.kernel_stats(c)[2] for c in range(capi.K_COUNT))   # algorithmic, one solve
-        for c in range(capi.K_COUNT):
-            s.set_timing(c, False)
-        dominant = max((c for c in by_cls if c not in (capi.K_OTHER, capi.K_REDUCE)),
-                       key=lambda c: by_cls[c])
-        breakdown = {capi.KERNEL_NAMES[c]: round(by_cls[c], 4) for c in by_cls}
-        s.reset_stats()
-        s.set_timing(dominant, True, fine_only=True)   # one event pair per cycle
-        print("bench: timing %d solves" % a.steps, file=sys.stderr, flush=True)
-        total, lat, reps = time_solves(s, a.steps, a.warmup)
-        s.set_timing(dominant, False)
-        roof = kernel_roofline(s, dominant)
-        einf, e2h = s.errors()
-        apply257 = stencil_apply_roofline(s)
-        rep = reps[-1]
-    ms = total / a.steps * 1e3
-    out["value"] = n / (ms * 1e-3)
-    out["ms_per_step"] = ms
-    out["latency_ms"] = {"p50": pct(lat, 0.5), "p99": pct(lat, 0.99)}
-    out["config"] = {"workload": "3D Poisson 257^3 vertex grid (16,974,593 DOF), 7-point stencil, "
-                                 "u=sin(pi x)sin(pi y)sin(pi z), FP64 CG + 7-level V(1,1) "
-                                 "Chebyshev(1)/Jacobi, 5^3 Cholesky, rtol 1e-10 (preconditioned "
-                                 "norm, PETSc's CG default)",
-                     "grid": [257, 257, 257], "levels": 7, "its": rep.its,
-                     "true_relres": rep.true_relres, "errinf": einf,
-                     "errinf_closed_form": 1.254995e-05, "parallelism": "single GPU"}
-    traffic = measured_traffic(capi.KERNEL_NAMES[dominant])
-    out["roofline"] = {"kernel": capi.KERNEL_NAMES[dominant], "bound": "hbm",
-                       "achieved": roof["gbs"] if roof else None, "peak": HBM_PEAK_GBS,
-                       "unit": "GB/s", "frac": (roof["gbs"] / HBM_PEAK_GBS) if roof else None,
-                       "traffic": traffic["traffic_bytes"] if traffic else None,
-                       "traffic_kernel": traffic["kernel"] if traffic else None,
-                       "traffic_source": TRAFFIC_FILE if traffic else None,
-                       "avg_ms": roof["avg_ms"] if roof else None,
-                       "algorithmic_bytes_per_launch": roof["bytes_per_launch"] if roof else None,
-                       "launches": roof["launches"] if roof else None}
-    out["kernel_ms_per_solve"] = breakdown
-    # whole solve: algorithmic bytes of every launch (all levels) / solve time
-    out["solve_algorithmic_gb"] = solve_bytes / 1e9
-    out["solve_effective_gbs"] = solve_bytes / (ms * 1e-3) / 1e9
-    out["stencil_apply_257"] = apply257
+    # ---- headline: C3 513^3, FP64 MG-CG (value) and mixed precision beside it
+    print("bench: 513^3 FP64, %d + %d solves" % (a.warmup, a.steps), file=sys.stderr, flush=True)
+    d = timed_config(C3_FP64, a.steps, a.warmup, 513, "double")
+    print("bench: 513^3 mixed, %d + %d solves" % (a.warmup, a.steps), file=sys.stderr, flush=True)
+    m = timed_config(C3_MIXED, a.steps, a.warmup, 513, "mixed")
+    diff = float(np.max(np.abs(m["_solution"] - d["_solution"])) / np.max(np.abs(d["_solution"])))
+    del d["_solution"], m["_solution"]
+    out["value"] = d["dof_per_s"]
+    out["ms_per_step"] = d["ms_per_solve"]
+    out["latency_ms"] = d["latency_ms"]
+    out["config"] = {"workload": "C3: 3D Poisson 513^3 vertex grid (135,005,697 DOF), 7-point stencil, "
+                                 "f = 3 pi^2 sin(pi x)sin(pi y)sin(pi z) + N(0,1e-3) noise (Philox, seed 0), "
+                                 "CG + 8-level V(2,2) Chebyshev(2)/Jacobi, 5^3 Cholesky, to "
+                                 "|b-Ax|_2/|b|_2 <= 1e-10; value = FP64 path, mixed = FP32 V-cycle in "
+                                 "FP64 iterative refinement",
+                     "grid": [513, 513, 513], "levels": 8, "its": d["its"],
+                     "true_relres": d["true_relres"], "errinf": d["errinf_vs_exact"],
+                     "errinf_closed_form_noise_free": 3.137469e-06, "parallelism": "single GPU"}
+    out["roofline"] = d.pop("roofline")
+    out["mixed"] = {"value": m["dof_per_s"], "ms_per_step": m["ms_per_solve"], "latency_ms": m["latency_ms"],
+                    "its": m["its"], "inner_its": m["inner_its"], "true_relres": m["true_relres"],
+                    "errinf": m["errinf_vs_exact"], "dtype": "f32 V-cycle and Krylov vectors, f64 iterate, "
+                    "residual replacement, dots and coarse solve",
+                    "speedup_over_fp64": m["dof_per_s"] / d["dof_per_s"],
+                    "rel_inf_diff_vs_fp64": diff, "roofline": m.get("roofline"),
+                    "kernel_ms_per_solve_evented": m.get("kernel_ms_per_solve_evented"),
+                    "solve_algorithmic_gb": m.get("solve_algorithmic_gb")}
+    out["fp64_detail"] = {"kernel_ms_per_solve_evented": d.get("kernel_ms_per_solve_evented"),
+                          "solve_algorithmic_gb": d.get("solve_algorithmic_gb"),
+                          "solve_effective_gbs": d.get("solve_effective_gbs"),
+                          "stencil_apply_513": d.get("stencil_apply")}
 
-    # ---- C3: 513^3 mixed vs FP64 -----------------------------------------
-    if not a.no_513:
-        c3 = {}
-        sols = {}
-        for prec in ("double", "mixed"):
-            extra = " -ksp_norm_type unpreconditioned" if prec == "double" else " -p4g_precision mixed"
-            print("bench: 513^3 %s" % prec, file=sys.stderr, flush=True)
-            with capi.Session(C3 + extra) as s:
-                tot, l3, r3 = time_solves(s, 3, 1)
-                m3 = tot / 3 * 1e3
-                e3, _ = s.errors()
-                sols[prec] = s.solution()
-                c3[prec] = {"dof_per_s": s.n / (m3 * 1e-3), "ms_per_solve": m3, "its": r3[-1].its,
-                            "inner_its": r3[-1].inner_its, "true_relres": r3[-1].true_relres,
-                            "errinf_vs_exact": e3}
-                if prec == "double":
-                    c3["stencil_apply_513"] = stencil_apply_roofline(s)
-        import numpy as np
-        d = float(np.max(np.abs(sols["mixed"] - sols["double"])) / np.max(np.abs(sols["double"])))
-        del sols
-        c3["speedup_mixed_over_fp64"] = c3["mixed"]["dof_per_s"] / c3["double"]["dof_per_s"]
-        c3["mixed_vs_fp64_rel_inf_diff"] = d
-        c3["workload"] = ("3D Poisson 513^3 (135,005,697 DOF), f = manufactured + N(0,1e-3) noise, "
-                          "8-level V(2,2) Chebyshev/Jacobi; FP64: CG on the FP64 hierarchy; mixed: "
-                          "FP64 iterative refinement around FP32 PCG + FP32 V-cycle; both to "
-                          "|b-Ax|_2/|b|_2 <= 1e-10")
-        out["mixed_vs_fp64_513"] = c3
+    if not a.no_extra:
+        # ---- C2: 257^3 FP64 -------------------------------------------------
+        print("bench: 257^3 FP64", file=sys.stderr, flush=True)
+        c2 = timed_config(C2, a.steps, a.warmup, 257, "double")
+        del c2["_solution"]
+        c2["errinf_closed_form"] = 1.254995e-05
+        c2["workload"] = ("C2: 3D Poisson 257^3 (16,974,593 DOF), u = sin(pi x)sin(pi y)sin(pi z), FP64 CG + "
+                          "7-level V(1,1) Chebyshev(1)/Jacobi, 5^3 Cholesky, rtol 1e-10 (preconditioned norm, "
+                          "PETSc's CG default)")
+        out["fp64_257"] = c2
 
-    # ---- C4: 257^3 variable coefficient, FP64 and mixed ---------------------
-    if not a.no_513:
+        # ---- C4: 257^3 variable coefficient, FP64 and mixed -----------------
         c4 = {}
         for prec in ("double", "mixed"):
             extra = " -ksp_norm_type unpreconditioned" if prec == "double" else " -p4g_precision mixed"
             print("bench: 257^3 variable coefficient %s" % prec, file=sys.stderr, flush=True)
-            with capi.Session(C4 + extra) as s:
-                tot, l4, r4 = time_solves(s, 3, 1)
-                m4 = tot / 3 * 1e3
-                e4, _ = s.errors()
-                c4[prec] = {"dof_per_s": s.n / (m4 * 1e-3), "ms_per_solve": m4, "its": r4[-1].its,
-                            "true_relres": r4[-1].true_relres, "errinf_vs_exact": e4}
+            r4 = timed_config(C4 + extra, 3, 1, 257, prec, roofline=False)
+            r4.pop("_solution", None)
+            r4.pop("stencil_apply", None)
+            c4[prec] = r4
         c4["speedup_mixed_over_fp64"] = c4["mixed"]["dof_per_s"] / c4["double"]["dof_per_s"]
         c4["workload"] = ("3D -div(k grad u) = f on 257^3 vertices (256^3 cells), k = exp(G), G a seeded "
                           "Gaussian random field (random Fourier features, sigma 1, correlation length 0.1), "
@@ -478,23 +551,24 @@ def main():
                           "7-level V(1,1) Chebyshev/Jacobi; both to |b-Ax|_2/|b|_2 <= 1e-10")
         out["varcoef_257"] = c4
 
-    # ---- C5: serving stream ------------------------------------------------
-    if not a.no_serve:
-        print("bench: serving stream", file=sys.stderr, flush=True)
-        out["serving"] = serving()
+        # ---- C5: serving stream ----------------------------------------------
+        if not a.no_serve:
+            print("bench: serving stream", file=sys.stderr, flush=True)
+            out["serving"] = serving()
 
-    # ---- obstacle problem (c/ch12/obstacle.c), grid-sequenced to 2049^2 ---
-    if not a.no_obstacle:
-        print("bench: obstacle 2049^2", file=sys.stderr, flush=True)
-        out["obstacle_2049"] = obstacle(not a.no_cpu)
-        out["obstacle_cluster_2049"] = obstacle_cluster(not a.no_cpu)
-    if not a.no_minimal:
-        out["minimal_2049"] = minimal(not a.no_cpu)
+        # ---- obstacle problem (c/ch12/obstacle.c), grid-sequenced to 2049^2
+        if not a.no_obstacle:
+            print("bench: obstacle 2049^2", file=sys.stderr, flush=True)
+            out["obstacle_2049"] = obstacle(not a.no_cpu)
+            out["obstacle_cluster_2049"] = obstacle_cluster(not a.no_cpu)
+        if not a.no_minimal:
+            print("bench: minimal surface 2049^2", file=sys.stderr, flush=True)
+            out["minimal_2049"] = minimal(not a.no_cpu)
 
-    # ---- CPU baseline ------------------------------------------------------
+    # ---- CPU baseline: the same 513^3 FP64 system -----------------------------
     if not a.no_cpu:
-        print("bench: CPU baseline (oracle)", file=sys.stderr, flush=True)
-        out["cpu_baseline"] = cpu_baseline(C2)
+        print("bench: CPU baseline (oracle, 513^3)", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(C3_FP64, "513^3")
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)

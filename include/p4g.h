@@ -95,13 +95,27 @@ int p4g_da_destroy(p4g_DA *da);
 /* --------------------------------------------------------------- solver */
 
 /* SNES KSPONLY + KSPCG + options (fish.c:223-234).  Supported options:
- *   -ksp_type cg, -ksp_rtol, -ksp_atol, -ksp_divtol, -ksp_max_it,
+ *   -ksp_type cg|gmres|richardson (gmres: FP64, left preconditioning,
+ *       -ksp_gmres_restart n; richardson: -ksp_richardson_scale s),
+ *   -ksp_rtol, -ksp_atol, -ksp_divtol, -ksp_max_it, -ksp_monitor,
  *   -ksp_norm_type preconditioned|unpreconditioned|natural,
- *   -pc_type mg|jacobi|none, -pc_mg_levels L, -pc_mg_cycle_type v|w,
- *   -mg_levels_ksp_type chebyshev|richardson, -mg_levels_pc_type jacobi|none,
+ *   -pc_type mg|jacobi|none|ilu|icc (ilu/icc: ILU(0) = IC(0) of this
+ *       operator, PETSc's serial default PC; FP64, constant coefficients),
+ *   -pc_mg_levels L, -pc_mg_cycle_type v|w,
+ *   -pc_mg_type multiplicative|full|kaskade, -pc_mg_multiplicative_cycles n,
+ *   -pc_mg_galerkin [both|pmat|mat|none] (P^T A P coarse operators, FP64),
+ *   -mg_levels_ksp_type chebyshev|richardson,
+ *   -mg_levels_pc_type sor|jacobi|none|ilu|icc (sor, PETSc's default: the
+ *       symmetric omega = 1 sweep, one iteration; -p4g_sor_blocks N sweeps
+ *       N ranks' z-blocks locally as PETSc's parallel SOR does; FP64,
+ *       constant coefficients),
  *   -mg_levels_ksp_max_it n, -mg_levels_ksp_richardson_scale s,
  *   -mg_levels_ksp_chebyshev_esteig_steps n,
- *   -mg_coarse_ksp_type preonly, -mg_coarse_pc_type lu|cholesky|redundant,
+ *   -mg_coarse_ksp_type preonly with -mg_coarse_pc_type lu|cholesky|redundant,
+ *       or -mg_coarse_ksp_type cg -mg_coarse_ksp_convergence_test skip
+ *       -mg_coarse_ksp_max_it n with -mg_coarse_pc_type icc|ilu|jacobi|none
+ *       (a fixed-count PCG coarse solve),
+ *   -ksp_view_mat|-ksp_view_rhs|-ksp_view_solution binary:<file>,
  *   -p4g_precision double|mixed  (mixed: PCG with FP32 direction, residual
  *       and preconditioner vectors and an FP32 V-cycle, an FP64 iterate
  *       (x += alpha p in FP64), FP64 dot accumulation and coarsest solve, and
@@ -121,9 +135,12 @@ int p4g_da_destroy(p4g_DA *da);
  *       larger than 768 MiB (below that the separate streaming passes are
  *       faster); -p4g_fuse_r and -p4g_fuse_dir (auto|1|0, default: the
  *       value of -p4g_fuse_cg) set the two fusions separately.
- * Options of PETSc objects the device path does not implement (-pc_type
- * ilu|sor|lu, -mg_levels_pc_type sor, -pc_mg_galerkin, ...) are rejected
- * with PETSC_ERR_SUP. */
+ * Options and combinations the device path does not implement are rejected
+ * with PETSC_ERR_SUP and a message naming them, never silently replaced:
+ * e.g. -pc_type lu|sor|asm, -pc_mg_type additive, ILU with fill, SOR with
+ * omega != 1 or one-directional sweeps, -mg_levels_ksp_norm_type, and SOR,
+ * ILU, GMRES, Richardson, Galerkin or kaskade together with
+ * -p4g_precision mixed. */
 int p4g_solver_create(p4g_DA da, const char *options, p4g_Solver *s);
 /* PCSetUp_MG equivalent: Jacobian callbacks per level, stencil extraction,
  * device hierarchy, Chebyshev eigen-estimates, coarse Cholesky. */
@@ -308,6 +325,13 @@ int p4g_vec_load_binary(const char *path, int index, long *n, double *x);
  * this DMDA, as an AIJ matrix: sorted column indices per row, the entries
  * the callback inserts (MatView of the assembled Jacobian); host only */
 int p4g_da_mat_view_binary(p4g_DA da, const char *path, int append);
+/* MatIsSymmetric(J, tol) of the matrix the Jacobian callback assembles on
+ * this DMDA at the iterate u (host, the grid's size; NULL: zero): every
+ * entry it inserts (the last value at a position, as INSERT_VALUES keeps)
+ * against its transpose, |a_ij - a_ji| <= tol, before any compression into
+ * the device stencil.  What -mat_is_symmetric tol reports after each
+ * assembly (c/ch6/makefile:14,20,33; c/ch12/makefile:17); host only. */
+int p4g_da_mat_is_symmetric(p4g_DA da, const double *u, double tol, int *symmetric);
 
 /* ---------------------------------------------------- device utilities */
 

@@ -113,6 +113,15 @@ void oracle_minimal_fdjac(int mx, int my, int problem, double q, double H, doubl
 int oracle_fish_time(int argc, const char **argv, int nthreads, int warmup,
                      int nsolves, double *times_ms, fish_result *res);
 
+/* As oracle_fish_time, on one set-up (whose wall time goes to *setup_ms),
+ * followed, when sample_its > 0, by one solve on a single thread capped at
+ * sample_its iterations: its wall time to *sample_ms and its iteration count
+ * to *sample_done. */
+int oracle_fish_time_sample(int argc, const char **argv, int nthreads, int warmup,
+                            int nsolves, double *times_ms, fish_result *res,
+                            double *setup_ms, int sample_its, double *sample_ms,
+                            int *sample_done);
+
 /* One preconditioner application z = B r (PCApply), one operator
  * application y = A x (MatMult), and F(u0), for the system fish builds
  * from argv.  Arrays have mx*my*mz doubles. */

@@ -109,10 +109,10 @@ static int session_setup(osession *s, int argc, const char **argv, char *out, in
     const char *sym = oopts_get(o, "mat_is_symmetric");
     if (sym) {
         double tol = *sym ? atof(sym) : 0.0;
-        if (ocsr_is_symmetric(&s->A, tol)) {
-            oprintf(out, outlen, pos, "Matrix is symmetric (tolerance %g)\n", tol);
-            oprintf(out, outlen, pos, "Matrix is symmetric (tolerance %g)\n", tol);
-        }
+        /* after each assembly: the preallocated zero matrix, then the Jacobian */
+        oprintf(out, outlen, pos, "Matrix is symmetric (tolerance %g)\n", tol);
+        oprintf(out, outlen, pos, "Matrix is %ssymmetric (tolerance %g)\n",
+                ocsr_is_symmetric(&s->A, tol) ? "" : "not ", tol);
     }
     oopts_get(o, "snes_fd_color");   /* exact Jacobian is used (linear problem) */
     /* KSP (fish.c:231-234) */
@@ -271,13 +271,23 @@ int oracle_rhs(int argc, const char **argv, double *F) {
 
 int oracle_fish_time(int argc, const char **argv, int nthreads, int warmup, int nsolves,
                      double *times_ms, fish_result *res) {
+    return oracle_fish_time_sample(argc, argv, nthreads, warmup, nsolves, times_ms, res,
+                                   NULL, 0, NULL, NULL);
+}
+
+int oracle_fish_time_sample(int argc, const char **argv, int nthreads, int warmup,
+                            int nsolves, double *times_ms, fish_result *res,
+                            double *setup_ms, int sample_its, double *sample_ms,
+                            int *sample_done) {
 #ifdef _OPENMP
     int saved = omp_get_max_threads();
     if (nthreads > 0) omp_set_num_threads(nthreads);
 #endif
     osession s;
     int pos = 0;
+    double ts = now_ms();
     int e = session_setup(&s, argc, argv, NULL, 0, &pos);
+    if (setup_ms) *setup_ms = now_ms() - ts;
     if (e) { session_free(&s); return e; }
     fish_result r;
     memset(&r, 0, sizeof r);
@@ -289,6 +299,20 @@ int oracle_fish_time(int argc, const char **argv, int nthreads, int warmup, int 
     }
     r.dim = s.g.dim; r.mx = s.g.mx; r.my = s.g.my; r.mz = s.g.mz;
     if (res) *res = r;
+    if (sample_its > 0) {
+#ifdef _OPENMP
+        omp_set_num_threads(1);
+#endif
+        int maxit = s.ksp.maxit;
+        s.ksp.maxit = sample_its;
+        fish_result r1;
+        memset(&r1, 0, sizeof r1);
+        double t0 = now_ms();
+        int its = session_solve(&s, &r1);
+        if (sample_ms) *sample_ms = now_ms() - t0;
+        if (sample_done) *sample_done = its;
+        s.ksp.maxit = maxit;
+    }
     session_free(&s);
 #ifdef _OPENMP
     omp_set_num_threads(saved);

@@ -97,6 +97,36 @@ def fish_time(args, nthreads=1, warmup=1, nsolves=3):
     return list(times), res
 
 
+def build_native(outdir, timeout=300):
+    """Build the CPU-baseline copy of the oracle for this machine's
+    instruction set (-march=native) into outdir; returns its path."""
+    import subprocess
+    subprocess.run(["make", "-s", "-B", "-C", _HERE, "native", "NATIVE=" + outdir],
+                   check=True, timeout=timeout)
+    return os.path.join(outdir, "liboracle.so")
+
+
+def fish_time_sample(args, nthreads, warmup, nsolves, sample_its, libpath=None):
+    """oracle_fish_time_sample: solve times (ms) on nthreads threads, the
+    set-up time, and one single-thread solve capped at sample_its iterations.
+    libpath selects another build of the oracle (e.g. build_native's)."""
+    L = ctypes.CDLL(libpath) if libpath else lib()
+    f = L.oracle_fish_time_sample
+    f.restype = ctypes.c_int
+    argc, argv = _argv(args)
+    times = (ctypes.c_double * nsolves)()
+    res = FishResult()
+    setup = ctypes.c_double()
+    sms = ctypes.c_double()
+    sdone = ctypes.c_int()
+    rc = f(argc, argv, nthreads, warmup, nsolves, times, ctypes.byref(res), ctypes.byref(setup),
+           sample_its, ctypes.byref(sms), ctypes.byref(sdone))
+    if rc:
+        raise RuntimeError("oracle_fish_time_sample failed: %d" % rc)
+    return {"times_ms": list(times), "res": res, "setup_ms": setup.value,
+            "sample_ms": sms.value, "sample_its": sdone.value}
+
+
 def _dptr(a):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
 

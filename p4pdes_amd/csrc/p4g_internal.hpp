@@ -77,6 +77,9 @@ struct p4g_mat_s {
     bool   have_s[3] = {false, false, false};
     std::string why;                 // first mismatch
     std::vector<char> seen;          // rows inserted
+    bool   keep = false;             // also record every entry (MatIsSymmetric)
+    std::vector<long> er, ec;        // recorded rows and columns, insertion order
+    std::vector<double> ev;          // recorded values
 };
 
 // ------------------------------------------------------------ host vectors

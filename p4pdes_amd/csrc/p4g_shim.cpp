@@ -308,6 +308,14 @@ PetscErrorCode MatSetValuesStencil(Mat A, PetscInt m, const MatStencil idxm[], P
             const int di = idxn[c].i - i, dj = dim > 1 ? idxn[c].j - j : 0,
                       dk = dim > 2 ? idxn[c].k - k : 0;
             const double val = v[r * n + c];
+            if (A->keep) {
+                const int ci = idxn[c].i, cj = dim > 1 ? idxn[c].j : 0, ck = dim > 2 ? idxn[c].k : 0;
+                if (ci < 0 || ci >= mx || cj < 0 || cj >= my || ck < 0 || ck >= mz)
+                    return set_error(PETSC_ERR_ARG_OUTOFRANGE, "MatSetValuesStencil: column out of range");
+                A->er.push_back(row);
+                A->ec.push_back(((long)ck * my + cj) * mx + ci);
+                A->ev.push_back(val);
+            }
             if (di == 0 && dj == 0 && dk == 0) {
                 hasdiag = true;
                 if (bnd) {

@@ -2717,6 +2717,53 @@ int p4g_da_mat_view_binary(DM da, const char* path, int append) {
     return mat_view_binary(tmp.lv.back(), path, append != 0);
 }
 
+int p4g_da_mat_is_symmetric(DM da, const double* u, double tol, int* symmetric) {
+    clear_error();
+    if (!da || !symmetric) return set_error(PETSC_ERR_ARG_WRONG, "p4g_da_mat_is_symmetric: null argument");
+    if (!da->jac) return set_error(PETSC_ERR_ARG_WRONG, "no Jacobian callback: call p4g_da_set_jacobian_local");
+    const long n = da->n();
+    p4g_mat_s M;
+    M.da = da;
+    M.seen.assign(n, 0);
+    M.keep = true;
+    std::vector<double> ux(u ? u : nullptr, u ? u + n : nullptr);
+    if (!u) ux.assign(n, 0.0);
+    std::vector<double*> rows;
+    std::vector<double**> planes;
+    void* au = make_view(da, ux.data(), rows, planes);
+    DMDALocalInfo info;
+    info_of(da, &info);
+    P4G_TRY(da->jac(&info, au, &M, &M, da->jacctx));
+    // INSERT_VALUES: the last value inserted at (row, column) is the entry
+    const size_t ne = M.ev.size();
+    std::vector<size_t> ord(ne);
+    for (size_t q = 0; q < ne; ++q) ord[q] = q;
+    std::stable_sort(ord.begin(), ord.end(), [&](size_t a, size_t b) {
+        return M.er[a] != M.er[b] ? M.er[a] < M.er[b] : M.ec[a] < M.ec[b];
+    });
+    std::vector<long> r, c;
+    std::vector<double> v;
+    for (size_t q = 0; q < ne; ++q) {
+        const size_t e = ord[q];
+        if (!r.empty() && r.back() == M.er[e] && c.back() == M.ec[e]) { v.back() = M.ev[e]; continue; }
+        r.push_back(M.er[e]); c.push_back(M.ec[e]); v.push_back(M.ev[e]);
+    }
+    // |a_ij - a_ji| <= tol for every entry (an absent entry is zero)
+    auto find = [&](long i, long j) -> double {
+        size_t lo = 0, hi = r.size();
+        while (lo < hi) {
+            const size_t mid = (lo + hi) / 2;
+            if (r[mid] < i || (r[mid] == i && c[mid] < j)) lo = mid + 1; else hi = mid;
+        }
+        return (lo < r.size() && r[lo] == i && c[lo] == j) ? v[lo] : 0.0;
+    };
+    int sym = 1;
+    for (size_t q = 0; q < r.size() && sym; ++q)
+        if (!(std::fabs(v[q] - find(c[q], r[q])) <= tol)) sym = 0;
+    *symmetric = sym;
+    return 0;
+}
+
 int p4g_ksp_solve_device(p4g_Solver s, const double* d_b, double* d_x, p4g_Report* rep) {
     clear_error();
     if (!s || !d_b || !d_x) return set_error(PETSC_ERR_ARG_WRONG, "p4g_ksp_solve_device: null argument");

@@ -259,6 +259,11 @@ int fish_run(int argc, const char **argv, fish_result *res, double *u_out, char 
                      u, &S.ctx.pc);
     PetscScalar *ua = NULL;
     VecGetArray(u, &ua);
+    const char *sym = opt_get(&a, "mat_is_symmetric");
+    int jac_sym = 1;
+    if (sym && !e) {   /* MatIsSymmetric of the Jacobian the callback assembles at u0 */
+        e = p4g_da_mat_is_symmetric(S.da, ua, *sym ? atof(sym) : 0.0, &jac_sym);
+    }
     p4g_Report rep;
     memset(&rep, 0, sizeof rep);
     if (!e) e = p4g_snes_solve(S.solver, ua, &rep);
@@ -270,11 +275,11 @@ int fish_run(int argc, const char **argv, fish_result *res, double *u_out, char 
     }
     const int monitor = opt_get(&a, "snes_monitor_short") || opt_get(&a, "snes_monitor");
     if (monitor) snes_line(&o, 0, rep.fnorm0);
-    const char *sym = opt_get(&a, "mat_is_symmetric");
-    if (sym) {   /* the extracted stencil is symmetric by construction */
+    if (sym) {   /* PETSc reports after each assembly: the preallocated (zero, so
+                    symmetric) matrix at SNES set-up, then the Jacobian */
         const double tol = *sym ? atof(sym) : 0.0;
         outf(&o, "Matrix is symmetric (tolerance %g)\n", tol);
-        outf(&o, "Matrix is symmetric (tolerance %g)\n", tol);
+        outf(&o, "Matrix is %ssymmetric (tolerance %g)\n", jac_sym ? "" : "not ", tol);
     }
     if (opt_get(&a, "ksp_monitor")) {   /* KSPMonitorResidual, at the KSP's tab level under SNES */
         int nh = 0;

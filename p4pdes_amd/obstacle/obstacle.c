@@ -205,7 +205,12 @@ int obstacle_run(int argc, const char **argv, obstacle_result *res, double *u_ou
                 u = uf;
             }
             p4g_VIReport rep;
-            e = p4g_vi_solve(s, xl, NULL, u, F, &rep);
+            /* -mat_is_symmetric: the Jacobian the callback assembles, checked at
+             * the stage's first iterate (Poisson2DJacobianLocal does not depend
+             * on the iterate, so every Newton step assembles the same matrix) */
+            int jac_sym = 1;
+            if (sv) e = p4g_da_mat_is_symmetric(da, u, *sv ? atof(sv) : 0.0, &jac_sym);
+            if (!e) e = p4g_vi_solve(s, xl, NULL, u, F, &rep);
             const int ind = 2 * (seq - stage);
             if (!e) {
                 int nh = 0, nk = 0;
@@ -214,10 +219,11 @@ int obstacle_run(int argc, const char **argv, obstacle_result *res, double *u_ou
                 double *fh = (double *)malloc(sizeof(double) * cap);
                 int *ki = (int *)malloc(sizeof(int) * cap), *kr = (int *)malloc(sizeof(int) * cap);
                 p4g_vi_get_history(s, fh, ki, kr, cap, &nh, &nk);
-                if (sv) outf(&o, "%*sMatrix is symmetric (tolerance %s)\n", ind, "", symtol);
+                if (sv) outf(&o, "%*sMatrix is symmetric (tolerance %s)\n", ind, "", symtol);   /* preallocated, zero */
                 if (monitor && nh > 0) snes_line(&o, ind, 0, fh[0]);
                 for (int q = 0; q < nk; q++) {
-                    if (sv) outf(&o, "%*sMatrix is symmetric (tolerance %s)\n", ind, "", symtol);
+                    if (sv) outf(&o, "%*sMatrix is %ssymmetric (tolerance %s)\n", ind, "", jac_sym ? "" : "not ",
+                                 symtol);
                     if (kreason) {
                         if (kr[q] > 0)
                             outf(&o, "%*s    Linear solve converged due to %s iterations %d\n", ind, "",

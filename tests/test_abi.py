@@ -5,6 +5,7 @@ import ctypes
 import os
 import re
 
+import numpy as np
 import pytest
 
 from p4pdes_amd import capi
@@ -123,3 +124,93 @@ def test_grid_shape_matches_oracle():
     from oracle import oracle_py
     for a in ("-fsh_dim 3 -da_refine 3", "-fsh_dim 2 -da_refine 6", "-fsh_dim 1 -da_refine 4 -da_grid_x 5"):
         assert capi.grid_shape(a) == oracle_py.grid_shape(a)
+
+
+REFERENCE_CH6 = "/root/reference/c/ch6"
+
+
+def _cc(args, cwd):
+    import subprocess
+    r = subprocess.run(["gcc"] + args, cwd=cwd, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    return r
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(REFERENCE_CH6, "poissonfunctions.c")),
+                    reason="the reference tree is not present on this machine")
+def test_reference_callbacks_compile_unchanged_and_match(tmp_path):
+    """The drop-in claim: the reference's own c/ch6/poissonfunctions.c, compiled
+    unchanged against include/petsc.h (-std=c99 -Wall -Werror), registers with
+    p4g_da_set_function_local / p4g_da_set_jacobian_local, and defines the same
+    residual and the same operator (written by p4g_da_mat_view_binary) as this
+    package's callbacks, byte for byte, in 1D, 2D and 3D."""
+    import subprocess
+    inc = os.path.join(ROOT, "include")
+    lib = os.path.dirname(capi.LIBP4G)
+    drv = os.path.join(ROOT, "tests", "boundary_driver.c")
+    exes = {}
+    for tag, hdir in (("ref", REFERENCE_CH6), ("prod", os.path.join(ROOT, "p4pdes_amd", "fish"))):
+        flags = ["-std=c99", "-Wall", "-Werror", "-I" + inc, "-I" + hdir]
+        _cc(flags + ["-c", os.path.join(hdir, "poissonfunctions.c"), "-o", "pf_%s.o" % tag], tmp_path)
+        _cc(flags + ["-c", drv, "-o", "drv_%s.o" % tag], tmp_path)
+        _cc(["drv_%s.o" % tag, "pf_%s.o" % tag, "-L" + lib, "-lp4g", "-Wl,-rpath," + lib, "-lm",
+             "-o", "drv_%s" % tag], tmp_path)
+        exes[tag] = str(tmp_path / ("drv_%s" % tag))
+    for dim, m in ((1, 17), (2, 9), (3, 9)):
+        outs = {}
+        for tag, exe in exes.items():
+            d = tmp_path / ("%s%d" % (tag, dim))
+            d.mkdir()
+            r = subprocess.run([exe, str(dim), str(m), str(d)], capture_output=True, text=True)
+            assert r.returncode == 0, r.stderr
+            outs[tag] = ((d / "A.bin").read_bytes(), (d / "F.bin").read_bytes())
+        assert outs["ref"][0] == outs["prod"][0], "operator differs in %dD" % dim
+        assert outs["ref"][1] == outs["prod"][1], "residual differs in %dD" % dim
+        F = capi.vec_load_binary(tmp_path / ("ref%d" % dim) / "F.bin")
+        n = m * (m + 2 if dim > 1 else 1) * (m + 4 if dim > 2 else 1)
+        assert F.size == n and np.count_nonzero(F) > n // 2
+        # the AIJ header: class id, rows, columns, nonzeros (PETSc binary, big-endian int32)
+        hdr = np.frombuffer(outs["ref"][0][:16], dtype=">i4")
+        assert hdr[1] == n and hdr[2] == n and n < hdr[3] <= (2 * dim + 1) * n
+
+
+def test_header_compiles_hosted_by_petsc(tmp_path):
+    """With P4G_HOSTED_BY_PETSC (a program that also includes PETSc's own
+    headers) p4g.h stands alone: the PETSc-shaped surface is left out and the
+    solver ABI still compiles, as C and as C++."""
+    src = tmp_path / "hosted.c"
+    src.write_text("#define P4G_HOSTED_BY_PETSC 1\n#include \"p4g.h\"\n"
+                   "#ifdef P4G_PETSC_H\n#error p4g_petsc.h was included\n#endif\n"
+                   "int use(p4g_DA da, p4g_Solver s, const double *r, double *z) {\n"
+                   "    p4g_Report rep; int n;\n"
+                   "    if (p4g_solver_create(da, \"-pc_type mg\", &s)) return 1;\n"
+                   "    if (p4g_pc_apply(s, r, z)) return 2;\n"
+                   "    if (p4g_ksp_solve(s, r, z, &rep)) return 3;\n"
+                   "    return p4g_device_count(&n);\n}\n")
+    inc = "-I" + os.path.join(ROOT, "include")
+    _cc(["-std=c99", "-Wall", "-Werror", inc, "-c", str(src), "-o", "h.o"], tmp_path)
+    import subprocess
+    r = subprocess.run(["g++", "-x", "c++", "-std=c++17", "-Wall", "-Werror", inc, "-c", str(src),
+                        "-o", "hpp.o"], cwd=tmp_path, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+
+
+def test_mat_is_symmetric_checks_the_callback(tmp_path):
+    """-mat_is_symmetric is a check of the entries the Jacobian callback
+    inserts: the package's callbacks are symmetric; a copy with one coupling
+    re-inserted 0.1% larger is not, at tolerances below that difference, and
+    is within a tolerance above it."""
+    import subprocess
+    inc = os.path.join(ROOT, "include")
+    lib = os.path.dirname(capi.LIBP4G)
+    fish = os.path.join(ROOT, "p4pdes_amd", "fish")
+    exe = str(tmp_path / "drv")
+    _cc(["-std=c99", "-Wall", "-Werror", "-I" + inc, "-I" + fish, os.path.join(ROOT, "tests", "boundary_driver.c"),
+         os.path.join(fish, "poissonfunctions.c"), "-L" + lib, "-lp4g", "-Wl,-rpath," + lib, "-lm", "-o", exe],
+        tmp_path)
+    for dim, m in ((1, 9), (2, 9), (3, 5)):
+        out = subprocess.run([exe, "sym", str(dim), str(m), "1e-7"], capture_output=True, text=True)
+        assert out.returncode == 0, out.stderr
+        assert out.stdout.split() == ["symmetric", "1", "skewed", "0"], (dim, out.stdout)
+        out = subprocess.run([exe, "sym", str(dim), str(m), "1"], capture_output=True, text=True)
+        assert out.stdout.split() == ["symmetric", "1", "skewed", "1"], (dim, out.stdout)

@@ -272,9 +272,10 @@ def test_obstacle_device_golden_test3_default_smoother():
 @pytest.mark.gpu
 @pytest.mark.parametrize("num", [1, 3])
 def test_obstacle_device_golden_lines(num):
-    # the reference's commands with the device's multigrid (PETSc's default
-    # ILU (test1) and SOR smoothing (test3) are not implemented on the device):
-    # the Newton history and the report do not depend on the preconditioner
+    # the reference's commands with Jacobi-smoothed multigrid in place of
+    # their own PCs (test1: PETSc's default ILU(0); test3: Chebyshev/SOR,
+    # both run byte for byte in their own tests): the Newton history and the
+    # report do not depend on the preconditioner
     from p4pdes_amd import capi
     t = GOLDEN[num]
     args = t["options"] + " -pc_type mg -mg_levels_pc_type jacobi"
