@@ -81,8 +81,13 @@ def measured_traffic(grid, prec, cls_name):
     with open(path) as f:
         d = json.load(f)
     k = d.get("kernels", {})
-    k = k.get(prec, k) if prec in k else k
-    t = k.get(cls_name)
+    if "classes" in d:   # per precision: class -> the finest-level kernel's name
+        name = d["classes"].get(prec, {}).get(cls_name)
+        t = k.get(prec, {}).get(name) if name else None
+        if t is not None:
+            t = dict(t, kernel=name)
+    else:                # keyed by class
+        t = k.get(cls_name)
     if t is not None:
         t = dict(t, source=rel)
     return t

@@ -181,6 +181,9 @@ enum {
     P4G_K_REDUCE,          /* finalisation of fused dot products               */
     P4G_K_APPLY,           /* y = A x, plain operator apply (MatMult)          */
     P4G_K_CG_RESID,        /* r -= a w, the CG residual update on its own      */
+    P4G_K_SMOOTH_ZG,       /* a smoothing step from a zero guess fused with the
+                              CG residual update (r = r - a w formed and stored) */
+    P4G_K_SMOOTH_LAST,     /* the cycle's last smoothing step, with its dots    */
     P4G_K_OTHER,
     P4G_K_COUNT
 };

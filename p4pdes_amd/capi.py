@@ -16,9 +16,9 @@ LIBMIN = os.path.join(LIBDIR, "libp4g_minimal.so")
 
 # kernel classes (enum in p4g.h)
 K_STENCIL_DOT, K_CG_UPDATE, K_RESID_ZG, K_RESTRICT, K_PROLONG, K_SMOOTH, K_COARSE, \
-    K_REDUCE, K_APPLY, K_CG_RESID, K_OTHER, K_COUNT = range(12)
+    K_REDUCE, K_APPLY, K_CG_RESID, K_SMOOTH_ZG, K_SMOOTH_LAST, K_OTHER, K_COUNT = range(14)
 KERNEL_NAMES = ["stencil_dot", "cg_update", "resid_zg", "restrict", "prolong", "smooth",
-                "coarse", "reduce", "apply", "cg_resid", "other"]
+                "coarse", "reduce", "apply", "cg_resid", "smooth_zg", "smooth_last", "other"]
 
 REASONS = {2: "CONVERGED_RTOL", 3: "CONVERGED_ATOL", 4: "CONVERGED_ITS", -3: "DIVERGED_ITS",
            -4: "DIVERGED_DTOL", -5: "DIVERGED_BREAKDOWN", -8: "DIVERGED_INDEFINITE_MAT",

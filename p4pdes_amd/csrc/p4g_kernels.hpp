@@ -171,6 +171,14 @@ struct SrcArr {                       // u = x
     __device__ __forceinline__ void put(long, T) const {}
     template <int V>
     __device__ __forceinline__ void putv(long, const TV<T, V>&) const {}
+    // LDS staging (k_stencil_lds): the arrays read at stencil neighbours, the
+    // operand from their staged values (s: lds_scalar()), the own-point store
+    static constexpr int NA = 1;
+    __device__ __forceinline__ const T* lds_arr(int) const { return x; }
+    __device__ __forceinline__ T lds_scalar() const { return T(0); }
+    __device__ __forceinline__ T lds_comb(T v0, T, T) const { return v0; }
+    template <int V>
+    __device__ __forceinline__ void lds_put(long, const TV<T, V>&, const TV<T, V>&) const {}
 };
 
 template <typename T>
@@ -207,6 +215,12 @@ struct SrcDir {                       // p_new = z + beta p_old, stored (CG dire
     __device__ __forceinline__ void put(long q, T v) const { pnew[q] = v; }
     template <int V>
     __device__ __forceinline__ void putv(long q, const TV<T, V>& u) const { stv<T, V>(pnew + q, u); }
+    static constexpr int NA = 2;
+    __device__ __forceinline__ const T* lds_arr(int a) const { return a ? pold : z; }
+    __device__ __forceinline__ T lds_scalar() const { return T(*beta); }
+    __device__ __forceinline__ T lds_comb(T v0, T v1, T bt) const { return v0 + bt * v1; }
+    template <int V>
+    __device__ __forceinline__ void lds_put(long q, const TV<T, V>& u, const TV<T, V>&) const { stv<T, V>(pnew + q, u); }
 };
 
 // p_new = z + beta p_old, stored; at the own point also the deferred CG
@@ -249,6 +263,24 @@ struct SrcDirX {
             stv<X, VX>(x + q + h, xx);
         }
     }
+    static constexpr int NA = 2;
+    __device__ __forceinline__ const T* lds_arr(int a) const { return a ? pold : z; }
+    __device__ __forceinline__ T lds_scalar() const { return T(*beta); }
+    __device__ __forceinline__ T lds_comb(T v0, T v1, T bt) const { return v0 + bt * v1; }
+    // pp: p_old at the own points, as staged
+    template <int V>
+    __device__ __forceinline__ void lds_put(long q, const TV<T, V>& u, const TV<T, V>& pp) const {
+        stv<T, V>(pnew + q, u);
+        const X al = X(*alpha);
+        constexpr int VX = 16 / sizeof(X) < V ? 16 / sizeof(X) : V;
+#pragma unroll
+        for (int h = 0; h < V; h += VX) {
+            TV<X, VX> xx = ldv<X, VX>(x + q + h);
+#pragma unroll
+            for (int e = 0; e < VX; ++e) xx.v[e] += al * X(pp.v[h + e]);
+            stv<X, VX>(x + q + h, xx);
+        }
+    }
 };
 
 // u = r - alpha w, stored to rn: the CG residual update fused into the
@@ -273,6 +305,12 @@ struct SrcUpd {
     __device__ __forceinline__ void put(long q, T v) const { rn[q] = v; }
     template <int V>
     __device__ __forceinline__ void putv(long q, const TV<T, V>& u) const { stv<T, V>(rn + q, u); }
+    static constexpr int NA = 2;
+    __device__ __forceinline__ const T* lds_arr(int a) const { return a ? w : r; }
+    __device__ __forceinline__ T lds_scalar() const { return T(*alpha); }
+    __device__ __forceinline__ T lds_comb(T v0, T v1, T al) const { return v0 - al * v1; }
+    template <int V>
+    __device__ __forceinline__ void lds_put(long q, const TV<T, V>& u, const TV<T, V>&) const { stv<T, V>(rn + q, u); }
 };
 
 // u = x + alpha p (X wide, P narrow), stored to xnew: a pending iterate
@@ -852,6 +890,216 @@ k_stencil(int mx, int my, int mz, int kch, Op op, Src src, Out out, double* __re
     const Tile tl = xcd_tile();
     stencil_body<T, DIM, Op, Src, Out>(tl.bx, tl.by, tl.bz, mx, my, mz, kch, op, src, out, a0, a1);
     block_partials<Out::NDOT>(a0, a1, part, tl.id);
+}
+
+// ------------------------------------------------ LDS-staged stencil pass
+//
+// The 3D marching pass of stencil_vec with the source's arrays staged in
+// LDS.  A block of TYD waves covers the rows j0 .. j0+TYD-1 (j0 = TYD by) of
+// W = 64 V points (i = is .. is+W-1, is = 1 + W bx) and marches the planes
+// k0 .. k1-1.  Every plane's rows j0-1 .. j0+TYD of each array the source
+// reads at stencil neighbours go into a ring of S = P + 3 LDS slots by
+// LDS-DMA loads issued P planes ahead of use: one global_load_lds_dwordx4
+// lands a row's W values (64 lanes x 16 bytes), one 4-byte load per lane
+// the two x-halo values (is-1 and is+W) a dword at a time.  A wave waits for
+// its own loads of the planes it needs (s_waitcnt vmcnt, counting only
+// those loads, so the wait also covers any later memory operation) and the
+// block's barrier makes every wave's rows visible; a slot is refilled only
+// after the barrier that follows its last read.  The operand of the own row
+// at planes k-1, k is kept in registers, the plane k+1 and rows j+-1 come
+// from LDS, so each staged value is read from memory once per block where
+// stencil_vec reads rows j+-1 again through the cache.  The output's
+// per-point operands are plain loads one plane ahead.  The arithmetic per
+// point is stencil_vec's (same operand formula, same operation order).
+// Requires 3D, (mx - 1) % W == 0 and 4- or 8-byte elements.
+// Grid (ceil((mx-1)/W), ceil(my/TYD), ceil(mz/kch)), block (64, TYD).
+
+// s_waitcnt vmcnt(n) alone (the other counters not waited for)
+#define P4G_VMW(N)                                          \
+    case N:                                                 \
+        asm volatile("s_waitcnt vmcnt(" #N ")" ::: "memory"); \
+        break;
+__device__ __forceinline__ void wait_vmcnt(int n) {
+    switch (n) {
+        P4G_VMW(0) P4G_VMW(1) P4G_VMW(2) P4G_VMW(3) P4G_VMW(4) P4G_VMW(5) P4G_VMW(6) P4G_VMW(7) P4G_VMW(8)
+        P4G_VMW(9) P4G_VMW(10) P4G_VMW(11) P4G_VMW(12) P4G_VMW(13) P4G_VMW(14) P4G_VMW(15) P4G_VMW(16)
+        default: asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); break;
+    }
+}
+#undef P4G_VMW
+// a workgroup barrier that orders the surrounding memory operations in the
+// compiler without waiting for outstanding loads (the waits are explicit)
+__device__ __forceinline__ void lds_barrier() { asm volatile("s_barrier" ::: "memory"); }
+
+typedef __attribute__((address_space(3))) void lds_void_t;
+
+template <typename T, class Op, class Src, class Out, int TYD, int P>
+__device__ __forceinline__ void stencil_lds(int bx, int by, int bz, int mx, int my, int mz, int kch, const Op& op,
+                                            const Src& src, const Out& out, double& a0, double& a1) {
+    constexpr int V = VecOf<T>::V, W = TX * V, S = P + 3, NA = Src::NA;
+    constexpr int DPE = sizeof(T) / 4;               // dwords per element
+    constexpr int NP = Out::NPRE, NPA = NP > 0 ? NP : 1;
+    static_assert(V * sizeof(T) == 16 && (DPE == 1 || DPE == 2), "16-byte lanes of 4- or 8-byte elements");
+    __shared__ __attribute__((aligned(16))) T ring[NA][S][TYD + 2][W];
+    __shared__ __attribute__((aligned(16))) T halo[NA][S][TYD + 2][2];
+    const int lane = threadIdx.x;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.y);
+    const int is = 1 + W * bx;
+    const int j0 = TYD * by;
+    const int k0 = bz * kch, k1 = min(k0 + kch, mz);
+    if (k0 >= k1) return;                            // uniform per block
+    const int klast = min(k1, mz - 1);               // last plane read
+    // the rows this wave stages: r = w and r = w + TYD (rows j0 - 1 + r)
+    const int ra = w, rb = w + TYD;
+    const bool oka = j0 - 1 + ra >= 0 && j0 - 1 + ra < my;
+    const bool okb = rb < TYD + 2 && j0 - 1 + rb < my;
+    const int nload = 2 * NA * ((int)oka + (int)okb);   // DMA instructions per plane
+    const bool has_e = is + W < mx;
+    const long sm = (long)mx * my;
+    auto stage_row = [&](int kk, int r) {
+        const int slot = (kk - k0 + 1) % S;
+        const long rowq = (long)kk * sm + (long)(j0 - 1 + r) * mx;
+#pragma unroll
+        for (int a = 0; a < NA; ++a) {
+            const T* g = src.lds_arr(a) + rowq;
+            __builtin_amdgcn_global_load_lds((const void*)(g + is + V * lane), (lds_void_t*)&ring[a][slot][r][0], 16,
+                                             0, 0);
+            if (lane < 2 * DPE) {                   // W dwords on lanes [0, DPE), E dwords on [DPE, 2 DPE)
+                const bool east = lane >= DPE;
+                if (!east || has_e)
+                    __builtin_amdgcn_global_load_lds(
+                        (const void*)((const char*)(g + (east ? is + W : is - 1)) + 4 * (lane % DPE)),
+                        (lds_void_t*)&halo[a][slot][r][0], 4, 0, 0);
+            }
+        }
+    };
+    auto stage = [&](int kk) {
+        if (kk < 0 || kk > klast) return;
+        if (oka) stage_row(kk, ra);
+        if (okb) stage_row(kk, rb);
+    };
+    for (int kk = k0 - 1; kk <= k0 + P; ++kk) stage(kk);
+
+    const int j = j0 + w;
+    const bool rowok = j < my;                       // uniform per wave
+    const bool jb = j == 0 || j == my - 1;
+    const bool ms = j - 1 > 0, mn = j + 1 < my - 1;
+    const int r = w + 1;
+    const int i0 = is + V * lane;
+    const T z0 = T(0);
+    const T sc_ = src.lds_scalar();
+    // the operand at LDS row rr of slot sl: V values of the lane, one element
+    auto opv = [&](int sl, int rr) -> TV<T, V> {
+        TV<T, V> v = ldv<T, V>(&ring[0][sl][rr][V * lane]);
+        if constexpr (NA == 2) {
+            const TV<T, V> v1 = ldv<T, V>(&ring[1][sl][rr][V * lane]);
+#pragma unroll
+            for (int e = 0; e < V; ++e) v.v[e] = src.lds_comb(v.v[e], v1.v[e], sc_);
+        }
+        return v;
+    };
+    auto op1 = [&](int sl, int rr, int idx) -> T {
+        T v = ring[0][sl][rr][idx];
+        if constexpr (NA == 2) v = src.lds_comb(v, ring[1][sl][rr][idx], sc_);
+        return v;
+    };
+    auto oph = [&](int sl, int rr, int e) -> T {
+        T v = halo[0][sl][rr][e];
+        if constexpr (NA == 2) v = src.lds_comb(v, halo[1][sl][rr][e], sc_);
+        return v;
+    };
+    TV<T, V> zv;
+#pragma unroll
+    for (int e = 0; e < V; ++e) zv.v[e] = z0;
+    TV<T, V> co[NPA], ro[NPA];
+    T cox[NPA], rox[NPA];
+#pragma unroll
+    for (int q = 0; q < NPA; ++q) {
+        co[q] = ro[q] = zv;
+        cox[q] = rox[q] = z0;
+    }
+    auto pre_own = [&](long qq, TV<T, V>* o, T* ox) {
+        if constexpr (NP > 0) {
+            out.template pre<V>(qq, o);
+            if (i0 == 1) out.pre1(qq - 1, ox);
+        }
+    };
+    long q = (long)k0 * sm + (long)j * mx + i0;
+    if (rowok) pre_own(q, co, cox);
+    TV<T, V> ud = zv, uc = zv;       // the own row's operand at planes k-1, k
+    for (int k = k0; k < k1; ++k, q += sm) {
+        // planes up to k+1 staged: at most the planes k+2 .. min(k+P, klast) outstanding
+        wait_vmcnt(max(0, min(k + P, klast) - (k + 1)) * nload);
+        lds_barrier();
+        stage(k + 1 + P);
+        if (!rowok) continue;                        // uniform per wave (no barrier skipped)
+        if (k + 1 < k1) pre_own(q + sm, ro, rox);
+        const int sd = (k - k0) % S, sc = (k - k0 + 1) % S, su = (k - k0 + 2) % S;
+        if (k == k0) {
+            uc = opv(sc, r);
+            if (k > 0) ud = opv(sd, r);
+        }
+        const TV<T, V> uu = k + 1 < mz ? opv(su, r) : zv;   // plane k+1 (read only when k+1 < mz)
+        TV<T, V> Au = zv, idg = zv;
+        if (jb || k == 0 || k == mz - 1) {
+#pragma unroll
+            for (int e = 0; e < V; ++e) op.bnd(i0 + e, j, k, uc.v[e], Au.v[e], idg.v[e]);
+        } else {
+            const TV<T, V> sv = ms ? opv(sc, r - 1) : zv, nv = mn ? opv(sc, r + 1) : zv;
+            const TV<T, V> dn = k - 1 > 0 ? ud : zv, up = k + 1 < mz - 1 ? uu : zv;
+            const T wv = lane > 0 ? op1(sc, r, V * lane - 1) : oph(sc, r, 0);
+            const T ev = lane < TX - 1 ? op1(sc, r, V * lane + V) : oph(sc, r, 1);
+#pragma unroll
+            for (int e = 0; e < V; ++e) {
+                const int i = i0 + e;
+                if (i == mx - 1) {
+                    op.bnd(i, j, k, uc.v[e], Au.v[e], idg.v[e]);
+                } else {
+                    const T wn = e == 0 ? (i - 1 > 0 ? wv : z0) : uc.v[e > 0 ? e - 1 : 0];
+                    const T east = i + 1 < mx - 1 ? (e == V - 1 ? ev : uc.v[e < V - 1 ? e + 1 : 0]) : z0;
+                    op.interior(i, j, k, uc.v[e], wn, east, sv.v[e], nv.v[e], dn.v[e], up.v[e], Au.v[e], idg.v[e]);
+                }
+            }
+        }
+        if constexpr (NA == 2) src.template lds_put<V>(q, uc, ldv<T, V>(&ring[1][sc][r][V * lane]));
+        else src.template lds_put<V>(q, uc, uc);
+        if constexpr (NP > 0) out.template pointvp<V>(q, uc, Au, idg, co, a0, a1);
+        else out.template pointv<V>(q, uc, Au, idg, a0, a1);
+        if (i0 == 1) {                               // the boundary point i = 0
+            const T u0 = oph(sc, r, 0);
+            T A0, i00;
+            src.put(q - 1, u0);
+            op.bnd(0, j, k, u0, A0, i00);
+            if constexpr (NP > 0) out.pointp(q - 1, u0, A0, i00, cox, a0, a1);
+            else out.point(q - 1, u0, A0, i00, a0, a1);
+        }
+        ud = uc;
+        uc = uu;
+#pragma unroll
+        for (int qq = 0; qq < NPA; ++qq) {
+            co[qq] = ro[qq];
+            cox[qq] = rox[qq];
+        }
+    }
+}
+
+// sources with LDS staging (those that define NA)
+template <class S, class = void>
+struct lds_stageable {
+    static constexpr bool value = false;
+};
+template <class S>
+struct lds_stageable<S, decltype((void)S::NA)> {
+    static constexpr bool value = S::NA >= 1;
+};
+
+template <typename T, class Op, class Src, class Out, int TYD, int P>
+__global__ void __launch_bounds__(TX * TYD)
+k_stencil_lds(int mx, int my, int mz, int kch, Op op, Src src, Out out, double* __restrict__ part) {
+    double a0 = 0.0, a1 = 0.0;
+    const Tile tl = xcd_tile();
+    stencil_lds<T, Op, Src, Out, TYD, P>(tl.bx, tl.by, tl.bz, mx, my, mz, kch, op, src, out, a0, a1);
+    block_partials<Out::NDOT, TYD>(a0, a1, part, tl.id);
 }
 
 // ------------------------------------------- Galerkin coarse operators
@@ -1491,6 +1739,224 @@ k_prolong_smooth(int mx, int my, int mz, int mxc, int myc, int kch, Op op, const
     block_partials<Out::NDOT>(a0, a1, part, tl.id);
 }
 
+// ----------------------------- LDS-staged fused prolongation + smoothing
+//
+// prolong_smooth_3d (one smoothing step applied to t = base + P x_c without
+// storing t) with its inputs staged in LDS, FP64 (two fine points per lane).
+// A block of TYD waves covers the fine rows j0 .. j0+TYD-1 (j0 = TYD by) of
+// W = 128 points (is = 1 + W bx, coarse columns 64 bx .. 64 bx + 65) and
+// marches the planes k0 .. k1-1.  LDS-DMA loads stage each base plane's
+// rows j0-1 .. j0+TYD (and their x-halo values) one plane ahead, and each
+// coarse plane's rows (j0-1)/2 .. (j0+TYD)/2 when the first fine plane that
+// interpolates from it is staged.  At step k the waves form t on plane k+1
+// (rows j0-1 .. j0+TYD, points is-1 .. is+W) into a ring of three t planes,
+// with prolong_smooth_3d's arithmetic (rows mixed, then planes, then x), a
+// barrier makes it visible, and each wave applies the stencil to its row of
+// plane k from t on planes k-1 .. k+1, as stencil_lds does.  Every wave
+// waits for all its loads at each step (s_waitcnt vmcnt(0)), so loads of
+// the next plane overlap one step of arithmetic.  Base slots are refilled
+// two steps after their last read, t and coarse slots likewise, each behind
+// a barrier.  Requires (mx - 1) % W == 0; the stencil's per-point order is
+// prolong_smooth_3d's, the block partition (and so the dot-product
+// summation order) is stencil_lds's.
+template <class Op, bool ZG, class Out, int TYD>
+__device__ __forceinline__ void prolong_smooth_lds(int bx, int by, int bz, int mx, int my, int mz, int mxc, int myc,
+                                                   int kch, const Op& op, const double* __restrict__ xb, double c,
+                                                   const double* __restrict__ xc, const Out& out, double& a0,
+                                                   double& a1) {
+    typedef double T;
+    constexpr int V = 2, W = TX * V, CW = W / 2 + 2, NCR = TYD / 2 + 2;
+    constexpr int SB = 3, ST = 3, SC = 4;
+    constexpr int NP = ZG ? 0 : Out::NPRE, NPA = NP > 0 ? NP : 1;
+    static_assert(!ZG || Out::NPRE == 1, "with ZG the base is the output's b");
+    __shared__ __attribute__((aligned(16))) T bring[SB][TYD + 2][W];     // base rows
+    __shared__ __attribute__((aligned(16))) T bhalo[SB][TYD + 2][2];     //   and x-halo values
+    __shared__ __attribute__((aligned(16))) T tring[ST][TYD + 2][W];     // t rows
+    __shared__ __attribute__((aligned(16))) T thalo[ST][TYD + 2][2];
+    __shared__ __attribute__((aligned(16))) T cring[SC][NCR][CW + 2];    // coarse rows
+    const int lane = threadIdx.x;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.y);
+    const int is = 1 + W * bx, ic = (W / 2) * bx;    // first fine point, first coarse column
+    const int j0 = TYD * by;
+    const int k0 = bz * kch, k1 = min(k0 + kch, mz);
+    if (k0 >= k1) return;                            // uniform per block
+    const int mzc = (mz - 1) / 2 + 1;
+    const int klast = min(k1, mz - 1);               // last plane t is formed on
+    const int Jlo = max(0, (j0 - 1) >> 1);           // first staged coarse row
+    const long sm = (long)mx * my, smc = (long)mxc * myc;
+    const bool has_e = is + W < mx;                  // fine is+W exists (then so does coarse ic+65)
+    const T h = T(0.5), z0 = T(0);
+    // base plane kk: rows w and w + TYD of this wave (rows j0 - 1 + r)
+    auto stage_base_row = [&](int kk, int r) {
+        const int jr = j0 - 1 + r;
+        if (jr < 0 || jr >= my) return;
+        const int slot = (kk + 3) % SB;
+        const T* g = xb + (long)kk * sm + (long)jr * mx;
+        __builtin_amdgcn_global_load_lds((const void*)(g + is + V * lane), (lds_void_t*)&bring[slot][r][0], 16, 0, 0);
+        if (lane < 4) {
+            const bool east = lane >= 2;
+            if (!east || has_e)
+                __builtin_amdgcn_global_load_lds((const void*)((const char*)(g + (east ? is + W : is - 1)) + 4 * (lane & 1)),
+                                                 (lds_void_t*)&bhalo[slot][r][0], 4, 0, 0);
+        }
+    };
+    // coarse plane K: row Jlo + w of this wave (w < NCR), columns ic .. ic+65
+    auto stage_coarse = [&](int K) {
+        if (K < 0 || K >= mzc || w >= NCR) return;
+        const int J = Jlo + w;
+        if (J >= myc) return;
+        const T* g = xc + (long)K * smc + (long)J * mxc + ic;
+        if (lane < TX / 2)
+            __builtin_amdgcn_global_load_lds((const void*)(g + 2 * lane), (lds_void_t*)&cring[K % SC][w][0], 16, 0, 0);
+        if (lane < 4) {   // columns ic+64 (lanes 0, 1) and ic+65 (lanes 2, 3; in range when fine is+W is)
+            if (lane < 2 || has_e)
+                __builtin_amdgcn_global_load_lds((const void*)((const char*)(g + W / 2 + (lane >> 1)) + 4 * (lane & 1)),
+                                                 (lds_void_t*)&cring[K % SC][w][W / 2], 4, 0, 0);
+        }
+    };
+    int kst = -1;   // highest coarse plane staged (uniform)
+    auto stage = [&](int kk) {
+        if (kk < 0 || kk > klast) return;
+        stage_base_row(kk, w);
+        if (w + TYD < TYD + 2) stage_base_row(kk, w + TYD);
+        const int kn = (kk >> 1) + (kk & 1);   // the coarse planes kk interpolates from: kk/2 .. kn
+        if (kst < (kk >> 1) - 1) kst = (kk >> 1) - 1;
+        while (kst < kn) stage_coarse(++kst);
+    };
+    // t on plane kk (rows j0-1 .. j0+TYD) into its ring slot: rows w, w + TYD
+    auto form_row = [&](int kk, int r) {
+        const int j = j0 - 1 + r;
+        if (j < 0 || j >= my) return;
+        const int bs = (kk + 3) % SB, ts = (kk + 3) % ST;
+        const int Ka = kk >> 1, Kb = Ka + (kk & 1);
+        const int ra = (j >> 1) - Jlo, rb = ra + (j & 1);
+        // P x_c at local coarse column cl of this row and plane (rows, then planes)
+        auto pc = [&](int cl) -> T {
+            const T* ca = &cring[Ka % SC][0][0];
+            const T* cb = &cring[Kb % SC][0][0];
+            T ya = ca[ra * (CW + 2) + cl], yb;
+            if (j & 1) ya = h * (ya + ca[rb * (CW + 2) + cl]);
+            if (kk & 1) {
+                yb = cb[ra * (CW + 2) + cl];
+                if (j & 1) yb = h * (yb + cb[rb * (CW + 2) + cl]);
+                ya = h * (ya + yb);
+            }
+            return ya;
+        };
+        const T pw = pc(lane), pv = pc(lane + 1);        // coarse I-1, I of the lane's pair (2I-1, 2I)
+        const T2<T> bb = *(const T2<T>*)&bring[bs][r][V * lane];
+        const T b0 = ZG ? c * bb.a : bb.a, b1 = ZG ? c * bb.b : bb.b;
+        *(T2<T>*)&tring[ts][r][V * lane] = T2<T>{b0 + h * (pw + pv), b1 + pv};
+        if (lane == 0) {                                 // fine is-1 (even): coarse ic
+            const T x = bhalo[bs][r][0];
+            thalo[ts][r][0] = (ZG ? c * x : x) + pw;
+        }
+        if (lane == TX - 1 && has_e) {                   // fine is+W (odd): coarse ic+64, ic+65
+            const T x = bhalo[bs][r][1];
+            thalo[ts][r][1] = (ZG ? c * x : x) + h * (pv + pc(lane + 2));
+        }
+    };
+    const int j = j0 + w;
+    const bool rowok = j < my;                       // uniform per wave
+    const bool jb = j == 0 || j == my - 1;
+    const bool ms = j - 1 > 0, mn = j + 1 < my - 1;
+    const int r = w + 1;
+    const int i0 = is + V * lane;
+    TV<T, V> zv;
+    zv.v[0] = zv.v[1] = z0;
+    TV<T, V> co[NPA], ro[NPA];
+    T cox[NPA], rox[NPA];
+#pragma unroll
+    for (int q = 0; q < NPA; ++q) {
+        co[q] = ro[q] = zv;
+        cox[q] = rox[q] = z0;
+    }
+    auto pre_own = [&](long qq, TV<T, V>* o, T* ox) {
+        if constexpr (NP > 0) {
+            out.template pre<V>(qq, o);
+            if (i0 == 1) out.pre1(qq - 1, ox);
+        }
+    };
+    auto tv = [&](int ts, int rr) -> TV<T, V> { return ldv<T, V>(&tring[ts][rr][V * lane]); };
+    long q = (long)k0 * sm + (long)j * mx + i0;
+    if (rowok) pre_own(q, co, cox);
+    TV<T, V> ud = zv, uc = zv;
+    stage(k0 - 1);
+    for (int k = k0 - 2; k < k1; ++k) {
+        wait_vmcnt(0);
+        lds_barrier();
+        stage(k + 2);
+        if (k + 1 >= 0 && k + 1 >= k0 - 1 && k + 1 <= klast) {
+            form_row(k + 1, w);
+            if (w + TYD < TYD + 2) form_row(k + 1, w + TYD);
+        }
+        lds_barrier();
+        if (k < k0 || !rowok) continue;             // uniform per wave (no barrier skipped)
+        if (k + 1 < k1) pre_own(q + sm, ro, rox);
+        const int td = (k + 2) % ST, tc = (k + 3) % ST, tu = (k + 4) % ST;
+        if (k == k0) {
+            uc = tv(tc, r);
+            if (k > 0) ud = tv(td, r);
+        }
+        const TV<T, V> uu = k + 1 < mz ? tv(tu, r) : zv;
+        TV<T, V> Au = zv, idg = zv;
+        if (jb || k == 0 || k == mz - 1) {
+            op.bnd(i0, j, k, uc.v[0], Au.v[0], idg.v[0]);
+            op.bnd(i0 + 1, j, k, uc.v[1], Au.v[1], idg.v[1]);
+        } else {
+            const TV<T, V> sv = ms ? tv(tc, r - 1) : zv, nv = mn ? tv(tc, r + 1) : zv;
+            const TV<T, V> dn = k - 1 > 0 ? ud : zv, up = k + 1 < mz - 1 ? uu : zv;
+            const T wv = lane > 0 ? tring[tc][r][V * lane - 1] : thalo[tc][r][0];
+            const T ev = lane < TX - 1 ? tring[tc][r][V * lane + V] : thalo[tc][r][1];
+            op.interior(i0, j, k, uc.v[0], i0 - 1 > 0 ? wv : z0, i0 + 1 < mx - 1 ? uc.v[1] : z0, sv.v[0], nv.v[0],
+                        dn.v[0], up.v[0], Au.v[0], idg.v[0]);
+            if (i0 + 1 == mx - 1) op.bnd(i0 + 1, j, k, uc.v[1], Au.v[1], idg.v[1]);
+            else
+                op.interior(i0 + 1, j, k, uc.v[1], uc.v[0], i0 + 2 < mx - 1 ? ev : z0, sv.v[1], nv.v[1], dn.v[1],
+                            up.v[1], Au.v[1], idg.v[1]);
+        }
+        if constexpr (ZG) {
+            const TV<T, V> bb = ldv<T, V>(&bring[(k + 3) % SB][r][V * lane]);
+            out.template pointvp<V>(q, uc, Au, idg, &bb, a0, a1);
+        } else if constexpr (NP > 0) {
+            out.template pointvp<V>(q, uc, Au, idg, co, a0, a1);
+        } else {
+            out.template pointv<V>(q, uc, Au, idg, a0, a1);
+        }
+        if (i0 == 1) {                               // the boundary point i = 0
+            const T t0 = thalo[tc][r][0];
+            T A0, i00;
+            op.bnd(0, j, k, t0, A0, i00);
+            if constexpr (ZG) {
+                const T x0 = bhalo[(k + 3) % SB][r][0];
+                out.pointp(q - 1, t0, A0, i00, &x0, a0, a1);
+            } else if constexpr (NP > 0) {
+                out.pointp(q - 1, t0, A0, i00, cox, a0, a1);
+            } else {
+                out.point(q - 1, t0, A0, i00, a0, a1);
+            }
+        }
+        ud = uc;
+        uc = uu;
+#pragma unroll
+        for (int qq = 0; qq < NPA; ++qq) {
+            co[qq] = ro[qq];
+            cox[qq] = rox[qq];
+        }
+        q += sm;
+    }
+}
+
+template <class Op, bool ZG, class Out, int TYD>
+__global__ void __launch_bounds__(TX * TYD)
+k_prolong_smooth_lds(int mx, int my, int mz, int mxc, int myc, int kch, Op op, const double* __restrict__ xb,
+                     double c, const double* __restrict__ xc, Out out, double* __restrict__ part) {
+    double a0 = 0.0, a1 = 0.0;
+    const Tile tl = xcd_tile();
+    prolong_smooth_lds<Op, ZG, Out, TYD>(tl.bx, tl.by, tl.bz, mx, my, mz, mxc, myc, kch, op, xb, c, xc, out, a0, a1);
+    block_partials<Out::NDOT, TYD>(a0, a1, part, tl.id);
+}
+
 // ------------------------------------------------------- coarse solve
 
 // x = Ainv b, Ainv = A_0^-1 (n x n, row major, FP64): the coarsest-grid
@@ -2007,7 +2473,12 @@ k_ilu27_plane(SorOp o, const double* __restrict__ u, int p, int backward, const 
 // c = 9..12 of a_c x_c) / a_13; backward the same over c = 9..12, 14..17,
 // the lower neighbours holding forward values; a null row stays zero.
 // flag: per strip, base + points done (bases grow from sweep to sweep); a
-// wait longer than 2^24 polls sets *err and ends the waiting (no hang).
+// wait longer than spin_limit polls sets *err and ends the waiting (no
+// hang), and a launch that finds *err already set waits for nothing: after
+// a stall every later sweep runs through at once until the host, which
+// checks *err after each linear solve, stops the solve.  need_extra is added
+// to every wait's target (0; the host's stall test sets a target no strip
+// reaches, so that every wait gives up after spin_limit polls).
 constexpr int SOR9_SLACK = 64;
 
 // lane i receives lane i-1's value (DPP wave_shr:1; lane 0 keeps its own)
@@ -2047,7 +2518,8 @@ constexpr int SOR9_AHEAD = 4;
 
 __global__ void __launch_bounds__(64)
 k_sor9_strips(int mx, int my, const double* __restrict__ pk, const double* __restrict__ b, double* x,
-              unsigned long long* flag, unsigned long long base, int backward, int* err) {
+              unsigned long long* flag, unsigned long long base, int backward, int* err, long spin_limit,
+              unsigned long long need_extra) {
     const int s = blockIdx.x, r = threadIdx.x;
     const int row0 = 64 * s;
     const int rows = min(64, my - row0);
@@ -2061,7 +2533,7 @@ k_sor9_strips(int mx, int my, const double* __restrict__ pk, const double* __res
     __shared__ double ring[256];
     const int jn = backward ? my - row0 : row0 - 1;   // lane 0's neighbour row (s > 0), the same for all lanes
     int copied = 0;                                    // positions in the ring (wave-uniform)
-    bool dead = false;
+    bool dead = __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
     double h1 = 0.0, h2 = 0.0, h3 = 0.0, xo = 0.0;
     Sor9Ops pf[SOR9_AHEAD];   // a ring of prefetched operands, indexed by step mod SOR9_AHEAD
     auto fetch = [&](int t, Sor9Ops& o) {
@@ -2085,10 +2557,11 @@ k_sor9_strips(int mx, int my, const double* __restrict__ pk, const double* __res
         if (s > 0 && t < mx && copied < min(t + 2, mx)) {   // uniform
             unsigned long long kn = 0;
             if (r == 0) {
-                const unsigned long long need = base + (unsigned long long)min(t == 0 ? SOR9_SLACK : t + 2, mx);
+                const unsigned long long need =
+                    base + (unsigned long long)min(t == 0 ? SOR9_SLACK : t + 2, mx) + need_extra;
                 long spins = 0;
                 while (!dead && (kn = __hip_atomic_load(flag + s - 1, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT)) < need) {
-                    if (++spins > (1L << 24)) {
+                    if (++spins > spin_limit) {
                         atomicExch(err, 1);
                         dead = true;
                     }

@@ -374,6 +374,7 @@ static int ms_solve(p4g_solver_s* s, double q, const double* gb, double* x, p4g_
     out.fnorm0 = fnorm;
     s->ms_fhist.push_back(fnorm);
     P4G_TRY(monitor(0, fnorm));
+    P4G_TRY(sor_pipes_reset(s));
     int reason = vi_converged(s, 0, 0.0, 0.0, fnorm, ttol, rnorm0);
     int i = 0, iter = 0;
     for (; !reason && i < s->snes_maxit; ++i) {
@@ -389,6 +390,7 @@ static int ms_solve(p4g_solver_s* s, double q, const double* gb, double* x, p4g_
         else
             P4G_TRY(s->e64->cg(R.F, R.Y, s->normtype, s->rtol, s->atol, s->dtol, s->maxit, co, s->cx.S));
         HIP_TRY(hipStreamSynchronize(R.st));
+        P4G_TRY(sor_pipes_check(s, "p4g_ms_solve"));
         s->ms_kits.push_back(co.its);
         s->ms_kreason.push_back(co.reason);
         s->hist = co.hist;
@@ -408,12 +410,6 @@ static int ms_solve(p4g_solver_s* s, double q, const double* gb, double* x, p4g_
     }
     if (!reason && i >= s->snes_maxit) reason = -5;   // DIVERGED_MAX_IT
     HIP_TRY(hipMemcpy(x, R.X, n * sizeof(double), hipMemcpyDeviceToHost));
-    for (const SorPipe& P : s->ms_pipes) {
-        if (!P.err) continue;
-        int e = 0;
-        HIP_TRY(hipMemcpy(&e, P.err, sizeof(int), hipMemcpyDeviceToHost));
-        if (e) return set_error(PETSC_ERR_LIB, "p4g_ms_solve: a pipelined SOR sweep stalled");
-    }
     s->cx.tm.collect();
     out.its = iter;
     out.reason = reason;

@@ -77,6 +77,8 @@ struct SorPipe {
     unsigned long long* flags = nullptr;
     int* err = nullptr;
     unsigned long long epoch = 0;
+    long spin_limit = 1L << 24;           // polls before a wait gives up
+    unsigned long long need_extra = 0;    // -p4g_sor_test_stall: a target no strip reaches
 };
 
 struct LevelInfo {
@@ -104,9 +106,11 @@ static int sor_run(const LevelInfo& L, const double* g27, const T* r, T* z, hipS
             const int ns = (L.my + 63) / 64;
             SorPipe* P = L.pipe;
             const unsigned long long b0 = (++P->epoch) << 22;
-            k_sor9_strips<<<ns, 64, 0, st>>>(L.mx, L.my, P->pk, r, z, P->flags, b0, 0, P->err);
+            k_sor9_strips<<<ns, 64, 0, st>>>(L.mx, L.my, P->pk, r, z, P->flags, b0, 0, P->err, P->spin_limit,
+                                             P->need_extra);
             const unsigned long long b1 = (++P->epoch) << 22;
-            k_sor9_strips<<<ns, 64, 0, st>>>(L.mx, L.my, P->pk, r, z, P->flags, b1, 1, P->err);
+            k_sor9_strips<<<ns, 64, 0, st>>>(L.mx, L.my, P->pk, r, z, P->flags, b1, 1, P->err, P->spin_limit,
+                                             P->need_extra);
             LAUNCH_CHECK();
             return 0;
         }
@@ -388,6 +392,10 @@ struct Engine {
     // first kernel of the cycle (fuse_r), the direction update with the
     // operator apply (fuse_dir)
     bool fuse_r = true, fuse_dir = true;
+    // 3D constant-coefficient passes with the source staged in LDS
+    // (k_stencil_lds): -1 on levels of 400 or more points per side, 1 on
+    // every level whose rows fill the waves, 0 never (-p4g_lds)
+    int lds = -1;
     // per level: 27-point Galerkin coefficients (FP64, device) or null
     std::vector<const double*> g27;
     bool fmg = false;          // full multigrid: an F-cycle per apply
@@ -516,6 +524,16 @@ struct Engine {
         return o;
     }
 
+    // whether a 3D constant-coefficient pass on level L runs k_stencil_lds:
+    // the rows fill whole waves ((mx - 1) a multiple of 64 V) and -p4g_lds
+    // asks for it (auto: levels of 400 or more points per side, where the
+    // operand misses the last-level cache between passes)
+    bool lds_level(const LevelInfo& L) const {
+        constexpr int W = TX * VecOf<T>::V;
+        if (L.dim != 3 || L.var || L.gal || lds == 0 || (L.mx - 1) % W != 0) return false;
+        return lds == 1 || L.mz >= 400;
+    }
+
     // one marching stencil pass on level l
     template <class Src, class Out>
     int stencil(int l, Src src, Out out, int cls, double passes, long* nb, bool jac, double* pbuf = nullptr) {
@@ -547,6 +565,14 @@ struct Engine {
             });
         }
         OpConst<T> o = opc(L, jac);
+        if constexpr (lds_stageable<Src>::value) {
+            if (lds_level(L)) {   // the same grid and block as k_stencil (TYD = TY)
+                return cx->launch(cls, fine, bytes, [&] {
+                    k_stencil_lds<T, OpConst<T>, Src, Out, TY, 1><<<g, b, 0, s>>>(
+                        L.mx, L.my, L.mz, kchf(L.dim, L.my, L.mz), o, src, out, part);
+                });
+            }
+        }
         return cx->launch(cls, fine, bytes, [&] {
             if (L.dim == 3)
                 k_stencil<T, 3, OpConst<T>, Src, Out><<<g, b, 0, s>>>(L.mx, L.my, L.mz, kchf(L.dim, L.my, L.mz), o, src,
@@ -830,6 +856,8 @@ struct Engine {
         };
         for (int m = mstart; m <= nu; ++m) {
             const bool last = (m == nu);
+            // kernel class: the cycle's last step (with its dot products) apart
+            const int scls = (last && dots) ? P4G_K_SMOOTH_LAST : P4G_K_SMOOTH;
             T* dst = last ? out : pick(pm1, pm2);
             if (!dst) return set_error(PETSC_ERR_LIB, "internal: smoother buffers exhausted");
             double c0 = 0.0, c1 = 1.0, c2 = s;
@@ -854,10 +882,10 @@ struct Engine {
                     // y = c2 D^-1 b
                     if (last && dots)
                         P4G_TRY(stencil(l, SrcZero<T>{}, OutCheb<T, false, 2>{b, nullptr, dst, T(0), T(0), T(c2)},
-                                        P4G_K_SMOOTH, 2, &nbk, jac));
+                                        scls, 2, &nbk, jac));
                     else
                         P4G_TRY(stencil(l, SrcZero<T>{}, OutCheb<T, false, 0>{b, nullptr, dst, T(0), T(0), T(c2)},
-                                        P4G_K_SMOOTH, 2, &nbk, jac));
+                                        scls, 2, &nbk, jac));
                 } else {
                     P4G_TRY(scale(L.n, T(s * idg), b, dst, last && dots, fine));
                     nbk = vblocks(L.n);
@@ -880,35 +908,35 @@ struct Engine {
                     const OC oc{cb, OutCheb<T, false, 0>{b, nullptr, dst, T(c0), T(c1), T(c2)}};
                     pend.on = false;
                     if (pend.rr)
-                        P4G_TRY(stencil(l, su, OutRR<T, OC>{oc}, P4G_K_SMOOTH, 4, &pend.nparts, jac, cx->part_rr));
+                        P4G_TRY(stencil(l, su, OutRR<T, OC>{oc}, P4G_K_SMOOTH_ZG, 4, &pend.nparts, jac, cx->part_rr));
                     else
-                        P4G_TRY(stencil(l, su, oc, P4G_K_SMOOTH, 4, &pend.nparts, jac));
+                        P4G_TRY(stencil(l, su, oc, P4G_K_SMOOTH_ZG, 4, &pend.nparts, jac));
                 } else if (last && dots) {
                     P4G_TRY(pend_flush());
                     typedef OutScaleU<T, OutCheb<T, false, 2>> OC;
                     P4G_TRY(stencil(l, SrcArr<T>{b}, OC{cb, OutCheb<T, false, 2>{b, nullptr, dst, T(c0), T(c1), T(c2)}},
-                                    P4G_K_SMOOTH, 2, &nbk, jac));
+                                    scls, 2, &nbk, jac));
                 } else {
                     if (fine) P4G_TRY(pend_flush());
                     typedef OutScaleU<T, OutCheb<T, false, 0>> OC;
                     P4G_TRY(stencil(l, SrcArr<T>{b}, OC{cb, OutCheb<T, false, 0>{b, nullptr, dst, T(c0), T(c1), T(c2)}},
-                                    P4G_K_SMOOTH, 2, &nbk, jac));
+                                    scls, 2, &nbk, jac));
                 }
                 pm1_implicit = false;
             } else if (prev) {
                 if (last && dots)
                     P4G_TRY(stencil(l, SrcArr<T>{pm1}, OutCheb<T, true, 2>{b, pm2, dst, T(c0), T(c1), T(c2)},
-                                    P4G_K_SMOOTH, 4, &nbk, jac));
+                                    scls, 4, &nbk, jac));
                 else
                     P4G_TRY(stencil(l, SrcArr<T>{pm1}, OutCheb<T, true, 0>{b, pm2, dst, T(c0), T(c1), T(c2)},
-                                    P4G_K_SMOOTH, 4, &nbk, jac));
+                                    scls, 4, &nbk, jac));
             } else {
                 if (last && dots)
                     P4G_TRY(stencil(l, SrcArr<T>{pm1}, OutCheb<T, false, 2>{b, nullptr, dst, T(c0), T(c1), T(c2)},
-                                    P4G_K_SMOOTH, 3, &nbk, jac));
+                                    scls, 3, &nbk, jac));
                 else
                     P4G_TRY(stencil(l, SrcArr<T>{pm1}, OutCheb<T, false, 0>{b, nullptr, dst, T(c0), T(c1), T(c2)},
-                                    P4G_K_SMOOTH, 3, &nbk, jac));
+                                    scls, 3, &nbk, jac));
             }
             if (nb) *nb = nbk;
             pm2 = pm1;
@@ -1565,6 +1593,8 @@ struct p4g_solver_s {
     std::vector<double*> ilu_u;       // per level: ILU(0) pivots (device) or null
     int coarse_its = 0, coarse_pc = 0;   // -mg_coarse_ksp_type cg: iterations and PC (0 none, 1 jacobi, 2 ilu)
     int sor_blocks = 1;               // -p4g_sor_blocks: local sweeps of that many ranks
+    int lds = -1;                     // -p4g_lds auto|1|0: LDS-staged 3D stencil passes
+    bool sor_test_stall = false;      // -p4g_sor_test_stall: every pipelined-SOR wait gives up at once
     // reduced-space VI Newton (p4g_vi_solve): SNES options and state
     double snes_rtol = 1e-8, snes_atol = 1e-50, snes_stol = 1e-8, snes_divtol = 1e4;
     int snes_maxit = 50;
@@ -1977,6 +2007,7 @@ static int configure(p4g_solver_s* s) {
             o.geti("mg_levels_pc_sor_lits", 1) != 1)
             return set_error(PETSC_ERR_SUP, "the device SOR smoother has omega 1 and one sweep");
         s->sor_blocks = o.geti("p4g_sor_blocks", 1);
+        s->sor_test_stall = o.getb("p4g_sor_test_stall", false);
         if (s->sor_blocks < 1 || s->sor_blocks > 16)
             return set_error(PETSC_ERR_ARG_OUTOFRANGE, "-p4g_sor_blocks must be in 1..16");
     }
@@ -2018,6 +2049,7 @@ static int configure(p4g_solver_s* s) {
             else return set_error(PETSC_ERR_ARG_OUTOFRANGE, "-%s must be auto, 1 or 0", name);
             return 0;
         };
+        P4G_TRY(tri(o.gets("p4g_lds", "auto"), "p4g_lds", s->lds));
         const std::string both = o.gets("p4g_fuse_cg", "auto");
         P4G_TRY(tri(both, "p4g_fuse_cg", s->fuse_r));
         P4G_TRY(tri(o.gets("p4g_fuse_r", both), "p4g_fuse_r", s->fuse_r));
@@ -2262,6 +2294,7 @@ int solver_setup(p4g_solver_s* s) {
         e->coarse_its = s->coarse_its;
         e->coarse_pc = s->coarse_pc;
         e->ilu_u.assign(s->ilu_u.begin(), s->ilu_u.end());
+        e->lds = s->lds;
     };
     if (F.var) {
         const size_t nc = F.kcell.size();
@@ -2989,6 +3022,33 @@ const char* p4g_version(void) { return "p4g 0.1 (gfx950)"; }
 }  // extern "C"
 
 // continuous-batching server (uses the types above)
+namespace p4g {
+// The stall words of the pipelined SOR sweeps (k_sor9_strips): cleared at the
+// start of a solve, read after each linear solve, where the solve has just
+// synchronized its stream, so a stall stops the solve at the Newton step it
+// happened in.
+static int sor_pipes_reset(p4g_solver_s* s) {
+    for (SorPipe& P : s->ms_pipes) {
+        if (!P.err) continue;
+        P.spin_limit = s->sor_test_stall ? 64 : (1L << 24);
+        P.need_extra = s->sor_test_stall ? (1ull << 21) : 0;
+        HIP_TRY(hipMemsetAsync(P.err, 0, sizeof(int), s->cx.stream));
+    }
+    return 0;
+}
+
+static int sor_pipes_check(p4g_solver_s* s, const char* who) {
+    for (const SorPipe& P : s->ms_pipes) {
+        if (!P.err) continue;
+        int e = 0;
+        HIP_TRY(hipMemcpyAsync(&e, P.err, sizeof(int), hipMemcpyDeviceToHost, s->cx.stream));
+        HIP_TRY(hipStreamSynchronize(s->cx.stream));
+        if (e) return set_error(PETSC_ERR_LIB, "%s: a pipelined SOR sweep stalled", who);
+    }
+    return 0;
+}
+}  // namespace p4g
+
 #include "p4g_serve_impl.hpp"
 // reduced-space VI Newton (SNESVINEWTONRSLS) over the solver's levels
 #include "p4g_vi_impl.hpp"

@@ -276,6 +276,7 @@ static int vi_solve(p4g_solver_s* s, const double* xl, const double* xu, double*
     std::memset(&out, 0, sizeof out);
     out.fnorm0 = fnorm;
     s->vi_fhist.push_back(fnorm);
+    P4G_TRY(sor_pipes_reset(s));
     int reason = vi_converged(s, 0, 0.0, 0.0, fnorm, ttol, rnorm0);
     int i = 0, iter = 0;
     for (; !reason && i < s->snes_maxit; ++i) {
@@ -289,6 +290,7 @@ static int vi_solve(p4g_solver_s* s, const double* xl, const double* xu, double*
         else
             P4G_TRY(s->e64->cg(R.Fm, R.Y, s->normtype, s->rtol, s->atol, s->dtol, s->maxit, co, s->cx.S));
         HIP_TRY(hipStreamSynchronize(R.st));
+        P4G_TRY(sor_pipes_check(s, "p4g_vi_solve"));
         s->vi_kits.push_back(co.its);
         s->vi_kreason.push_back(co.reason);
         s->hist = co.hist;
@@ -317,12 +319,6 @@ static int vi_solve(p4g_solver_s* s, const double* xl, const double* xu, double*
     }
     HIP_TRY(hipMemcpy(x, R.X, n * sizeof(double), hipMemcpyDeviceToHost));
     if (Fout) HIP_TRY(hipMemcpy(Fout, R.F, n * sizeof(double), hipMemcpyDeviceToHost));
-    for (const SorPipe& P : s->ms_pipes) {
-        if (!P.err) continue;
-        int e = 0;
-        HIP_TRY(hipMemcpy(&e, P.err, sizeof(int), hipMemcpyDeviceToHost));
-        if (e) return set_error(PETSC_ERR_LIB, "p4g_vi_solve: a pipelined SOR sweep stalled");
-    }
     s->cx.tm.collect();
     out.its = iter;
     out.reason = reason;

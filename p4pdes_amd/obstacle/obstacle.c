@@ -164,6 +164,8 @@ int obstacle_run(int argc, const char **argv, obstacle_result *res, double *u_ou
     const int kreason = opt_get(&a, "ksp_converged_reason") != NULL;
     const int sreason = opt_get(&a, "snes_converged_reason") != NULL;
     const char *sv = opt_get(&a, "mat_is_symmetric");
+    const char *dump = opt_get(&a, "obs_dump_binary");   /* default name as obstacle.c:77 */
+    if (dump && !*dump) dump = "dump.dat";
     char symtol[32] = "";
     if (sv) petsc_g(symtol, sizeof symtol, *sv ? atof(sv) : 0.0);
     PoissonCtx user;   /* obstacle.c:97-102 */
@@ -246,7 +248,12 @@ int obstacle_run(int argc, const char **argv, obstacle_result *res, double *u_ou
                 }
                 res->setup_ms += rep.setup_ms;
                 res->solve_ms += rep.solve_ms;
-                if (stage == seq) {   /* report (obstacle.c:160-198) */
+                if (stage == seq && dump) {   /* -obs_dump_binary: VecView of u, then of psi (obstacle.c:137-146) */
+                    outf(&o, "writing u,psi in binary format to %s ...\n", dump);
+                    e = p4g_vec_view_binary(dump, n, u, 0);
+                    if (!e) e = p4g_vec_view_binary(dump, n, xl, 1);
+                }
+                if (stage == seq && !e) {   /* report (obstacle.c:160-198) */
                     res->mx = mx; res->my = my;
                     res->its = rep.its; res->reason = rep.reason; res->last_ksp_its = rep.last_ksp_its;
                     res->ksp_its_total = rep.ksp_its_total; res->fnorm0 = rep.fnorm0; res->fnorm = rep.fnorm;

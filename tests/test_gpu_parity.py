@@ -98,6 +98,12 @@ CASES = {
                       "-ksp_rtol 1e-10" % MG,
     "3d_kaskade_cg": "-fsh_dim 3 -fsh_problem manusin -da_refine 4 %s -pc_mg_type kaskade "
                      "-mg_levels_ksp_max_it 4 -ksp_rtol 1e-10" % MG,
+    # the LDS-staged 3D passes (k_stencil_lds) forced on where rows fill the
+    # waves (FP64: 129^3), with and without the fused CG updates
+    "3d_lds_cheb2_129": ("-fsh_dim 3 -fsh_problem manusin -fsh_noise_sigma 1e-3 -da_refine 6 %s "
+                         "-ksp_norm_type unpreconditioned -ksp_rtol 1e-10 -p4g_lds 1" % MG),
+    "3d_lds_cheb1_fused_cg_129": ("-fsh_dim 3 -fsh_problem manusin -da_refine 6 %s -mg_levels_ksp_max_it 1 "
+                                  "-ksp_rtol 1e-10 -p4g_fuse_cg 1 -p4g_lds 1" % MG),
     "3d_kaskade_outer_richardson": ("-fsh_dim 3 -fsh_problem manusin -da_refine 4 %s -pc_mg_type kaskade "
                                     "-mg_levels_ksp_max_it 4 -ksp_type richardson -ksp_monitor -ksp_rtol 1e-10" % MG),
 }
@@ -475,3 +481,29 @@ def test_c4_varcoef_257_mixed_and_fp64():
         assert abs(fm.errinf - f64.errinf) < 1e-4 * f64.errinf
     ratio = res[(6, "-ksp_norm_type unpreconditioned")][0].errinf / res[(7, "-ksp_norm_type unpreconditioned")][0].errinf
     assert 3.0 < ratio < 5.0, ratio
+
+
+LDS_PAIRS = {
+    "fp64_129_cheb2_fused": ("-fsh_dim 3 -fsh_problem manusin -fsh_noise_sigma 1e-3 -da_refine 6 %s "
+                             "-ksp_norm_type unpreconditioned -ksp_rtol 1e-10 -p4g_fuse_cg 1" % MG),
+    "fp64_129_cheb1": ("-fsh_dim 3 -fsh_problem manusin -da_refine 6 %s -mg_levels_ksp_max_it 1 "
+                       "-ksp_rtol 1e-10" % MG),
+    "mixed_257_cheb2": ("-fsh_dim 3 -fsh_problem manusin -fsh_noise_sigma 1e-3 -da_refine 7 %s "
+                        "-ksp_rtol 1e-10 -p4g_precision mixed" % MG),
+    "mixed_257_cheb1_fused": ("-fsh_dim 3 -fsh_problem manusin -da_refine 7 %s -mg_levels_ksp_max_it 1 "
+                              "-ksp_rtol 1e-10 -p4g_precision mixed -p4g_fuse_cg 1" % MG),
+}
+
+
+@pytest.mark.parametrize("name", sorted(LDS_PAIRS))
+def test_lds_passes_match_register_passes(name):
+    # k_stencil_lds forms the same operand with the same operation order as
+    # k_stencil, and its blocks sum their dot products in the same order:
+    # the solve is the same bit for bit with the LDS staging on or off
+    args = LDS_PAIRS[name]
+    print("lds pair", name, args, flush=True)
+    rc0, t0, r0, u0 = capi.fish_run(args + " -p4g_lds 0", want_u=True)
+    rc1, t1, r1, u1 = capi.fish_run(args + " -p4g_lds 1", want_u=True)
+    assert rc0 == 0 and rc1 == 0, (t0, t1)
+    assert r0.its == r1.its and r0.reason > 0
+    assert np.array_equal(u0, u1), float(np.max(np.abs(u0 - u1)))

@@ -196,6 +196,10 @@ CASES = {
     "seq5_to_513": "-da_refine 3 -snes_grid_sequence 5 -pc_type mg -mg_levels_pc_type jacobi",
     "129_max_it": "-da_refine 6 -pc_type mg -mg_levels_pc_type jacobi -snes_max_it 5",
     "65_sor": "-da_refine 5 -pc_type mg",
+    # pipelined strip SOR on the VI Galerkin levels: two full 64-row strips
+    # and a one-row strip at 129^2; a grid sequence ending at 257^2
+    "129_sor": "-da_refine 6 -pc_type mg",
+    "seq5_to_257_sor": "-da_refine 2 -snes_grid_sequence 5 -pc_type mg",
     "65_ilu": "-da_refine 5 -ksp_rtol 1e-8",
     "seq3_ilu": "-snes_grid_sequence 3",
     "seq4_sor_w": "-snes_grid_sequence 4 -pc_type mg -pc_mg_cycle_type w",
@@ -291,3 +295,41 @@ def test_obstacle_device_golden_lines(num):
         # count, the report and the errors do not
         got, want = got[-3:], want[-3:]
     assert got == want
+
+
+@pytest.mark.gpu
+def test_obstacle_sor_stall_stops_the_solve():
+    # A pipelined SOR sweep whose waits cannot succeed (the host gives every
+    # wait a target no strip reaches and a 64-poll limit, so every kernel
+    # still exits): the solve stops after its first linear solve with
+    # PETSC_ERR_LIB, and the next solve on a fresh solver runs normally.
+    from p4pdes_amd import capi
+    print("obstacle stall test", flush=True)
+    rc, text, _, _ = capi.obstacle_run("-da_refine 5 -pc_type mg -p4g_sor_test_stall 1")
+    assert rc == 76, text
+    assert "pipelined SOR sweep stalled" in text
+    rc, text, res, _ = capi.obstacle_run("-da_refine 5 -pc_type mg")
+    assert rc == 0, text
+
+
+@pytest.mark.gpu
+def test_obstacle_dump_binary(tmp_path):
+    # -obs_dump_binary (c/ch12/obstacle.c:80-83,137-146): the solution, then
+    # the obstacle psi, as two PETSc binary Vecs in one file, announced by the
+    # reference's line before the report
+    from p4pdes_amd import capi
+    path = tmp_path / "dump.dat"
+    args = "-da_refine 3 -pc_type mg -mg_levels_pc_type jacobi -obs_dump_binary %s" % path
+    print("obstacle dump", args, flush=True)
+    rc, text, res, u = capi.obstacle_run(args, want_u=True)
+    assert rc == 0, text
+    lines = text.splitlines()
+    assert lines[-3] == "writing u,psi in binary format to %s ..." % path
+    assert lines[-2].startswith("done on 17 x 17 grid")
+    ud = capi.vec_load_binary(path, 0)
+    pd = capi.vec_load_binary(path, 1)
+    assert np.array_equal(ud, u)
+    xs = -2.0 + 4.0 * np.arange(17) / 16
+    X, Y = np.meshgrid(xs, xs)
+    assert np.allclose(pd, psi(X, Y).ravel(), rtol=0, atol=1e-15)
+    assert np.all(ud >= pd - 1e-12)
