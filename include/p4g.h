@@ -155,6 +155,17 @@ int p4g_ksp_solve_device(p4g_Solver s, const double *d_b, double *d_x, p4g_Repor
 /* PCApply: z = B r, one multigrid cycle (PCShellSetApply-compatible). */
 int p4g_pc_apply(p4g_Solver s, const double *r, double *z);
 int p4g_pc_apply_device(p4g_Solver s, const double *d_r, double *d_z);
+/* p4g_pc_apply_device without any allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory at the call
+ * (code under HIP graph capture, tight serving loops).  The same cycle, its
+ * work arrays (every level's vectors and the block partial sums) carved from
+ * one 128 MiB device array the library holds on each GPU until the process
+ * ends: calls on one GPU share it, so calls that overlap on one GPU race.
+ * The solver must be FP64 and already set up (p4g_solver_setup:
+ * PETSC_ERR_ORDER otherwise); a hierarchy whose work arrays exceed 128 MiB
+ * (FP64 V(1,1) hierarchies above about 129^3 or 2049^2) is refused with
+ * PETSC_ERR_ARG_OUTOFRANGE. */
+int p4g_pc_apply_device_noalloc(p4g_Solver s, const double *d_r, double *d_z);
 /* MatMult with the finest-level operator: y = A x. */
 int p4g_mat_mult(p4g_Solver s, const double *x, double *y);
 int p4g_mat_mult_device(p4g_Solver s, const double *d_x, double *d_y);

@@ -121,6 +121,7 @@ def lib():
         L.p4g_ksp_solve_device.argtypes = [vp, vp, vp, ctypes.POINTER(Report)]
         L.p4g_pc_apply.argtypes = [vp, vp, vp]
         L.p4g_pc_apply_device.argtypes = [vp, vp, vp]
+        L.p4g_pc_apply_device_noalloc.argtypes = [vp, vp, vp]
         L.p4g_mat_mult.argtypes = [vp, vp, vp]
         L.p4g_mat_mult_device.argtypes = [vp, vp, vp]
         L.p4g_solver_get_report.argtypes = [vp, ctypes.POINTER(Report)]

@@ -23,6 +23,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <type_traits>
 #include <vector>
 
@@ -1152,6 +1153,21 @@ struct Engine {
         long nb = 0;
         P4G_TRY(pc_apply_dots(rr, zz, &nb));
         return 0;
+    }
+
+    // every per-call work array of a preconditioner application (each
+    // level's b, x, r and smoother temporaries) with its length in elements;
+    // f may re-point them (operator data such as kc, g27, Ainv is not here)
+    template <class F>
+    void each_work(F&& f) {
+        for (int l = 0; l < nlev; ++l) {
+            Dev& d = dv[l];
+            const long n = (*lv)[l].n;
+            f(d.x, n);
+            f(d.b, n);
+            f(d.r, n);
+            for (int q = 0; q < 5; ++q) f(d.t[q], n);
+        }
     }
 
     int mat_mult(const T* xx, T* yy) {
@@ -2872,6 +2888,97 @@ int p4g_pc_apply_device(p4g_Solver s, const double* d_r, double* d_z) {
     HIP_TRY(hipStreamSynchronize(s->cx.stream));
     s->cx.tm.collect();
     return 0;
+}
+
+// ---------------------------------------------- no-allocation PCApply
+// The work arrays of one multigrid cycle of p4g_pc_apply_device_noalloc are
+// carved from this file-scope device array instead of the solver's own
+// allocations.  Every GPU holds its own copy (at its own address, looked up
+// once per device with hipGetSymbolAddress) until the process ends; calls on
+// one GPU share it and overlapping calls race.  The cycle writes every work
+// element before it reads it, so nothing in it is cleared between calls.
+constexpr size_t kNoAllocBytes = size_t(128) << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+constexpr int kNoAllocDevices = 64;
+static std::mutex noalloc_mu;
+static unsigned char* noalloc_ptr[kNoAllocDevices];
+
+static int noalloc_scratch(unsigned char** base) {
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kNoAllocDevices)
+        return set_error(PETSC_ERR_SUP, "p4g_pc_apply_device_noalloc: device %d beyond the %d supported", dev,
+                         kNoAllocDevices);
+    std::lock_guard<std::mutex> g(noalloc_mu);
+    if (!noalloc_ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return set_error(PETSC_ERR_LIB, "p4g_pc_apply_device_noalloc: hipGetSymbolAddress(scratch) failed: %s",
+                             hipGetErrorString(e));
+        noalloc_ptr[dev] = (unsigned char*)p;
+    }
+    *base = noalloc_ptr[dev];
+    return 0;
+}
+
+int p4g_pc_apply_device_noalloc(p4g_Solver s, const double* d_r, double* d_z) {
+    clear_error();
+    if (!s || !d_r || !d_z) return set_error(PETSC_ERR_ARG_WRONG, "p4g_pc_apply_device_noalloc: null argument");
+    if (!s->ready)
+        return set_error(PETSC_ERR_ORDER, "p4g_pc_apply_device_noalloc: set the solver up first (p4g_solver_setup); "
+                                          "this entry point allocates nothing");
+    if (s->mixed || !s->e64)
+        return set_error(PETSC_ERR_SUP, "p4g_pc_apply_device_noalloc: FP64 solvers only (-p4g_precision double)");
+    Engine<double>& E = *s->e64;
+    if (E.pend.on) return set_error(PETSC_ERR_LIB, "internal: pending residual update");
+    // the carve: each work array at a 256-byte aligned offset, then the
+    // block partial sums (two buffers of 2 npart_cap doubles)
+    size_t off = 0;
+    auto room = [&](long ndoubles) -> size_t {
+        const size_t o = off;
+        off += ((size_t)ndoubles * sizeof(double) + 255) & ~size_t(255);
+        return o;
+    };
+    E.each_work([&](double*& p, long n) { if (p) room(n); });
+    room(2 * s->cx.npart_cap);
+    room(2 * s->cx.npart_cap);
+    if (off > kNoAllocBytes)
+        return set_error(PETSC_ERR_ARG_OUTOFRANGE,
+                         "p4g_pc_apply_device_noalloc: this hierarchy's cycle needs %zu bytes of work arrays, more than "
+                         "the %zu MiB scratch (FP64 V(1,1) hierarchies up to 129^3 or 2049^2 fit)", off,
+                         kNoAllocBytes >> 20);
+    unsigned char* base = nullptr;
+    P4G_TRY(noalloc_scratch(&base));
+    // swap the carve in, run the cycle, swap the solver's arrays back
+    constexpr int kMaxWork = 8 * 32;
+    double* saved[kMaxWork];
+    int nsaved = 0;
+    E.each_work([&](double*&, long) { ++nsaved; });
+    if (nsaved > kMaxWork) return set_error(PETSC_ERR_SUP, "p4g_pc_apply_device_noalloc: too many levels");
+    off = 0;
+    nsaved = 0;
+    E.each_work([&](double*& p, long n) {
+        saved[nsaved++] = p;
+        if (p) p = (double*)(base + room(n));
+    });
+    double* const part0 = s->cx.part;
+    double* const part_rr0 = s->cx.part_rr;
+    s->cx.part = (double*)(base + room(2 * s->cx.npart_cap));
+    s->cx.part_rr = (double*)(base + room(2 * s->cx.npart_cap));
+    int on[P4G_K_COUNT];   // HIP-event timing off: its event pool grows by allocation
+    for (int c = 0; c < P4G_K_COUNT; ++c) { on[c] = s->cx.tm.on[c]; s->cx.tm.on[c] = 0; }
+    int e = E.pc_apply(d_r, d_z);
+    if (!e) {
+        const hipError_t h = hipStreamSynchronize(s->cx.stream);
+        if (h != hipSuccess) e = set_error(PETSC_ERR_LIB, "p4g_pc_apply_device_noalloc: %s", hipGetErrorString(h));
+    }
+    size_t q = 0;
+    E.each_work([&](double*& p, long) { p = saved[q++]; });
+    s->cx.part = part0;
+    s->cx.part_rr = part_rr0;
+    for (int c = 0; c < P4G_K_COUNT; ++c) s->cx.tm.on[c] = on[c];
+    return e;
 }
 
 int p4g_pc_apply(p4g_Solver s, const double* r, double* z) {
