@@ -232,6 +232,7 @@ def serving(rate=40.0, nreq=80, seed=2026, budget=64 << 30):
         b, xh = sine_mode_request(dim, m, seed * 1000 + q)
         reqs.append((dim, r, m, b, xh if q < 12 else None))
     out = []
+    alone = {}
     with capi.Server(budget) as srv:
         t0 = time.perf_counter()
         ids = []
@@ -243,6 +244,14 @@ def serving(rate=40.0, nreq=80, seed=2026, budget=64 << 30):
         for q, rid in enumerate(ids):
             out.append(srv.wait(rid))
         st = srv.stats()
+        # each size alone through the same server: three requests one at a time
+        for dim, r in SERVE_SHAPES:
+            m = 2 ** (r + 1) + 1
+            b, _ = sine_mode_request(dim, m, seed * 7 + r)
+            lat_alone = []
+            for _ in range(3):
+                lat_alone.append(srv.wait(srv.submit(sess[(dim, r)].solver, b))[1].solve_ms)
+            alone[(dim, r)] = sorted(lat_alone)[1]
     errs = []
     for q, (dim, r, m, b, xh) in enumerate(reqs):
         if xh is not None:
@@ -250,7 +259,16 @@ def serving(rate=40.0, nreq=80, seed=2026, budget=64 << 30):
     for s in sess.values():
         s.close()
     its = [rep.its for _, rep in out]
+    per_size = {}
+    for (dim, r) in SERVE_SHAPES:
+        lat = [out[q][1].solve_ms for q in range(len(reqs)) if (reqs[q][0], reqs[q][1]) == (dim, r)]
+        m = 2 ** (r + 1) + 1
+        per_size["%d^%d" % (m, dim)] = {"requests": len(lat), "alone_ms": alone[(dim, r)],
+                                         "mix_p50_ms": pct(lat, 0.5), "mix_p99_ms": pct(lat, 0.99)}
+    largest_alone = max(alone.values())
     return {"p50_ms": st.p50_ms, "p99_ms": st.p99_ms, "mean_ms": st.mean_ms, "max_ms": st.max_ms,
+            "largest_alone_ms": largest_alone, "p99_over_largest_alone": st.p99_ms / largest_alone,
+            "per_size": per_size,
             "queue_p50_ms": st.queue_p50_ms, "throughput_dof_per_s": st.dof_per_s,
             "requests": int(st.completed), "rate_per_s": rate, "wall_s": st.wall_s,
             "rounds": int(st.rounds), "graph_captures": int(st.graph_captures),
@@ -419,6 +437,45 @@ def obstacle_cluster(with_oracle):
     return o
 
 
+FISH_CLUSTER = "-fsh_dim 3 -da_refine 7 -pc_mg_levels 6 -pc_type mg"
+
+
+def fish_cluster(with_oracle):
+    """fish as c/ch8/cluster.sh:58-63 runs it: 257^3 (manuexp), CG + 6-level
+    multigrid with PETSc's default level smoother, Chebyshev(2) with SOR (the
+    sweeps as pipelined strips, one launch per direction), rtol 1e-5.  KSP
+    solve time (median of 3 after a warm-up), iterations and the error line;
+    the same solve with one launch per hyperplane; the oracle's solve time."""
+    from p4pdes_amd import capi
+
+    def run(opts, n):
+        with capi.Session(opts) as s:
+            s.solve()
+            capi.check(capi.lib().p4g_device_synchronize())
+            ts = []
+            for _ in range(n):
+                t0 = time.perf_counter()
+                rep = s.solve()
+                capi.check(capi.lib().p4g_device_synchronize())
+                ts.append((time.perf_counter() - t0) * 1e3)
+            return sorted(ts)[len(ts) // 2], rep, s.n
+
+    ms, rep, n = run(FISH_CLUSTER, 3)
+    ms_planes, rep_p, _ = run(FISH_CLUSTER + " -p4g_sor_pipeline 0", 1)
+    o = {"workload": "fish -fsh_dim 3 -da_refine 7 -pc_mg_levels 6 -pc_type mg (c/ch8/cluster.sh:63): 257^3, "
+                     "CG + V(2,2) Chebyshev/SOR multigrid, 9^3 coarse LU, rtol 1e-5",
+         "options": FISH_CLUSTER, "ms_per_solve": ms, "dof_per_s": n / (ms * 1e-3), "its": rep.its,
+         "ms_per_solve_hyperplane_sor": ms_planes, "its_hyperplane_sor": rep_p.its}
+    if with_oracle:
+        from oracle import oracle_py
+        nt = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        times, ores = oracle_py.fish_time(FISH_CLUSTER, nthreads=nt, warmup=0, nsolves=1)
+        o["oracle_ms_per_solve"] = times[0]
+        o["oracle_its"] = ores.its
+        o["oracle_threads"] = nt
+    return o
+
+
 MINIMAL = ("-da_grid_x 33 -da_grid_y 33 -snes_grid_sequence 6 -snes_fd_color -snes_converged_reason "
            "-ksp_converged_reason -pc_type mg")
 
@@ -569,6 +626,8 @@ def main():
         if not a.no_minimal:
             print("bench: minimal surface 2049^2", file=sys.stderr, flush=True)
             out["minimal_2049"] = minimal(not a.no_cpu)
+        print("bench: fish 257^3 with Chebyshev/SOR (cluster.sh)", file=sys.stderr, flush=True)
+        out["fish_cluster_257"] = fish_cluster(not a.no_cpu)
 
     # ---- CPU baseline: the same 513^3 FP64 system -----------------------------
     if not a.no_cpu:

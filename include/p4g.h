@@ -240,7 +240,14 @@ int p4g_solver_reset_stats(p4g_Solver s);
  * (+infinity).  x: initial iterate in, solution out; F (may be NULL): the
  * residual at the solution (SNESGetFunction).  Host arrays of the grid's
  * size.  The solver's fine operator stays the last reduced one afterwards:
- * use a solver for VI solves only. */
+ * use a solver for VI solves only.
+ * -snes_type vinewtonssls (SNESVINEWTONSSLS, obstacle.test4 of
+ * c/ch12/makefile:26-27): the semismooth Newton method on the Fischer-
+ * Burmeister function phi = Phi(x - xl, F) (xu must be NULL), per Newton
+ * step the semismooth Jacobian diag(Db) J + diag(Da), the solver's KSP with
+ * PC ilu (default) or none, the bt line search on |phi| with projected
+ * trial points; fnorm0/fnorm are then |phi| and active counts {x <= xl +
+ * 1e-8, F > 0} at the solution. */
 typedef struct {
     int    its;            /* Newton iterations (snes->iter)                   */
     int    reason;         /* SNESConvergedReason (3 CONVERGED_FNORM_RELATIVE) */
@@ -262,7 +269,7 @@ int p4g_vi_get_history(p4g_Solver s, double *fnorm, int *ksp_its, int *ksp_reaso
 
 /* ------------------------------------------------- minimal surface */
 
-/* SNESSolve with SNESNEWTONLS and -snes_fd_color for c/ch7/minimal.c's
+/* SNESSolve with SNESNEWTONLS and -snes_fd_color (or -snes_mf_operator) for c/ch7/minimal.c's
  * FormFunctionLocal (minimal.c:230-301): the minimal-surface equation
  * -div((1 + |grad u|^2)^q grad u) = 0 on a 2D grid of the unit square
  * (DMDASetUniformCoordinates(0,1,0,1)), u = g on the boundary.  The device
@@ -274,8 +281,13 @@ int p4g_vi_get_history(p4g_Solver s, double *fnorm, int *ksp_its, int *ksp_reaso
  * KSP (GMRES unless -ksp_type cg; -ksp_gmres_restart) with PC ILU(0)
  * (default), mg (each coarser level: the finite-difference Jacobian at the
  * injected iterate; coarse band LU) or none, the bt (alpha 1e-4) or basic
- * line search, SNESConvergedDefault.  SNES options as for p4g_vi_solve but
- * -snes_type newtonls; -mat_is_symmetric tol checks each Jacobian.  x:
+ * line search, SNESConvergedDefault.  -snes_mf_operator: the Krylov
+ * operator is MatMFFD at the iterate ("wp" step h = 1.49e-8 sqrt(1 + |x|) /
+ * |a|), the grid's Jacobian callback (minimal.c registers the Poisson
+ * Jacobian) the preconditioning matrix on every level; with neither option
+ * that callback's matrix is the Newton Jacobian; -snes_mf is refused.  SNES
+ * options as for p4g_vi_solve but -snes_type newtonls; -mat_is_symmetric
+ * tol checks each Jacobian.  x:
  * initial iterate in, solution out (host, the grid's size).  mon (may be
  * NULL) is called with the host iterate after the initial residual and
  * after each Newton step, as SNESMonitorSet's monitors are; a nonzero

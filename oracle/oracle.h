@@ -106,6 +106,9 @@ void oracle_minimal_residual(int mx, int my, int problem, double q, double H, do
                              const double *u, double *F);
 void oracle_minimal_fdjac(int mx, int my, int problem, double q, double H, double c,
                           const double *u, double *J9);
+/* y = J(u) a by the matrix-free finite difference of -snes_mf_operator */
+void oracle_minimal_mffd(int mx, int my, int problem, double q, double H, double c,
+                         const double *u, const double *a, double *y);
 
 /* Time `nsolves` solves (after `warmup`) of the linear system fish builds
  * for argv, with `nthreads` OpenMP threads.  Setup is excluded.  Fills

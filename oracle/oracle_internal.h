@@ -142,7 +142,9 @@ typedef struct {
     double  est_tf[4];       /* emin = tf0*min+tf1*max, emax = tf2*min+tf3*max */
     const long *gidx;        /* estimate start vector: noisy(gidx[i]), else noisy(i) */
     opc     pc;
-    const ocsr *A;
+    const ocsr *A;           /* the operator, or the PC's matrix when amult is set */
+    void  (*amult)(void *ctx, const double *x, double *y);   /* shell operator (Amat != Pmat) */
+    void   *actx;
     double *w[5];
     FILE   *mon; char *mbuf; int mlen; int *mpos; const char *prefix;
 } oksp;

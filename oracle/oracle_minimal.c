@@ -29,11 +29,20 @@
  *     estimate of oracle_solver.c; PETSc's estimator for a nonsymmetric
  *     operator may differ: parity with PETSc unpinned for -pc_type mg.
  *
+ *   - -snes_mf_operator: the Krylov operator is MatMFFD ("wp" step) at the
+ *     current iterate; without -snes_fd_color the preconditioning matrix on
+ *     every level is the registered Poisson2DJacobianLocal (minimal.c:159-
+ *     160); PCMG's finest level smooths with the matrix-free operator and
+ *     SOR on the Poisson matrix; -oracle_nranks N simulates N processes'
+ *     y-split DMDA (local SOR sweeps on each level).
+ *
  * Pinning: minimal.test1 (the |F| history 1.08276 .. 1.772e-10 of Newton /
  * GMRES / ILU on the 5x5 catenoid, the reason and the error line),
  * minimal.test2 (q = 0: "Matrix is symmetric" lines and CG/ILU iteration
- * counts 5 and 6 on the 9x9 tent) and minimal.test4 (grid-sequenced tent:
- * Newton counts 3, 5, 5).  tests/test_minimal.py also checks the
+ * counts 5 and 6 on the 9x9 tent), minimal.test3 (-snes_mf_operator with
+ * PCMG on two processes, grid-sequenced 3x3 -> 9x9: every area/D line of
+ * -ms_monitor and the Newton counts 5, 3, 3) and minimal.test4 (grid-
+ * sequenced tent: Newton counts 3, 5, 5).  tests/test_minimal.py also checks the
  * finite-difference Jacobian against the analytic derivative of the
  * residual and, for q = 0, against the 5-point Laplacian.
  */
@@ -148,6 +157,58 @@ static void ms_fdjac(const ogrid *g, const msctx *m, const double *x, const doub
     free(w3); free(w2);
 }
 
+/* ------------------------------------------------ -snes_mf_operator */
+
+/* MatMFFD at the base point (U, F(U)) with the default "wp" step
+ * (MatMFFDCompute_WP, MatMult_MFFD): h = eps sqrt(1 + |U|_2) / |a|_2,
+ * y = (F(U + h a) - F(U)) * (1/h); a = 0 gives y = 0. */
+typedef struct {
+    const ogrid *g;
+    const msctx *m;
+    const double *U, *F;
+    double *w;
+    long nmult;
+} msmf;
+
+static void msmf_mult(void *ctx, const double *a, double *y) {
+    msmf *c = (msmf *)ctx;
+    const long n = c->g->n;
+    c->nmult++;
+    const double norma = ovnorm2(n, a);
+    if (norma == 0.0) { ovset(n, 0.0, y); return; }
+    const double h = FD_EPS * sqrt(1.0 + ovnorm2(n, c->U)) / norma;
+    for (long i = 0; i < n; i++) c->w[i] = h * a[i] + c->U[i];
+    ms_residual(c->g, c->m, c->w, y);
+    const double ih = 1.0 / h;
+    for (long i = 0; i < n; i++) y[i] = (y[i] - c->F[i]) * ih;
+}
+
+/* the Jacobian that minimal.c registers (Poisson2DJacobianLocal,
+ * c/ch6/poissonfunctions.c:152-193, cx = cy = 1): every row's diagonal
+ * 2 (hy/hx + hx/hy), interior rows coupled to interior neighbours by
+ * -hy/hx and -hx/hy; on the box DMDA's 9-point pattern with the other
+ * entries explicit zeros, as DMCreateMatrix preallocates them (this shapes
+ * ILU(0)) */
+static void ms_poisson(const ogrid *g, ocsr *J) {
+    const int mx = g->mx, my = g->my;
+    const double hx = 1.0 / (mx - 1), hy = 1.0 / (my - 1);
+    const double scx = hy / hx, scy = hx / hy, scd = 2.0 * (scx + scy);
+    box_pattern(g, J);
+    for (int j = 0; j < my; j++)
+        for (int i = 0; i < mx; i++) {
+            const long q = (long)j * mx + i;
+            const int in = i > 0 && i < mx - 1 && j > 0 && j < my - 1;
+            for (int64_t p = J->rp[q]; p < J->rp[q + 1]; p++) {
+                const long c = J->ci[p];
+                const int ci = (int)(c % mx), cj = (int)(c / mx);
+                const int cin = ci > 0 && ci < mx - 1 && cj > 0 && cj < my - 1;
+                if (c == q) J->v[p] = scd;
+                else if (in && cin && cj == j && (ci == i - 1 || ci == i + 1)) J->v[p] = -scx;
+                else if (in && cin && ci == i && (cj == j - 1 || cj == j + 1)) J->v[p] = -scy;
+            }
+        }
+}
+
 /* ------------------------------------------- PCMG over colored Jacobians */
 
 typedef struct {
@@ -156,7 +217,18 @@ typedef struct {
     ocsr   *A, *P;        /* A[nlev-1] borrowed; P[l]: level l-1 -> l       */
     oksp   *sm;
     double **x, **b, **r;
+    msmf   *mf;           /* -snes_mf_operator: the finest level's operator */
 } msmg;
+
+/* r = b - A x on level l (the matrix-free operator on the finest level) */
+static void msmg_resid(msmg *m, int l, const double *b, const double *x, double *r) {
+    if (m->mf && l == m->nlev - 1) {
+        msmf_mult(m->mf, x, r);
+        for (long i = 0; i < m->g[l].n; i++) r[i] = b[i] - r[i];
+    } else {
+        ocsr_resid(&m->A[l], b, x, r);
+    }
+}
 
 static void msmg_free(msmg *m) {
     for (int l = 0; l < m->nlev; l++) {
@@ -170,9 +242,10 @@ static void msmg_free(msmg *m) {
 }
 
 static int msmg_setup(msmg *m, const ogrid *fine, const msctx *mc, int nlev, const double *X, const ocsr *J,
-                      oopts *o, char *err, int errlen) {
+                      int fd, msmf *mf, int nranks, oopts *o, char *err, int errlen) {
     memset(m, 0, sizeof *m);
     m->nlev = nlev;
+    m->mf = mf;
     m->g = (ogrid *)calloc(nlev, sizeof(ogrid));
     m->A = (ocsr *)calloc(nlev, sizeof(ocsr));
     m->P = (ocsr *)calloc(nlev, sizeof(ocsr));
@@ -183,6 +256,10 @@ static int msmg_setup(msmg *m, const ogrid *fine, const msctx *mc, int nlev, con
     const char *ct = oopts_get(o, "pc_mg_cycle_type");
     m->cycles = (ct && !strcmp(ct, "w")) ? 2 : 1;
     const int L = nlev - 1;
+    if (nranks > 1 && nlev > 16) {
+        snprintf(err, errlen, "oracle: simulated ranks with more than 16 levels");
+        return 63;
+    }
     m->g[L] = *fine;
     for (int l = L; l > 0; l--)
         if (ogrid_coarsen(&m->g[l], &m->g[l - 1])) {
@@ -195,12 +272,18 @@ static int msmg_setup(msmg *m, const ogrid *fine, const msctx *mc, int nlev, con
     for (int l = L; l > 0; l--) {
         const ogrid *gf = &m->g[l], *gc = &m->g[l - 1];
         oq1_build_P(gc, gf, &m->P[l]);
-        /* injection of the solution, then the colored Jacobian there */
+        /* injection of the solution, then the colored Jacobian there (without
+         * -snes_fd_color the registered Poisson Jacobian, which does not
+         * depend on the iterate) */
         double *Xc = (double *)malloc(sizeof(double) * gc->n), *Fc = (double *)malloc(sizeof(double) * gc->n);
         for (int j = 0; j < gc->my; j++)
             for (int i = 0; i < gc->mx; i++) Xc[(long)j * gc->mx + i] = Xf[(long)(2 * j) * gf->mx + 2 * i];
-        ms_residual(gc, mc, Xc, Fc);
-        ms_fdjac(gc, mc, Xc, Fc, &m->A[l - 1]);
+        if (fd) {
+            ms_residual(gc, mc, Xc, Fc);
+            ms_fdjac(gc, mc, Xc, Fc, &m->A[l - 1]);
+        } else {
+            ms_poisson(gc, &m->A[l - 1]);
+        }
         free(Xf); free(Fc);
         Xf = Xc;
     }
@@ -216,10 +299,34 @@ static int msmg_setup(msmg *m, const ogrid *fine, const msctx *mc, int nlev, con
         snprintf(err, errlen, "oracle: unsupported level smoother options");
         return 56;
     }
+    /* simulated ranks: the DMDA's y-split of every level (rank q owns rows
+     * split[q] .. split[q+1]-1; a coarse row I belongs to the owner of fine
+     * row 2I), SOR sweeps local to each rank's rows */
+    int split[16][17];
+    if (nranks > 16) nranks = 16;
+    if (nranks > 1) {
+        split[L][0] = 0;
+        for (int q = 0; q < nranks; q++)
+            split[L][q + 1] = split[L][q] + m->g[L].my / nranks + (m->g[L].my % nranks > q);
+        for (int l = L; l > 0; l--) {
+            split[l - 1][0] = 0;
+            for (int q = 0; q < nranks; q++) {
+                int cnt = 0;
+                for (int I = 0; I < m->g[l - 1].my; I++)
+                    if (2 * I >= split[l][q] && 2 * I < split[l][q + 1]) cnt++;
+                split[l - 1][q + 1] = split[l - 1][q] + cnt;
+            }
+        }
+    }
     for (int l = 0; l < nlev; l++) {
         const long n = m->g[l].n;
         if (l > 0 || nlev == 1) {
             oksp_init(&m->sm[l], ltype, &m->A[l], lpc);
+            if (mf && l == L) { m->sm[l].amult = msmf_mult; m->sm[l].actx = mf; }
+            if (nranks > 1 && lpc == PC_SOR && l < 16) {
+                m->sm[l].pc.nblk = nranks;
+                for (int q = 0; q <= nranks; q++) m->sm[l].pc.blk[q] = (long)split[l][q] * m->g[l].mx;
+            }
             m->sm[l].maxit = lits;
             m->sm[l].rich_scale = lscale;
             m->sm[l].esteig_steps = esteig < 1 ? 1 : (esteig > 15 ? 15 : esteig);
@@ -250,7 +357,7 @@ static void msmg_cycle(msmg *m, int l) {
     const ocsr *P = &m->P[l];
     const long nf = m->g[l].n, nc = m->g[l - 1].n;
     oksp_solve(&m->sm[l], m->b[l], m->x[l], &reason, &r0, &rn);
-    ocsr_resid(&m->A[l], m->b[l], m->x[l], m->r[l]);
+    msmg_resid(m, l, m->b[l], m->x[l], m->r[l]);
     ovset(nc, 0.0, m->b[l - 1]);
     for (long f = 0; f < nf; f++)
         for (int64_t q = P->rp[f]; q < P->rp[f + 1]; q++) m->b[l - 1][P->ci[q]] += P->v[q] * m->r[l][f];
@@ -287,6 +394,9 @@ typedef struct {
     oopts *o;
     msctx  m;
     int    ktype, pctype, kmaxit, restart, ls, maxit, monitor, msmon, ksp_reason, sym, mg_levels;
+    int    fd, mf;       /* -snes_fd_color: the colored Jacobian is the preconditioning matrix (else the
+                            registered Poisson Jacobian); -snes_mf_operator: the operator is matrix-free */
+    int    nranks;       /* simulated ranks (-oracle_nranks) */
     double krtol, katol, kdtol, rtol, atol, stol, divtol;
     char   symtol[32];
 } ms_cfg;
@@ -373,8 +483,8 @@ typedef struct {
 } ms_stage;
 
 /* BT or basic line search: X <- X - lambda Y, F <- F(X) */
-static int line_search(const ms_cfg *cfg, const ogrid *g, const ocsr *J, double *X, double *F, double *fnorm,
-                       const double *Y, double *xnorm, double *ynorm, double *W, double *G) {
+static int line_search(const ms_cfg *cfg, const ogrid *g, const ocsr *J, msmf *mf, double *X, double *F,
+                       double *fnorm, const double *Y, double *xnorm, double *ynorm, double *W, double *G) {
     const long n = g->n;
     const double alpha = 1e-4, steptol = 1e-12;
     double lambda = 1.0;
@@ -395,7 +505,8 @@ static int line_search(const ms_cfg *cfg, const ogrid *g, const ocsr *J, double 
         return 0;
     }
     if (*ynorm == 0.0) return 1;
-    ocsr_mult(J, Y, W);
+    if (mf) msmf_mult(mf, Y, W);   /* SNES's Jacobian is the matrix-free operator */
+    else ocsr_mult(J, Y, W);
     double initslope = ovdot(n, F, W);
     if (initslope > 0.0) initslope = -initslope;
     if (initslope == 0.0) initslope = -1.0;
@@ -451,6 +562,7 @@ static int newtonls(const ms_cfg *cfg, const ogrid *g, double *X, int ind, ms_st
     memset(st, 0, sizeof *st);
     double *F = (double *)malloc(sizeof(double) * n), *Y = (double *)calloc(n, sizeof(double));
     double *W = (double *)malloc(sizeof(double) * n), *G = (double *)malloc(sizeof(double) * n);
+    double *Wmf = (double *)malloc(sizeof(double) * n);
     int e = 0;
     if (cfg->sym) oprintf(out, outlen, pos, "Matrix is symmetric (tolerance %s)\n", cfg->symtol);
     ms_residual(g, &cfg->m, X, F);
@@ -466,7 +578,9 @@ static int newtonls(const ms_cfg *cfg, const ogrid *g, double *X, int ind, ms_st
     int i = 0, iter = 0;
     for (; !reason && i < cfg->maxit; i++) {
         ocsr J;
-        ms_fdjac(g, &cfg->m, X, F, &J);
+        msmf mfc = {g, &cfg->m, X, F, Wmf, 0};
+        if (cfg->fd) ms_fdjac(g, &cfg->m, X, F, &J);   /* the preconditioning matrix */
+        else ms_poisson(g, &J);
         if (cfg->sym) {
             const int symm = ocsr_is_symmetric(&J, atof(cfg->symtol));
             oprintf(out, outlen, pos, "Matrix is %ssymmetric (tolerance %s)\n", symm ? "" : "not ", cfg->symtol);
@@ -475,13 +589,15 @@ static int newtonls(const ms_cfg *cfg, const ogrid *g, double *X, int ind, ms_st
         int pct = cfg->pctype;
         if (pct == PC_MG) {
             const int nlev = cfg->mg_levels;
-            e = msmg_setup(&mg, g, &cfg->m, nlev, X, &J, cfg->o, err, errlen);
+            e = msmg_setup(&mg, g, &cfg->m, nlev, X, &J, cfg->fd, cfg->mf ? &mfc : NULL, cfg->nranks, cfg->o, err,
+                           errlen);
             if (e) { ocsr_free(&J); break; }
             pct = PC_SHELL;
         }
         oksp k;
         oksp_init(&k, cfg->ktype, &J, pct);
         if (pct == PC_SHELL) { k.pc.shell = msmg_apply; k.pc.shell_ctx = &mg; }
+        if (cfg->mf) { k.amult = msmf_mult; k.actx = &mfc; }
         k.rtol = cfg->krtol; k.atol = cfg->katol; k.dtol = cfg->kdtol; k.maxit = cfg->kmaxit;
         k.restart = cfg->restart;
         int kreason = 0;
@@ -501,7 +617,7 @@ static int newtonls(const ms_cfg *cfg, const ogrid *g, double *X, int ind, ms_st
         }
         if (kreason < 0) { ocsr_free(&J); reason = -3; break; }
         double gnorm = fnorm;
-        const int lsfail = line_search(cfg, g, &J, X, F, &gnorm, Y, &xnorm, &ynorm, W, G);
+        const int lsfail = line_search(cfg, g, &J, cfg->mf ? &mfc : NULL, X, F, &gnorm, Y, &xnorm, &ynorm, W, G);
         ocsr_free(&J);
         if (lsfail) {
             reason = (cfg->stol * xnorm > ynorm) ? 4 : -6;
@@ -521,7 +637,7 @@ static int newtonls(const ms_cfg *cfg, const ogrid *g, double *X, int ind, ms_st
     st->its = iter;
     st->reason = reason;
     st->fnorm = fnorm;
-    free(F); free(Y); free(W); free(G);
+    free(F); free(Y); free(W); free(G); free(Wmf);
     return e;
 }
 
@@ -553,6 +669,18 @@ void oracle_minimal_fdjac(int mx, int my, int problem, double q, double H, doubl
         }
     ocsr_free(&J);
     free(F);
+}
+
+void oracle_minimal_mffd(int mx, int my, int problem, double q, double H, double c, const double *u,
+                         const double *a, double *y) {
+    msctx m = {problem, 3, q, H, c};
+    ogrid g;
+    ogrid_set(&g, 2, mx, my, 1, 1.0, 1.0, 1.0);
+    double *F = (double *)malloc(sizeof(double) * g.n), *w = (double *)malloc(sizeof(double) * g.n);
+    ms_residual(&g, &m, u, F);
+    msmf mf = {&g, &m, u, F, w, 0};
+    msmf_mult(&mf, a, y);
+    free(F); free(w);
 }
 
 static double now_ms(void) {
@@ -609,9 +737,17 @@ int oracle_minimal_run(int argc, const char **argv, minimal_result *res, double 
         return 4;
     }
     const char *st = oopts_get(&o, "snes_type");
-    if ((st && strcmp(st, "newtonls")) || oopts_has(&o, "snes_mf_operator") || oopts_has(&o, "snes_mf")) {
-        oprintf(out, outlen, &pos, "[0]PETSC ERROR: oracle: only -snes_type newtonls with -snes_fd_color\n");
+    if ((st && strcmp(st, "newtonls")) || oopts_has(&o, "snes_mf")) {
+        oprintf(out, outlen, &pos,
+                "[0]PETSC ERROR: oracle: only -snes_type newtonls with -snes_fd_color or -snes_mf_operator\n");
         return 56;
+    }
+    c.fd = oopts_has(&o, "snes_fd_color");
+    c.mf = oopts_has(&o, "snes_mf_operator");
+    c.nranks = oopts_int(&o, "oracle_nranks", 1);
+    if (c.nranks < 1 || c.nranks > 16) {
+        oprintf(out, outlen, &pos, "[0]PETSC ERROR: -oracle_nranks must be 1..16\n");
+        return 63;
     }
     const char *lt = oopts_get(&o, "snes_linesearch_type");
     c.ls = (lt && !strcmp(lt, "basic")) ? LS_BASIC : LS_BT;

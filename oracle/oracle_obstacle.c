@@ -17,6 +17,15 @@
  *   - SNESLineSearchApply_BT with alpha = 0 (the RSLS default): trial
  *     points projected onto the bounds, the inactive-set norm, quadratic
  *     then cubic backtracking (Dennis & Schnabel, A6.3.1);
+ *   - SNESSolve_VINEWTONSSLS (-snes_type vinewtonssls): the Fischer-
+ *     Burmeister reformulation phi_i = Phi(x_i - l_i, F_i), Phi(a, b) = a + b
+ *     - sqrt(a^2 + b^2) (the obstacle sets lower bounds only), |phi| as the
+ *     SNES norm; per iteration the semismooth Jacobian diag(Db) J + diag(Da)
+ *     from the B-subdifferential (Da_i = Phi_a(x_i - l_i, F_i), Db_i =
+ *     Phi_a(F_i, x_i - l_i), Phi_a(0, 0) taken as 1 - sqrt(2)/2), KSPSolve
+ *     for the direction, the BT line search on phi with trial points
+ *     projected onto the bounds (alpha 0; test4 does not tell alpha 0 from
+ *     1e-4), no projection of the initial iterate;
  *   - PCMG under RSLS uses Galerkin coarse operators on coarse inactive
  *     sets (a coarse vertex is inactive when the restriction of the fine
  *     inactive indicator is nonzero there) with the interpolation
@@ -25,7 +34,9 @@
  * Pinning: obstacle.test1 (|F| history 3.86571, 1.3323, < 1e-11, 11 CG/ILU
  * iterations per Newton step, the error line) and obstacle.test3 (grid-
  * sequenced SNES iteration counts 1, 2, 2, 3, last KSP count 4 and the
- * error line).  A one-level PCMG (test3's 3x3 grid) applies the level
+ * error line), obstacle.test4 (SSLS, grid-sequenced Newton counts 4, 6, 5,
+ * last KSP count 6 and the error line; unprojected trial points give other
+ * counts).  A one-level PCMG (test3's 3x3 grid) applies the level
  * smoother, as PETSc's PCMGSetLevels makes it.  The backtracking branch of
  * the line search is not reached by those tests: parity unpinned there.
  */
@@ -301,6 +312,9 @@ typedef struct {
     double rtol, atol, stol, divtol;
     int    maxit, ls;
     int    monitor, ksp_reason, sym, mg_levels;
+    int    ssls;          /* -snes_type vinewtonssls */
+    double ls_alpha;      /* the BT line search's sufficient-decrease constant */
+    int    ls_project;    /* trial points projected onto the bounds */
     char   symtol[32];
 } obs_cfg;
 
@@ -355,12 +369,34 @@ typedef struct {
     const ocsr  *A;
     const double *xl, *xu;
     double *W, *G;
+    int ssls, project;   /* SSLS: G is the Fischer-Burmeister function, FW the residual at W */
+    double *FW;
 } lsctx;
 
-/* trial point W = proj(X - lambda Y), G = F(W), returns |G| on W's inactive set */
+/* the Fischer-Burmeister function and its derivative in its first argument
+ * (the B-subdifferential element PETSc's SNESVINEWTONSSLS takes at 0) */
+static double fb_phi(double a, double b) { return a + b - sqrt(a * a + b * b); }
+static double fb_dphi(double a, double b) {
+    if (fabs(a) <= 1.0e-30 && fabs(b) <= 1.0e-30) return 1.0 - sqrt(2.0) / 2.0;
+    return 1.0 - a / sqrt(a * a + b * b);
+}
+
+/* SNESVIComputeFunction with lower bounds only: F = F(X), phi_i = Phi(x_i - l_i, F_i) */
+static void ssls_function(const ogrid *g, const double *X, const double *xl, double *F, double *phi) {
+    obs_residual(g, X, F);
+    for (long i = 0; i < g->n; i++) phi[i] = fb_phi(X[i] - xl[i], F[i]);
+}
+
+/* trial point W = proj(X - lambda Y), G = F(W), returns |G| on W's inactive
+ * set; SSLS: W = X - lambda Y (projected when asked), G = phi(W), |G| */
 static double ls_trial(lsctx *c, const double *X, const double *Y, double lambda) {
     const long n = c->g->n;
     for (long i = 0; i < n; i++) c->W[i] = X[i] - lambda * Y[i];
+    if (c->ssls) {
+        if (c->project) vi_project(n, c->W, c->xl, c->xu);
+        ssls_function(c->g, c->W, c->xl, c->FW, c->G);
+        return ovnorm2(n, c->G);
+    }
     vi_project(n, c->W, c->xl, c->xu);
     obs_residual(c->g, c->W, c->G);
     return vi_fnorm(n, c->G, c->W, c->xl, c->xu);
@@ -371,7 +407,7 @@ static double ls_trial(lsctx *c, const double *X, const double *Y, double lambda
 static int line_search(const obs_cfg *cfg, lsctx *c, double *X, double *F, double *fnorm, const double *Y,
                        double *xnorm, double *ynorm) {
     const long n = c->g->n;
-    const double alpha = 0.0, steptol = 1e-12;
+    const double alpha = cfg->ls_alpha, steptol = 1e-12;
     double lambda = 1.0;
     *ynorm = ovnorm2(n, Y);
     *xnorm = ovnorm2(n, X);
@@ -525,6 +561,81 @@ static int rsls(const obs_cfg *cfg, const ogrid *g, double *X, double *F, const 
     return e;
 }
 
+/* SNESSolve_VINEWTONSSLS on grid g from X (in/out); F receives F(X).  The
+ * semismooth reformulation phi(X) = Phi(X - xl, F(X)) (Fischer-Burmeister);
+ * per iteration the Jacobian J, the B-subdifferential vectors Da_i =
+ * Phi_a(x_i - l_i, F_i), Db_i = Phi_b = Phi_a(F_i, x_i - l_i), the
+ * semismooth Jacobian diag(Db) J + diag(Da) (MatDiagonalScale, then
+ * MatDiagonalSet ADD_VALUES), KSPSolve(Jss, phi) for the direction, the
+ * line search on |phi|, SNESConvergedDefault on |phi|. */
+static int ssls(const obs_cfg *cfg, const ogrid *g, double *X, double *F, const double *xl, int ind,
+                obs_stage *st, char *out, int outlen, int *pos) {
+    const long n = g->n;
+    memset(st, 0, sizeof *st);
+    oprob pr;
+    memset(&pr, 0, sizeof pr);
+    pr.dim = 2; pr.cx = pr.cy = pr.cz = 1.0;
+    double *phi = (double *)malloc(sizeof(double) * n), *Y = (double *)calloc(n, sizeof(double));
+    double *W = (double *)malloc(sizeof(double) * n), *G = (double *)malloc(sizeof(double) * n);
+    double *FW = (double *)malloc(sizeof(double) * n);
+    ssls_function(g, X, xl, F, phi);
+    double phinorm = ovnorm2(n, phi), xnorm = ovnorm2(n, X), ynorm = 0.0, ttol = 0.0, rnorm0 = 0.0;
+    st->fnorm0 = st->fnorm = phinorm;
+    int reason = snes_converged(cfg, 0, 0.0, 0.0, phinorm, &ttol, &rnorm0);
+    if (cfg->monitor) snes_line(out, outlen, pos, ind, 0, phinorm);
+    int i = 0, iter = 0;
+    for (; !reason && i < cfg->maxit; i++) {
+        if (cfg->sym) oprintf(out, outlen, pos, "%*sMatrix is symmetric (tolerance %s)\n", ind, "", cfg->symtol);
+        ocsr J;
+        oracle_jacobian(g, &pr, &J);
+        for (long r = 0; r < n; r++) {
+            const double a = X[r] - xl[r], da = fb_dphi(a, F[r]), db = fb_dphi(F[r], a);
+            for (int64_t q = J.rp[r]; q < J.rp[r + 1]; q++) J.v[q] *= db;
+            for (int64_t q = J.rp[r]; q < J.rp[r + 1]; q++)
+                if (J.ci[q] == r) J.v[q] += da;
+        }
+        oksp k;
+        oksp_init(&k, cfg->ktype, &J, cfg->pctype);
+        k.rtol = cfg->krtol; k.atol = cfg->katol; k.dtol = cfg->kdtol; k.maxit = cfg->kmaxit;
+        if (cfg->knormset) k.normtype = cfg->normtype;
+        int kreason = 0;
+        double r0, rn;
+        const int kits = oksp_solve(&k, phi, Y, &kreason, &r0, &rn);
+        oksp_free(&k);
+        st->last_kspits = kits;
+        st->ksp_total += kits;
+        if (cfg->ksp_reason) {
+            if (kreason > 0)
+                oprintf(out, outlen, pos, "%*s    Linear solve converged due to %s iterations %d\n", ind, "",
+                        oracle_reason_name(kreason), kits);
+            else
+                oprintf(out, outlen, pos, "%*s    Linear solve did not converge due to %s iterations %d\n", ind, "",
+                        oracle_reason_name(kreason), kits);
+        }
+        if (kreason < 0) { ocsr_free(&J); reason = -3; break; }
+        lsctx lc = {g, &J, xl, NULL, W, G, 1, cfg->ls_project, FW};
+        double gnorm = phinorm;
+        const int lsfail = line_search(cfg, &lc, X, phi, &gnorm, Y, &xnorm, &ynorm);
+        ocsr_free(&J);
+        memcpy(F, FW, sizeof(double) * n);   /* the residual at the accepted point */
+        if (lsfail) {
+            reason = (cfg->stol * xnorm > ynorm) ? 4 : -6;
+            break;
+        }
+        phinorm = gnorm;
+        iter = i + 1;
+        xnorm = ovnorm2(n, X);
+        reason = snes_converged(cfg, iter, xnorm, ynorm, phinorm, &ttol, &rnorm0);
+        if (cfg->monitor) snes_line(out, outlen, pos, ind, iter, phinorm);
+    }
+    if (!reason && i >= cfg->maxit) reason = -5;
+    st->its = iter;
+    st->reason = reason;
+    st->fnorm = phinorm;
+    free(phi); free(Y); free(W); free(G); free(FW);
+    return 0;
+}
+
 /* --------------------------------------------------------------- driver */
 
 static double now_ms(void) {
@@ -546,10 +657,13 @@ int oracle_obstacle_run(int argc, const char **argv, obstacle_result *res, doubl
     memset(&c, 0, sizeof c);
     c.o = &o;
     const char *st = oopts_get(&o, "snes_type");
-    if (st && strcmp(st, "vinewtonrsls")) {
+    if (st && strcmp(st, "vinewtonrsls") && strcmp(st, "vinewtonssls")) {
         oprintf(out, outlen, &pos, "[0]PETSC ERROR: oracle: -snes_type %s not supported\n", st);
         return 56;
     }
+    c.ssls = st && !strcmp(st, "vinewtonssls");
+    c.ls_alpha = oopts_real(&o, "oracle_ls_alpha", 0.0);
+    c.ls_project = oopts_int(&o, "oracle_ls_project", 1);   /* 0: a variant the tests show test4 rules out */
     const char *lt = oopts_get(&o, "snes_linesearch_type");
     c.ls = (lt && !strcmp(lt, "basic")) ? LS_BASIC : LS_BT;
     if (lt && strcmp(lt, "basic") && strcmp(lt, "bt")) {
@@ -608,7 +722,16 @@ int oracle_obstacle_run(int argc, const char **argv, obstacle_result *res, doubl
         obs_bounds(&g, xl);
         c.mg_levels = lev_opt > 0 ? lev_opt : refine + stage + 1;
         const int ind = 2 * (seq - stage);
-        e = rsls(&c, &g, X, F, xl, NULL, ind, &sg, out, outlen, &pos, err, sizeof err);
+        if (c.ssls) {
+            if (c.pctype == PC_MG) {
+                snprintf(err, sizeof err, "oracle: -snes_type vinewtonssls with -pc_type mg");
+                e = 56;
+            } else {
+                e = ssls(&c, &g, X, F, xl, ind, &sg, out, outlen, &pos);
+            }
+        } else {
+            e = rsls(&c, &g, X, F, xl, NULL, ind, &sg, out, outlen, &pos, err, sizeof err);
+        }
         if (e) { free(xl); free(F); break; }
         if (stage < 16) res->stage_its[stage] = sg.its;
         if (snes_reason) {

@@ -262,6 +262,17 @@ def minimal_fdjac(mx, my, problem, q, H, c, u):
     return J
 
 
+def minimal_mffd(mx, my, problem, q, H, c, u, a):
+    """y = J(u) a by -snes_mf_operator's matrix-free finite difference."""
+    y = np.zeros(mx * my)
+    dp = ctypes.POINTER(ctypes.c_double)
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    lib().oracle_minimal_mffd(mx, my, problem, ctypes.c_double(q), ctypes.c_double(H), ctypes.c_double(c),
+                              u.ctypes.data_as(dp), a.ctypes.data_as(dp), y.ctypes.data_as(dp))
+    return y
+
+
 def obstacle_shape(args):
     """(mx, my) of the final grid obstacle builds for these options."""
     if isinstance(args, str):

@@ -272,6 +272,20 @@ void oksp_init(oksp *k, int type, const ocsr *A, int pctype) {
     else { k->pc.type = PC_MG; k->pc.A = A; }
 }
 
+/* the operator of a KSP: its matrix, or a shell product (amult) such as a
+ * matrix-free finite-difference Jacobian (MatMFFD) */
+static void kmult(const oksp *k, const double *x, double *y) {
+    if (k->amult) k->amult(k->actx, x, y);
+    else ocsr_mult(k->A, x, y);
+}
+
+/* r = b - A x (KSPInitialResidual: the product, then VecAYPX) */
+static void kresid(const oksp *k, const double *b, const double *x, double *r) {
+    if (!k->amult) { ocsr_resid(k->A, b, x, r); return; }
+    k->amult(k->actx, x, r);
+    for (long i = 0; i < k->A->n; i++) r[i] = b[i] - r[i];
+}
+
 void oksp_free(oksp *k) {
     for (int q = 0; q < 5; q++) free(k->w[q]);
     opc_free(&k->pc);
@@ -337,7 +351,7 @@ void oksp_estimate_eigs(oksp *k) {
             be[it - 1] = beta;
             for (long i = 0; i < n; i++) p[i] = z[i] + beta * p[i];
         }
-        ocsr_mult(A, p, w);
+        kmult(k, p, w);
         double pw = ovdot(n, p, w);
         if (!(pw > 0.0)) break;
         double alpha = rz / pw;
@@ -385,7 +399,7 @@ static int cg_solve(oksp *k, const double *b, double *x, int *reason, double *rn
     long n = A->n;
     double *r = k->w[0], *z = k->w[1], *p = k->w[2], *w = k->w[3];
     double dp = 0.0, beta = 0.0, betaold = 0.0, dpi = 0.0, dpiold = 0.0, ttol = 0.0;
-    if (k->guess_nonzero) ocsr_resid(A, b, x, r);
+    if (k->guess_nonzero) kresid(k, b, x, r);
     else { ovset(n, 0.0, x); ovcopy(n, b, r); }
     if (k->normtype == NORM_PRECONDITIONED) { opc_apply(&k->pc, r, z); dp = ovnorm2(n, z); }
     else if (k->normtype == NORM_UNPRECONDITIONED) dp = ovnorm2(n, r);
@@ -407,7 +421,7 @@ static int cg_solve(oksp *k, const double *b, double *x, int *reason, double *rn
             for (long q = 0; q < n; q++) p[q] = z[q] + bb * p[q];
         }
         dpiold = dpi;
-        ocsr_mult(A, p, w);
+        kmult(k, p, w);
         dpi = ovdot(n, p, w);
         betaold = beta;
         if (dpi == 0.0 || (i > 0 && ((dpi > 0) != (dpiold > 0)))) {
@@ -456,7 +470,7 @@ static int gmres_solve(oksp *k, const double *b, double *x, int *reason, double 
     while (!*reason) {
         /* KSPInitialResidual: V0 = B (b - A x), or B b from a zero guess */
         if (first && !k->guess_nonzero) { ovset(n, 0.0, x); opc_apply(&k->pc, b, V[0]); }
-        else { ocsr_resid(A, b, x, t); opc_apply(&k->pc, t, V[0]); }
+        else { kresid(k, b, x, t); opc_apply(&k->pc, t, V[0]); }
         /* KSPGMRESCycle */
         double res = ovnorm2(n, V[0]);
         if (res != 0.0) { const double s = 1.0 / res; for (long i = 0; i < n; i++) V[0][i] *= s; }
@@ -473,7 +487,7 @@ static int gmres_solve(oksp *k, const double *b, double *x, int *reason, double 
         while (!*reason && it < m && its < k->maxit) {
             if (it) ksp_monitor(k, its, res);
             /* V[it+1] = B A V[it] */
-            ocsr_mult(A, V[it], aw);
+            kmult(k, V[it], aw);
             opc_apply(&k->pc, aw, V[it + 1]);
             /* classical Gram-Schmidt: h = V^T v, v -= V h */
             for (int j = 0; j <= it; j++) h[j] = ovdot(n, V[it + 1], V[j]);
@@ -557,7 +571,7 @@ static int cheb_solve(oksp *k, const double *b, double *x) {
     double *p[3] = {x, k->w[0], k->w[1]}, *r = k->w[2];
     int km1 = 0, kk = 1, kp1 = 2;
     c[km1] = 1.0; c[kk] = mu;
-    if (k->guess_nonzero) ocsr_resid(A, b, x, r);
+    if (k->guess_nonzero) kresid(k, b, x, r);
     else { ovset(n, 0.0, x); ovcopy(n, b, r); }
     opc_apply(&k->pc, r, p[kk]);
     {
@@ -566,7 +580,7 @@ static int cheb_solve(oksp *k, const double *b, double *x) {
         for (long q = 0; q < n; q++) pk[q] = scale * pk[q] + pkm1[q];
     }
     for (int i = 1; i < k->maxit; i++) {
-        ocsr_resid(A, b, p[kk], r);
+        kresid(k, b, p[kk], r);
         opc_apply(&k->pc, r, p[kp1]);
         c[kp1] = 2.0 * mu * c[kk] - c[km1];
         double omega = omegaprod * c[kk] / c[kp1];
@@ -592,7 +606,7 @@ static int rich_solve(oksp *k, const double *b, double *x) {
     double *r = k->w[0], *z = k->w[1];
     for (int i = 0; i < k->maxit; i++) {
         if (i == 0 && !k->guess_nonzero) { ovset(n, 0.0, x); ovcopy(n, b, r); }
-        else ocsr_resid(A, b, x, r);
+        else kresid(k, b, x, r);
         opc_apply(&k->pc, r, z);
         for (long q = 0; q < n; q++) x[q] += k->rich_scale * z[q];
     }
@@ -650,7 +664,7 @@ static int rich_outer(oksp *k, const double *b, double *x, int *reason, double *
         if (k->normtype == NORM_UNPRECONDITIONED) opc_apply(&k->pc, r, z);
         for (long q = 0; q < n; q++) x[q] += k->rich_scale * z[q];
         its++;
-        ocsr_resid(A, b, x, r);
+        kresid(k, b, x, r);
     }
     if (!*reason) {
         if (k->normtype == NORM_UNPRECONDITIONED) rn = ovnorm2(n, r);

@@ -514,6 +514,39 @@ struct OutRR {
     }
 };
 
+// Stores the operand u at every point (the prolonged iterate t of a fused
+// prolongation + smoothing pass, kept for the next Chebyshev step), then the
+// inner output.
+template <typename T, class Out>
+struct OutKeepU {
+    T* __restrict__ keep;
+    Out inner;
+    static constexpr int NDOT = Out::NDOT, NPRE = Out::NPRE;
+    template <int V>
+    __device__ __forceinline__ void pre(long q, TV<T, V>* p) const { inner.template pre<V>(q, p); }
+    __device__ __forceinline__ void pre1(long q, T* p) const { inner.pre1(q, p); }
+    __device__ __forceinline__ void pointp(long q, T u, T Au, T idg, const T* p, double& a0, double& a1) const {
+        keep[q] = u;
+        inner.pointp(q, u, Au, idg, p, a0, a1);
+    }
+    template <int V>
+    __device__ __forceinline__ void pointvp(long q, const TV<T, V>& u, const TV<T, V>& Au, const TV<T, V>& idg,
+                                            const TV<T, V>* p, double& a0, double& a1) const {
+        stv<T, V>(keep + q, u);
+        inner.template pointvp<V>(q, u, Au, idg, p, a0, a1);
+    }
+    __device__ __forceinline__ void point(long q, T u, T Au, T idg, double& a0, double& a1) const {
+        keep[q] = u;
+        inner.point(q, u, Au, idg, a0, a1);
+    }
+    template <int V>
+    __device__ __forceinline__ void pointv(long q, const TV<T, V>& u, const TV<T, V>& Au, const TV<T, V>& idg,
+                                           double& a0, double& a1) const {
+        stv<T, V>(keep + q, u);
+        inner.template pointv<V>(q, u, Au, idg, a0, a1);
+    }
+};
+
 // y = c0 prev + c1 u + c2 D^-1 (b - A u); DOTS: acc0 += y.b, acc1 += y.y
 template <typename T, bool PREV, int DOTS>
 struct OutCheb {
@@ -898,7 +931,7 @@ k_stencil(int mx, int my, int mz, int kch, Op op, Src src, Out out, double* __re
 // LDS.  A block of TYD waves covers the rows j0 .. j0+TYD-1 (j0 = TYD by) of
 // W = 64 V points (i = is .. is+W-1, is = 1 + W bx) and marches the planes
 // k0 .. k1-1.  Every plane's rows j0-1 .. j0+TYD of each array the source
-// reads at stencil neighbours go into a ring of S = P + 3 LDS slots by
+// reads at stencil neighbours go into a ring of S = P + 2 LDS slots by
 // LDS-DMA loads issued P planes ahead of use: one global_load_lds_dwordx4
 // lands a row's W values (64 lanes x 16 bytes), one 4-byte load per lane
 // the two x-halo values (is-1 and is+W) a dword at a time.  A wave waits for
@@ -906,7 +939,8 @@ k_stencil(int mx, int my, int mz, int kch, Op op, Src src, Out out, double* __re
 // those loads, so the wait also covers any later memory operation) and the
 // block's barrier makes every wave's rows visible; a slot is refilled only
 // after the barrier that follows its last read.  The operand of the own row
-// at planes k-1, k is kept in registers, the plane k+1 and rows j+-1 come
+// at planes k-1, k is kept in registers (at a chunk's first plane, plane
+// k0-1 is read from memory by the source), the plane k+1 and rows j+-1 come
 // from LDS, so each staged value is read from memory once per block where
 // stencil_vec reads rows j+-1 again through the cache.  The output's
 // per-point operands are plain loads one plane ahead.  The arithmetic per
@@ -936,7 +970,7 @@ typedef __attribute__((address_space(3))) void lds_void_t;
 template <typename T, class Op, class Src, class Out, int TYD, int P>
 __device__ __forceinline__ void stencil_lds(int bx, int by, int bz, int mx, int my, int mz, int kch, const Op& op,
                                             const Src& src, const Out& out, double& a0, double& a1) {
-    constexpr int V = VecOf<T>::V, W = TX * V, S = P + 3, NA = Src::NA;
+    constexpr int V = VecOf<T>::V, W = TX * V, S = P + 2, NA = Src::NA;
     constexpr int DPE = sizeof(T) / 4;               // dwords per element
     constexpr int NP = Out::NPRE, NPA = NP > 0 ? NP : 1;
     static_assert(V * sizeof(T) == 16 && (DPE == 1 || DPE == 2), "16-byte lanes of 4- or 8-byte elements");
@@ -957,7 +991,7 @@ __device__ __forceinline__ void stencil_lds(int bx, int by, int bz, int mx, int 
     const bool has_e = is + W < mx;
     const long sm = (long)mx * my;
     auto stage_row = [&](int kk, int r) {
-        const int slot = (kk - k0 + 1) % S;
+        const int slot = (kk - k0 + S) % S;
         const long rowq = (long)kk * sm + (long)(j0 - 1 + r) * mx;
 #pragma unroll
         for (int a = 0; a < NA; ++a) {
@@ -978,7 +1012,7 @@ __device__ __forceinline__ void stencil_lds(int bx, int by, int bz, int mx, int 
         if (oka) stage_row(kk, ra);
         if (okb) stage_row(kk, rb);
     };
-    for (int kk = k0 - 1; kk <= k0 + P; ++kk) stage(kk);
+    for (int kk = k0; kk <= k0 + P; ++kk) stage(kk);
 
     const int j = j0 + w;
     const bool rowok = j < my;                       // uniform per wave
@@ -1027,6 +1061,7 @@ __device__ __forceinline__ void stencil_lds(int bx, int by, int bz, int mx, int 
     long q = (long)k0 * sm + (long)j * mx + i0;
     if (rowok) pre_own(q, co, cox);
     TV<T, V> ud = zv, uc = zv;       // the own row's operand at planes k-1, k
+    if (rowok && k0 > 0) ud = src.template loadv<V>(q - sm);
     for (int k = k0; k < k1; ++k, q += sm) {
         // planes up to k+1 staged: at most the planes k+2 .. min(k+P, klast) outstanding
         wait_vmcnt(max(0, min(k + P, klast) - (k + 1)) * nload);
@@ -1034,11 +1069,8 @@ __device__ __forceinline__ void stencil_lds(int bx, int by, int bz, int mx, int 
         stage(k + 1 + P);
         if (!rowok) continue;                        // uniform per wave (no barrier skipped)
         if (k + 1 < k1) pre_own(q + sm, ro, rox);
-        const int sd = (k - k0) % S, sc = (k - k0 + 1) % S, su = (k - k0 + 2) % S;
-        if (k == k0) {
-            uc = opv(sc, r);
-            if (k > 0) ud = opv(sd, r);
-        }
+        const int sc = (k - k0) % S, su = (k - k0 + 1) % S;
+        if (k == k0) uc = opv(sc, r);
         const TV<T, V> uu = k + 1 < mz ? opv(su, r) : zv;   // plane k+1 (read only when k+1 < mz)
         TV<T, V> Au = zv, idg = zv;
         if (jb || k == 0 || k == mz - 1) {
@@ -1348,6 +1380,203 @@ k_restrict(int mxf, int myf, int mzf, int mxc, int myc, int mzc, int kch, const 
            T* __restrict__ bc) {
     const Tile tl = xcd_tile();
     restrict_body<T, DIM>(tl.bx, tl.by, tl.bz, mxf, myf, mzf, mxc, myc, mzc, kch, rf, bc);
+}
+
+// b_c = P^T (b - A x) on a 3D constant-coefficient level without storing
+// the fine residual: restrict_body applied to the residual that the pass
+// stencil(SrcArr x, OutResid b) would store.  A block of TYC waves covers
+// the coarse rows J0 .. J0+TYC-1 (J0 = TYC by) and the coarse columns
+// I = 1 + 64 bx + lane (the lane at I = 1 also I = 0) of coarse planes
+// K0 .. K1-1 (kch of them), marching the fine planes 2K0-1 .. 2K1-1.  Wave
+// J forms the residual on the fine rows 2J-1, 2J, 2J+1 at the lane's fine
+// pair (2I-1, 2I) (the wave's last lane also at 2I+1, the lane at I = 1 at
+// fine 0) from the operand's rows 2J0-2 .. 2J0+2TYC, staged in LDS by
+// LDS-DMA loads one plane ahead with the x-halo values is-1, is+128 and
+// is+129 (is = 1 + 128 bx), and b read directly one plane ahead.  The
+// residual at a point is OutResid's (the same operand, neighbours and
+// operation order as stencil_vec with two points per lane) and the sums
+// are restrict_body's: b_c equals the two separate passes up to rounding
+// (the same formulas, whose FMA contractions the compiler chooses per
+// kernel; FP64 cycles differ by about 2e-16 relative).
+// Requires (mxf - 1) % 128 == 0.  Grid (ceil((mxc-1)/64), ceil(myc/TYC),
+// ceil(mzc/kch)), block (64, TYC).
+template <typename T, class Op, int TYC>
+__device__ __forceinline__ void resid_restrict_lds(int bx, int by, int bz, int mx, int my, int mz, int mxc, int myc,
+                                                   int mzc, int kch, const Op& op, const T* __restrict__ x,
+                                                   const T* __restrict__ b, T* __restrict__ bc) {
+    constexpr int W = 128, NR = 2 * TYC + 3, S = 4;
+    constexpr int DPE = sizeof(T) / 4, NLD = W * (int)sizeof(T) / 16, EPL = 16 / (int)sizeof(T);
+    static_assert(DPE == 1 || DPE == 2, "4- or 8-byte elements");
+    __shared__ __attribute__((aligned(16))) T ring[S][NR][W];
+    __shared__ __attribute__((aligned(16))) T halo[S][NR][4];   // is-1, is+128, is+129
+    const int lane = threadIdx.x;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.y);
+    const int is = 1 + W * bx;
+    const int J0 = TYC * by, rbase = 2 * J0 - 2;              // first staged fine row
+    const int K0 = bz * kch, K1 = min(K0 + kch, mzc);
+    if (K0 >= K1) return;                                      // uniform per block
+    const int kf0 = max(2 * K0 - 1, 0), kf1 = min(2 * K1 - 1, mz - 1);
+    const bool has_e = is + W < mx;                            // then is + W + 1 < mx too (full tiles)
+    const long sm = (long)mx * my;
+    const T h = T(0.5), z0 = T(0);
+    auto stage_row = [&](int kk, int ro) {
+        const int jr = rbase + ro;
+        if (jr < 0 || jr >= my) return;
+        const int slot = (kk + 4) % S;
+        const T* g = x + (long)kk * sm + (long)jr * mx;
+        if (lane < NLD)
+            __builtin_amdgcn_global_load_lds((const void*)(g + is + EPL * lane), (lds_void_t*)&ring[slot][ro][0], 16,
+                                             0, 0);
+        if (lane < 3 * DPE) {
+            const int e = lane / DPE;                          // 0: is-1, 1: is+128, 2: is+129
+            if (e == 0 || has_e)
+                __builtin_amdgcn_global_load_lds(
+                    (const void*)((const char*)(g + (e == 0 ? is - 1 : is + W - 1 + e)) + 4 * (lane % DPE)),
+                    (lds_void_t*)&halo[slot][ro][0], 4, 0, 0);
+        }
+    };
+    auto stage = [&](int kk) {
+        if (kk < kf0 - 1 || kk > kf1 + 1 || kk < 0 || kk >= mz) return;
+        for (int ro = w; ro < NR; ro += TYC) stage_row(kk, ro);
+    };
+    const int J = J0 + w;
+    const bool jok = J < myc;                                  // uniform per wave
+    const int I = 1 + 64 * bx + lane;
+    const bool own0 = I == 1;
+    const int i0 = 2 * I - 1;                                  // the lane's fine pair (i0, i0 + 1)
+    const bool he = i0 + 2 < mx;                               // fine 2I+1 exists
+    // b at the lane's points of fine row jr, plane kk: the pair, fine 2I+1
+    // (last lane), fine 0 (lane at I = 1)
+    struct BV {
+        T2<T> p;
+        T e, z;
+    };
+    auto bload = [&](int kk, int jr) -> BV {
+        BV v{T2<T>{z0, z0}, z0, z0};
+        if (jr < 0 || jr >= my) return v;
+        const long rq = (long)kk * sm + (long)jr * mx;
+        v.p = ld2(b + rq + i0);
+        if (lane == TX - 1 && he) v.e = b[rq + i0 + 2];
+        if (own0) v.z = b[rq];
+        return v;
+    };
+    // the residual at fine row jr (staged row ro) of plane kk (slots sd, sc, su)
+    struct RV {
+        T2<T> p;
+        T e, z;
+    };
+    auto resid_row = [&](int kk, int jr, int sd, int sc, int su, const BV& bb) -> RV {
+        RV o{T2<T>{z0, z0}, z0, z0};
+        if (jr < 0 || jr >= my) return o;
+        const int ro = jr - rbase;
+        const T2<T> uc = *(const T2<T>*)&ring[sc][ro][2 * lane];
+        const bool pb = jr == 0 || jr == my - 1 || kk == 0 || kk == mz - 1;   // boundary row or plane
+        const bool ms = jr - 1 > 0, mn = jr + 1 < my - 1, md = kk - 1 > 0, mu = kk + 1 < mz - 1;
+        T A0, A1, d0;
+        if (pb) {
+            op.bnd(i0, jr, kk, uc.a, A0, d0);
+            op.bnd(i0 + 1, jr, kk, uc.b, A1, d0);
+        } else {
+            const T2<T> zz{z0, z0};
+            const T2<T> sv = ms ? *(const T2<T>*)&ring[sc][ro - 1][2 * lane] : zz;
+            const T2<T> nv = mn ? *(const T2<T>*)&ring[sc][ro + 1][2 * lane] : zz;
+            const T2<T> dv = md ? *(const T2<T>*)&ring[sd][ro][2 * lane] : zz;
+            const T2<T> uv = mu ? *(const T2<T>*)&ring[su][ro][2 * lane] : zz;
+            const T wv = lane > 0 ? ring[sc][ro][2 * lane - 1] : halo[sc][ro][0];
+            const T ev = lane < TX - 1 ? ring[sc][ro][2 * lane + 2] : halo[sc][ro][1];
+            op.interior(i0, jr, kk, uc.a, i0 - 1 > 0 ? wv : z0, i0 + 1 < mx - 1 ? uc.b : z0, sv.a, nv.a, dv.a, uv.a,
+                        A0, d0);
+            if (i0 + 1 == mx - 1) op.bnd(i0 + 1, jr, kk, uc.b, A1, d0);
+            else
+                op.interior(i0 + 1, jr, kk, uc.b, uc.a, i0 + 2 < mx - 1 ? ev : z0, sv.b, nv.b, dv.b, uv.b, A1, d0);
+            if (lane == TX - 1 && he) {                        // fine is+128: the next tile's first point
+                const int ie = i0 + 2;
+                const T xe = halo[sc][ro][1];
+                if (ie == mx - 1) {
+                    op.bnd(ie, jr, kk, xe, o.e, d0);
+                } else {
+                    const T se = ms ? halo[sc][ro - 1][1] : z0, ne = mn ? halo[sc][ro + 1][1] : z0;
+                    const T de = md ? halo[sd][ro][1] : z0, ue = mu ? halo[su][ro][1] : z0;
+                    op.interior(ie, jr, kk, xe, ie - 1 > 0 ? uc.b : z0, ie + 1 < mx - 1 ? halo[sc][ro][2] : z0, se, ne,
+                                de, ue, o.e, d0);
+                }
+                o.e = bb.e - o.e;
+            }
+        }
+        if (pb && lane == TX - 1 && he) {
+            T Ae;
+            op.bnd(i0 + 2, jr, kk, halo[sc][ro][1], Ae, d0);
+            o.e = bb.e - Ae;
+        }
+        o.p = T2<T>{bb.p.a - A0, bb.p.b - A1};
+        if (own0) {                                            // fine 0: a boundary point
+            T Az;
+            op.bnd(0, jr, kk, halo[sc][ro][0], Az, d0);
+            o.z = bb.z - Az;
+        }
+        return o;
+    };
+    // restrict_body's x-weighted sum of a fine row: a = coarse I, b = coarse 0
+    auto rsum = [&](const RV& v) -> T2<T> {
+        T nx = __shfl_down(v.p.a, 1, TX);
+        if (lane == TX - 1) nx = v.e;
+        T2<T> o;
+        o.a = v.p.b + h * (v.p.a + (he ? nx : T(0)));
+        o.b = own0 ? v.z + h * v.p.a : T(0);
+        return o;
+    };
+    auto comb = [&](T2<T> mid, T2<T> lo, T2<T> hi) -> T2<T> {
+        return T2<T>{mid.a + h * (lo.a + hi.a), mid.b + h * (lo.b + hi.b)};
+    };
+    auto store = [&](int K, T2<T> v) {
+        const long cb = ((long)K * myc + J) * mxc;
+        if (I < mxc) bc[cb + I] = v.a;
+        if (own0) bc[cb] = v.b;
+    };
+    const bool hlo = 2 * J - 1 >= 0, hhi = 2 * J + 1 < my;
+    BV cb0, cb1, cb2, nb0, nb1, nb2;
+    if (jok) {
+        cb0 = bload(kf0, 2 * J - 1);
+        cb1 = bload(kf0, 2 * J);
+        cb2 = bload(kf0, 2 * J + 1);
+    }
+    for (int kk = kf0 - 1; kk <= kf0 + 1; ++kk) stage(kk);
+    const T2<T> zz{z0, z0};
+    T2<T> slo = zz, smid = zz;
+    for (int k = kf0; k <= kf1; ++k) {
+        wait_vmcnt(0);
+        lds_barrier();
+        stage(k + 2);
+        if (!jok) continue;                                    // uniform per wave (no barrier skipped)
+        if (k + 1 <= kf1) {
+            nb0 = bload(k + 1, 2 * J - 1);
+            nb1 = bload(k + 1, 2 * J);
+            nb2 = bload(k + 1, 2 * J + 1);
+        }
+        const int sd = (k + 3) % S, sc = (k + 4) % S, su = (k + 5) % S;
+        const RV r1 = resid_row(k, 2 * J, sd, sc, su, cb1);
+        const RV r0 = resid_row(k, 2 * J - 1, sd, sc, su, cb0);
+        const RV r2 = resid_row(k, 2 * J + 1, sd, sc, su, cb2);
+        const T2<T> sv = comb(rsum(r1), hlo ? rsum(r0) : zz, hhi ? rsum(r2) : zz);
+        if (k & 1) {                                           // plane 2K+1: the last of K, the first of K+1
+            if (k > 2 * K0) store(k >> 1, comb(smid, slo, sv));
+            slo = sv;
+        } else {
+            smid = sv;
+        }
+        cb0 = nb0;
+        cb1 = nb1;
+        cb2 = nb2;
+    }
+    if (jok && !(kf1 & 1)) store(kf1 >> 1, comb(smid, slo, zz));   // the last fine plane
+}
+
+template <typename T, class Op, int TYC>
+__global__ void __launch_bounds__(TX * TYC)
+k_resid_restrict_lds(int mx, int my, int mz, int mxc, int myc, int mzc, int kch, Op op, const T* __restrict__ x,
+                     const T* __restrict__ b, T* __restrict__ bc) {
+    const Tile tl = xcd_tile();
+    resid_restrict_lds<T, Op, TYC>(tl.bx, tl.by, tl.bz, mx, my, mz, mxc, myc, mzc, kch, op, x, b, bc);
 }
 
 // out = base + P x_c, base = c b (ZG: zero-guess smoothing result) or x.
@@ -2564,6 +2793,9 @@ k_sor9_strips(int mx, int my, const double* __restrict__ pk, const double* __res
                     if (++spins > spin_limit) {
                         atomicExch(err, 1);
                         dead = true;
+                    } else if ((spins & 63) == 0 &&
+                               __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+                        dead = true;   // another workgroup gave up: stop waiting too
                     }
                     __builtin_amdgcn_s_sleep(1);
                 }
@@ -2638,6 +2870,142 @@ k_sor9_strips(int mx, int my, const double* __restrict__ pk, const double* __res
 #pragma unroll
         for (int k = 0; k < SOR9_AHEAD; ++k)
             if (t0 + k < nsteps) step(t0 + k, pf[k]);
+    }
+}
+
+// One direction of a symmetric SOR sweep (omega 1, zero initial guess) of a
+// 3D constant-coefficient 7-point level in lexicographic order, pipelined:
+// a 64-lane workgroup owns a strip of 64 rows of one plane (rows in sweep
+// order: j = 64 s + r forward, my - 1 - 64 s - r backward), lane r its row,
+// and marches along the row one point per step, lane r one step behind lane
+// r-1, so the neighbour toward the sweep's origin in the same plane is lane
+// r-1's result of the previous step (a DPP shift) and the one in the same
+// row the lane's own.  Workgroup g = nstrips k' + s (k' = k forward, mz-1-k
+// backward) needs the strip before it in its plane (lane 0's neighbour row)
+// and the same strip in the plane before it (every lane's neighbour plane):
+// both have lower indices.  Steps run in batches of SOR7_BATCH: before a
+// batch the first lane waits (agent-scope acquire) until those two have
+// published enough steps, then every lane loads the batch's operands (b,
+// the neighbour plane's new values, lane 0's neighbour row; backward also
+// the three lower neighbours, which still hold forward values: the points
+// after a point in the backward order run later), computes the batch, fences
+// its stores, and the last lane publishes the steps done (release).
+// Forward: x = (b + sz x_{k-1} + sy x_{j-1} + sx x_{i-1}) / d over interior
+// neighbours, in k_sor_plane's order; backward: (b + sz x_{k-1} + sy
+// x_{j-1} + sx x_{i-1} + sx x_{i+1} + sy x_{j+1} + sz x_{k+1}) / d; a
+// boundary row b / db.  flag: per workgroup, base + steps done (bases grow
+// from sweep to sweep); a wait longer than spin_limit polls sets *err and
+// gives up, and a launch that finds *err set (at entry, or every 64 polls
+// of a wait) waits for nothing (the host checks *err after the solve); need_extra is added to every wait's target
+// (0; the host's stall test sets one no workgroup reaches).  Grid nstrips *
+// mz, block 64.
+constexpr int SOR7_BATCH = 8;
+template <typename T>
+__global__ void __launch_bounds__(64)
+k_sor7_strips(int mx, int my, int mz, double d, double db, double sx, double sy, double sz,
+              const T* __restrict__ b, T* x, unsigned long long* flag, unsigned long long base, int backward,
+              int* err, long spin_limit, unsigned long long need_extra) {
+    static_assert(sizeof(T) == 8, "FP64 sweeps (the neighbours' new values are read as 8-byte atomics)");
+    constexpr int B = SOR7_BATCH;
+    const int nstrips = (my + 63) / 64;
+    const int g = blockIdx.x, r = threadIdx.x;
+    const int s = g % nstrips, kk = g / nstrips;        // strip and plane in sweep order
+    const int k = backward ? mz - 1 - kk : kk;
+    const int jr = 64 * s + r;                          // row in sweep order
+    const int j = backward ? my - 1 - jr : jr;
+    const bool live = jr < my;
+    const int jn = backward ? j + 1 : j - 1;            // neighbour row toward the origin
+    const int kn = backward ? k + 1 : k - 1;            // neighbour plane toward the origin
+    const long dy = backward ? (long)mx : -(long)mx, dz = backward ? (long)mx * my : -((long)mx * my);
+    const long sy_ = mx, sz_ = (long)mx * my;
+    const bool bndjk = j == 0 || j == my - 1 || k == 0 || k == mz - 1;
+    auto inner = [&](int ii, int jj, int kq) {
+        return ii > 0 && ii < mx - 1 && jj > 0 && jj < my - 1 && kq > 0 && kq < mz - 1;
+    };
+    const int nsteps = mx + 63;
+    bool dead = __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
+    const unsigned long long* fs = s > 0 ? flag + (g - 1) : nullptr;          // previous strip, same plane
+    const unsigned long long* fk = kk > 0 ? flag + (g - nstrips) : nullptr;   // same strip, previous plane
+    double prev = 0.0;                                  // the lane's result of the previous step
+    for (int t0 = 0; t0 < nsteps; t0 += B) {
+        const int t1 = min(t0 + B, nsteps);
+        // the batch needs the previous strip through its step t1 - 1 + 64 (its
+        // last lane at this strip's lane-0 columns) and the previous plane's
+        // strip through its step t1 - 1
+        if (r == 0) {
+            const unsigned long long need_s = base + (unsigned long long)min(t1 + 64, nsteps) + need_extra;
+            const unsigned long long need_k = base + (unsigned long long)t1 + need_extra;
+            long spins = 0;
+            while (!dead && ((fs && __hip_atomic_load(fs, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < need_s) ||
+                             (fk && __hip_atomic_load(fk, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < need_k))) {
+                if (++spins > spin_limit) {
+                    atomicExch(err, 1);
+                    dead = true;
+                } else if ((spins & 63) == 0 &&
+                           __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+                    dead = true;   // another workgroup gave up: stop waiting too
+                }
+                __builtin_amdgcn_s_sleep(1);
+            }
+        }
+        __syncthreads();
+        // the batch's operands
+        double bq[B], vk[B], vj[B], lk[B], lj[B], li[B];
+#pragma unroll
+        for (int e = 0; e < B; ++e) {
+            bq[e] = vk[e] = vj[e] = lk[e] = lj[e] = li[e] = 0.0;
+            const int c = t0 + e - r;
+            if (!(t0 + e < t1 && live && c >= 0 && c < mx)) continue;
+            const int i = backward ? mx - 1 - c : c;
+            const long q = ((long)k * my + j) * mx + i;
+            bq[e] = double(b[q]);
+            if (bndjk || i == 0 || i == mx - 1) continue;
+            if (inner(i, j, kn)) vk[e] = ld_agent((const double*)(x + q + dz));
+            if (r == 0 && inner(i, jn, k)) vj[e] = ld_agent((const double*)(x + q + dy));
+            if (backward) {
+                if (inner(i, j, k - 1)) lk[e] = double(x[q - sz_]);
+                if (inner(i, j - 1, k)) lj[e] = double(x[q - sy_]);
+                if (inner(i - 1, j, k)) li[e] = double(x[q - 1]);
+            }
+        }
+#pragma unroll
+        for (int e = 0; e < B; ++e) {
+            const int t = t0 + e;
+            const double nbv = wave_shr1(prev);         // lane r-1's result of step t-1: (i, jn)
+            const int c = t - r;
+            double res = prev;
+            if (t < t1 && live && c >= 0 && c < mx) {
+                const int i = backward ? mx - 1 - c : c;
+                const int in_ = backward ? i + 1 : i - 1;
+                const long q = ((long)k * my + j) * mx + i;
+                double sum = bq[e];
+                if (bndjk || i == 0 || i == mx - 1) {
+                    res = sum * (1.0 / db);
+                } else {
+                    const bool hk = inner(i, j, kn), hj = inner(i, jn, k), hi = inner(in_, j, k);
+                    const double xj = r > 0 ? nbv : vj[e];
+                    if (!backward) {
+                        if (hk) sum += sz * vk[e];
+                        if (hj) sum += sy * xj;
+                        if (hi) sum += sx * prev;
+                    } else {
+                        if (inner(i, j, k - 1)) sum += sz * lk[e];
+                        if (inner(i, j - 1, k)) sum += sy * lj[e];
+                        if (inner(i - 1, j, k)) sum += sx * li[e];
+                        if (hi) sum += sx * prev;
+                        if (hj) sum += sy * xj;
+                        if (hk) sum += sz * vk[e];
+                    }
+                    res = sum * (1.0 / d);
+                }
+                x[q] = T(res);
+            }
+            prev = res;
+        }
+        // every lane's stores of the batch before the published count
+        __threadfence();
+        if (r == 63)
+            __hip_atomic_store(flag + g, base + (unsigned long long)t1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
 
@@ -2891,6 +3259,72 @@ k_vi_dot(long n, const double* __restrict__ x, const double* __restrict__ y, dou
     block_partials<2, 4>(a0, a1, part, blockIdx.x);
 }
 
+// SNESVINEWTONSSLS (semismooth Newton on the Fischer-Burmeister
+// reformulation, lower bounds only as the obstacle problem sets them):
+// phi(a, b) = a + b - sqrt(a^2 + b^2) and its derivative in a, with PETSc's
+// B-subdifferential choice 1 - sqrt(2)/2 where a = b = 0.  No FMA
+// contraction: the same IEEE operations as the plain formula.
+__device__ __forceinline__ double fb_phi(double a, double b) {
+#pragma clang fp contract(off)
+    return a + b - sqrt(a * a + b * b);
+}
+__device__ __forceinline__ double fb_dphi(double a, double b) {
+#pragma clang fp contract(off)
+    if (fabs(a) <= 1.0e-30 && fabs(b) <= 1.0e-30) return 1.0 - sqrt(2.0) / 2.0;
+    return 1.0 - a / sqrt(a * a + b * b);
+}
+
+// phi_q = Phi(x_q - l_q, F_q)
+__global__ void __launch_bounds__(256)
+k_ssls_phi(long n, const double* __restrict__ x, const double* __restrict__ xl, const double* __restrict__ F,
+           double* __restrict__ phi) {
+    for (long q = blockIdx.x * 256L + threadIdx.x; q < n; q += (long)gridDim.x * 256L)
+        phi[q] = fb_phi(x[q] - xl[q], F[q]);
+}
+
+// the semismooth Jacobian diag(Db) J + diag(Da) in 27-point form: J the
+// constant star operator of g (Dirichlet rows diagonal-only, interior rows
+// coupled to interior neighbours), Da_q = Phi_a(x_q - l_q, F_q), Db_q =
+// Phi_a(F_q, x_q - l_q); each row scaled by Db_q, then Da_q added to its
+// diagonal (MatDiagonalScale, MatDiagonalSet with ADD_VALUES)
+__global__ void __launch_bounds__(256)
+k_ssls_op27(ViGrid g, const double* __restrict__ x, const double* __restrict__ xl, const double* __restrict__ F,
+            double* __restrict__ g27) {
+#pragma clang fp contract(off)
+    for (long q = blockIdx.x * 256L + threadIdx.x; q < g.n; q += (long)gridDim.x * 256L) {
+        int i, j, k;
+        vi_ijk(g, q, i, j, k);
+        const double a = x[q] - xl[q], f = F[q];
+        const double da = fb_dphi(a, f), db = fb_dphi(f, a);
+        double c[27];
+        for (int t = 0; t < 27; ++t) c[t] = 0.0;
+        const bool bnd = vi_bnd(g, i, j, k);
+        c[13] = bnd ? g.db : g.d;
+        if (!bnd) {
+            if (!vi_bnd(g, i - 1, j, k)) c[12] = -g.sx;
+            if (!vi_bnd(g, i + 1, j, k)) c[14] = -g.sx;
+            if (g.dim >= 2) {
+                if (!vi_bnd(g, i, j - 1, k)) c[10] = -g.sy;
+                if (!vi_bnd(g, i, j + 1, k)) c[16] = -g.sy;
+            }
+            if (g.dim == 3) {
+                if (!vi_bnd(g, i, j, k - 1)) c[4] = -g.sz;
+                if (!vi_bnd(g, i, j, k + 1)) c[22] = -g.sz;
+            }
+        }
+        for (int t = 0; t < 27; ++t) c[t] *= db;
+        c[13] += da;
+        const int z0 = g.dim == 3 ? 0 : 1, z1 = g.dim == 3 ? 2 : 1;
+        const int y0 = g.dim >= 2 ? 0 : 1, y1 = g.dim >= 2 ? 2 : 1;
+        for (int oz = z0; oz <= z1; ++oz)
+            for (int oy = y0; oy <= y1; ++oy)
+                for (int ox = 0; ox < 3; ++ox) {
+                    const int t = 9 * oz + 3 * oy + ox;
+                    g27[(long)t * g.n + q] = c[t];
+                }
+    }
+}
+
 // the reduced operator in 27-point form (coefficient c = 9(oz+1) + 3(oy+1)
 // + (ox+1) of row q at g27[c n + q]): inactive rows keep the diagonal and
 // the couplings to inactive interior neighbours; active rows are null
@@ -3019,6 +3453,59 @@ k_ms_fdjac(MsGrid g, const double* __restrict__ u, const double* __restrict__ G,
                 g27[(long)c * n + q] = v;
             }
     }
+}
+
+// -snes_mf_operator: MatMult_MFFD's two vector steps around the residual
+// evaluation, w = h a + U (VecWAXPY) and y = (F(w) - F(U)) (1/h) (VecAXPY,
+// VecScale), without FMA contraction
+__global__ void __launch_bounds__(256)
+k_mffd_base(long n, double h, const double* __restrict__ a, const double* __restrict__ U, double* __restrict__ w) {
+#pragma clang fp contract(off)
+    for (long q = blockIdx.x * 256L + threadIdx.x; q < n; q += (long)gridDim.x * 256L) w[q] = h * a[q] + U[q];
+}
+
+__global__ void __launch_bounds__(256)
+k_mffd_diff(long n, const double* __restrict__ F, double ih, double* __restrict__ y) {
+#pragma clang fp contract(off)
+    for (long q = blockIdx.x * 256L + threadIdx.x; q < n; q += (long)gridDim.x * 256L) y[q] = (y[q] - F[q]) * ih;
+}
+
+// A constant 2D 5-point operator (a Dirichlet row's diagonal db, an
+// interior row's d and -sx, -sy toward interior neighbours: the rows
+// Poisson2DJacobianLocal sets) in the 2D planes 9..17 of the 27-plane form
+__global__ void __launch_bounds__(256)
+k_const5_to_g27(int mx, int my, double d, double db, double sx, double sy, double* __restrict__ g27) {
+    const long n = (long)mx * my;
+    for (long q = blockIdx.x * 256L + threadIdx.x; q < n; q += (long)gridDim.x * 256L) {
+        const int i = (int)(q % mx), j = (int)(q / mx);
+        const bool in = i > 0 && i < mx - 1 && j > 0 && j < my - 1;
+        for (int c = 9; c < 18; ++c) g27[(long)c * n + q] = 0.0;
+        g27[13L * n + q] = in ? d : db;
+        if (!in) continue;
+        if (i - 1 > 0) g27[12L * n + q] = -sx;
+        if (i + 1 < mx - 1) g27[14L * n + q] = -sx;
+        if (j - 1 > 0) g27[10L * n + q] = -sy;
+        if (j + 1 < my - 1) g27[16L * n + q] = -sy;
+    }
+}
+
+// The output policy of a stencil pass applied to a product A u formed
+// elsewhere (a matrix-free operator): Out(u, Au) at every point, u or Au
+// null meaning zero; the Jacobi factor from the diagonal of the 27-plane
+// preconditioning operator g.
+template <typename T, class Out>
+__global__ void __launch_bounds__(256)
+k_out_au(long n, const T* __restrict__ u, const T* __restrict__ Au, const double* __restrict__ g, int jac, Out out,
+         double* __restrict__ part) {
+    double a0 = 0.0, a1 = 0.0;
+    for (long q = blockIdx.x * 256L + threadIdx.x; q < n; q += (long)gridDim.x * 256L) {
+        const T uc = u ? u[q] : T(0);
+        const T au = Au ? Au[q] : T(0);
+        const T dq = T(g[13 * n + q]);
+        const T idg = jac ? (dq != T(0) ? T(1) / dq : T(0)) : T(1);
+        out.point(q, uc, au, idg, a0, a1);
+    }
+    block_partials<Out::NDOT, 4>(a0, a1, part, blockIdx.x);
 }
 
 // coarse (I, J) <- fine (2I, 2J)

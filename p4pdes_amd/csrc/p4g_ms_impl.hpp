@@ -69,6 +69,7 @@ struct MsRun {
     hipStream_t st = nullptr;
     double *X = nullptr, *F = nullptr, *Y = nullptr, *W = nullptr, *G = nullptr;
     double* S = nullptr;                       // the second half of cx.S
+    double *mfw = nullptr, *mfy = nullptr;     // -snes_mf_operator: X + h a, and the product
     std::vector<MsGrid> grid;                  // per level
     std::vector<double*> gb, xl, fl;           // per level: boundary values, injected iterate, its residual
 
@@ -122,14 +123,55 @@ static int ms_fdjac_level(p4g_solver_s* s, MsRun& R, int l, const double* x, con
     return 0;
 }
 
-// the Jacobian hierarchy of the current iterate, then the PC's set-up
-static int ms_operator(p4g_solver_s* s, MsRun& R) {
+// the registered Jacobian callback's operator on level l (recorded at
+// set-up as a constant 5-point stencil: minimal.c registers
+// Poisson2DJacobianLocal) into the level's 27-plane table
+static int ms_const_level(p4g_solver_s* s, MsRun& R, int l) {
+    const LevelInfo& Lv = s->lv[l];
+    const int nb = vblocks(Lv.n);
+    k_const5_to_g27<<<nb, 256, 0, R.st>>>(Lv.mx, Lv.my, Lv.d, Lv.db, Lv.sx, Lv.sy, s->g27[l]);
+    if (Lv.pipe) k_sor9_pack<<<nb, 256, 0, R.st>>>(Lv.n, s->g27[l], Lv.pipe->pk);
+    LAUNCH_CHECK();
+    return 0;
+}
+
+// -snes_mf_operator: y = J(X) a by MatMult_MFFD with the "wp" step
+// (MatMFFDCompute_WP, |U| recomputed at every product): h = eps sqrt(1 +
+// |X|_2) / |a|_2, y = (F(X + h a) - F(X)) (1/h); a = 0 gives y = 0.  The
+// base point is the current iterate and its residual (SNES's MatMFFDSetBase).
+static int ms_mffd(p4g_solver_s* s, MsRun& R, double* w, const double* a, double* y) {
+    double t, na, nx;
+    P4G_TRY(R.dot(a, a, t, na));
+    if (na == 0.0) {
+        k_fill<double><<<R.nb, 256, 0, R.st>>>(R.n, 0.0, y);
+        LAUNCH_CHECK();
+        return 0;
+    }
+    P4G_TRY(R.dot(R.X, R.X, t, nx));
+    const double h = 1.490116119384766e-08 * std::sqrt(1.0 + nx) / na;   // PETSC_SQRT_MACHINE_EPSILON
+    k_mffd_base<<<R.nb, 256, 0, R.st>>>(R.n, h, a, R.X, w);
+    LAUNCH_CHECK();
+    P4G_TRY(R.fun((int)R.grid.size() - 1, w, y));
+    k_mffd_diff<<<R.nb, 256, 0, R.st>>>(R.n, R.F, 1.0 / h, y);
+    LAUNCH_CHECK();
+    (void)s;
+    return 0;
+}
+
+// the Jacobian hierarchy of the current iterate, then the PC's set-up:
+// colored finite differences (-snes_fd_color) or the registered Jacobian
+static int ms_operator(p4g_solver_s* s, MsRun& R, bool fd) {
     const int L = s->nlev - 1;
     Engine<double>& e = *s->e64;
-    P4G_TRY(ms_fdjac_level(s, R, L, R.X, R.F));
+    if (fd) P4G_TRY(ms_fdjac_level(s, R, L, R.X, R.F));
+    else P4G_TRY(ms_const_level(s, R, L));
     if (s->pctype == PCT_ILU) return ilu9_factor(s->lv[L], s->g27[L], s->ms_ilu9, R.st);
     if (s->pctype != PCT_MG) return 0;
     for (int l = L - 1; l >= 0; --l) {
+        if (!fd) {
+            P4G_TRY(ms_const_level(s, R, l));
+            continue;
+        }
         const MsGrid &gc = R.grid[l], &gf = R.grid[l + 1];
         const double* xf = (l + 1 == L) ? R.X : R.xl[l + 1];
         k_inject2<<<vblocks(gc.n), 256, 0, R.st>>>(gc.mx, gc.my, gf.mx, xf, R.xl[l]);
@@ -237,9 +279,12 @@ static int ms_solve(p4g_solver_s* s, double q, const double* gb, double* x, p4g_
         return set_error(PETSC_ERR_SUP, "-snes_type %s: p4g_ms_solve implements newtonls", s->snes_type.c_str());
     if (s->ls_type != "bt" && s->ls_type != "basic")
         return set_error(PETSC_ERR_SUP, "-snes_linesearch_type %s is not implemented (bt, basic)", s->ls_type.c_str());
-    if (s->opt.has("snes_mf_operator") || s->opt.has("snes_mf"))
-        return set_error(PETSC_ERR_SUP, "p4g_ms_solve forms the Jacobian by colored finite differences "
-                                        "(-snes_fd_color); -snes_mf_operator is not implemented");
+    if (s->opt.has("snes_mf"))
+        return set_error(PETSC_ERR_SUP, "p4g_ms_solve: -snes_mf (no preconditioning matrix) is not implemented; "
+                                        "use -snes_fd_color or -snes_mf_operator");
+    // the preconditioning matrix: colored finite differences, else the
+    // registered Jacobian; the operator: matrix-free under -snes_mf_operator
+    const bool fd = s->opt.has("snes_fd_color"), mf = s->opt.has("snes_mf_operator");
     if (s->mixed) return set_error(PETSC_ERR_SUP, "p4g_ms_solve: mixed precision is not implemented");
     if (s->fmg || s->kaskade || s->mcycles != 1 || s->coarse_its > 0 || s->lilu || s->galerkin)
         return set_error(PETSC_ERR_SUP, "p4g_ms_solve: multiplicative V/W cycles over the finite-difference "
@@ -282,7 +327,7 @@ static int ms_solve(p4g_solver_s* s, double q, const double* gb, double* x, p4g_
         R.grid[l] = MsGrid{Lv.mx, Lv.my, Lv.n, 1.0 / (Lv.mx - 1), 1.0 / (Lv.my - 1), q};
     }
     if (!s->ms_buf) {
-        long tot = 5 * n;
+        long tot = 7 * n;   // X F Y W G, and the matrix-free product's two vectors
         for (int l = l0; l <= L; ++l) tot += (l == L ? 1 : 3) * s->lv[l].n;
         P4G_TRY(dalloc(s, (void**)&s->ms_buf, tot * sizeof(double)));
         for (int l = l0; l <= L; ++l) {
@@ -317,7 +362,8 @@ static int ms_solve(p4g_solver_s* s, double q, const double* gb, double* x, p4g_
     {
         double* b = s->ms_buf;
         R.X = b; R.F = b + n; R.Y = b + 2 * n; R.W = b + 3 * n; R.G = b + 4 * n;
-        b += 5 * n;
+        R.mfw = b + 5 * n; R.mfy = b + 6 * n;
+        b += 7 * n;
         for (int l = L; l >= l0; --l) {
             R.gb[l] = b;
             b += s->lv[l].n;
@@ -377,8 +423,18 @@ static int ms_solve(p4g_solver_s* s, double q, const double* gb, double* x, p4g_
     P4G_TRY(sor_pipes_reset(s));
     int reason = vi_converged(s, 0, 0.0, 0.0, fnorm, ttol, rnorm0);
     int i = 0, iter = 0;
+    // the engine's finest operator is the matrix-free product while this
+    // solve runs (R holds its base point)
+    struct MfScope {
+        Engine<double>* e;
+        ~MfScope() { e->mf_op = nullptr; e->mf_tmp = nullptr; }
+    } mf_scope{s->e64.get()};
+    if (mf) {
+        s->e64->mf_tmp = R.mfy;
+        s->e64->mf_op = [s, &R](const double* a, double* y) { return ms_mffd(s, R, R.mfw, a, y); };
+    }
     for (; !reason && i < s->snes_maxit; ++i) {
-        P4G_TRY(ms_operator(s, R));
+        P4G_TRY(ms_operator(s, R, fd));
         if (symcheck) {
             int sym = 0;
             P4G_TRY(ms_symmetric(s, R, symtol, &sym));

@@ -55,6 +55,16 @@ struct SrvReq {
     double* S = nullptr;   // device scalar block (32 doubles) of the slot
 };
 
+// a launch of a batched kernel over a list of items
+struct BLaunch {
+    int kind = 0;          // which kernel
+    int dim = 3;
+    size_t off = 0;        // byte offset of the items in the device buffer
+    int nitems = 0;
+    long blocks = 0;
+    size_t len = 0;        // bytes of the items
+};
+
 }  // namespace p4g
 
 struct p4g_server_s {
@@ -74,12 +84,17 @@ struct p4g_server_s {
     hipStream_t st = nullptr;
     double *S = nullptr, *hS = nullptr;
     char *d_round = nullptr, *h_round = nullptr;
-    size_t round_cap = 0;
+    size_t round_cap = 0, round_bytes = 0;
     char *d_vc = nullptr, *h_vc = nullptr;
-    size_t vc_cap = 0;
+    size_t vc_cap = 0, vc_bytes = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
-    std::vector<long> graph_key;
+    std::vector<long> graph_key;          // the active set the graph was captured for
+    std::vector<long> vc_key;             // the active set the V-cycle tables were built for
+    std::vector<p4g::BLaunch> vseq;       // their launch sequence
+    bool vc_built = false;
+    int vc_stable = 0;                    // rounds the active set has held
+    int graph_after = 3;                  // -p4g_serve_graph_after: rounds before a capture
     // statistics
     std::vector<double> lat_ms, queue_ms;
     long submitted = 0, completed = 0, rejected = 0, rounds = 0, captures = 0;
@@ -194,14 +209,6 @@ struct Packer {
     }
 };
 
-// a launch of a batched kernel over a list of items
-struct BLaunch {
-    int kind = 0;          // which kernel
-    int dim = 3;
-    size_t off = 0;        // byte offset of the items in the device buffer
-    int nitems = 0;
-    long blocks = 0;
-};
 
 enum { BK_DIR = 0, BK_FIN, BK_UPD, BK_REP, BK_RESID, BK_RESTRICT, BK_PSMOOTH, BK_COARSE };
 
@@ -210,8 +217,11 @@ typedef StencilItem<OpConst<double>, SrcArr<double>, OutResidCvt<double, float>>
 typedef StencilItem<OpConst<float>, SrcArr<float>, OutScaleU<float, OutResid<float, 0>>> ResidItem;
 typedef ProlongSmoothItem<float, OpConst<float>, OutChebR<float>> PSmoothItem;
 
-static int blaunch(p4g_server_s* v, const BLaunch& L, const char* base) {
+static int blaunch(p4g_server_s* v, const BLaunch& L, const char* base, size_t table_bytes) {
     if (L.nitems == 0 || L.blocks == 0) return 0;
+    if (!base || L.off + L.len > table_bytes)
+        return set_error(PETSC_ERR_LIB, "internal: batched launch outside its item table (%zu + %zu > %zu)", L.off,
+                         L.len, table_bytes);
     const void* items = base + L.off;
     hipStream_t st = v->st;
     const dim3 g((unsigned)L.blocks);
@@ -250,8 +260,9 @@ static int blaunch(p4g_server_s* v, const BLaunch& L, const char* base) {
     return 0;
 }
 
-static int upload(p4g_server_s* v, const Packer& pk, char** dbuf, char** hbuf, size_t* cap) {
+static int upload(p4g_server_s* v, const Packer& pk, char** dbuf, char** hbuf, size_t* cap, size_t* used) {
     const size_t bytes = pk.buf.size();
+    *used = bytes;
     if (bytes == 0) return 0;
     // work queued earlier may still read the device table or the staging buffer
     HIP_TRY(hipStreamSynchronize(v->st));
@@ -273,6 +284,7 @@ static int upload(p4g_server_s* v, const Packer& pk, char** dbuf, char** hbuf, s
 // launch.  Descending sizes: zero-guess smoothing + residual, restriction;
 // all coarsest solves; ascending sizes: prolongation (+ c b), smoothing.
 static int build_vcycle(p4g_server_s* v, Packer& pk, std::vector<BLaunch>& seq) {
+    if (!seq.empty()) return set_error(PETSC_ERR_LIB, "internal: V-cycle plan built onto a non-empty sequence");
     struct LvRef { SrvReq* r; int l; };
     std::map<std::pair<int, int>, std::vector<LvRef>, std::greater<std::pair<int, int>>> groups;
     for (SrvReq* r : v->active)
@@ -307,10 +319,10 @@ static int build_vcycle(p4g_server_s* v, Packer& pk, std::vector<BLaunch>& seq) 
             rs.push_back(rt);
         }
         BLaunch a; a.kind = BK_RESID; a.dim = dim; a.nitems = (int)ri.size(); a.blocks = s1;
-        a.off = pk.put(ri.data(), ri.size() * sizeof(ResidItem));
+        a.off = pk.put(ri.data(), ri.size() * sizeof(ResidItem)); a.len = ri.size() * sizeof(ResidItem);
         seq.push_back(a);
         BLaunch b; b.kind = BK_RESTRICT; b.dim = dim; b.nitems = (int)rs.size(); b.blocks = s2;
-        b.off = pk.put(rs.data(), rs.size() * sizeof(RestrictItem<float>));
+        b.off = pk.put(rs.data(), rs.size() * sizeof(RestrictItem<float>)); b.len = rs.size() * sizeof(RestrictItem<float>);
         seq.push_back(b);
     }
     // coarsest solves
@@ -325,7 +337,7 @@ static int build_vcycle(p4g_server_s* v, Packer& pk, std::vector<BLaunch>& seq) 
             ci.push_back(it);
         }
         BLaunch c; c.kind = BK_COARSE; c.nitems = (int)ci.size(); c.blocks = c.nitems;
-        c.off = pk.put(ci.data(), ci.size() * sizeof(CoarseItem<float>));
+        c.off = pk.put(ci.data(), ci.size() * sizeof(CoarseItem<float>)); c.len = ci.size() * sizeof(CoarseItem<float>);
         seq.push_back(c);
     }
     // ascent: prolongation fused with the post-smoothing step
@@ -353,7 +365,7 @@ static int build_vcycle(p4g_server_s* v, Packer& pk, std::vector<BLaunch>& seq) 
             pi.push_back(m);
         }
         BLaunch a; a.kind = BK_PSMOOTH; a.dim = dim; a.nitems = (int)pi.size(); a.blocks = s1;
-        a.off = pk.put(pi.data(), pi.size() * sizeof(PSmoothItem));
+        a.off = pk.put(pi.data(), pi.size() * sizeof(PSmoothItem)); a.len = pi.size() * sizeof(PSmoothItem);
         seq.push_back(a);
     }
     return 0;
@@ -482,7 +494,7 @@ static int round_once(p4g_server_s* v) {
         if (!cnt) return;
         BLaunch a;
         a.kind = kind; a.dim = dim; a.nitems = (int)cnt; a.blocks = blocks;
-        a.off = pk.put(data, each * cnt);
+        a.off = pk.put(data, each * cnt); a.len = each * cnt;
         seq.push_back(a);
     };
     add(BK_DIR, 3, dir3.data(), sizeof(DirItem), dir3.size(), sd3);
@@ -495,24 +507,38 @@ static int round_once(p4g_server_s* v) {
     add(BK_FIN, 0, fraw.data(), sizeof(FinItem), fraw.size(), (long)fraw.size());
     const size_t nbefore = seq.size();
     add(BK_FIN, 0, fbeta.data(), sizeof(FinItem), fbeta.size(), (long)fbeta.size());
-    P4G_TRY(upload(v, pk, &v->d_round, &v->h_round, &v->round_cap));
-    for (size_t q = 0; q < nbefore; ++q) P4G_TRY(blaunch(v, seq[q], v->d_round));
-    // V-cycle: graph replay, re-captured when the active set changes
+    P4G_TRY(upload(v, pk, &v->d_round, &v->h_round, &v->round_cap, &v->round_bytes));
+    for (size_t q = 0; q < nbefore; ++q) P4G_TRY(blaunch(v, seq[q], v->d_round, v->round_bytes));
+    // V-cycle: launched directly while the active set changes, captured in a
+    // hipGraph once the set has held for graph_after rounds (a capture costs
+    // far more than one round's launches) and replayed until it changes
     std::vector<long> key;
     for (SrvReq* r : v->active) key.push_back(r->id);
-    if (!v->use_graph || key != v->graph_key || !v->gexec) {
+    if (key != v->vc_key) {
+        v->vc_key = key;
+        v->vc_stable = 0;
+        v->vc_built = false;
+    } else {
+        v->vc_stable++;
+    }
+    const bool want_graph = v->use_graph && v->vc_stable >= v->graph_after;
+    if (!v->vc_built || (want_graph && (key != v->graph_key || !v->gexec))) {
         Packer vp;
-        std::vector<BLaunch> vseq;
-        P4G_TRY(build_vcycle(v, vp, vseq));
-        P4G_TRY(upload(v, vp, &v->d_vc, &v->h_vc, &v->vc_cap));
+        v->vseq.clear();   // build_vcycle appends; the old entries index the old table
+        v->vc_built = false;
+        P4G_TRY(build_vcycle(v, vp, v->vseq));
+        P4G_TRY(upload(v, vp, &v->d_vc, &v->h_vc, &v->vc_cap, &v->vc_bytes));
+        v->vc_built = true;
         if (v->gexec) { hipGraphExecDestroy(v->gexec); v->gexec = nullptr; }
         if (v->graph) { hipGraphDestroy(v->graph); v->graph = nullptr; }
-        if (v->use_graph) {
+        v->graph_key.clear();
+        const std::vector<BLaunch>& vseq = v->vseq;
+        if (want_graph) {
             HIP_TRY(hipStreamSynchronize(v->st));
             HIP_TRY(hipStreamBeginCapture(v->st, hipStreamCaptureModeThreadLocal));
             int e = 0;
             for (const BLaunch& L : vseq)
-                if (!e) e = blaunch(v, L, v->d_vc);
+                if (!e) e = blaunch(v, L, v->d_vc, v->vc_bytes);
             hipGraph_t g = nullptr;
             hipError_t ce = hipStreamEndCapture(v->st, &g);
             if (e) { if (g) hipGraphDestroy(g); return e; }
@@ -521,12 +547,14 @@ static int round_once(p4g_server_s* v) {
             HIP_TRY(hipGraphInstantiate(&v->gexec, g, nullptr, nullptr, 0));
             v->graph_key = key;
             v->captures++;
-        } else {
-            for (const BLaunch& L : vseq) P4G_TRY(blaunch(v, L, v->d_vc));
         }
     }
-    if (v->use_graph) HIP_TRY(hipGraphLaunch(v->gexec, v->st));
-    for (size_t q = nbefore; q < seq.size(); ++q) P4G_TRY(blaunch(v, seq[q], v->d_round));
+    if (want_graph && v->gexec) {
+        HIP_TRY(hipGraphLaunch(v->gexec, v->st));
+    } else {
+        for (const BLaunch& L : v->vseq) P4G_TRY(blaunch(v, L, v->d_vc, v->vc_bytes));
+    }
+    for (size_t q = nbefore; q < seq.size(); ++q) P4G_TRY(blaunch(v, seq[q], v->d_round, v->round_bytes));
     HIP_TRY(hipMemcpyAsync(v->hS, v->S, sizeof(double) * 32 * v->max_slots, hipMemcpyDeviceToHost, v->st));
     HIP_TRY(hipStreamSynchronize(v->st));
     v->rounds++;
@@ -657,6 +685,7 @@ int p4g_server_create(size_t hbm_budget_bytes, const char* options, p4g_Server* 
     p4g_server_s* v = new p4g_server_s;
     v->budget = hbm_budget_bytes;
     v->use_graph = o.getb("p4g_serve_graph", true);
+    v->graph_after = std::max(0, o.geti("p4g_serve_graph_after", 3));
     v->max_slots = std::max(1, std::min(1024, o.geti("p4g_serve_max_slots", 64)));
     hipGetDevice(&v->device);
     int e = 0;

@@ -22,6 +22,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <memory>
 #include <mutex>
 #include <type_traits>
@@ -103,6 +104,19 @@ struct LevelInfo {
 template <typename T>
 static int sor_run(const LevelInfo& L, const double* g27, const T* r, T* z, hipStream_t st) {
     if constexpr (std::is_same<T, double>::value) {
+        if (L.dim == 3 && !L.gal && !L.var && L.pipe && L.pipe->flags && !L.pipe->pk && L.sor_nblk <= 1) {
+            // 3D 7-point level: pipelined strips, one launch per sweep direction
+            SorPipe* P = L.pipe;
+            const unsigned nwg = (unsigned)(((L.my + 63) / 64) * L.mz);
+            const unsigned long long b0 = (++P->epoch) << 22;
+            k_sor7_strips<double><<<nwg, 64, 0, st>>>(L.mx, L.my, L.mz, L.d, L.db, L.sx, L.sy, L.sz, r, z, P->flags,
+                                                       b0, 0, P->err, P->spin_limit, P->need_extra);
+            const unsigned long long b1 = (++P->epoch) << 22;
+            k_sor7_strips<double><<<nwg, 64, 0, st>>>(L.mx, L.my, L.mz, L.d, L.db, L.sx, L.sy, L.sz, r, z, P->flags,
+                                                       b1, 1, P->err, P->spin_limit, P->need_extra);
+            LAUNCH_CHECK();
+            return 0;
+        }
         if (L.gal && L.dim == 2 && L.pipe && L.pipe->pk && L.sor_nblk <= 1 && g27) {   // one launch per direction
             const int ns = (L.my + 63) / 64;
             SorPipe* P = L.pipe;
@@ -397,6 +411,11 @@ struct Engine {
     // (k_stencil_lds): -1 on levels of 400 or more points per side, 1 on
     // every level whose rows fill the waves, 0 never (-p4g_lds)
     int lds = -1;
+    // the residual and restriction of a cycle in one pass (-p4g_resid_restrict)
+    int resid_restrict_mode = -1;
+    // the prolongation and the first of several post-smoothing steps in one
+    // pass (-p4g_post_fuse)
+    int post2_mode = -1;
     // per level: 27-point Galerkin coefficients (FP64, device) or null
     std::vector<const double*> g27;
     bool fmg = false;          // full multigrid: an F-cycle per apply
@@ -411,6 +430,12 @@ struct Engine {
     // zero at active vertices; null otherwise
     const double* vidiag = nullptr;
     const double* ilu9 = nullptr;   // a 2D box-pattern ILU(0) factor of the finest level, when set
+    // a matrix-free operator on the finest level (-snes_mf_operator): y = A u
+    // formed by mf_op into mf_tmp, the pass's output applied to it; the
+    // level's 27-plane table is then only the preconditioning matrix
+    std::function<int(const T*, T*)> mf_op;
+    T* mf_tmp = nullptr;
+    bool mf_bypass = false;   // a pass on the preconditioning matrix (PCApplyRichardson_SOR's sweeps)
     bool vi = false;   // levels may become null-row Galerkin levels later (full temporaries)
     int mcycles = 1;        // -pc_mg_multiplicative_cycles
     bool kaskade = false;   // -pc_mg_type kaskade
@@ -535,10 +560,37 @@ struct Engine {
         return lds == 1 || L.mz >= 400;
     }
 
+    // a pass of the finest level under a matrix-free operator: the product,
+    // then the output per point (plain sources only: the operand must exist)
+    template <class Src, class Out>
+    int mf_stencil(Src src, Out out, int cls, double passes, long* nb, bool jac, double* pbuf) {
+        const LevelInfo& L = (*lv)[nlev - 1];
+        const T* u = nullptr;
+        const T* au = nullptr;
+        if constexpr (std::is_same<Src, SrcArr<T>>::value) {
+            P4G_TRY(mf_op(src.x, mf_tmp));
+            u = src.x;
+            au = mf_tmp;
+        } else if constexpr (!std::is_same<Src, SrcZero<T>>::value) {
+            return set_error(PETSC_ERR_LIB, "internal: a fused source over the matrix-free operator");
+        }
+        if (!mf_tmp || g27.size() < (size_t)nlev || !g27[nlev - 1])
+            return set_error(PETSC_ERR_LIB, "internal: matrix-free level without its buffers");
+        const int gb = vblocks(L.n);
+        if (nb) *nb = gb;
+        double* part = pbuf ? pbuf : cx->part;
+        hipStream_t s = cx->stream;
+        const double* gp = g27[nlev - 1];
+        return cx->launch(cls, true, passes * L.n * sizeof(T), [&] {
+            k_out_au<T, Out><<<gb, 256, 0, s>>>(L.n, u, au, gp, jac ? 1 : 0, out, part);
+        });
+    }
+
     // one marching stencil pass on level l
     template <class Src, class Out>
     int stencil(int l, Src src, Out out, int cls, double passes, long* nb, bool jac, double* pbuf = nullptr) {
         const LevelInfo& L = (*lv)[l];
+        if (mf_op && !mf_bypass && l == nlev - 1) return mf_stencil(src, out, cls, passes, nb, jac, pbuf);
         if (L.gal) {   // a Galerkin level: the 27-point pass
             const int gb = vblocks(L.n);
             if (nb) *nb = gb;
@@ -587,6 +639,29 @@ struct Engine {
         });
     }
 
+    // the fused residual + restriction (k_resid_restrict_lds) on level l:
+    // 3D constant coefficient, rows of 128 fine points, -p4g_resid_restrict
+    // (auto: where the LDS-staged passes run)
+    bool rr_fusable(const LevelInfo& L) const {
+        if (L.dim != 3 || L.var || L.gal || resid_restrict_mode == 0 || (L.mx - 1) % 128 != 0) return false;
+        return resid_restrict_mode == 1 || lds_level(L);
+    }
+
+    // b_c = P^T (b - A x), level l -> l-1, the fine residual not stored
+    int resid_restrict(int l, const T* xx, const T* bb, T* bc) {
+        const LevelInfo& F = (*lv)[l];
+        const LevelInfo& C = (*lv)[l - 1];
+        constexpr int TYC = 4;
+        const int kch = kch_restrict(C.dim, C.my, C.mz);
+        const dim3 g((C.mx - 1 + TX - 1) / TX, (C.my + TYC - 1) / TYC, (C.mz + kch - 1) / kch), b(TX, TYC);
+        hipStream_t s = cx->stream;
+        OpConst<T> o = opc(F, false);
+        return cx->launch(P4G_K_RESID_ZG, l == nlev - 1, double(2 * F.n + C.n) * sizeof(T), [&] {
+            k_resid_restrict_lds<T, OpConst<T>, TYC><<<g, b, 0, s>>>(F.mx, F.my, F.mz, C.mx, C.my, C.mz, kch, o, xx, bb,
+                                                                    bc);
+        });
+    }
+
     int restrict_(int l, const T* rf, T* bc) {   // level l -> l-1
         const LevelInfo& F = (*lv)[l];
         const LevelInfo& C = (*lv)[l - 1];
@@ -625,6 +700,18 @@ struct Engine {
             });
         }
         OpConst<T> o = opc(L, jac);
+        if constexpr (std::is_same<T, double>::value) {
+            if (lds_level(L)) {   // LDS-staged form: blocks of TY output rows
+                const dim3 gl(g.x, (L.my + TY - 1) / TY, (L.mz + kch - 1) / kch), bl(TX, TY);
+                if (nb) *nb = nblocks(gl);
+                if (Out::NDOT && nblocks(gl) > cx->npart_cap)
+                    return set_error(PETSC_ERR_LIB, "internal: partial-sum buffer too small");
+                return cx->launch(P4G_K_SMOOTH, l == nlev - 1, bytes, [&] {
+                    k_prolong_smooth_lds<OpConst<T>, ZG, Out, TY><<<gl, bl, 0, s>>>(L.mx, L.my, L.mz, C.mx, C.my, kch,
+                                                                                   o, base, cz, xc, out, cx->part);
+                });
+            }
+        }
         return cx->launch(P4G_K_SMOOTH, l == nlev - 1, bytes, [&] {
             if (L.dim == 3)
                 k_prolong_smooth<T, 3, OpConst<T>, ZG, Out><<<g, b, 0, s>>>(L.mx, L.my, L.mz, C.mx, C.my, kch, o, base,
@@ -790,10 +877,16 @@ struct Engine {
         const T* pm1 = x0;          // iterate m-1 (nullptr: zero)
         const T* pm2 = nullptr;     // iterate m-2
         const int gb = vblocks(n);
+        // Richardson with SOR and scale 1 is PETSc's PCApplyRichardson_SOR:
+        // sweeps on the PC's matrix, so a matrix-free operator is not used
+        const bool pmat_resid = smoother == SM_RICH && rich_scale == 1.0 && !lilu;
         for (int m = 1; m <= nu; ++m) {
             const T* rr = b;
             if (pm1) {
-                P4G_TRY(stencil(l, SrcArr<T>{pm1}, OutResid<T, 0>{b, r}, P4G_K_SMOOTH, 3, nullptr, false));
+                mf_bypass = pmat_resid;
+                const int e = stencil(l, SrcArr<T>{pm1}, OutResid<T, 0>{b, r}, P4G_K_SMOOTH, 3, nullptr, false);
+                mf_bypass = false;
+                P4G_TRY(e);
                 rr = r;
             }
             if (lilu) P4G_TRY(ilu_run<T>(L, ilu_u.size() > (size_t)l ? ilu_u[l] : nullptr, rr, z, st));
@@ -1001,7 +1094,7 @@ struct Engine {
         const double idg = jac ? 1.0 / L.d : 1.0;
         const double s = smoother == SM_CHEB ? 2.0 / (L.emax + L.emin) : rich_scale;
         // pre-smoothing and residual
-        bool zgf = false;
+        bool zgf = false, restricted = false;
         T* xpre = nullptr;
         if (l == nlev - 1 && pend.on && !(xzero && pend_fusable(L))) P4G_TRY(pend_flush());
         if (xzero && nu == 1 && zg_fusable(L)) {
@@ -1025,11 +1118,16 @@ struct Engine {
             xpre = d.t[3] ? d.t[3] : d.t[0];
             if (xpre == x && !xzero) return set_error(PETSC_ERR_LIB, "internal buffer aliasing");
             P4G_TRY(smooth(l, b, xzero ? nullptr : x, xpre, false, nullptr));
-            P4G_TRY(stencil(l, SrcArr<T>{xpre}, OutResid<T, 0>{b, d.r}, P4G_K_RESID_ZG, 3, nullptr, jac));
+            if (rr_fusable(L)) {   // residual and restriction in one pass, the fine residual not stored
+                P4G_TRY(resid_restrict(l, xpre, b, dv[l - 1].b));
+                restricted = true;
+            } else {
+                P4G_TRY(stencil(l, SrcArr<T>{xpre}, OutResid<T, 0>{b, d.r}, P4G_K_RESID_ZG, 3, nullptr, jac));
+            }
         }
         // coarse-grid correction
         Dev& c = dv[l - 1];
-        P4G_TRY(restrict_(l, d.r, c.b));
+        if (!restricted) P4G_TRY(restrict_(l, d.r, c.b));
         for (int q = 0; q < cycles; ++q) P4G_TRY(mcycle(l - 1, c.b, c.x, q == 0, false, nullptr));
         // x = pre + P x_c, then post-smoothing into x
         T* x0 = zgf ? d.t[0] : xpre;
@@ -1040,9 +1138,34 @@ struct Engine {
             else P4G_TRY(post_first<false>(l, zgf, pre, T(s * idg), c.x, b, x, T(s), nb));
             return 0;
         }
+        if (post2_fusable(L) && !zgf) {
+            // the first of several Chebyshev post-smoothing steps fused with the
+            // prolongation; t = pre + P x_c stored for the next step's recurrence
+            T *tt = nullptr, *y1 = nullptr;
+            for (int q = 0; q < 5; ++q) {
+                T* u = d.t[q];
+                if (!u || u == pre || u == b || u == x) continue;
+                if (!tt) tt = u;
+                else if (!y1) y1 = u;
+            }
+            if (!tt || !y1) return set_error(PETSC_ERR_LIB, "internal: smoother buffers exhausted");
+            typedef OutKeepU<T, OutCheb<T, false, 0>> OK;
+            const OK ok{tt, OutCheb<T, false, 0>{b, nullptr, y1, T(0), T(1), T(s)}};
+            P4G_TRY(prolong_smooth<false>(l, pre, T(0), c.x, ok, 4.0, nullptr, jac));
+            return smooth(l, b, y1, x, dots, nb, 2, tt);
+        }
         P4G_TRY(prolong(l, zgf, pre, T(s * idg), c.x, x0));
         P4G_TRY(smooth(l, b, x0, x, dots, nb));
         return 0;
+    }
+
+    // prolongation + first post-smoothing step in one pass for two or more
+    // Chebyshev steps (the prolonged iterate is stored for the recurrence):
+    // FP64 3D levels of the LDS-staged passes, -p4g_post_fuse auto|1|0
+    bool post2_fusable(const LevelInfo& L) const {
+        if (!std::is_same<T, double>::value || L.dim != 3 || L.var || L.gal || sor || vidiag) return false;
+        if (smoother != SM_CHEB || nu < 2 || post2_mode == 0 || !lds_level(L)) return false;
+        return true;
     }
 
     // PCMG type full (PETSc's PCMGFCycle_Private): b restricted through every
@@ -1610,7 +1733,10 @@ struct p4g_solver_s {
     int coarse_its = 0, coarse_pc = 0;   // -mg_coarse_ksp_type cg: iterations and PC (0 none, 1 jacobi, 2 ilu)
     int sor_blocks = 1;               // -p4g_sor_blocks: local sweeps of that many ranks
     int lds = -1;                     // -p4g_lds auto|1|0: LDS-staged 3D stencil passes
+    int resid_restrict = -1;          // -p4g_resid_restrict auto|1|0: fused residual + restriction
+    int post_fuse = -1;               // -p4g_post_fuse auto|1|0: prolongation + first of several post steps
     bool sor_test_stall = false;      // -p4g_sor_test_stall: every pipelined-SOR wait gives up at once
+    bool sor_pipeline = true;         // -p4g_sor_pipeline 1|0: 3D SOR by pipelined strips (else hyperplanes)
     // reduced-space VI Newton (p4g_vi_solve): SNES options and state
     double snes_rtol = 1e-8, snes_atol = 1e-50, snes_stol = 1e-8, snes_divtol = 1e4;
     int snes_maxit = 50;
@@ -1907,7 +2033,12 @@ static int estimate_level(p4g_solver_s* s, int l) {
         if (it > 0) { beta = rz / rzold; be[it - 1] = beta; }
         s->cx.hS[16 + S_BCOEF] = beta;
         HIP_TRY(hipMemcpyAsync(S + S_BCOEF, &s->cx.hS[16 + S_BCOEF], sizeof(double), hipMemcpyHostToDevice, st));
-        if (L.gal)
+        if (s->e64 && s->e64->mf_op && l == s->nlev - 1) {   // matrix-free: p = z + beta p, w = A p, (p, w)
+            k_comb3<double><<<gb, 256, 0, st>>>(n, 0.0, nullptr, beta, pold, 1.0, z, pnew);
+            LAUNCH_CHECK();
+            P4G_TRY(s->e64->mf_op(pnew, w));
+            k_dot2<double><<<gb, 256, 0, st>>>(n, pnew, w, s->cx.part);
+        } else if (L.gal)
             k_stencil27<double, SrcDir<double>, OutApplyDot<double>><<<vblocks(n), 256, 0, st>>>(
                 og, SrcDir<double>{z, pold, pnew, S + S_BCOEF}, OutApplyDot<double>{w}, s->cx.part);
         else if (L.var)
@@ -2024,6 +2155,7 @@ static int configure(p4g_solver_s* s) {
             return set_error(PETSC_ERR_SUP, "the device SOR smoother has omega 1 and one sweep");
         s->sor_blocks = o.geti("p4g_sor_blocks", 1);
         s->sor_test_stall = o.getb("p4g_sor_test_stall", false);
+        s->sor_pipeline = o.getb("p4g_sor_pipeline", true);
         if (s->sor_blocks < 1 || s->sor_blocks > 16)
             return set_error(PETSC_ERR_ARG_OUTOFRANGE, "-p4g_sor_blocks must be in 1..16");
     }
@@ -2066,6 +2198,8 @@ static int configure(p4g_solver_s* s) {
             return 0;
         };
         P4G_TRY(tri(o.gets("p4g_lds", "auto"), "p4g_lds", s->lds));
+        P4G_TRY(tri(o.gets("p4g_resid_restrict", "auto"), "p4g_resid_restrict", s->resid_restrict));
+        P4G_TRY(tri(o.gets("p4g_post_fuse", "auto"), "p4g_post_fuse", s->post_fuse));
         const std::string both = o.gets("p4g_fuse_cg", "auto");
         P4G_TRY(tri(both, "p4g_fuse_cg", s->fuse_r));
         P4G_TRY(tri(o.gets("p4g_fuse_r", both), "p4g_fuse_r", s->fuse_r));
@@ -2249,6 +2383,21 @@ int solver_setup(p4g_solver_s* s) {
     const double t0 = now_ms();
     P4G_TRY(build_levels(s));
     sor_splits(s);
+    // pipelined 3D SOR sweeps (k_sor7_strips): progress flags per level
+    if (s->pctype == PCT_MG && s->sor && !s->lilu && s->sor_pipeline && !s->vi_mode && !s->ms_mode) {
+        s->ms_pipes.resize(s->nlev);
+        for (int l = 1; l < s->nlev; ++l) {
+            LevelInfo& L = s->lv[l];
+            if (L.dim != 3 || L.sor_nblk > 1) continue;
+            const long nf = (long)((L.my + 63) / 64) * L.mz;
+            SorPipe& P = s->ms_pipes[l];
+            const size_t bytes = (nf + 1) * sizeof(unsigned long long) + sizeof(int);
+            P4G_TRY(dalloc(s, (void**)&P.flags, bytes));
+            HIP_TRY(hipMemset(P.flags, 0, bytes));
+            P.err = (int*)(P.flags + nf + 1);
+            L.pipe = &P;
+        }
+    }
     // ILU(0) pivots: the finest level for -pc_type ilu, the smoothing levels
     // for -mg_levels_pc_type ilu (computed once on the host, in the factor's
     // own row order)
@@ -2311,6 +2460,8 @@ int solver_setup(p4g_solver_s* s) {
         e->coarse_pc = s->coarse_pc;
         e->ilu_u.assign(s->ilu_u.begin(), s->ilu_u.end());
         e->lds = s->lds;
+        e->resid_restrict_mode = s->resid_restrict;
+        e->post2_mode = s->post_fuse;
     };
     if (F.var) {
         const size_t nc = F.kcell.size();
@@ -2614,8 +2765,12 @@ static int ksp_views(p4g_solver_s* s, const double* b, double* x) {
     return 0;
 }
 
+static int sor_pipes_reset(p4g_solver_s* s);
+static int sor_pipes_check(p4g_solver_s* s, const char* who);
+
 int ksp_solve_device(p4g_solver_s* s, const double* b, double* x) {
     P4G_TRY(solver_setup(s));
+    P4G_TRY(sor_pipes_reset(s));
     hipStream_t st = s->cx.stream;
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0));
@@ -2646,6 +2801,7 @@ int ksp_solve_device(p4g_solver_s* s, const double* b, double* x) {
     hipEventDestroy(e1);
     R.solve_ms = ms;
     s->cx.tm.collect();
+    P4G_TRY(sor_pipes_check(s, "KSPSolve"));
     double rel = 0.0;
     P4G_TRY(true_residual(s, b, x, s->mixed ? s->xb : s->e64->w, &rel));
     R.true_relres = rel;

@@ -20,6 +20,11 @@ struct ViRun {
     double *F0 = nullptr, *X = nullptr, *F = nullptr, *Y = nullptr, *W = nullptr, *G = nullptr,
            *Fm = nullptr, *xl = nullptr, *xu = nullptr;
     double* S = nullptr;   // scalar block: the second half of cx.S
+    // -snes_type vinewtonssls: the merit function is |phi| (phi the Fischer-
+    // Burmeister function of X), a trial point's phi goes to G and its
+    // residual to FW
+    bool ssls = false;
+    double *phi = nullptr, *FW = nullptr;
 
     int sums(double& a, double& b) {
         P4G_TRY(s->cx.finalize(nb, S, FIN_RAW, 0));
@@ -54,10 +59,18 @@ struct ViRun {
         yn = std::sqrt(b);
         return 0;
     }
-    // w = proj(x0 - lambda y), gv = F(w), gn = |gv| on w's inactive set
+    // w = proj(x0 - lambda y), gv = F(w), gn = |gv| on w's inactive set;
+    // SSLS: FW = F(w), gv = phi(w), gn = |gv|
     int trial(const double* x0, const double* y, double lambda, double* w, double* gv, double& gn) {
         k_vi_trial<<<nb, 256, 0, st>>>(n, x0, y, lambda, xl, xu, w);
         LAUNCH_CHECK();
+        if (ssls) {
+            P4G_TRY(fun(w, FW, true));
+            k_ssls_phi<<<nb, 256, 0, st>>>(n, w, xl, FW, gv);
+            LAUNCH_CHECK();
+            double t;
+            return dot(gv, gv, t, gn);
+        }
         P4G_TRY(fun(w, gv, true));
         double xn;
         return norms(w, gv, gn, xn);
@@ -96,6 +109,23 @@ static int vi_operator(p4g_solver_s* s, ViRun& R) {
     return 0;
 }
 
+// the semismooth Jacobian diag(Db) J + diag(Da) at the current X (SSLS) as
+// the finest level's 27-point operator, and its ILU(0) pivots
+static int ssls_operator(p4g_solver_s* s, ViRun& R) {
+    const int L = s->nlev - 1;
+    Engine<double>& e = *s->e64;
+    if ((int)s->g27.size() != s->nlev) s->g27.assign(s->nlev, nullptr);
+    if ((int)e.g27.size() != s->nlev) e.g27.assign(s->nlev, nullptr);
+    k_ssls_op27<<<R.nb, 256, 0, R.st>>>(R.g, R.X, R.xl, R.F, s->vi_g27);
+    LAUNCH_CHECK();
+    s->lv[L].gal = true;
+    s->g27[L] = s->vi_g27;
+    e.g27[L] = s->vi_g27;
+    e.vidiag = nullptr;
+    if (s->pctype == PCT_ILU) return ilu27_pivots(s->lv[L], s->vi_g27, s->ilu_u[L], R.st);
+    return 0;
+}
+
 // SNESConvergedDefault (SNESConvergedReason numbering)
 static int vi_converged(const p4g_solver_s* s, int it, double xnorm, double snorm, double fnorm, double& ttol,
                         double& rnorm0) {
@@ -112,7 +142,10 @@ static int vi_converged(const p4g_solver_s* s, int it, double xnorm, double snor
 
 // SNESLineSearchApply_BT with alpha = 0 (or _Basic).  On return X, F hold
 // the accepted point (pointers swapped with the trial buffers), fnorm its
-// norm, ynorm = lambda |Y|; returns 1 when the search failed.
+// norm, ynorm = lambda |Y|; returns 1 when the search failed.  SSLS: the
+// function is phi, its slope (phi, Jss Y) with the semismooth Jacobian, and
+// X, phi, F move to the accepted point.  The trial points are projected
+// onto the bounds in both methods.
 static int vi_line_search(p4g_solver_s* s, ViRun& R, double& fnorm, double& xnorm, double& ynorm, int* fail) {
     const double alpha = 0.0, steptol = 1e-12;
     double lambda = 1.0, ydot;
@@ -125,9 +158,14 @@ static int vi_line_search(p4g_solver_s* s, ViRun& R, double& fnorm, double& xnor
     } else {
         if (ynorm == 0.0) { *fail = 1; return 0; }
         // initial slope of |F|^2/2 along -Y: -(F, J Y)
-        P4G_TRY(R.fun(R.Y, R.W, false));
         double initslope, wn;
-        P4G_TRY(R.dot(R.F, R.W, initslope, wn));
+        if (R.ssls) {
+            P4G_TRY(s->e64->mat_mult(R.Y, R.W));
+            P4G_TRY(R.dot(R.phi, R.W, initslope, wn));
+        } else {
+            P4G_TRY(R.fun(R.Y, R.W, false));
+            P4G_TRY(R.dot(R.F, R.W, initslope, wn));
+        }
         if (initslope > 0.0) initslope = -initslope;
         if (initslope == 0.0) initslope = -1.0;
         const double f = fnorm * fnorm;
@@ -174,7 +212,12 @@ static int vi_line_search(p4g_solver_s* s, ViRun& R, double& fnorm, double& xnor
         *fail = ok ? 0 : 1;
     }
     std::swap(R.X, R.W);
-    std::swap(R.F, R.G);
+    if (R.ssls) {
+        std::swap(R.phi, R.G);
+        std::swap(R.F, R.FW);
+    } else {
+        std::swap(R.F, R.G);
+    }
     fnorm = gnorm;
     double xx;
     P4G_TRY(R.dot(R.X, R.X, xx, xnorm));
@@ -187,8 +230,13 @@ static int vi_solve(p4g_solver_s* s, const double* xl, const double* xu, double*
     const double t0 = now_ms();
     p4g_da_s* da = s->da;
     if (!da->fn) return set_error(PETSC_ERR_ARG_WRONG, "no residual callback: call p4g_da_set_function_local");
-    if (!s->snes_type.empty() && s->snes_type != "vinewtonrsls")
-        return set_error(PETSC_ERR_SUP, "-snes_type %s: p4g_vi_solve implements vinewtonrsls", s->snes_type.c_str());
+    if (!s->snes_type.empty() && s->snes_type != "vinewtonrsls" && s->snes_type != "vinewtonssls")
+        return set_error(PETSC_ERR_SUP, "-snes_type %s: p4g_vi_solve implements vinewtonrsls and vinewtonssls",
+                         s->snes_type.c_str());
+    const bool ssls = s->snes_type == "vinewtonssls";
+    if (ssls && (s->pctype == PCT_MG || s->pctype == PCT_JACOBI || xu))
+        return set_error(PETSC_ERR_SUP, "p4g_vi_solve: vinewtonssls is implemented for lower bounds with -pc_type "
+                                        "ilu (the default) or none");
     if (s->ls_type != "bt" && s->ls_type != "basic")
         return set_error(PETSC_ERR_SUP, "-snes_linesearch_type %s is not implemented (bt, basic)", s->ls_type.c_str());
     if (s->mixed) return set_error(PETSC_ERR_SUP, "p4g_vi_solve: mixed precision is not implemented");
@@ -223,7 +271,7 @@ static int vi_solve(p4g_solver_s* s, const double* xl, const double* xu, double*
     R.g.dim = FL.dim; R.g.mx = FL.mx; R.g.my = FL.my; R.g.mz = FL.mz; R.g.n = n;
     R.g.d = FL.d; R.g.db = FL.db; R.g.sx = FL.sx; R.g.sy = FL.sy; R.g.sz = FL.sz;
     if (!s->vi_buf) {
-        P4G_TRY(dalloc(s, (void**)&s->vi_buf, 9 * n * sizeof(double)));
+        P4G_TRY(dalloc(s, (void**)&s->vi_buf, 11 * n * sizeof(double)));
         P4G_TRY(dalloc(s, (void**)&s->vi_act, n));
         P4G_TRY(dalloc(s, (void**)&s->vi_g27, 27 * n * sizeof(double)));
         // the Chebyshev bounds are re-estimated every Newton step: keep their
@@ -249,6 +297,8 @@ static int vi_solve(p4g_solver_s* s, const double* xl, const double* xu, double*
     double* b = s->vi_buf;
     R.F0 = b; R.X = b + n; R.F = b + 2 * n; R.Y = b + 3 * n; R.W = b + 4 * n; R.G = b + 5 * n;
     R.Fm = b + 6 * n; R.xl = b + 7 * n; R.xu = xu ? b + 8 * n : nullptr;
+    R.ssls = ssls;
+    R.phi = b + 9 * n; R.FW = b + 10 * n;
     {   // F(0) by the residual callback: F(x) = J x + F(0) for this linear residual
         std::vector<double> z(n, 0.0), F0(n, 0.0);
         std::vector<double*> r1, r2;
@@ -266,12 +316,21 @@ static int vi_solve(p4g_solver_s* s, const double* xl, const double* xu, double*
     s->vi_fhist.clear();
     s->vi_kits.clear();
     s->vi_kreason.clear();
-    // SNESVIProjectOntoBounds, F(X), the inactive-set norm
-    k_vi_trial<<<R.nb, 256, 0, R.st>>>(n, R.W, nullptr, 0.0, R.xl, R.xu, R.X);
-    LAUNCH_CHECK();
-    P4G_TRY(R.fun(R.X, R.F, true));
     double fnorm, xnorm, ynorm = 0.0, ttol = 0.0, rnorm0 = 0.0;
-    P4G_TRY(R.norms(R.X, R.F, fnorm, xnorm));
+    if (ssls) {   // F(X), phi(X), |phi| (no projection of the initial iterate)
+        HIP_TRY(hipMemcpyAsync(R.X, R.W, n * sizeof(double), hipMemcpyDeviceToDevice, R.st));
+        P4G_TRY(R.fun(R.X, R.F, true));
+        k_ssls_phi<<<R.nb, 256, 0, R.st>>>(n, R.X, R.xl, R.F, R.phi);
+        LAUNCH_CHECK();
+        double t;
+        P4G_TRY(R.dot(R.phi, R.phi, t, fnorm));
+        P4G_TRY(R.dot(R.X, R.X, t, xnorm));
+    } else {   // SNESVIProjectOntoBounds, F(X), the inactive-set norm
+        k_vi_trial<<<R.nb, 256, 0, R.st>>>(n, R.W, nullptr, 0.0, R.xl, R.xu, R.X);
+        LAUNCH_CHECK();
+        P4G_TRY(R.fun(R.X, R.F, true));
+        P4G_TRY(R.norms(R.X, R.F, fnorm, xnorm));
+    }
     p4g_VIReport out;
     std::memset(&out, 0, sizeof out);
     out.fnorm0 = fnorm;
@@ -280,15 +339,20 @@ static int vi_solve(p4g_solver_s* s, const double* xl, const double* xu, double*
     int reason = vi_converged(s, 0, 0.0, 0.0, fnorm, ttol, rnorm0);
     int i = 0, iter = 0;
     for (; !reason && i < s->snes_maxit; ++i) {
-        // active set, reduced right-hand side, reduced operator
-        k_vi_sets<<<R.nb, 256, 0, R.st>>>(n, R.X, R.F, R.xl, R.xu, s->vi_act, R.Fm, s->cx.part);
-        LAUNCH_CHECK();
-        P4G_TRY(vi_operator(s, R));
+        const double* rhs = R.Fm;
+        if (ssls) {   // the semismooth Jacobian at X; Jss Y = phi
+            P4G_TRY(ssls_operator(s, R));
+            rhs = R.phi;
+        } else {      // active set, reduced right-hand side, reduced operator
+            k_vi_sets<<<R.nb, 256, 0, R.st>>>(n, R.X, R.F, R.xl, R.xu, s->vi_act, R.Fm, s->cx.part);
+            LAUNCH_CHECK();
+            P4G_TRY(vi_operator(s, R));
+        }
         typename Engine<double>::CGOut co;
         if (s->ksp_gmres)
-            P4G_TRY(s->e64->gmres(R.Fm, R.Y, s->rtol, s->atol, s->dtol, s->maxit, s->gmres_restart, co, s->cx.S));
+            P4G_TRY(s->e64->gmres(rhs, R.Y, s->rtol, s->atol, s->dtol, s->maxit, s->gmres_restart, co, s->cx.S));
         else
-            P4G_TRY(s->e64->cg(R.Fm, R.Y, s->normtype, s->rtol, s->atol, s->dtol, s->maxit, co, s->cx.S));
+            P4G_TRY(s->e64->cg(rhs, R.Y, s->normtype, s->rtol, s->atol, s->dtol, s->maxit, co, s->cx.S));
         HIP_TRY(hipStreamSynchronize(R.st));
         P4G_TRY(sor_pipes_check(s, "p4g_vi_solve"));
         s->vi_kits.push_back(co.its);
@@ -308,8 +372,15 @@ static int vi_solve(p4g_solver_s* s, const double* xl, const double* xu, double*
         reason = vi_converged(s, iter, xnorm, ynorm, fnorm, ttol, rnorm0);
     }
     if (!reason && i >= s->snes_maxit) reason = -5;   // DIVERGED_MAX_IT
-    // the size of the last active set
-    {
+    // the size of the last active set (SSLS: {x <= xl + 1e-8 and F > 0} at the solution)
+    if (ssls) {
+        std::vector<double> hx(n), hf(n), hl(n);
+        HIP_TRY(hipStreamSynchronize(R.st));
+        HIP_TRY(hipMemcpy(hx.data(), R.X, n * sizeof(double), hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(hf.data(), R.F, n * sizeof(double), hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(hl.data(), R.xl, n * sizeof(double), hipMemcpyDeviceToHost));
+        for (long q = 0; q < n; ++q) out.active += (hx[q] <= hl[q] + 1.0e-8 && hf[q] > 0.0) ? 1 : 0;
+    } else {
         std::vector<unsigned char> act(n);
         HIP_TRY(hipStreamSynchronize(R.st));
         if (!s->vi_kits.empty()) {

@@ -11,10 +11,15 @@ The oracle is pinned by
     residual, and for q = 0 against the 5-point Laplacian it must equal;
   * GMRES against the minimizer of the preconditioned residual over the
     Krylov space, computed by numpy.
-minimal.test3 needs -snes_mf_operator (a matrix-free Jacobian with the
-Poisson matrix as preconditioning material), which neither the oracle nor
-the device implements: both refuse it.
+  * minimal.test3: -snes_mf_operator (the matrix-free finite-difference
+    Jacobian as the operator, the registered Poisson Jacobian as
+    preconditioning material) under PCMG on two processes, whose y-split
+    local SOR sweeps the oracle simulates (-oracle_nranks 2; one process
+    prints a different area digit);
+  * the matrix-free product against the colored Jacobian's product, and for
+    q = 0 (a linear residual) against the 5-point Laplacian.
 """
+import re
 import json
 import os
 
@@ -27,16 +32,76 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 GOLDEN = {t["test"]: t for t in json.load(open(os.path.join(HERE, "golden", "minimal_golden.json")))["tests"]}
 
 
-@pytest.mark.parametrize("num", [1, 2, 4])
+def _nranks(num, dev=False):
+    n = GOLDEN[num]["nprocs"]
+    if n <= 1:
+        return ""
+    return (" -p4g_sor_blocks %d" if dev else " -oracle_nranks %d") % n
+
+
+@pytest.mark.parametrize("num", [1, 2, 3, 4])
 def test_oracle_golden(num):
-    rc, text, res, _ = O.minimal_run(GOLDEN[num]["options"])
+    rc, text, res, _ = O.minimal_run(GOLDEN[num]["options"] + _nranks(num))
     assert rc == 0, text
     assert text == GOLDEN[num]["expected"]
 
 
-def test_oracle_refuses_mf_operator():
+def test_oracle_test3_needs_the_process_split():
+    # one process: the same Newton counts, but an area line differs in its
+    # last digit (the two processes' local SOR sweeps are a different smoother)
     rc, text, _, _ = O.minimal_run(GOLDEN[3]["options"])
-    assert rc == 56 and "snes_fd_color" in text
+    assert rc == 0, text
+    assert text != GOLDEN[3]["expected"]
+    got, want = text.splitlines(), GOLDEN[3]["expected"].splitlines()
+    assert len(got) == len(want)
+    assert [g for g in got if "Nonlinear" in g] == [w for w in want if "Nonlinear" in w]
+
+
+def test_oracle_refuses_pure_mf():
+    rc, text, _, _ = O.minimal_run("-snes_mf -da_refine 1")
+    assert rc == 56 and "snes_mf_operator" in text
+
+
+def test_mffd_product_is_the_jacobian_product():
+    mx = my = 17
+    c = 1.1
+    u = _catenoid(mx, my, c)
+    jj, ii = np.meshgrid(np.arange(my), np.arange(mx), indexing="ij")
+    inner = ((ii > 0) & (ii < mx - 1) & (jj > 0) & (jj < my - 1)).ravel()
+    u = u + 0.05 * np.sin(3.0 * ii.ravel() + 2.0 * jj.ravel()) * inner
+    a = np.random.default_rng(3).standard_normal(mx * my)
+    y = O.minimal_mffd(mx, my, 1, -0.5, 1.0, c, u, a)
+    J9 = O.minimal_fdjac(mx, my, 1, -0.5, 1.0, c, u)
+    Ja = np.zeros(mx * my)
+    offs = _offsets(mx)
+    for q in range(mx * my):
+        for k, o in enumerate(offs):
+            if J9[q, k] != 0.0:
+                Ja[q] += J9[q, k] * a[q + o]
+    assert np.max(np.abs(y - Ja)) <= 1e-5 * np.max(np.abs(Ja))
+    # the zero direction gives zero (MatMult_MFFD's zeroa)
+    assert not np.any(O.minimal_mffd(mx, my, 1, -0.5, 1.0, c, u, np.zeros(mx * my)))
+
+
+def test_mffd_q0_is_the_laplacian():
+    mx = my = 9
+    u = np.random.default_rng(4).standard_normal(mx * my)
+    a = np.random.default_rng(5).standard_normal(mx * my)
+    y = O.minimal_mffd(mx, my, 0, 0.0, 1.0, 1.1, u, a)
+    jj, ii = np.meshgrid(np.arange(my), np.arange(mx), indexing="ij")
+    ii, jj = ii.ravel(), jj.ravel()
+    want = np.zeros(mx * my)
+    for q in range(mx * my):
+        i, j = ii[q], jj[q]
+        if i in (0, mx - 1) or j in (0, my - 1):
+            want[q] = a[q]                    # F = u - g on the boundary
+            continue
+        s = 4.0 * a[q]                        # hx = hy: -(u_e - 2u_c + u_w) - (u_n - 2u_c + u_s)
+        for o, inside in ((1, i + 1 < mx - 1), (-1, i - 1 > 0), (mx, j + 1 < my - 1), (-mx, j - 1 > 0)):
+            if inside:                        # a boundary neighbour is g: no dependence on u
+                s -= a[q + o]
+        want[q] = s
+    assert np.max(np.abs(y - want)) <= 1e-6 * np.max(np.abs(want))
 
 
 def _catenoid(mx, my, c):
@@ -141,6 +206,15 @@ MS_CASES = {
     "catenoid_seq6_mg_sor_3strips": "-snes_grid_sequence 6 -snes_fd_color -pc_type mg",
     "tent_H10_basic_monitor": ("-da_refine 2 -ms_problem tent -ms_tent_H 10 -snes_linesearch_type basic "
                                "-snes_fd_color -ms_monitor -ms_quaddegree 2"),
+    # -snes_mf_operator: the matrix-free operator with the Poisson Jacobian as
+    # preconditioning material, on one or two (simulated) processes
+    "mf_catenoid_seq4_mg_2ranks": "-snes_mf_operator -pc_type mg -snes_grid_sequence 4 -oracle_nranks 2",
+    "mf_tent_seq3_mg": "-snes_mf_operator -pc_type mg -snes_grid_sequence 3 -ms_problem tent",
+    "mf_catenoid_17_w_richardson": ("-snes_mf_operator -pc_type mg -da_refine 3 -pc_mg_cycle_type w "
+                                    "-mg_levels_ksp_type richardson"),
+    "mf_catenoid_seq2_ilu_restart5": "-snes_mf_operator -snes_grid_sequence 2 -da_refine 1 -ksp_gmres_restart 5",
+    # neither option: Newton on the registered Poisson Jacobian
+    "poisson_jacobian_tent_H01_mg": "-ms_problem tent -ms_tent_H 0.1 -da_refine 3 -pc_type mg",
 }
 
 
@@ -148,8 +222,10 @@ MS_CASES = {
 @pytest.mark.parametrize("name", sorted(MS_CASES))
 def test_minimal_device_parity(name):
     args = MS_CASES[name] + " -snes_converged_reason -ksp_converged_reason -snes_monitor"
-    print("minimal case", name, args, flush=True)
-    rc, text, res, u = capi.minimal_run(args, want_u=True)
+    # the device reads -p4g_sor_blocks, the oracle -oracle_nranks
+    dev = re.sub(r"-oracle_nranks (\d+)", r"-p4g_sor_blocks \1", args)
+    print("minimal case", name, dev, flush=True)
+    rc, text, res, u = capi.minimal_run(dev, want_u=True)
     assert rc == 0, text
     orc, otext, ores, ou = O.minimal_run(args, want_u=True)
     assert orc == 0, otext
@@ -182,18 +258,18 @@ def _same_output(text, expected):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("num", [1, 2, 4])
+@pytest.mark.parametrize("num", [1, 2, 3, 4])
 def test_minimal_device_golden(num):
-    args = GOLDEN[num]["options"]
+    args = GOLDEN[num]["options"] + _nranks(num, dev=True)
     print("minimal golden", num, args, flush=True)
     rc, text, res, _ = capi.minimal_run(args)
     assert rc == 0, text
     assert _same_output(text, GOLDEN[num]["expected"]), text
-    if num in (2, 4):   # no printed norms: byte for byte
+    if num in (2, 3, 4):   # no printed norms: byte for byte
         assert text == GOLDEN[num]["expected"]
 
 
 @pytest.mark.gpu
-def test_minimal_device_refuses_mf_operator():
-    rc, text, _, _ = capi.minimal_run(GOLDEN[3]["options"])
+def test_minimal_device_refuses_pure_mf():
+    rc, text, _, _ = capi.minimal_run("-snes_mf -da_refine 1")
     assert rc == 56 and "snes_mf_operator" in text

@@ -8,9 +8,14 @@ Oracle pins:
   * obstacle.test3 byte for byte (grid sequencing, Chebyshev/SOR multigrid
     on the inactive sets; its 3x3 grid is a one-level multigrid, the level
     smoother);
-  * the error lines of obstacle.test2/4 (GMRES+ASM on 4 ranks, the
-    semismooth method: not restated) are the 9x9 VI solution's, which the
-    oracle reproduces with its own solvers;
+  * obstacle.test4 byte for byte (-snes_type vinewtonssls: semismooth
+    Newton on the Fischer-Burmeister reformulation, grid-sequenced Newton
+    counts 4, 6, 5 and the last KSP count 6); the line search's trial points
+    are projected onto the bounds, and without that projection the oracle
+    prints other counts (checked);
+  * the error line of obstacle.test2 (GMRES+ASM on 4 ranks: not restated)
+    is the 9x9 VI solution's, which the oracle reproduces with its own
+    solvers;
   * an independent projected SOR solve of the discrete complementarity
     problem (numpy) on 17x17 and 33x33;
   * multigrid on the inactive sets converges in a grid-independent number
@@ -66,6 +71,23 @@ def test_oracle_golden_test3():
     assert rc == 0
     assert text == t["expected"]
     assert res.stage_its[:4] == [1, 2, 2, 3]
+
+
+def test_oracle_golden_test4_ssls():
+    t = GOLDEN[4]
+    rc, text, res, _ = O.obstacle_run(t["options"])
+    assert rc == 0, text
+    assert text == t["expected"]
+    assert res.stage_its[:3] == [4, 6, 5]
+
+
+def test_oracle_ssls_projection_is_what_test4_pins():
+    # the same method with unprojected trial points: other Newton and KSP counts
+    t = GOLDEN[4]
+    rc, text, res, _ = O.obstacle_run(t["options"] + " -oracle_ls_project 0")
+    assert rc == 0, text
+    assert text != t["expected"]
+    assert text.splitlines()[-1] == t["expected"].splitlines()[-1]   # the same VI solution
 
 
 @pytest.mark.parametrize("num", [2, 4])
@@ -203,6 +225,10 @@ CASES = {
     "65_ilu": "-da_refine 5 -ksp_rtol 1e-8",
     "seq3_ilu": "-snes_grid_sequence 3",
     "seq4_sor_w": "-snes_grid_sequence 4 -pc_type mg -pc_mg_cycle_type w",
+    # the semismooth method (SNESVINEWTONSSLS)
+    "ssls_33_ilu": "-snes_type vinewtonssls -da_refine 4",
+    "ssls_seq4_gmres": "-snes_type vinewtonssls -snes_grid_sequence 4 -ksp_type gmres",
+    "ssls_17_none": "-snes_type vinewtonssls -da_refine 3 -pc_type none",
 }
 SHOW = " -snes_monitor_short -ksp_converged_reason -snes_converged_reason"
 
@@ -268,6 +294,18 @@ def test_obstacle_device_golden_test3_default_smoother():
     from p4pdes_amd import capi
     t = GOLDEN[3]
     print("obstacle golden 3 default smoother", flush=True)
+    rc, text, res, _ = capi.obstacle_run(t["options"])
+    assert rc == 0, text
+    assert text == t["expected"]
+
+
+@pytest.mark.gpu
+def test_obstacle_device_golden_test4_ssls():
+    # obstacle.test4 as the reference runs it (-snes_type vinewtonssls, CG
+    # with PETSc's default ILU(0) on the semismooth Jacobian), byte for byte
+    from p4pdes_amd import capi
+    t = GOLDEN[4]
+    print("obstacle golden 4 ssls", flush=True)
     rc, text, res, _ = capi.obstacle_run(t["options"])
     assert rc == 0, text
     assert text == t["expected"]
