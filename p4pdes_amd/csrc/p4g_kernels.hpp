@@ -2789,7 +2789,9 @@ k_sor9_strips(int mx, int my, const double* __restrict__ pk, const double* __res
                 const unsigned long long need =
                     base + (unsigned long long)min(t == 0 ? SOR9_SLACK : t + 2, mx) + need_extra;
                 long spins = 0;
-                while (!dead && (kn = __hip_atomic_load(flag + s - 1, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT)) < need) {
+                // relaxed polls (an acquire per poll would invalidate the
+                // cache every time), one acquire fence once the count is there
+                while (!dead && (kn = __hip_atomic_load(flag + s - 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) < need) {
                     if (++spins > spin_limit) {
                         atomicExch(err, 1);
                         dead = true;
@@ -2799,6 +2801,7 @@ k_sor9_strips(int mx, int my, const double* __restrict__ pk, const double* __res
                     }
                     __builtin_amdgcn_s_sleep(1);
                 }
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
                 if (dead) kn = base + (unsigned long long)mx;
             }
             kn = __shfl(kn, 0, 64);
@@ -2853,11 +2856,10 @@ k_sor9_strips(int mx, int my, const double* __restrict__ pk, const double* __res
             x[q] = res;
             xo = res;
             // a full strip publishes its last row's progress every 32 points
-            if (r == 63 && ((c & 31) == 31 || c == mx - 1)) {
-                __threadfence();
+            // (the consumer reads this lane's row only: its own release store)
+            if (r == 63 && ((c & 31) == 31 || c == mx - 1))
                 __hip_atomic_store(flag + s, base + (unsigned long long)(c + 1), __ATOMIC_RELEASE,
                                    __HIP_MEMORY_SCOPE_AGENT);
-            }
         }
         fetch(t + SOR9_AHEAD, cur);   // this slot now holds step t + SOR9_AHEAD
         h3 = h2;
@@ -2884,12 +2886,14 @@ k_sor9_strips(int mx, int my, const double* __restrict__ pk, const double* __res
 // backward) needs the strip before it in its plane (lane 0's neighbour row)
 // and the same strip in the plane before it (every lane's neighbour plane):
 // both have lower indices.  Steps run in batches of SOR7_BATCH: before a
-// batch the first lane waits (agent-scope acquire) until those two have
+// batch the first lane waits (relaxed polls, then an agent-scope acquire
+// fence) until those two have
 // published enough steps, then every lane loads the batch's operands (b,
 // the neighbour plane's new values, lane 0's neighbour row; backward also
 // the three lower neighbours, which still hold forward values: the points
 // after a point in the backward order run later), computes the batch, fences
-// its stores, and the last lane publishes the steps done (release).
+// its stores (one agent-scope release fence), and the last lane publishes
+// the steps done.
 // Forward: x = (b + sz x_{k-1} + sy x_{j-1} + sx x_{i-1}) / d over interior
 // neighbours, in k_sor_plane's order; backward: (b + sz x_{k-1} + sy
 // x_{j-1} + sx x_{i-1} + sx x_{i+1} + sy x_{j+1} + sz x_{k+1}) / d; a
@@ -2936,8 +2940,10 @@ k_sor7_strips(int mx, int my, int mz, double d, double db, double sx, double sy,
             const unsigned long long need_s = base + (unsigned long long)min(t1 + 64, nsteps) + need_extra;
             const unsigned long long need_k = base + (unsigned long long)t1 + need_extra;
             long spins = 0;
-            while (!dead && ((fs && __hip_atomic_load(fs, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < need_s) ||
-                             (fk && __hip_atomic_load(fk, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < need_k))) {
+            // relaxed polls, then one acquire fence (an acquire per poll would
+            // invalidate the cache every time)
+            while (!dead && ((fs && __hip_atomic_load(fs, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < need_s) ||
+                             (fk && __hip_atomic_load(fk, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < need_k))) {
                 if (++spins > spin_limit) {
                     atomicExch(err, 1);
                     dead = true;
@@ -2947,6 +2953,7 @@ k_sor7_strips(int mx, int my, int mz, double d, double db, double sx, double sy,
                 }
                 __builtin_amdgcn_s_sleep(1);
             }
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         }
         __syncthreads();
         // the batch's operands
@@ -3002,10 +3009,11 @@ k_sor7_strips(int mx, int my, int mz, double d, double db, double sx, double sy,
             }
             prev = res;
         }
-        // every lane's stores of the batch before the published count
-        __threadfence();
+        // every lane's stores of the batch before the published count: one
+        // release fence for the wave, then the count
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
         if (r == 63)
-            __hip_atomic_store(flag + g, base + (unsigned long long)t1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(flag + g, base + (unsigned long long)t1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
 
