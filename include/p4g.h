@@ -134,7 +134,16 @@ int p4g_da_destroy(p4g_DA *da);
  *       with the operator apply; auto fuses when one fine-level vector is
  *       larger than 768 MiB (below that the separate streaming passes are
  *       faster); -p4g_fuse_r and -p4g_fuse_dir (auto|1|0, default: the
- *       value of -p4g_fuse_cg) set the two fusions separately.
+ *       value of -p4g_fuse_cg) set the two fusions separately,
+ *   -p4g_lds auto|1|0 (default auto: 3D levels of 400+ points per side):
+ *       stencil passes with their neighbour rows staged in LDS,
+ *   -p4g_resid_restrict auto|1|0 (default auto: where -p4g_lds applies):
+ *       the cycle's residual and restriction in one pass, -p4g_rr_rows 8|4
+ *       its coarse rows per block,
+ *   -p4g_post_fuse auto|1|0: the prolongation fused with the first
+ *       post-smoothing step,
+ *   -p4g_sor_pipeline 1|0 (default 1): SOR levels swept by pipelined
+ *       strips, one launch per direction (else one per hyperplane).
  * Options and combinations the device path does not implement are rejected
  * with PETSC_ERR_SUP and a message naming them, never silently replaced:
  * e.g. -pc_type lu|sor|asm, -pc_mg_type additive, ILU with fill, SOR with
@@ -186,7 +195,8 @@ enum {
     P4G_K_CG_UPDATE,       /* p = z + b p, x += a p (or x and r together)      */
     P4G_K_RESID_ZG,        /* r = b - A (c b): zero-guess smoothing + residual */
     P4G_K_RESTRICT,        /* b_c = P^T r                                      */
-    P4G_K_PROLONG,         /* x = c b + P x_c                                  */
+    P4G_K_PROLONG,         /* x = c b + P x_c (with the first post-smoothing
+                              step fused where the solver fuses it)            */
     P4G_K_SMOOTH,          /* x <- x + s D^-1 (b - A x) (Chebyshev step)       */
     P4G_K_COARSE,          /* coarsest-grid Cholesky solve                     */
     P4G_K_REDUCE,          /* finalisation of fused dot products               */
@@ -222,7 +232,12 @@ int p4g_solver_reset_stats(p4g_Solver s);
  * until the active set changes) and releases a request's memory when it
  * finishes.  b and x are host arrays owned by the caller; both must stay
  * valid until the request has finished.
- * Options: -p4g_serve_graph 1|0, -p4g_serve_max_slots n (default 64). */
+ * Options: -p4g_serve_graph 1|0, -p4g_serve_graph_after k (default 32:
+ * a round is captured in a hipGraph once the active set has held k rounds),
+ * -p4g_serve_max_slots n (default 64), -p4g_serve_staging_mb m (default 1024:
+ * pinned host staging for right-hand sides and results, allocated with the
+ * server).  b is copied at submit; x is written by a completion thread before
+ * the request is reported done. */
 /* ------------------------------------------- variational inequalities */
 
 /* SNESSolve with SNESVINEWTONRSLS and SNESVISetComputeVariableBounds

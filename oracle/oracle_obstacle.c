@@ -135,6 +135,160 @@ static void csr_submatrix(const ocsr *A, long nk, const long *keep, const long *
     }
 }
 
+/* ------------------------------------------------ PCASM (restricted) */
+
+/* The PETSc DMDA 2D process grid for P ranks of an mx x my grid
+ * (DMSetUp_DA_2D's squarish choice) and its ownership ranges: rank
+ * (rx, ry) = ry px + rx owns columns xs[rx] .. xs[rx+1]-1, rows ys[ry] .. */
+static void dmda_split2(int mx, int my, int P, int *px, int *py, int xs[17], int ys[17]) {
+    int m = (int)(0.5 + sqrt((double)mx * P / my));
+    if (!m) m = 1;
+    int n = 1;
+    while (m > 0) {
+        n = P / m;
+        if (m * n == P) break;
+        m--;
+    }
+    if (mx > my && m < n) { const int t = m; m = n; n = t; }
+    *px = m; *py = n;
+    xs[0] = 0;
+    for (int q = 0; q < m; q++) xs[q + 1] = xs[q] + mx / m + (mx % m > q);
+    ys[0] = 0;
+    for (int q = 0; q < n; q++) ys[q + 1] = ys[q] + my / n + (my % n > q);
+}
+
+/* PCASM with one subdomain per simulated rank: the rank's rows of the
+ * (reduced) matrix, grown by `overlap` levels of the matrix graph
+ * (MatIncreaseOverlap), each solved exactly (-sub_pc_type lu), and the
+ * restricted form (PC_ASM_RESTRICT, the default): a subdomain's solution is
+ * added back on the rank's own rows only. */
+typedef struct {
+    int nq;
+    long n;
+    long *cnt, **idx;        /* subdomain q: cnt[q] indices, ascending */
+    char **own;              /* whether idx[q][i] is owned by rank q */
+    double **inv;            /* dense inverse of A[idx, idx], row-major */
+    double *w1, *w2;
+} oasm;
+
+static void oasm_free(oasm *a) {
+    for (int q = 0; q < a->nq; q++) { free(a->idx[q]); free(a->own[q]); free(a->inv[q]); }
+    free(a->cnt); free(a->idx); free(a->own); free(a->inv); free(a->w1); free(a->w2);
+    memset(a, 0, sizeof *a);
+}
+
+/* dense inverse by Gauss-Jordan with partial pivoting (an exact solve up to
+ * rounding, as the subdomain LU is) */
+static int dense_inverse(int m, double *A, double *X) {
+    for (int i = 0; i < m; i++)
+        for (int j = 0; j < m; j++) X[(long)i * m + j] = (i == j) ? 1.0 : 0.0;
+    for (int c = 0; c < m; c++) {
+        int pr = c;
+        for (int r = c + 1; r < m; r++)
+            if (fabs(A[(long)r * m + c]) > fabs(A[(long)pr * m + c])) pr = r;
+        if (A[(long)pr * m + c] == 0.0) return 1;
+        if (pr != c)
+            for (int j = 0; j < m; j++) {
+                double t = A[(long)c * m + j]; A[(long)c * m + j] = A[(long)pr * m + j]; A[(long)pr * m + j] = t;
+                t = X[(long)c * m + j]; X[(long)c * m + j] = X[(long)pr * m + j]; X[(long)pr * m + j] = t;
+            }
+        const double piv = A[(long)c * m + c];
+        for (int j = 0; j < m; j++) { A[(long)c * m + j] /= piv; X[(long)c * m + j] /= piv; }
+        for (int r = 0; r < m; r++) {
+            if (r == c) continue;
+            const double f = A[(long)r * m + c];
+            if (f == 0.0) continue;
+            for (int j = 0; j < m; j++) {
+                A[(long)r * m + j] -= f * A[(long)c * m + j];
+                X[(long)r * m + j] -= f * X[(long)c * m + j];
+            }
+        }
+    }
+    return 0;
+}
+
+/* S: the reduced matrix on the kept vertices keep[0..nk-1] of grid g */
+static int oasm_setup(oasm *a, const ogrid *g, const ocsr *S, long nk, const long *keep, int P, int overlap,
+                      char *err, int errlen) {
+    memset(a, 0, sizeof *a);
+    int px, py, xs[17], ys[17];
+    dmda_split2(g->mx, g->my, P, &px, &py, xs, ys);
+    a->nq = P; a->n = nk;
+    a->cnt = (long *)calloc(P, sizeof(long));
+    a->idx = (long **)calloc(P, sizeof(long *));
+    a->own = (char **)calloc(P, sizeof(char *));
+    a->inv = (double **)calloc(P, sizeof(double *));
+    a->w1 = (double *)malloc(sizeof(double) * (nk ? nk : 1));
+    a->w2 = (double *)malloc(sizeof(double) * (nk ? nk : 1));
+    int *rank = (int *)malloc(sizeof(int) * (nk ? nk : 1));
+    for (long r = 0; r < nk; r++) {
+        const int i = (int)(keep[r] % g->mx), j = (int)(keep[r] / g->mx);
+        int rx = 0, ry = 0;
+        while (i >= xs[rx + 1]) rx++;
+        while (j >= ys[ry + 1]) ry++;
+        rank[r] = ry * px + rx;
+    }
+    char *in = (char *)malloc(nk ? nk : 1), *nx = (char *)malloc(nk ? nk : 1);
+    for (int q = 0; q < P; q++) {
+        for (long r = 0; r < nk; r++) in[r] = rank[r] == q;
+        for (int o = 0; o < overlap; o++) {
+            memcpy(nx, in, nk);
+            for (long r = 0; r < nk; r++)
+                if (in[r])
+                    for (int64_t t = S->rp[r]; t < S->rp[r + 1]; t++) nx[S->ci[t]] = 1;
+            memcpy(in, nx, nk);
+        }
+        long c = 0;
+        for (long r = 0; r < nk; r++) c += in[r];
+        a->cnt[q] = c;
+        a->idx[q] = (long *)malloc(sizeof(long) * (c ? c : 1));
+        a->own[q] = (char *)malloc(c ? c : 1);
+        long w = 0;
+        for (long r = 0; r < nk; r++)
+            if (in[r]) { a->own[q][w] = rank[r] == q; a->idx[q][w++] = r; }
+        if (c > 2048) {
+            snprintf(err, errlen, "oracle: an ASM subdomain of %ld unknowns (dense solve up to 2048)", c);
+            free(rank); free(in); free(nx);
+            return 56;
+        }
+        if (!c) continue;
+        double *A = (double *)calloc((size_t)c * c, sizeof(double));
+        long *loc = (long *)malloc(sizeof(long) * nk);
+        for (long r = 0; r < nk; r++) loc[r] = -1;
+        for (long i2 = 0; i2 < c; i2++) loc[a->idx[q][i2]] = i2;
+        for (long i2 = 0; i2 < c; i2++) {
+            const long r = a->idx[q][i2];
+            for (int64_t t = S->rp[r]; t < S->rp[r + 1]; t++)
+                if (loc[S->ci[t]] >= 0) A[i2 * c + loc[S->ci[t]]] = S->v[t];
+        }
+        a->inv[q] = (double *)malloc(sizeof(double) * (size_t)c * c);
+        const int bad = dense_inverse((int)c, A, a->inv[q]);
+        free(A); free(loc);
+        if (bad) {
+            snprintf(err, errlen, "oracle: singular ASM subdomain matrix");
+            free(rank); free(in); free(nx);
+            return 76;
+        }
+    }
+    free(rank); free(in); free(nx);
+    return 0;
+}
+
+static void oasm_apply(void *ctx, const double *r, double *z) {
+    oasm *a = (oasm *)ctx;
+    for (long i = 0; i < a->n; i++) z[i] = 0.0;
+    for (int q = 0; q < a->nq; q++) {
+        const long c = a->cnt[q];
+        for (long i = 0; i < c; i++) a->w1[i] = r[a->idx[q][i]];
+        for (long i = 0; i < c; i++) {
+            if (!a->own[q][i]) continue;
+            double s2 = 0.0;
+            for (long j = 0; j < c; j++) s2 += a->inv[q][i * c + j] * a->w1[j];
+            z[a->idx[q][i]] += s2;
+        }
+    }
+}
+
 /* ----------------------------------------- PCMG on the inactive sets */
 
 typedef struct {
@@ -313,6 +467,7 @@ typedef struct {
     int    maxit, ls;
     int    monitor, ksp_reason, sym, mg_levels;
     int    ssls;          /* -snes_type vinewtonssls */
+    int    pcasm, nranks, asm_overlap;   /* -pc_type asm over -oracle_nranks simulated ranks */
     double ls_alpha;      /* the BT line search's sufficient-decrease constant */
     int    ls_project;    /* trial points projected onto the bounds */
     char   symtol[32];
@@ -509,22 +664,28 @@ static int rsls(const obs_cfg *cfg, const ogrid *g, double *X, double *F, const 
         int kreason = 3, kits = 0;
         if (nk > 0) {
             vimg mg;
+            oasm as;
             int pct = cfg->pctype;
             if (pct == PC_MG) {
                 int nlev = cfg->mg_levels;
                 e = vimg_setup(&mg, g, nlev, nk, keep, &Jii, cfg->o, err, errlen);
                 if (e) { ocsr_free(&Jii); break; }
                 pct = PC_SHELL;
+            } else if (cfg->pcasm) {
+                e = oasm_setup(&as, g, &Jii, nk, keep, cfg->nranks, cfg->asm_overlap, err, errlen);
+                if (e) { oasm_free(&as); ocsr_free(&Jii); break; }
             }
             oksp k;
             oksp_init(&k, cfg->ktype, &Jii, pct);
-            if (pct == PC_SHELL) { k.pc.shell = vimg_apply; k.pc.shell_ctx = &mg; }
+            if (pct == PC_SHELL && cfg->pcasm) { k.pc.shell = oasm_apply; k.pc.shell_ctx = &as; }
+            else if (pct == PC_SHELL) { k.pc.shell = vimg_apply; k.pc.shell_ctx = &mg; }
             k.rtol = cfg->krtol; k.atol = cfg->katol; k.dtol = cfg->kdtol; k.maxit = cfg->kmaxit;
             if (cfg->knormset) k.normtype = cfg->normtype;
             double r0, rn;
             kits = oksp_solve(&k, Fi, Yi, &kreason, &r0, &rn);
             oksp_free(&k);
             if (cfg->pctype == PC_MG) vimg_free(&mg);
+            if (cfg->pcasm) oasm_free(&as);
         }
         ocsr_free(&Jii);
         ovset(n, 0.0, Y);
@@ -671,7 +832,18 @@ int oracle_obstacle_run(int argc, const char **argv, obstacle_result *res, doubl
         return 56;
     }
     c.ktype = oracle_parse_ksp(oopts_get(&o, "ksp_type"), KSP_CG);   /* obstacle.c:126 */
-    c.pctype = oracle_parse_pc(oopts_get(&o, "pc_type"), PC_ILU);
+    const char *pcs = oopts_get(&o, "pc_type");
+    c.pcasm = pcs && !strcmp(pcs, "asm");
+    c.pctype = c.pcasm ? PC_SHELL : oracle_parse_pc(pcs, PC_ILU);
+    c.nranks = oopts_int(&o, "oracle_nranks", 1);
+    c.asm_overlap = oopts_int(&o, "pc_asm_overlap", 1);
+    if (c.pcasm) {
+        const char *sp = oopts_get(&o, "sub_pc_type");
+        if (!sp || strcmp(sp, "lu") || c.nranks < 1 || c.nranks > 16 || c.asm_overlap < 0) {
+            oprintf(out, outlen, &pos, "[0]PETSC ERROR: oracle: -pc_type asm needs -sub_pc_type lu, 1..16 ranks\n");
+            return 56;
+        }
+    }
     if (c.ktype < 0 || c.pctype < 0 || c.ktype == KSP_PREONLY) {
         oprintf(out, outlen, &pos, "[0]PETSC ERROR: oracle: unsupported -ksp_type/-pc_type\n");
         return 56;

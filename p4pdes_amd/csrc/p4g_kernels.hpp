@@ -3267,6 +3267,25 @@ k_vi_dot(long n, const double* __restrict__ x, const double* __restrict__ y, dou
     block_partials<2, 4>(a0, a1, part, blockIdx.x);
 }
 
+// PCASM's restricted application: every row a subdomain owns (4 longs per
+// row: the offset of its inverse row, of its subdomain's vertex list, the
+// subdomain size, the row's vertex) gets the row of the subdomain's inverse
+// applied to r gathered over the subdomain; rows no subdomain owns keep z's
+// value (the caller fills z with 0 first)
+__global__ void __launch_bounds__(256)
+k_asm_apply(long nrows, const long* __restrict__ rows, const int* __restrict__ idx, const double* __restrict__ inv,
+            const double* __restrict__ r, double* __restrict__ z) {
+    for (long k = blockIdx.x * 256L + threadIdx.x; k < nrows; k += (long)gridDim.x * 256L) {
+        const long* R = rows + 4 * k;
+        const double* a = inv + R[0];
+        const int* c = idx + R[1];
+        const long cnt = R[2];
+        double s = 0.0;
+        for (long j = 0; j < cnt; ++j) s += a[j] * r[c[j]];
+        z[R[3]] = s;
+    }
+}
+
 // SNESVINEWTONSSLS (semismooth Newton on the Fischer-Burmeister
 // reformulation, lower bounds only as the obstacle problem sets them):
 // phi(a, b) = a + b - sqrt(a^2 + b^2) and its derivative in a, with PETSc's

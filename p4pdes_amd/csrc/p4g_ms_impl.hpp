@@ -291,7 +291,7 @@ static int ms_solve(p4g_solver_s* s, double q, const double* gb, double* x, p4g_
                                         "Jacobians, with a direct coarse solve and Chebyshev/Richardson smoothing "
                                         "(Jacobi or SOR)");
     if (s->ksp_rich) return set_error(PETSC_ERR_SUP, "p4g_ms_solve: -ksp_type gmres (the default) or cg");
-    if (s->pctype == PCT_JACOBI)
+    if (s->pctype == PCT_JACOBI || s->pctype == PCT_ASM)
         return set_error(PETSC_ERR_SUP, "p4g_ms_solve: -pc_type ilu (the default), mg or none");
     const bool gmres = s->ksp_gmres || !s->opt.has("ksp_type");   // SNES's KSP defaults to GMRES
     if (!s->ms_mode) {

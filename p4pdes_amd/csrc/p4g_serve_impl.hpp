@@ -12,7 +12,12 @@
 // order, when their device footprint fits the HBM budget; finished requests
 // release theirs.  Each request's arithmetic is independent of the others
 // (per-request blocks and partial sums), so its result is the same alone
-// and in a mix.
+// and in a mix.  Nothing slow runs in the worker's loop: a request's device
+// vectors come from a per-solver pool (no hipMalloc/hipFree once warm), its
+// right-hand side is copied into pooled pinned host memory by the submitting
+// thread, and its result goes back through that pinned buffer, copied into
+// the caller's array by a completion thread once the device copy's event has
+// fired.
 
 #include <condition_variable>
 #include <deque>
@@ -53,6 +58,11 @@ struct SrvReq {
     float *pold = nullptr, *pnew = nullptr;
     std::string err;
     double* S = nullptr;   // device scalar block (32 doubles) of the slot
+    std::vector<size_t> layout;    // byte sizes of allocs, in order (the pool's key)
+    double* pin = nullptr;         // pinned host staging: b at submit, x at the end
+    hipEvent_t ev = nullptr;       // the result's device-to-host copy
+    hipEvent_t ev_in = nullptr;    // b's upload
+    hipEvent_t ev_k = nullptr;     // the request's last kernel
 };
 
 // a launch of a batched kernel over a list of items
@@ -94,7 +104,27 @@ struct p4g_server_s {
     std::vector<p4g::BLaunch> vseq;       // their launch sequence
     bool vc_built = false;
     int vc_stable = 0;                    // rounds the active set has held
-    int graph_after = 3;                  // -p4g_serve_graph_after: rounds before a capture
+    int graph_after = 32;                 // -p4g_serve_graph_after: rounds before a capture
+    // pools: idle device vector sets per layout (the byte sizes of a
+    // request's vectors in allocation order), pinned staging buffers per
+    // length; pool_bytes counts the idle device sets against the budget
+    // together with used
+    std::map<std::vector<size_t>, std::vector<std::vector<void*>>> dpool;
+    std::map<long, std::vector<double*>> ppool;
+    // a pinned staging arena allocated with the server (-p4g_serve_staging_mb,
+    // default 1024): first-time staging buffers are carved from it instead of
+    // pinned one by one (pinning 100+ MB takes tens of ms); free ranges by offset
+    char* arena = nullptr;
+    size_t arena_size = 0;
+    std::map<size_t, size_t> arena_free;
+    size_t pool_bytes = 0;
+    // requests whose result is being copied back; the completion thread
+    std::deque<p4g::SrvReq*> copying;
+    std::condition_variable cv_copy;
+    bool stop_copy = false;
+    std::vector<std::thread> th_copy;     // completion threads
+    hipStream_t cst = nullptr;            // uploads and result copies
+    std::vector<p4g::SrvReq*> loading;    // admitted, b still uploading (worker thread only)
     // statistics
     std::vector<double> lat_ms, queue_ms;
     long submitted = 0, completed = 0, rejected = 0, rounds = 0, captures = 0;
@@ -373,16 +403,51 @@ static int build_vcycle(p4g_server_s* v, Packer& pk, std::vector<BLaunch>& seq) 
 
 // ---------------------------------------------------------- request memory
 
-static int req_alloc(SrvReq* r, void** p, size_t bytes) {
-    hipError_t e = hipMalloc(p, std::max<size_t>(bytes, 8));
-    if (e != hipSuccess) return set_error(PETSC_ERR_MEM, "serving: hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
-    r->allocs.push_back(*p);
-    return 0;
-}
-
 static void req_free(SrvReq* r) {
     for (void* p : r->allocs) hipFree(p);
     r->allocs.clear();
+}
+
+// a pinned staging buffer of n doubles (pooled per length, else carved
+// from the arena, else pinned on its own); call without the lock
+static int pin_get(p4g_server_s* v, long n, double** out) {
+    {
+        std::lock_guard<std::mutex> lk(v->mu);
+        std::vector<double*>& q = v->ppool[n];
+        if (!q.empty()) {
+            *out = q.back();
+            q.pop_back();
+            return 0;
+        }
+        const size_t want = ((std::max<long>(n, 1) * sizeof(double)) + 4095) & ~(size_t)4095;
+        for (auto it = v->arena_free.begin(); it != v->arena_free.end(); ++it) {
+            if (it->second < want) continue;
+            const size_t off = it->first, len = it->second;
+            v->arena_free.erase(it);
+            if (len > want) v->arena_free[off + want] = len - want;
+            *out = (double*)(v->arena + off);
+            return 0;   // the buffer stays with its length class (ppool) once released
+        }
+    }
+    if (hipHostMalloc((void**)out, std::max<long>(n, 1) * sizeof(double), hipHostMallocDefault) != hipSuccess)
+        return set_error(PETSC_ERR_MEM, "serving: pinned staging of %ld doubles", n);
+    return 0;
+}
+
+// the byte sizes of a request's device vectors in allocation order: b64,
+// x64, the five FP32 Krylov vectors, each level's b, x (below the finest)
+// and r (above the coarsest), the partial sums
+static std::vector<size_t> req_layout(p4g_solver_s* s) {
+    const long n = s->lv.back().n;
+    std::vector<size_t> z = {size_t(n) * 8, size_t(n) * 8};
+    for (int q = 0; q < 5; ++q) z.push_back(size_t(n) * 4);
+    for (int l = 0; l < s->nlev; ++l) {
+        const size_t nl = size_t(s->lv[l].n);
+        if (l + 1 < s->nlev) { z.push_back(nl * 4); z.push_back(nl * 4); }
+        if (l > 0) z.push_back(nl * 4);
+    }
+    z.push_back(2 * size_t(std::max<long>(max_part_blocks(s->lv), vblocks(n))) * 8);
+    return z;
 }
 
 static int req_admit(p4g_server_s* v, SrvReq* r) {
@@ -392,30 +457,77 @@ static int req_admit(p4g_server_s* v, SrvReq* r) {
     r->nlev = s->nlev;
     r->dim = F.dim;
     r->n = n;
-    P4G_TRY(req_alloc(r, (void**)&r->b64, n * 8));
-    P4G_TRY(req_alloc(r, (void**)&r->x64, n * 8));
+    r->layout = req_layout(s);
+    // an idle vector set of this exact layout, else new allocations (idle
+    // sets of other layouts released first when the budget needs their room)
+    std::vector<void*> set;
+    std::vector<void*> evict;
+    {
+        std::lock_guard<std::mutex> lk(v->mu);
+        auto it = v->dpool.find(r->layout);
+        if (it != v->dpool.end() && !it->second.empty()) {
+            set = std::move(it->second.back());
+            it->second.pop_back();
+            v->pool_bytes -= r->bytes;
+        } else {
+            for (auto& kv : v->dpool) {
+                size_t each = 0;
+                for (size_t b : kv.first) each += b;
+                while (!kv.second.empty() && v->used + v->pool_bytes > v->budget) {
+                    for (void* p : kv.second.back()) evict.push_back(p);
+                    kv.second.pop_back();
+                    v->pool_bytes -= each;
+                }
+            }
+        }
+    }
+    for (void* p : evict) hipFree(p);
+    if (set.empty()) {
+        for (size_t b : r->layout) {
+            void* p = nullptr;
+            const hipError_t e = hipMalloc(&p, std::max<size_t>(b, 8));
+            if (e != hipSuccess) {
+                for (void* q : set) hipFree(q);
+                return set_error(PETSC_ERR_MEM, "serving: hipMalloc(%zu): %s", b, hipGetErrorString(e));
+            }
+            set.push_back(p);
+        }
+    }
+    if (set.size() != r->layout.size()) return set_error(PETSC_ERR_LIB, "internal: request vector set of another layout");
+    r->allocs = set;
+    size_t k = 0;
+    r->b64 = (double*)set[k++];
+    r->x64 = (double*)set[k++];
     float** f32[5] = {&r->r32, &r->z32, &r->pa, &r->pb, &r->w32};
-    for (float** q : f32) P4G_TRY(req_alloc(r, (void**)q, n * 4));
+    for (float** q : f32) *q = (float*)set[k++];
     r->lv.assign(r->nlev, SrvLevel());
     for (int l = 0; l < r->nlev; ++l) {
-        const long nl = s->lv[l].n;
         if (l + 1 < r->nlev) {
-            P4G_TRY(req_alloc(r, (void**)&r->lv[l].b, nl * 4));
-            P4G_TRY(req_alloc(r, (void**)&r->lv[l].x, nl * 4));
+            r->lv[l].b = (float*)set[k++];
+            r->lv[l].x = (float*)set[k++];
         }
-        if (l > 0) P4G_TRY(req_alloc(r, (void**)&r->lv[l].r, nl * 4));
+        if (l > 0) r->lv[l].r = (float*)set[k++];
     }
     r->npart = std::max<long>(max_part_blocks(s->lv), vblocks(n));
-    P4G_TRY(req_alloc(r, (void**)&r->part, 2 * r->npart * 8));
+    r->part = (double*)set[k++];
     r->slot = v->free_slots.back();
     v->free_slots.pop_back();
     r->S = v->S + 32 * r->slot;
-    // initial state: x = 0, p = 0, r32 = b, |b|^2 -> outer TMP0
+    // b (staged by submit) on the copy stream; the request joins the rounds
+    // once this upload's event has fired (req_activate)
+    HIP_TRY(hipMemcpyAsync(r->b64, r->pin, n * 8, hipMemcpyHostToDevice, v->cst));
+    HIP_TRY(hipEventRecord(r->ev_in, v->cst));
+    return 0;
+}
+
+// the initial state once b is on the device: x = 0, p = 0, r32 = b, |b|^2
+// -> outer TMP0
+static int req_activate(p4g_server_s* v, SrvReq* r) {
+    const long n = r->n;
     hipStream_t st = v->st;
     k_fill<double><<<1, 256, 0, st>>>(32, 0.0, r->S);
     k_fill<double><<<1, 256, 0, st>>>(1, 1.0, r->S + S_ONE);
     k_fill<double><<<1, 256, 0, st>>>(1, 1.0, r->S + 16 + S_ONE);
-    HIP_TRY(hipMemcpyAsync(r->b64, r->hb, n * 8, hipMemcpyHostToDevice, st));
     const int gb = vblocks(n);
     k_fill<double><<<gb, 256, 0, st>>>(n, 0.0, r->x64);
     k_fill<float><<<gb, 256, 0, st>>>(n, 0.0f, r->pa);
@@ -592,12 +704,60 @@ static bool req_step(p4g_server_s* v, SrvReq* r) {
     return false;
 }
 
+// the result's copy into the pinned staging buffer, on the copy stream after
+// the request's last kernel (the rounds do not wait for it; nothing of this
+// request's is reused until the completion thread has seen the copy's event)
 static int req_finish(p4g_server_s* v, SrvReq* r) {
-    hipStream_t st = v->st;
-    HIP_TRY(hipMemcpyAsync(r->hx, r->x64, r->n * 8, hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipStreamSynchronize(st));
-    r->t_done = wall_ms();
+    HIP_TRY(hipEventRecord(r->ev_k, v->st));   // the request's last kernel
+    HIP_TRY(hipStreamWaitEvent(v->cst, r->ev_k, 0));
+    HIP_TRY(hipMemcpyAsync(r->pin, r->x64, r->n * 8, hipMemcpyDeviceToHost, v->cst));
+    HIP_TRY(hipEventRecord(r->ev, v->cst));
     return 0;
+}
+
+// completion thread: waits for each result copy in order, copies the pinned
+// result into the caller's array, then releases the request's device vectors
+// to the pool and its staging buffer, and reports it done
+static void copy_loop(p4g_server_s* v) {
+    hipSetDevice(v->device);
+    for (;;) {
+        SrvReq* r = nullptr;
+        {
+            std::unique_lock<std::mutex> lk(v->mu);
+            v->cv_copy.wait(lk, [&] { return v->stop_copy || !v->copying.empty(); });
+            if (v->copying.empty()) break;   // stop_copy and nothing left
+            r = v->copying.front();
+            v->copying.pop_front();
+        }
+        const hipError_t he = hipEventSynchronize(r->ev);
+        if (he == hipSuccess) std::memcpy(r->hx, r->pin, r->n * sizeof(double));
+        const double t = wall_ms();
+        std::lock_guard<std::mutex> lk(v->mu);
+        hipEventDestroy(r->ev);
+        hipEventDestroy(r->ev_in);
+        hipEventDestroy(r->ev_k);
+        r->ev = r->ev_in = r->ev_k = nullptr;
+        v->ppool[r->n].push_back(r->pin);
+        r->pin = nullptr;
+        v->dpool[r->layout].push_back(std::move(r->allocs));
+        r->allocs.clear();
+        v->pool_bytes += r->bytes;
+        v->used -= r->bytes;
+        if (he != hipSuccess) {
+            r->state = RQ_FAILED;
+            r->err = hipGetErrorString(he);
+        } else {
+            r->t_done = t;
+            r->state = RQ_DONE;
+            v->lat_ms.push_back(r->t_done - r->t_submit);
+            v->queue_ms.push_back(r->t_admit - r->t_submit);
+            v->completed++;
+            v->dof_done += (double)r->n;
+            v->t_last = r->t_done;
+        }
+        v->cv_done.notify_all();
+        v->cv.notify_all();   // the pool may now hold what a queued request needs
+    }
 }
 
 static void serve_loop(p4g_server_s* v) {
@@ -606,16 +766,17 @@ static void serve_loop(p4g_server_s* v) {
         std::vector<SrvReq*> newly;
         {
             std::unique_lock<std::mutex> lk(v->mu);
-            v->cv.wait(lk, [&] { return v->stop || !v->queue.empty() || !v->active.empty(); });
-            if (v->stop && v->active.empty() && v->queue.empty()) break;
+            v->cv.wait(lk, [&] {
+                return v->stop || !v->queue.empty() || !v->active.empty() || !v->loading.empty();
+            });
+            if (v->stop && v->active.empty() && v->queue.empty() && v->loading.empty()) break;
             // admission in arrival order, skipping requests that do not fit yet
             for (auto it = v->queue.begin(); it != v->queue.end();) {
                 SrvReq* r = *it;
-                if ((int)v->active.size() >= v->max_slots) break;
+                if ((int)(v->active.size() + v->loading.size() + newly.size()) >= v->max_slots) break;
                 if (v->used + r->bytes <= v->budget) {
                     v->used += r->bytes;
                     v->peak = std::max(v->peak, v->used);
-                    v->active.push_back(r);
                     newly.push_back(r);
                     it = v->queue.erase(it);
                 } else {
@@ -624,8 +785,30 @@ static void serve_loop(p4g_server_s* v) {
             }
         }
         int e = 0;
-        for (SrvReq* r : newly)
+        for (SrvReq* r : newly) {   // vectors, slot and b's upload on the copy stream
+            v->loading.push_back(r);
             if (!e) e = req_admit(v, r);
+        }
+        // requests whose upload has finished join the rounds; with none
+        // active, wait for the oldest upload instead of spinning
+        if (!e && !v->loading.empty() && v->active.empty()) {
+            const hipError_t he = hipEventSynchronize(v->loading.front()->ev_in);
+            if (he != hipSuccess) e = set_error(PETSC_ERR_LIB, "serving: upload: %s", hipGetErrorString(he));
+        }
+        for (auto it = v->loading.begin(); !e && it != v->loading.end();) {
+            const hipError_t q = hipEventQuery((*it)->ev_in);
+            if (q == hipSuccess) {
+                e = req_activate(v, *it);
+                std::lock_guard<std::mutex> lk(v->mu);
+                v->active.push_back(*it);
+                it = v->loading.erase(it);
+            } else if (q == hipErrorNotReady) {
+                ++it;
+            } else {
+                e = set_error(PETSC_ERR_LIB, "serving: upload: %s", hipGetErrorString(q));
+            }
+        }
+        if (!e && v->active.empty()) continue;
         if (!e) e = round_once(v);
         std::vector<SrvReq*> done;
         if (!e)
@@ -636,27 +819,32 @@ static void serve_loop(p4g_server_s* v) {
         std::lock_guard<std::mutex> lk(v->mu);
         if (e) {
             v->fatal = p4g_last_error();
-            for (SrvReq* r : v->active) { r->state = RQ_FAILED; r->err = v->fatal; req_free(r); }
-            for (SrvReq* r : v->queue) { r->state = RQ_FAILED; r->err = v->fatal; }
+            auto drop = [&](SrvReq* r, bool held) {
+                r->state = RQ_FAILED;
+                r->err = v->fatal;
+                if (held) req_free(r);
+                if (r->pin) { v->ppool[r->n].push_back(r->pin); r->pin = nullptr; }
+                for (hipEvent_t* q : {&r->ev, &r->ev_in, &r->ev_k})
+                    if (*q) { hipEventDestroy(*q); *q = nullptr; }
+            };
+            for (SrvReq* r : v->active) drop(r, true);
+            for (SrvReq* r : v->loading) drop(r, true);
+            for (SrvReq* r : v->queue) drop(r, false);
             v->active.clear();
+            v->loading.clear();
             v->queue.clear();
-            v->used = 0;
+            size_t held = 0;   // results still being copied keep their vectors until the completion thread is done
+            for (SrvReq* r : v->copying) held += r->bytes;
+            v->used = held;
             v->cv_done.notify_all();
             continue;
         }
-        for (SrvReq* r : done) {
-            req_free(r);
-            v->used -= r->bytes;
+        for (SrvReq* r : done) {   // to the completion thread (budget, pools and statistics there)
             v->free_slots.push_back(r->slot);
-            r->state = RQ_DONE;
-            v->lat_ms.push_back(r->t_done - r->t_submit);
-            v->queue_ms.push_back(r->t_admit - r->t_submit);
-            v->completed++;
-            v->dof_done += (double)r->n;
-            v->t_last = r->t_done;
             v->active.erase(std::find(v->active.begin(), v->active.end(), r));
+            v->copying.push_back(r);
         }
-        if (!done.empty()) v->cv_done.notify_all();
+        if (!done.empty()) v->cv_copy.notify_all();
     }
 }
 
@@ -685,22 +873,40 @@ int p4g_server_create(size_t hbm_budget_bytes, const char* options, p4g_Server* 
     p4g_server_s* v = new p4g_server_s;
     v->budget = hbm_budget_bytes;
     v->use_graph = o.getb("p4g_serve_graph", true);
-    v->graph_after = std::max(0, o.geti("p4g_serve_graph_after", 3));
+    v->graph_after = std::max(0, o.geti("p4g_serve_graph_after", 32));
     v->max_slots = std::max(1, std::min(1024, o.geti("p4g_serve_max_slots", 64)));
     hipGetDevice(&v->device);
     int e = 0;
     // non-blocking (default priority): while the worker captures a graph on
     // it, synchronous copies on the legacy stream from other threads do not
     // depend on it (the server orders its own work with explicit syncs)
-    if (hipStreamCreateWithFlags(&v->st, hipStreamNonBlocking) != hipSuccess)
+    if (hipStreamCreateWithFlags(&v->st, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&v->cst, hipStreamNonBlocking) != hipSuccess)
         e = set_error(PETSC_ERR_LIB, "p4g_server_create: stream");
     if (!e && hipMalloc(&v->S, sizeof(double) * 32 * v->max_slots) != hipSuccess)
         e = set_error(PETSC_ERR_MEM, "p4g_server_create: scalar blocks");
     if (!e && hipHostMalloc((void**)&v->hS, sizeof(double) * 32 * v->max_slots, hipHostMallocDefault) != hipSuccess)
         e = set_error(PETSC_ERR_MEM, "p4g_server_create: pinned scalars");
-    if (e) { delete v; return e; }
+    const long staging_mb = std::max(0, o.geti("p4g_serve_staging_mb", 1024));
+    if (!e && staging_mb > 0) {
+        v->arena_size = (size_t)staging_mb << 20;
+        if (hipHostMalloc((void**)&v->arena, v->arena_size, hipHostMallocDefault) != hipSuccess)
+            e = set_error(PETSC_ERR_MEM, "p4g_server_create: %ld MiB pinned staging", staging_mb);
+        else
+            v->arena_free[0] = v->arena_size;
+    }
+    if (e) {
+        if (v->S) hipFree(v->S);
+        if (v->hS) hipHostFree(v->hS);
+        if (v->arena) hipHostFree(v->arena);
+        if (v->st) hipStreamDestroy(v->st);
+        if (v->cst) hipStreamDestroy(v->cst);
+        delete v;
+        return e;
+    }
     for (int q = v->max_slots - 1; q >= 0; --q) v->free_slots.push_back(q);
     v->th = std::thread(serve_loop, v);
+    for (int q = 0; q < 4; ++q) v->th_copy.emplace_back(copy_loop, v);   // results of several requests at once
     *out = v;
     return 0;
 }
@@ -723,9 +929,32 @@ int p4g_server_submit(p4g_Server v, p4g_Solver s, const double* b, double* x, lo
     r->hb = b;
     r->hx = x;
     r->bytes = req_bytes(s);
+    r->n = s->lv.back().n;
     r->t_submit = wall_ms();
+    // b into pinned staging here, in the caller's thread; the worker's
+    // upload is then an asynchronous pinned copy
+    if (r->bytes <= v->budget) {
+        int pe = pin_get(v, r->n, &r->pin);
+        for (hipEvent_t* q : {&r->ev, &r->ev_in, &r->ev_k})
+            if (!pe && hipEventCreateWithFlags(q, hipEventDisableTiming) != hipSuccess)
+                pe = set_error(PETSC_ERR_LIB, "p4g_server_submit: event");
+        if (pe) {
+            if (r->pin) { std::lock_guard<std::mutex> lk(v->mu); v->ppool[r->n].push_back(r->pin); }
+            for (hipEvent_t* q : {&r->ev, &r->ev_in, &r->ev_k})
+                if (*q) hipEventDestroy(*q);
+            delete r;
+            return pe;
+        }
+        std::memcpy(r->pin, b, r->n * sizeof(double));
+    }
     std::lock_guard<std::mutex> lk(v->mu);
-    if (!v->fatal.empty()) { delete r; return set_error(PETSC_ERR_LIB, "server failed: %s", v->fatal.c_str()); }
+    if (!v->fatal.empty()) {
+        if (r->pin) v->ppool[r->n].push_back(r->pin);
+        for (hipEvent_t* q : {&r->ev, &r->ev_in, &r->ev_k})
+            if (*q) hipEventDestroy(*q);
+        delete r;
+        return set_error(PETSC_ERR_LIB, "server failed: %s", v->fatal.c_str());
+    }
     r->id = v->next_id++;
     *id = r->id;
     v->all[r->id] = r;
@@ -808,7 +1037,26 @@ int p4g_server_destroy(p4g_Server* pv) {
         v->cv.notify_all();
     }
     if (v->th.joinable()) v->th.join();
-    for (auto& kv : v->all) delete kv.second;
+    {
+        std::lock_guard<std::mutex> lk(v->mu);
+        v->stop_copy = true;
+        v->cv_copy.notify_all();
+    }
+    for (std::thread& t : v->th_copy)
+        if (t.joinable()) t.join();
+    for (auto& kv : v->all) {
+        for (hipEvent_t q : {kv.second->ev, kv.second->ev_in, kv.second->ev_k})
+            if (q) hipEventDestroy(q);
+        if (kv.second->pin) hipHostFree(kv.second->pin);
+        delete kv.second;
+    }
+    for (auto& kv : v->dpool)
+        for (auto& set : kv.second)
+            for (void* p : set) hipFree(p);
+    for (auto& kv : v->ppool)
+        for (double* p : kv.second)
+            if (!(v->arena && (char*)p >= v->arena && (char*)p < v->arena + v->arena_size)) hipHostFree(p);
+    if (v->arena) hipHostFree(v->arena);
     if (v->gexec) hipGraphExecDestroy(v->gexec);
     if (v->graph) hipGraphDestroy(v->graph);
     if (v->d_round) hipFree(v->d_round);
@@ -818,6 +1066,7 @@ int p4g_server_destroy(p4g_Server* pv) {
     if (v->S) hipFree(v->S);
     if (v->hS) hipHostFree(v->hS);
     if (v->st) hipStreamDestroy(v->st);
+    if (v->cst) hipStreamDestroy(v->cst);
     delete v;
     *pv = nullptr;
     return 0;

@@ -51,7 +51,7 @@ using namespace p4g;
 
 namespace p4g {
 
-enum { PCT_MG = 0, PCT_JACOBI, PCT_NONE, PCT_ILU };
+enum { PCT_MG = 0, PCT_JACOBI, PCT_NONE, PCT_ILU, PCT_ASM };
 enum { SM_CHEB = 0, SM_RICH };
 enum { NT_PRECOND = 0, NT_UNPRECOND, NT_NATURAL };
 
@@ -411,8 +411,10 @@ struct Engine {
     // (k_stencil_lds): -1 on levels of 400 or more points per side, 1 on
     // every level whose rows fill the waves, 0 never (-p4g_lds)
     int lds = -1;
-    // the residual and restriction of a cycle in one pass (-p4g_resid_restrict)
+    // the residual and restriction of a cycle in one pass (-p4g_resid_restrict),
+    // with rr_rows coarse rows per block (-p4g_rr_rows 8|4)
     int resid_restrict_mode = -1;
+    int rr_rows = 8;
     // the prolongation and the first of several post-smoothing steps in one
     // pass (-p4g_post_fuse)
     int post2_mode = -1;
@@ -435,6 +437,7 @@ struct Engine {
     // level's 27-plane table is then only the preconditioning matrix
     std::function<int(const T*, T*)> mf_op;
     T* mf_tmp = nullptr;
+    std::function<int(const T*, T*)> asm_op;   // PCASM's z = B r (set by the VI solve)
     bool mf_bypass = false;   // a pass on the preconditioning matrix (PCApplyRichardson_SOR's sweeps)
     bool vi = false;   // levels may become null-row Galerkin levels later (full temporaries)
     int mcycles = 1;        // -pc_mg_multiplicative_cycles
@@ -648,17 +651,22 @@ struct Engine {
     }
 
     // b_c = P^T (b - A x), level l -> l-1, the fine residual not stored
+    // (rr_rows coarse rows per block: 8, or 4)
     int resid_restrict(int l, const T* xx, const T* bb, T* bc) {
         const LevelInfo& F = (*lv)[l];
         const LevelInfo& C = (*lv)[l - 1];
-        constexpr int TYC = 4;
+        const int tyc = rr_rows == 4 ? 4 : 8;
         const int kch = kch_restrict(C.dim, C.my, C.mz);
-        const dim3 g((C.mx - 1 + TX - 1) / TX, (C.my + TYC - 1) / TYC, (C.mz + kch - 1) / kch), b(TX, TYC);
+        const dim3 g((C.mx - 1 + TX - 1) / TX, (C.my + tyc - 1) / tyc, (C.mz + kch - 1) / kch), b(TX, tyc);
         hipStream_t s = cx->stream;
         OpConst<T> o = opc(F, false);
         return cx->launch(P4G_K_RESID_ZG, l == nlev - 1, double(2 * F.n + C.n) * sizeof(T), [&] {
-            k_resid_restrict_lds<T, OpConst<T>, TYC><<<g, b, 0, s>>>(F.mx, F.my, F.mz, C.mx, C.my, C.mz, kch, o, xx, bb,
-                                                                    bc);
+            if (tyc == 4)
+                k_resid_restrict_lds<T, OpConst<T>, 4><<<g, b, 0, s>>>(F.mx, F.my, F.mz, C.mx, C.my, C.mz, kch, o, xx,
+                                                                      bb, bc);
+            else
+                k_resid_restrict_lds<T, OpConst<T>, 8><<<g, b, 0, s>>>(F.mx, F.my, F.mz, C.mx, C.my, C.mz, kch, o, xx,
+                                                                      bb, bc);
         });
     }
 
@@ -694,7 +702,7 @@ struct Engine {
         if (L.var) {
             if (L.dim != 3) return set_error(PETSC_ERR_SUP, "variable coefficient needs 3D");
             OpVar<T> o = opv(l, jac);
-            return cx->launch(P4G_K_SMOOTH, l == nlev - 1, bytes, [&] {
+            return cx->launch(P4G_K_PROLONG, l == nlev - 1, bytes, [&] {
                 k_prolong_smooth<T, 3, OpVar<T>, ZG, Out><<<g, b, 0, s>>>(L.mx, L.my, L.mz, C.mx, C.my, kch, o, base,
                                                                          cz, xc, out, cx->part);
             });
@@ -706,13 +714,13 @@ struct Engine {
                 if (nb) *nb = nblocks(gl);
                 if (Out::NDOT && nblocks(gl) > cx->npart_cap)
                     return set_error(PETSC_ERR_LIB, "internal: partial-sum buffer too small");
-                return cx->launch(P4G_K_SMOOTH, l == nlev - 1, bytes, [&] {
+                return cx->launch(P4G_K_PROLONG, l == nlev - 1, bytes, [&] {
                     k_prolong_smooth_lds<OpConst<T>, ZG, Out, TY><<<gl, bl, 0, s>>>(L.mx, L.my, L.mz, C.mx, C.my, kch,
                                                                                    o, base, cz, xc, out, cx->part);
                 });
             }
         }
-        return cx->launch(P4G_K_SMOOTH, l == nlev - 1, bytes, [&] {
+        return cx->launch(P4G_K_PROLONG, l == nlev - 1, bytes, [&] {
             if (L.dim == 3)
                 k_prolong_smooth<T, 3, OpConst<T>, ZG, Out><<<g, b, 0, s>>>(L.mx, L.my, L.mz, C.mx, C.my, kch, o, base,
                                                                            cz, xc, out, cx->part);
@@ -1253,6 +1261,16 @@ struct Engine {
         P4G_TRY(pend_flush());
         // one level: PCMGSetLevels makes it a smoother level, applied once
         if (pctype == PCT_MG) return smooth(0, rr, nullptr, zz, true, nb);
+        if (pctype == PCT_ASM) {
+            if (!asm_op) return set_error(PETSC_ERR_SUP, "-pc_type asm is implemented for p4g_vi_solve");
+            P4G_TRY(asm_op(rr, zz));
+            const long n = F.n;
+            hipStream_t s = cx->stream;
+            *nb = vblocks(n);
+            return cx->launch(P4G_K_OTHER, true, 2.0 * n * sizeof(T), [&] {
+                k_dot2<T><<<vblocks(n), 256, 0, s>>>(n, zz, rr, cx->part);
+            });
+        }
         if (pctype == PCT_ILU) {
             const long n = F.n;
             hipStream_t s = cx->stream;
@@ -1734,6 +1752,16 @@ struct p4g_solver_s {
     int sor_blocks = 1;               // -p4g_sor_blocks: local sweeps of that many ranks
     int lds = -1;                     // -p4g_lds auto|1|0: LDS-staged 3D stencil passes
     int resid_restrict = -1;          // -p4g_resid_restrict auto|1|0: fused residual + restriction
+    int rr_rows = 8;                  // -p4g_rr_rows 8|4: its coarse rows per block
+    int asm_overlap = 1;              // -pc_asm_overlap (PCASM, VI solves)
+    int ranks = 1;                    // -p4g_ranks: the processes whose DMDA split PCASM's subdomains follow
+    // PCASM's subdomains on the device: per owned row its subdomain's index
+    // list and inverse row (rebuilt every Newton step of a VI solve)
+    int* asm_idx = nullptr;           // subdomain vertex indices, concatenated
+    double* asm_inv = nullptr;        // subdomain inverses, row-major, concatenated
+    long* asm_row = nullptr;          // per owned row: [inverse row offset, index offset, count, vertex]
+    long asm_nrows = 0;
+    size_t asm_cap_idx = 0, asm_cap_inv = 0, asm_cap_row = 0;
     int post_fuse = -1;               // -p4g_post_fuse auto|1|0: prolongation + first of several post steps
     bool sor_test_stall = false;      // -p4g_sor_test_stall: every pipelined-SOR wait gives up at once
     bool sor_pipeline = true;         // -p4g_sor_pipeline 1|0: 3D SOR by pipelined strips (else hyperplanes)
@@ -2100,7 +2128,16 @@ static int configure(p4g_solver_s* s) {
     else if (pt == "jacobi") s->pctype = PCT_JACOBI;
     else if (pt == "none") s->pctype = PCT_NONE;
     else if (pt.empty() || pt == "ilu" || pt == "icc") s->pctype = PCT_ILU;   // PETSc's serial default
+    else if (pt == "asm") s->pctype = PCT_ASM;   // VI solves (p4g_vi_solve) only
     else return set_error(PETSC_ERR_SUP, "-pc_type %s is not implemented on the device", pt.c_str());
+    if (s->pctype == PCT_ASM) {
+        if (o.gets("sub_pc_type", "ilu") != "lu")
+            return set_error(PETSC_ERR_SUP, "-pc_type asm: the subdomain solves are exact (-sub_pc_type lu)");
+        s->asm_overlap = o.geti("pc_asm_overlap", 1);
+        if (s->asm_overlap < 0) return set_error(PETSC_ERR_ARG_OUTOFRANGE, "-pc_asm_overlap must be >= 0");
+    }
+    s->ranks = o.geti("p4g_ranks", 1);
+    if (s->ranks < 1 || s->ranks > 64) return set_error(PETSC_ERR_ARG_OUTOFRANGE, "-p4g_ranks must be 1..64");
     for (const char** u = kUnsupported; *u; ++u)
         if (o.has(*u)) return set_error(PETSC_ERR_SUP, "-%s is not implemented on the device", *u);
     const std::string mt = o.gets("pc_mg_type", "multiplicative");
@@ -2199,6 +2236,9 @@ static int configure(p4g_solver_s* s) {
         };
         P4G_TRY(tri(o.gets("p4g_lds", "auto"), "p4g_lds", s->lds));
         P4G_TRY(tri(o.gets("p4g_resid_restrict", "auto"), "p4g_resid_restrict", s->resid_restrict));
+        s->rr_rows = o.geti("p4g_rr_rows", 8);
+        if (s->rr_rows != 4 && s->rr_rows != 8)
+            return set_error(PETSC_ERR_ARG_OUTOFRANGE, "-p4g_rr_rows must be 4 or 8");
         P4G_TRY(tri(o.gets("p4g_post_fuse", "auto"), "p4g_post_fuse", s->post_fuse));
         const std::string both = o.gets("p4g_fuse_cg", "auto");
         P4G_TRY(tri(both, "p4g_fuse_cg", s->fuse_r));
@@ -2461,6 +2501,7 @@ int solver_setup(p4g_solver_s* s) {
         e->ilu_u.assign(s->ilu_u.begin(), s->ilu_u.end());
         e->lds = s->lds;
         e->resid_restrict_mode = s->resid_restrict;
+        e->rr_rows = s->rr_rows;
         e->post2_mode = s->post_fuse;
     };
     if (F.var) {
@@ -2512,6 +2553,15 @@ static int fine64(p4g_solver_s* s, Src src, Out out, int cls, double bytes, long
     }
     OpConst<double> oc;
     oc.d = F.d; oc.db = F.db; oc.sx = F.sx; oc.sy = F.sy; oc.sz = F.sz; oc.id = oc.idb = 1.0;
+    if constexpr (lds_stageable<Src>::value) {
+        // the LDS-staged pass where the engine's passes use it (the same rule)
+        constexpr int W = TX * VecOf<double>::V;
+        if (F.dim == 3 && !F.gal && s->lds != 0 && (F.mx - 1) % W == 0 && (s->lds == 1 || F.mz >= 400))
+            return s->cx.launch(cls, true, bytes, [&] {
+                k_stencil_lds<double, OpConst<double>, Src, Out, TY, 1><<<g, bk, 0, st>>>(
+                    F.mx, F.my, F.mz, kchf(F.dim, F.my, F.mz), oc, src, out, part);
+            });
+    }
     return s->cx.launch(cls, true, bytes, [&] {
         if (F.dim == 3)
             k_stencil<double, 3, OpConst<double>, Src, Out><<<g, bk, 0, st>>>(F.mx, F.my, F.mz, kchf(F.dim, F.my, F.mz), oc, src, out, part);

@@ -77,6 +77,8 @@ struct ViRun {
     }
 };
 
+static int asm_setup(p4g_solver_s* s, ViRun& R);
+
 // the finest operator of the current active set and, under -pc_type mg, its
 // Galerkin coarse levels, coarse factor and Chebyshev bounds
 static int vi_operator(p4g_solver_s* s, ViRun& R) {
@@ -92,6 +94,7 @@ static int vi_operator(p4g_solver_s* s, ViRun& R) {
     e.g27[L] = s->vi_g27;
     e.vidiag = s->vi_g27 + 13 * R.n;
     if (s->pctype == PCT_ILU) return ilu27_pivots(s->lv[L], s->vi_g27, s->ilu_u[L], R.st);   // PETSc's default PC
+    if (s->pctype == PCT_ASM) return asm_setup(s, R);
     if (s->pctype != PCT_MG) return 0;
     P4G_TRY(galerkin_levels(s));
     for (int l = 0; l < L; ++l) e.g27[l] = s->g27[l];
@@ -106,6 +109,146 @@ static int vi_operator(p4g_solver_s* s, ViRun& R) {
     }
     if (s->smoother == SM_CHEB)   // one level: the level smoother is the preconditioner
         for (int l = L > 0 ? 1 : 0; l <= L; ++l) P4G_TRY(estimate_level(s, l));
+    return 0;
+}
+
+// dense inverse by Gauss-Jordan elimination with partial pivoting (the
+// subdomain solve is exact up to rounding, as PETSc's subdomain LU is)
+static bool dense_inverse(int m, std::vector<double>& A, double* X) {
+    for (int i = 0; i < m; ++i)
+        for (int j = 0; j < m; ++j) X[(long)i * m + j] = i == j ? 1.0 : 0.0;
+    for (int c = 0; c < m; ++c) {
+        int pr = c;
+        for (int r = c + 1; r < m; ++r)
+            if (std::fabs(A[(long)r * m + c]) > std::fabs(A[(long)pr * m + c])) pr = r;
+        if (A[(long)pr * m + c] == 0.0) return false;
+        if (pr != c)
+            for (int j = 0; j < m; ++j) {
+                std::swap(A[(long)c * m + j], A[(long)pr * m + j]);
+                std::swap(X[(long)c * m + j], X[(long)pr * m + j]);
+            }
+        const double piv = A[(long)c * m + c];
+        for (int j = 0; j < m; ++j) { A[(long)c * m + j] /= piv; X[(long)c * m + j] /= piv; }
+        for (int r = 0; r < m; ++r) {
+            if (r == c) continue;
+            const double f = A[(long)r * m + c];
+            if (f == 0.0) continue;
+            for (int j = 0; j < m; ++j) {
+                A[(long)r * m + j] -= f * A[(long)c * m + j];
+                X[(long)r * m + j] -= f * X[(long)c * m + j];
+            }
+        }
+    }
+    return true;
+}
+
+// PCASM over the reduced (inactive-set) operator, as PETSc's default PCASM
+// runs on -p4g_ranks processes: one subdomain per process, its inactive
+// vertices of the DMDA's ownership block (DMSetUp_DA_2D's process grid),
+// grown by -pc_asm_overlap levels of the reduced operator's graph
+// (MatIncreaseOverlap), solved exactly (-sub_pc_type lu), the restricted
+// form (a subdomain's solution kept on its own rows only).  The subdomains
+// follow the active set, so they are rebuilt on the host every Newton step
+// and uploaded as index lists and dense inverses (subdomains up to 2048
+// vertices).
+static int asm_setup(p4g_solver_s* s, ViRun& R) {
+    const int mx = R.g.mx, my = R.g.my;
+    const long n = R.n;
+    if (R.g.dim != 2) return set_error(PETSC_ERR_SUP, "-pc_type asm is implemented for 2D grids");
+    std::vector<unsigned char> act(n);
+    HIP_TRY(hipStreamSynchronize(R.st));
+    HIP_TRY(hipMemcpy(act.data(), s->vi_act, n, hipMemcpyDeviceToHost));
+    const int P = s->ranks;
+    int m = (int)(0.5 + std::sqrt((double)mx * P / my));
+    if (!m) m = 1;
+    int nn = 1;
+    while (m > 0) {
+        nn = P / m;
+        if (m * nn == P) break;
+        --m;
+    }
+    if (mx > my && m < nn) std::swap(m, nn);
+    std::vector<int> xs(m + 1, 0), ys(nn + 1, 0);
+    for (int q = 0; q < m; ++q) xs[q + 1] = xs[q] + mx / m + (mx % m > q);
+    for (int q = 0; q < nn; ++q) ys[q + 1] = ys[q] + my / nn + (my % nn > q);
+    std::vector<int> rx(mx), ry(my);
+    for (int i = 0, q = 0; i < mx; ++i) { while (i >= xs[q + 1]) ++q; rx[i] = q; }
+    for (int j = 0, q = 0; j < my; ++j) { while (j >= ys[q + 1]) ++q; ry[j] = q; }
+    auto inner = [&](long v) {
+        const int i = (int)(v % mx), j = (int)(v / mx);
+        return i > 0 && i < mx - 1 && j > 0 && j < my - 1;
+    };
+    const long nb4[4] = {-(long)mx, -1, 1, (long)mx};
+    std::vector<int> idx_all;
+    std::vector<double> inv_all;
+    std::vector<long> rows;
+    std::vector<char> in(n), nx(n);
+    std::vector<long> loc(n, -1);
+    for (int q = 0; q < P; ++q) {
+        for (long v = 0; v < n; ++v)
+            in[v] = !act[v] && (ry[v / mx] * m + rx[v % mx]) == q;
+        for (int o = 0; o < s->asm_overlap; ++o) {
+            nx = in;
+            for (long v = 0; v < n; ++v)
+                if (in[v] && inner(v))
+                    for (long d : nb4)
+                        if (inner(v + d) && !act[v + d]) nx[v + d] = 1;
+            in.swap(nx);
+        }
+        std::vector<long> list;
+        for (long v = 0; v < n; ++v)
+            if (in[v]) list.push_back(v);
+        const long c = (long)list.size();
+        if (c == 0) continue;
+        if (c > MAX_COARSE)
+            return set_error(PETSC_ERR_SUP, "-pc_type asm: a subdomain of %ld vertices (dense solves up to %d)", c,
+                             MAX_COARSE);
+        for (long k = 0; k < c; ++k) loc[list[k]] = k;
+        std::vector<double> A((size_t)c * c, 0.0);
+        for (long k = 0; k < c; ++k) {
+            const long v = list[k];
+            const bool iv = inner(v);
+            A[(size_t)k * c + k] = iv ? R.g.d : R.g.db;
+            if (!iv) continue;
+            for (long d : nb4) {
+                const long u = v + d;
+                if (!inner(u) || act[u] || loc[u] < 0) continue;
+                A[(size_t)k * c + loc[u]] = (d == 1 || d == -1) ? -R.g.sx : -R.g.sy;
+            }
+        }
+        const long ioff = (long)inv_all.size(), xoff = (long)idx_all.size();
+        inv_all.resize(inv_all.size() + (size_t)c * c);
+        if (!dense_inverse((int)c, A, inv_all.data() + ioff))
+            return set_error(PETSC_ERR_LIB, "-pc_type asm: singular subdomain matrix");
+        for (long k = 0; k < c; ++k) {
+            idx_all.push_back((int)list[k]);
+            const long v = list[k];
+            if ((ry[v / mx] * m + rx[v % mx]) == q) {   // owned: the restricted form writes it back
+                rows.push_back(ioff + k * c);
+                rows.push_back(xoff);
+                rows.push_back(c);
+                rows.push_back(v);
+            }
+        }
+        for (long k = 0; k < c; ++k) loc[list[k]] = -1;
+    }
+    auto grow = [&](void** p, size_t* cap, size_t bytes) -> int {
+        if (*cap >= bytes && *p) return 0;
+        if (*p) hipFree(*p);
+        *p = nullptr;
+        *cap = 0;
+        const size_t want = std::max<size_t>(bytes * 2, 4096);
+        if (hipMalloc(p, want) != hipSuccess) return set_error(PETSC_ERR_MEM, "hipMalloc (PCASM)");
+        *cap = want;
+        return 0;
+    };
+    P4G_TRY(grow((void**)&s->asm_idx, &s->asm_cap_idx, idx_all.size() * sizeof(int)));
+    P4G_TRY(grow((void**)&s->asm_inv, &s->asm_cap_inv, inv_all.size() * sizeof(double)));
+    P4G_TRY(grow((void**)&s->asm_row, &s->asm_cap_row, rows.size() * sizeof(long)));
+    if (!idx_all.empty()) HIP_TRY(hipMemcpy(s->asm_idx, idx_all.data(), idx_all.size() * sizeof(int), hipMemcpyHostToDevice));
+    if (!inv_all.empty()) HIP_TRY(hipMemcpy(s->asm_inv, inv_all.data(), inv_all.size() * sizeof(double), hipMemcpyHostToDevice));
+    if (!rows.empty()) HIP_TRY(hipMemcpy(s->asm_row, rows.data(), rows.size() * sizeof(long), hipMemcpyHostToDevice));
+    s->asm_nrows = (long)rows.size() / 4;
     return 0;
 }
 
@@ -234,7 +377,7 @@ static int vi_solve(p4g_solver_s* s, const double* xl, const double* xu, double*
         return set_error(PETSC_ERR_SUP, "-snes_type %s: p4g_vi_solve implements vinewtonrsls and vinewtonssls",
                          s->snes_type.c_str());
     const bool ssls = s->snes_type == "vinewtonssls";
-    if (ssls && (s->pctype == PCT_MG || s->pctype == PCT_JACOBI || xu))
+    if (ssls && (s->pctype == PCT_MG || s->pctype == PCT_JACOBI || s->pctype == PCT_ASM || xu))
         return set_error(PETSC_ERR_SUP, "p4g_vi_solve: vinewtonssls is implemented for lower bounds with -pc_type "
                                         "ilu (the default) or none");
     if (s->ls_type != "bt" && s->ls_type != "basic")
@@ -336,6 +479,21 @@ static int vi_solve(p4g_solver_s* s, const double* xl, const double* xu, double*
     out.fnorm0 = fnorm;
     s->vi_fhist.push_back(fnorm);
     P4G_TRY(sor_pipes_reset(s));
+    // PCASM: the engine's preconditioner while this solve runs
+    struct AsmScope {
+        Engine<double>* e;
+        ~AsmScope() { e->asm_op = nullptr; }
+    } asm_scope{s->e64.get()};
+    if (s->pctype == PCT_ASM) {
+        s->e64->asm_op = [s, &R](const double* r, double* z) -> int {
+            k_fill<double><<<R.nb, 256, 0, R.st>>>(R.n, 0.0, z);
+            if (s->asm_nrows > 0)
+                k_asm_apply<<<(int)std::min<long>((s->asm_nrows + 255) / 256, VBLOCKS), 256, 0, R.st>>>(
+                    s->asm_nrows, s->asm_row, s->asm_idx, s->asm_inv, r, z);
+            LAUNCH_CHECK();
+            return 0;
+        };
+    }
     int reason = vi_converged(s, 0, 0.0, 0.0, fnorm, ttol, rnorm0);
     int i = 0, iter = 0;
     for (; !reason && i < s->snes_maxit; ++i) {

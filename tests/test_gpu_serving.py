@@ -113,3 +113,26 @@ def test_no_graph_matches_graph(sessions):
     with capi.Server(8 << 30, "-p4g_serve_graph 0") as g0:
         x0, _ = g0.wait(g0.submit(s.solver, b))
     assert np.array_equal(x0, x1)
+
+
+def test_pooled_vectors_and_staging_reuse(sessions):
+    # a request's device vectors come back to the server's pool and its
+    # pinned staging buffer is reused: a later request of the same layout
+    # (after another layout and another right-hand side) gives the first
+    # result bit for bit; the submitted array may change right after submit
+    s, t = sessions["3d65"], sessions["2d257"]
+    b1 = _rhs(s)
+    b2 = b1 * -0.5 + 0.25 * np.roll(b1, 7)
+    with capi.Server(8 << 30) as srv:
+        x1, _ = srv.wait(srv.submit(s.solver, b1))
+        srv.wait(srv.submit(t.solver, _rhs(t)))
+        bb = b2.copy()
+        rid = srv.submit(s.solver, bb)
+        bb[:] = np.nan                     # staged at submit: the request does not see this
+        x2, rep2 = srv.wait(rid)
+        x3, _ = srv.wait(srv.submit(s.solver, b1))
+        st = srv.stats()
+    assert rep2.reason > 0 and np.all(np.isfinite(x2))
+    assert not np.array_equal(x1, x2)
+    assert np.array_equal(x1, x3)
+    assert st.completed == 4

@@ -13,9 +13,11 @@ Oracle pins:
     counts 4, 6, 5 and the last KSP count 6); the line search's trial points
     are projected onto the bounds, and without that projection the oracle
     prints other counts (checked);
-  * the error line of obstacle.test2 (GMRES+ASM on 4 ranks: not restated)
-    is the 9x9 VI solution's, which the oracle reproduces with its own
-    solvers;
+  * obstacle.test2 byte for byte (GMRES with PCASM on 4 processes: one
+    subdomain per process of the 2x2 DMDA split, overlap 1, exact subdomain
+    solves, the restricted form), with the 4 processes simulated
+    (-oracle_nranks 4); one or two processes print other histories
+    (checked);
   * an independent projected SOR solve of the discrete complementarity
     problem (numpy) on 17x17 and 33x33;
   * multigrid on the inactive sets converges in a grid-independent number
@@ -71,6 +73,22 @@ def test_oracle_golden_test3():
     assert rc == 0
     assert text == t["expected"]
     assert res.stage_its[:4] == [1, 2, 2, 3]
+
+
+def test_oracle_golden_test2_asm():
+    t = GOLDEN[2]
+    rc, text, res, _ = O.obstacle_run(t["options"] + " -oracle_nranks %d" % t["nprocs"])
+    assert rc == 0, text
+    assert text == t["expected"]
+
+
+@pytest.mark.parametrize("nranks", [1, 2])
+def test_oracle_test2_needs_its_four_subdomains(nranks):
+    t = GOLDEN[2]
+    rc, text, res, _ = O.obstacle_run(t["options"] + " -oracle_nranks %d" % nranks)
+    assert rc == 0, text
+    assert text != t["expected"]
+    assert text.splitlines()[-1] == t["expected"].splitlines()[-1]   # the same VI solution
 
 
 def test_oracle_golden_test4_ssls():
@@ -229,6 +247,12 @@ CASES = {
     "ssls_33_ilu": "-snes_type vinewtonssls -da_refine 4",
     "ssls_seq4_gmres": "-snes_type vinewtonssls -snes_grid_sequence 4 -ksp_type gmres",
     "ssls_17_none": "-snes_type vinewtonssls -da_refine 3 -pc_type none",
+    # PCASM (restricted, exact subdomain solves) over simulated processes
+    "asm_33_gmres_4ranks": "-da_refine 4 -ksp_type gmres -pc_type asm -sub_pc_type lu -oracle_nranks 4",
+    "asm_17_cg_2ranks_overlap2": ("-da_refine 3 -pc_type asm -sub_pc_type lu -pc_asm_overlap 2 "
+                                  "-oracle_nranks 2"),
+    "asm_seq3_gmres_6ranks": ("-snes_grid_sequence 3 -ksp_type gmres -pc_type asm -sub_pc_type lu "
+                              "-oracle_nranks 6"),
 }
 SHOW = " -snes_monitor_short -ksp_converged_reason -snes_converged_reason"
 
@@ -252,8 +276,9 @@ def _fnorms(text):
 def test_obstacle_device_parity(case):
     from p4pdes_amd import capi
     args = CASES[case] + SHOW
-    print("obstacle case", case, args, flush=True)
-    rc, text, res, u = capi.obstacle_run(args, want_u=True)
+    dev = re.sub(r"-oracle_nranks (\d+)", r"-p4g_ranks \1", args)   # the device's simulated processes
+    print("obstacle case", case, dev, flush=True)
+    rc, text, res, u = capi.obstacle_run(dev, want_u=True)
     assert rc == 0, text
     rco, texto, reso, uo = O.obstacle_run(args, want_u=True)
     assert rco == 0, texto
@@ -295,6 +320,18 @@ def test_obstacle_device_golden_test3_default_smoother():
     t = GOLDEN[3]
     print("obstacle golden 3 default smoother", flush=True)
     rc, text, res, _ = capi.obstacle_run(t["options"])
+    assert rc == 0, text
+    assert text == t["expected"]
+
+
+@pytest.mark.gpu
+def test_obstacle_device_golden_test2_asm():
+    # obstacle.test2 as the reference runs it on 4 processes (GMRES with
+    # PCASM, one subdomain per process), byte for byte
+    from p4pdes_amd import capi
+    t = GOLDEN[2]
+    print("obstacle golden 2 asm", flush=True)
+    rc, text, res, _ = capi.obstacle_run(t["options"] + " -p4g_ranks %d" % t["nprocs"])
     assert rc == 0, text
     assert text == t["expected"]
 
