@@ -584,3 +584,26 @@ def test_sor3_stall_stops_the_solve():
     assert "pipelined SOR sweep stalled" in text
     rc, text, _, _ = capi.fish_run(args)
     assert rc == 0, text
+
+
+def test_c3_513_fp64_and_mixed():
+    # C3 at its full size (513^3, 135 M unknowns; the LDS-staged passes and
+    # the fused residual + restriction and post-smoothing run on its finest
+    # level): both precisions to the true residual 1e-10, the discretization
+    # error of the noisy manufactured problem (its noise-free closed form is
+    # 3.137469e-6), and the two solutions agree to 1e-9
+    c3 = ("-fsh_dim 3 -fsh_problem manusin -fsh_noise_sigma 1e-3 -fsh_noise_seed 0 -da_refine 8 "
+          "-pc_mg_levels 8 %s -mg_levels_ksp_max_it 2 -ksp_rtol 1e-10" % MG)
+    assert abs(_eig_err_3d(513) - 3.137469e-06) < 1e-11
+    sol = {}
+    for prec, extra in (("double", " -ksp_norm_type unpreconditioned"), ("mixed", " -p4g_precision mixed")):
+        print("C3 513^3", prec, flush=True)
+        with capi.Session(c3 + extra) as s:
+            rep = s.solve()
+            print("  its", rep.its, "relres", rep.true_relres, flush=True)
+            assert rep.reason > 0
+            assert rep.true_relres <= 1e-10 * 1.0001
+            einf, _ = s.errors()
+            assert abs(einf - 3.1329e-06) < 2e-3 * 3.1329e-06
+            sol[prec] = s.solution()
+    assert _rel(sol["mixed"], sol["double"]) < 1e-9

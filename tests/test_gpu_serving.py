@@ -136,3 +136,31 @@ def test_pooled_vectors_and_staging_reuse(sessions):
     assert not np.array_equal(x1, x2)
     assert np.array_equal(x1, x3)
     assert st.completed == 4
+
+
+def test_full_size_set_one_request_each():
+    # C5's full size set, 33^3 to 257^3 and 257^2 to 2049^2, one request of
+    # each submitted together: every result solves its system to the
+    # solver's rtol (true FP64 residual through the device operator)
+    import bench
+    sess = {}
+    try:
+        for dim, r in bench.SERVE_SHAPES:
+            sess[(dim, r)] = capi.Session("-fsh_dim %d -da_refine %d -pc_mg_levels %d %s"
+                                          % (dim, r, r, bench.SERVE_OPTS))
+        with capi.Server(64 << 30) as srv:
+            ids = {}
+            for (dim, r), s in sess.items():
+                m = 2 ** (r + 1) + 1
+                b, xh = bench.sine_mode_request(dim, m, 100 + r)
+                ids[(dim, r)] = (srv.submit(s.solver, b), b, xh)
+            for key, (rid, b, xh) in ids.items():
+                x, rep = srv.wait(rid)
+                print("size", key, "its", rep.its, "relres", rep.true_relres, flush=True)
+                assert rep.reason > 0
+                assert rep.true_relres <= 1e-8 * 1.0001
+                # the exact discrete solution of the sine-mode right-hand side
+                assert np.max(np.abs(x - xh)) <= 1e-6 * np.max(np.abs(xh))
+    finally:
+        for s in sess.values():
+            s.close()
