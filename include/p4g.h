@@ -129,6 +129,12 @@ int p4g_da_destroy(p4g_DA *da);
  *   -p4g_mixed_x64 1|0 (default 1): 0 keeps the correction since the last
  *       replacement in FP32 and folds it into the FP64 iterate at each
  *       replacement,
+ *   -p4g_x_defer d (0..7, default 6; FP64 iterate): the FP64 iterate
+ *       updates x += alpha p of up to d iterations are deferred, their FP32
+ *       directions kept (d more FP32 vectors), and applied in iteration
+ *       order at the next replacement (the same FP64 rounding as applying
+ *       each at once; a replacement is forced when d are pending); 0
+ *       applies each in the next direction pass,
  *   -p4g_fuse_cg auto|1|0 (default auto): the CG residual update fused into
  *       the first kernel of the multigrid cycle and the direction update
  *       with the operator apply; auto fuses when one fine-level vector is
@@ -137,19 +143,20 @@ int p4g_da_destroy(p4g_DA *da);
  *       value of -p4g_fuse_cg) set the two fusions separately,
  *   -p4g_lds auto|1|0 (default auto: 3D levels of 400+ points per side):
  *       stencil passes with their neighbour rows staged in LDS,
- *   -p4g_resid_restrict auto|1|0 (default auto: where -p4g_lds applies):
- *       the cycle's residual and restriction in one pass, -p4g_rr_rows 8|4
- *       its coarse rows per block,
- *   -p4g_post_fuse auto|1|0: the prolongation fused with the first
- *       post-smoothing step,
+ *   -p4g_resid_restrict auto|1|0 (default auto: FP64 levels where -p4g_lds
+ *       applies): the cycle's residual and restriction in one pass,
+ *       -p4g_rr_rows 8|4 its coarse rows per block,
+ *   -p4g_post_fuse auto|1|0 (auto and 1: levels where -p4g_lds applies,
+ *       with two or more Chebyshev steps): the prolongation fused with the
+ *       first post-smoothing step,
  *   -p4g_sor_pipeline 1|0 (default 1): SOR levels swept by pipelined
  *       strips, one launch per direction (else one per hyperplane).
  * Options and combinations the device path does not implement are rejected
  * with PETSC_ERR_SUP and a message naming them, never silently replaced:
- * e.g. -pc_type lu|sor|asm, -pc_mg_type additive, ILU with fill, SOR with
- * omega != 1 or one-directional sweeps, -mg_levels_ksp_norm_type, and SOR,
- * ILU, GMRES, Richardson, Galerkin or kaskade together with
- * -p4g_precision mixed. */
+ * e.g. -pc_type lu|sor, -pc_type asm outside p4g_vi_solve, -pc_mg_type
+ * additive, ILU with fill, SOR with omega != 1 or one-directional sweeps,
+ * -mg_levels_ksp_norm_type, and SOR, ILU, GMRES, Richardson, Galerkin or
+ * kaskade together with -p4g_precision mixed. */
 int p4g_solver_create(p4g_DA da, const char *options, p4g_Solver *s);
 /* PCSetUp_MG equivalent: Jacobian callbacks per level, stencil extraction,
  * device hierarchy, Chebyshev eigen-estimates, coarse Cholesky. */

@@ -225,7 +225,7 @@ struct SrcDir {                       // p_new = z + beta p_old, stored (CG dire
 
 // p_new = z + beta p_old, stored; at the own point also the deferred CG
 // iterate update x += alpha p_old (alpha of the previous iteration; X may be
-// wider than T).
+// wider than T), or x = alpha p_old with init (x not read).
 template <typename T, typename X>
 struct SrcDirX {
     const T* __restrict__ z;
@@ -234,6 +234,17 @@ struct SrcDirX {
     const double* __restrict__ beta;  // device scalar
     X* __restrict__ x;
     const double* __restrict__ alpha; // device scalar
+    bool init = false;
+    template <int VX>
+    __device__ __forceinline__ TV<X, VX> xload(long q) const {
+        if (init) {
+            TV<X, VX> z0;
+#pragma unroll
+            for (int e = 0; e < VX; ++e) z0.v[e] = X(0);
+            return z0;
+        }
+        return ldv<X, VX>(x + q);
+    }
     __device__ __forceinline__ T load(long q) const { return z[q] + T(*beta) * pold[q]; }
     template <int V>
     __device__ __forceinline__ TV<T, V> loadv(long q) const {
@@ -246,7 +257,7 @@ struct SrcDirX {
     }
     __device__ __forceinline__ void put(long q, T v) const {
         pnew[q] = v;
-        x[q] += X(*alpha) * X(pold[q]);
+        x[q] = (init ? X(0) : x[q]) + X(*alpha) * X(pold[q]);
     }
     template <int V>
     __device__ __forceinline__ void putv(long q, const TV<T, V>& u) const {
@@ -257,7 +268,7 @@ struct SrcDirX {
         constexpr int VX = 16 / sizeof(X) < V ? 16 / sizeof(X) : V;
 #pragma unroll
         for (int h = 0; h < V; h += VX) {
-            TV<X, VX> xx = ldv<X, VX>(x + q + h);
+            TV<X, VX> xx = xload<VX>(q + h);
 #pragma unroll
             for (int e = 0; e < VX; ++e) xx.v[e] += al * X(pp.v[h + e]);
             stv<X, VX>(x + q + h, xx);
@@ -275,7 +286,7 @@ struct SrcDirX {
         constexpr int VX = 16 / sizeof(X) < V ? 16 / sizeof(X) : V;
 #pragma unroll
         for (int h = 0; h < V; h += VX) {
-            TV<X, VX> xx = ldv<X, VX>(x + q + h);
+            TV<X, VX> xx = xload<VX>(q + h);
 #pragma unroll
             for (int e = 0; e < VX; ++e) xx.v[e] += al * X(pp.v[h + e]);
             stv<X, VX>(x + q + h, xx);
@@ -334,6 +345,40 @@ struct SrcAxpyPut {
     __device__ __forceinline__ void put(long q, X v) const { xnew[q] = v; }
     template <int V>
     __device__ __forceinline__ void putv(long q, const TV<X, V>& u) const { stv<X, V>(xnew + q, u); }
+};
+
+// u = x + e + alpha p (X wide; e, p narrow), stored to xnew: an FP32
+// correction and a pending iterate update folded into an FP64 residual pass.
+template <typename X, typename P>
+struct SrcAxpySumPut {
+    const X* __restrict__ x;
+    const P* __restrict__ e;
+    const P* __restrict__ p;
+    const double* __restrict__ alpha;  // device scalar
+    X* __restrict__ xnew;
+    __device__ __forceinline__ X load(long q) const { return x[q] + X(e[q]) + X(*alpha) * X(p[q]); }
+    template <int V>
+    __device__ __forceinline__ TV<X, V> loadv(long q) const {
+        const X al = X(*alpha);
+        TV<X, V> xx = ldv<X, V>(x + q);
+        const TV<P, V> ee = ldv<P, V>(e + q);
+        const TV<P, V> pp = ldv<P, V>(p + q);
+#pragma unroll
+        for (int k = 0; k < V; ++k) xx.v[k] = xx.v[k] + X(ee.v[k]) + al * X(pp.v[k]);
+        return xx;
+    }
+    __device__ __forceinline__ void put(long q, X v) const { xnew[q] = v; }
+    template <int V>
+    __device__ __forceinline__ void putv(long q, const TV<X, V>& u) const { stv<X, V>(xnew + q, u); }
+};
+
+// the CG directions p_j (FP32) and step lengths alpha_j (device scalars) of
+// the iterations whose iterate update is deferred, in iteration order
+constexpr int kFoldMax = 8;
+struct FoldList {
+    const float* p[kFoldMax];
+    const double* a[kFoldMax];
+    int m;
 };
 
 template <typename T, typename E>
@@ -1971,7 +2016,8 @@ k_prolong_smooth(int mx, int my, int mz, int mxc, int myc, int kch, Op op, const
 // ----------------------------- LDS-staged fused prolongation + smoothing
 //
 // prolong_smooth_3d (one smoothing step applied to t = base + P x_c without
-// storing t) with its inputs staged in LDS, FP64 (two fine points per lane).
+// storing t) with its inputs staged in LDS, FP64 or FP32 (two fine points per
+// lane either way).
 // A block of TYD waves covers the fine rows j0 .. j0+TYD-1 (j0 = TYD by) of
 // W = 128 points (is = 1 + W bx, coarse columns 64 bx .. 64 bx + 65) and
 // marches the planes k0 .. k1-1.  LDS-DMA loads stage each base plane's
@@ -1988,13 +2034,17 @@ k_prolong_smooth(int mx, int my, int mz, int mxc, int myc, int kch, Op op, const
 // a barrier.  Requires (mx - 1) % W == 0; the stencil's per-point order is
 // prolong_smooth_3d's, the block partition (and so the dot-product
 // summation order) is stencil_lds's.
-template <class Op, bool ZG, class Out, int TYD>
+template <typename T, class Op, bool ZG, class Out, int TYD>
 __device__ __forceinline__ void prolong_smooth_lds(int bx, int by, int bz, int mx, int my, int mz, int mxc, int myc,
-                                                   int kch, const Op& op, const double* __restrict__ xb, double c,
-                                                   const double* __restrict__ xc, const Out& out, double& a0,
+                                                   int kch, const Op& op, const T* __restrict__ xb, T c,
+                                                   const T* __restrict__ xc, const Out& out, double& a0,
                                                    double& a1) {
-    typedef double T;
     constexpr int V = 2, W = TX * V, CW = W / 2 + 2, NCR = TYD / 2 + 2;
+    // LDS-DMA geometry: dwords per element, elements per 16-byte lane load,
+    // lanes that load a W-point row and a W/2-point coarse row
+    constexpr int DPE = (int)sizeof(T) / 4, EPL = 16 / (int)sizeof(T);
+    constexpr int NLD = W * (int)sizeof(T) / 16, NLDC = (W / 2) * (int)sizeof(T) / 16;
+    static_assert(DPE == 1 || DPE == 2, "4- or 8-byte elements");
     constexpr int SB = 3, ST = 3, SC = 4;
     constexpr int NP = ZG ? 0 : Out::NPRE, NPA = NP > 0 ? NP : 1;
     static_assert(!ZG || Out::NPRE == 1, "with ZG the base is the output's b");
@@ -2021,12 +2071,15 @@ __device__ __forceinline__ void prolong_smooth_lds(int bx, int by, int bz, int m
         if (jr < 0 || jr >= my) return;
         const int slot = (kk + 3) % SB;
         const T* g = xb + (long)kk * sm + (long)jr * mx;
-        __builtin_amdgcn_global_load_lds((const void*)(g + is + V * lane), (lds_void_t*)&bring[slot][r][0], 16, 0, 0);
-        if (lane < 4) {
-            const bool east = lane >= 2;
+        if (lane < NLD)
+            __builtin_amdgcn_global_load_lds((const void*)(g + is + EPL * lane), (lds_void_t*)&bring[slot][r][0], 16, 0,
+                                             0);
+        if (lane < 2 * DPE) {   // dword lane % DPE of the west (is-1) or east (is+W) value
+            const bool east = lane >= DPE;
             if (!east || has_e)
-                __builtin_amdgcn_global_load_lds((const void*)((const char*)(g + (east ? is + W : is - 1)) + 4 * (lane & 1)),
-                                                 (lds_void_t*)&bhalo[slot][r][0], 4, 0, 0);
+                __builtin_amdgcn_global_load_lds(
+                    (const void*)((const char*)(g + (east ? is + W : is - 1)) + 4 * (lane % DPE)),
+                    (lds_void_t*)&bhalo[slot][r][0], 4, 0, 0);
         }
     };
     // coarse plane K: row Jlo + w of this wave (w < NCR), columns ic .. ic+65
@@ -2035,12 +2088,14 @@ __device__ __forceinline__ void prolong_smooth_lds(int bx, int by, int bz, int m
         const int J = Jlo + w;
         if (J >= myc) return;
         const T* g = xc + (long)K * smc + (long)J * mxc + ic;
-        if (lane < TX / 2)
-            __builtin_amdgcn_global_load_lds((const void*)(g + 2 * lane), (lds_void_t*)&cring[K % SC][w][0], 16, 0, 0);
-        if (lane < 4) {   // columns ic+64 (lanes 0, 1) and ic+65 (lanes 2, 3; in range when fine is+W is)
-            if (lane < 2 || has_e)
-                __builtin_amdgcn_global_load_lds((const void*)((const char*)(g + W / 2 + (lane >> 1)) + 4 * (lane & 1)),
-                                                 (lds_void_t*)&cring[K % SC][w][W / 2], 4, 0, 0);
+        if (lane < NLDC)
+            __builtin_amdgcn_global_load_lds((const void*)(g + EPL * lane), (lds_void_t*)&cring[K % SC][w][0], 16, 0,
+                                             0);
+        if (lane < 2 * DPE) {   // columns ic+64 and ic+65 (in range when fine is+W is), a dword per lane
+            if (lane < DPE || has_e)
+                __builtin_amdgcn_global_load_lds(
+                    (const void*)((const char*)(g + W / 2 + lane / DPE) + 4 * (lane % DPE)),
+                    (lds_void_t*)&cring[K % SC][w][W / 2], 4, 0, 0);
         }
     };
     int kst = -1;   // highest coarse plane staged (uniform)
@@ -2176,13 +2231,14 @@ __device__ __forceinline__ void prolong_smooth_lds(int bx, int by, int bz, int m
     }
 }
 
-template <class Op, bool ZG, class Out, int TYD>
+template <typename T, class Op, bool ZG, class Out, int TYD>
 __global__ void __launch_bounds__(TX * TYD)
-k_prolong_smooth_lds(int mx, int my, int mz, int mxc, int myc, int kch, Op op, const double* __restrict__ xb,
-                     double c, const double* __restrict__ xc, Out out, double* __restrict__ part) {
+k_prolong_smooth_lds(int mx, int my, int mz, int mxc, int myc, int kch, Op op, const T* __restrict__ xb, T c,
+                     const T* __restrict__ xc, Out out, double* __restrict__ part) {
     double a0 = 0.0, a1 = 0.0;
     const Tile tl = xcd_tile();
-    prolong_smooth_lds<Op, ZG, Out, TYD>(tl.bx, tl.by, tl.bz, mx, my, mz, mxc, myc, kch, op, xb, c, xc, out, a0, a1);
+    prolong_smooth_lds<T, Op, ZG, Out, TYD>(tl.bx, tl.by, tl.bz, mx, my, mz, mxc, myc, kch, op, xb, c, xc, out, a0,
+                                            a1);
     block_partials<Out::NDOT, TYD>(a0, a1, part, tl.id);
 }
 
@@ -2213,34 +2269,6 @@ k_coarse(int n, const double* __restrict__ Ainv, const T* __restrict__ b, T* __r
 }
 
 // ------------------------------------------------------- vector kernels
-
-// x = keep x + a p ; r -= a w  (a = S[ialpha], keep = S[ikeep] in {0,1});
-// NORM: acc0 += r.r
-template <typename T, int NORM>
-__global__ void __launch_bounds__(256)
-k_cg_update(long n, const double* __restrict__ S, int ialpha, int ikeep, T* __restrict__ x,
-            const T* __restrict__ p, T* __restrict__ r, const T* __restrict__ w,
-            double* __restrict__ part) {
-    const T a = T(S[ialpha]);
-    const bool keep = S[ikeep] != 0.0;
-    double acc = 0.0;
-    for (long q = blockIdx.x * 256L + threadIdx.x; q < n; q += (long)gridDim.x * 256) {
-        x[q] = keep ? x[q] + a * p[q] : a * p[q];
-        const T rv = r[q] - a * w[q];
-        r[q] = rv;
-        if (NORM) acc += double(rv) * double(rv);
-    }
-    if (NORM) {
-        __shared__ double sm[4];
-        acc = wave_sum(acc);
-        if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = acc;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            part[2 * blockIdx.x] = sm[0] + sm[1] + sm[2] + sm[3];
-            part[2 * blockIdx.x + 1] = 0.0;
-        }
-    }
-}
 
 // y = c x (pointwise), optional dots acc0 += y.x, acc1 += y.y
 template <typename T, int DOTS>
@@ -2292,13 +2320,13 @@ k_dot2(long n, const T* __restrict__ x, const T* __restrict__ y, double* __restr
 template <typename T, typename X>
 __global__ void __launch_bounds__(256)
 k_dir_x(long n, const double* __restrict__ beta, const double* __restrict__ alpha, const T* __restrict__ z,
-        const T* __restrict__ pold, T* __restrict__ pnew, X* __restrict__ x) {
+        const T* __restrict__ pold, T* __restrict__ pnew, X* __restrict__ x, int init) {
     const T bt = T(*beta);
     const X al = X(*alpha);
     for (long q = blockIdx.x * 256L + threadIdx.x; q < n; q += (long)gridDim.x * 256) {
         const T p = pold[q];
         pnew[q] = z[q] + bt * p;
-        x[q] += al * X(p);
+        x[q] = (init ? X(0) : x[q]) + al * X(p);
     }
 }
 
@@ -2326,36 +2354,40 @@ k_r_update(long n, const double* __restrict__ S, int ialpha, const T* __restrict
     }
 }
 
-// x += a p (X wide, P narrow; a = S[ialpha])
-template <typename X, typename P>
-__global__ void __launch_bounds__(256)
-k_x_update(long n, const double* __restrict__ S, int ialpha, X* __restrict__ x, const P* __restrict__ p) {
-    const X a = X(S[ialpha]);
-    for (long q = blockIdx.x * 256L + threadIdx.x; q < n; q += (long)gridDim.x * 256) x[q] += a * X(p[q]);
+// x += alpha_0 p_0 + alpha_1 p_1 + ... + alpha_{M-1} p_{M-1}, one term at a
+// time in that order (the rounding of per-iteration updates): deferred
+// iterate updates, four points per thread (16-byte accesses), M = f.m
+template <int M>
+__global__ void __launch_bounds__(256) k_fold_x(long n, double* __restrict__ x, FoldList f) {
+    double al[M];
+#pragma unroll
+    for (int j = 0; j < M; ++j) al[j] = *f.a[j];
+    const long n4 = n / 4;
+    for (long t = blockIdx.x * 256L + threadIdx.x; t < n4; t += (long)gridDim.x * 256) {
+        const long q = 4 * t;
+        TV<double, 2> u0 = ldv<double, 2>(x + q), u1 = ldv<double, 2>(x + q + 2);
+#pragma unroll
+        for (int j = 0; j < M; ++j) {
+            const TV<float, 4> pp = ldv<float, 4>(f.p[j] + q);
+            u0.v[0] += al[j] * double(pp.v[0]);
+            u0.v[1] += al[j] * double(pp.v[1]);
+            u1.v[0] += al[j] * double(pp.v[2]);
+            u1.v[1] += al[j] * double(pp.v[3]);
+        }
+        stv<double, 2>(x + q, u0);
+        stv<double, 2>(x + q + 2, u1);
+    }
+    for (long q = 4 * n4 + blockIdx.x * 256L + threadIdx.x; q < n; q += (long)gridDim.x * 256) {
+        double u = x[q];
+#pragma unroll
+        for (int j = 0; j < M; ++j) u += al[j] * double(f.p[j][q]);
+        x[q] = u;
+    }
 }
 
-// x += a p in FP64 (x FP64, p FP32); r -= a w in FP32; acc0 += r.r
-__global__ void __launch_bounds__(256)
-k_cg_update_x64(long n, const double* __restrict__ S, int ialpha, double* __restrict__ x,
-                const float* __restrict__ p, float* __restrict__ r, const float* __restrict__ w,
-                double* __restrict__ part) {
-    const double a = S[ialpha];
-    const float af = float(a);
-    double acc = 0.0;
-    for (long q = blockIdx.x * 256L + threadIdx.x; q < n; q += (long)gridDim.x * 256) {
-        x[q] += a * double(p[q]);
-        const float rv = r[q] - af * w[q];
-        r[q] = rv;
-        acc += double(rv) * double(rv);
-    }
-    __shared__ double sm[4];
-    acc = wave_sum(acc);
-    if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = acc;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        part[2 * blockIdx.x] = sm[0] + sm[1] + sm[2] + sm[3];
-        part[2 * blockIdx.x + 1] = 0.0;
-    }
+// *dst = *src (a device scalar saved before the slot is overwritten)
+__global__ void __launch_bounds__(64) k_copy_scalar(const double* __restrict__ src, double* __restrict__ dst) {
+    if (threadIdx.x == 0) *dst = *src;
 }
 
 // x += (double) c   (fold an FP32 correction into an FP64 iterate)

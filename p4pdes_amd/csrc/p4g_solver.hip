@@ -647,7 +647,8 @@ struct Engine {
     // (auto: where the LDS-staged passes run)
     bool rr_fusable(const LevelInfo& L) const {
         if (L.dim != 3 || L.var || L.gal || resid_restrict_mode == 0 || (L.mx - 1) % 128 != 0) return false;
-        return resid_restrict_mode == 1 || lds_level(L);
+        // auto: FP64 LDS levels (in FP32 the fused pass is no faster than the two)
+        return resid_restrict_mode == 1 || (std::is_same<T, double>::value && lds_level(L));
     }
 
     // b_c = P^T (b - A x), level l -> l-1, the fine residual not stored
@@ -708,17 +709,15 @@ struct Engine {
             });
         }
         OpConst<T> o = opc(L, jac);
-        if constexpr (std::is_same<T, double>::value) {
-            if (lds_level(L)) {   // LDS-staged form: blocks of TY output rows
-                const dim3 gl(g.x, (L.my + TY - 1) / TY, (L.mz + kch - 1) / kch), bl(TX, TY);
-                if (nb) *nb = nblocks(gl);
-                if (Out::NDOT && nblocks(gl) > cx->npart_cap)
-                    return set_error(PETSC_ERR_LIB, "internal: partial-sum buffer too small");
-                return cx->launch(P4G_K_PROLONG, l == nlev - 1, bytes, [&] {
-                    k_prolong_smooth_lds<OpConst<T>, ZG, Out, TY><<<gl, bl, 0, s>>>(L.mx, L.my, L.mz, C.mx, C.my, kch,
-                                                                                   o, base, cz, xc, out, cx->part);
-                });
-            }
+        if (lds_level(L) && (L.mx - 1) % (2 * TX) == 0) {   // LDS-staged form: blocks of TY output rows
+            const dim3 gl(g.x, (L.my + TY - 1) / TY, (L.mz + kch - 1) / kch), bl(TX, TY);
+            if (nb) *nb = nblocks(gl);
+            if (Out::NDOT && nblocks(gl) > cx->npart_cap)
+                return set_error(PETSC_ERR_LIB, "internal: partial-sum buffer too small");
+            return cx->launch(P4G_K_PROLONG, l == nlev - 1, bytes, [&] {
+                k_prolong_smooth_lds<T, OpConst<T>, ZG, Out, TY><<<gl, bl, 0, s>>>(L.mx, L.my, L.mz, C.mx, C.my, kch, o,
+                                                                                  base, cz, xc, out, cx->part);
+            });
         }
         return cx->launch(P4G_K_PROLONG, l == nlev - 1, bytes, [&] {
             if (L.dim == 3)
@@ -1068,15 +1067,18 @@ struct Engine {
                (nu == 1 || !(nu >= 3 && smoother == SM_CHEB));
     }
     // p_new = z + beta p_old ; w = A p_new ; (p_new, w) ; x += alpha p_old
+    // (init: x = alpha p_old, x not read)
     template <typename X>
-    int direction(const T* zz, const T* pold, T* pnew, X* xx, const double* beta, const double* alpha, long* nbs) {
+    int direction(const T* zz, const T* pold, T* pnew, X* xx, const double* beta, const double* alpha, long* nbs,
+                  bool init = false) {
         const long n = (*lv)[nlev - 1].n;
+        const double xb = (init ? 1.0 : 2.0) * sizeof(X);   // x written (and read unless init)
         if (fuse_dir)
-            return stencil(nlev - 1, SrcDirX<T, X>{zz, pold, pnew, beta, xx, alpha}, OutApplyDot<T>{w},
-                           P4G_K_STENCIL_DOT, 4.0 + 2.0 * sizeof(X) / sizeof(T), nbs, false);
+            return stencil(nlev - 1, SrcDirX<T, X>{zz, pold, pnew, beta, xx, alpha, init}, OutApplyDot<T>{w},
+                           P4G_K_STENCIL_DOT, 4.0 + xb / sizeof(T), nbs, false);
         hipStream_t s = cx->stream;
-        P4G_TRY(cx->launch(P4G_K_CG_UPDATE, true, 3.0 * n * sizeof(T) + 2.0 * n * sizeof(X), [&] {
-            k_dir_x<T, X><<<vblocks(n), 256, 0, s>>>(n, beta, alpha, zz, pold, pnew, xx);
+        P4G_TRY(cx->launch(P4G_K_CG_UPDATE, true, 3.0 * n * sizeof(T) + xb * n, [&] {
+            k_dir_x<T, X><<<vblocks(n), 256, 0, s>>>(n, beta, alpha, zz, pold, pnew, xx, init ? 1 : 0);
         }));
         return stencil(nlev - 1, SrcArr<T>{pnew}, OutApplyDot<T>{w}, P4G_K_STENCIL_DOT, 2, nbs, false);
     }
@@ -1169,9 +1171,9 @@ struct Engine {
 
     // prolongation + first post-smoothing step in one pass for two or more
     // Chebyshev steps (the prolonged iterate is stored for the recurrence):
-    // FP64 3D levels of the LDS-staged passes, -p4g_post_fuse auto|1|0
+    // 3D levels of the LDS-staged passes, -p4g_post_fuse auto|1|0
     bool post2_fusable(const LevelInfo& L) const {
-        if (!std::is_same<T, double>::value || L.dim != 3 || L.var || L.gal || sor || vidiag) return false;
+        if (L.dim != 3 || L.var || L.gal || sor || vidiag) return false;
         if (smoother != SM_CHEB || nu < 2 || post2_mode == 0 || !lds_level(L)) return false;
         return true;
     }
@@ -1735,6 +1737,8 @@ struct p4g_solver_s {
     double reliable_delta = 1e-1;
     bool fp32_diff = true;    // FP32 engine: difference-form rows
     bool mixed_x64 = true;    // mixed: accumulate the iterate in FP64 every iteration
+    int x_defer = 6;          // mixed, FP64 iterate: updates deferred to the next replacement (at most)
+    float* pdef[kFoldMax] = {};   //   their direction buffers
     int fuse_r = -1, fuse_dir = -1;   // CG updates fused into stencil passes: 1 on, 0 off, -1 by rule
     bool galerkin = false;            // coarse operators P^T A P (-pc_mg_galerkin)
     bool fmg = false;                 // -pc_mg_type full: one F-cycle per preconditioner apply
@@ -2226,6 +2230,9 @@ static int configure(p4g_solver_s* s) {
     s->reliable_delta = o.getr("p4g_reliable_delta", 1e-1);
     s->fp32_diff = o.getb("p4g_fp32_diff_form", true);
     s->mixed_x64 = o.getb("p4g_mixed_x64", true);
+    s->x_defer = o.geti("p4g_x_defer", 6);
+    if (s->x_defer < 0 || s->x_defer > kFoldMax - 1)
+        return set_error(PETSC_ERR_ARG_OUTOFRANGE, "-p4g_x_defer must be in 0..%d", kFoldMax - 1);
     {
         auto tri = [&](const std::string& f, const char* name, int& v) -> int {
             if (f == "auto") v = -1;
@@ -2461,8 +2468,8 @@ int solver_setup(p4g_solver_s* s) {
         HIP_TRY(hipMalloc(&s->cx.part, 2 * cap * sizeof(double)));
         HIP_TRY(hipMalloc(&s->cx.part_rr, 2 * cap * sizeof(double)));
         s->cx.npart_cap = cap;
-        HIP_TRY(hipMalloc(&s->cx.S, 32 * sizeof(double)));
-        double init[32] = {0};
+        HIP_TRY(hipMalloc(&s->cx.S, 64 * sizeof(double)));   // 32..47: saved step lengths
+        double init[64] = {0};
         init[S_ONE] = init[16 + S_ONE] = 1.0;
         HIP_TRY(hipMemcpy(s->cx.S, init, sizeof init, hipMemcpyHostToDevice));
         HIP_TRY(hipHostMalloc((void**)&s->cx.hS, 32 * sizeof(double), hipHostMallocDefault));
@@ -2520,6 +2527,8 @@ int solver_setup(p4g_solver_s* s) {
         P4G_TRY(dalloc(s, (void**)&s->xb, n * sizeof(double)));
         P4G_TRY(dalloc(s, (void**)&s->r32, n * sizeof(float)));
         P4G_TRY(dalloc(s, (void**)&s->e32v, n * sizeof(float)));
+        if (s->mixed_x64)
+            for (int j = 0; j < s->x_defer; ++j) P4G_TRY(dalloc(s, (void**)&s->pdef[j], n * sizeof(float)));
     } else {
         s->e64.reset(new Engine<double>);
         cfg(s->e64);
@@ -2572,14 +2581,43 @@ static int fine64(p4g_solver_s* s, Src src, Out out, int cls, double bytes, long
     });
 }
 
+// x += alpha_0 p_0 + ... + alpha_{m-1} p_{m-1} (f.m = m terms, in order), then
+// the list is empty
+static int fold_x(p4g_solver_s* s, double* x, FoldList& f) {
+    const long n = s->lv.back().n;
+    hipStream_t st = s->cx.stream;
+    const int gb = vblocks((n + 3) / 4);
+    if (f.m < 1 || f.m > kFoldMax) return set_error(PETSC_ERR_LIB, "internal: fold list of %d", f.m);
+    P4G_TRY(s->cx.launch(P4G_K_OTHER, true, (16.0 + 4.0 * f.m) * n, [&] {
+        switch (f.m) {
+        case 1: k_fold_x<1><<<gb, 256, 0, st>>>(n, x, f); break;
+        case 2: k_fold_x<2><<<gb, 256, 0, st>>>(n, x, f); break;
+        case 3: k_fold_x<3><<<gb, 256, 0, st>>>(n, x, f); break;
+        case 4: k_fold_x<4><<<gb, 256, 0, st>>>(n, x, f); break;
+        case 5: k_fold_x<5><<<gb, 256, 0, st>>>(n, x, f); break;
+        case 6: k_fold_x<6><<<gb, 256, 0, st>>>(n, x, f); break;
+        case 7: k_fold_x<7><<<gb, 256, 0, st>>>(n, x, f); break;
+        default: k_fold_x<8><<<gb, 256, 0, st>>>(n, x, f); break;
+        }
+    }));
+    f.m = 0;
+    return 0;
+}
+
 // Mixed precision: FP32 PCG (FP32 vectors, FP32 V-cycle, FP64 coarse
 // solve and dot accumulation) with FP64 residual replacement ("reliable
-// updates", van der Vorst & Ye, SISC 22 (2000)): the FP32 iterate holds
-// the correction since the last replacement; when the iterated residual
-// has fallen by delta (or below the target), the correction is folded into
-// the FP64 iterate and r = b - A x is recomputed in FP64 in the same pass.
-// The Krylov recurrence continues across replacements; convergence is
-// decided on that FP64 residual: |b - A x|_2 <= max(rtol |b|_2, atol).
+// updates", van der Vorst & Ye, SISC 22 (2000)): when the iterated residual
+// has fallen by delta (or below the target), r = b - A x is recomputed in
+// FP64 in the pass that applies the iteration's update to x.  The iterate
+// update x += alpha p goes to the FP64 iterate (-p4g_mixed_x64 1): the
+// directions and step lengths of up to -p4g_x_defer iterations are kept and
+// the replacement pass applies them in order (a replacement is forced when
+// the list is full), or with -p4g_x_defer 0 the next direction pass applies
+// each; or (-p4g_mixed_x64 0) the next direction pass adds it to an FP32
+// correction x32 holding the updates since the last replacement, which the
+// replacement pass folds into the FP64 iterate.  The Krylov recurrence continues across
+// replacements; convergence is decided on that FP64 residual:
+// |b - A x|_2 <= max(rtol |b|_2, atol).
 static int mixed_solve(p4g_solver_s* s, const double* b, double* x) {
     Engine<float>& e = *s->e32;
     const LevelInfo& F = s->lv.back();
@@ -2591,7 +2629,7 @@ static int mixed_solve(p4g_solver_s* s, const double* b, double* x) {
     const double* hs = cx->hS;
     const double* ho = cx->hS + 16;
     const int gb = vblocks(n);
-    float* x32 = e.xbuf;        // correction since the last replacement
+    float* x32 = e.xbuf;        // correction since the last replacement (x64 off)
     float* r32 = e.r;
     P4G_TRY(cx->launch(P4G_K_OTHER, true, 20.0 * n, [&] {
         k_fill<double><<<gb, 256, 0, st>>>(n, 0.0, s->xa);
@@ -2619,24 +2657,45 @@ static int mixed_solve(p4g_solver_s* s, const double* b, double* x) {
     double* xnext = s->xb;
     if (bnorm <= s->atol) R.reason = P4G_CONVERGED_ATOL;
     const double delta = s->reliable_delta;
-    const bool x64 = s->mixed_x64;    // FP64 iterate updated every iteration (no FP32 correction)
+    const bool x64 = s->mixed_x64;    // deferred updates into the FP64 iterate (else into x32)
     double rr_last = bnorm * bnorm;   // |r|^2 at the last replacement
-    bool replace_now = false, folded = true;
-    int keep = S_ZERO;                // the first update sets x32 = alpha p
+    bool replace_now = false;
+    bool x32live = false;             // x32 holds updates not yet folded into the FP64 iterate
     float* pold = e.pa;
     float* pnew = e.pb;
     double dpi_old = 0.0;
     int i = 0, nrep = 0;
-    float* r32n = e.r2;               // x64: the residual update goes to r32n
-    bool xpend = false;               // x64: x += alpha p of the last iteration not yet applied
-    // direction update (with the deferred FP64 iterate update when one is
+    float* r32n = e.r2;               // the residual update goes to r32n
+    bool xpend = false;               // x += alpha p of the last iteration not yet applied
+    // deferred FP64 iterate updates: p_j in a ring of FP32 buffers (never
+    // overwritten while listed), alpha_j copied to S[32 + j]
+    const int D = x64 ? s->x_defer : 0;
+    std::vector<float*> ring = {e.pa, e.pb};
+    for (int j = 0; j < D; ++j) ring.push_back(s->pdef[j]);
+    FoldList fl{};
+    double* Sal = cx->S + 32;
+    auto listed = [&](const float* p) {
+        for (int j = 0; j < fl.m; ++j)
+            if (fl.p[j] == p) return true;
+        return false;
+    };
+    // direction update (with the deferred iterate update when one is
     // pending); the next one is queued before the host waits for the
     // scalars of the current iteration (it needs no host decision)
     auto direction = [&]() -> int {
         long nbs = 0;
-        if (x64 && xpend)
+        if (D) {   // the new direction into a buffer neither current nor listed
+            pnew = nullptr;
+            for (float* b : ring)
+                if (b != pold && !listed(b)) { pnew = b; break; }
+            if (!pnew) return set_error(PETSC_ERR_LIB, "internal: direction buffers exhausted");
+        }
+        if (xpend && x64) {
             P4G_TRY(e.direction<double>(e.z, pold, pnew, xcur, S + S_BCOEF, S + S_ALPHA, &nbs));
-        else
+        } else if (xpend) {
+            P4G_TRY(e.direction<float>(e.z, pold, pnew, x32, S + S_BCOEF, S + S_ALPHA, &nbs, !x32live));
+            x32live = true;
+        } else
             P4G_TRY(e.stencil(s->nlev - 1, SrcDir<float>{e.z, pold, pnew, S + S_BCOEF}, OutApplyDot<float>{e.w},
                               P4G_K_STENCIL_DOT, 4, &nbs, false));
         return cx->finalize(nbs, S, FIN_ALPHA, 0);
@@ -2645,53 +2704,58 @@ static int mixed_solve(p4g_solver_s* s, const double* b, double* x) {
     for (; i < s->maxit && !R.reason; ++i) {
         R.its = i + 1;
         if (hs[S_RZ] == 0.0) replace_now = true;   // exact breakdown: certify by replacement
+        if (D && fl.m == D) replace_now = true;    // the deferred list is full
         std::swap(pold, pnew);
         const bool replaced = replace_now;
-        if (x64) {
-            if (replace_now) {
-                // x <- x + alpha p and r = b - A x in FP64 in one pass; the
-                // recursive residual update of this iteration is not needed
-                long nbk = 0;
-                P4G_TRY(fine64(s, SrcAxpyPut<double, float>{xcur, pold, S + S_ALPHA, xnext},
-                               OutResidCvt<double, float>{b, r32}, P4G_K_OTHER, 32.0 * n, &nbk));
-                P4G_TRY(cx->finalize(nbk, So, FIN_RAW, 0));
-                std::swap(xcur, xnext);
-                xpend = false;
-                replace_now = false;
-                ++nrep;
-                P4G_TRY(e.pc_apply_dots(r32, e.z, &nb));
+        if (replace_now) {
+            // x <- x (+ x32) + alpha p and r = b - A x in FP64 in one pass,
+            // or the deferred updates and this one applied to x in order and
+            // then r = b - A x (the residual pass then stages x in LDS); the
+            // recursive residual update of this iteration is not needed
+            long nbk = 0;
+            if (D) {
+                fl.p[fl.m] = pold;
+                fl.a[fl.m] = S + S_ALPHA;
+                ++fl.m;
+                P4G_TRY(fold_x(s, xcur, fl));
+                P4G_TRY(fine64(s, SrcArr<double>{xcur}, OutResidCvt<double, float>{b, r32}, P4G_K_OTHER, 28.0 * n,
+                               &nbk));
             } else {
-                // r - alpha w fused into the first kernel of the V-cycle
-                e.pend.r = r32;
-                e.pend.w = e.w;
-                e.pend.alpha = S + S_ALPHA;
-                e.pend.rn = r32n;
-                e.pend.rr = true;
-                e.pend.on = true;
-                xpend = true;
-                P4G_TRY(e.pc_apply_dots(r32n, e.z, &nb));
-                std::swap(r32, r32n);
-                P4G_TRY(cx->finalize(e.pend.nparts, S, FIN_RR, 0, cx->part_rr));
-            }
-        } else {
-            P4G_TRY(cx->launch(P4G_K_CG_UPDATE, true, 24.0 * n, [&] {
-                k_cg_update<float, 1><<<gb, 256, 0, st>>>(n, S, S_ALPHA, keep, x32, pold, r32, e.w, cx->part);
-            }));
-            P4G_TRY(cx->finalize(gb, S, FIN_RR, 0));
-            keep = S_ONE;
-            folded = false;
-            if (replace_now) {
-                long nbk = 0;
-                P4G_TRY(fine64(s, SrcSum<double, float>{xcur, x32, xnext}, OutResidCvt<double, float>{b, r32},
-                               P4G_K_OTHER, 36.0 * n, &nbk));
+                if (x32live)
+                    P4G_TRY(fine64(s, SrcAxpySumPut<double, float>{xcur, x32, pold, S + S_ALPHA, xnext},
+                                   OutResidCvt<double, float>{b, r32}, P4G_K_OTHER, 36.0 * n, &nbk));
+                else
+                    P4G_TRY(fine64(s, SrcAxpyPut<double, float>{xcur, pold, S + S_ALPHA, xnext},
+                                   OutResidCvt<double, float>{b, r32}, P4G_K_OTHER, 32.0 * n, &nbk));
                 std::swap(xcur, xnext);
-                keep = S_ZERO;
-                folded = true;
-                P4G_TRY(cx->finalize(nbk, So, FIN_RAW, 0));
-                replace_now = false;
-                ++nrep;
             }
+            P4G_TRY(cx->finalize(nbk, So, FIN_RAW, 0));
+            xpend = false;
+            x32live = false;
+            replace_now = false;
+            ++nrep;
             P4G_TRY(e.pc_apply_dots(r32, e.z, &nb));
+        } else {
+            // r - alpha w fused into the first kernel of the V-cycle
+            e.pend.r = r32;
+            e.pend.w = e.w;
+            e.pend.alpha = S + S_ALPHA;
+            e.pend.rn = r32n;
+            e.pend.rr = true;
+            e.pend.on = true;
+            if (D) {   // x += alpha p deferred to the next replacement
+                P4G_TRY(cx->launch(P4G_K_OTHER, false, 0.0, [&] {
+                    k_copy_scalar<<<1, 64, 0, st>>>(S + S_ALPHA, Sal + fl.m);
+                }));
+                fl.p[fl.m] = pold;
+                fl.a[fl.m] = Sal + fl.m;
+                ++fl.m;
+            } else {
+                xpend = true;
+            }
+            P4G_TRY(e.pc_apply_dots(r32n, e.z, &nb));
+            std::swap(r32, r32n);
+            P4G_TRY(cx->finalize(e.pend.nparts, S, FIN_RR, 0, cx->part_rr));
         }
         P4G_TRY(cx->finalize(nb, S, FIN_BETA_PRECOND, 0));
         P4G_TRY(cx->readS_begin(cx->hS));
@@ -2717,7 +2781,8 @@ static int mixed_solve(p4g_solver_s* s, const double* b, double* x) {
     }
     if (!R.reason) R.reason = P4G_DIVERGED_ITS;
     R.inner_its = nrep;   // number of FP64 residual replacements
-    if (!folded) {        // fold the last correction: x = x + x32
+    if (fl.m) P4G_TRY(fold_x(s, xcur, fl));   // deferred updates not yet applied
+    if (x32live) {        // fold the last correction: x = x + x32
         P4G_TRY(cx->launch(P4G_K_OTHER, true, 20.0 * n, [&] {
             k_axpy_mixed<<<gb, 256, 0, st>>>(n, xcur, x32);
         }));

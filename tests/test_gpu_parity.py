@@ -441,7 +441,10 @@ def test_mixed_matches_fp64(refine):
           _rel(u, ou))
 
 
-@pytest.mark.parametrize("variant", ["-p4g_mixed_x64 0", "-p4g_fp32_diff_form 0", "-p4g_reliable_delta 0.01",
+@pytest.mark.parametrize("variant", ["-p4g_mixed_x64 0", "-p4g_mixed_x64 1", "-p4g_mixed_x64 0 -p4g_fuse_cg 1",
+                                     "-p4g_mixed_x64 1 -p4g_fuse_cg 1", "-p4g_x_defer 0", "-p4g_x_defer 1",
+                                     "-p4g_x_defer 7 -p4g_reliable_delta 0.001",
+                                     "-p4g_fp32_diff_form 0", "-p4g_reliable_delta 0.01",
                                      "-fsh_problem varcoef", "-p4g_fuse_cg 1", "-p4g_fuse_cg 1 -fsh_problem varcoef",
                                      "-p4g_fuse_r 1 -p4g_fuse_dir 0", "-p4g_fuse_r 0 -p4g_fuse_dir 1",
                                      "-pc_mg_type full"])
@@ -460,6 +463,23 @@ def test_mixed_variants(variant):
     assert abs(res.errinf - ores.errinf) < 1e-4 * ores.errinf
     assert _rel(u, ou) < 1e-7
     print("  its", res.its, "oracle its", ores.its, "replacements", res.inner_its, flush=True)
+
+
+@pytest.mark.parametrize("case", ["-da_refine 5 -pc_mg_levels 4", "-da_refine 7 -p4g_fuse_cg 1",
+                                  "-da_refine 6 -p4g_reliable_delta 0.001 -pc_mg_levels 5"])
+def test_x_defer_matches_per_iteration_updates(case):
+    # the FP64 iterate updates deferred to the replacement pass add the same
+    # alpha_j p_j in the same order as the per-iteration updates of the
+    # direction pass: the same solve bit for bit
+    base = ("-fsh_dim 3 -fsh_problem manusin -fsh_noise_sigma 1e-3 %s %s -ksp_rtol 1e-10 "
+            "-p4g_precision mixed" % (MG, case))
+    print("case", case, flush=True)
+    rc0, t0, r0, u0 = capi.fish_run(base + " -p4g_x_defer 0", want_u=True)
+    rc1, t1, r1, u1 = capi.fish_run(base, want_u=True)
+    assert rc0 == 0 and rc1 == 0, (t0, t1)
+    assert r0.reason > 0 and r0.its == r1.its and r0.inner_its == r1.inner_its
+    print("  its", r1.its, "replacements", r1.inner_its, flush=True)
+    assert np.array_equal(u0, u1), _rel(u1, u0)
 
 
 def test_c4_varcoef_257_mixed_and_fp64():
@@ -494,6 +514,10 @@ LDS_PAIRS = {
                         "-ksp_rtol 1e-10 -p4g_precision mixed" % MG),
     "mixed_257_cheb1_fused": ("-fsh_dim 3 -fsh_problem manusin -da_refine 7 %s -mg_levels_ksp_max_it 1 "
                               "-ksp_rtol 1e-10 -p4g_precision mixed -p4g_fuse_cg 1" % MG),
+    "mixed_257_cheb2_fused_x32": ("-fsh_dim 3 -fsh_problem manusin -da_refine 7 %s -ksp_rtol 1e-10 "
+                                  "-p4g_precision mixed -p4g_fuse_cg 1 -p4g_mixed_x64 0" % MG),
+    "mixed_257_cheb2_fused_x64": ("-fsh_dim 3 -fsh_problem manusin -da_refine 7 %s -ksp_rtol 1e-10 "
+                                  "-p4g_precision mixed -p4g_fuse_cg 1 -p4g_mixed_x64 1" % MG),
 }
 
 
@@ -536,6 +560,24 @@ def test_fused_resid_restrict_pc_apply(prec):
     for mode in ("0", "1"):
         print("resid_restrict", prec, mode, flush=True)
         with capi.Session(args + " -p4g_resid_restrict " + mode) as sess:
+            zs.append(capi.Solver(sess).pc_apply(r))
+    assert _rel(zs[1], zs[0]) < (1e-14 if prec == "double" else 1e-5)
+
+
+@pytest.mark.parametrize("prec", ["double", "mixed"])
+def test_fused_prolong_smooth_pc_apply(prec):
+    # one multigrid cycle z = B r with the prolongation fused into the first
+    # of two Chebyshev post-smoothing steps (k_prolong_smooth_lds, two fine
+    # points per lane, the prolonged iterate kept for the second step) and as
+    # separate passes: the same per-point formulas up to the FMA
+    # contractions the compiler picks per kernel
+    args = ("-fsh_dim 3 -fsh_problem manusin -da_refine 7 %s -mg_levels_ksp_max_it 2 -p4g_lds 1 "
+            "-p4g_precision %s" % (MG, prec))
+    r = np.random.default_rng(6).standard_normal(257 ** 3)
+    zs = []
+    for mode in ("0", "1"):
+        print("post_fuse", prec, mode, flush=True)
+        with capi.Session(args + " -p4g_post_fuse " + mode) as sess:
             zs.append(capi.Solver(sess).pc_apply(r))
     assert _rel(zs[1], zs[0]) < (1e-14 if prec == "double" else 1e-5)
 
