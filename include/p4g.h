@@ -145,7 +145,8 @@ int p4g_da_destroy(p4g_DA *da);
  *       stencil passes with their neighbour rows staged in LDS,
  *   -p4g_resid_restrict auto|1|0 (default auto: FP64 levels where -p4g_lds
  *       applies): the cycle's residual and restriction in one pass,
- *       -p4g_rr_rows 8|4 its coarse rows per block,
+ *       -p4g_rr_rows 8|4 its coarse rows per block, -p4g_rr_planes n its
+ *       coarse planes per block (default 0: by grid size),
  *   -p4g_post_fuse auto|1|0 (auto and 1: levels where -p4g_lds applies,
  *       with two or more Chebyshev steps): the prolongation fused with the
  *       first post-smoothing step,

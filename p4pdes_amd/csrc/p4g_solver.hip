@@ -415,6 +415,7 @@ struct Engine {
     // with rr_rows coarse rows per block (-p4g_rr_rows 8|4)
     int resid_restrict_mode = -1;
     int rr_rows = 8;
+    int rr_planes = 0;   // its coarse planes per block (-p4g_rr_planes; 0: by size)
     // the prolongation and the first of several post-smoothing steps in one
     // pass (-p4g_post_fuse)
     int post2_mode = -1;
@@ -657,7 +658,7 @@ struct Engine {
         const LevelInfo& F = (*lv)[l];
         const LevelInfo& C = (*lv)[l - 1];
         const int tyc = rr_rows == 4 ? 4 : 8;
-        const int kch = kch_restrict(C.dim, C.my, C.mz);
+        const int kch = rr_planes > 0 ? rr_planes : kch_restrict(C.dim, C.my, C.mz);
         const dim3 g((C.mx - 1 + TX - 1) / TX, (C.my + tyc - 1) / tyc, (C.mz + kch - 1) / kch), b(TX, tyc);
         hipStream_t s = cx->stream;
         OpConst<T> o = opc(F, false);
@@ -1757,6 +1758,7 @@ struct p4g_solver_s {
     int lds = -1;                     // -p4g_lds auto|1|0: LDS-staged 3D stencil passes
     int resid_restrict = -1;          // -p4g_resid_restrict auto|1|0: fused residual + restriction
     int rr_rows = 8;                  // -p4g_rr_rows 8|4: its coarse rows per block
+    int rr_planes = 0;                // -p4g_rr_planes n: its coarse planes per block (0: by size)
     int asm_overlap = 1;              // -pc_asm_overlap (PCASM, VI solves)
     int ranks = 1;                    // -p4g_ranks: the processes whose DMDA split PCASM's subdomains follow
     // PCASM's subdomains on the device: per owned row its subdomain's index
@@ -2246,6 +2248,9 @@ static int configure(p4g_solver_s* s) {
         s->rr_rows = o.geti("p4g_rr_rows", 8);
         if (s->rr_rows != 4 && s->rr_rows != 8)
             return set_error(PETSC_ERR_ARG_OUTOFRANGE, "-p4g_rr_rows must be 4 or 8");
+        s->rr_planes = o.geti("p4g_rr_planes", 0);
+        if (s->rr_planes < 0 || s->rr_planes > 64)
+            return set_error(PETSC_ERR_ARG_OUTOFRANGE, "-p4g_rr_planes must be in 0..64");
         P4G_TRY(tri(o.gets("p4g_post_fuse", "auto"), "p4g_post_fuse", s->post_fuse));
         const std::string both = o.gets("p4g_fuse_cg", "auto");
         P4G_TRY(tri(both, "p4g_fuse_cg", s->fuse_r));
@@ -2509,6 +2514,7 @@ int solver_setup(p4g_solver_s* s) {
         e->lds = s->lds;
         e->resid_restrict_mode = s->resid_restrict;
         e->rr_rows = s->rr_rows;
+        e->rr_planes = s->rr_planes;
         e->post2_mode = s->post_fuse;
     };
     if (F.var) {
