@@ -2016,10 +2016,10 @@ k_prolong_smooth(int mx, int my, int mz, int mxc, int myc, int kch, Op op, const
 // ----------------------------- LDS-staged fused prolongation + smoothing
 //
 // prolong_smooth_3d (one smoothing step applied to t = base + P x_c without
-// storing t) with its inputs staged in LDS, FP64 or FP32 (two fine points per
-// lane either way).
+// storing t) with its inputs staged in LDS, V fine points per lane (FP64: 2,
+// FP32: 4, 16-byte accesses either way).
 // A block of TYD waves covers the fine rows j0 .. j0+TYD-1 (j0 = TYD by) of
-// W = 128 points (is = 1 + W bx, coarse columns 64 bx .. 64 bx + 65) and
+// W = 64 V points (is = 1 + W bx, coarse columns W/2 bx .. W/2 bx + W/2 + 1) and
 // marches the planes k0 .. k1-1.  LDS-DMA loads stage each base plane's
 // rows j0-1 .. j0+TYD (and their x-halo values) one plane ahead, and each
 // coarse plane's rows (j0-1)/2 .. (j0+TYD)/2 when the first fine plane that
@@ -2034,12 +2034,13 @@ k_prolong_smooth(int mx, int my, int mz, int mxc, int myc, int kch, Op op, const
 // a barrier.  Requires (mx - 1) % W == 0; the stencil's per-point order is
 // prolong_smooth_3d's, the block partition (and so the dot-product
 // summation order) is stencil_lds's.
-template <typename T, class Op, bool ZG, class Out, int TYD>
+template <typename T, int V, class Op, bool ZG, class Out, int TYD>
 __device__ __forceinline__ void prolong_smooth_lds(int bx, int by, int bz, int mx, int my, int mz, int mxc, int myc,
                                                    int kch, const Op& op, const T* __restrict__ xb, T c,
                                                    const T* __restrict__ xc, const Out& out, double& a0,
                                                    double& a1) {
-    constexpr int V = 2, W = TX * V, CW = W / 2 + 2, NCR = TYD / 2 + 2;
+    static_assert(V == 2 || V == 4, "two or four fine points per lane");
+    constexpr int W = TX * V, CW = W / 2 + 2, NCR = TYD / 2 + 2, VC = V / 2;   // VC coarse intervals per lane
     // LDS-DMA geometry: dwords per element, elements per 16-byte lane load,
     // lanes that load a W-point row and a W/2-point coarse row
     constexpr int DPE = (int)sizeof(T) / 4, EPL = 16 / (int)sizeof(T);
@@ -2091,7 +2092,7 @@ __device__ __forceinline__ void prolong_smooth_lds(int bx, int by, int bz, int m
         if (lane < NLDC)
             __builtin_amdgcn_global_load_lds((const void*)(g + EPL * lane), (lds_void_t*)&cring[K % SC][w][0], 16, 0,
                                              0);
-        if (lane < 2 * DPE) {   // columns ic+64 and ic+65 (in range when fine is+W is), a dword per lane
+        if (lane < 2 * DPE) {   // columns ic+W/2 and ic+W/2+1 (in range when fine is+W is), a dword per lane
             if (lane < DPE || has_e)
                 __builtin_amdgcn_global_load_lds(
                     (const void*)((const char*)(g + W / 2 + lane / DPE) + 4 * (lane % DPE)),
@@ -2127,17 +2128,26 @@ __device__ __forceinline__ void prolong_smooth_lds(int bx, int by, int bz, int m
             }
             return ya;
         };
-        const T pw = pc(lane), pv = pc(lane + 1);        // coarse I-1, I of the lane's pair (2I-1, 2I)
-        const T2<T> bb = *(const T2<T>*)&bring[bs][r][V * lane];
-        const T b0 = ZG ? c * bb.a : bb.a, b1 = ZG ? c * bb.b : bb.b;
-        *(T2<T>*)&tring[ts][r][V * lane] = T2<T>{b0 + h * (pw + pv), b1 + pv};
+        // coarse local columns VC lane .. VC lane + VC: fine point e of the
+        // lane is odd (between two of them) for even e, on one for odd e
+        T pcl[VC + 1];
+#pragma unroll
+        for (int m = 0; m <= VC; ++m) pcl[m] = pc(VC * lane + m);
+        const TV<T, V> bb = ldv<T, V>(&bring[bs][r][V * lane]);
+        TV<T, V> tt;
+#pragma unroll
+        for (int e = 0; e < V; ++e) {
+            const T be = ZG ? c * bb.v[e] : bb.v[e];
+            tt.v[e] = (e & 1) ? be + pcl[(e + 1) / 2] : be + h * (pcl[e / 2] + pcl[e / 2 + 1]);
+        }
+        stv<T, V>(&tring[ts][r][V * lane], tt);
         if (lane == 0) {                                 // fine is-1 (even): coarse ic
             const T x = bhalo[bs][r][0];
-            thalo[ts][r][0] = (ZG ? c * x : x) + pw;
+            thalo[ts][r][0] = (ZG ? c * x : x) + pcl[0];
         }
-        if (lane == TX - 1 && has_e) {                   // fine is+W (odd): coarse ic+64, ic+65
+        if (lane == TX - 1 && has_e) {                   // fine is+W (odd): coarse ic+W/2, ic+W/2+1
             const T x = bhalo[bs][r][1];
-            thalo[ts][r][1] = (ZG ? c * x : x) + h * (pv + pc(lane + 2));
+            thalo[ts][r][1] = (ZG ? c * x : x) + h * (pcl[VC] + pc(VC * lane + VC + 1));
         }
     };
     const int j = j0 + w;
@@ -2147,7 +2157,8 @@ __device__ __forceinline__ void prolong_smooth_lds(int bx, int by, int bz, int m
     const int r = w + 1;
     const int i0 = is + V * lane;
     TV<T, V> zv;
-    zv.v[0] = zv.v[1] = z0;
+#pragma unroll
+    for (int e = 0; e < V; ++e) zv.v[e] = z0;
     TV<T, V> co[NPA], ro[NPA];
     T cox[NPA], rox[NPA];
 #pragma unroll
@@ -2185,19 +2196,33 @@ __device__ __forceinline__ void prolong_smooth_lds(int bx, int by, int bz, int m
         const TV<T, V> uu = k + 1 < mz ? tv(tu, r) : zv;
         TV<T, V> Au = zv, idg = zv;
         if (jb || k == 0 || k == mz - 1) {
-            op.bnd(i0, j, k, uc.v[0], Au.v[0], idg.v[0]);
-            op.bnd(i0 + 1, j, k, uc.v[1], Au.v[1], idg.v[1]);
+#pragma unroll
+            for (int e = 0; e < V; ++e) op.bnd(i0 + e, j, k, uc.v[e], Au.v[e], idg.v[e]);
         } else {
+            // neighbours that are boundary points are left out (elementwise
+            // selects: a select of whole vectors can make the compiler
+            // address them through scratch memory)
             const TV<T, V> sv = ms ? tv(tc, r - 1) : zv, nv = mn ? tv(tc, r + 1) : zv;
-            const TV<T, V> dn = k - 1 > 0 ? ud : zv, up = k + 1 < mz - 1 ? uu : zv;
+            const bool hd = k - 1 > 0, hu = k + 1 < mz - 1;
+            TV<T, V> dn, up;
+#pragma unroll
+            for (int e = 0; e < V; ++e) {
+                dn.v[e] = hd ? ud.v[e] : z0;
+                up.v[e] = hu ? uu.v[e] : z0;
+            }
             const T wv = lane > 0 ? tring[tc][r][V * lane - 1] : thalo[tc][r][0];
             const T ev = lane < TX - 1 ? tring[tc][r][V * lane + V] : thalo[tc][r][1];
-            op.interior(i0, j, k, uc.v[0], i0 - 1 > 0 ? wv : z0, i0 + 1 < mx - 1 ? uc.v[1] : z0, sv.v[0], nv.v[0],
-                        dn.v[0], up.v[0], Au.v[0], idg.v[0]);
-            if (i0 + 1 == mx - 1) op.bnd(i0 + 1, j, k, uc.v[1], Au.v[1], idg.v[1]);
-            else
-                op.interior(i0 + 1, j, k, uc.v[1], uc.v[0], i0 + 2 < mx - 1 ? ev : z0, sv.v[1], nv.v[1], dn.v[1],
-                            up.v[1], Au.v[1], idg.v[1]);
+#pragma unroll
+            for (int e = 0; e < V; ++e) {
+                const int ie = i0 + e;
+                if (e > 0 && ie == mx - 1) {             // the last point of the last tile: boundary
+                    op.bnd(ie, j, k, uc.v[e], Au.v[e], idg.v[e]);
+                    continue;
+                }
+                const T we = e == 0 ? (ie - 1 > 0 ? wv : z0) : uc.v[e - 1];
+                const T ee = ie + 1 < mx - 1 ? (e == V - 1 ? ev : uc.v[e + 1]) : z0;
+                op.interior(ie, j, k, uc.v[e], we, ee, sv.v[e], nv.v[e], dn.v[e], up.v[e], Au.v[e], idg.v[e]);
+            }
         }
         if constexpr (ZG) {
             const TV<T, V> bb = ldv<T, V>(&bring[(k + 3) % SB][r][V * lane]);
@@ -2231,14 +2256,14 @@ __device__ __forceinline__ void prolong_smooth_lds(int bx, int by, int bz, int m
     }
 }
 
-template <typename T, class Op, bool ZG, class Out, int TYD>
+template <typename T, int V, class Op, bool ZG, class Out, int TYD>
 __global__ void __launch_bounds__(TX * TYD)
 k_prolong_smooth_lds(int mx, int my, int mz, int mxc, int myc, int kch, Op op, const T* __restrict__ xb, T c,
                      const T* __restrict__ xc, Out out, double* __restrict__ part) {
     double a0 = 0.0, a1 = 0.0;
     const Tile tl = xcd_tile();
-    prolong_smooth_lds<T, Op, ZG, Out, TYD>(tl.bx, tl.by, tl.bz, mx, my, mz, mxc, myc, kch, op, xb, c, xc, out, a0,
-                                            a1);
+    prolong_smooth_lds<T, V, Op, ZG, Out, TYD>(tl.bx, tl.by, tl.bz, mx, my, mz, mxc, myc, kch, op, xb, c, xc, out,
+                                               a0, a1);
     block_partials<Out::NDOT, TYD>(a0, a1, part, tl.id);
 }
 

@@ -710,14 +710,15 @@ struct Engine {
             });
         }
         OpConst<T> o = opc(L, jac);
-        if (lds_level(L) && (L.mx - 1) % (2 * TX) == 0) {   // LDS-staged form: blocks of TY output rows
-            const dim3 gl(g.x, (L.my + TY - 1) / TY, (L.mz + kch - 1) / kch), bl(TX, TY);
+        constexpr int V = VecOf<T>::V;   // fine points per lane (16-byte accesses)
+        if (lds_level(L) && (L.mx - 1) % (V * TX) == 0) {   // LDS-staged form: blocks of TY output rows
+            const dim3 gl((L.mx - 1) / (V * TX), (L.my + TY - 1) / TY, (L.mz + kch - 1) / kch), bl(TX, TY);
             if (nb) *nb = nblocks(gl);
             if (Out::NDOT && nblocks(gl) > cx->npart_cap)
                 return set_error(PETSC_ERR_LIB, "internal: partial-sum buffer too small");
             return cx->launch(P4G_K_PROLONG, l == nlev - 1, bytes, [&] {
-                k_prolong_smooth_lds<T, OpConst<T>, ZG, Out, TY><<<gl, bl, 0, s>>>(L.mx, L.my, L.mz, C.mx, C.my, kch, o,
-                                                                                  base, cz, xc, out, cx->part);
+                k_prolong_smooth_lds<T, V, OpConst<T>, ZG, Out, TY><<<gl, bl, 0, s>>>(
+                    L.mx, L.my, L.mz, C.mx, C.my, kch, o, base, cz, xc, out, cx->part);
             });
         }
         return cx->launch(P4G_K_PROLONG, l == nlev - 1, bytes, [&] {
