@@ -129,12 +129,14 @@ int p4g_da_destroy(p4g_DA *da);
  *   -p4g_mixed_x64 1|0 (default 1): 0 keeps the correction since the last
  *       replacement in FP32 and folds it into the FP64 iterate at each
  *       replacement,
- *   -p4g_x_defer d (0..7, default 6; FP64 iterate): the FP64 iterate
- *       updates x += alpha p of up to d iterations are deferred, their FP32
- *       directions kept (d more FP32 vectors), and applied in iteration
- *       order at the next replacement (the same FP64 rounding as applying
- *       each at once; a replacement is forced when d are pending); 0
- *       applies each in the next direction pass,
+ *   -p4g_x_defer d (0..7; default 6 in mixed precision, 7 in the FP64 CG
+ *       of p4g_ksp_solve / p4g_snes_solve, 0 in the Newton solves): the
+ *       FP64 iterate updates x += alpha p of up to d iterations are
+ *       deferred, their directions kept (d more vectors), and applied in
+ *       iteration order (the same FP64 rounding as applying each at once):
+ *       in mixed precision at the next residual replacement (one is forced
+ *       when d are pending), in FP64 when d are pending and at the end of
+ *       the solve; 0 applies each in the next direction pass,
  *   -p4g_fuse_cg auto|1|0 (default auto): the CG residual update fused into
  *       the first kernel of the multigrid cycle and the direction update
  *       with the operator apply; auto fuses when one fine-level vector is

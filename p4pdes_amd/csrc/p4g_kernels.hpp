@@ -372,14 +372,17 @@ struct SrcAxpySumPut {
     __device__ __forceinline__ void putv(long q, const TV<X, V>& u) const { stv<X, V>(xnew + q, u); }
 };
 
-// the CG directions p_j (FP32) and step lengths alpha_j (device scalars) of
-// the iterations whose iterate update is deferred, in iteration order
+// the CG directions p_j (FP32 or FP64) and step lengths alpha_j (device
+// scalars) of the iterations whose iterate update is deferred, in
+// iteration order
 constexpr int kFoldMax = 8;
-struct FoldList {
-    const float* p[kFoldMax];
+template <typename P>
+struct FoldListT {
+    const P* p[kFoldMax];
     const double* a[kFoldMax];
     int m;
 };
+typedef FoldListT<float> FoldList;
 
 template <typename T, typename E>
 struct SrcSum {                       // x_new = x + e (refinement update), stored
@@ -2382,8 +2385,8 @@ k_r_update(long n, const double* __restrict__ S, int ialpha, const T* __restrict
 // x += alpha_0 p_0 + alpha_1 p_1 + ... + alpha_{M-1} p_{M-1}, one term at a
 // time in that order (the rounding of per-iteration updates): deferred
 // iterate updates, four points per thread (16-byte accesses), M = f.m
-template <int M>
-__global__ void __launch_bounds__(256) k_fold_x(long n, double* __restrict__ x, FoldList f) {
+template <int M, typename P>
+__global__ void __launch_bounds__(256) k_fold_x(long n, double* __restrict__ x, FoldListT<P> f) {
     double al[M];
 #pragma unroll
     for (int j = 0; j < M; ++j) al[j] = *f.a[j];
@@ -2393,11 +2396,18 @@ __global__ void __launch_bounds__(256) k_fold_x(long n, double* __restrict__ x, 
         TV<double, 2> u0 = ldv<double, 2>(x + q), u1 = ldv<double, 2>(x + q + 2);
 #pragma unroll
         for (int j = 0; j < M; ++j) {
-            const TV<float, 4> pp = ldv<float, 4>(f.p[j] + q);
-            u0.v[0] += al[j] * double(pp.v[0]);
-            u0.v[1] += al[j] * double(pp.v[1]);
-            u1.v[0] += al[j] * double(pp.v[2]);
-            u1.v[1] += al[j] * double(pp.v[3]);
+            P pp[4];
+            if constexpr (sizeof(P) == 4) {
+                const TV<P, 4> v = ldv<P, 4>(f.p[j] + q);
+                pp[0] = v.v[0]; pp[1] = v.v[1]; pp[2] = v.v[2]; pp[3] = v.v[3];
+            } else {
+                const TV<P, 2> v0 = ldv<P, 2>(f.p[j] + q), v1 = ldv<P, 2>(f.p[j] + q + 2);
+                pp[0] = v0.v[0]; pp[1] = v0.v[1]; pp[2] = v1.v[0]; pp[3] = v1.v[1];
+            }
+            u0.v[0] += al[j] * double(pp[0]);
+            u0.v[1] += al[j] * double(pp[1]);
+            u1.v[0] += al[j] * double(pp[2]);
+            u1.v[1] += al[j] * double(pp[3]);
         }
         stv<double, 2>(x + q, u0);
         stv<double, 2>(x + q + 2, u1);

@@ -389,6 +389,29 @@ static inline int kch_restrict(int dim, int my, int mz) {
     return slow >= 200 ? 4 : (slow >= 100 ? 2 : 1);
 }
 
+// x += alpha_0 p_0 + ... + alpha_{m-1} p_{m-1} (f.m = m terms, in order:
+// deferred CG iterate updates), then the list is empty
+template <typename P>
+static int launch_fold_x(Ctx* cx, long n, double* x, FoldListT<P>& f) {
+    hipStream_t st = cx->stream;
+    const int gb = vblocks((n + 3) / 4);
+    if (f.m < 1 || f.m > kFoldMax) return set_error(PETSC_ERR_LIB, "internal: fold list of %d", f.m);
+    P4G_TRY(cx->launch(P4G_K_OTHER, true, (16.0 + sizeof(P) * f.m) * n, [&] {
+        switch (f.m) {
+        case 1: k_fold_x<1, P><<<gb, 256, 0, st>>>(n, x, f); break;
+        case 2: k_fold_x<2, P><<<gb, 256, 0, st>>>(n, x, f); break;
+        case 3: k_fold_x<3, P><<<gb, 256, 0, st>>>(n, x, f); break;
+        case 4: k_fold_x<4, P><<<gb, 256, 0, st>>>(n, x, f); break;
+        case 5: k_fold_x<5, P><<<gb, 256, 0, st>>>(n, x, f); break;
+        case 6: k_fold_x<6, P><<<gb, 256, 0, st>>>(n, x, f); break;
+        case 7: k_fold_x<7, P><<<gb, 256, 0, st>>>(n, x, f); break;
+        default: k_fold_x<8, P><<<gb, 256, 0, st>>>(n, x, f); break;
+        }
+    }));
+    f.m = 0;
+    return 0;
+}
+
 // ------------------------------------------------------------- engine
 
 // Device hierarchy and Krylov work vectors in precision T.
@@ -439,6 +462,10 @@ struct Engine {
     std::function<int(const T*, T*)> mf_op;
     T* mf_tmp = nullptr;
     std::function<int(const T*, T*)> asm_op;   // PCASM's z = B r (set by the VI solve)
+    // CG (FP64): the iterate updates x += alpha p of up to x_defer iterations
+    // deferred and applied in order (directions kept in pdef)
+    int x_defer = 0;
+    T* pdef[kFoldMax] = {};
     bool mf_bypass = false;   // a pass on the preconditioning matrix (PCApplyRichardson_SOR's sweeps)
     bool vi = false;   // levels may become null-row Galerkin levels later (full temporaries)
     int mcycles = 1;        // -pc_mg_multiplicative_cycles
@@ -516,6 +543,8 @@ struct Engine {
         P4G_TRY(alloc(&w, F.n));
         P4G_TRY(alloc(&xbuf, F.n));
         P4G_TRY(alloc(&r2, F.n));
+        if (x_defer < 0 || x_defer > kFoldMax - 1) return set_error(PETSC_ERR_LIB, "internal: x_defer %d", x_defer);
+        for (int j = 0; j < x_defer; ++j) P4G_TRY(alloc(&pdef[j], F.n));
         if (pctype == PCT_MG) {
             n0 = n0_;
             void* q = nullptr;
@@ -1361,6 +1390,49 @@ struct Engine {
         T* rc = r;      // current residual; the update r - alpha w goes to rx
         T* rx = r2;
         double dpi_old = 0.0;
+        // with x_defer (FP64) the direction pass leaves x alone: p_i stays in
+        // a ring of x_defer + 2 buffers (a listed buffer is not reused) with
+        // alpha_i copied to S[32 + j], and the listed updates are applied in
+        // iteration order when the list is full and on exit (the same FP64
+        // rounding as applying each in the next direction pass)
+        const int D = std::is_same<T, double>::value ? x_defer : 0;
+        FoldListT<T> fl{};
+        double* Sal = cx->S + 32;
+        std::vector<T*> ring = {pa, pb};
+        for (int j = 0; j < D; ++j) ring.push_back(pdef[j]);
+        auto listed = [&](const T* p) {
+            for (int j = 0; j < fl.m; ++j)
+                if (fl.p[j] == p) return true;
+            return false;
+        };
+        auto fold = [&]() -> int {
+            if constexpr (std::is_same<T, double>::value) return launch_fold_x(cx, n, x, fl);
+            return set_error(PETSC_ERR_LIB, "internal: deferred updates need an FP64 iterate");
+        };
+        // the next direction (x += alpha_i p_i applied or listed)
+        auto next_direction = [&](bool first) -> int {
+            long nbd = 0;
+            if (!D) {
+                P4G_TRY(direction<T>(z, pold, pnew, x, Sblk + S_BCOEF, Sblk + S_ALPHA, &nbd));
+                return cx->finalize(nbd, Sblk, FIN_ALPHA, 0);
+            }
+            if (!first) {
+                if (fl.m == D) P4G_TRY(fold());
+                P4G_TRY(cx->launch(P4G_K_OTHER, false, 0.0, [&] {
+                    k_copy_scalar<<<1, 64, 0, st>>>(Sblk + S_ALPHA, Sal + fl.m);
+                }));
+                fl.p[fl.m] = pold;
+                fl.a[fl.m] = Sal + fl.m;
+                ++fl.m;
+            }
+            pnew = nullptr;
+            for (T* q : ring)
+                if (q != pold && !listed(q)) { pnew = q; break; }
+            if (!pnew) return set_error(PETSC_ERR_LIB, "internal: direction buffers exhausted");
+            P4G_TRY(stencil(nlev - 1, SrcDir<T>{z, pold, pnew, Sblk + S_BCOEF}, OutApplyDot<T>{w}, P4G_K_STENCIL_DOT,
+                            4, &nbd, false));
+            return cx->finalize(nbd, Sblk, FIN_ALPHA, 0);
+        };
         // x += alpha p is deferred into the next direction update; alpha = 0
         // before the first iteration.  The direction update of iteration i+1
         // needs no host decision (beta is formed on the device), so it is
@@ -1369,9 +1441,7 @@ struct Engine {
         // deferred update, so x is complete.
         k_fill<double><<<1, 256, 0, st>>>(1, 0.0, Sblk + S_ALPHA);
         LAUNCH_CHECK();
-        long nbs = 0;
-        P4G_TRY(direction<T>(z, pold, pnew, x, Sblk + S_BCOEF, Sblk + S_ALPHA, &nbs));
-        P4G_TRY(cx->finalize(nbs, Sblk, FIN_ALPHA, 0));
+        P4G_TRY(next_direction(true));
         int i = 0;
         for (; i < maxit; ++i) {
             o.its = i + 1;
@@ -1388,9 +1458,7 @@ struct Engine {
             if (normtype == NT_UNPRECOND) P4G_TRY(cx->finalize(pend.nparts, Sblk, FIN_RR, 0, cx->part_rr));
             P4G_TRY(cx->finalize(nb, Sblk, fmode, 0));
             P4G_TRY(cx->readS_begin(cx->hS));
-            // next direction (applies x += alpha_i p_i)
-            P4G_TRY(direction<T>(z, pold, pnew, x, Sblk + S_BCOEF, Sblk + S_ALPHA, &nbs));
-            P4G_TRY(cx->finalize(nbs, Sblk, FIN_ALPHA, 0));
+            P4G_TRY(next_direction(false));   // next direction; x += alpha_i p_i applied or listed
             P4G_TRY(cx->readS_end());
             const double dpi = hs[S_PW];
             if (dpi == 0.0 || (i > 0 && ((dpi > 0) != (dpi_old > 0)))) {
@@ -1411,6 +1479,7 @@ struct Engine {
             }
         }
         if (i >= maxit && !o.reason) o.reason = P4G_DIVERGED_ITS;
+        if (fl.m) P4G_TRY(fold());   // the listed updates: x complete on every exit
         return 0;
     }
 
@@ -1739,7 +1808,7 @@ struct p4g_solver_s {
     double reliable_delta = 1e-1;
     bool fp32_diff = true;    // FP32 engine: difference-form rows
     bool mixed_x64 = true;    // mixed: accumulate the iterate in FP64 every iteration
-    int x_defer = 6;          // mixed, FP64 iterate: updates deferred to the next replacement (at most)
+    int x_defer = -1;         // CG iterate updates deferred (at most; -1: by path)
     float* pdef[kFoldMax] = {};   //   their direction buffers
     int fuse_r = -1, fuse_dir = -1;   // CG updates fused into stencil passes: 1 on, 0 off, -1 by rule
     bool galerkin = false;            // coarse operators P^T A P (-pc_mg_galerkin)
@@ -1819,6 +1888,9 @@ struct p4g_solver_s {
         if (cx.stream) hipStreamDestroy(cx.stream);
     }
 };
+
+// deferred FP64 iterate updates of the mixed-precision PCG (-p4g_x_defer, default 6)
+static inline int mixed_defer(const p4g_solver_s* s) { return s->x_defer < 0 ? 6 : s->x_defer; }
 
 namespace p4g {
 
@@ -2233,8 +2305,8 @@ static int configure(p4g_solver_s* s) {
     s->reliable_delta = o.getr("p4g_reliable_delta", 1e-1);
     s->fp32_diff = o.getb("p4g_fp32_diff_form", true);
     s->mixed_x64 = o.getb("p4g_mixed_x64", true);
-    s->x_defer = o.geti("p4g_x_defer", 6);
-    if (s->x_defer < 0 || s->x_defer > kFoldMax - 1)
+    s->x_defer = o.geti("p4g_x_defer", -1);   // -1: 6 (mixed), 7 (FP64 CG of KSP solves)
+    if (s->x_defer < -1 || s->x_defer > kFoldMax - 1)
         return set_error(PETSC_ERR_ARG_OUTOFRANGE, "-p4g_x_defer must be in 0..%d", kFoldMax - 1);
     {
         auto tri = [&](const std::string& f, const char* name, int& v) -> int {
@@ -2535,12 +2607,14 @@ int solver_setup(p4g_solver_s* s) {
         P4G_TRY(dalloc(s, (void**)&s->r32, n * sizeof(float)));
         P4G_TRY(dalloc(s, (void**)&s->e32v, n * sizeof(float)));
         if (s->mixed_x64)
-            for (int j = 0; j < s->x_defer; ++j) P4G_TRY(dalloc(s, (void**)&s->pdef[j], n * sizeof(float)));
+            for (int j = 0; j < mixed_defer(s); ++j) P4G_TRY(dalloc(s, (void**)&s->pdef[j], n * sizeof(float)));
     } else {
         s->e64.reset(new Engine<double>);
         cfg(s->e64);
         fuse_rules(s, s->e64.get(), F.n * sizeof(double));
         if (s->vi_mode) s->e64->fuse_r = s->e64->fuse_dir = false;
+        // KSP solves (not the Newton solves of p4g_vi_solve / p4g_ms_solve)
+        s->e64->x_defer = s->vi_mode || s->ms_mode ? 0 : (s->x_defer < 0 ? 7 : s->x_defer);
         P4G_TRY(s->e64->setup(&s->cx, &s->lv, s->Ainv, s->n0));
     }
     HIP_TRY(hipStreamSynchronize(s->cx.stream));
@@ -2588,28 +2662,9 @@ static int fine64(p4g_solver_s* s, Src src, Out out, int cls, double bytes, long
     });
 }
 
-// x += alpha_0 p_0 + ... + alpha_{m-1} p_{m-1} (f.m = m terms, in order), then
-// the list is empty
-static int fold_x(p4g_solver_s* s, double* x, FoldList& f) {
-    const long n = s->lv.back().n;
-    hipStream_t st = s->cx.stream;
-    const int gb = vblocks((n + 3) / 4);
-    if (f.m < 1 || f.m > kFoldMax) return set_error(PETSC_ERR_LIB, "internal: fold list of %d", f.m);
-    P4G_TRY(s->cx.launch(P4G_K_OTHER, true, (16.0 + 4.0 * f.m) * n, [&] {
-        switch (f.m) {
-        case 1: k_fold_x<1><<<gb, 256, 0, st>>>(n, x, f); break;
-        case 2: k_fold_x<2><<<gb, 256, 0, st>>>(n, x, f); break;
-        case 3: k_fold_x<3><<<gb, 256, 0, st>>>(n, x, f); break;
-        case 4: k_fold_x<4><<<gb, 256, 0, st>>>(n, x, f); break;
-        case 5: k_fold_x<5><<<gb, 256, 0, st>>>(n, x, f); break;
-        case 6: k_fold_x<6><<<gb, 256, 0, st>>>(n, x, f); break;
-        case 7: k_fold_x<7><<<gb, 256, 0, st>>>(n, x, f); break;
-        default: k_fold_x<8><<<gb, 256, 0, st>>>(n, x, f); break;
-        }
-    }));
-    f.m = 0;
-    return 0;
-}
+// x += alpha_0 p_0 + ... + alpha_{m-1} p_{m-1} (f.m = m terms, in order) on
+// the finest level, then the list is empty
+static int fold_x(p4g_solver_s* s, double* x, FoldList& f) { return launch_fold_x(&s->cx, s->lv.back().n, x, f); }
 
 // Mixed precision: FP32 PCG (FP32 vectors, FP32 V-cycle, FP64 coarse
 // solve and dot accumulation) with FP64 residual replacement ("reliable
@@ -2676,7 +2731,7 @@ static int mixed_solve(p4g_solver_s* s, const double* b, double* x) {
     bool xpend = false;               // x += alpha p of the last iteration not yet applied
     // deferred FP64 iterate updates: p_j in a ring of FP32 buffers (never
     // overwritten while listed), alpha_j copied to S[32 + j]
-    const int D = x64 ? s->x_defer : 0;
+    const int D = x64 ? mixed_defer(s) : 0;
     std::vector<float*> ring = {e.pa, e.pb};
     for (int j = 0; j < D; ++j) ring.push_back(s->pdef[j]);
     FoldList fl{};

@@ -482,6 +482,28 @@ def test_x_defer_matches_per_iteration_updates(case):
     assert np.array_equal(u0, u1), _rel(u1, u0)
 
 
+@pytest.mark.parametrize("case", ["-fsh_dim 3 -da_refine 5 -pc_type mg -pc_mg_levels 4 -mg_levels_ksp_type chebyshev "
+                                  "-mg_levels_pc_type jacobi -ksp_rtol 1e-10",
+                                  "-fsh_dim 3 -da_refine 7 -pc_type mg -mg_levels_ksp_type chebyshev "
+                                  "-mg_levels_pc_type jacobi -ksp_rtol 1e-10 -p4g_fuse_cg 1",
+                                  "-fsh_dim 2 -da_refine 6 -pc_type jacobi -ksp_rtol 1e-8",
+                                  "-fsh_dim 2 -da_refine 5 -pc_type none -ksp_norm_type unpreconditioned "
+                                  "-ksp_rtol 1e-8"])
+def test_fp64_x_defer_matches_per_iteration_updates(case):
+    # FP64 CG with the iterate updates deferred (up to 7 listed, applied in
+    # iteration order when the list is full and on exit) adds the same
+    # alpha_j p_j in the same order as the direction pass would: the same
+    # solve bit for bit (the Jacobi and unpreconditioned cases fold many times)
+    print("case", case, flush=True)
+    rc0, t0, r0, u0 = capi.fish_run("-fsh_problem manusin " + case + " -p4g_x_defer 0", want_u=True)
+    rc1, t1, r1, u1 = capi.fish_run("-fsh_problem manusin " + case, want_u=True)
+    assert rc0 == 0 and rc1 == 0, (t0, t1)
+    assert r0.its == r1.its and r0.reason == r1.reason
+    print("  its", r1.its, "reason", r1.reason, flush=True)
+    assert np.array_equal(u0, u1), _rel(u1, u0)
+    assert t0 == t1
+
+
 def test_c4_varcoef_257_mixed_and_fp64():
     # C4 at full size: both precisions reach the FP64 true residual, agree with
     # each other, and the discretization error falls ~4x from 129^3 (O(h^2))
