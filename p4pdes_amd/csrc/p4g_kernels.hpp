@@ -375,7 +375,7 @@ struct SrcAxpySumPut {
 // the CG directions p_j (FP32 or FP64) and step lengths alpha_j (device
 // scalars) of the iterations whose iterate update is deferred, in
 // iteration order
-constexpr int kFoldMax = 8;
+constexpr int kFoldMax = 16;
 template <typename P>
 struct FoldListT {
     const P* p[kFoldMax];

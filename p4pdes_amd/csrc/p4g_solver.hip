@@ -405,7 +405,15 @@ static int launch_fold_x(Ctx* cx, long n, double* x, FoldListT<P>& f) {
         case 5: k_fold_x<5, P><<<gb, 256, 0, st>>>(n, x, f); break;
         case 6: k_fold_x<6, P><<<gb, 256, 0, st>>>(n, x, f); break;
         case 7: k_fold_x<7, P><<<gb, 256, 0, st>>>(n, x, f); break;
-        default: k_fold_x<8, P><<<gb, 256, 0, st>>>(n, x, f); break;
+        case 8: k_fold_x<8, P><<<gb, 256, 0, st>>>(n, x, f); break;
+        case 9: k_fold_x<9, P><<<gb, 256, 0, st>>>(n, x, f); break;
+        case 10: k_fold_x<10, P><<<gb, 256, 0, st>>>(n, x, f); break;
+        case 11: k_fold_x<11, P><<<gb, 256, 0, st>>>(n, x, f); break;
+        case 12: k_fold_x<12, P><<<gb, 256, 0, st>>>(n, x, f); break;
+        case 13: k_fold_x<13, P><<<gb, 256, 0, st>>>(n, x, f); break;
+        case 14: k_fold_x<14, P><<<gb, 256, 0, st>>>(n, x, f); break;
+        case 15: k_fold_x<15, P><<<gb, 256, 0, st>>>(n, x, f); break;
+        default: k_fold_x<16, P><<<gb, 256, 0, st>>>(n, x, f); break;
         }
     }));
     f.m = 0;
@@ -2305,7 +2313,7 @@ static int configure(p4g_solver_s* s) {
     s->reliable_delta = o.getr("p4g_reliable_delta", 1e-1);
     s->fp32_diff = o.getb("p4g_fp32_diff_form", true);
     s->mixed_x64 = o.getb("p4g_mixed_x64", true);
-    s->x_defer = o.geti("p4g_x_defer", -1);   // -1: 6 (mixed), 7 (FP64 CG of KSP solves)
+    s->x_defer = o.geti("p4g_x_defer", -1);   // -1: 6 (mixed), by memory (FP64 CG of KSP solves)
     if (s->x_defer < -1 || s->x_defer > kFoldMax - 1)
         return set_error(PETSC_ERR_ARG_OUTOFRANGE, "-p4g_x_defer must be in 0..%d", kFoldMax - 1);
     {
@@ -2614,7 +2622,10 @@ int solver_setup(p4g_solver_s* s) {
         fuse_rules(s, s->e64.get(), F.n * sizeof(double));
         if (s->vi_mode) s->e64->fuse_r = s->e64->fuse_dir = false;
         // KSP solves (not the Newton solves of p4g_vi_solve / p4g_ms_solve)
-        s->e64->x_defer = s->vi_mode || s->ms_mode ? 0 : (s->x_defer < 0 ? 7 : s->x_defer);
+        // (default: as many as 16 GiB of FP64 directions hold, at most 15)
+        const long dbytes = F.n * (long)sizeof(double);
+        const int dauto = (int)std::min<long>(kFoldMax - 1, (16L << 30) / std::max(1L, dbytes));
+        s->e64->x_defer = s->vi_mode || s->ms_mode ? 0 : (s->x_defer < 0 ? dauto : s->x_defer);
         P4G_TRY(s->e64->setup(&s->cx, &s->lv, s->Ainv, s->n0));
     }
     HIP_TRY(hipStreamSynchronize(s->cx.stream));

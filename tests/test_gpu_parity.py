@@ -443,7 +443,7 @@ def test_mixed_matches_fp64(refine):
 
 @pytest.mark.parametrize("variant", ["-p4g_mixed_x64 0", "-p4g_mixed_x64 1", "-p4g_mixed_x64 0 -p4g_fuse_cg 1",
                                      "-p4g_mixed_x64 1 -p4g_fuse_cg 1", "-p4g_x_defer 0", "-p4g_x_defer 1",
-                                     "-p4g_x_defer 7 -p4g_reliable_delta 0.001",
+                                     "-p4g_x_defer 15 -p4g_reliable_delta 0.001",
                                      "-p4g_fp32_diff_form 0", "-p4g_reliable_delta 0.01",
                                      "-fsh_problem varcoef", "-p4g_fuse_cg 1", "-p4g_fuse_cg 1 -fsh_problem varcoef",
                                      "-p4g_fuse_r 1 -p4g_fuse_dir 0", "-p4g_fuse_r 0 -p4g_fuse_dir 1",
