@@ -146,8 +146,8 @@ int p4g_da_destroy(p4g_DA *da);
  *       value of -p4g_fuse_cg) set the two fusions separately,
  *   -p4g_lds auto|1|0 (default auto: 3D levels of 400+ points per side):
  *       stencil passes with their neighbour rows staged in LDS,
- *   -p4g_resid_restrict auto|1|0 (default auto: FP64 levels where -p4g_lds
- *       applies): the cycle's residual and restriction in one pass,
+ *   -p4g_resid_restrict auto|1|0 (default auto: where -p4g_lds applies):
+ *       the cycle's residual and restriction in one pass,
  *       -p4g_rr_rows 8|4 its coarse rows per block, -p4g_rr_planes n its
  *       coarse planes per block (default 0: by grid size),
  *   -p4g_post_fuse auto|1|0 (auto and 1: levels where -p4g_lds applies,

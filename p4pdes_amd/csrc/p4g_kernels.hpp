@@ -1434,29 +1434,31 @@ k_restrict(int mxf, int myf, int mzf, int mxc, int myc, int mzc, int kch, const 
 // the fine residual: restrict_body applied to the residual that the pass
 // stencil(SrcArr x, OutResid b) would store.  A block of TYC waves covers
 // the coarse rows J0 .. J0+TYC-1 (J0 = TYC by) and the coarse columns
-// I = 1 + 64 bx + lane (the lane at I = 1 also I = 0) of coarse planes
-// K0 .. K1-1 (kch of them), marching the fine planes 2K0-1 .. 2K1-1.  Wave
-// J forms the residual on the fine rows 2J-1, 2J, 2J+1 at the lane's fine
-// pair (2I-1, 2I) (the wave's last lane also at 2I+1, the lane at I = 1 at
-// fine 0) from the operand's rows 2J0-2 .. 2J0+2TYC, staged in LDS by
-// LDS-DMA loads one plane ahead with the x-halo values is-1, is+128 and
-// is+129 (is = 1 + 128 bx), and b read directly one plane ahead.  The
-// residual at a point is OutResid's (the same operand, neighbours and
-// operation order as stencil_vec with two points per lane) and the sums
-// are restrict_body's: b_c equals the two separate passes up to rounding
-// (the same formulas, whose FMA contractions the compiler chooses per
-// kernel; FP64 cycles differ by about 2e-16 relative).
-// Requires (mxf - 1) % 128 == 0.  Grid (ceil((mxc-1)/64), ceil(myc/TYC),
+// I = 1 + 32 V bx + V/2 lane .. + V/2 - 1 (the lane at I = 1 also I = 0) of
+// coarse planes K0 .. K1-1 (kch of them), marching the fine planes 2K0-1 ..
+// 2K1-1.  Wave J forms the residual on the fine rows 2J-1, 2J, 2J+1 at the
+// lane's V fine points 2I-1 .. 2I+V-2 (the wave's last lane also at the
+// next one, the lane at I = 1 at fine 0) from the operand's rows 2J0-2 ..
+// 2J0+2TYC, staged in LDS by LDS-DMA loads one plane ahead with the x-halo
+// values is-1, is+W and is+W+1 (is = 1 + W bx, W = 64 V), and b read
+// directly one plane ahead.  V = 2 (FP64) or 4 (FP32): 16-byte accesses.
+// The residual at a point is OutResid's (the same operand, neighbours and
+// operation order as stencil_vec) and the sums are restrict_body's: b_c
+// equals the two separate passes up to rounding (the same formulas, whose
+// FMA contractions the compiler chooses per kernel; FP64 cycles differ by
+// about 2e-16 relative).
+// Requires (mxf - 1) % W == 0.  Grid ((mxf-1)/W, ceil(myc/TYC),
 // ceil(mzc/kch)), block (64, TYC).
-template <typename T, class Op, int TYC>
+template <typename T, int V, class Op, int TYC>
 __device__ __forceinline__ void resid_restrict_lds(int bx, int by, int bz, int mx, int my, int mz, int mxc, int myc,
                                                    int mzc, int kch, const Op& op, const T* __restrict__ x,
                                                    const T* __restrict__ b, T* __restrict__ bc) {
-    constexpr int W = 128, NR = 2 * TYC + 3, S = 4;
+    static_assert(V == 2 || V == 4, "two or four fine points per lane");
+    constexpr int W = TX * V, VC = V / 2, NR = 2 * TYC + 3, S = 4;
     constexpr int DPE = sizeof(T) / 4, NLD = W * (int)sizeof(T) / 16, EPL = 16 / (int)sizeof(T);
     static_assert(DPE == 1 || DPE == 2, "4- or 8-byte elements");
     __shared__ __attribute__((aligned(16))) T ring[S][NR][W];
-    __shared__ __attribute__((aligned(16))) T halo[S][NR][4];   // is-1, is+128, is+129
+    __shared__ __attribute__((aligned(16))) T halo[S][NR][4];   // is-1, is+W, is+W+1
     const int lane = threadIdx.x;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.y);
     const int is = 1 + W * bx;
@@ -1476,7 +1478,7 @@ __device__ __forceinline__ void resid_restrict_lds(int bx, int by, int bz, int m
             __builtin_amdgcn_global_load_lds((const void*)(g + is + EPL * lane), (lds_void_t*)&ring[slot][ro][0], 16,
                                              0, 0);
         if (lane < 3 * DPE) {
-            const int e = lane / DPE;                          // 0: is-1, 1: is+128, 2: is+129
+            const int e = lane / DPE;                          // 0: is-1, 1: is+W, 2: is+W+1
             if (e == 0 || has_e)
                 __builtin_amdgcn_global_load_lds(
                     (const void*)((const char*)(g + (e == 0 ? is - 1 : is + W - 1 + e)) + 4 * (lane % DPE)),
@@ -1489,74 +1491,85 @@ __device__ __forceinline__ void resid_restrict_lds(int bx, int by, int bz, int m
     };
     const int J = J0 + w;
     const bool jok = J < myc;                                  // uniform per wave
-    const int I = 1 + 64 * bx + lane;
+    const int I = 1 + (W / 2) * bx + VC * lane;                // the lane's first coarse column
     const bool own0 = I == 1;
-    const int i0 = 2 * I - 1;                                  // the lane's fine pair (i0, i0 + 1)
-    const bool he = i0 + 2 < mx;                               // fine 2I+1 exists
-    // b at the lane's points of fine row jr, plane kk: the pair, fine 2I+1
-    // (last lane), fine 0 (lane at I = 1)
+    const int i0 = 2 * I - 1;                                  // the lane's fine points i0 .. i0+V-1
+    const bool he = i0 + V < mx;                               // fine i0+V exists
+    TV<T, V> zv;
+#pragma unroll
+    for (int e = 0; e < V; ++e) zv.v[e] = z0;
+    // b at the lane's points of fine row jr, plane kk; fine i0+V (last
+    // lane); fine 0 (lane at I = 1)
     struct BV {
-        T2<T> p;
+        TV<T, V> p;
         T e, z;
     };
     auto bload = [&](int kk, int jr) -> BV {
-        BV v{T2<T>{z0, z0}, z0, z0};
+        BV v{zv, z0, z0};
         if (jr < 0 || jr >= my) return v;
         const long rq = (long)kk * sm + (long)jr * mx;
-        v.p = ld2(b + rq + i0);
-        if (lane == TX - 1 && he) v.e = b[rq + i0 + 2];
+        v.p = ldv<T, V>(b + rq + i0);
+        if (lane == TX - 1 && he) v.e = b[rq + i0 + V];
         if (own0) v.z = b[rq];
         return v;
     };
     // the residual at fine row jr (staged row ro) of plane kk (slots sd, sc, su)
     struct RV {
-        T2<T> p;
+        TV<T, V> p;
         T e, z;
     };
+    auto rl = [&](int sl, int ro) -> TV<T, V> { return ldv<T, V>(&ring[sl][ro][V * lane]); };
     auto resid_row = [&](int kk, int jr, int sd, int sc, int su, const BV& bb) -> RV {
-        RV o{T2<T>{z0, z0}, z0, z0};
+        RV o{zv, z0, z0};
         if (jr < 0 || jr >= my) return o;
         const int ro = jr - rbase;
-        const T2<T> uc = *(const T2<T>*)&ring[sc][ro][2 * lane];
+        const TV<T, V> uc = rl(sc, ro);
         const bool pb = jr == 0 || jr == my - 1 || kk == 0 || kk == mz - 1;   // boundary row or plane
         const bool ms = jr - 1 > 0, mn = jr + 1 < my - 1, md = kk - 1 > 0, mu = kk + 1 < mz - 1;
-        T A0, A1, d0;
+        TV<T, V> A;
+        T d0;
         if (pb) {
-            op.bnd(i0, jr, kk, uc.a, A0, d0);
-            op.bnd(i0 + 1, jr, kk, uc.b, A1, d0);
+#pragma unroll
+            for (int e = 0; e < V; ++e) op.bnd(i0 + e, jr, kk, uc.v[e], A.v[e], d0);
         } else {
-            const T2<T> zz{z0, z0};
-            const T2<T> sv = ms ? *(const T2<T>*)&ring[sc][ro - 1][2 * lane] : zz;
-            const T2<T> nv = mn ? *(const T2<T>*)&ring[sc][ro + 1][2 * lane] : zz;
-            const T2<T> dv = md ? *(const T2<T>*)&ring[sd][ro][2 * lane] : zz;
-            const T2<T> uv = mu ? *(const T2<T>*)&ring[su][ro][2 * lane] : zz;
-            const T wv = lane > 0 ? ring[sc][ro][2 * lane - 1] : halo[sc][ro][0];
-            const T ev = lane < TX - 1 ? ring[sc][ro][2 * lane + 2] : halo[sc][ro][1];
-            op.interior(i0, jr, kk, uc.a, i0 - 1 > 0 ? wv : z0, i0 + 1 < mx - 1 ? uc.b : z0, sv.a, nv.a, dv.a, uv.a,
-                        A0, d0);
-            if (i0 + 1 == mx - 1) op.bnd(i0 + 1, jr, kk, uc.b, A1, d0);
-            else
-                op.interior(i0 + 1, jr, kk, uc.b, uc.a, i0 + 2 < mx - 1 ? ev : z0, sv.b, nv.b, dv.b, uv.b, A1, d0);
-            if (lane == TX - 1 && he) {                        // fine is+128: the next tile's first point
-                const int ie = i0 + 2;
+            const TV<T, V> sv = ms ? rl(sc, ro - 1) : zv;
+            const TV<T, V> nv = mn ? rl(sc, ro + 1) : zv;
+            const TV<T, V> dv = md ? rl(sd, ro) : zv;
+            const TV<T, V> uv = mu ? rl(su, ro) : zv;
+            const T wv = lane > 0 ? ring[sc][ro][V * lane - 1] : halo[sc][ro][0];
+            const T ev = lane < TX - 1 ? ring[sc][ro][V * lane + V] : halo[sc][ro][1];
+#pragma unroll
+            for (int e = 0; e < V; ++e) {
+                const int ie = i0 + e;
+                if (e > 0 && ie == mx - 1) {
+                    op.bnd(ie, jr, kk, uc.v[e], A.v[e], d0);
+                    continue;
+                }
+                const T we = e == 0 ? (ie - 1 > 0 ? wv : z0) : uc.v[e - 1];
+                const T ee = ie + 1 < mx - 1 ? (e == V - 1 ? ev : uc.v[e + 1]) : z0;
+                op.interior(ie, jr, kk, uc.v[e], we, ee, sv.v[e], nv.v[e], dv.v[e], uv.v[e], A.v[e], d0);
+            }
+            if (lane == TX - 1 && he) {                        // fine is+W: the next tile's first point
+                const int ie = i0 + V;
                 const T xe = halo[sc][ro][1];
                 if (ie == mx - 1) {
                     op.bnd(ie, jr, kk, xe, o.e, d0);
                 } else {
                     const T se = ms ? halo[sc][ro - 1][1] : z0, ne = mn ? halo[sc][ro + 1][1] : z0;
                     const T de = md ? halo[sd][ro][1] : z0, ue = mu ? halo[su][ro][1] : z0;
-                    op.interior(ie, jr, kk, xe, ie - 1 > 0 ? uc.b : z0, ie + 1 < mx - 1 ? halo[sc][ro][2] : z0, se, ne,
-                                de, ue, o.e, d0);
+                    op.interior(ie, jr, kk, xe, ie - 1 > 0 ? uc.v[V - 1] : z0, ie + 1 < mx - 1 ? halo[sc][ro][2] : z0,
+                                se, ne, de, ue, o.e, d0);
                 }
                 o.e = bb.e - o.e;
             }
         }
         if (pb && lane == TX - 1 && he) {
             T Ae;
-            op.bnd(i0 + 2, jr, kk, halo[sc][ro][1], Ae, d0);
+            op.bnd(i0 + V, jr, kk, halo[sc][ro][1], Ae, d0);
             o.e = bb.e - Ae;
         }
-        o.p = T2<T>{bb.p.a - A0, bb.p.b - A1};
+#pragma unroll
+        for (int e = 0; e < V; ++e) o.p.v[e] = bb.p.v[e] - A.v[e];
         if (own0) {                                            // fine 0: a boundary point
             T Az;
             op.bnd(0, jr, kk, halo[sc][ro][0], Az, d0);
@@ -1564,21 +1577,36 @@ __device__ __forceinline__ void resid_restrict_lds(int bx, int by, int bz, int m
         }
         return o;
     };
-    // restrict_body's x-weighted sum of a fine row: a = coarse I, b = coarse 0
-    auto rsum = [&](const RV& v) -> T2<T> {
-        T nx = __shfl_down(v.p.a, 1, TX);
+    // restrict_body's x-weighted sums of a fine row: a[m] = coarse I + m,
+    // b = coarse 0
+    struct CS {
+        TV<T, VC> a;
+        T b;
+    };
+    auto rsum = [&](const RV& v) -> CS {
+        T nx = __shfl_down(v.p.v[0], 1, TX);
         if (lane == TX - 1) nx = v.e;
-        T2<T> o;
-        o.a = v.p.b + h * (v.p.a + (he ? nx : T(0)));
-        o.b = own0 ? v.z + h * v.p.a : T(0);
+        CS o;
+#pragma unroll
+        for (int m = 0; m < VC; ++m) {
+            const T right = 2 * m + 2 < V ? v.p.v[2 * m + 2] : (he ? nx : T(0));
+            o.a.v[m] = v.p.v[2 * m + 1] + h * (v.p.v[2 * m] + right);
+        }
+        o.b = own0 ? v.z + h * v.p.v[0] : T(0);
         return o;
     };
-    auto comb = [&](T2<T> mid, T2<T> lo, T2<T> hi) -> T2<T> {
-        return T2<T>{mid.a + h * (lo.a + hi.a), mid.b + h * (lo.b + hi.b)};
+    auto comb = [&](const CS& mid, const CS& lo, const CS& hi) -> CS {
+        CS o;
+#pragma unroll
+        for (int m = 0; m < VC; ++m) o.a.v[m] = mid.a.v[m] + h * (lo.a.v[m] + hi.a.v[m]);
+        o.b = mid.b + h * (lo.b + hi.b);
+        return o;
     };
-    auto store = [&](int K, T2<T> v) {
+    auto store = [&](int K, const CS& v) {
         const long cb = ((long)K * myc + J) * mxc;
-        if (I < mxc) bc[cb + I] = v.a;
+#pragma unroll
+        for (int m = 0; m < VC; ++m)
+            if (I + m < mxc) bc[cb + I + m] = v.a.v[m];
         if (own0) bc[cb] = v.b;
     };
     const bool hlo = 2 * J - 1 >= 0, hhi = 2 * J + 1 < my;
@@ -1589,8 +1617,11 @@ __device__ __forceinline__ void resid_restrict_lds(int bx, int by, int bz, int m
         cb2 = bload(kf0, 2 * J + 1);
     }
     for (int kk = kf0 - 1; kk <= kf0 + 1; ++kk) stage(kk);
-    const T2<T> zz{z0, z0};
-    T2<T> slo = zz, smid = zz;
+    CS zz;
+#pragma unroll
+    for (int m = 0; m < VC; ++m) zz.a.v[m] = z0;
+    zz.b = z0;
+    CS slo = zz, smid = zz;
     for (int k = kf0; k <= kf1; ++k) {
         wait_vmcnt(0);
         lds_barrier();
@@ -1605,7 +1636,7 @@ __device__ __forceinline__ void resid_restrict_lds(int bx, int by, int bz, int m
         const RV r1 = resid_row(k, 2 * J, sd, sc, su, cb1);
         const RV r0 = resid_row(k, 2 * J - 1, sd, sc, su, cb0);
         const RV r2 = resid_row(k, 2 * J + 1, sd, sc, su, cb2);
-        const T2<T> sv = comb(rsum(r1), hlo ? rsum(r0) : zz, hhi ? rsum(r2) : zz);
+        const CS sv = comb(rsum(r1), hlo ? rsum(r0) : zz, hhi ? rsum(r2) : zz);
         if (k & 1) {                                           // plane 2K+1: the last of K, the first of K+1
             if (k > 2 * K0) store(k >> 1, comb(smid, slo, sv));
             slo = sv;
@@ -1619,12 +1650,12 @@ __device__ __forceinline__ void resid_restrict_lds(int bx, int by, int bz, int m
     if (jok && !(kf1 & 1)) store(kf1 >> 1, comb(smid, slo, zz));   // the last fine plane
 }
 
-template <typename T, class Op, int TYC>
+template <typename T, int V, class Op, int TYC>
 __global__ void __launch_bounds__(TX * TYC)
 k_resid_restrict_lds(int mx, int my, int mz, int mxc, int myc, int mzc, int kch, Op op, const T* __restrict__ x,
                      const T* __restrict__ b, T* __restrict__ bc) {
     const Tile tl = xcd_tile();
-    resid_restrict_lds<T, Op, TYC>(tl.bx, tl.by, tl.bz, mx, my, mz, mxc, myc, mzc, kch, op, x, b, bc);
+    resid_restrict_lds<T, V, Op, TYC>(tl.bx, tl.by, tl.bz, mx, my, mz, mxc, myc, mzc, kch, op, x, b, bc);
 }
 
 // out = base + P x_c, base = c b (ZG: zero-guess smoothing result) or x.

@@ -681,12 +681,12 @@ struct Engine {
     }
 
     // the fused residual + restriction (k_resid_restrict_lds) on level l:
-    // 3D constant coefficient, rows of 128 fine points, -p4g_resid_restrict
+    // 3D constant coefficient, rows of 64 V fine points, -p4g_resid_restrict
     // (auto: where the LDS-staged passes run)
     bool rr_fusable(const LevelInfo& L) const {
-        if (L.dim != 3 || L.var || L.gal || resid_restrict_mode == 0 || (L.mx - 1) % 128 != 0) return false;
-        // auto: FP64 LDS levels (in FP32 the fused pass is no faster than the two)
-        return resid_restrict_mode == 1 || (std::is_same<T, double>::value && lds_level(L));
+        if (L.dim != 3 || L.var || L.gal || resid_restrict_mode == 0 || (L.mx - 1) % (TX * VecOf<T>::V) != 0)
+            return false;
+        return resid_restrict_mode == 1 || lds_level(L);
     }
 
     // b_c = P^T (b - A x), level l -> l-1, the fine residual not stored
@@ -696,16 +696,18 @@ struct Engine {
         const LevelInfo& C = (*lv)[l - 1];
         const int tyc = rr_rows == 4 ? 4 : 8;
         const int kch = rr_planes > 0 ? rr_planes : kch_restrict(C.dim, C.my, C.mz);
-        const dim3 g((C.mx - 1 + TX - 1) / TX, (C.my + tyc - 1) / tyc, (C.mz + kch - 1) / kch), b(TX, tyc);
+        constexpr int V = VecOf<T>::V;   // fine points per lane (16-byte accesses)
+        if ((F.mx - 1) % (V * TX) != 0) return set_error(PETSC_ERR_LIB, "internal: residual + restriction tile width");
+        const dim3 g((F.mx - 1) / (V * TX), (C.my + tyc - 1) / tyc, (C.mz + kch - 1) / kch), b(TX, tyc);
         hipStream_t s = cx->stream;
         OpConst<T> o = opc(F, false);
         return cx->launch(P4G_K_RESID_ZG, l == nlev - 1, double(2 * F.n + C.n) * sizeof(T), [&] {
             if (tyc == 4)
-                k_resid_restrict_lds<T, OpConst<T>, 4><<<g, b, 0, s>>>(F.mx, F.my, F.mz, C.mx, C.my, C.mz, kch, o, xx,
-                                                                      bb, bc);
+                k_resid_restrict_lds<T, V, OpConst<T>, 4><<<g, b, 0, s>>>(F.mx, F.my, F.mz, C.mx, C.my, C.mz, kch, o,
+                                                                         xx, bb, bc);
             else
-                k_resid_restrict_lds<T, OpConst<T>, 8><<<g, b, 0, s>>>(F.mx, F.my, F.mz, C.mx, C.my, C.mz, kch, o, xx,
-                                                                      bb, bc);
+                k_resid_restrict_lds<T, V, OpConst<T>, 8><<<g, b, 0, s>>>(F.mx, F.my, F.mz, C.mx, C.my, C.mz, kch, o,
+                                                                         xx, bb, bc);
         });
     }
 
