@@ -384,24 +384,6 @@ struct FoldListT {
 };
 typedef FoldListT<float> FoldList;
 
-template <typename T, typename E>
-struct SrcSum {                       // x_new = x + e (refinement update), stored
-    const T* __restrict__ x;
-    const E* __restrict__ e;
-    T* __restrict__ xnew;
-    __device__ __forceinline__ T load(long q) const { return x[q] + T(e[q]); }
-    template <int V>
-    __device__ __forceinline__ TV<T, V> loadv(long q) const {
-        TV<T, V> xx = ldv<T, V>(x + q);
-#pragma unroll
-        for (int k = 0; k < V; ++k) xx.v[k] += T(e[q + k]);
-        return xx;
-    }
-    __device__ __forceinline__ void put(long q, T v) const { xnew[q] = v; }
-    template <int V>
-    __device__ __forceinline__ void putv(long q, const TV<T, V>& u) const { stv<T, V>(xnew + q, u); }
-};
-
 // ------------------------------------------------------ output policies
 //
 // An output consumes u, A u and the Jacobi factor at a point (point) or at
