@@ -140,8 +140,9 @@ int p4g_da_destroy(p4g_DA *da);
  *       the solve; 0 applies each in the next direction pass,
  *   -p4g_fuse_cg auto|1|0 (default auto): the CG residual update fused into
  *       the first kernel of the multigrid cycle and the direction update
- *       with the operator apply; auto fuses when one fine-level vector is
- *       larger than 768 MiB (below that the separate streaming passes are
+ *       with the operator apply; auto fuses the residual update when one
+ *       fine-level vector is larger than 256 MiB and the direction update
+ *       from 768 MiB (below that the separate streaming passes are
  *       faster); -p4g_fuse_r and -p4g_fuse_dir (auto|1|0, default: the
  *       value of -p4g_fuse_cg) set the two fusions separately,
  *   -p4g_lds auto|1|0 (default auto: 3D levels of 400+ points per side):

@@ -2481,8 +2481,10 @@ static int build_levels(p4g_solver_s* s) {
 // residual / 93.7 both.  Both are fused above 768 MiB per fine vector.
 template <typename T>
 static void fuse_rules(const p4g_solver_s* s, Engine<T>* e, double vec_bytes) {
-    const bool big = vec_bytes > 768.0 * 1048576.0;
-    e->fuse_r = s->fuse_r >= 0 ? s->fuse_r == 1 : big;
+    // by rule: the residual update once a vector exceeds the 256 MB last-level
+    // cache (513^3: FP64 and FP32), the direction update from 768 MiB
+    const bool beyond_llc = vec_bytes > 256.0 * 1048576.0, big = vec_bytes > 768.0 * 1048576.0;
+    e->fuse_r = s->fuse_r >= 0 ? s->fuse_r == 1 : beyond_llc;
     e->fuse_dir = s->fuse_dir >= 0 ? s->fuse_dir == 1 : big;
 }
 
