@@ -2708,8 +2708,14 @@ static int mixed_solve(p4g_solver_s* s, const double* b, double* x) {
     const int gb = vblocks(n);
     float* x32 = e.xbuf;        // correction since the last replacement (x64 off)
     float* r32 = e.r;
+    const bool x64 = s->mixed_x64;    // deferred updates into the FP64 iterate (else into x32)
+    const int D = x64 ? mixed_defer(s) : 0;
+    // the FP64 iterate: in place in x when it is never swapped (deferred
+    // updates applied in place), else in xa / xb and copied to x at the end
+    double* xcur = D > 0 && x != b ? x : s->xa;
+    double* xnext = s->xb;
     P4G_TRY(cx->launch(P4G_K_OTHER, true, 20.0 * n, [&] {
-        k_fill<double><<<gb, 256, 0, st>>>(n, 0.0, s->xa);
+        k_fill<double><<<gb, 256, 0, st>>>(n, 0.0, xcur);
         k_fill<float><<<gb, 256, 0, st>>>(n, 0.0f, e.pa);
         k_convert<double, float><<<gb, 256, 0, st>>>(n, b, r32);
         k_dot2<double><<<gb, 256, 0, st>>>(n, b, b, cx->part);
@@ -2730,11 +2736,8 @@ static int mixed_solve(p4g_solver_s* s, const double* b, double* x) {
     for (int q = 0; q < 64; ++q) R.history[q] = 0.0;
     R.history[0] = bnorm;
     s->hist.assign(1, bnorm);
-    double* xcur = s->xa;
-    double* xnext = s->xb;
     if (bnorm <= s->atol) R.reason = P4G_CONVERGED_ATOL;
     const double delta = s->reliable_delta;
-    const bool x64 = s->mixed_x64;    // deferred updates into the FP64 iterate (else into x32)
     double rr_last = bnorm * bnorm;   // |r|^2 at the last replacement
     bool replace_now = false;
     bool x32live = false;             // x32 holds updates not yet folded into the FP64 iterate
@@ -2746,7 +2749,6 @@ static int mixed_solve(p4g_solver_s* s, const double* b, double* x) {
     bool xpend = false;               // x += alpha p of the last iteration not yet applied
     // deferred FP64 iterate updates: p_j in a ring of FP32 buffers (never
     // overwritten while listed), alpha_j copied to S[32 + j]
-    const int D = x64 ? mixed_defer(s) : 0;
     std::vector<float*> ring = {e.pa, e.pb};
     for (int j = 0; j < D; ++j) ring.push_back(s->pdef[j]);
     FoldList fl{};
@@ -2864,7 +2866,7 @@ static int mixed_solve(p4g_solver_s* s, const double* b, double* x) {
             k_axpy_mixed<<<gb, 256, 0, st>>>(n, xcur, x32);
         }));
     }
-    HIP_TRY(hipMemcpyAsync(x, xcur, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+    if (xcur != x) HIP_TRY(hipMemcpyAsync(x, xcur, n * sizeof(double), hipMemcpyDeviceToDevice, st));
     return 0;
 }
 
