@@ -2397,16 +2397,23 @@ k_r_update(long n, const double* __restrict__ S, int ialpha, const T* __restrict
 
 // x += alpha_0 p_0 + alpha_1 p_1 + ... + alpha_{M-1} p_{M-1}, one term at a
 // time in that order (the rounding of per-iteration updates): deferred
-// iterate updates, four points per thread (16-byte accesses), M = f.m
+// iterate updates, four points per thread (16-byte accesses), M = f.m;
+// init: x taken as 0, not read
 template <int M, typename P>
-__global__ void __launch_bounds__(256) k_fold_x(long n, double* __restrict__ x, FoldListT<P> f) {
+__global__ void __launch_bounds__(256) k_fold_x(long n, double* __restrict__ x, FoldListT<P> f, int init) {
     double al[M];
 #pragma unroll
     for (int j = 0; j < M; ++j) al[j] = *f.a[j];
     const long n4 = n / 4;
     for (long t = blockIdx.x * 256L + threadIdx.x; t < n4; t += (long)gridDim.x * 256) {
         const long q = 4 * t;
-        TV<double, 2> u0 = ldv<double, 2>(x + q), u1 = ldv<double, 2>(x + q + 2);
+        TV<double, 2> u0, u1;
+        if (init) {
+            u0.v[0] = u0.v[1] = u1.v[0] = u1.v[1] = 0.0;
+        } else {
+            u0 = ldv<double, 2>(x + q);
+            u1 = ldv<double, 2>(x + q + 2);
+        }
 #pragma unroll
         for (int j = 0; j < M; ++j) {
             P pp[4];
@@ -2426,7 +2433,7 @@ __global__ void __launch_bounds__(256) k_fold_x(long n, double* __restrict__ x, 
         stv<double, 2>(x + q + 2, u1);
     }
     for (long q = 4 * n4 + blockIdx.x * 256L + threadIdx.x; q < n; q += (long)gridDim.x * 256) {
-        double u = x[q];
+        double u = init ? 0.0 : x[q];
 #pragma unroll
         for (int j = 0; j < M; ++j) u += al[j] * double(f.p[j][q]);
         x[q] = u;

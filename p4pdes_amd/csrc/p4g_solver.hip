@@ -390,30 +390,32 @@ static inline int kch_restrict(int dim, int my, int mz) {
 }
 
 // x += alpha_0 p_0 + ... + alpha_{m-1} p_{m-1} (f.m = m terms, in order:
-// deferred CG iterate updates), then the list is empty
+// deferred CG iterate updates; init: x = the sum, x not read), then the
+// list is empty
 template <typename P>
-static int launch_fold_x(Ctx* cx, long n, double* x, FoldListT<P>& f) {
+static int launch_fold_x(Ctx* cx, long n, double* x, FoldListT<P>& f, bool init = false) {
     hipStream_t st = cx->stream;
     const int gb = vblocks((n + 3) / 4);
+    const int in = init ? 1 : 0;
     if (f.m < 1 || f.m > kFoldMax) return set_error(PETSC_ERR_LIB, "internal: fold list of %d", f.m);
-    P4G_TRY(cx->launch(P4G_K_OTHER, true, (16.0 + sizeof(P) * f.m) * n, [&] {
+    P4G_TRY(cx->launch(P4G_K_OTHER, true, ((init ? 8.0 : 16.0) + sizeof(P) * f.m) * n, [&] {
         switch (f.m) {
-        case 1: k_fold_x<1, P><<<gb, 256, 0, st>>>(n, x, f); break;
-        case 2: k_fold_x<2, P><<<gb, 256, 0, st>>>(n, x, f); break;
-        case 3: k_fold_x<3, P><<<gb, 256, 0, st>>>(n, x, f); break;
-        case 4: k_fold_x<4, P><<<gb, 256, 0, st>>>(n, x, f); break;
-        case 5: k_fold_x<5, P><<<gb, 256, 0, st>>>(n, x, f); break;
-        case 6: k_fold_x<6, P><<<gb, 256, 0, st>>>(n, x, f); break;
-        case 7: k_fold_x<7, P><<<gb, 256, 0, st>>>(n, x, f); break;
-        case 8: k_fold_x<8, P><<<gb, 256, 0, st>>>(n, x, f); break;
-        case 9: k_fold_x<9, P><<<gb, 256, 0, st>>>(n, x, f); break;
-        case 10: k_fold_x<10, P><<<gb, 256, 0, st>>>(n, x, f); break;
-        case 11: k_fold_x<11, P><<<gb, 256, 0, st>>>(n, x, f); break;
-        case 12: k_fold_x<12, P><<<gb, 256, 0, st>>>(n, x, f); break;
-        case 13: k_fold_x<13, P><<<gb, 256, 0, st>>>(n, x, f); break;
-        case 14: k_fold_x<14, P><<<gb, 256, 0, st>>>(n, x, f); break;
-        case 15: k_fold_x<15, P><<<gb, 256, 0, st>>>(n, x, f); break;
-        default: k_fold_x<16, P><<<gb, 256, 0, st>>>(n, x, f); break;
+        case 1: k_fold_x<1, P><<<gb, 256, 0, st>>>(n, x, f, in); break;
+        case 2: k_fold_x<2, P><<<gb, 256, 0, st>>>(n, x, f, in); break;
+        case 3: k_fold_x<3, P><<<gb, 256, 0, st>>>(n, x, f, in); break;
+        case 4: k_fold_x<4, P><<<gb, 256, 0, st>>>(n, x, f, in); break;
+        case 5: k_fold_x<5, P><<<gb, 256, 0, st>>>(n, x, f, in); break;
+        case 6: k_fold_x<6, P><<<gb, 256, 0, st>>>(n, x, f, in); break;
+        case 7: k_fold_x<7, P><<<gb, 256, 0, st>>>(n, x, f, in); break;
+        case 8: k_fold_x<8, P><<<gb, 256, 0, st>>>(n, x, f, in); break;
+        case 9: k_fold_x<9, P><<<gb, 256, 0, st>>>(n, x, f, in); break;
+        case 10: k_fold_x<10, P><<<gb, 256, 0, st>>>(n, x, f, in); break;
+        case 11: k_fold_x<11, P><<<gb, 256, 0, st>>>(n, x, f, in); break;
+        case 12: k_fold_x<12, P><<<gb, 256, 0, st>>>(n, x, f, in); break;
+        case 13: k_fold_x<13, P><<<gb, 256, 0, st>>>(n, x, f, in); break;
+        case 14: k_fold_x<14, P><<<gb, 256, 0, st>>>(n, x, f, in); break;
+        case 15: k_fold_x<15, P><<<gb, 256, 0, st>>>(n, x, f, in); break;
+        default: k_fold_x<16, P><<<gb, 256, 0, st>>>(n, x, f, in); break;
         }
     }));
     f.m = 0;
@@ -1366,7 +1368,11 @@ struct Engine {
         const long n = F.n;
         hipStream_t st = cx->stream;
         o = CGOut();
-        P4G_TRY(fill(n, T(0), x));
+        // with deferred updates x is first written by the first fold (or
+        // zeroed on an exit before any)
+        const int D = std::is_same<T, double>::value ? x_defer : 0;
+        bool xset = D == 0;
+        if (xset) P4G_TRY(fill(n, T(0), x));
         P4G_TRY(fill(n, T(0), pa));
         HIP_TRY(hipMemcpyAsync(r, b, n * sizeof(T), hipMemcpyDeviceToDevice, st));
         const int fmode = normtype == NT_PRECOND ? FIN_BETA_PRECOND
@@ -1392,9 +1398,10 @@ struct Engine {
         o.rnorm = dp;
         o.hist.push_back(dp);
         const double ttol = std::max(rtol * dp, atol);
-        if (std::isnan(dp) || std::isinf(dp)) { o.reason = P4G_DIVERGED_NANORINF; return 0; }
-        if (dp <= ttol) { o.reason = dp < atol ? P4G_CONVERGED_ATOL : P4G_CONVERGED_RTOL; return 0; }
-        if (hs[S_RZ] == 0.0) { o.its = 1; o.reason = P4G_CONVERGED_ATOL; return 0; }
+        if (std::isnan(dp) || std::isinf(dp)) o.reason = P4G_DIVERGED_NANORINF;
+        else if (dp <= ttol) o.reason = dp < atol ? P4G_CONVERGED_ATOL : P4G_CONVERGED_RTOL;
+        else if (hs[S_RZ] == 0.0) { o.its = 1; o.reason = P4G_CONVERGED_ATOL; }
+        if (o.reason) return xset ? 0 : fill(n, T(0), x);   // x = 0
         T* pold = pa;
         T* pnew = pb;
         T* rc = r;      // current residual; the update r - alpha w goes to rx
@@ -1405,7 +1412,6 @@ struct Engine {
         // alpha_i copied to S[32 + j], and the listed updates are applied in
         // iteration order when the list is full and on exit (the same FP64
         // rounding as applying each in the next direction pass)
-        const int D = std::is_same<T, double>::value ? x_defer : 0;
         FoldListT<T> fl{};
         double* Sal = cx->S + 32;
         std::vector<T*> ring = {pa, pb};
@@ -1416,7 +1422,11 @@ struct Engine {
             return false;
         };
         auto fold = [&]() -> int {
-            if constexpr (std::is_same<T, double>::value) return launch_fold_x(cx, n, x, fl);
+            if constexpr (std::is_same<T, double>::value) {
+                P4G_TRY(launch_fold_x(cx, n, x, fl, !xset));
+                xset = true;
+                return 0;
+            }
             return set_error(PETSC_ERR_LIB, "internal: deferred updates need an FP64 iterate");
         };
         // the next direction (x += alpha_i p_i applied or listed)
@@ -1490,6 +1500,7 @@ struct Engine {
         }
         if (i >= maxit && !o.reason) o.reason = P4G_DIVERGED_ITS;
         if (fl.m) P4G_TRY(fold());   // the listed updates: x complete on every exit
+        if (!xset) P4G_TRY(fill(n, T(0), x));
         return 0;
     }
 
@@ -2679,7 +2690,9 @@ static int fine64(p4g_solver_s* s, Src src, Out out, int cls, double bytes, long
 
 // x += alpha_0 p_0 + ... + alpha_{m-1} p_{m-1} (f.m = m terms, in order) on
 // the finest level, then the list is empty
-static int fold_x(p4g_solver_s* s, double* x, FoldList& f) { return launch_fold_x(&s->cx, s->lv.back().n, x, f); }
+static int fold_x(p4g_solver_s* s, double* x, FoldList& f, bool init) {
+    return launch_fold_x(&s->cx, s->lv.back().n, x, f, init);
+}
 
 // Mixed precision: FP32 PCG (FP32 vectors, FP32 V-cycle, FP64 coarse
 // solve and dot accumulation) with FP64 residual replacement ("reliable
@@ -2714,8 +2727,9 @@ static int mixed_solve(p4g_solver_s* s, const double* b, double* x) {
     // updates applied in place), else in xa / xb and copied to x at the end
     double* xcur = D > 0 && x != b ? x : s->xa;
     double* xnext = s->xb;
+    bool xset = D == 0;   // with deferred updates the first fold writes x (x zeroed on an exit before)
     P4G_TRY(cx->launch(P4G_K_OTHER, true, 20.0 * n, [&] {
-        k_fill<double><<<gb, 256, 0, st>>>(n, 0.0, xcur);
+        if (xset) k_fill<double><<<gb, 256, 0, st>>>(n, 0.0, xcur);
         k_fill<float><<<gb, 256, 0, st>>>(n, 0.0f, e.pa);
         k_convert<double, float><<<gb, 256, 0, st>>>(n, b, r32);
         k_dot2<double><<<gb, 256, 0, st>>>(n, b, b, cx->part);
@@ -2796,7 +2810,8 @@ static int mixed_solve(p4g_solver_s* s, const double* b, double* x) {
                 fl.p[fl.m] = pold;
                 fl.a[fl.m] = S + S_ALPHA;
                 ++fl.m;
-                P4G_TRY(fold_x(s, xcur, fl));
+                P4G_TRY(fold_x(s, xcur, fl, !xset));
+                xset = true;
                 P4G_TRY(fine64(s, SrcArr<double>{xcur}, OutResidCvt<double, float>{b, r32}, P4G_K_OTHER, 28.0 * n,
                                &nbk));
             } else {
@@ -2860,7 +2875,13 @@ static int mixed_solve(p4g_solver_s* s, const double* b, double* x) {
     }
     if (!R.reason) R.reason = P4G_DIVERGED_ITS;
     R.inner_its = nrep;   // number of FP64 residual replacements
-    if (fl.m) P4G_TRY(fold_x(s, xcur, fl));   // deferred updates not yet applied
+    if (fl.m) {           // deferred updates not yet applied
+        P4G_TRY(fold_x(s, xcur, fl, !xset));
+        xset = true;
+    }
+    if (!xset) {          // no update applied: x = 0
+        P4G_TRY(cx->launch(P4G_K_OTHER, true, 8.0 * n, [&] { k_fill<double><<<gb, 256, 0, st>>>(n, 0.0, xcur); }));
+    }
     if (x32live) {        // fold the last correction: x = x + x32
         P4G_TRY(cx->launch(P4G_K_OTHER, true, 20.0 * n, [&] {
             k_axpy_mixed<<<gb, 256, 0, st>>>(n, xcur, x32);
