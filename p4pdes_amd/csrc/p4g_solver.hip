@@ -1374,18 +1374,19 @@ struct Engine {
         bool xset = D == 0;
         if (xset) P4G_TRY(fill(n, T(0), x));
         P4G_TRY(fill(n, T(0), pa));
-        HIP_TRY(hipMemcpyAsync(r, b, n * sizeof(T), hipMemcpyDeviceToDevice, st));
+        // r_0 = b - A 0 = b: the first iteration reads b itself, its update
+        // goes to r2, and the residuals then alternate between r and r2
         const int fmode = normtype == NT_PRECOND ? FIN_BETA_PRECOND
                         : normtype == NT_NATURAL ? FIN_BETA_NATURAL : FIN_BETA_UNPRECOND;
         const int soff = (int)(Sblk - cx->S);
         if (normtype == NT_UNPRECOND) {   // (r, r) of the initial residual
             P4G_TRY(cx->launch(P4G_K_OTHER, true, 2.0 * n * sizeof(T), [&] {
-                k_scale<T, 2><<<vblocks(n), 256, 0, st>>>(n, T(1), r, w, cx->part);
+                k_scale<T, 2><<<vblocks(n), 256, 0, st>>>(n, T(1), b, w, cx->part);
             }));
             P4G_TRY(cx->finalize(vblocks(n), Sblk, FIN_RAW, 0));
         }
         long nb = 0;
-        P4G_TRY(pc_apply_dots(r, z, &nb));
+        P4G_TRY(pc_apply_dots(b, z, &nb));
         if (normtype == NT_UNPRECOND) {
             // S_RR <- TMP1 of the raw pass above (kept by a copy on device)
             HIP_TRY(hipMemcpyAsync(Sblk + S_RR, Sblk + S_TMP1, sizeof(double), hipMemcpyDeviceToDevice, st));
@@ -1404,7 +1405,7 @@ struct Engine {
         if (o.reason) return xset ? 0 : fill(n, T(0), x);   // x = 0
         T* pold = pa;
         T* pnew = pb;
-        T* rc = r;      // current residual; the update r - alpha w goes to rx
+        const T* rc = b;   // current residual; the update r - alpha w goes to rx
         T* rx = r2;
         double dpi_old = 0.0;
         // with x_defer (FP64) the direction pass leaves x alone: p_i stays in
@@ -1474,7 +1475,9 @@ struct Engine {
             pend.rr = normtype == NT_UNPRECOND;
             pend.on = true;
             P4G_TRY(pc_apply_dots(rx, z, &nb));
-            std::swap(rc, rx);
+            T* const next = rc == b ? r : const_cast<T*>(rc);   // b is never written
+            rc = rx;
+            rx = next;
             if (normtype == NT_UNPRECOND) P4G_TRY(cx->finalize(pend.nparts, Sblk, FIN_RR, 0, cx->part_rr));
             P4G_TRY(cx->finalize(nb, Sblk, fmode, 0));
             P4G_TRY(cx->readS_begin(cx->hS));
