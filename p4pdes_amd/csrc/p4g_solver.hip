@@ -1373,7 +1373,7 @@ struct Engine {
         const int D = std::is_same<T, double>::value ? x_defer : 0;
         bool xset = D == 0;
         if (xset) P4G_TRY(fill(n, T(0), x));
-        P4G_TRY(fill(n, T(0), pa));
+        if (D == 0) P4G_TRY(fill(n, T(0), pa));   // p_{-1} = 0 (with D, z stands in: beta_0 = 0)
         // r_0 = b - A 0 = b: the first iteration reads b itself, its update
         // goes to r2, and the residuals then alternate between r and r2
         const int fmode = normtype == NT_PRECOND ? FIN_BETA_PRECOND
@@ -1403,7 +1403,10 @@ struct Engine {
         else if (dp <= ttol) o.reason = dp < atol ? P4G_CONVERGED_ATOL : P4G_CONVERGED_RTOL;
         else if (hs[S_RZ] == 0.0) { o.its = 1; o.reason = P4G_CONVERGED_ATOL; }
         if (o.reason) return xset ? 0 : fill(n, T(0), x);   // x = 0
-        T* pold = pa;
+        // the first direction p_0 = z + beta_0 p_{-1}, beta_0 = 0: p_{-1} is a
+        // zeroed buffer, or z itself where the ring picks the direction
+        // buffers (z is then never a direction pass's output)
+        T* pold = D > 0 ? z : pa;
         T* pnew = pb;
         const T* rc = b;   // current residual; the update r - alpha w goes to rx
         T* rx = r2;
@@ -2733,7 +2736,7 @@ static int mixed_solve(p4g_solver_s* s, const double* b, double* x) {
     bool xset = D == 0;   // with deferred updates the first fold writes x (x zeroed on an exit before)
     P4G_TRY(cx->launch(P4G_K_OTHER, true, 20.0 * n, [&] {
         if (xset) k_fill<double><<<gb, 256, 0, st>>>(n, 0.0, xcur);
-        k_fill<float><<<gb, 256, 0, st>>>(n, 0.0f, e.pa);
+        if (D == 0) k_fill<float><<<gb, 256, 0, st>>>(n, 0.0f, e.pa);   // p_{-1} = 0 (with D: z, beta_0 = 0)
         k_convert<double, float><<<gb, 256, 0, st>>>(n, b, r32);
         k_dot2<double><<<gb, 256, 0, st>>>(n, b, b, cx->part);
     }));
@@ -2758,7 +2761,7 @@ static int mixed_solve(p4g_solver_s* s, const double* b, double* x) {
     double rr_last = bnorm * bnorm;   // |r|^2 at the last replacement
     bool replace_now = false;
     bool x32live = false;             // x32 holds updates not yet folded into the FP64 iterate
-    float* pold = e.pa;
+    float* pold = D > 0 ? e.z : e.pa;   // p_{-1} (beta_0 = 0)
     float* pnew = e.pb;
     double dpi_old = 0.0;
     int i = 0, nrep = 0;
