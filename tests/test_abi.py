@@ -103,6 +103,23 @@ def test_fusion_options_validated(opts):
     L.p4g_da_destroy(ctypes.byref(da))
 
 
+@pytest.mark.parametrize("opts,fragment", [
+    ("-p4g_x_defer 16", "-p4g_x_defer must be in 0..15"), ("-p4g_x_defer -2", "-p4g_x_defer must be in 0..15"),
+    ("-p4g_rr_rows 6", "-p4g_rr_rows must be 4 or 8"), ("-p4g_rr_planes 65", "-p4g_rr_planes must be in 0..64"),
+    ("-p4g_post_fuse 3", "must be auto, 1 or 0"), ("-p4g_lds yes2", "must be auto, 1 or 0")])
+def test_kernel_form_options_validated(opts, fragment):
+    # the device options that choose kernel forms: out-of-range values are
+    # rejected at creation with PETSC_ERR_ARG_OUTOFRANGE and a message
+    L = capi.lib()
+    da = ctypes.c_void_p()
+    capi.check(L.p4g_da_create(3, 17, 17, 17, 1.0, 1.0, 1.0, ctypes.byref(da)))
+    s = ctypes.c_void_p()
+    rc = L.p4g_solver_create(da, ("-pc_type mg -mg_levels_pc_type jacobi " + opts).encode(), ctypes.byref(s))
+    assert rc == 63, capi.last_error()   # PETSC_ERR_ARG_OUTOFRANGE
+    assert fragment in capi.last_error()
+    L.p4g_da_destroy(ctypes.byref(da))
+
+
 @pytest.mark.parametrize("args,code", [
     ("-fsh_cx -1 -pc_type mg", 2), ("-fsh_problem manuexp -fsh_cx 2 -pc_type mg", 3),
     ("-fsh_dim 4 -pc_type mg", 1)])
