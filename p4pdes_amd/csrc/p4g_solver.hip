@@ -2643,9 +2643,11 @@ int solver_setup(p4g_solver_s* s) {
         fuse_rules(s, s->e64.get(), F.n * sizeof(double));
         if (s->vi_mode) s->e64->fuse_r = s->e64->fuse_dir = false;
         // KSP solves (not the Newton solves of p4g_vi_solve / p4g_ms_solve)
-        // (default: as many as 16 GiB of FP64 directions hold, at most 15)
+        // (default: as many as 16 GiB of FP64 directions hold, at most 15;
+        // none below 3, where the folds move as much as the per-iteration updates)
         const long dbytes = F.n * (long)sizeof(double);
-        const int dauto = (int)std::min<long>(kFoldMax - 1, (16L << 30) / std::max(1L, dbytes));
+        int dauto = (int)std::min<long>(kFoldMax - 1, (16L << 30) / std::max(1L, dbytes));
+        if (dauto < 3) dauto = 0;
         s->e64->x_defer = s->vi_mode || s->ms_mode ? 0 : (s->x_defer < 0 ? dauto : s->x_defer);
         P4G_TRY(s->e64->setup(&s->cx, &s->lv, s->Ainv, s->n0));
     }
