@@ -131,7 +131,7 @@ int p4g_da_destroy(p4g_DA *da);
  *       replacement,
  *   -p4g_x_defer d (0..15; default 6 in mixed precision; in the FP64 CG
  *       of p4g_ksp_solve / p4g_snes_solve as many as 16 GiB of directions
- *       hold, at most 15; 0 in the Newton solves): the
+ *       hold, at most 15, 0 if fewer than 3; 0 in the Newton solves): the
  *       FP64 iterate updates x += alpha p of up to d iterations are
  *       deferred, their directions kept (d more vectors), and applied in
  *       iteration order (the same FP64 rounding as applying each at once):
