@@ -460,6 +460,42 @@ struct OutResid {                     // r = b - A u ; acc0 += r.r (NORM)
     }
 };
 
+template <typename T>
+struct OutResidNorms {                // acc0 += (b - A u).(b - A u), acc1 += b.b (nothing stored)
+    const T* __restrict__ b;
+    static constexpr int NDOT = 2, NPRE = 1;
+    template <int V>
+    __device__ __forceinline__ void pre(long q, TV<T, V>* p) const { p[0] = ldv<T, V>(b + q); }
+    __device__ __forceinline__ void pre1(long q, T* p) const { p[0] = b[q]; }
+    __device__ __forceinline__ void pointp(long, T, T Au, T, const T* p, double& a0, double& a1) const {
+        const T v = p[0] - Au;
+        a0 += double(v) * double(v);
+        a1 += double(p[0]) * double(p[0]);
+    }
+    template <int V>
+    __device__ __forceinline__ void pointvp(long, const TV<T, V>&, const TV<T, V>& Au, const TV<T, V>&,
+                                            const TV<T, V>* p, double& a0, double& a1) const {
+#pragma unroll
+        for (int e = 0; e < V; ++e) {
+            const T v = p[0].v[e] - Au.v[e];
+            a0 += double(v) * double(v);
+            a1 += double(p[0].v[e]) * double(p[0].v[e]);
+        }
+    }
+    __device__ __forceinline__ void point(long q, T u, T Au, T idg, double& a0, double& a1) const {
+        T p[1];
+        pre1(q, p);
+        pointp(q, u, Au, idg, p, a0, a1);
+    }
+    template <int V>
+    __device__ __forceinline__ void pointv(long q, const TV<T, V>& u, const TV<T, V>& Au, const TV<T, V>& idg,
+                                           double& a0, double& a1) const {
+        TV<T, V> p[1];
+        pre<V>(q, p);
+        pointvp<V>(q, u, Au, idg, p, a0, a1);
+    }
+};
+
 template <typename T, typename R>
 struct OutResidCvt {                  // r = b - A u in T, stored as R ; acc0 += r.r
     const T* __restrict__ b;
@@ -2333,6 +2369,23 @@ k_scale(long n, T c, const T* __restrict__ x, T* __restrict__ y, double* __restr
             part[2 * blockIdx.x + 1] = sm[1][0] + sm[1][1] + sm[1][2] + sm[1][3];
         }
     }
+}
+
+// x converted to y (a narrower type); acc0 = acc1 += x.x
+template <typename S, typename D>
+__global__ void __launch_bounds__(256) k_convert_norm(long n, const S* __restrict__ x, D* __restrict__ y,
+                                                     double* __restrict__ part) {
+    double a0 = 0.0;
+    for (long q = blockIdx.x * 256L + threadIdx.x; q < n; q += (long)gridDim.x * 256) {
+        const double xv = double(x[q]);
+        y[q] = D(x[q]);
+        a0 += xv * xv;
+    }
+    __shared__ double sm[4];
+    a0 = wave_sum(a0);
+    if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = a0;
+    __syncthreads();
+    if (threadIdx.x == 0) part[2 * blockIdx.x] = part[2 * blockIdx.x + 1] = sm[0] + sm[1] + sm[2] + sm[3];
 }
 
 // acc0 += x.y, acc1 += x.x

@@ -1379,9 +1379,9 @@ struct Engine {
         const int fmode = normtype == NT_PRECOND ? FIN_BETA_PRECOND
                         : normtype == NT_NATURAL ? FIN_BETA_NATURAL : FIN_BETA_UNPRECOND;
         const int soff = (int)(Sblk - cx->S);
-        if (normtype == NT_UNPRECOND) {   // (r, r) of the initial residual
-            P4G_TRY(cx->launch(P4G_K_OTHER, true, 2.0 * n * sizeof(T), [&] {
-                k_scale<T, 2><<<vblocks(n), 256, 0, st>>>(n, T(1), b, w, cx->part);
+        if (normtype == NT_UNPRECOND) {   // (r, r) of the initial residual r = b, twice (TMP0, TMP1)
+            P4G_TRY(cx->launch(P4G_K_OTHER, true, 1.0 * n * sizeof(T), [&] {
+                k_dot2<T><<<vblocks(n), 256, 0, st>>>(n, b, b, cx->part);
             }));
             P4G_TRY(cx->finalize(vblocks(n), Sblk, FIN_RAW, 0));
         }
@@ -2739,8 +2739,7 @@ static int mixed_solve(p4g_solver_s* s, const double* b, double* x) {
     P4G_TRY(cx->launch(P4G_K_OTHER, true, 20.0 * n, [&] {
         if (xset) k_fill<double><<<gb, 256, 0, st>>>(n, 0.0, xcur);
         if (D == 0) k_fill<float><<<gb, 256, 0, st>>>(n, 0.0f, e.pa);   // p_{-1} = 0 (with D: z, beta_0 = 0)
-        k_convert<double, float><<<gb, 256, 0, st>>>(n, b, r32);
-        k_dot2<double><<<gb, 256, 0, st>>>(n, b, b, cx->part);
+        k_convert_norm<double, float><<<gb, 256, 0, st>>>(n, b, r32, cx->part);   // r32 = b, (b, b)
     }));
     P4G_TRY(cx->finalize(gb, So, FIN_RAW, 0));
     long nb = 0;
@@ -2899,20 +2898,16 @@ static int mixed_solve(p4g_solver_s* s, const double* b, double* x) {
     return 0;
 }
 
-// |b - A x|_2 / |b|_2 in FP64 on the finest level, scratch of n doubles
-static int true_residual(p4g_solver_s* s, const double* b, const double* x, double* scratch, double* rel) {
+// |b - A x|_2 / |b|_2 in FP64 on the finest level (one pass, nothing stored)
+static int true_residual(p4g_solver_s* s, const double* b, const double* x, double* rel) {
     const long n = s->lv.back().n;
-    hipStream_t st = s->cx.stream;
     double* S = s->cx.S + 16;
     long nbk = 0;
-    P4G_TRY(fine64(s, SrcArr<double>{x}, OutResid<double, 1>{b, scratch}, P4G_K_OTHER, 24.0 * n, &nbk));
+    // (b - A x, b - A x) and (b, b) in one pass, nothing stored
+    P4G_TRY(fine64(s, SrcArr<double>{x}, OutResidNorms<double>{b}, P4G_K_OTHER, 16.0 * n, &nbk));
     P4G_TRY(s->cx.finalize(nbk, S, FIN_RAW, 0));
-    const int gb = vblocks(n);
-    k_dot2<double><<<gb, 256, 0, st>>>(n, b, b, s->cx.part);
-    LAUNCH_CHECK();
-    P4G_TRY(s->cx.finalize(gb, S + 2, FIN_RAW, 0));
     P4G_TRY(s->cx.readS(s->cx.hS));
-    const double rr = s->cx.hS[16 + S_TMP0], bb = s->cx.hS[16 + 2 + S_TMP0];
+    const double rr = s->cx.hS[16 + S_TMP0], bb = s->cx.hS[16 + S_TMP1];
     *rel = bb > 0 ? std::sqrt(rr / bb) : 0.0;
     return 0;
 }
@@ -3026,7 +3021,7 @@ int ksp_solve_device(p4g_solver_s* s, const double* b, double* x) {
     s->cx.tm.collect();
     P4G_TRY(sor_pipes_check(s, "KSPSolve"));
     double rel = 0.0;
-    P4G_TRY(true_residual(s, b, x, s->mixed ? s->xb : s->e64->w, &rel));
+    P4G_TRY(true_residual(s, b, x, &rel));
     R.true_relres = rel;
     return ksp_views(s, b, x);
 }
