@@ -1433,6 +1433,17 @@ struct Engine {
             }
             return set_error(PETSC_ERR_LIB, "internal: deferred updates need an FP64 iterate");
         };
+        // the current direction and its alpha listed (the list folded first when full)
+        auto list_current = [&]() -> int {
+            if (fl.m == D) P4G_TRY(fold());
+            P4G_TRY(cx->launch(P4G_K_OTHER, false, 0.0, [&] {
+                k_copy_scalar<<<1, 64, 0, st>>>(Sblk + S_ALPHA, Sal + fl.m);
+            }));
+            fl.p[fl.m] = pold;
+            fl.a[fl.m] = Sal + fl.m;
+            ++fl.m;
+            return 0;
+        };
         // the next direction (x += alpha_i p_i applied or listed)
         auto next_direction = [&](bool first) -> int {
             long nbd = 0;
@@ -1440,15 +1451,7 @@ struct Engine {
                 P4G_TRY(direction<T>(z, pold, pnew, x, Sblk + S_BCOEF, Sblk + S_ALPHA, &nbd));
                 return cx->finalize(nbd, Sblk, FIN_ALPHA, 0);
             }
-            if (!first) {
-                if (fl.m == D) P4G_TRY(fold());
-                P4G_TRY(cx->launch(P4G_K_OTHER, false, 0.0, [&] {
-                    k_copy_scalar<<<1, 64, 0, st>>>(Sblk + S_ALPHA, Sal + fl.m);
-                }));
-                fl.p[fl.m] = pold;
-                fl.a[fl.m] = Sal + fl.m;
-                ++fl.m;
-            }
+            if (!first) P4G_TRY(list_current());
             pnew = nullptr;
             for (T* q : ring)
                 if (q != pold && !listed(q)) { pnew = q; break; }
@@ -1483,9 +1486,30 @@ struct Engine {
             rx = next;
             if (normtype == NT_UNPRECOND) P4G_TRY(cx->finalize(pend.nparts, Sblk, FIN_RR, 0, cx->part_rr));
             P4G_TRY(cx->finalize(nb, Sblk, fmode, 0));
+            // the next direction is queued before the host reads this
+            // iteration's scalars, unless (deferred updates) the norm is
+            // predicted, from the last two, to meet the tolerance now: then
+            // the host tests first, so no pass is spent on a direction that
+            // a converged solve does not use
+            const size_t nh = o.hist.size();
+            const double pred = nh >= 2 ? o.hist[nh - 1] * (o.hist[nh - 1] / o.hist[nh - 2]) : o.hist[nh - 1];
+            const bool spec = D == 0 || pred > ttol;
             P4G_TRY(cx->readS_begin(cx->hS));
-            P4G_TRY(next_direction(false));   // next direction; x += alpha_i p_i applied or listed
+            if (spec) P4G_TRY(next_direction(false));   // next direction; x += alpha_i p_i applied or listed
             P4G_TRY(cx->readS_end());
+            if (!spec) {
+                const double dn = hs[S_DP];
+                if (std::isnan(dn) || std::isinf(dn) || dn <= ttol) {
+                    P4G_TRY(list_current());   // this iteration's update, no next direction
+                    o.rnorm = dn;
+                    o.hist.push_back(dn);
+                    o.reason = std::isnan(dn) || std::isinf(dn) ? P4G_DIVERGED_NANORINF
+                             : dn < atol ? P4G_CONVERGED_ATOL : P4G_CONVERGED_RTOL;
+                    break;
+                }
+                P4G_TRY(next_direction(false));
+                P4G_TRY(cx->readS(cx->hS));
+            }
             const double dpi = hs[S_PW];
             if (dpi == 0.0 || (i > 0 && ((dpi > 0) != (dpi_old > 0)))) {
                 o.reason = P4G_DIVERGED_INDEFINITE_MAT;
@@ -2761,6 +2785,7 @@ static int mixed_solve(p4g_solver_s* s, const double* b, double* x) {
     const double delta = s->reliable_delta;
     double rr_last = bnorm * bnorm;   // |r|^2 at the last replacement
     bool replace_now = false;
+    bool for_target = false;          // the requested replacement may end the solve (|r| <= target)
     bool x32live = false;             // x32 holds updates not yet folded into the FP64 iterate
     float* pold = D > 0 ? e.z : e.pa;   // p_{-1} (beta_0 = 0)
     float* pnew = e.pb;
@@ -2859,9 +2884,28 @@ static int mixed_solve(p4g_solver_s* s, const double* b, double* x) {
             P4G_TRY(cx->finalize(e.pend.nparts, S, FIN_RR, 0, cx->part_rr));
         }
         P4G_TRY(cx->finalize(nb, S, FIN_BETA_PRECOND, 0));
+        // the next direction is queued before the host reads the scalars,
+        // unless this replacement was asked for by an iterated residual
+        // below the target: then the host tests first, and a converged
+        // solve spends no pass on a direction it does not use (a
+        // replacement iteration leaves no pending x update to apply)
+        const bool spec = !(replaced && for_target);
         P4G_TRY(cx->readS_begin(cx->hS));
-        P4G_TRY(direction());   // applies a pending x += alpha p: x is complete on every exit
+        if (spec) P4G_TRY(direction());   // applies a pending x += alpha p: x is complete on every exit
         P4G_TRY(cx->readS_end());
+        if (!spec) {
+            const double rt = std::sqrt(ho[S_TMP0]);
+            if (rt <= target) {
+                R.rnorm = rt;
+                if (i + 1 < 64) R.history[i + 1] = rt;
+                s->hist.push_back(rt);
+                R.reason = rt < s->atol ? P4G_CONVERGED_ATOL : P4G_CONVERGED_RTOL;
+                break;
+            }
+            P4G_TRY(direction());
+            P4G_TRY(cx->readS(cx->hS));
+        }
+        for_target = false;
         const double dpi = hs[S_PW];
         if (dpi == 0.0 || (i > 0 && ((dpi > 0) != (dpi_old > 0)))) { R.reason = P4G_DIVERGED_INDEFINITE_MAT; break; }
         dpi_old = dpi;
@@ -2878,7 +2922,10 @@ static int mixed_solve(p4g_solver_s* s, const double* b, double* x) {
         if (std::isnan(rn) || std::isinf(rn)) { R.reason = P4G_DIVERGED_NANORINF; break; }
         if (replaced && rn <= target) { R.reason = rn < s->atol ? P4G_CONVERGED_ATOL : P4G_CONVERGED_RTOL; break; }
         if (rn >= s->dtol * bnorm) { R.reason = P4G_DIVERGED_DTOL; break; }
-        if (!replaced && (rn <= target || rn * rn <= delta * delta * rr_last)) replace_now = true;
+        if (!replaced && (rn <= target || rn * rn <= delta * delta * rr_last)) {
+            replace_now = true;
+            for_target = rn <= target;
+        }
     }
     if (!R.reason) R.reason = P4G_DIVERGED_ITS;
     R.inner_its = nrep;   // number of FP64 residual replacements
