@@ -50,8 +50,9 @@ typedef struct p4g_da_s *p4g_DA;   /* the DM of p4g_petsc.h */
 #define P4G_DIVERGED_ITS (-3)
 #define P4G_DIVERGED_DTOL (-4)
 #define P4G_DIVERGED_BREAKDOWN (-5)
-#define P4G_DIVERGED_INDEFINITE_MAT (-8)
+#define P4G_DIVERGED_INDEFINITE_PC (-8)
 #define P4G_DIVERGED_NANORINF (-9)
+#define P4G_DIVERGED_INDEFINITE_MAT (-10)
 
 typedef struct {
     int    its;            /* outer iterations (PETSc ksp->its)                   */

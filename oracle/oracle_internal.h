@@ -106,8 +106,9 @@ enum { NORM_NONE = 0, NORM_PRECONDITIONED, NORM_UNPRECONDITIONED, NORM_NATURAL }
 #define R_DIVERGED_ITS (-3)
 #define R_DIVERGED_DTOL (-4)
 #define R_DIVERGED_BREAKDOWN (-5)
-#define R_DIVERGED_INDEFINITE_MAT (-8)
+#define R_DIVERGED_INDEFINITE_PC (-8)
 #define R_DIVERGED_NANORINF (-9)
+#define R_DIVERGED_INDEFINITE_MAT (-10)
 const char *oracle_reason_name(int reason);
 
 struct opc;

@@ -30,6 +30,7 @@ const char *oracle_reason_name(int r) {
     case R_DIVERGED_ITS: return "DIVERGED_ITS";
     case R_DIVERGED_DTOL: return "DIVERGED_DTOL";
     case R_DIVERGED_BREAKDOWN: return "DIVERGED_BREAKDOWN";
+    case R_DIVERGED_INDEFINITE_PC: return "DIVERGED_INDEFINITE_PC";
     case R_DIVERGED_INDEFINITE_MAT: return "DIVERGED_INDEFINITE_MAT";
     case R_DIVERGED_NANORINF: return "DIVERGED_NANORINF";
     }
@@ -414,6 +415,7 @@ static int cg_solve(oksp *k, const double *b, double *x, int *reason, double *rn
     do {
         its = i + 1;
         if (beta == 0.0) { *reason = R_CONVERGED_ATOL; break; }
+        if (i > 0 && beta * betaold < 0.0) { *reason = R_DIVERGED_INDEFINITE_PC; break; }
         if (i == 0) ovcopy(n, z, p);
         else {
             double bb = beta / betaold;

@@ -21,7 +21,7 @@ KERNEL_NAMES = ["stencil_dot", "cg_update", "resid_zg", "restrict", "prolong", "
                 "coarse", "reduce", "apply", "cg_resid", "smooth_zg", "smooth_last", "other"]
 
 REASONS = {2: "CONVERGED_RTOL", 3: "CONVERGED_ATOL", 4: "CONVERGED_ITS", -3: "DIVERGED_ITS",
-           -4: "DIVERGED_DTOL", -5: "DIVERGED_BREAKDOWN", -8: "DIVERGED_INDEFINITE_MAT",
+           -4: "DIVERGED_DTOL", -5: "DIVERGED_BREAKDOWN", -8: "DIVERGED_INDEFINITE_PC", -10: "DIVERGED_INDEFINITE_MAT",
            -9: "DIVERGED_NANORINF"}
 
 

@@ -54,7 +54,7 @@ struct SrvReq {
     double t_submit = 0, t_admit = 0, t_done = 0;
     int its = 0, nrep = 0, reason = 0;
     bool replace_now = false, replaced = false;
-    double bnorm = 0, target = 0, rr_last = 0, dpi_old = 0, rn = 0;
+    double bnorm = 0, target = 0, rr_last = 0, dpi_old = 0, rn = 0, rz_old = 0;
     float *pold = nullptr, *pnew = nullptr;
     std::string err;
     double* S = nullptr;   // device scalar block (32 doubles) of the slot
@@ -683,6 +683,7 @@ static bool req_step(p4g_server_s* v, SrvReq* r) {
         r->target = std::max(s->rtol * r->bnorm, s->atol);
         r->rr_last = r->bnorm * r->bnorm;
         r->rn = r->bnorm;
+        r->rz_old = hs[S_RZ];
         r->state = RQ_ITER;
         if (r->bnorm <= s->atol) { r->reason = P4G_CONVERGED_ATOL; return true; }
         return false;
@@ -699,6 +700,8 @@ static bool req_step(p4g_server_s* v, SrvReq* r) {
     if (r->replaced && rn <= r->target) { r->reason = rn < s->atol ? P4G_CONVERGED_ATOL : P4G_CONVERGED_RTOL; return true; }
     if (rn >= s->dtol * r->bnorm) { r->reason = P4G_DIVERGED_DTOL; return true; }
     if (r->its >= s->maxit) { r->reason = P4G_DIVERGED_ITS; return true; }
+    if (hs[S_RZ] * r->rz_old < 0.0) { r->its++; r->reason = P4G_DIVERGED_INDEFINITE_PC; return true; }   // beta changed sign
+    r->rz_old = hs[S_RZ];
     if (hs[S_RZ] == 0.0 || (!r->replaced && (rn <= r->target || rn * rn <= s->reliable_delta * s->reliable_delta * r->rr_last)))
         r->replace_now = true;
     return false;

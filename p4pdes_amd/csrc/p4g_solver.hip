@@ -4,8 +4,10 @@
 // Semantics follow what the reference path gets from PETSc with
 // `-ksp_type cg -pc_type mg` (c/ch6/fish.c:231-239):
 //   * KSPCG: preconditioned PCG; default test |B r_k| <= max(rtol |B r_0|, atol);
-//     beta == 0 -> CONVERGED_ATOL; a sign change of (p, A p) ->
-//     DIVERGED_INDEFINITE_MAT; |r| >= dtol |r_0| -> DIVERGED_DTOL.
+//     beta == 0 -> CONVERGED_ATOL; a sign change of beta = (z, r) ->
+//     DIVERGED_INDEFINITE_PC; (p, A p) == 0 or a sign change of it ->
+//     DIVERGED_INDEFINITE_MAT (before x += alpha p); |r| >= dtol |r_0| ->
+//     DIVERGED_DTOL.
 //   * PCMG multiplicative cycle: pre-smooth, r = b - A x, b_c = P^T r,
 //     x_c = 0, recurse (twice for W), x += P x_c, post-smooth; one smoother
 //     per level with a nonzero initial guess; coarsest level solved directly.
@@ -1408,6 +1410,7 @@ struct Engine {
         // buffers (z is then never a direction pass's output)
         T* pold = D > 0 ? z : pa;
         T* pnew = pb;
+        double rz_old = hs[S_RZ];   // beta of the previous iteration (its sign is tested)
         const T* rc = b;   // current residual; the update r - alpha w goes to rx
         T* rx = r2;
         double dpi_old = 0.0;
@@ -1497,7 +1500,9 @@ struct Engine {
             P4G_TRY(cx->readS_begin(cx->hS));
             if (spec) P4G_TRY(next_direction(false));   // next direction; x += alpha_i p_i applied or listed
             P4G_TRY(cx->readS_end());
-            if (!spec) {
+            const double dpi = hs[S_PW];
+            const bool indefinite = dpi == 0.0 || (i > 0 && ((dpi > 0) != (dpi_old > 0)));
+            if (!spec && !indefinite) {
                 const double dn = hs[S_DP];
                 if (std::isnan(dn) || std::isinf(dn) || dn <= ttol) {
                     P4G_TRY(list_current());   // this iteration's update, no next direction
@@ -1510,8 +1515,11 @@ struct Engine {
                 P4G_TRY(next_direction(false));
                 P4G_TRY(cx->readS(cx->hS));
             }
-            const double dpi = hs[S_PW];
-            if (dpi == 0.0 || (i > 0 && ((dpi > 0) != (dpi_old > 0)))) {
+            if (indefinite) {
+                // PETSc stops before x += alpha_i p_i: a listed update is
+                // dropped (it is the last one listed); with x_defer 0 the
+                // direction pass has already applied it
+                if (spec && D > 0) --fl.m;
                 o.reason = P4G_DIVERGED_INDEFINITE_MAT;
                 break;
             }
@@ -1522,11 +1530,19 @@ struct Engine {
             if (std::isnan(dp) || std::isinf(dp)) { o.reason = P4G_DIVERGED_NANORINF; break; }
             if (dp <= ttol) { o.reason = dp < atol ? P4G_CONVERGED_ATOL : P4G_CONVERGED_RTOL; break; }
             if (dp >= dtol * o.rnorm0) { o.reason = P4G_DIVERGED_DTOL; break; }
-            if (hs[S_RZ] == 0.0 && i + 1 < maxit) {   // beta = 0: PETSc stops at the top of the next iteration
+            // beta = 0, or beta changing sign: PETSc stops at the top of the next iteration
+            const double rz = hs[S_RZ];
+            if (rz == 0.0 && i + 1 < maxit) {
                 o.its = i + 2;
                 o.reason = P4G_CONVERGED_ATOL;
                 break;
             }
+            if (rz * rz_old < 0.0 && i + 1 < maxit) {
+                o.its = i + 2;
+                o.reason = P4G_DIVERGED_INDEFINITE_PC;
+                break;
+            }
+            rz_old = rz;
         }
         if (i >= maxit && !o.reason) o.reason = P4G_DIVERGED_ITS;
         if (fl.m) P4G_TRY(fold());   // the listed updates: x complete on every exit
@@ -2790,6 +2806,7 @@ static int mixed_solve(p4g_solver_s* s, const double* b, double* x) {
     float* pold = D > 0 ? e.z : e.pa;   // p_{-1} (beta_0 = 0)
     float* pnew = e.pb;
     double dpi_old = 0.0;
+    double rz_old = hs[S_RZ];         // beta of the previous iteration (its sign is tested)
     int i = 0, nrep = 0;
     float* r32n = e.r2;               // the residual update goes to r32n
     bool xpend = false;               // x += alpha p of the last iteration not yet applied
@@ -2926,6 +2943,14 @@ static int mixed_solve(p4g_solver_s* s, const double* b, double* x) {
             replace_now = true;
             for_target = rn <= target;
         }
+        // beta changing sign (an indefinite V-cycle): stopped as PETSc's CG does
+        const double rz = hs[S_RZ];
+        if (rz * rz_old < 0.0 && i + 1 < s->maxit) {
+            R.its = i + 2;
+            R.reason = P4G_DIVERGED_INDEFINITE_PC;
+            break;
+        }
+        rz_old = rz;
     }
     if (!R.reason) R.reason = P4G_DIVERGED_ITS;
     R.inner_its = nrep;   // number of FP64 residual replacements

@@ -211,6 +211,7 @@ static const char *reason_name(int r) {
     case P4G_DIVERGED_DTOL: return "DIVERGED_DTOL";
     case P4G_DIVERGED_NULL: return "DIVERGED_NULL";
     case P4G_DIVERGED_BREAKDOWN: return "DIVERGED_BREAKDOWN";
+    case P4G_DIVERGED_INDEFINITE_PC: return "DIVERGED_INDEFINITE_PC";
     case P4G_DIVERGED_INDEFINITE_MAT: return "DIVERGED_INDEFINITE_MAT";
     case P4G_DIVERGED_NANORINF: return "DIVERGED_NANORINF";
     }

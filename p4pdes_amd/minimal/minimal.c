@@ -147,6 +147,7 @@ static const char *ksp_reason_name(int r) {
     case P4G_DIVERGED_ITS: return "DIVERGED_ITS";
     case P4G_DIVERGED_DTOL: return "DIVERGED_DTOL";
     case P4G_DIVERGED_BREAKDOWN: return "DIVERGED_BREAKDOWN";
+    case P4G_DIVERGED_INDEFINITE_PC: return "DIVERGED_INDEFINITE_PC";
     case P4G_DIVERGED_INDEFINITE_MAT: return "DIVERGED_INDEFINITE_MAT";
     case P4G_DIVERGED_NANORINF: return "DIVERGED_NANORINF";
     }

@@ -130,6 +130,39 @@ def test_fish_parity(name):
     assert text.splitlines()[-1] == otext.splitlines()[-1]
 
 
+# V-cycles that are indefinite preconditioners (Richardson smoothing at a
+# negative or too large scale): CG stops on the sign change of beta = (z, r)
+# with DIVERGED_INDEFINITE_PC, at the oracle's iteration and with its iterate
+INDEFINITE_PC = {
+    "2d_richardson_negative_scale": ("-fsh_dim 2 -da_refine 3 -pc_type mg -mg_levels_ksp_type richardson "
+                                     "-mg_levels_ksp_richardson_scale -1 -mg_levels_pc_type jacobi"),
+    "3d_richardson_scale_2.5": ("-fsh_dim 3 -da_refine 3 -pc_type mg -mg_levels_ksp_type richardson "
+                                "-mg_levels_ksp_richardson_scale 2.5 -mg_levels_pc_type jacobi"),
+}
+
+
+@pytest.mark.parametrize("variant", ["", " -p4g_x_defer 0"])
+@pytest.mark.parametrize("name", sorted(INDEFINITE_PC))
+def test_indefinite_preconditioner_stops_as_oracle(name, variant):
+    args = INDEFINITE_PC[name] + " -ksp_converged_reason"
+    print("case", name, variant, flush=True)
+    rc, text, res, u = capi.fish_run(args + variant, want_u=True)
+    assert rc == 0, text
+    orc, otext, ores, ou = O.fish_run(args, want_u=True)
+    assert orc == 0, otext
+    assert ores.reason == -8 and res.reason == -8, (res.reason, ores.reason)
+    assert res.its == ores.its
+    assert _rel(u, ou) < 1e-8
+    assert [l for l in text.splitlines() if "Linear solve" in l] == \
+        [l for l in otext.splitlines() if "Linear solve" in l]
+
+
+def test_mixed_indefinite_preconditioner_stops():
+    rc, text, res, _ = capi.fish_run(INDEFINITE_PC["3d_richardson_scale_2.5"] + " -p4g_precision mixed")
+    assert rc == 0, text
+    assert res.reason == -8, res.reason
+
+
 @pytest.mark.parametrize("name", sorted(n for n in CASES if "galerkin" in n))
 def test_galerkin_trajectory(name):
     # coarse operators P^T A P formed on the device in 27-point form against

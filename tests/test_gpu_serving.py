@@ -164,3 +164,19 @@ def test_full_size_set_one_request_each():
     finally:
         for s in sess.values():
             s.close()
+
+
+def test_indefinite_preconditioner_stops():
+    # Richardson smoothing at scale -1 makes the FP32 V(1,1) cycle indefinite:
+    # the request stops on the sign change of beta = (z, r), as the
+    # single-solve mixed path does, with DIVERGED_INDEFINITE_PC
+    s = capi.Session("-fsh_dim 3 -da_refine 3 -pc_type mg -mg_levels_ksp_type richardson "
+                     "-mg_levels_ksp_richardson_scale -1 -mg_levels_pc_type jacobi -mg_levels_ksp_max_it 1 "
+                     "-p4g_precision mixed")
+    try:
+        with capi.Server(64 << 30) as srv:
+            _, rep = srv.wait(srv.submit(s.solver, _rhs(s)))
+        print("its", rep.its, "reason", rep.reason, flush=True)
+        assert rep.reason == -8, rep.reason
+    finally:
+        s.close()

@@ -301,3 +301,26 @@ def test_fcycle_matches_vcycle_on_two_levels_after_the_coarse_start():
     pv = O.interp_add(2, 5, 5, 1, v.ravel(), np.zeros(9 * 9))
     x = O.pc_apply(args, O.mat_mult(args, pv))
     assert np.max(np.abs(x - pv)) <= 1e-12 * np.max(np.abs(pv))
+
+
+def test_indefinite_preconditioner_stops_cg():
+    # Richardson smoothing at a negative scale makes the V-cycle a symmetric
+    # but indefinite preconditioner B (formed column by column below, on the
+    # 17 x 17 grid); CG then stops, as PETSc's KSPCG does, when beta = (z, r)
+    # changes sign, with DIVERGED_INDEFINITE_PC (-8) at the top of the second
+    # iteration.  The default V-cycle is positive definite and converges.
+    base = "-fsh_dim 2 -da_refine 3 -ksp_converged_reason -pc_type mg -mg_levels_pc_type jacobi"
+    bad = base + " -mg_levels_ksp_type richardson -mg_levels_ksp_richardson_scale -1"
+    n = 17 * 17
+    for args, indefinite in ((bad, True), (base, False)):
+        B = np.column_stack([O.pc_apply(args.split(), e) for e in np.eye(n)])
+        assert np.max(np.abs(B - B.T)) <= 1e-12 * np.max(np.abs(B))
+        ev = np.linalg.eigvalsh(0.5 * (B + B.T))
+        assert (ev.min() < 0.0 < ev.max()) == indefinite, (ev.min(), ev.max())
+        rc, text, res, _ = O.fish_run(args.split())
+        assert rc == 0
+        if indefinite:
+            assert (res.reason, res.its) == (-8, 2)
+            assert "Linear solve did not converge due to DIVERGED_INDEFINITE_PC iterations 2" in text
+        else:
+            assert res.reason > 0
