@@ -306,12 +306,12 @@ def test_fcycle_matches_vcycle_on_two_levels_after_the_coarse_start():
 def test_indefinite_preconditioner_stops_cg():
     # Richardson smoothing at a negative scale makes the V-cycle a symmetric
     # but indefinite preconditioner B (formed column by column below, on the
-    # 17 x 17 grid); CG then stops, as PETSc's KSPCG does, when beta = (z, r)
+    # 9 x 9 grid); CG then stops, as PETSc's KSPCG does, when beta = (z, r)
     # changes sign, with DIVERGED_INDEFINITE_PC (-8) at the top of the second
     # iteration.  The default V-cycle is positive definite and converges.
-    base = "-fsh_dim 2 -da_refine 3 -ksp_converged_reason -pc_type mg -mg_levels_pc_type jacobi"
+    base = "-fsh_dim 2 -da_refine 2 -ksp_converged_reason -pc_type mg -mg_levels_pc_type jacobi"
     bad = base + " -mg_levels_ksp_type richardson -mg_levels_ksp_richardson_scale -1"
-    n = 17 * 17
+    n = 9 * 9
     for args, indefinite in ((bad, True), (base, False)):
         B = np.column_stack([O.pc_apply(args.split(), e) for e in np.eye(n)])
         assert np.max(np.abs(B - B.T)) <= 1e-12 * np.max(np.abs(B))
