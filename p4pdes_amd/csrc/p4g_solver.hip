@@ -2910,7 +2910,9 @@ static int mixed_solve(p4g_solver_s* s, const double* b, double* x) {
         P4G_TRY(cx->readS_begin(cx->hS));
         if (spec) P4G_TRY(direction());   // applies a pending x += alpha p: x is complete on every exit
         P4G_TRY(cx->readS_end());
-        if (!spec) {
+        const double dpi = hs[S_PW];      // (p, A p) of this iteration's direction (read before a re-read)
+        const bool indefinite = dpi == 0.0 || (i > 0 && ((dpi > 0) != (dpi_old > 0)));
+        if (!spec && !indefinite) {
             const double rt = std::sqrt(ho[S_TMP0]);
             if (rt <= target) {
                 R.rnorm = rt;
@@ -2923,8 +2925,7 @@ static int mixed_solve(p4g_solver_s* s, const double* b, double* x) {
             P4G_TRY(cx->readS(cx->hS));
         }
         for_target = false;
-        const double dpi = hs[S_PW];
-        if (dpi == 0.0 || (i > 0 && ((dpi > 0) != (dpi_old > 0)))) { R.reason = P4G_DIVERGED_INDEFINITE_MAT; break; }
+        if (indefinite) { R.reason = P4G_DIVERGED_INDEFINITE_MAT; break; }
         dpi_old = dpi;
         double rn;
         if (replaced) {
