@@ -31,6 +31,26 @@ def test_library_exports_every_declared_symbol(header):
     assert not missing, missing
 
 
+# PETSc's KSPConvergedReason values (petscksp.h, PETSc >= 3.21) for the
+# reasons the solvers report
+PETSC_KSP_REASONS = {"CONVERGED_RTOL": 2, "CONVERGED_ATOL": 3, "CONVERGED_ITS": 4, "DIVERGED_NULL": -2,
+                     "DIVERGED_ITS": -3, "DIVERGED_DTOL": -4, "DIVERGED_BREAKDOWN": -5,
+                     "DIVERGED_INDEFINITE_PC": -8, "DIVERGED_NANORINF": -9, "DIVERGED_INDEFINITE_MAT": -10}
+
+
+def test_reason_codes_follow_petsc():
+    text = open(os.path.join(ROOT, "include", "p4g.h")).read()
+    macros = {m: int(v) for m, v in re.findall(r"^#define P4G_((?:CONVERGED|DIVERGED)_[A-Z_]+) \(?(-?\d+)\)?",
+                                                 text, flags=re.M)}
+    assert macros == PETSC_KSP_REASONS
+    oracle = open(os.path.join(ROOT, "oracle", "oracle_internal.h")).read()
+    omacros = {m: int(v) for m, v in re.findall(r"^#define R_((?:CONVERGED|DIVERGED)_[A-Z_]+) \(?(-?\d+)\)?",
+                                                  oracle, flags=re.M)}
+    assert omacros == PETSC_KSP_REASONS
+    # the Python names of the reported reasons
+    assert all(PETSC_KSP_REASONS[name] == code for code, name in capi.REASONS.items())
+
+
 def test_fish_library_exports():
     F = capi.fishlib()
     for n in ("fish_run", "fish_session_create", "fish_session_solve", "fish_session_errors",
