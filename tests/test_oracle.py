@@ -324,3 +324,31 @@ def test_indefinite_preconditioner_stops_cg():
             assert "Linear solve did not converge due to DIVERGED_INDEFINITE_PC iterations 2" in text
         else:
             assert res.reason > 0
+
+
+def test_cg_norm_types_closed_form():
+    # the three CG norms of one run are fixed by B: with B = I they are the
+    # same history; with Jacobi on a grid whose diagonal is 4 everywhere
+    # (checked below), B = I/4, so |z| = |r|/4 (preconditioned) and
+    # sqrt((z, r)) = |r|/2 (natural), at every iteration, with the same
+    # iteration count (B is a multiple of I, so the iterates are the same)
+    import re
+
+    def hist(args):
+        rc, text, res, _ = O.fish_run(args.split())
+        assert rc == 0
+        return np.array([float(v) for v in re.findall(r"KSP Residual norm (\S+)", text)]), res.its
+
+    base = "-fsh_dim 2 -da_refine 2 -ksp_monitor -ksp_norm_type "
+    n = 9 * 9
+    diag = np.array([O.mat_mult(base.split()[:4], e)[j] for j, e in enumerate(np.eye(n))])
+    assert np.all(diag == 4.0)
+    for pc, scale in (("none", {"preconditioned": 1.0, "natural": 1.0}),
+                      ("jacobi", {"preconditioned": 0.25, "natural": 0.5})):
+        hu, itu = hist(base + "unpreconditioned -pc_type " + pc)
+        assert len(hu) == itu + 1 and itu > 5
+        for nt, f in scale.items():
+            h, its = hist(base + nt + " -pc_type " + pc)
+            assert its == itu
+            # the monitor prints 13 significant digits
+            assert np.allclose(h, f * hu, rtol=1e-11, atol=0.0), (pc, nt)
