@@ -16,7 +16,12 @@ fails:
   * a two-level Galerkin K-cycle maps b = A P v to P v exactly, and is
     linear;
   * PCApplyRichardson_MG (no monitor): one or two iterations equal one or
-    two cycles from zero.
+    two cycles from zero;
+  * the three-level V- and W-cycles equal the two-grid recursion
+    I - B A = S (I - P C P^T A) S built from the one-level cycle (the
+    smoother, S = I - B_1 A) and the two-level cycle B_c of the coarser grid,
+    with C = B_c (V) and C = 2 B_c - B_c A_c B_c (W: the coarse cycle twice,
+    the second from the first's iterate).
 GPU parity of the same paths is in tests/test_gpu_parity.py.
 """
 import re
@@ -118,3 +123,28 @@ def test_richardson_mg_shortcut_stops_after_presmoothing():
     assert r1.reason == r2.reason == 2
     assert r1.its <= r2.its
     assert r1.true_relres <= 1e-8
+
+
+def test_three_level_v_and_w_cycles_are_the_two_grid_recursion():
+    # all operators formed column by column on a 17 x 17 grid (levels 17, 9,
+    # 5); the smoother's Chebyshev bounds are estimated on each level's own
+    # operator, so the level smoother of every hierarchy is the same S
+    def mat(f, n):
+        return np.column_stack([f(e) for e in np.eye(n)])
+    sm = " -mg_levels_ksp_type chebyshev -mg_levels_pc_type jacobi"
+    fine = "-fsh_dim 2 -fsh_problem manusin -da_refine 3 -pc_type mg"
+    coarse = "-fsh_dim 2 -fsh_problem manusin -da_refine 2 -pc_type mg"
+    nf, nc = 17 * 17, 9 * 9
+    A = mat(lambda e: O.mat_mult(fine, e), nf)
+    Ac = mat(lambda e: O.mat_mult(coarse, e), nc)
+    B1 = mat(lambda e: O.pc_apply(fine + " -pc_mg_levels 1" + sm, e), nf)
+    Bc = mat(lambda e: O.pc_apply(coarse + " -pc_mg_levels 2" + sm, e), nc)
+    P = mat(lambda e: O.interp_add(2, 9, 9, 1, e, np.zeros(nf)), nc)
+    I = np.eye(nf)
+    S = I - B1 @ A
+    Bs = {}
+    for cyc, C in (("v", Bc), ("w", 2 * Bc - Bc @ Ac @ Bc)):
+        Bs[cyc] = mat(lambda e: O.pc_apply(fine + " -pc_mg_levels 3 -pc_mg_cycle_type " + cyc + sm, e), nf)
+        E = I - Bs[cyc] @ A
+        assert np.max(np.abs(E - S @ (I - P @ C @ P.T @ A) @ S)) <= 1e-13 * np.max(np.abs(E)), cyc
+    assert np.max(np.abs(Bs["w"] - Bs["v"])) > 1e-3 * np.max(np.abs(Bs["v"]))
