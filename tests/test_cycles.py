@@ -21,7 +21,9 @@ fails:
     I - B A = S (I - P C P^T A) S built from the one-level cycle (the
     smoother, S = I - B_1 A) and the two-level cycle B_c of the coarser grid,
     with C = B_c (V) and C = 2 B_c - B_c A_c B_c (W: the coarse cycle twice,
-    the second from the first's iterate).
+    the second from the first's iterate);
+  * the two-level Galerkin cycle is S (I - P (P^T A P)^-1 P^T A) S, and the
+    re-discretized one is not.
 GPU parity of the same paths is in tests/test_gpu_parity.py.
 """
 import re
@@ -148,3 +150,27 @@ def test_three_level_v_and_w_cycles_are_the_two_grid_recursion():
         E = I - Bs[cyc] @ A
         assert np.max(np.abs(E - S @ (I - P @ C @ P.T @ A) @ S)) <= 1e-13 * np.max(np.abs(E)), cyc
     assert np.max(np.abs(Bs["w"] - Bs["v"])) > 1e-3 * np.max(np.abs(Bs["v"]))
+
+
+@pytest.mark.parametrize("fine,shape", [
+    ("-fsh_dim 2 -fsh_problem manupoly -fsh_cx 0.3 -da_refine 3 -pc_type mg", (2, 9, 9, 1, 17 * 17, 9 * 9)),
+    ("-fsh_dim 3 -fsh_problem manusin -da_refine 2 -pc_type mg", (3, 5, 5, 5, 9 ** 3, 5 ** 3)),
+], ids=["2d_anisotropic", "3d"])
+def test_two_level_galerkin_cycle_inverts_ptap(fine, shape):
+    # -pc_mg_galerkin: the coarse operator is P^T A P (formed here from the
+    # matrices of A and P), solved exactly on the coarse level
+    dim, mxc, myc, mzc, nf, nc = shape
+
+    def mat(f, n):
+        return np.column_stack([f(e) for e in np.eye(n)])
+    sm = " -mg_levels_ksp_type chebyshev -mg_levels_pc_type jacobi"
+    A = mat(lambda e: O.mat_mult(fine, e), nf)
+    B1 = mat(lambda e: O.pc_apply(fine + " -pc_mg_levels 1" + sm, e), nf)
+    P = mat(lambda e: O.interp_add(dim, mxc, myc, mzc, e, np.zeros(nf)), nc)
+    I = np.eye(nf)
+    S = I - B1 @ A
+    E2 = S @ (I - P @ np.linalg.solve(P.T @ A @ P, P.T @ A)) @ S
+    for opt, exact in ((" -pc_mg_galerkin", True), ("", False)):
+        E = I - mat(lambda e: O.pc_apply(fine + " -pc_mg_levels 2" + opt + sm, e), nf) @ A
+        err = np.max(np.abs(E - E2)) / np.max(np.abs(E))
+        assert (err <= 1e-13) if exact else (err > 1e-3), (opt, err)
