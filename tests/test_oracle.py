@@ -352,3 +352,30 @@ def test_cg_norm_types_closed_form():
             assert its == itu
             # the monitor prints 13 significant digits
             assert np.allclose(h, f * hu, rtol=1e-11, atol=0.0), (pc, nt)
+
+
+def test_chebyshev_smoother_is_the_chebyshev_polynomial():
+    # k steps of the Chebyshev/Jacobi smoother from x = 0 leave the error
+    # q_k(D^-1 A) e with q_k(l) = T_k((theta - l)/delta) / T_k(theta/delta),
+    # theta, delta the centre and half-width of [0.1, 1.1] * estimate (the
+    # textbook Chebyshev iteration): S = I - B_1 A formed column by column
+    # from the one-level cycle, which is the smoother
+    fine = ("-fsh_dim 2 -fsh_problem manupoly -da_refine 3 -pc_type mg -mg_levels_ksp_type chebyshev "
+            "-mg_levels_pc_type jacobi")
+    n = 17 * 17
+    A = np.column_stack([O.mat_mult(fine, e) for e in np.eye(n)])
+    M = A / np.diag(A)[:, None]
+    _, _, res, _ = O.fish_run(fine + " -pc_mg_levels 2")
+    est = res.emax_fine / 1.1
+    emax, emin = 1.1 * est, 0.1 * est
+    theta, delta = (emax + emin) / 2, (emax - emin) / 2
+    I = np.eye(n)
+    X = (theta * I - M) / delta
+    Tm, T = I, X                                     # T_0(X), T_1(X)
+    for k in range(1, 5):
+        B1 = np.column_stack([O.pc_apply(fine + " -pc_mg_levels 1 -mg_levels_ksp_max_it %d" % k, e)
+                              for e in np.eye(n)])
+        S = I - B1 @ A
+        Q = T / np.polynomial.chebyshev.chebval(theta / delta, [0] * k + [1])
+        assert np.max(np.abs(S - Q)) <= 1e-13 * np.max(np.abs(S)), k
+        Tm, T = T, 2 * X @ T - Tm
