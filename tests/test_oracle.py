@@ -379,3 +379,34 @@ def test_chebyshev_smoother_is_the_chebyshev_polynomial():
         Q = T / np.polynomial.chebyshev.chebval(theta / delta, [0] * k + [1])
         assert np.max(np.abs(S - Q)) <= 1e-13 * np.max(np.abs(S)), k
         Tm, T = T, 2 * X @ T - Tm
+
+
+@pytest.mark.parametrize("fine,n", [
+    ("-fsh_dim 2 -fsh_problem manupoly -fsh_cx 0.3 -da_refine 3 -pc_type mg -pc_mg_levels 1", 17 * 17),
+    ("-fsh_dim 3 -fsh_problem manusin -da_refine 2 -pc_type mg -pc_mg_levels 1", 9 ** 3),
+], ids=["2d_anisotropic", "3d"])
+def test_richardson_smoothers_closed_form(fine, n):
+    # k Richardson steps of the level smoother (the one-level cycle) leave the
+    # error S = I - B_1 A: with SOR (PETSc's symmetric sweep, omega = 1) the
+    # k-th power of the SSOR operator, a forward lexicographic sweep
+    # (D + L)^-1 then a backward one (D + U)^-1 (the reverse order is not
+    # it); with Jacobi at scale s, (I - s D^-1 A)^k
+    A = np.column_stack([O.mat_mult(fine, e) for e in np.eye(n)])
+    I = np.eye(n)
+    fwd = I - np.linalg.solve(np.tril(A), A)
+    bwd = I - np.linalg.solve(np.triu(A), A)
+    jac = I - 0.7 * A / np.diag(A)[:, None]
+
+    def S(opts, k):
+        B1 = np.column_stack([O.pc_apply(fine + " -mg_levels_ksp_type richardson " + opts +
+                                         " -mg_levels_ksp_max_it %d" % k, e) for e in np.eye(n)])
+        return I - B1 @ A
+
+    def rel(X, Y):
+        return np.max(np.abs(X - Y)) / np.max(np.abs(X))
+    for k in (1, 2):
+        s_sor = S("-mg_levels_pc_type sor", k)
+        assert rel(s_sor, np.linalg.matrix_power(bwd @ fwd, k)) <= 1e-12
+        assert rel(s_sor, np.linalg.matrix_power(fwd @ bwd, k)) > 0.1
+        s_jac = S("-mg_levels_pc_type jacobi -mg_levels_ksp_richardson_scale 0.7", k)
+        assert rel(s_jac, np.linalg.matrix_power(jac, k)) <= 1e-13
