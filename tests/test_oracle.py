@@ -410,3 +410,25 @@ def test_richardson_smoothers_closed_form(fine, n):
         assert rel(s_sor, np.linalg.matrix_power(fwd @ bwd, k)) > 0.1
         s_jac = S("-mg_levels_pc_type jacobi -mg_levels_ksp_richardson_scale 0.7", k)
         assert rel(s_jac, np.linalg.matrix_power(jac, k)) <= 1e-13
+
+
+@pytest.mark.parametrize("args,n", [("-fsh_dim 2 -fsh_problem manupoly -fsh_cx 0.3 -da_refine 3", 17 * 17),
+                                    ("-fsh_dim 3 -fsh_problem manusin -da_refine 2", 9 ** 3)],
+                         ids=["2d_anisotropic", "3d"])
+def test_ilu0_and_icc0_are_the_pattern_restricted_factorization(args, n):
+    # PETSc's serial ILU(0) / ICC(0) in the natural ordering: Gaussian
+    # elimination that keeps only the entries in A's sparsity pattern (a
+    # dense restatement here); B = (L U)^-1, which is not A^-1
+    A = np.column_stack([O.mat_mult(args, e) for e in np.eye(n)])
+    LU = A.copy()
+    pat = A != 0
+    for i in range(1, n):
+        for k in np.nonzero(pat[i, :i])[0]:
+            LU[i, k] /= LU[k, k]
+            m = pat[i, k + 1:]
+            LU[i, k + 1:][m] -= LU[i, k] * LU[k, k + 1:][m]
+    Bref = np.linalg.inv((np.tril(LU, -1) + np.eye(n)) @ np.triu(LU))
+    for pc in ("ilu", "icc"):
+        B = np.column_stack([O.pc_apply(args + " -pc_type " + pc, e) for e in np.eye(n)])
+        assert np.max(np.abs(B - Bref)) <= 1e-13 * np.max(np.abs(Bref)), pc
+        assert np.max(np.abs(B - np.linalg.inv(A))) > 0.05 * np.max(np.abs(B))
