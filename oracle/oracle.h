@@ -22,6 +22,15 @@
  *   - V-cycle symmetry <Bx,y> = <x,By>, coarse Cholesky vs dense solve;
  *   - Lanczos eigen-estimate vs closed-form lambda_max(D^-1 A) (upper bound)
  *     and vs the GMRES/Hessenberg estimate (equal for constant diagonal).
+ *   - operators formed as matrices on small grids against closed forms
+ *     (tests/test_cycles.py, tests/test_oracle.py): the three-level V/W
+ *     cycles against the two-grid recursion S (I - P C P^T A) S with the
+ *     coarser grid's two-level cycle; the two-level Galerkin cycle against
+ *     (P^T A P)^-1; the Chebyshev smoother against T_k((theta - D^-1 A) /
+ *     delta) / T_k(theta/delta); the SOR and Jacobi Richardson smoothers
+ *     against the SSOR and (I - s D^-1 A) powers; ILU(0)/ICC(0) against a
+ *     dense pattern-restricted elimination; the CG norm types against
+ *     B = I and B = I/4; DIVERGED_INDEFINITE_PC on an indefinite V-cycle.
  *   - variable-coefficient operator (config "varcoef"): parity unpinned
  *     against the reference, which has constant cx, cy, cz only; pinned by
  *     symmetry, constant-k reduction to the constant-coefficient stencil
