@@ -28,7 +28,8 @@
  *     coarser grid's two-level cycle; the two-level Galerkin cycle against
  *     (P^T A P)^-1; the Chebyshev smoother against T_k((theta - D^-1 A) /
  *     delta) / T_k(theta/delta); the SOR and Jacobi Richardson smoothers
- *     against the SSOR and (I - s D^-1 A) powers; ILU(0)/ICC(0) against a
+ *     against the SSOR and (I - s D^-1 A) powers, rank-local SOR against
+ *     block SSOR over the z-slabs; ILU(0)/ICC(0) against a
  *     dense pattern-restricted elimination; the CG norm types against
  *     B = I and B = I/4; DIVERGED_INDEFINITE_PC on an indefinite V-cycle.
  *   - variable-coefficient operator (config "varcoef"): parity unpinned

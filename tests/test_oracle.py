@@ -432,3 +432,29 @@ def test_ilu0_and_icc0_are_the_pattern_restricted_factorization(args, n):
         B = np.column_stack([O.pc_apply(args + " -pc_type " + pc, e) for e in np.eye(n)])
         assert np.max(np.abs(B - Bref)) <= 1e-13 * np.max(np.abs(Bref)), pc
         assert np.max(np.abs(B - np.linalg.inv(A))) > 0.05 * np.max(np.abs(B))
+
+
+@pytest.mark.parametrize("nranks,planes", [(2, (5, 4)), (3, (3, 3, 3))])
+def test_rank_local_sor_is_block_ssor(nranks, planes):
+    # PETSc's parallel SOR from a zero guess sweeps each rank's diagonal block
+    # (here the ranks' z-slabs of the 9^3 grid, the first ranks taking the
+    # extra plane): one Richardson step gives B = blockdiag((I - S_b) A_b^-1)
+    # with S_b the forward-then-backward SSOR operator of the block
+    m = 9
+    n = m ** 3
+    base = ("-fsh_dim 3 -fsh_problem manusin -da_refine 2 -pc_type mg -pc_mg_levels 1 -mg_levels_ksp_type "
+            "richardson -mg_levels_pc_type sor -mg_levels_ksp_max_it 1")
+    A = np.column_stack([O.mat_mult(base, e) for e in np.eye(n)])
+    Bref = np.zeros((n, n))
+    o = 0
+    for p in planes:
+        sz = p * m * m
+        Ab = A[o:o + sz, o:o + sz]
+        Ib = np.eye(sz)
+        Sb = (Ib - np.linalg.solve(np.triu(Ab), Ab)) @ (Ib - np.linalg.solve(np.tril(Ab), Ab))
+        Bref[o:o + sz, o:o + sz] = (Ib - Sb) @ np.linalg.inv(Ab)
+        o += sz
+    B = np.column_stack([O.pc_apply(base + " -oracle_nranks %d" % nranks, e) for e in np.eye(n)])
+    assert np.max(np.abs(B - Bref)) <= 1e-13 * np.max(np.abs(B))
+    B1 = np.column_stack([O.pc_apply(base, e) for e in np.eye(n)])
+    assert np.max(np.abs(B - B1)) > 1e-3 * np.max(np.abs(B1))
