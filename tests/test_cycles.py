@@ -23,7 +23,10 @@ fails:
     with C = B_c (V) and C = 2 B_c - B_c A_c B_c (W: the coarse cycle twice,
     the second from the first's iterate);
   * the two-level Galerkin cycle is S (I - P (P^T A P)^-1 P^T A) S, and the
-    re-discretized one is not.
+    re-discretized one is not;
+  * the three-level F-cycle is the coarsest solve of the twice-restricted b,
+    then on each finer level the interpolated iterate followed by one cycle
+    of that level's hierarchy from it.
 GPU parity of the same paths is in tests/test_gpu_parity.py.
 """
 import re
@@ -174,3 +177,29 @@ def test_two_level_galerkin_cycle_inverts_ptap(fine, shape):
         E = I - mat(lambda e: O.pc_apply(fine + " -pc_mg_levels 2" + opt + sm, e), nf) @ A
         err = np.max(np.abs(E - E2)) / np.max(np.abs(E))
         assert (err <= 1e-13) if exact else (err > 1e-3), (opt, err)
+
+
+def test_three_level_fcycle_is_the_nested_iteration():
+    # PCMG full (F) cycle on levels 5^2, 9^2, 17^2: b restricted twice and
+    # solved on the coarsest level, x_1 = P_1 x_0 + B_c (P_2^T b - A_1 P_1 x_0)
+    # with B_c the 9^2 grid's two-level cycle, then x = P_2 x_1 + B_V (b - A
+    # P_2 x_1) with B_V the three-level V-cycle (each cycle from the
+    # interpolated iterate is linear: x + B (b - A x))
+    def mat(f, n):
+        return np.column_stack([f(e) for e in np.eye(n)])
+    sm = " -mg_levels_ksp_type chebyshev -mg_levels_pc_type jacobi"
+    grid = lambda r: "-fsh_dim 2 -fsh_problem manusin -da_refine %d -pc_type mg" % r
+    n2, n1, n0 = 17 * 17, 9 * 9, 5 * 5
+    A = mat(lambda e: O.mat_mult(grid(3), e), n2)
+    A1 = mat(lambda e: O.mat_mult(grid(2), e), n1)
+    A0 = mat(lambda e: O.mat_mult(grid(1), e), n0)
+    P2 = mat(lambda e: O.interp_add(2, 9, 9, 1, e, np.zeros(n2)), n1)
+    P1 = mat(lambda e: O.interp_add(2, 5, 5, 1, e, np.zeros(n1)), n0)
+    BV = mat(lambda e: O.pc_apply(grid(3) + " -pc_mg_levels 3" + sm, e), n2)
+    Bc = mat(lambda e: O.pc_apply(grid(2) + " -pc_mg_levels 2" + sm, e), n1)
+    BF = mat(lambda e: O.pc_apply(grid(3) + " -pc_mg_levels 3 -pc_mg_type full" + sm, e), n2)
+    X0 = np.linalg.solve(A0, P1.T @ P2.T)
+    X1 = P1 @ X0 + Bc @ (P2.T - A1 @ P1 @ X0)
+    X2 = P2 @ X1 + BV @ (np.eye(n2) - A @ P2 @ X1)
+    assert np.max(np.abs(BF - X2)) <= 1e-13 * np.max(np.abs(BF))
+    assert np.max(np.abs(BF - BV)) > 1e-2 * np.max(np.abs(BV))
