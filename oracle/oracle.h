@@ -25,8 +25,8 @@
  *   - operators formed as matrices on small grids against closed forms
  *     (tests/test_cycles.py, tests/test_oracle.py): the three-level V/W
  *     cycles against the two-grid recursion S (I - P C P^T A) S with the
- *     coarser grid's two-level cycle, the three-level F-cycle against the
- *     nested iteration; the two-level Galerkin cycle against
+ *     coarser grid's two-level cycle, the three-level F- and Kaskade cycles
+ *     against nested iterations; the two-level Galerkin cycle against
  *     (P^T A P)^-1; the Chebyshev smoother against T_k((theta - D^-1 A) /
  *     delta) / T_k(theta/delta); the SOR and Jacobi Richardson smoothers
  *     against the SSOR and (I - s D^-1 A) powers, rank-local SOR against

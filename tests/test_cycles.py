@@ -26,7 +26,8 @@ fails:
     re-discretized one is not;
   * the three-level F-cycle is the coarsest solve of the twice-restricted b,
     then on each finer level the interpolated iterate followed by one cycle
-    of that level's hierarchy from it.
+    of that level's hierarchy from it; the Kaskade cycle the same climb with
+    one level smoother application in place of each cycle.
 GPU parity of the same paths is in tests/test_gpu_parity.py.
 """
 import re
@@ -179,7 +180,7 @@ def test_two_level_galerkin_cycle_inverts_ptap(fine, shape):
         assert (err <= 1e-13) if exact else (err > 1e-3), (opt, err)
 
 
-def test_three_level_fcycle_is_the_nested_iteration():
+def test_three_level_f_and_kaskade_cycles_are_nested_iterations():
     # PCMG full (F) cycle on levels 5^2, 9^2, 17^2: b restricted twice and
     # solved on the coarsest level, x_1 = P_1 x_0 + B_c (P_2^T b - A_1 P_1 x_0)
     # with B_c the 9^2 grid's two-level cycle, then x = P_2 x_1 + B_V (b - A
@@ -203,3 +204,12 @@ def test_three_level_fcycle_is_the_nested_iteration():
     X2 = P2 @ X1 + BV @ (np.eye(n2) - A @ P2 @ X1)
     assert np.max(np.abs(BF - X2)) <= 1e-13 * np.max(np.abs(BF))
     assert np.max(np.abs(BF - BV)) > 1e-2 * np.max(np.abs(BV))
+    # the Kaskade cycle: the same climb with one level smoother application
+    # (the one-level cycle B_1 of that grid) in place of each cycle
+    B1f = mat(lambda e: O.pc_apply(grid(3) + " -pc_mg_levels 1" + sm, e), n2)
+    B1m = mat(lambda e: O.pc_apply(grid(2) + " -pc_mg_levels 1" + sm, e), n1)
+    BK = mat(lambda e: O.pc_apply(grid(3) + " -pc_mg_levels 3 -pc_mg_type kaskade" + sm, e), n2)
+    K1 = P1 @ X0 + B1m @ (P2.T - A1 @ P1 @ X0)
+    K2 = P2 @ K1 + B1f @ (np.eye(n2) - A @ P2 @ K1)
+    assert np.max(np.abs(BK - K2)) <= 1e-13 * np.max(np.abs(BK))
+    assert np.max(np.abs(BK - BF)) > 1e-2 * np.max(np.abs(BF))
